@@ -1,0 +1,1147 @@
+/*
+ * wave_oracle.c -- CPU restatement of WAVE-Simulation's forward-solver time loop.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the parity oracle: it is imported by
+ * tests/, by __graft_entry__.smoke() and by bench.py's cpu_baseline leg, and
+ * nowhere else.  The product (wave-simulation_amd/) never links or calls it.
+ *
+ * It restates the reference operator by operator ("LAMA-structured"): every
+ * derivative is an assembled CSR matrix built with the reference's row
+ * algorithm (including edge order reduction), every update is a separate
+ * dense-vector pass in the reference's order, CPML is applied as
+ *   psi = b*psi + a*d ; d += psi
+ * on full-grid coefficient vectors that are zero outside the frame.
+ *
+ * Built twice: -DREAL=float  -> liboracle_f32.so (the "reference float build")
+ *              -DREAL=double -> liboracle_f64.so (the FP64 reference)
+ *
+ * Reference files followed (paths relative to the WAVE-Simulation repo):
+ *   FD coefficients            src/ForwardSolver/Derivatives/Derivatives.cpp:2001-2042
+ *   sparse Dxf/Dyf/Dzf          Derivatives.cpp:129-186, 210-280, 304-359
+ *   sparse Dxb/Dyb/Dzb          Derivatives.cpp:706-763, 771-843, 1177-1234
+ *   Dyf/Dyb free-surface image  Derivatives.cpp:367-526
+ *   DT scaling                  src/ForwardSolver/Derivatives/FDTD2D.cpp:156-232, FDTD3D.cpp
+ *   stencil-matrix operators    FDTD2D.cpp:156-185 (Dxb = -Dxf^T, *= DT/DH)
+ *   averaging                   src/Modelparameter/Modelparameter.cpp:336-639
+ *   shear-modulus averaging     src/Modelparameter/ModelparameterSeismic.cpp:421-433
+ *   EM Ca/Cb                    src/ForwardSolverEM/ForwardSolverEM.cpp:14-68
+ *   EM averaging, constants     src/ModelparameterEM/TMEM.cpp:366-373, Modelparameter.hpp:359-360
+ *   CPML coefficients           src/ForwardSolver/BoundaryCondition/CPML.cpp:39-95
+ *   CPML placement              CPML2D.cpp:170-282, CPML2DAcoustic.cpp:106-210,
+ *                               CPMLEM2D.cpp:138-243, CPML3D.cpp:222-366
+ *   ABS                         ABS2D.cpp:110-178, ABS3D.cpp:150-215
+ *   free surface                FreeSurfaceElastic.cpp:11-90, FreeSurface2Delastic.cpp:14-45,
+ *                               FreeSurface3Delastic.cpp, FreeSurface.cpp:14-21
+ *   time steps                  ForwardSolver2Dacoustic.cpp:91-194, ForwardSolver2Delastic.cpp:117-294,
+ *                               ForwardSolver2Dtmem.cpp:108-171, ForwardSolver3Delastic.cpp:120-416
+ *   source / receiver           SourceReceiverImpl.cpp:12-37, SourceReceiverImplSeismic.cpp:34-105,
+ *                               FDTD2Delastic.cpp:12-49, FDTDacoustic.cpp:12-45
+ *   Ricker wavelet              src/Acquisition/SourceSignal/Ricker.cpp:29-53
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifndef REAL
+#define REAL float
+#endif
+typedef REAL real;
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+
+/* ------------------------------------------------------------------------- */
+/* FD coefficients: exact staggered Taylor coefficients (half-point offsets), */
+/* stored in the reference's order (Derivatives.cpp:2001-2042).               */
+/* ------------------------------------------------------------------------- */
+static const double FD_NUM[7][12] = {
+    {0},
+    {-1, 1},
+    {1, -9, 9, -1},
+    {-3, 25, -75, 75, -25, 3},
+    {5, -49, 245, -1225, 1225, -245, 49, -5},
+    {-35, 405, -567, 735, -19845, 19845, -735, 567, -405, 35},
+    {63, -847, 5445, -22869, 12705, -160083, 160083, -12705, 22869, -5445, 847, -63}};
+static const double FD_DEN[7][12] = {
+    {0},
+    {1, 1},
+    {24, 8, 8, 24},
+    {640, 384, 64, 64, 384, 640},
+    {7168, 5120, 3072, 1024, 1024, 3072, 5120, 7168},
+    {294912, 229376, 40960, 8192, 16384, 16384, 8192, 40960, 229376, 294912},
+    {2883584, 2359296, 1835008, 1310720, 131072, 131072, 131072, 131072, 1310720, 1835008, 2359296, 2883584}};
+
+static real fdc(int order, int j) { return (real)(FD_NUM[order / 2][j] / FD_DEN[order / 2][j]); }
+
+int oracle_fd_coef(int order, double *out)
+{
+    if (order < 2 || order > 12 || (order & 1)) return -1;
+    for (int j = 0; j < order; j++) out[j] = FD_NUM[order / 2][j] / FD_DEN[order / 2][j];
+    return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* CSR matrices                                                               */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+    long n;
+    long *rowptr;
+    long *col;
+    real *val;
+} csr_t;
+
+typedef struct {
+    long n, nnz, cap;
+    long *rowptr, *col;
+    real *val;
+    long row;
+} asm_t;
+
+static void asm_init(asm_t *a, long n, int per_row)
+{
+    a->n = n;
+    a->nnz = 0;
+    a->cap = n * per_row + 16;
+    a->rowptr = (long *)calloc((size_t)n + 1, sizeof(long));
+    a->col = (long *)malloc(sizeof(long) * (size_t)a->cap);
+    a->val = (real *)malloc(sizeof(real) * (size_t)a->cap);
+    a->row = 0;
+}
+/* rows are assembled in increasing order */
+static void asm_begin_row(asm_t *a, long r) { a->row = r; }
+static void asm_push(asm_t *a, long c, real v)
+{
+    if (a->nnz == a->cap) {
+        a->cap *= 2;
+        a->col = (long *)realloc(a->col, sizeof(long) * (size_t)a->cap);
+        a->val = (real *)realloc(a->val, sizeof(real) * (size_t)a->cap);
+    }
+    a->col[a->nnz] = c;
+    a->val[a->nnz] = v;
+    a->nnz++;
+}
+static void asm_end_row(asm_t *a) { a->rowptr[a->row + 1] = a->nnz; }
+static csr_t asm_finish(asm_t *a)
+{
+    csr_t m;
+    m.n = a->n;
+    m.rowptr = a->rowptr;
+    m.col = a->col;
+    m.val = a->val;
+    return m;
+}
+static void csr_free(csr_t *m)
+{
+    free(m->rowptr);
+    free(m->col);
+    free(m->val);
+    memset(m, 0, sizeof(*m));
+}
+static void csr_scale(csr_t *m, real s)
+{
+    long nnz = m->rowptr[m->n];
+    for (long k = 0; k < nnz; k++) m->val[k] *= s;
+}
+
+/* y = A x, each row summed in stored order */
+static void spmv(const csr_t *A, const real *x, real *y)
+{
+    long n = A->n;
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n; i++) {
+        real t = 0;
+        for (long k = A->rowptr[i]; k < A->rowptr[i + 1]; k++) t += A->val[k] * x[A->col[k]];
+        y[i] = t;
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* grid                                                                       */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+    long NX, NY, NZ, n;
+} grid_t;
+
+static long gidx(const grid_t *g, long x, long y, long z) { return x + z * g->NX + y * g->NX * g->NZ; }
+static void gcoord(const grid_t *g, long i, long *x, long *y, long *z)
+{
+    *y = i / (g->NX * g->NZ);
+    i -= *y * g->NX * g->NZ;
+    *z = i / g->NX;
+    *x = i - *z * g->NX;
+}
+
+/* axis: 0 = x, 1 = y, 2 = z */
+static long axis_len(const grid_t *g, int axis) { return axis == 0 ? g->NX : (axis == 1 ? g->NY : g->NZ); }
+static long axis_pos(long x, long y, long z, int axis) { return axis == 0 ? x : (axis == 1 ? y : z); }
+static long axis_idx(const grid_t *g, long x, long y, long z, int axis, long p)
+{
+    if (axis == 0) return gidx(g, p, y, z);
+    if (axis == 1) return gidx(g, x, p, z);
+    return gidx(g, x, y, p);
+}
+
+/*
+ * Sparse forward derivative with edge order reduction (Derivatives.cpp:129-186,
+ * 210-280, 304-359).  Values are fd/DH, scaled by DT afterwards.
+ */
+static csr_t build_forward(const grid_t *g, int axis, int order, real DH)
+{
+    asm_t a;
+    asm_init(&a, g->n, order);
+    long L = axis_len(g, axis);
+    for (long i = 0; i < g->n; i++) {
+        long x, y, z;
+        gcoord(g, i, &x, &y, &z);
+        long c = axis_pos(x, y, z, axis);
+        int N = order;
+        asm_begin_row(&a, i);
+        for (int j = 0; j < N; j++) {
+            long P = c + (j - N / 2 + 1);
+            long Pmin = c + (-N / 2 + 1);
+            long Pmax = c + N / 2;
+            if (Pmin < 0) {
+                N += 2 * (int)Pmin;
+                j--;
+            } else if (Pmax >= L) {
+                N -= 2 * (int)(Pmax - L + 1);
+                if (N == 0) {
+                    N = 2;
+                    c -= 1;
+                }
+                j--;
+            } else {
+                asm_push(&a, axis_idx(g, x, y, z, axis, P), fdc(N, j) / DH);
+            }
+        }
+        asm_end_row(&a);
+    }
+    return asm_finish(&a);
+}
+
+/* Sparse backward derivative with edge order reduction (Derivatives.cpp:706-843, 1177-1234) */
+static csr_t build_backward(const grid_t *g, int axis, int order, real DH)
+{
+    asm_t a;
+    asm_init(&a, g->n, order);
+    long L = axis_len(g, axis);
+    for (long i = 0; i < g->n; i++) {
+        long x, y, z;
+        gcoord(g, i, &x, &y, &z);
+        long c = axis_pos(x, y, z, axis);
+        int N = order;
+        asm_begin_row(&a, i);
+        for (int j = 0; j < N; j++) {
+            long P = c + (j - N / 2);
+            long Pmin = c + (-N / 2);
+            long Pmax = c + (N / 2 - 1);
+            if (Pmin < 0) {
+                N += 2 * (int)Pmin;
+                if (N == 0) {
+                    N = 2;
+                    c += 1;
+                }
+                j--;
+            } else if (Pmax >= L) {
+                N -= 2 * (int)(Pmax - L + 1);
+                j--;
+            } else {
+                asm_push(&a, axis_idx(g, x, y, z, axis, P), fdc(N, j) / DH);
+            }
+        }
+        asm_end_row(&a);
+    }
+    return asm_finish(&a);
+}
+
+/*
+ * LAMA StencilMatrix operator (useStencilMatrix=1, FDTD2D.cpp:156-185): the
+ * forward stencil has offsets j - N/2 + 1 and drops points outside the grid;
+ * the backward one is -transpose(forward); both are scaled by DT/DH.
+ * Returned unscaled (coefficient * sign) -- the caller scales by DT/DH.
+ */
+static csr_t build_stencil(const grid_t *g, int axis, int order, int backward)
+{
+    asm_t a;
+    asm_init(&a, g->n, order);
+    long L = axis_len(g, axis);
+    int N = order;
+    for (long i = 0; i < g->n; i++) {
+        long x, y, z;
+        gcoord(g, i, &x, &y, &z);
+        long c = axis_pos(x, y, z, axis);
+        asm_begin_row(&a, i);
+        for (int j = 0; j < N; j++) {
+            long P;
+            real v;
+            if (!backward) {
+                P = c + (j - N / 2 + 1);
+                v = fdc(N, j);
+            } else {
+                /* row c of -F^T: F[k][c] = fd[j] where c = k + j - N/2 + 1 */
+                long k = c - (j - N / 2 + 1);
+                P = k;
+                v = -fdc(N, j);
+            }
+            if (P >= 0 && P < L) asm_push(&a, axis_idx(g, x, y, z, axis, P), v);
+        }
+        asm_end_row(&a);
+    }
+    return asm_finish(&a);
+}
+
+/* free-surface image operators, non-hybrid, regular grid (Derivatives.cpp:367-526) */
+static csr_t build_dyf_fs(const grid_t *g, int order, real DH)
+{
+    asm_t a;
+    asm_init(&a, g->n, order);
+    int N = order;
+    for (long i = 0; i < g->n; i++) {
+        long x, y, z;
+        gcoord(g, i, &x, &y, &z);
+        asm_begin_row(&a, i);
+        for (int j = 0; j < N; j++) {
+            long Y = y + (j - N / 2 + 1);
+            real fd = fdc(N, j), diff = 0;
+            if (N >= 2 + 2 * y + j) diff = fdc(N, (int)(N - 2 - 2 * y - j));
+            if (Y >= 0 && Y < g->NY) asm_push(&a, gidx(g, x, Y, z), (fd - diff) / DH);
+        }
+        asm_end_row(&a);
+    }
+    return asm_finish(&a);
+}
+
+static csr_t build_dyb_fs(const grid_t *g, int order, real DH)
+{
+    asm_t a;
+    asm_init(&a, g->n, order);
+    int N = order;
+    for (long i = 0; i < g->n; i++) {
+        long x, y, z;
+        gcoord(g, i, &x, &y, &z);
+        asm_begin_row(&a, i);
+        for (int j = 0; j < N; j++) {
+            long Y = y + (j - N / 2);
+            real fd = fdc(N, j), diff = 0;
+            if (N >= 1 + 2 * y + j) diff = fdc(N, (int)(N - 1 - 2 * y - j));
+            if (Y >= 0 && Y < g->NY) asm_push(&a, gidx(g, x, Y, z), (fd - diff) / DH);
+        }
+        asm_end_row(&a);
+    }
+    return asm_finish(&a);
+}
+
+/* ------------------------------------------------------------------------- */
+/* dense vector helpers                                                       */
+/* ------------------------------------------------------------------------- */
+static real *vnew(long n) { return (real *)calloc((size_t)n, sizeof(real)); }
+
+#define VLOOP(n) _Pragma("omp parallel for schedule(static)") for (long i = 0; i < (n); i++)
+
+static void v_copy(long n, real *d, const real *s) { VLOOP(n) d[i] = s[i]; }
+static void v_add(long n, real *d, const real *s) { VLOOP(n) d[i] += s[i]; }
+static void v_sub(long n, real *d, const real *s) { VLOOP(n) d[i] -= s[i]; }
+static void v_mul(long n, real *d, const real *s) { VLOOP(n) d[i] *= s[i]; }
+static void v_sub2(long n, real *d, const real *s) { VLOOP(n) d[i] -= (real)2.0 * s[i]; }
+
+/* CPML.cpp:84-95 : temp = a; psi *= b; temp *= vec; psi += temp; vec += psi */
+static void cpml_apply(long n, real *vec, real *psi, const real *a, const real *b)
+{
+    VLOOP(n)
+    {
+        real t = a[i] * vec[i];
+        real p = psi[i] * b[i];
+        p = p + t;
+        psi[i] = p;
+        vec[i] = vec[i] + p;
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* coefficient preparation                                                    */
+/* ------------------------------------------------------------------------- */
+
+/* 1/((rho_i + rho_{i+s})/2); last point 1/rho_i; inf/nan -> 0 (Modelparameter.cpp:336-446, 633-639) */
+static void inverse_average_2pt(const grid_t *g, int axis, const real *m, real *out)
+{
+    long L = axis_len(g, axis);
+    for (long i = 0; i < g->n; i++) {
+        long x, y, z;
+        gcoord(g, i, &x, &y, &z);
+        long p = axis_pos(x, y, z, axis);
+        real s;
+        if (p + 1 < L) {
+            real h = (real)0.5;
+            s = h * m[i];
+            s += h * m[axis_idx(g, x, y, z, axis, p + 1)];
+        } else {
+            s = (real)1.0 * m[i];
+        }
+        real v = (real)1 / s;
+        if (isnan(v) || isinf(v)) v = 0;
+        out[i] = v;
+    }
+}
+
+/*
+ * 4-point average rows (Modelparameter.cpp:449-526).  axes (a0, a1) are the
+ * two axes of the plane; the row averages the points (p0,p1), (p0+1,p1),
+ * (p0,p1+1), (p0+1,p1+1) with edge fallbacks.
+ */
+static real avg4_row(const grid_t *g, const real *v, long x, long y, long z, int a0, int a1)
+{
+    long pX[5] = {0, x, x, x, x}, pY[5] = {0, y, y, y, y}, pZ[5] = {0, z, z, z, z};
+    long *P0 = a0 == 0 ? pX : (a0 == 1 ? pY : pZ);
+    long *P1 = a1 == 0 ? pX : (a1 == 1 ? pY : pZ);
+    P0[2] += 1;
+    P1[3] += 1;
+    P0[4] += 1;
+    P1[4] += 1;
+    long maxX = 0, maxY = 0, maxZ = 0;
+    for (int k = 0; k < 5; k++) {
+        if (pX[k] > maxX) maxX = pX[k];
+        if (pY[k] > maxY) maxY = pY[k];
+        if (pZ[k] > maxZ) maxZ = pZ[k];
+    }
+    int inX = maxX < g->NX, inY = maxY < g->NY, inZ = maxZ < g->NZ;
+    long row = gidx(g, x, y, z);
+    real q = (real)0.25, h = (real)0.5, s = 0;
+    if (inX && inY && inZ) {
+        s = q * v[row];
+        s += q * v[gidx(g, pX[2], pY[2], pZ[2])];
+        s += q * v[gidx(g, pX[3], pY[3], pZ[3])];
+        s += q * v[gidx(g, pX[4], pY[4], pZ[4])];
+    }
+    if (inX && !inY && inZ) s = h * v[gidx(g, maxX, pY[1], maxZ)] + h * v[row];
+    if (!inX && inY && inZ) s = h * v[gidx(g, pX[1], maxY, maxZ)] + h * v[row];
+    if (inX && inY && !inZ) s = h * v[gidx(g, maxX, maxY, pZ[1])] + h * v[row];
+    if (!inX && !inY && inZ) s = v[row];
+    if (inX && !inY && !inZ) s = v[row];
+    if (!inX && inY && !inZ) s = v[row];
+    return s;
+}
+
+/* ModelparameterSeismic.cpp:421-433: mu<1 -> 1 (in place), 1/avg(1/mu), <4 -> 0 */
+static void shear_average(const grid_t *g, real *mu, int a0, int a1, real *out)
+{
+    for (long i = 0; i < g->n; i++)
+        if (mu[i] < (real)1.0) mu[i] = (real)1.0;
+    real *inv = vnew(g->n);
+    for (long i = 0; i < g->n; i++) inv[i] = (real)1 / mu[i];
+    for (long i = 0; i < g->n; i++) {
+        long x, y, z;
+        gcoord(g, i, &x, &y, &z);
+        real v = (real)1 / avg4_row(g, inv, x, y, z, a0, a1);
+        if (v < (real)4.0) v = 0;
+        out[i] = v;
+    }
+    free(inv);
+}
+
+/* CPML.cpp:39-68 */
+static void cpml_profile(int W, real NPower, real fc, real vmax, real DT, real DH, int shift, real *a, real *b)
+{
+    real sh = shift ? (real)0.5 : (real)0;
+    real R = (real)0.0008;
+    real alpha_max = (real)(2.0 * M_PI * (fc / 2.0));
+    real d0 = (real)(-(NPower + 1) * vmax * log(R) / (2.0 * W * DH));
+    for (int i = 0; i < W; i++) {
+        real pos = (real)(W - i - sh) / (real)W;
+        real d = (real)(d0 * pow(pos, NPower));
+        real alpha = (real)(alpha_max * (1.0 - pos));
+        b[i] = (real)exp(-(d + alpha) * DT);
+        if (fabs(d) > 1.0e-6)
+            a[i] = (real)(d * (b[i] - 1.0) / (d + alpha));
+        else
+            a[i] = 0;
+    }
+}
+
+typedef struct {
+    real *a[3], *b[3], *ah[3], *bh[3]; /* full-grid coefficient vectors, 0 outside frame */
+} cpml_coef_t;
+
+/* CPML2D.cpp:228-270, CPML3D.cpp:290-352 */
+static void cpml_layout(const grid_t *g, int dim, int W, int fs, real NPower, real fc, real vmax, real DT, real DH,
+                        cpml_coef_t *c)
+{
+    real *pa = vnew(W), *pb = vnew(W), *pah = vnew(W), *pbh = vnew(W);
+    cpml_profile(W, NPower, fc, vmax, DT, DH, 0, pa, pb);
+    cpml_profile(W, NPower, fc, vmax, DT, DH, 1, pah, pbh);
+    for (int ax = 0; ax < 3; ax++) {
+        c->a[ax] = vnew(g->n);
+        c->b[ax] = vnew(g->n);
+        c->ah[ax] = vnew(g->n);
+        c->bh[ax] = vnew(g->n);
+    }
+    for (long i = 0; i < g->n; i++) {
+        long x, y, z;
+        gcoord(g, i, &x, &y, &z);
+        for (int ax = 0; ax < dim; ax++) {
+            int axis = (dim == 2) ? ax : ax; /* 0=x 1=y 2=z */
+            long p = axis_pos(x, y, z, axis), L = axis_len(g, axis);
+            long d = (L - 1 - p < p) ? (L - 1 - p) : p;
+            if (d >= W) continue;
+            if (p < W) {
+                if (axis == 1 && fs != 0) continue;
+                c->a[axis][i] = pa[d];
+                c->b[axis][i] = pb[d];
+                c->ah[axis][i] = pah[d];
+                c->bh[axis][i] = pbh[d];
+            } else {
+                c->a[axis][i] = pah[d];
+                c->b[axis][i] = pbh[d];
+                c->ah[axis][i] = pa[d];
+                c->bh[axis][i] = pb[d];
+            }
+        }
+    }
+    free(pa);
+    free(pb);
+    free(pah);
+    free(pbh);
+}
+
+/* ABS2D.cpp:110-178, ABS3D.cpp:150-215 */
+static real *abs_layout(const grid_t *g, int dim, int W, real coeffpct, int fs)
+{
+    real *damp = vnew(g->n);
+    real *coeff = vnew(W);
+    real amp = (real)(1.0 - coeffpct / 100.0);
+    real a = (real)sqrt(-log(amp) / ((double)W * W));
+    for (int j = 0; j < W; j++) coeff[j] = (real)exp(-(a * a * (real)(W - j) * (real)(W - j)));
+    for (long i = 0; i < g->n; i++) {
+        long x, y, z;
+        gcoord(g, i, &x, &y, &z);
+        long dx = (g->NX - 1 - x < x) ? g->NX - 1 - x : x;
+        long dy = (g->NY - 1 - y < y) ? g->NY - 1 - y : y;
+        long dz = (g->NZ - 1 - z < z) ? g->NZ - 1 - z : z;
+        real v = 1;
+        if (dim == 2) {
+            long mn = (dx < dy) ? dx : dy;
+            if (fs == 0) {
+                if (mn < W) v = coeff[mn];
+            } else {
+                if (y < W) {
+                    if (dx < W) v = coeff[dx];
+                } else if (mn < W)
+                    v = coeff[mn];
+            }
+        } else {
+            long mn = (dx < dy) ? dx : dy;
+            if (dz < mn) mn = dz;
+            if (fs == 0) {
+                if (mn < W) v = coeff[mn];
+            } else {
+                long mxz = !(dx < dz) ? dz : dx;
+                if (y < W) {
+                    if (dz < W || dx < W) v = coeff[mxz];
+                } else if (mn < W)
+                    v = coeff[mn];
+            }
+        }
+        damp[i] = v;
+    }
+    free(coeff);
+    return damp;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Ricker wavelet (Ricker.cpp:29-53)                                          */
+/* ------------------------------------------------------------------------- */
+void oracle_ricker(int NT, double DT, double FC, double AMP, double TSHIFT, real *out)
+{
+    real dt = (real)DT, fc = (real)FC, amp = (real)AMP, ts = (real)TSHIFT;
+    real help = (real)(1.5 / (double)fc + (double)ts);
+    real pf = (real)(M_PI * (double)fc);
+    for (int k = 0; k < NT; k++) {
+        real t = (real)k * dt;
+        real tau = (t - help) * pf;
+        real h2 = tau * tau;
+        real e = (real)exp(-(double)h2);
+        real w = (real)1 - (real)2.0 * h2;
+        out[k] = amp * w * e;
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* solver                                                                     */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+    int dim;     /* 2 or 3 */
+    int physics; /* 0 acoustic, 1 elastic, 2 tmem */
+    int NX, NY, NZ;
+    int NT;
+    int order;
+    int stencil;      /* 1: LAMA StencilMatrix operators */
+    int free_surface; /* FreeSurface key */
+    int damping;      /* 0 none, 1 ABS, 2 CPML */
+    int width;
+    int nthreads;
+    double DH, DT;
+    double damping_coeff, npower, fc_cpml, vmax_cpml;
+} oracle_params;
+
+enum { PH_ACOUSTIC = 0, PH_ELASTIC = 1, PH_TMEM = 2 };
+
+typedef struct {
+    csr_t Dxf, Dxb, Dyf, Dyb, Dzf, Dzb, DyfFS, DybFS;
+    int have_fs;
+} ops_t;
+
+static void ops_build(const grid_t *g, const oracle_params *p, ops_t *o)
+{
+    memset(o, 0, sizeof(*o));
+    real DH = (real)p->DH, DT = (real)p->DT;
+    int axes = p->dim == 3 ? 3 : 2;
+    csr_t *F[3] = {&o->Dxf, &o->Dyf, &o->Dzf}, *B[3] = {&o->Dxb, &o->Dyb, &o->Dzb};
+    for (int ax = 0; ax < axes; ax++) {
+        if (p->stencil) {
+            real s = DT / DH;
+            *F[ax] = build_stencil(g, ax, p->order, 0);
+            *B[ax] = build_stencil(g, ax, p->order, 1);
+            csr_scale(F[ax], s);
+            csr_scale(B[ax], s);
+        } else {
+            *F[ax] = build_forward(g, ax, p->order, DH);
+            *B[ax] = build_backward(g, ax, p->order, DH);
+            csr_scale(F[ax], DT);
+            csr_scale(B[ax], DT);
+        }
+    }
+    if (p->free_surface == 1) {
+        o->have_fs = 1;
+        o->DyfFS = build_dyf_fs(g, p->order, DH);
+        csr_scale(&o->DyfFS, DT);
+        if (p->physics == PH_ELASTIC) {
+            o->DybFS = build_dyb_fs(g, p->order, DH);
+            csr_scale(&o->DybFS, DT);
+        }
+    }
+}
+
+static void ops_free(ops_t *o)
+{
+    csr_t *all[8] = {&o->Dxf, &o->Dxb, &o->Dyf, &o->Dyb, &o->Dzf, &o->Dzb, &o->DyfFS, &o->DybFS};
+    for (int k = 0; k < 8; k++)
+        if (all[k]->rowptr) csr_free(all[k]);
+}
+
+/* export the assembled operators for brute-force checks: which 0..7 as in ops_t order */
+long oracle_operator(const oracle_params *p, int which, long *rowptr, long *col, real *val, long cap)
+{
+    grid_t g = {p->NX, p->NY, p->NZ, (long)p->NX * p->NY * p->NZ};
+    ops_t o;
+    ops_build(&g, p, &o);
+    csr_t *all[8] = {&o.Dxf, &o.Dxb, &o.Dyf, &o.Dyb, &o.Dzf, &o.Dzb, &o.DyfFS, &o.DybFS};
+    csr_t *m = all[which];
+    long nnz = m->rowptr ? m->rowptr[m->n] : -1;
+    if (nnz >= 0 && rowptr && nnz <= cap) {
+        memcpy(rowptr, m->rowptr, sizeof(long) * (size_t)(m->n + 1));
+        memcpy(col, m->col, sizeof(long) * (size_t)nnz);
+        memcpy(val, m->val, sizeof(real) * (size_t)nnz);
+    }
+    ops_free(&o);
+    return nnz;
+}
+
+/* export coefficient vectors for checks: kind 0 invRhoX 1 invRhoY 2 invRhoZ 3 muXY 4 muXZ 5 muYZ 6 M 7 mu(clamped) */
+int oracle_elastic_coefficients(const oracle_params *p, const real *vp, const real *vs, const real *rho, int kind,
+                                real *out)
+{
+    grid_t g = {p->NX, p->NY, p->NZ, (long)p->NX * p->NY * p->NZ};
+    real *M = vnew(g.n), *mu = vnew(g.n);
+    for (long i = 0; i < g.n; i++) {
+        M[i] = rho[i] * vp[i] * vp[i];
+        mu[i] = rho[i] * vs[i] * vs[i];
+    }
+    real *tmp = vnew(g.n);
+    switch (kind) {
+    case 0: inverse_average_2pt(&g, 0, rho, out); break;
+    case 1: inverse_average_2pt(&g, 1, rho, out); break;
+    case 2: inverse_average_2pt(&g, 2, rho, out); break;
+    case 3: shear_average(&g, mu, 0, 1, out); break;
+    case 4: shear_average(&g, mu, 0, 2, out); break;
+    case 5: shear_average(&g, mu, 1, 2, out); break;
+    case 6: memcpy(out, M, sizeof(real) * (size_t)g.n); break;
+    case 7:
+        shear_average(&g, mu, 0, 1, tmp);
+        memcpy(out, mu, sizeof(real) * (size_t)g.n);
+        break;
+    default: free(M); free(mu); free(tmp); return -1;
+    }
+    free(M);
+    free(mu);
+    free(tmp);
+    return 0;
+}
+
+int oracle_cpml_profile(int W, double NPower, double fc, double vmax, double DT, double DH, int shift, real *a, real *b)
+{
+    cpml_profile(W, (real)NPower, (real)fc, (real)vmax, (real)DT, (real)DH, shift, a, b);
+    return 0;
+}
+
+int oracle_abs_layout(const oracle_params *p, real *out)
+{
+    grid_t g = {p->NX, p->NY, p->NZ, (long)p->NX * p->NY * p->NZ};
+    real *d = abs_layout(&g, p->dim, p->width, (real)p->damping_coeff, p->free_surface);
+    memcpy(out, d, sizeof(real) * (size_t)g.n);
+    free(d);
+    return 0;
+}
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/*
+ * Run one shot.  Model arrays by physics:
+ *   acoustic: m0 = vp, m1 unused, m2 = rho
+ *   elastic : m0 = vp, m1 = vs,   m2 = rho
+ *   tmem    : m0 = epsilon_r, m1 = sigma, m2 = mu_r
+ * Source / receiver type: 1 = P (EZ for tmem), 2 = VX, 3 = VY, 4 = VZ.
+ * src_sig is [nsrc][NT]; seis is [nrec][NT], in receiver input order.
+ * fields_out (optional) receives the final wavefields:
+ *   acoustic 2D: vx vy p ; elastic 2D: vx vy sxx syy sxy ; tmem: hx hy ez ;
+ *   elastic 3D: vx vy vz sxx syy szz syz sxz sxy
+ * t_end (<= NT) allows truncated runs for timing; seis columns past it stay 0.
+ */
+int oracle_run(const oracle_params *p, const real *m0, const real *m1, const real *m2, int nsrc, const long *src_idx,
+               const int *src_type, const real *src_sig, int nrec, const long *rec_idx, const int *rec_type,
+               real *seis, real *fields_out, int t_end)
+{
+#ifdef _OPENMP
+    if (p->nthreads > 0) omp_set_num_threads(p->nthreads);
+#endif
+    if (p->dim == 3 && p->physics != PH_ELASTIC) return -2;
+    grid_t g = {p->NX, p->NY, p->NZ, (long)p->NX * p->NY * p->NZ};
+    long n = g.n;
+    int NT = p->NT;
+    if (t_end < 0 || t_end > NT) t_end = NT;
+    real DT = (real)p->DT;
+    int fs = p->free_surface == 1;
+
+    ops_t o;
+    ops_build(&g, p, &o);
+
+    /* model coefficients */
+    real *M = NULL, *mu = NULL, *irx = NULL, *iry = NULL, *irz = NULL, *muxy = NULL, *muxz = NULL, *muyz = NULL;
+    real *Ca = NULL, *Cb = NULL, *imxz = NULL, *imyz = NULL;
+    if (p->physics == PH_ACOUSTIC || p->physics == PH_ELASTIC) {
+        M = vnew(n);
+        for (long i = 0; i < n; i++) M[i] = m2[i] * m0[i] * m0[i];
+        irx = vnew(n);
+        iry = vnew(n);
+        inverse_average_2pt(&g, 0, m2, irx);
+        inverse_average_2pt(&g, 1, m2, iry);
+        if (p->dim == 3) {
+            irz = vnew(n);
+            inverse_average_2pt(&g, 2, m2, irz);
+        }
+        if (p->physics == PH_ELASTIC) {
+            mu = vnew(n);
+            for (long i = 0; i < n; i++) mu[i] = m2[i] * m1[i] * m1[i];
+            muxy = vnew(n);
+            shear_average(&g, mu, 0, 1, muxy);
+            if (p->dim == 3) {
+                muxz = vnew(n);
+                muyz = vnew(n);
+                shear_average(&g, mu, 0, 2, muxz);
+                shear_average(&g, mu, 1, 2, muyz);
+            }
+        }
+    } else {
+        real eps0 = (real)8.8541878176e-12, mu0 = (real)1.2566370614e-6;
+        real *eps = vnew(n), *mup = vnew(n);
+        for (long i = 0; i < n; i++) {
+            eps[i] = m0[i] * eps0;
+            mup[i] = m2[i] * mu0;
+        }
+        imxz = vnew(n);
+        imyz = vnew(n);
+        inverse_average_2pt(&g, 0, mup, imxz);
+        inverse_average_2pt(&g, 1, mup, imyz);
+        Ca = vnew(n);
+        Cb = vnew(n);
+        for (long i = 0; i < n; i++) {
+            /* ForwardSolverEM.cpp:14-68 */
+            real c = (real)0.5 / eps[i];
+            c *= m1[i];
+            c *= DT;
+            real cinv = (real)1 / (c + (real)1);
+            real ca = (real)1 - c;
+            ca *= cinv;
+            real cb = (real)1 / eps[i];
+            cb *= cinv;
+            Ca[i] = ca;
+            Cb[i] = cb;
+        }
+        free(eps);
+        free(mup);
+    }
+
+    /* free surface scaling (FreeSurfaceElastic.cpp:11-47) */
+    real *fsH = NULL, *fsV = NULL, *fsZero = NULL;
+    if (fs) {
+        fsZero = vnew(n);
+        for (long i = 0; i < n; i++) {
+            long x, y, z;
+            gcoord(&g, i, &x, &y, &z);
+            fsZero[i] = (y == 0) ? (real)0 : (real)1;
+        }
+        if (p->physics == PH_ELASTIC) {
+            fsH = vnew(n);
+            fsV = vnew(n);
+            for (long i = 0; i < n; i++) {
+                long x, y, z;
+                gcoord(&g, i, &x, &y, &z);
+                if (y != 0) continue;
+                real t = M[i] - (real)2 * mu[i];
+                fsV[i] = t;
+                real t2 = t * t;
+                t2 /= M[i];
+                t2 *= (real)-1;
+                fsH[i] = t2;
+            }
+        }
+    }
+
+    cpml_coef_t cp;
+    int use_cpml = p->damping == 2;
+    if (use_cpml)
+        cpml_layout(&g, p->dim, p->width, p->free_surface, (real)p->npower, (real)p->fc_cpml, (real)p->vmax_cpml, DT,
+                    (real)p->DH, &cp);
+    real *damp = NULL;
+    if (p->damping == 1) damp = abs_layout(&g, p->dim, p->width, (real)p->damping_coeff, p->free_surface);
+
+    /* wavefields and scratch */
+    int nf = p->physics == PH_ELASTIC ? (p->dim == 3 ? 9 : 5) : 3;
+    real *F[9];
+    for (int k = 0; k < nf; k++) F[k] = vnew(n);
+    real *upd = vnew(n), *tmp = vnew(n), *vxx = vnew(n), *vyy = vnew(n), *vzz = vnew(n);
+    real *psi[18];
+    for (int k = 0; k < 18; k++) psi[k] = use_cpml ? vnew(n) : NULL;
+
+    real *ax = use_cpml ? cp.a[0] : NULL, *bx = use_cpml ? cp.b[0] : NULL;
+    real *axh = use_cpml ? cp.ah[0] : NULL, *bxh = use_cpml ? cp.bh[0] : NULL;
+    real *ay = use_cpml ? cp.a[1] : NULL, *by = use_cpml ? cp.b[1] : NULL;
+    real *ayh = use_cpml ? cp.ah[1] : NULL, *byh = use_cpml ? cp.bh[1] : NULL;
+    real *az = use_cpml ? cp.a[2] : NULL, *bz = use_cpml ? cp.b[2] : NULL;
+    real *azh = use_cpml ? cp.ah[2] : NULL, *bzh = use_cpml ? cp.bh[2] : NULL;
+
+    for (int t = 0; t < t_end; t++) {
+        if (p->physics == PH_ACOUSTIC) {
+            /* ForwardSolver2Dacoustic.cpp:91-194 ; F: vx vy p */
+            real *vx = F[0], *vy = F[1], *pr = F[2];
+            spmv(&o.Dxf, pr, upd);
+            if (use_cpml) cpml_apply(n, upd, psi[0], axh, bxh);
+            v_mul(n, upd, irx);
+            v_add(n, vx, upd);
+            spmv(fs ? &o.DyfFS : &o.Dyf, pr, upd);
+            if (use_cpml) cpml_apply(n, upd, psi[1], ayh, byh);
+            v_mul(n, upd, iry);
+            v_add(n, vy, upd);
+            spmv(&o.Dxb, vx, upd);
+            if (use_cpml) cpml_apply(n, upd, psi[2], ax, bx);
+            spmv(&o.Dyb, vy, tmp);
+            if (use_cpml) cpml_apply(n, tmp, psi[3], ay, by);
+            v_add(n, upd, tmp);
+            v_mul(n, upd, M);
+            v_add(n, pr, upd);
+            if (damp) {
+                v_mul(n, pr, damp);
+                v_mul(n, vx, damp);
+                v_mul(n, vy, damp);
+            }
+            if (fs) v_mul(n, pr, fsZero);
+        } else if (p->physics == PH_ELASTIC && p->dim == 2) {
+            /* ForwardSolver2Delastic.cpp:117-294 ; F: vx vy sxx syy sxy */
+            real *vx = F[0], *vy = F[1], *sxx = F[2], *syy = F[3], *sxy = F[4];
+            spmv(&o.Dxf, sxx, upd);
+            if (use_cpml) cpml_apply(n, upd, psi[0], axh, bxh); /* sxx_x */
+            spmv(fs ? &o.DybFS : &o.Dyb, sxy, tmp);
+            if (use_cpml) cpml_apply(n, tmp, psi[1], ay, by); /* sxy_y */
+            v_add(n, upd, tmp);
+            v_mul(n, upd, irx);
+            v_add(n, vx, upd);
+
+            spmv(&o.Dxb, sxy, upd);
+            if (use_cpml) cpml_apply(n, upd, psi[2], ax, bx); /* sxy_x */
+            spmv(fs ? &o.DyfFS : &o.Dyf, syy, tmp);
+            if (use_cpml) cpml_apply(n, tmp, psi[3], ayh, byh); /* syy_y */
+            v_add(n, upd, tmp);
+            v_mul(n, upd, iry);
+            v_add(n, vy, upd);
+
+            spmv(&o.Dxb, vx, vxx);
+            spmv(&o.Dyb, vy, vyy);
+            if (use_cpml) {
+                cpml_apply(n, vxx, psi[4], ax, bx);
+                cpml_apply(n, vyy, psi[5], ay, by);
+            }
+            v_copy(n, upd, vxx);
+            v_add(n, upd, vyy);
+            v_mul(n, upd, M);
+            v_add(n, sxx, upd);
+            v_add(n, syy, upd);
+            v_copy(n, upd, vyy);
+            v_mul(n, upd, mu);
+            v_sub2(n, sxx, upd);
+            v_copy(n, upd, vxx);
+            v_mul(n, upd, mu);
+            v_sub2(n, syy, upd);
+
+            spmv(&o.Dyf, vx, upd);
+            if (use_cpml) cpml_apply(n, upd, psi[6], ayh, byh); /* vxy */
+            spmv(&o.Dxf, vy, tmp);
+            if (use_cpml) cpml_apply(n, tmp, psi[7], axh, bxh); /* vyx */
+            v_add(n, upd, tmp);
+            v_mul(n, upd, muxy);
+            v_add(n, sxy, upd);
+
+            if (fs) {
+                /* FreeSurface2Delastic.cpp:14-45 */
+                for (long i = 0; i < g.NX; i++) {
+                    real a = fsH[i] * vxx[i];
+                    sxx[i] += a;
+                    real b = fsV[i] * vyy[i];
+                    sxx[i] -= b;
+                }
+                v_mul(n, syy, fsZero);
+            }
+            if (damp) {
+                v_mul(n, sxx, damp);
+                v_mul(n, syy, damp);
+                v_mul(n, sxy, damp);
+                v_mul(n, vx, damp);
+                v_mul(n, vy, damp);
+            }
+        } else if (p->physics == PH_ELASTIC && p->dim == 3) {
+            /* ForwardSolver3Delastic.cpp:120-416 ; F: vx vy vz sxx syy szz syz sxz sxy */
+            real *vx = F[0], *vy = F[1], *vz = F[2], *sxx = F[3], *syy = F[4], *szz = F[5], *syz = F[6], *sxz = F[7],
+                 *sxy = F[8];
+            spmv(&o.Dxf, sxx, upd);
+            if (use_cpml) cpml_apply(n, upd, psi[0], axh, bxh); /* sxx_x */
+            spmv(fs ? &o.DybFS : &o.Dyb, sxy, tmp);
+            if (use_cpml) cpml_apply(n, tmp, psi[1], ay, by); /* sxy_y */
+            v_add(n, upd, tmp);
+            spmv(&o.Dzb, sxz, tmp);
+            if (use_cpml) cpml_apply(n, tmp, psi[2], az, bz); /* sxz_z */
+            v_add(n, upd, tmp);
+            v_mul(n, upd, irx);
+            v_add(n, vx, upd);
+
+            spmv(&o.Dxb, sxy, upd);
+            if (use_cpml) cpml_apply(n, upd, psi[3], ax, bx); /* sxy_x */
+            spmv(fs ? &o.DyfFS : &o.Dyf, syy, tmp);
+            if (use_cpml) cpml_apply(n, tmp, psi[4], ayh, byh); /* syy_y */
+            v_add(n, upd, tmp);
+            spmv(&o.Dzb, syz, tmp);
+            if (use_cpml) cpml_apply(n, tmp, psi[5], az, bz); /* syz_z */
+            v_add(n, upd, tmp);
+            v_mul(n, upd, iry);
+            v_add(n, vy, upd);
+
+            spmv(&o.Dxb, sxz, upd);
+            if (use_cpml) cpml_apply(n, upd, psi[6], ax, bx); /* sxz_x */
+            spmv(fs ? &o.DybFS : &o.Dyb, syz, tmp);
+            if (use_cpml) cpml_apply(n, tmp, psi[7], ay, by); /* syz_y */
+            v_add(n, upd, tmp);
+            spmv(&o.Dzf, szz, tmp);
+            if (use_cpml) cpml_apply(n, tmp, psi[8], azh, bzh); /* szz_z */
+            v_add(n, upd, tmp);
+            v_mul(n, upd, irz);
+            v_add(n, vz, upd);
+
+            spmv(&o.Dxb, vx, vxx);
+            spmv(&o.Dyb, vy, vyy);
+            spmv(&o.Dzb, vz, vzz);
+            if (use_cpml) {
+                cpml_apply(n, vxx, psi[9], ax, bx);
+                cpml_apply(n, vyy, psi[10], ay, by);
+                cpml_apply(n, vzz, psi[11], az, bz);
+            }
+            v_copy(n, upd, vxx);
+            v_add(n, upd, vyy);
+            v_add(n, upd, vzz);
+            v_mul(n, upd, M);
+            v_add(n, sxx, upd);
+            v_add(n, syy, upd);
+            v_add(n, szz, upd);
+            VLOOP(n) upd[i] = vyy[i] + vzz[i];
+            v_mul(n, upd, mu);
+            v_sub2(n, sxx, upd);
+            VLOOP(n) upd[i] = vxx[i] + vzz[i];
+            v_mul(n, upd, mu);
+            v_sub2(n, syy, upd);
+            VLOOP(n) upd[i] = vxx[i] + vyy[i];
+            v_mul(n, upd, mu);
+            v_sub2(n, szz, upd);
+
+            spmv(&o.Dyf, vx, upd);
+            if (use_cpml) cpml_apply(n, upd, psi[12], ayh, byh); /* vxy */
+            spmv(&o.Dxf, vy, tmp);
+            if (use_cpml) cpml_apply(n, tmp, psi[13], axh, bxh); /* vyx */
+            v_add(n, upd, tmp);
+            v_mul(n, upd, muxy);
+            v_add(n, sxy, upd);
+
+            spmv(&o.Dzf, vx, upd);
+            if (use_cpml) cpml_apply(n, upd, psi[14], azh, bzh); /* vxz */
+            spmv(&o.Dxf, vz, tmp);
+            if (use_cpml) cpml_apply(n, tmp, psi[15], axh, bxh); /* vzx */
+            v_add(n, upd, tmp);
+            v_mul(n, upd, muxz);
+            v_add(n, sxz, upd);
+
+            spmv(&o.Dzf, vy, upd);
+            if (use_cpml) cpml_apply(n, upd, psi[16], azh, bzh); /* vyz */
+            spmv(&o.Dyf, vz, tmp);
+            if (use_cpml) cpml_apply(n, tmp, psi[17], ayh, byh); /* vzy */
+            v_add(n, upd, tmp);
+            v_mul(n, upd, muyz);
+            v_add(n, syz, upd);
+
+            if (fs) {
+                /* FreeSurface3Delastic.cpp: setSurfaceZero(Syy) then the exchange */
+                v_mul(n, syy, fsZero);
+                for (long i = 0; i < g.NX * g.NZ; i++) {
+                    real s = vxx[i] + vzz[i];
+                    real a = fsH[i] * s;
+                    sxx[i] += a;
+                    szz[i] += a;
+                    real b = fsV[i] * vyy[i];
+                    sxx[i] -= b;
+                    szz[i] -= b;
+                }
+            }
+            if (damp) {
+                real *all[9] = {sxx, syy, szz, sxy, sxz, syz, vx, vy, vz};
+                for (int k = 0; k < 9; k++) v_mul(n, all[k], damp);
+            }
+        } else {
+            /* ForwardSolver2Dtmem.cpp:108-171 ; F: hx hy ez */
+            real *hx = F[0], *hy = F[1], *ez = F[2];
+            spmv(&o.Dyf, ez, upd);
+            if (use_cpml) cpml_apply(n, upd, psi[0], ayh, byh); /* ezy */
+            v_mul(n, upd, imyz);
+            v_sub(n, hx, upd);
+            spmv(&o.Dxf, ez, tmp);
+            if (use_cpml) cpml_apply(n, tmp, psi[1], axh, bxh); /* ezx */
+            VLOOP(n) upd[i] = -tmp[i];
+            v_mul(n, upd, imxz);
+            v_sub(n, hy, upd);
+            spmv(&o.Dxb, hy, upd);
+            spmv(&o.Dyb, hx, tmp);
+            if (use_cpml) {
+                cpml_apply(n, upd, psi[2], ax, bx); /* hyx */
+                cpml_apply(n, tmp, psi[3], ay, by); /* hxy */
+            }
+            v_sub(n, upd, tmp);
+            v_mul(n, upd, Cb);
+            VLOOP(n) ez[i] = Ca[i] * ez[i] + upd[i];
+            if (damp) {
+                v_mul(n, ez, damp);
+                v_mul(n, hx, damp);
+                v_mul(n, hy, damp);
+            }
+        }
+
+        /* sources: P, VX, VY, VZ in order, each in input order (SourceReceiverImplSeismic.cpp:89-105) */
+        for (int ty = 1; ty <= 4; ty++)
+            for (int s = 0; s < nsrc; s++) {
+                if (src_type[s] != ty) continue;
+                real v = src_sig[(long)s * NT + t];
+                long i = src_idx[s];
+                if (p->physics == PH_ACOUSTIC) {
+                    if (ty == 1) F[2][i] += v;
+                    if (ty == 2) F[0][i] += v;
+                    if (ty == 3) F[1][i] += v;
+                } else if (p->physics == PH_ELASTIC && p->dim == 2) {
+                    if (ty == 1) {
+                        F[2][i] += v;
+                        F[3][i] += v;
+                    }
+                    if (ty == 2) F[0][i] += v;
+                    if (ty == 3) F[1][i] += v;
+                } else if (p->physics == PH_ELASTIC) {
+                    if (ty == 1) {
+                        F[3][i] += v;
+                        F[4][i] += v;
+                        F[5][i] += v;
+                    }
+                    if (ty == 2) F[0][i] += v;
+                    if (ty == 3) F[1][i] += v;
+                    if (ty == 4) F[2][i] += v;
+                } else {
+                    if (ty == 1) F[2][i] += v;
+                }
+            }
+        /* receivers (SourceReceiverImplSeismic.cpp:34-47) */
+        for (int r = 0; r < nrec; r++) {
+            long i = rec_idx[r];
+            int ty = rec_type[r];
+            real v = 0;
+            if (p->physics == PH_ACOUSTIC) {
+                v = ty == 1 ? F[2][i] : (ty == 2 ? F[0][i] : F[1][i]);
+            } else if (p->physics == PH_ELASTIC && p->dim == 2) {
+                if (ty == 1) {
+                    v = F[2][i];
+                    v += F[3][i];
+                    v *= (real)0.5;
+                } else
+                    v = ty == 2 ? F[0][i] : F[1][i];
+            } else if (p->physics == PH_ELASTIC) {
+                if (ty == 1) {
+                    v = F[3][i];
+                    v += F[4][i];
+                    v += F[5][i];
+                    v /= (real)3;
+                } else
+                    v = ty == 2 ? F[0][i] : (ty == 3 ? F[1][i] : F[2][i]);
+            } else {
+                v = F[2][i];
+            }
+            seis[(long)r * NT + t] = v;
+        }
+    }
+
+    if (fields_out)
+        for (int k = 0; k < nf; k++) memcpy(fields_out + (long)k * n, F[k], sizeof(real) * (size_t)n);
+
+    for (int k = 0; k < nf; k++) free(F[k]);
+    free(upd);
+    free(tmp);
+    free(vxx);
+    free(vyy);
+    free(vzz);
+    for (int k = 0; k < 18; k++) free(psi[k]);
+    if (use_cpml)
+        for (int ax2 = 0; ax2 < 3; ax2++) {
+            free(cp.a[ax2]);
+            free(cp.b[ax2]);
+            free(cp.ah[ax2]);
+            free(cp.bh[ax2]);
+        }
+    free(damp);
+    free(M);
+    free(mu);
+    free(irx);
+    free(iry);
+    free(irz);
+    free(muxy);
+    free(muxz);
+    free(muyz);
+    free(Ca);
+    free(Cb);
+    free(imxz);
+    free(imyz);
+    free(fsH);
+    free(fsV);
+    free(fsZero);
+    ops_free(&o);
+    return 0;
+}
