@@ -1,0 +1,171 @@
+"""Benchmark of the MI355X forward-solver time loop.
+
+Workload (one "step" of this benchmark = one full shot of the 2D elastic P-SV
+configuration): 1000 x 1000 grid, dx 1 m, dt 1e-4 s, 4000 time steps, order-8
+staggered operators, 30-cell CPML, 10 layers + 3 % Gaussian random
+perturbations (seeds 2 / 102), one 30 Hz Ricker VY force source, 500 VX + 500
+VY receivers.  Synthetic data, FP32 arithmetic, inputs resident in HBM before
+the timed region.
+
+Printed JSON line: metric = grid-cell updates/s of the fused batched path;
+roofline of the dominant kernel (HIP-event timing on the solver's stream);
+the same workload on the reference-sequence GPU path ("plain", one kernel per
+LAMA operation) for the batched-vs-plain ratio; and the CPU baseline: the
+oracle (C restatement with the reference's operator sequence, OpenMP) on a
+bounded sample of the same workload, timed on this host's cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (ROOT, os.path.join(ROOT, "wave-simulation_amd")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+HBM_PEAK_GBS = 8000.0
+# algorithmic bytes per 2D elastic cell-update: 5 fields read + written, 5 coefficients read (FP32)
+BYTES_PER_CELL_UPDATE = 4 * (2 * 5 + 5)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3, help="timed shots (each = 4000 time steps)")
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--graph", type=int, default=50, help="time steps per hipGraph segment")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--plain-steps", type=int, default=400)
+    ap.add_argument("--cpu-steps", type=int, default=150)
+    return ap.parse_args()
+
+
+def cpu_baseline(case, nsteps):
+    """Oracle (reference operator sequence, FP32, OpenMP) on a truncated run of the same workload."""
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle import pyoracle as po
+    from _cases import oracle_params
+
+    threads = min(16, os.cpu_count() or 1)
+    p = oracle_params(case, precision_threads=threads)
+    sh = case.shots[0]
+    args = (case.m0, case.m1, case.m2, sh.src_index, sh.src_type, sh.src_signal, sh.rec_index, sh.rec_type)
+    t0 = time.perf_counter()
+    po.run(p, *args, t_end=10)
+    t1 = time.perf_counter()
+    po.run(p, *args, t_end=10 + nsteps)
+    t2 = time.perf_counter()
+    loop = (t2 - t1) - (t1 - t0)  # setup and the first 10 steps cancel
+    rate = case.cells * nsteps / loop
+    return {"value": rate, "unit": "grid-cell updates/s", "cores": threads, "kind": "port",
+            "sample": f"{nsteps} time steps of the same 1000x1000 order-8 CPML shot (loop only, setup subtracted)",
+            "seconds": round(loop, 3)}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported in this build "
+              "(shots shard across GPUs with no in-loop communication; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+
+    import wavesim
+    from wavesim import cases
+
+    case = cases.elastic_c2()
+    NT = case.settings["NT"]
+    sol = cases.make_solver(case, steps_per_graph=args.graph)
+
+    for _ in range(args.warmup):
+        sol.reset()
+        sol.run(0, NT)
+    sol.synchronize()
+    t0 = time.perf_counter()
+    gpu_ms = 0.0
+    for _ in range(args.steps):
+        sol.reset()
+        sol.run(0, NT)
+        gpu_ms += sol.last_run_ms()
+    sol.synchronize()
+    wall = time.perf_counter() - t0
+    seis = sol.seismograms(0)
+    assert np.isfinite(seis).all() and np.abs(seis).max() > 0
+
+    updates = case.cells * NT * args.steps
+    value = updates / wall
+    ms_per_step = wall * 1e3 / args.steps
+    step_bytes = BYTES_PER_CELL_UPDATE * case.cells * NT
+
+    # per-kernel timing pass (eager launches between hipEvents on the solver stream)
+    import ctypes as C
+
+    L = wavesim.load()
+    wavesim._abi.check(L.wv_set_kernel_timing(sol._h, 1))
+    sol.reset()
+    sol.run(0, NT)
+    sol.synchronize()
+    kernels = []
+    for kind in range(3):
+        name = C.c_char_p()
+        avg = C.c_double()
+        n = C.c_longlong()
+        b = C.c_double()
+        wavesim._abi.check(L.wv_kernel_stats(sol._h, kind, C.byref(name), C.byref(avg), C.byref(n), C.byref(b)))
+        kernels.append(dict(name=name.value.decode(), avg_ms=avg.value, launches=n.value, bytes=b.value))
+    wavesim._abi.check(L.wv_set_kernel_timing(sol._h, 0))
+    dom = max(kernels, key=lambda k: k["avg_ms"] * k["launches"])
+    achieved = dom["bytes"] / (dom["avg_ms"] * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom["name"],
+                "kernel_avg_us": round(dom["avg_ms"] * 1e3, 3),
+                "bytes_per_launch": dom["bytes"]}
+
+    out = {
+        "metric": "grid-cell updates/s (2D elastic order-8 fused, 1000x1000, 4000 steps)",
+        "value": round(value, 1),
+        "unit": "grid-cell updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "2D elastic P-SV order 8, 1000x1000, dx 1 m, dt 1e-4 s, 4000 steps, 30-cell CPML, "
+                               "10 layers + 3% Gaussian perturbation, 1 VY Ricker 30 Hz shot, 1000 receivers",
+                   "path": "fused", "steps_per_graph": args.graph, "shots_per_step": 1},
+        "hbm_frac_step": round(step_bytes * args.steps / wall / 1e9 / HBM_PEAK_GBS, 4),
+        "gpu_event_ms_per_step": round(gpu_ms / args.steps, 3),
+        "kernels": [{k: (round(v, 5) if isinstance(v, float) else v) for k, v in d.items()} for d in kernels],
+        "roofline": roofline,
+    }
+
+    if not args.no_plain:
+        ps = args.plain_steps
+        plain = cases.make_solver(case, path="plain")
+        plain.run(0, 20)
+        plain.synchronize()
+        t1 = time.perf_counter()
+        plain.run(20, 20 + ps)
+        plain.synchronize()
+        tp = time.perf_counter() - t1
+        plain_shot_s = tp * NT / ps
+        out["plain"] = {"shots_per_s": round(1.0 / plain_shot_s, 4), "fused_shots_per_s": round(args.steps / wall, 4),
+                        "ratio": round((args.steps / wall) * plain_shot_s, 2),
+                        "sample": f"{ps} time steps, scaled to {NT}"}
+
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(case, args.cpu_steps)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

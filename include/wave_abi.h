@@ -1,0 +1,126 @@
+/*
+ * wave_abi.h -- C ABI of the MI355X forward-solver time loop (libwavehip.so).
+ *
+ * This is the drop-in boundary for WAVE-Simulation's ForwardSolver /
+ * ForwardSolverEM hot path.  Each entry point replaces one method of the
+ * reference interface (paths relative to the WAVE-Simulation repository):
+ *
+ *   wv_create          ForwardSolver::Factory<ValueType>::Create(dimension, type)
+ *                      src/ForwardSolver/ForwardSolverFactory.cpp:4-66
+ *   wv_init            ForwardSolver::initForwardSolver(config, derivatives, wavefield, model, coords, ctx, DT)
+ *                      src/ForwardSolver/ForwardSolver.hpp:52 (+ Derivatives init FDTD2D.cpp:49-59,
+ *                      Wavefields init Wavefields2Delastic.cpp:30-38, CPML/ABS init ForwardSolver.cpp:34-54)
+ *   wv_set_model       Modelparameter::prepareForModelling + ForwardSolver::prepareForModelling(model, DT)
+ *                      src/Modelparameter/Elastic.cpp:28-47, ForwardSolver.hpp:42, ForwardSolver2Dtmem.cpp:77-84
+ *   wv_set_acquisition Sources::init / Receivers::init for one shot
+ *                      src/Acquisition/Sources.cpp:16-46, Receivers.cpp:14-33
+ *   wv_reset           Wavefields::resetWavefields + ForwardSolver::resetCPML
+ *                      src/Wavefields/Wavefields2Delastic.cpp:104-113, ForwardSolver.hpp:40
+ *   wv_run             ForwardSolver::run(receiver, sources, model, wavefield, derivatives, t)
+ *                      for t in [tStart, tEnd)   src/ForwardSolver/ForwardSolver.hpp:38, Simulation.cpp:444-476
+ *   wv_get_seismograms receivers.getSeismogramHandler() data, src/Acquisition/Seismogram.cpp:82-156
+ *   wv_get_wavefield   Wavefields::getRef*() (read back one component)
+ *   wv_last_error      the message of the exception the reference would have thrown
+ *
+ * Conventions: status 0 = ok, negative = error (WV_E*); the message is then
+ * available from wv_last_error() (thread-local).  Host pointers only; the
+ * handle owns all device memory and one HIP stream (not thread-safe; separate
+ * handles may be used concurrently).  Grid index i = x + z*NX + y*NX*NZ
+ * (x fastest, y = depth slowest), as Coordinates.cpp:687.
+ */
+#ifndef WAVE_ABI_H
+#define WAVE_ABI_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WV_OK 0
+#define WV_EINVAL (-1)   /* invalid argument or unsupported combination */
+#define WV_ESTATE (-2)   /* call out of order (e.g. run before init)   */
+#define WV_EDEVICE (-3)  /* HIP runtime error                           */
+#define WV_ENOMEM (-4)   /* device allocation failed                    */
+
+/* operator edge mode */
+#define WV_OPS_SPARSE 0  /* sparse matrices with edge order reduction (Derivatives.cpp:129-1234) */
+#define WV_OPS_STENCIL 1 /* LAMA StencilMatrix: points outside the grid dropped (FDTD2D.cpp:156-185) */
+
+/* execution path */
+#define WV_PATH_FUSED 0  /* hand-written tiled stencil kernels, all shots per launch */
+#define WV_PATH_PLAIN 1  /* reference operator sequence: CSR SpMV + vector passes, shot by shot */
+
+typedef struct wv_solver wv_solver;
+
+typedef struct {
+    int NX, NY, NZ;        /* grid (NZ = 1 in 2D)                                  */
+    int NT;                /* number of time steps (seismogram length)             */
+    double DH, DT;         /* spacing [m], time step [s]                          */
+    int spatial_fd_order;  /* 2,4,6,8,10,12                                       */
+    int ops_mode;          /* WV_OPS_SPARSE (reference default) or WV_OPS_STENCIL  */
+    int free_surface;      /* FreeSurface key (0 or 1)                            */
+    int damping_boundary;  /* DampingBoundary key: 0 none, 1 ABS, 2 CPML          */
+    int boundary_width;    /* BoundaryWidth                                        */
+    double damping_coeff;  /* DampingCoeff (ABS)                                  */
+    double npower;         /* NPower (CPML)                                       */
+    double fc_cpml;        /* CenterFrequencyCPML                                  */
+    double vmax_cpml;      /* VMaxCPML                                             */
+    int nshots;            /* shots simulated together (>= 1)                      */
+    int path;              /* WV_PATH_FUSED or WV_PATH_PLAIN                      */
+    int steps_per_graph;   /* time steps per captured hipGraph segment (0 = none) */
+} wv_config;
+
+/* dimension "2D"/"3D", type "acoustic"/"elastic"/"tmem" (case-insensitive) */
+int wv_create(const char *dimension, const char *type, wv_solver **out);
+int wv_destroy(wv_solver *s);
+
+int wv_init(wv_solver *s, const wv_config *cfg);
+
+/* model vectors [NX*NY*NZ] in grid index order:
+ *   acoustic: m0 = velocityP, m1 ignored (may be NULL), m2 = density
+ *   elastic : m0 = velocityP, m1 = velocityS, m2 = density
+ *   tmem    : m0 = epsilon_r, m1 = sigma [S/m], m2 = mu_r                  */
+int wv_set_model(wv_solver *s, const float *m0, const float *m1, const float *m2);
+
+/* Acquisition of one shot.  type: 1 = P (EZ for tmem), 2 = VX, 3 = VY, 4 = VZ.
+ * src_signal is [nsrc][NT] (already scaled, as the reference's source seismogram).
+ * Receivers keep the order given; seismograms come back in that order.      */
+int wv_set_acquisition(wv_solver *s, int shot, int nsrc, const long long *src_index, const int *src_type,
+                       const float *src_signal, int nrec, const long long *rec_index, const int *rec_type);
+
+int wv_reset(wv_solver *s);
+
+/* advance all shots over time steps [t_start, t_end); asynchronous on the handle's stream */
+int wv_run(wv_solver *s, int t_start, int t_end);
+int wv_synchronize(wv_solver *s);
+
+/* seismograms of one shot, [nrec][NT] in receiver order (synchronizes) */
+int wv_get_seismograms(wv_solver *s, int shot, float *out);
+
+/* one wavefield component of one shot, [NX*NY*NZ] (synchronizes).
+ * field order: acoustic vx vy p ; elastic 2D vx vy sxx syy sxy ; tmem hx hy ez ;
+ *              elastic 3D vx vy vz sxx syy szz syz sxz sxy                    */
+int wv_get_wavefield(wv_solver *s, int shot, int field, float *out);
+int wv_set_wavefield(wv_solver *s, int shot, int field, const float *in);
+
+/* elapsed GPU time [ms] of the last wv_run (hipEvents on the handle's stream) */
+int wv_last_run_ms(wv_solver *s, double *ms);
+
+/* Per-kernel timing.  With wv_set_kernel_timing(s, 1), wv_run launches every
+ * kernel eagerly between hipEvents on the handle's stream and accumulates the
+ * elapsed time per kernel kind; wv_kernel_stats then reports, for kind
+ * 0 = velocity-type phase, 1 = stress-type phase, 2 = source/receiver epilogue:
+ * the kernel name, its average duration [ms] per launch, the number of
+ * launches, and the algorithmic HBM bytes one launch must move (every field
+ * and coefficient of the batch read once, updated fields written once, CPML
+ * memory variables read and written on frame cells).  Returns WV_EINVAL for an
+ * unknown kind.  No counterpart in the reference (instrumentation only). */
+int wv_set_kernel_timing(wv_solver *s, int on);
+int wv_kernel_stats(wv_solver *s, int kind, const char **name, double *avg_ms, long long *launches,
+                    double *bytes_per_launch);
+
+const char *wv_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

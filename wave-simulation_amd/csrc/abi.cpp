@@ -1,0 +1,848 @@
+// C ABI of the MI355X forward solver (include/wave_abi.h): handle management,
+// model/acquisition upload and the time loop driver.
+#include "../../include/wave_abi.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cctype>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "plain.hpp"
+#include "prep.hpp"
+#include "wave_internal.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                            \
+    do {                                                                                         \
+        hipError_t e_ = (expr);                                                                  \
+        if (e_ != hipSuccess)                                                                    \
+            return fail(WV_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));          \
+    } while (0)
+
+std::string lower(const char *s)
+{
+    std::string r = s ? s : "";
+    for (auto &c : r) c = (char)std::tolower((unsigned char)c);
+    return r;
+}
+
+int round_up(int v, int m) { return (v + m - 1) / m * m; }
+
+struct ShotAcq {
+    std::vector<long long> sidx;
+    std::vector<int> stype;
+    std::vector<float> ssig;
+    std::vector<long long> ridx;
+    std::vector<int> rtype;
+};
+
+}  // namespace
+
+struct wv_solver {
+    int dim = 2;
+    int physics = wv::ELASTIC;
+    wv_config cfg{};
+    bool inited = false;
+    bool model_ok = false;
+    hipStream_t st = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool ran = false;
+
+    wv::Layout2D L{};
+    int H = 0;
+    int nfields = 0;
+    std::vector<void *> allocs;
+
+    // device buffers
+    float *field[9] = {};
+    float *coef[5] = {};
+    float *damp = nullptr, *fsH = nullptr, *fsV = nullptr;
+    float *tap[6] = {};  // xf xb yf yb yf_v yb_v
+    float *cpx[4] = {}, *cpy[4] = {};
+    int *xi = nullptr, *yi = nullptr;
+    float *psx[4] = {}, *psy[4] = {};
+    size_t psx_n = 0, psy_n = 0;
+
+    // acquisition
+    std::vector<ShotAcq> shots;
+    bool acq_dirty = true;
+    std::vector<void *> acq_allocs;
+    int *d_src_ptr = nullptr, *d_src_type = nullptr, *d_rec_ptr = nullptr, *d_rec_type = nullptr;
+    long long *d_src_cell = nullptr, *d_rec_cell = nullptr;
+    float *d_src_sig = nullptr, *d_seis = nullptr;
+    int rec_total = 0;
+    std::vector<int> rec_ptr_h;
+    int *d_tbase = nullptr;
+
+    // graph of steps_per_graph time steps
+    hipGraphExec_t gexec = nullptr;
+    int gsteps = 0;
+
+    // reference-sequence path
+    wv::PlainSolver *plain = nullptr;
+
+    // per-kernel timing (wv_set_kernel_timing)
+    bool ktiming = false;
+    std::vector<hipEvent_t> kev;  // 4 events per step of a batch
+    double kms[3] = {0, 0, 0};
+    long long klaunch[3] = {0, 0, 0};
+
+    ~wv_solver();
+    template <typename T>
+    int dalloc(T **p, size_t n, std::vector<void *> &list)
+    {
+        void *q = nullptr;
+        if (n == 0) n = 1;
+        if (hipMalloc(&q, n * sizeof(T)) != hipSuccess) return fail(WV_ENOMEM, "hipMalloc failed");
+        if (hipMemsetAsync(q, 0, n * sizeof(T), st) != hipSuccess) return fail(WV_EDEVICE, "hipMemset failed");
+        list.push_back(q);
+        *p = (T *)q;
+        return 0;
+    }
+    template <typename T>
+    int upload(T **p, const std::vector<T> &h, std::vector<void *> &list)
+    {
+        int rc = dalloc(p, h.size(), list);
+        if (rc) return rc;
+        if (!h.empty() && hipMemcpyAsync(*p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, st) != hipSuccess)
+            return fail(WV_EDEVICE, "hipMemcpy failed");
+        return 0;
+    }
+    void free_list(std::vector<void *> &list)
+    {
+        for (void *p : list) (void)hipFree(p);
+        list.clear();
+    }
+    void drop_graph()
+    {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        gexec = nullptr;
+        gsteps = 0;
+    }
+    wv::Fields2D fields2d() const
+    {
+        wv::Fields2D F{};
+        for (int k = 0; k < 5; k++) F.f[k] = field[k];
+        return F;
+    }
+    wv::Coef2D coef2d() const
+    {
+        wv::Coef2D C{};
+        for (int k = 0; k < 5; k++) C.c[k] = coef[k];
+        C.damp = cfg.damping_boundary == 1 ? damp : nullptr;
+        C.fsH = fsH;
+        C.fsV = fsV;
+        C.free_surface = cfg.free_surface == 1;
+        return C;
+    }
+    wv::Taps taps() const
+    {
+        wv::Taps T{};
+        T.xf = tap[0];
+        T.xb = tap[1];
+        T.yf = tap[2];
+        T.yb = tap[3];
+        T.yf_v = tap[4];
+        T.yb_v = tap[5];
+        return T;
+    }
+    wv::Cpml2D cpml2d() const
+    {
+        wv::Cpml2D P{};
+        P.on = cfg.damping_boundary == 2;
+        P.W = cfg.boundary_width;
+        P.ax = cpx[0];
+        P.bx = cpx[1];
+        P.axh = cpx[2];
+        P.bxh = cpx[3];
+        P.ay = cpy[0];
+        P.by = cpy[1];
+        P.ayh = cpy[2];
+        P.byh = cpy[3];
+        P.xi = xi;
+        P.yi = yi;
+        for (int k = 0; k < 4; k++) {
+            P.psx[k] = psx[k];
+            P.psy[k] = psy[k];
+        }
+        return P;
+    }
+    wv::Acq acq() const
+    {
+        wv::Acq A{};
+        A.nshots = cfg.nshots;
+        A.NT = cfg.NT;
+        A.src_ptr = d_src_ptr;
+        A.src_cell = d_src_cell;
+        A.src_type = d_src_type;
+        A.src_sig = d_src_sig;
+        A.rec_ptr = d_rec_ptr;
+        A.rec_cell = d_rec_cell;
+        A.rec_type = d_rec_type;
+        A.rec_total = rec_total;
+        A.seis = d_seis;
+        A.tbase = d_tbase;
+        return A;
+    }
+    int enqueue_steps(int nsteps);
+    int enqueue_steps_timed(int nsteps);
+    int sync_acquisition();
+    double kernel_bytes(int kind) const;
+};
+
+wv_solver::~wv_solver()
+{
+    if (st) (void)hipStreamSynchronize(st);
+    for (hipEvent_t e : kev) (void)hipEventDestroy(e);
+    drop_graph();
+    delete plain;
+    free_list(acq_allocs);
+    free_list(allocs);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (st) (void)hipStreamDestroy(st);
+}
+
+namespace {
+__global__ void set_int_kernel(int *p, int v)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) p[0] = v;
+}
+}  // namespace
+
+int wv_solver::sync_acquisition()
+{
+    if (!acq_dirty) return 0;
+    drop_graph();
+    HIP_TRY(hipStreamSynchronize(st));
+    free_list(acq_allocs);
+    std::vector<int> sp(1, 0), rp(1, 0), stp, rtp;
+    std::vector<long long> sc, rc;
+    std::vector<float> sig;
+    for (auto &s : shots) {
+        for (size_t k = 0; k < s.sidx.size(); k++) {
+            const long long g = s.sidx[k];
+            const int x = (int)(g % L.NX), y = (int)(g / L.NX);
+            sc.push_back(L.at(x, y));
+            stp.push_back(s.stype[k]);
+        }
+        sig.insert(sig.end(), s.ssig.begin(), s.ssig.end());
+        sp.push_back((int)sc.size());
+        for (size_t k = 0; k < s.ridx.size(); k++) {
+            const long long g = s.ridx[k];
+            const int x = (int)(g % L.NX), y = (int)(g / L.NX);
+            rc.push_back(L.at(x, y));
+            rtp.push_back(s.rtype[k]);
+        }
+        rp.push_back((int)rc.size());
+    }
+    rec_total = (int)rc.size();
+    rec_ptr_h = rp;
+    int r;
+    if ((r = upload(&d_src_ptr, sp, acq_allocs)) || (r = upload(&d_rec_ptr, rp, acq_allocs)) ||
+        (r = upload(&d_src_cell, sc, acq_allocs)) || (r = upload(&d_rec_cell, rc, acq_allocs)) ||
+        (r = upload(&d_src_type, stp, acq_allocs)) || (r = upload(&d_rec_type, rtp, acq_allocs)) ||
+        (r = upload(&d_src_sig, sig, acq_allocs)) ||
+        (r = dalloc(&d_seis, (size_t)cfg.NT * std::max(rec_total, 1), acq_allocs)))
+        return r;
+    HIP_TRY(hipStreamSynchronize(st));
+    acq_dirty = false;
+    return 0;
+}
+
+// Enqueue `nsteps` time steps starting at the device step counter.
+int wv_solver::enqueue_steps(int nsteps)
+{
+    const wv::Fields2D F = fields2d();
+    const wv::Coef2D C = coef2d();
+    const wv::Taps T = taps();
+    const wv::Cpml2D P = cpml2d();
+    const wv::Acq A = acq();
+    for (int j = 0; j < nsteps; j++) {
+        wv::launch_step2d(physics, cfg.spatial_fd_order, L, cfg.nshots, F, C, T, P, st, 0);
+        wv::launch_step2d(physics, cfg.spatial_fd_order, L, cfg.nshots, F, C, T, P, st, 1);
+        if (cfg.damping_boundary == 1) wv::launch_abs_velocity2d(physics, L, cfg.nshots, F, C, st);
+        wv::launch_epilogue2d(physics, L, F, A, j, st);
+    }
+    wv::launch_advance(d_tbase, nsteps, st);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+// Same launches as enqueue_steps, each kernel bracketed by hipEvents on the
+// handle's stream; elapsed times are accumulated per kernel kind.
+int wv_solver::enqueue_steps_timed(int nsteps)
+{
+    const wv::Fields2D F = fields2d();
+    const wv::Coef2D C = coef2d();
+    const wv::Taps T = taps();
+    const wv::Cpml2D P = cpml2d();
+    const wv::Acq A = acq();
+    constexpr int BATCH = 32;
+    if (kev.empty()) {
+        kev.resize(4 * BATCH);
+        for (auto &e : kev) HIP_TRY(hipEventCreate(&e));
+    }
+    for (int j0 = 0; j0 < nsteps; j0 += BATCH) {
+        const int nb = std::min(BATCH, nsteps - j0);
+        for (int b = 0; b < nb; b++) {
+            hipEvent_t *e = &kev[4 * b];
+            HIP_TRY(hipEventRecord(e[0], st));
+            wv::launch_step2d(physics, cfg.spatial_fd_order, L, cfg.nshots, F, C, T, P, st, 0);
+            HIP_TRY(hipEventRecord(e[1], st));
+            wv::launch_step2d(physics, cfg.spatial_fd_order, L, cfg.nshots, F, C, T, P, st, 1);
+            HIP_TRY(hipEventRecord(e[2], st));
+            if (cfg.damping_boundary == 1) wv::launch_abs_velocity2d(physics, L, cfg.nshots, F, C, st);
+            wv::launch_epilogue2d(physics, L, F, A, j0 + b, st);
+            HIP_TRY(hipEventRecord(e[3], st));
+        }
+        HIP_TRY(hipEventSynchronize(kev[4 * (nb - 1) + 3]));
+        for (int b = 0; b < nb; b++) {
+            for (int k = 0; k < 3; k++) {
+                float ms = 0;
+                HIP_TRY(hipEventElapsedTime(&ms, kev[4 * b + k], kev[4 * b + k + 1]));
+                kms[k] += ms;
+                klaunch[k] += 1;
+            }
+        }
+    }
+    wv::launch_advance(d_tbase, nsteps, st);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+// Algorithmic HBM bytes of one launch of a kernel kind: fields of every shot
+// read once and updated fields written once, shared coefficients read once,
+// CPML memory variables read+written on their frame cells.
+double wv_solver::kernel_bytes(int kind) const
+{
+    const double cells = (double)cfg.NX * cfg.NY, B = cfg.nshots;
+    // {field reads+writes per shot-cell, coefficient reads per cell, psi per x-frame cell, psi per y-frame cell}
+    static const int tab[3][2][4] = {
+        {{5, 2, 1, 1}, {4, 1, 1, 1}},   // acoustic: velocity, pressure
+        {{7, 2, 2, 2}, {8, 3, 2, 2}},   // elastic: velocity, stress
+        {{5, 2, 1, 1}, {4, 2, 1, 1}},   // tmem: H, E
+    };
+    if (kind == 2) {
+        double nsrc = 0;
+        for (auto &sh : shots) nsrc += sh.sidx.size();
+        return 12.0 * nsrc + 8.0 * rec_total;
+    }
+    const int *t = tab[physics][kind];
+    double bytes = 4.0 * (cells * B * t[0] + cells * t[1]);
+    if (cfg.damping_boundary == 2) {
+        const double W = cfg.boundary_width;
+        const double xf = (double)cfg.NY * 2 * W, yf = (double)cfg.NX * (cfg.free_surface ? W : 2 * W);
+        bytes += 8.0 * B * (xf * t[2] + yf * t[3]);
+    }
+    if (cfg.damping_boundary == 1 && kind == 1) bytes += 4.0 * cells;
+    return bytes;
+}
+
+extern "C" {
+
+const char *wv_last_error(void) { return g_err.c_str(); }
+
+int wv_create(const char *dimension, const char *type, wv_solver **out)
+{
+    if (!out) return fail(WV_EINVAL, "null output pointer");
+    *out = nullptr;
+    std::string d = lower(dimension), t = lower(type);
+    if (d != "2d" && d != "3d") return fail(WV_EINVAL, "Unknown dimension " + d);
+    static const char *known[] = {"acoustic", "elastic", "viscoelastic", "sh", "viscosh",
+                                  "tmem", "emem", "viscotmem", "viscoemem"};
+    bool ok = false;
+    for (auto k : known) ok |= (t == k);
+    if (!ok) return fail(WV_EINVAL, "Unkown type " + t);
+    int phys;
+    if (t == "acoustic")
+        phys = wv::ACOUSTIC;
+    else if (t == "elastic")
+        phys = wv::ELASTIC;
+    else if (t == "tmem")
+        phys = wv::TMEM;
+    else
+        return fail(WV_EINVAL, "equationType " + t + " is not provided by this solver");
+    if (d == "3d") return fail(WV_EINVAL, "3D " + t + " is not provided by this build");
+    wv_solver *s = new wv_solver();
+    s->dim = d == "3d" ? 3 : 2;
+    s->physics = phys;
+    *out = s;
+    return WV_OK;
+}
+
+int wv_destroy(wv_solver *s)
+{
+    delete s;
+    return WV_OK;
+}
+
+int wv_init(wv_solver *s, const wv_config *c)
+{
+    if (!s || !c) return fail(WV_EINVAL, "null argument");
+    const wv_config &k = *c;
+    if (k.NX < 2 || k.NY < 2 || k.NZ != 1) return fail(WV_EINVAL, "2D grid needs NX, NY >= 2 and NZ = 1");
+    if (k.NT < 1) return fail(WV_EINVAL, "NT must be positive");
+    if (!(k.DH > 0) || !(k.DT > 0)) return fail(WV_EINVAL, "DH and DT must be positive");
+    const int N = k.spatial_fd_order;
+    if (N < 2 || N > 12 || (N & 1)) return fail(WV_EINVAL, "spatialFDorder must be one of 2,4,6,8,10,12");
+    if (k.ops_mode != WV_OPS_SPARSE && k.ops_mode != WV_OPS_STENCIL) return fail(WV_EINVAL, "bad ops_mode");
+    if (k.free_surface != 0 && k.free_surface != 1) return fail(WV_EINVAL, "FreeSurface must be 0 or 1");
+    if (k.free_surface && s->physics == wv::TMEM) return fail(WV_EINVAL, "FreeSurface is not available for tmem");
+    if (k.damping_boundary < 0 || k.damping_boundary > 2) return fail(WV_EINVAL, "DampingBoundary must be 0, 1 or 2");
+    if (k.damping_boundary && (k.boundary_width < 1 || 2 * k.boundary_width > std::min(k.NX, k.NY)))
+        return fail(WV_EINVAL, "BoundaryWidth out of range");
+    if (k.damping_boundary == 1 && !(k.damping_coeff > 0 && k.damping_coeff < 100))
+        return fail(WV_EINVAL, "DampingCoeff must be in (0,100)");
+    if (k.nshots < 1) return fail(WV_EINVAL, "nshots must be >= 1");
+    if (k.path != WV_PATH_FUSED && k.path != WV_PATH_PLAIN) return fail(WV_EINVAL, "bad path");
+    if (k.path == WV_PATH_PLAIN && k.nshots != 1) return fail(WV_EINVAL, "the plain path runs one shot per handle");
+    if (s->inited) return fail(WV_ESTATE, "solver already initialised");
+
+    s->cfg = k;
+    if (s->cfg.steps_per_graph < 0) s->cfg.steps_per_graph = 0;
+    HIP_TRY(hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreate(&s->ev0));
+    HIP_TRY(hipEventCreate(&s->ev1));
+
+    if (k.path == WV_PATH_PLAIN) {
+        s->plain = new wv::PlainSolver();
+        int rc = s->plain->init(s->physics, k, s->st);
+        if (rc) return fail(rc, s->plain->error);
+        s->shots.assign(1, ShotAcq());
+        s->inited = true;
+        return WV_OK;
+    }
+
+    wv::Layout2D &L = s->L;
+    s->H = N / 2;
+    L.NX = k.NX;
+    L.NY = k.NY;
+    L.H = s->H;
+    L.ypad = s->H;
+    L.pitch = wv::XPAD + round_up(k.NX, wv::TILE_X) + wv::XPAD;
+    L.rows = s->H + round_up(k.NY, wv::TILE_Y) + s->H;
+    L.plane = (long long)L.pitch * L.rows;
+    s->nfields = s->physics == wv::ELASTIC ? 5 : 3;
+
+    int rc;
+    for (int f = 0; f < s->nfields; f++)
+        if ((rc = s->dalloc(&s->field[f], (size_t)L.plane * k.nshots, s->allocs))) return rc;
+
+    // derivative taps
+    const int NXp = round_up(k.NX, wv::TILE_X), NYp = round_up(k.NY, wv::TILE_Y);
+    const bool stencil = k.ops_mode == WV_OPS_STENCIL;
+    const float DH = (float)k.DH, DT = (float)k.DT;
+    std::vector<float> t_xf, t_xb, t_yf, t_yb, t_yfv, t_ybv;
+    wv::build_taps(t_xf, k.NX, NXp, N, s->H, true, stencil, DH, DT);
+    wv::build_taps(t_xb, k.NX, NXp, N, s->H, false, stencil, DH, DT);
+    wv::build_taps(t_yf, k.NY, NYp, N, s->H, true, stencil, DH, DT);
+    wv::build_taps(t_yb, k.NY, NYp, N, s->H, false, stencil, DH, DT);
+    if (k.free_surface == 1) {
+        // velocity-type updates use the image operators (ForwardSolver2Delastic.cpp:168-202)
+        wv::build_taps_free_surface(t_yfv, k.NY, NYp, N, s->H, true, DH, DT);
+        if (s->physics == wv::ELASTIC)
+            wv::build_taps_free_surface(t_ybv, k.NY, NYp, N, s->H, false, DH, DT);
+        else
+            t_ybv = t_yb;
+    } else {
+        t_yfv = t_yf;
+        t_ybv = t_yb;
+    }
+    if ((rc = s->upload(&s->tap[0], t_xf, s->allocs)) || (rc = s->upload(&s->tap[1], t_xb, s->allocs)) ||
+        (rc = s->upload(&s->tap[2], t_yf, s->allocs)) || (rc = s->upload(&s->tap[3], t_yb, s->allocs)) ||
+        (rc = s->upload(&s->tap[4], t_yfv, s->allocs)) || (rc = s->upload(&s->tap[5], t_ybv, s->allocs)))
+        return rc;
+
+    // CPML
+    if (k.damping_boundary == 2) {
+        const int W = k.boundary_width;
+        std::vector<float> a, b, ah, bh;
+        std::vector<int> idx;
+        wv::cpml_axis(k.NX, NXp, W, false, k.npower, k.fc_cpml, k.vmax_cpml, k.DT, k.DH, a, b, ah, bh, idx);
+        if ((rc = s->upload(&s->cpx[0], a, s->allocs)) || (rc = s->upload(&s->cpx[1], b, s->allocs)) ||
+            (rc = s->upload(&s->cpx[2], ah, s->allocs)) || (rc = s->upload(&s->cpx[3], bh, s->allocs)) ||
+            (rc = s->upload(&s->xi, idx, s->allocs)))
+            return rc;
+        wv::cpml_axis(k.NY, NYp, W, k.free_surface != 0, k.npower, k.fc_cpml, k.vmax_cpml, k.DT, k.DH, a, b, ah, bh,
+                      idx);
+        if ((rc = s->upload(&s->cpy[0], a, s->allocs)) || (rc = s->upload(&s->cpy[1], b, s->allocs)) ||
+            (rc = s->upload(&s->cpy[2], ah, s->allocs)) || (rc = s->upload(&s->cpy[3], bh, s->allocs)) ||
+            (rc = s->upload(&s->yi, idx, s->allocs)))
+            return rc;
+        s->psx_n = (size_t)k.nshots * k.NY * 2 * W;
+        s->psy_n = (size_t)k.nshots * 2 * W * k.NX;
+        const int npsi = s->physics == wv::ELASTIC ? 4 : 2;
+        for (int q = 0; q < npsi; q++)
+            if ((rc = s->dalloc(&s->psx[q], s->psx_n, s->allocs)) || (rc = s->dalloc(&s->psy[q], s->psy_n, s->allocs)))
+                return rc;
+    }
+
+    // ABS
+    if (k.damping_boundary == 1) {
+        const int W = k.boundary_width;
+        std::vector<double> prof = wv::abs_profile(W, k.damping_coeff);
+        std::vector<float> d((size_t)L.plane, 0.f);
+        for (int y = 0; y < k.NY; y++)
+            for (int x = 0; x < k.NX; x++) {
+                const int dx = std::min(x, k.NX - 1 - x), dy = std::min(y, k.NY - 1 - y);
+                const int mn = std::min(dx, dy);
+                double v = 1.0;
+                if (k.free_surface == 0) {
+                    if (mn < W) v = prof[mn];
+                } else if (y < W) {
+                    if (dx < W) v = prof[dx];
+                } else if (mn < W) {
+                    v = prof[mn];
+                }
+                d[(size_t)L.at(x, y)] = (float)v;
+            }
+        if ((rc = s->upload(&s->damp, d, s->allocs))) return rc;
+    }
+    if ((rc = s->dalloc(&s->d_tbase, 1, s->allocs))) return rc;
+    s->shots.assign(k.nshots, ShotAcq());
+    HIP_TRY(hipStreamSynchronize(s->st));
+    s->inited = true;
+    return WV_OK;
+}
+
+int wv_set_model(wv_solver *s, const float *m0, const float *m1, const float *m2)
+{
+    if (!s || !s->inited) return fail(WV_ESTATE, "wv_set_model before wv_init");
+    if (!m0 || !m2 || (s->physics != wv::ACOUSTIC && !m1)) return fail(WV_EINVAL, "missing model vector");
+    if (s->plain) {
+        int rc = s->plain->set_model(m0, m1, m2);
+        if (rc) return fail(rc, s->plain->error);
+        s->model_ok = true;
+        return WV_OK;
+    }
+    const wv::Layout2D &L = s->L;
+    const int NX = L.NX, NY = L.NY;
+    auto gi = [&](int x, int y) { return (size_t)x + (size_t)y * NX; };
+    auto li = [&](int x, int y) { return (size_t)L.at(x, y); };
+    std::vector<std::vector<float>> c(5, std::vector<float>((size_t)L.plane, 0.f));
+    auto inv_avg = [&](const std::vector<double> &m, std::vector<float> &out, int ax) {
+        // 1/((m_i + m_next)/2), last point 1/m_i, inf/nan -> 0 (Modelparameter.cpp:336-446, 633-639)
+        for (int y = 0; y < NY; y++)
+            for (int x = 0; x < NX; x++) {
+                const bool last = ax == 0 ? (x + 1 >= NX) : (y + 1 >= NY);
+                const double a = m[gi(x, y)];
+                const double avg = last ? a : 0.5 * (a + m[ax == 0 ? gi(x + 1, y) : gi(x, y + 1)]);
+                double v = 1.0 / avg;
+                if (!std::isfinite(v)) v = 0.0;
+                out[li(x, y)] = (float)v;
+            }
+    };
+    std::vector<double> rho((size_t)NX * NY), M((size_t)NX * NY), mu((size_t)NX * NY);
+    if (s->physics == wv::ACOUSTIC || s->physics == wv::ELASTIC) {
+        for (int y = 0; y < NY; y++)
+            for (int x = 0; x < NX; x++) {
+                const size_t i = gi(x, y);
+                const double vp = m0[i], r = m2[i];
+                rho[i] = r;
+                M[i] = r * vp * vp;
+                if (s->physics == wv::ELASTIC) {
+                    const double vs = m1[i];
+                    mu[i] = std::max(r * vs * vs, 1.0);  // ModelparameterSeismic.cpp:424
+                }
+            }
+        for (int y = 0; y < NY; y++)
+            for (int x = 0; x < NX; x++) c[0][li(x, y)] = (float)M[gi(x, y)];
+        if (s->physics == wv::ACOUSTIC) {
+            inv_avg(rho, c[1], 0);
+            inv_avg(rho, c[2], 1);
+        } else {
+            for (int y = 0; y < NY; y++)
+                for (int x = 0; x < NX; x++) c[1][li(x, y)] = (float)mu[gi(x, y)];
+            inv_avg(rho, c[2], 0);
+            inv_avg(rho, c[3], 1);
+            // harmonic 4-point average of mu (Modelparameter.cpp:449-526, ModelparameterSeismic.cpp:421-433)
+            for (int y = 0; y < NY; y++)
+                for (int x = 0; x < NX; x++) {
+                    const bool inx = x + 1 < NX, iny = y + 1 < NY;
+                    double avg;
+                    const double i00 = 1.0 / mu[gi(x, y)];
+                    if (inx && iny)
+                        avg = 0.25 * (i00 + 1.0 / mu[gi(x + 1, y)] + 1.0 / mu[gi(x, y + 1)] + 1.0 / mu[gi(x + 1, y + 1)]);
+                    else if (inx)
+                        avg = 0.5 * (i00 + 1.0 / mu[gi(x + 1, y)]);
+                    else if (iny)
+                        avg = 0.5 * (i00 + 1.0 / mu[gi(x, y + 1)]);
+                    else
+                        avg = i00;
+                    double v = 1.0 / avg;
+                    if (v < 4.0) v = 0.0;
+                    c[4][li(x, y)] = (float)v;
+                }
+        }
+    } else {
+        const double eps0 = 8.8541878176e-12, mu0 = 1.2566370614e-6;
+        std::vector<double> perm((size_t)NX * NY);
+        for (int y = 0; y < NY; y++)
+            for (int x = 0; x < NX; x++) {
+                const size_t i = gi(x, y);
+                const double eps = (double)m0[i] * eps0, sig = m1[i];
+                perm[i] = (double)m2[i] * mu0;
+                // ForwardSolverEM.cpp:14-68
+                const double q = 0.5 * sig * s->cfg.DT / eps;
+                c[0][li(x, y)] = (float)((1.0 - q) / (1.0 + q));
+                c[1][li(x, y)] = (float)(1.0 / eps / (1.0 + q));
+            }
+        inv_avg(perm, c[2], 0);  // 1/mu averaged along x (TMEM.cpp:366-373)
+        inv_avg(perm, c[3], 1);  // 1/mu averaged along y
+    }
+    int rc;
+    for (int q = 0; q < 5; q++) {
+        if (!s->coef[q]) {
+            if ((rc = s->dalloc(&s->coef[q], (size_t)L.plane, s->allocs))) return rc;
+        }
+        HIP_TRY(hipMemcpyAsync(s->coef[q], c[q].data(), c[q].size() * sizeof(float), hipMemcpyHostToDevice, s->st));
+    }
+    if (s->physics == wv::ELASTIC && s->cfg.free_surface == 1) {
+        // FreeSurfaceElastic.cpp:11-47: scales of the horizontal/vertical updates at y = 0
+        const int NXp = round_up(NX, wv::TILE_X);
+        std::vector<float> h((size_t)NXp, 0.f), v((size_t)NXp, 0.f);
+        for (int x = 0; x < NX; x++) {
+            const double Mx = M[gi(x, 0)], t = Mx - 2.0 * mu[gi(x, 0)];
+            v[x] = (float)t;
+            h[x] = (float)(-(t * t) / Mx);
+        }
+        if (!s->fsH && ((rc = s->dalloc(&s->fsH, NXp, s->allocs)) || (rc = s->dalloc(&s->fsV, NXp, s->allocs))))
+            return rc;
+        HIP_TRY(hipMemcpyAsync(s->fsH, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice, s->st));
+        HIP_TRY(hipMemcpyAsync(s->fsV, v.data(), v.size() * sizeof(float), hipMemcpyHostToDevice, s->st));
+    }
+    HIP_TRY(hipStreamSynchronize(s->st));
+    s->model_ok = true;
+    return WV_OK;
+}
+
+int wv_set_acquisition(wv_solver *s, int shot, int nsrc, const long long *src_index, const int *src_type,
+                       const float *src_signal, int nrec, const long long *rec_index, const int *rec_type)
+{
+    if (!s || !s->inited) return fail(WV_ESTATE, "wv_set_acquisition before wv_init");
+    if (shot < 0 || shot >= s->cfg.nshots) return fail(WV_EINVAL, "shot index out of range");
+    if (nsrc < 0 || nrec < 0) return fail(WV_EINVAL, "negative count");
+    if ((nsrc && (!src_index || !src_type || !src_signal)) || (nrec && (!rec_index || !rec_type)))
+        return fail(WV_EINVAL, "missing acquisition array");
+    const long long n = (long long)s->cfg.NX * s->cfg.NY * s->cfg.NZ;
+    const int maxtype = s->physics == wv::TMEM ? 1 : 3;
+    for (int k = 0; k < nsrc; k++) {
+        if (src_index[k] < 0 || src_index[k] >= n) return fail(WV_EINVAL, "source outside the grid");
+        if (src_type[k] < 1 || src_type[k] > maxtype) return fail(WV_EINVAL, "unsupported source type");
+    }
+    for (int k = 0; k < nrec; k++) {
+        if (rec_index[k] < 0 || rec_index[k] >= n) return fail(WV_EINVAL, "receiver outside the grid");
+        if (rec_type[k] < 1 || rec_type[k] > maxtype) return fail(WV_EINVAL, "unsupported receiver type");
+    }
+    ShotAcq a;
+    a.sidx.assign(src_index, src_index + nsrc);
+    a.stype.assign(src_type, src_type + nsrc);
+    a.ssig.assign(src_signal, src_signal + (size_t)nsrc * s->cfg.NT);
+    a.ridx.assign(rec_index, rec_index + nrec);
+    a.rtype.assign(rec_type, rec_type + nrec);
+    if (s->plain) {
+        int rc = s->plain->set_acquisition(a.sidx, a.stype, a.ssig, a.ridx, a.rtype);
+        if (rc) return fail(rc, s->plain->error);
+    }
+    s->shots[shot] = std::move(a);
+    s->acq_dirty = true;
+    return WV_OK;
+}
+
+int wv_reset(wv_solver *s)
+{
+    if (!s || !s->inited) return fail(WV_ESTATE, "wv_reset before wv_init");
+    if (s->plain) {
+        int rc = s->plain->reset();
+        return rc ? fail(rc, s->plain->error) : WV_OK;
+    }
+    const size_t fn = (size_t)s->L.plane * s->cfg.nshots;
+    for (int f = 0; f < s->nfields; f++) HIP_TRY(hipMemsetAsync(s->field[f], 0, fn * sizeof(float), s->st));
+    for (int q = 0; q < 4; q++) {
+        if (s->psx[q]) HIP_TRY(hipMemsetAsync(s->psx[q], 0, s->psx_n * sizeof(float), s->st));
+        if (s->psy[q]) HIP_TRY(hipMemsetAsync(s->psy[q], 0, s->psy_n * sizeof(float), s->st));
+    }
+    if (s->d_seis) HIP_TRY(hipMemsetAsync(s->d_seis, 0, (size_t)s->cfg.NT * std::max(s->rec_total, 1) * 4, s->st));
+    return WV_OK;
+}
+
+int wv_run(wv_solver *s, int t_start, int t_end)
+{
+    if (!s || !s->inited) return fail(WV_ESTATE, "wv_run before wv_init");
+    if (!s->model_ok) return fail(WV_ESTATE, "wv_run before wv_set_model");
+    if (t_start < 0 || t_end > s->cfg.NT || t_start > t_end) return fail(WV_EINVAL, "time range out of [0, NT]");
+    if (s->plain) {
+        HIP_TRY(hipEventRecord(s->ev0, s->st));
+        int rc = s->plain->run(t_start, t_end);
+        if (rc) return fail(rc, s->plain->error);
+        HIP_TRY(hipEventRecord(s->ev1, s->st));
+        s->ran = true;
+        return WV_OK;
+    }
+    const bool fresh = s->acq_dirty;
+    int rc = s->sync_acquisition();
+    if (rc) return rc;
+    if (fresh) {
+        // seismogram buffer was reallocated (zeroed)
+    }
+    HIP_TRY(hipEventRecord(s->ev0, s->st));
+    set_int_kernel<<<1, 64, 0, s->st>>>(s->d_tbase, t_start);
+    int remaining = t_end - t_start;
+    const int K = s->ktiming ? 0 : s->cfg.steps_per_graph;
+    if (s->ktiming) {
+        rc = s->enqueue_steps_timed(remaining);
+        if (rc) return rc;
+        remaining = 0;
+    }
+    if (K > 0 && remaining >= K) {
+        if (!s->gexec || s->gsteps != K) {
+            s->drop_graph();
+            hipGraph_t g;
+            HIP_TRY(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
+            rc = s->enqueue_steps(K);
+            hipError_t e = hipStreamEndCapture(s->st, &g);
+            if (rc) return rc;
+            if (e != hipSuccess) return fail(WV_EDEVICE, std::string("graph capture: ") + hipGetErrorString(e));
+            e = hipGraphInstantiate(&s->gexec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) return fail(WV_EDEVICE, std::string("graph instantiate: ") + hipGetErrorString(e));
+            s->gsteps = K;
+        }
+        while (remaining >= K) {
+            HIP_TRY(hipGraphLaunch(s->gexec, s->st));
+            remaining -= K;
+        }
+    }
+    if (remaining > 0) {
+        rc = s->enqueue_steps(remaining);
+        if (rc) return rc;
+    }
+    HIP_TRY(hipEventRecord(s->ev1, s->st));
+    s->ran = true;
+    return WV_OK;
+}
+
+int wv_synchronize(wv_solver *s)
+{
+    if (!s || !s->inited) return fail(WV_ESTATE, "not initialised");
+    HIP_TRY(hipStreamSynchronize(s->st));
+    return WV_OK;
+}
+
+int wv_last_run_ms(wv_solver *s, double *ms)
+{
+    if (!s || !s->inited || !s->ran || !ms) return fail(WV_ESTATE, "no run recorded");
+    HIP_TRY(hipEventSynchronize(s->ev1));
+    float f = 0;
+    HIP_TRY(hipEventElapsedTime(&f, s->ev0, s->ev1));
+    *ms = f;
+    return WV_OK;
+}
+
+int wv_get_seismograms(wv_solver *s, int shot, float *out)
+{
+    if (!s || !s->inited) return fail(WV_ESTATE, "not initialised");
+    if (shot < 0 || shot >= s->cfg.nshots || !out) return fail(WV_EINVAL, "bad shot or output");
+    if (s->plain) {
+        int rc = s->plain->get_seismograms(out);
+        return rc ? fail(rc, s->plain->error) : WV_OK;
+    }
+    const int NT = s->cfg.NT;
+    const int nrec = (int)s->shots[shot].ridx.size();
+    if (nrec == 0) return WV_OK;
+    if (s->acq_dirty || !s->d_seis) {
+        std::fill(out, out + (size_t)nrec * NT, 0.f);
+        return WV_OK;
+    }
+    HIP_TRY(hipStreamSynchronize(s->st));
+    std::vector<float> h((size_t)NT * s->rec_total);
+    HIP_TRY(hipMemcpy(h.data(), s->d_seis, h.size() * sizeof(float), hipMemcpyDeviceToHost));
+    const int r0 = s->rec_ptr_h[shot];
+    for (int r = 0; r < nrec; r++)
+        for (int t = 0; t < NT; t++) out[(size_t)r * NT + t] = h[(size_t)t * s->rec_total + r0 + r];
+    return WV_OK;
+}
+
+static int field_io(wv_solver *s, int shot, int field, float *out, const float *in)
+{
+    if (!s || !s->inited) return fail(WV_ESTATE, "not initialised");
+    if (shot < 0 || shot >= s->cfg.nshots) return fail(WV_EINVAL, "bad shot");
+    if (s->plain) {
+        int rc = s->plain->field_io(field, out, in);
+        return rc ? fail(rc, s->plain->error) : WV_OK;
+    }
+    if (field < 0 || field >= s->nfields) return fail(WV_EINVAL, "bad field index");
+    const wv::Layout2D &L = s->L;
+    std::vector<float> h((size_t)L.plane);
+    float *d = s->field[field] + (size_t)shot * L.plane;
+    HIP_TRY(hipStreamSynchronize(s->st));
+    if (out) {
+        HIP_TRY(hipMemcpy(h.data(), d, h.size() * sizeof(float), hipMemcpyDeviceToHost));
+        for (int y = 0; y < L.NY; y++)
+            for (int x = 0; x < L.NX; x++) out[(size_t)y * L.NX + x] = h[(size_t)L.at(x, y)];
+    } else {
+        std::fill(h.begin(), h.end(), 0.f);
+        for (int y = 0; y < L.NY; y++)
+            for (int x = 0; x < L.NX; x++) h[(size_t)L.at(x, y)] = in[(size_t)y * L.NX + x];
+        HIP_TRY(hipMemcpy(d, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
+    return WV_OK;
+}
+
+int wv_get_wavefield(wv_solver *s, int shot, int field, float *out)
+{
+    if (!out) return fail(WV_EINVAL, "null output");
+    return field_io(s, shot, field, out, nullptr);
+}
+
+int wv_set_wavefield(wv_solver *s, int shot, int field, const float *in)
+{
+    if (!in) return fail(WV_EINVAL, "null input");
+    return field_io(s, shot, field, nullptr, in);
+}
+
+int wv_set_kernel_timing(wv_solver *s, int on)
+{
+    if (!s || !s->inited) return fail(WV_ESTATE, "not initialised");
+    if (s->plain) return fail(WV_EINVAL, "kernel timing is provided by the fused path");
+    s->ktiming = on != 0;
+    for (int k = 0; k < 3; k++) {
+        s->kms[k] = 0;
+        s->klaunch[k] = 0;
+    }
+    return WV_OK;
+}
+
+int wv_kernel_stats(wv_solver *s, int kind, const char **name, double *avg_ms, long long *launches,
+                    double *bytes_per_launch)
+{
+    if (!s || !s->inited) return fail(WV_ESTATE, "not initialised");
+    if (kind < 0 || kind > 2) return fail(WV_EINVAL, "unknown kernel kind");
+    if (s->plain) return fail(WV_EINVAL, "kernel timing is provided by the fused path");
+    static const char *names[3][3] = {{"ac2d_velocity", "ac2d_pressure", "epilogue2d"},
+                                      {"el2d_velocity", "el2d_stress", "epilogue2d"},
+                                      {"tm2d_h", "tm2d_e", "epilogue2d"}};
+    HIP_TRY(hipStreamSynchronize(s->st));
+    if (name) *name = names[s->physics][kind];
+    if (launches) *launches = s->klaunch[kind];
+    if (avg_ms) *avg_ms = s->klaunch[kind] ? s->kms[kind] / s->klaunch[kind] : 0.0;
+    if (bytes_per_launch) *bytes_per_launch = s->kernel_bytes(kind);
+    return WV_OK;
+}
+
+}  // extern "C"

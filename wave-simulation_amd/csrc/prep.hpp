@@ -1,0 +1,44 @@
+// Host-side preparation of everything the kernels consume: derivative tap
+// tables, CPML profiles, model coefficients.  Runs once per model / config.
+#pragma once
+
+#include <string>
+#include <vector>
+
+namespace wv {
+
+// Exact staggered-grid Taylor coefficients for offsets (j - N/2 + 1/2), j < N,
+// ordered as in the reference (Derivatives.cpp:2001-2042).
+double fd_coefficient(int order, int j);
+
+// Taps of one derivative operator along an axis of length n:
+// out[(c)*(2H+1) + (k)] is the weight of point c + k - H in the derivative
+// at position c, already multiplied by DT/DH.  padded_n >= n rows are written
+// (rows >= n are zero).
+//   forward=true : Dxf-type operator (result lives half a cell forward)
+//   stencil=false: sparse operator with edge order reduction (Derivatives.cpp:129-1234)
+//   stencil=true : StencilMatrix operator, points outside the axis dropped (FDTD2D.cpp:156-185)
+void build_taps(std::vector<float> &out, int n, int padded_n, int order, int H, bool forward, bool stencil,
+                float DH, float DT);
+
+// Free-surface image operators along depth (Derivatives.cpp:367-526, non-hybrid):
+// forward=true -> DyfFreeSurface, false -> DybFreeSurface.
+void build_taps_free_surface(std::vector<float> &out, int n, int padded_n, int order, int H, bool forward, float DH,
+                             float DT);
+
+// CPML damping profile a, b of width W (CPML.cpp:39-68), computed in FP64.
+void cpml_profile(int W, double npower, double fc, double vmax, double DT, double DH, bool half,
+                  std::vector<double> &a, std::vector<double> &b);
+
+// Per-position CPML coefficients along an axis (CPML2D.cpp:228-270): positions
+// within W of either end get the profile value at their distance; on the far
+// side full and half profiles swap; the near side of the depth axis is left
+// out when a free surface is on.  idx gets the compact frame index or -1.
+void cpml_axis(int n, int padded_n, int W, bool skip_near_side, double npower, double fc, double vmax, double DT,
+               double DH, std::vector<float> &a, std::vector<float> &b, std::vector<float> &ah, std::vector<float> &bh,
+               std::vector<int> &idx);
+
+// ABS damping factor of one grid cell (ABS2D.cpp:110-178, ABS3D.cpp:150-215).
+std::vector<double> abs_profile(int W, double damping_coeff);
+
+}  // namespace wv

@@ -1,0 +1,101 @@
+// Internal declarations shared by the host runtime (abi.cpp, prep.cpp) and the
+// HIP kernels.  Not part of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace wv {
+
+enum Physics { ACOUSTIC = 0, ELASTIC = 1, TMEM = 2 };
+
+// Maximum half-width of a derivative stencil window (order 12 -> 6).
+constexpr int MAX_H = 6;
+// Tile of the 2D fused kernels: 64 columns (one wavefront) x TILE_Y rows,
+// 256 threads per block (4 wavefronts stacked in y).
+constexpr int TILE_X = 64;
+constexpr int TILE_Y = 16;
+constexpr int BLOCK_Y = 4;
+// Left/right padding of a row in floats (keeps x = 0 on a 128-byte line).
+constexpr int XPAD = 32;
+
+// Padded 2D layout: element (x, y) of a field lives at
+// (y + ypad) * pitch + (x + XPAD); the padding is zero and never written,
+// so stencil taps that leave the grid read zeros.
+struct Layout2D {
+    int NX, NY;
+    int H;          // taps reach [-H, H]
+    int ypad;       // rows of zero padding above the grid (= H)
+    int pitch;      // floats per row
+    int rows;       // rows per field plane
+    long long plane;  // floats per plane (one field of one shot)
+    __host__ __device__ long long at(int x, int y) const {
+        return (long long)(y + ypad) * pitch + (x + XPAD);
+    }
+};
+
+// Tap tables: for every position along an axis, 2H+1 coefficients for the
+// offsets -H..H (already multiplied by DT/DH).  Rows beyond the grid are zero.
+struct Taps {
+    const float *xf, *xb;       // [NXpad][2H+1]
+    const float *yf, *yb;       // [NYpad][2H+1] plain operators
+    const float *yf_v, *yb_v;   // y operators used in the velocity-type update
+                                // (free-surface image versions when FreeSurface=1)
+};
+
+// CPML coefficients along each axis (zero outside the frame) and the compact
+// frame index of each position (-1 outside).  Memory variables of the x frame
+// are stored [shot][NY][2W], of the y frame [shot][2W][NX].
+struct Cpml2D {
+    int on;
+    int W;
+    const float *ax, *bx, *axh, *bxh;  // [NXpad]
+    const float *ay, *by, *ayh, *byh;  // [NYpad]
+    const int *xi;                     // [NXpad] compact x-frame column or -1
+    const int *yi;                     // [NYpad] compact y-frame row or -1
+    float *psx[4];                     // x-frame memory variables
+    float *psy[4];                     // y-frame memory variables
+};
+
+struct Fields2D {
+    float *f[5];  // acoustic: vx vy p ; elastic: vx vy sxx syy sxy ; tmem: hx hy ez
+};
+
+struct Coef2D {
+    // acoustic: c0 = M, c1 = 1/rho_x, c2 = 1/rho_y
+    // elastic : c0 = M, c1 = mu, c2 = 1/rho_x, c3 = 1/rho_y, c4 = mu_xy
+    // tmem    : c0 = Ca, c1 = Cb, c2 = 1/mu_xz, c3 = 1/mu_yz
+    const float *c[5];
+    const float *damp;   // ABS damping factors (padded layout) or nullptr
+    const float *fsH;    // [NXpad] free-surface horizontal scale at y = 0 (elastic) or nullptr
+    const float *fsV;    // [NXpad] free-surface vertical scale at y = 0 (elastic)
+    int free_surface;
+};
+
+struct Acq {
+    int nshots;
+    int NT;
+    const int *src_ptr;     // [nshots+1]
+    const long long *src_cell;  // field offsets (layout) of each source
+    const int *src_type;    // 1..4
+    const float *src_sig;   // [nsrc_total][NT]
+    const int *rec_ptr;     // [nshots+1]
+    const long long *rec_cell;
+    const int *rec_type;
+    int rec_total;
+    float *seis;            // time-major [NT][rec_total]
+    const int *tbase;       // device step counter base
+};
+
+// Launchers (kernels2d.hip).  All launches go to `stream`.
+void launch_step2d(int physics, int order, const Layout2D &L, int nshots, const Fields2D &F, const Coef2D &C,
+                   const Taps &T, const Cpml2D &P, hipStream_t stream, int which);
+void launch_epilogue2d(int physics, const Layout2D &L, const Fields2D &F, const Acq &A, int tofs,
+                       hipStream_t stream);
+void launch_abs_velocity2d(int physics, const Layout2D &L, int nshots, const Fields2D &F, const Coef2D &C,
+                           hipStream_t stream);
+void launch_advance(int *tbase, int k, hipStream_t stream);
+
+}  // namespace wv

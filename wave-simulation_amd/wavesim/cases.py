@@ -1,0 +1,179 @@
+"""Deterministic synthetic workloads (numpy PCG64) for the benchmark and tests.
+
+Each function returns a Case: the solver settings (reference configuration
+keys), model vectors in grid order (x fastest, then depth) and the per-shot
+acquisition.  Sizes default to the benchmark configurations; tests pass
+smaller grids / fewer steps of the same recipe.
+
+  acoustic_c1   2D acoustic, order 4, 200x200, dx 1 m, dt 2e-4 s, 2000 steps, 20-cell CPML,
+                homogeneous vp 2000 rho 2000, 50 Hz Ricker P source at the centre,
+                100 P receivers on a row 10 cells below the top
+  elastic_c2    2D elastic P-SV, order 8, 1000x1000, dx 1 m, dt 1e-4 s, 4000 steps, 30-cell CPML,
+                10 layers vp 1500-4000, vs = vp/1.8, rho 1800-2600, x(1 + 0.03 g) with g a unit-std
+                Gaussian random field (20 m correlation), 30 Hz Ricker VY force, 500 VX + 500 VY receivers
+  tmem_c3       2D TM-mode EM, order 8, 800x400, dx 0.02 m, dt 3e-11 s, 3000 steps, 20-cell CPML,
+                eps_r ~ U(1,25) and sigma ~ U(0,0.01) per 0.5 m block, 64 shots of 200 MHz Ricker EZ
+                sources with 20..300 EZ receivers each
+  elastic_c4    2D elastic, order 8, 600x600, 3000 steps, layered + 80 m-radius vp/vs anomaly (+10 %),
+                16 shots, VX/VY receivers every 2 m
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .acquisition import ricker
+
+
+@dataclass
+class Shot:
+    src_index: np.ndarray
+    src_type: np.ndarray
+    src_signal: np.ndarray
+    rec_index: np.ndarray
+    rec_type: np.ndarray
+
+
+@dataclass
+class Case:
+    name: str
+    dimension: str
+    type: str
+    settings: dict
+    m0: np.ndarray
+    m1: np.ndarray | None
+    m2: np.ndarray
+    shots: list = field(default_factory=list)
+
+    @property
+    def cells(self):
+        s = self.settings
+        return s["NX"] * s["NY"] * s.get("NZ", 1)
+
+
+def gaussian_field(shape, sigma, seed, clip=4.0):
+    """Unit-std Gaussian random field: white noise smoothed by a periodic Gaussian kernel of std `sigma` cells."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    w = rng.standard_normal(shape)
+    k = [np.fft.fftfreq(n) for n in shape]
+    grids = np.meshgrid(*k, indexing="ij")
+    r2 = sum(g * g for g in grids)
+    spec = np.exp(-2.0 * (np.pi * sigma) ** 2 * r2)
+    g = np.real(np.fft.ifftn(np.fft.fftn(w) * spec))
+    g = (g - g.mean()) / g.std()
+    return np.clip(g, -clip, clip)
+
+
+def layered(NY, NX, nlayers, lo, hi):
+    rows = np.arange(NY) * nlayers // NY
+    vals = lo + (hi - lo) * rows / (nlayers - 1)
+    return np.repeat(vals[:, None], NX, axis=1)
+
+
+def acoustic_c1(N=200, NT=2000, DT=2e-4):
+    NX = NY = N
+    s = dict(NX=NX, NY=NY, NT=NT, DH=1.0, DT=DT, spatial_fd_order=4, damping_boundary=2, boundary_width=20,
+             npower=4.0, fc_cpml=50.0, vmax_cpml=2000.0)
+    n = NX * NY
+    vp = np.full(n, 2000.0, np.float32)
+    rho = np.full(n, 2000.0, np.float32)
+    c = Case("acoustic_c1", "2D", "acoustic", s, vp, None, rho)
+    cx = NX // 2
+    rx = np.arange(NX // 4, NX // 4 + min(100, NX // 2))
+    c.shots.append(Shot(np.array([cx + (NY // 2) * NX]), np.array([1]), ricker(NT, DT, 50.0, 1.0)[None, :],
+                        rx + 10 * NX, np.ones(len(rx), np.int32)))
+    return c
+
+
+def elastic_model(NX, NY, seed_v, seed_r, sigma=20.0, pert=0.03):
+    vp = layered(NY, NX, 10, 1500.0, 4000.0)
+    rho = layered(NY, NX, 10, 1800.0, 2600.0)
+    g = gaussian_field((NY, NX), sigma, seed_v)
+    gr = gaussian_field((NY, NX), sigma, seed_r)
+    vp = vp * (1.0 + pert * g)
+    vs = vp / 1.8
+    rho = rho * (1.0 + pert * gr)
+    return (vp.astype(np.float32).ravel(), vs.astype(np.float32).ravel(), rho.astype(np.float32).ravel())
+
+
+def elastic_c2(N=1000, NT=4000, DT=1e-4, W=30):
+    NX = NY = N
+    s = dict(NX=NX, NY=NY, NT=NT, DH=1.0, DT=DT, spatial_fd_order=8, damping_boundary=2, boundary_width=W,
+             npower=4.0, fc_cpml=30.0, vmax_cpml=4000.0)
+    vp, vs, rho = elastic_model(NX, NY, 2, 102)
+    c = Case("elastic_c2", "2D", "elastic", s, vp, vs, rho)
+    x = np.arange(NX // 4, NX // 4 + NX // 2)
+    yr = max(W + 10, NY // 25)
+    rec = np.concatenate([x + yr * NX, x + yr * NX])
+    rty = np.concatenate([np.full(len(x), 2), np.full(len(x), 3)]).astype(np.int32)
+    c.shots.append(Shot(np.array([NX // 2 + (NY // 2) * NX]), np.array([3]), ricker(NT, DT, 30.0, 1.0)[None, :],
+                        rec, rty))
+    return c
+
+
+def tmem_c3(NX=800, NY=400, NT=3000, DT=3e-11, nshots=64, seed=3, block=25, W=20):
+    s = dict(NX=NX, NY=NY, NT=NT, DH=0.02, DT=DT, spatial_fd_order=8, damping_boundary=2, boundary_width=W,
+             npower=4.0, fc_cpml=2e8, vmax_cpml=3e8)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    nby, nbx = -(-NY // block), -(-NX // block)
+    eps_b = rng.uniform(1.0, 25.0, (nby, nbx))
+    sig_b = rng.uniform(0.0, 0.01, (nby, nbx))
+    eps = np.kron(eps_b, np.ones((block, block)))[:NY, :NX]
+    sig = np.kron(sig_b, np.ones((block, block)))[:NY, :NX]
+    mur = np.ones((NY, NX))
+    c = Case("tmem_c3", "2D", "tmem", s, eps.astype(np.float32).ravel(), sig.astype(np.float32).ravel(),
+             mur.astype(np.float32).ravel())
+    lo, hi = W + 5, NX - W - 5
+    y = W + 2
+    w = ricker(NT, DT, 2e8, 1.0)
+    for _ in range(nshots):
+        sx = int(rng.integers(lo, hi))
+        nrec = int(rng.integers(20, 301))
+        nrec = min(nrec, hi - lo)
+        rx = np.sort(rng.choice(np.arange(lo, hi), nrec, replace=False))
+        c.shots.append(Shot(np.array([sx + y * NX]), np.array([1]), w[None, :], rx + y * NX,
+                            np.ones(nrec, np.int32)))
+    return c
+
+
+def elastic_c4(N=600, NT=3000, DT=1e-4, nshots=16, seed=4, W=30, anomaly=True):
+    NX = NY = N
+    s = dict(NX=NX, NY=NY, NT=NT, DH=1.0, DT=DT, spatial_fd_order=8, damping_boundary=2, boundary_width=W,
+             npower=4.0, fc_cpml=30.0, vmax_cpml=4000.0)
+    vp = layered(NY, NX, 10, 1500.0, 4000.0)
+    rho = layered(NY, NX, 10, 1800.0, 2600.0)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    cy, cx = rng.uniform(N / 4, 3 * N / 4, 2)
+    if anomaly:
+        yy, xx = np.mgrid[0:NY, 0:NX]
+        disc = (xx - cx) ** 2 + (yy - cy) ** 2 <= (80.0 * N / 600) ** 2
+        vp = np.where(disc, vp * 1.10, vp)
+    vs = vp / 1.8
+    c = Case("elastic_c4", "2D", "elastic", s, vp.astype(np.float32).ravel(), vs.astype(np.float32).ravel(),
+             rho.astype(np.float32).ravel())
+    y = W + 10
+    xr = np.arange(W + 10, NX - W - 10, 2)
+    rec = np.concatenate([xr + y * NX, xr + y * NX])
+    rty = np.concatenate([np.full(len(xr), 2), np.full(len(xr), 3)]).astype(np.int32)
+    w = ricker(NT, DT, 30.0, 1.0)
+    for _ in range(nshots):
+        sx = int(rng.integers(W + 10, NX - W - 10))
+        c.shots.append(Shot(np.array([sx + y * NX]), np.array([3]), w[None, :], rec, rty))
+    return c
+
+
+def make_solver(case, nshots=None, path="fused", ops_mode="sparse", steps_per_graph=0, shots=None):
+    """Create and fully prepare a solver for `case` (the given shots, default all)."""
+    from .solver import create
+
+    shots = case.shots if shots is None else shots
+    n = len(shots) if nshots is None else nshots
+    s = dict(case.settings)
+    sol = create(case.dimension, case.type)
+    sol.init_forward_solver(nshots=n, path=path, ops_mode=ops_mode, steps_per_graph=steps_per_graph, **s)
+    sol.prepare_for_modelling(case.m0, case.m1, case.m2)
+    for k, sh in enumerate(shots[:n]):
+        sol.set_acquisition(k, sh.src_index, sh.src_type, sh.src_signal, sh.rec_index, sh.rec_type)
+    sol.reset()
+    return sol

@@ -1,0 +1,137 @@
+"""Python face of the MI355X forward solver.
+
+Mirrors the reference's ForwardSolver interface (src/ForwardSolver/ForwardSolver.hpp:26-70
+and ForwardSolverFactory.cpp:4-66): a factory ``create(dimension, type)``,
+``init_forward_solver`` (= initForwardSolver), ``prepare_for_modelling``
+(= Modelparameter/ForwardSolver prepareForModelling), ``run(t)`` for one step,
+``run(t_start, t_end)`` for a segment, ``reset_cpml`` / ``reset_wavefields``.
+All work happens in libwavehip.so on the GPU; this module only marshals arrays.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _abi
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class ForwardSolver:
+    """One solver handle: a grid, a model, nshots acquisitions, device wavefields."""
+
+    FIELDS = {
+        ("2D", "acoustic"): ("vx", "vy", "p"),
+        ("2D", "elastic"): ("vx", "vy", "sxx", "syy", "sxy"),
+        ("2D", "tmem"): ("hx", "hy", "ez"),
+    }
+
+    def __init__(self, dimension: str, type_: str):
+        L = _abi.load()
+        self._h = C.c_void_p()
+        _abi.check(L.wv_create(dimension.encode(), type_.encode(), C.byref(self._h)))
+        self.dimension = dimension.upper()
+        self.type = type_.lower()
+        self.cfg = None
+        self._nrec = {}
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and h.value:
+            _abi.load().wv_destroy(h)
+            self._h = C.c_void_p()
+
+    # -- reference-named lifecycle -------------------------------------------------
+    def init_forward_solver(self, NX, NY, NT, DH, DT, spatial_fd_order=8, NZ=1, ops_mode="sparse", free_surface=0,
+                            damping_boundary=0, boundary_width=0, damping_coeff=8.0, npower=4.0, fc_cpml=5.0,
+                            vmax_cpml=3500.0, nshots=1, path="fused", steps_per_graph=0):
+        cfg = _abi.Config()
+        cfg.NX, cfg.NY, cfg.NZ, cfg.NT = NX, NY, NZ, NT
+        cfg.DH, cfg.DT = DH, DT
+        cfg.spatial_fd_order = spatial_fd_order
+        cfg.ops_mode = {"sparse": _abi.WV_OPS_SPARSE, "stencil": _abi.WV_OPS_STENCIL}[ops_mode]
+        cfg.free_surface = free_surface
+        cfg.damping_boundary = damping_boundary
+        cfg.boundary_width = boundary_width
+        cfg.damping_coeff, cfg.npower, cfg.fc_cpml, cfg.vmax_cpml = damping_coeff, npower, fc_cpml, vmax_cpml
+        cfg.nshots = nshots
+        cfg.path = {"fused": _abi.WV_PATH_FUSED, "plain": _abi.WV_PATH_PLAIN}[path]
+        cfg.steps_per_graph = steps_per_graph
+        _abi.check(_abi.load().wv_init(self._h, C.byref(cfg)))
+        self.cfg = cfg
+        return self
+
+    def prepare_for_modelling(self, m0, m1, m2):
+        n = self.cfg.NX * self.cfg.NY * self.cfg.NZ
+        arrs = []
+        for m in (m0, m1, m2):
+            arrs.append(None if m is None else np.ascontiguousarray(m, dtype=np.float32).reshape(-1)[:n].copy())
+        for a in arrs:
+            if a is not None and a.size != n:
+                raise ValueError("model vector size mismatch")
+        ptr = [(_fp(a) if a is not None else None) for a in arrs]
+        _abi.check(_abi.load().wv_set_model(self._h, *ptr))
+        return self
+
+    def set_acquisition(self, shot, src_index, src_type, src_signal, rec_index, rec_type):
+        si = np.ascontiguousarray(src_index, dtype=np.int64)
+        st = np.ascontiguousarray(src_type, dtype=np.int32)
+        sg = np.ascontiguousarray(src_signal, dtype=np.float32).reshape(len(si), self.cfg.NT)
+        ri = np.ascontiguousarray(rec_index, dtype=np.int64)
+        rt = np.ascontiguousarray(rec_type, dtype=np.int32)
+        _abi.check(_abi.load().wv_set_acquisition(
+            self._h, shot, len(si), si.ctypes.data_as(C.POINTER(C.c_longlong)), st.ctypes.data_as(C.POINTER(C.c_int)),
+            _fp(sg), len(ri), ri.ctypes.data_as(C.POINTER(C.c_longlong)), rt.ctypes.data_as(C.POINTER(C.c_int))))
+        self._nrec[shot] = len(ri)
+        return self
+
+    def reset(self):
+        """Wavefields::resetWavefields + ForwardSolver::resetCPML (+ clears seismograms)."""
+        _abi.check(_abi.load().wv_reset(self._h))
+
+    reset_wavefields = reset
+    reset_cpml = reset
+
+    def run(self, t_start, t_end=None):
+        """run(t): one time step as the reference; run(t0, t1): steps [t0, t1)."""
+        if t_end is None:
+            t_end = t_start + 1
+        _abi.check(_abi.load().wv_run(self._h, int(t_start), int(t_end)))
+
+    def synchronize(self):
+        _abi.check(_abi.load().wv_synchronize(self._h))
+
+    def last_run_ms(self):
+        ms = C.c_double()
+        _abi.check(_abi.load().wv_last_run_ms(self._h, C.byref(ms)))
+        return ms.value
+
+    # -- results ------------------------------------------------------------------
+    def seismograms(self, shot=0):
+        out = np.zeros((self._nrec.get(shot, 0), self.cfg.NT), dtype=np.float32)
+        if out.size:
+            _abi.check(_abi.load().wv_get_seismograms(self._h, shot, _fp(out)))
+        return out
+
+    def wavefield(self, name_or_index, shot=0):
+        idx = name_or_index
+        if isinstance(idx, str):
+            idx = self.FIELDS[(self.dimension, self.type)].index(idx)
+        out = np.zeros(self.cfg.NX * self.cfg.NY * self.cfg.NZ, dtype=np.float32)
+        _abi.check(_abi.load().wv_get_wavefield(self._h, shot, idx, _fp(out)))
+        return out
+
+    def set_wavefield(self, name_or_index, values, shot=0):
+        idx = name_or_index
+        if isinstance(idx, str):
+            idx = self.FIELDS[(self.dimension, self.type)].index(idx)
+        v = np.ascontiguousarray(values, dtype=np.float32).reshape(-1)
+        _abi.check(_abi.load().wv_set_wavefield(self._h, shot, idx, _fp(v)))
+
+
+def create(dimension: str, type_: str) -> ForwardSolver:
+    """ForwardSolver::Factory::Create(dimension, type)."""
+    return ForwardSolver(dimension, type_)
