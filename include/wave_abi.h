@@ -108,7 +108,7 @@ int wv_last_run_ms(wv_solver *s, double *ms);
 /* Per-kernel timing.  With wv_set_kernel_timing(s, 1), wv_run launches every
  * kernel eagerly between hipEvents on the handle's stream and accumulates the
  * elapsed time per kernel kind; wv_kernel_stats then reports, for kind
- * 0 = velocity-type phase, 1 = stress-type phase, 2 = source/receiver epilogue:
+ * 0 = velocity-type phase, 1 = stress-type phase, 2 = end-of-segment source/receiver flush:
  * the kernel name, its average duration [ms] per launch, the number of
  * launches, and the algorithmic HBM bytes one launch must move (every field
  * and coefficient of the batch read once, updated fields written once, CPML

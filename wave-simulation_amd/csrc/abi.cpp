@@ -82,6 +82,8 @@ struct wv_solver {
     bool acq_dirty = true;
     std::vector<void *> acq_allocs;
     int *d_src_ptr = nullptr, *d_src_type = nullptr, *d_rec_ptr = nullptr, *d_rec_type = nullptr;
+    int *d_bsrc_ptr = nullptr, *d_bsrc = nullptr, *d_brec_ptr = nullptr, *d_brec = nullptr;
+    int nbx = 0, nby = 0;
     long long *d_src_cell = nullptr, *d_rec_cell = nullptr;
     float *d_src_sig = nullptr, *d_seis = nullptr;
     int rec_total = 0;
@@ -196,6 +198,12 @@ struct wv_solver {
         A.rec_total = rec_total;
         A.seis = d_seis;
         A.tbase = d_tbase;
+        A.nbx = nbx;
+        A.nby = nby;
+        A.bsrc_ptr = d_bsrc_ptr;
+        A.bsrc = d_bsrc;
+        A.brec_ptr = d_brec_ptr;
+        A.brec = d_brec;
         return A;
     }
     int enqueue_steps(int nsteps);
@@ -216,13 +224,6 @@ wv_solver::~wv_solver()
     if (ev1) (void)hipEventDestroy(ev1);
     if (st) (void)hipStreamDestroy(st);
 }
-
-namespace {
-__global__ void set_int_kernel(int *p, int v)
-{
-    if (threadIdx.x == 0 && blockIdx.x == 0) p[0] = v;
-}
-}  // namespace
 
 int wv_solver::sync_acquisition()
 {
@@ -252,7 +253,33 @@ int wv_solver::sync_acquisition()
     }
     rec_total = (int)rc.size();
     rec_ptr_h = rp;
+    // per (shot, block) lists of the sources / receivers a tile owns, in list order
+    nbx = (L.NX + wv::TILE_X - 1) / wv::TILE_X;
+    nby = (L.NY + wv::TILE_Y - 1) / wv::TILE_Y;
+    const size_t nsb = (size_t)cfg.nshots * nbx * nby;
+    std::vector<std::vector<int>> bs(nsb), br(nsb);
+    {
+        int gs = 0, gr = 0;
+        for (int sh = 0; sh < (int)shots.size(); sh++) {
+            auto blk = [&](long long g) {
+                const int x = (int)(g % L.NX), y = (int)(g / L.NX);
+                return ((size_t)sh * nby + y / wv::TILE_Y) * nbx + x / wv::TILE_X;
+            };
+            for (long long g : shots[sh].sidx) bs[blk(g)].push_back(gs++);
+            for (long long g : shots[sh].ridx) br[blk(g)].push_back(gr++);
+        }
+    }
+    std::vector<int> bsp(1, 0), bsl, brp(1, 0), brl;
+    for (size_t b = 0; b < nsb; b++) {
+        bsl.insert(bsl.end(), bs[b].begin(), bs[b].end());
+        bsp.push_back((int)bsl.size());
+        brl.insert(brl.end(), br[b].begin(), br[b].end());
+        brp.push_back((int)brl.size());
+    }
     int r;
+    if ((r = upload(&d_bsrc_ptr, bsp, acq_allocs)) || (r = upload(&d_bsrc, bsl, acq_allocs)) ||
+        (r = upload(&d_brec_ptr, brp, acq_allocs)) || (r = upload(&d_brec, brl, acq_allocs)))
+        return r;
     if ((r = upload(&d_src_ptr, sp, acq_allocs)) || (r = upload(&d_rec_ptr, rp, acq_allocs)) ||
         (r = upload(&d_src_cell, sc, acq_allocs)) || (r = upload(&d_rec_cell, rc, acq_allocs)) ||
         (r = upload(&d_src_type, stp, acq_allocs)) || (r = upload(&d_rec_type, rtp, acq_allocs)) ||
@@ -273,11 +300,11 @@ int wv_solver::enqueue_steps(int nsteps)
     const wv::Cpml2D P = cpml2d();
     const wv::Acq A = acq();
     for (int j = 0; j < nsteps; j++) {
-        wv::launch_step2d(physics, cfg.spatial_fd_order, L, cfg.nshots, F, C, T, P, st, 0);
-        wv::launch_step2d(physics, cfg.spatial_fd_order, L, cfg.nshots, F, C, T, P, st, 1);
+        wv::launch_step2d(physics, cfg.spatial_fd_order, L, cfg.nshots, F, C, T, P, A, j, j > 0, st, 0);
+        wv::launch_step2d(physics, cfg.spatial_fd_order, L, cfg.nshots, F, C, T, P, A, j, 0, st, 1);
         if (cfg.damping_boundary == 1) wv::launch_abs_velocity2d(physics, L, cfg.nshots, F, C, st);
-        wv::launch_epilogue2d(physics, L, F, A, j, st);
     }
+    if (nsteps > 0) wv::launch_flush2d(physics, L, F, A, nsteps - 1, st);
     wv::launch_advance(d_tbase, nsteps, st);
     HIP_TRY(hipGetLastError());
     return 0;
@@ -300,19 +327,21 @@ int wv_solver::enqueue_steps_timed(int nsteps)
     for (int j0 = 0; j0 < nsteps; j0 += BATCH) {
         const int nb = std::min(BATCH, nsteps - j0);
         for (int b = 0; b < nb; b++) {
+            const int j = j0 + b;
             hipEvent_t *e = &kev[4 * b];
             HIP_TRY(hipEventRecord(e[0], st));
-            wv::launch_step2d(physics, cfg.spatial_fd_order, L, cfg.nshots, F, C, T, P, st, 0);
+            wv::launch_step2d(physics, cfg.spatial_fd_order, L, cfg.nshots, F, C, T, P, A, j, j > 0, st, 0);
             HIP_TRY(hipEventRecord(e[1], st));
-            wv::launch_step2d(physics, cfg.spatial_fd_order, L, cfg.nshots, F, C, T, P, st, 1);
+            wv::launch_step2d(physics, cfg.spatial_fd_order, L, cfg.nshots, F, C, T, P, A, j, 0, st, 1);
             HIP_TRY(hipEventRecord(e[2], st));
             if (cfg.damping_boundary == 1) wv::launch_abs_velocity2d(physics, L, cfg.nshots, F, C, st);
-            wv::launch_epilogue2d(physics, L, F, A, j0 + b, st);
+            if (j == nsteps - 1) wv::launch_flush2d(physics, L, F, A, j, st);
             HIP_TRY(hipEventRecord(e[3], st));
         }
         HIP_TRY(hipEventSynchronize(kev[4 * (nb - 1) + 3]));
         for (int b = 0; b < nb; b++) {
             for (int k = 0; k < 3; k++) {
+                if (k == 2 && j0 + b != nsteps - 1) continue;
                 float ms = 0;
                 HIP_TRY(hipEventElapsedTime(&ms, kev[4 * b + k], kev[4 * b + k + 1]));
                 kms[k] += ms;
@@ -701,7 +730,7 @@ int wv_run(wv_solver *s, int t_start, int t_end)
         // seismogram buffer was reallocated (zeroed)
     }
     HIP_TRY(hipEventRecord(s->ev0, s->st));
-    set_int_kernel<<<1, 64, 0, s->st>>>(s->d_tbase, t_start);
+    wv::launch_set(s->d_tbase, t_start, s->st);
     int remaining = t_end - t_start;
     const int K = s->ktiming ? 0 : s->cfg.steps_per_graph;
     if (s->ktiming) {
@@ -834,9 +863,9 @@ int wv_kernel_stats(wv_solver *s, int kind, const char **name, double *avg_ms, l
     if (!s || !s->inited) return fail(WV_ESTATE, "not initialised");
     if (kind < 0 || kind > 2) return fail(WV_EINVAL, "unknown kernel kind");
     if (s->plain) return fail(WV_EINVAL, "kernel timing is provided by the fused path");
-    static const char *names[3][3] = {{"ac2d_velocity", "ac2d_pressure", "epilogue2d"},
-                                      {"el2d_velocity", "el2d_stress", "epilogue2d"},
-                                      {"tm2d_h", "tm2d_e", "epilogue2d"}};
+    static const char *names[3][3] = {{"ac2d_velocity", "ac2d_pressure", "flush2d"},
+                                      {"el2d_velocity", "el2d_stress", "flush2d"},
+                                      {"tm2d_h", "tm2d_e", "flush2d"}};
     HIP_TRY(hipStreamSynchronize(s->st));
     if (name) *name = names[s->physics][kind];
     if (launches) *launches = s->klaunch[kind];

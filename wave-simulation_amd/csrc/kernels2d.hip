@@ -1,17 +1,21 @@
-// Fused 2D staggered-grid kernels for gfx950: acoustic, elastic P-SV and TM-mode EM.
+// Tiled 2D staggered-grid kernels for gfx950: acoustic, elastic P-SV and TM-mode EM.
 //
 // One time step of the reference (ForwardSolver2Dacoustic.cpp:91-194,
 // ForwardSolver2Delastic.cpp:117-294, ForwardSolver2Dtmem.cpp:108-171) is
-// executed as
-//   phase 0: velocity-type update (vx, vy | hx, hy) from the stress-type fields
-//   phase 1: stress-type update (p | sxx, syy, sxy | ez) from the new velocities,
-//            with CPML, free surface and the stress part of ABS fused in
-//   [ABS only: damping of the velocity-type fields]
-//   epilogue: source injection and receiver gather for every shot.
-// Each phase kernel reads every field and coefficient of its tile once from
-// HBM (halo re-reads hit L2), keeps the derivative taps in registers (x) and
-// scalar registers (y), and updates the CPML memory variables of frame cells
-// only.  All shots of a batch are covered by one launch (blockIdx.z = shot).
+// executed as two launches covering every shot of the batch (blockIdx.z):
+//   phase 0: velocity-type update (vx, vy | hx, hy) from the stress-type fields;
+//            first injects the velocity-type sources / gathers the
+//            velocity-type receivers left pending by the previous step
+//   phase 1: stress-type update (p | sxx, syy, sxy | ez) from the new
+//            velocities, with CPML, free surface and ABS of the stress-type
+//            fields fused; then injects the stress-type sources and gathers
+//            the stress-type receivers of this step
+// Each block stages its 64 x TY tile plus an H-wide halo of the fields it
+// differentiates in LDS.  All global loads of a block (tile, centre values,
+// coefficients, CPML memory variables) are issued before the first wait so the
+// kernel streams at memory-level parallelism instead of serialised round trips.
+// Derivative taps along x sit in registers (one column per lane), taps along y
+// in scalar registers (one row per wavefront).
 #include "wave_internal.hpp"
 
 namespace wv {
@@ -20,30 +24,50 @@ namespace {
 
 constexpr int NTHREADS = TILE_X * BLOCK_Y;
 
-template <int H>
+template <int H, int TY>
 struct Tile {
     static constexpr int RW = TILE_X + 2 * H;
-    static constexpr int RH = TILE_Y + 2 * H;
+    static constexpr int RH = TY + 2 * H;
+    static constexpr int TOT = RW * RH;
+    static constexpr int IT = (TOT + NTHREADS - 1) / NTHREADS;
 };
 
-// Stage the region [x0-H, x0+64+H) x [y0-H, y0+TILE_Y+H) of one field plane in LDS.
-template <int H>
-__device__ __forceinline__ void load_tile(float (*S)[Tile<H>::RW], const float *__restrict__ g, const Layout2D &L,
-                                          int x0, int y0)
-{
-    constexpr int RW = Tile<H>::RW, RH = Tile<H>::RH;
-    const int tid = threadIdx.y * TILE_X + threadIdx.x;
-    const long long base = L.at(x0 - H, y0 - H);
-    for (int i = tid; i < RW * RH; i += NTHREADS) {
-        const int r = i / RW;
-        const int c = i - r * RW;
-        S[r][c] = g[base + (long long)r * L.pitch + c];
+// Registers holding one thread's share of the region
+// [x0-H, x0+64+H) x [y0-H, y0+TY+H) of one field plane.
+template <int H, int TY>
+struct TileRegs {
+    using T = Tile<H, TY>;
+    float v[T::IT];
+    __device__ __forceinline__ void load(const float *__restrict__ g, const Layout2D &L, int x0, int y0, int tid)
+    {
+        const long long base = L.at(x0 - H, y0 - H);
+#pragma unroll
+        for (int k = 0; k < T::IT; k++) {
+            const int i = tid + k * NTHREADS;
+            if (T::TOT % NTHREADS == 0 || i < T::TOT) {
+                const int r = i / T::RW;
+                const int c = i - r * T::RW;
+                v[k] = g[base + (long long)r * L.pitch + c];
+            }
+        }
     }
-}
+    __device__ __forceinline__ void store(float (*S)[T::RW], int tid) const
+    {
+#pragma unroll
+        for (int k = 0; k < T::IT; k++) {
+            const int i = tid + k * NTHREADS;
+            if (T::TOT % NTHREADS == 0 || i < T::TOT) {
+                const int r = i / T::RW;
+                const int c = i - r * T::RW;
+                S[r][c] = v[k];
+            }
+        }
+    }
+};
 
 // derivative along x at LDS row r, column tx (taps for offsets -H..H)
-template <int H>
-__device__ __forceinline__ float dx(const float (*S)[Tile<H>::RW], int r, int tx, const float *c)
+template <int H, int RW>
+__device__ __forceinline__ float dx(const float (*S)[RW], int r, int tx, const float *c)
 {
     float s = 0.f;
 #pragma unroll
@@ -52,8 +76,8 @@ __device__ __forceinline__ float dx(const float (*S)[Tile<H>::RW], int r, int tx
 }
 
 // derivative along y at LDS row r (center), column tx; c is wavefront-uniform
-template <int H>
-__device__ __forceinline__ float dy(const float (*S)[Tile<H>::RW], int r, int tx, const float *__restrict__ c)
+template <int H, int RW>
+__device__ __forceinline__ float dy(const float (*S)[RW], int r, int tx, const float *__restrict__ c)
 {
     float s = 0.f;
 #pragma unroll
@@ -62,12 +86,12 @@ __device__ __forceinline__ float dy(const float (*S)[Tile<H>::RW], int r, int tx
 }
 
 // psi = b*psi + a*d ; d += psi   (CPML.cpp:84-95)
-__device__ __forceinline__ void cpml(float &d, float *__restrict__ psi, float a, float b)
+__device__ __forceinline__ float cpml(float &d, float psi, float a, float b)
 {
-    float p = b * (*psi);
+    float p = b * psi;
     p = p + a * d;
-    *psi = p;
     d = d + p;
+    return p;
 }
 
 struct Col {
@@ -84,7 +108,7 @@ __device__ __forceinline__ Col column(const Layout2D &L, const Cpml2D &P, int x)
     c.xi = -1;
     c.ax = c.bx = c.axh = c.bxh = 0.f;
     if (P.on) {
-        c.xi = P.xi[x];
+        c.xi = c.in ? P.xi[x] : -1;
         if (c.xi >= 0) {
             c.ax = P.ax[x];
             c.bx = P.bx[x];
@@ -102,25 +126,89 @@ __device__ __forceinline__ void load_taps(float *dst, const float *__restrict__ 
     for (int k = 0; k <= 2 * H; k++) dst[k] = tab[(long long)pos * (2 * H + 1) + k];
 }
 
+// velocity-type components: VX (2) and VY (3); stress-type: P / EZ (1)
+__device__ __forceinline__ bool velocity_type(int ty) { return ty == 2 || ty == 3; }
+
+// Sources and receivers owned by this block (shot-block sb) at step t, for
+// either the velocity-type or the stress-type components.  Uniform per block.
+template <int PHYS>
+__device__ void block_acquisition(const Acq &A, const Fields2D &F, long long so, int sb, int t, bool vel)
+{
+    const int s0 = A.bsrc_ptr[sb], s1 = A.bsrc_ptr[sb + 1];
+    const int r0 = A.brec_ptr[sb], r1 = A.brec_ptr[sb + 1];
+    if (s0 == s1 && r0 == r1) return;
+    const int tid = threadIdx.y * TILE_X + threadIdx.x;
+    __syncthreads();  // this block's field stores are visible to its threads
+    if (tid == 0) {
+        // components in the reference's order P, VX, VY, VZ; each in list order
+        for (int ty = 1; ty <= 4; ty++) {
+            if (velocity_type(ty) != vel) continue;
+            for (int e = s0; e < s1; e++) {
+                const int s = A.bsrc[e];
+                if (A.src_type[s] != ty) continue;
+                const float v = A.src_sig[(long long)s * A.NT + t];
+                const long long c = so + A.src_cell[s];
+                if (PHYS == ELASTIC && ty == 1) {
+                    F.f[2][c] += v;
+                    F.f[3][c] += v;
+                } else if (ty == 1) {
+                    F.f[2][c] += v;
+                } else {
+                    F.f[ty - 2][c] += v;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (int e = r0 + tid; e < r1; e += NTHREADS) {
+        const int r = A.brec[e];
+        const int ty = A.rec_type[r];
+        if (velocity_type(ty) != vel) continue;
+        const long long c = so + A.rec_cell[r];
+        float v;
+        if (PHYS == ELASTIC && ty == 1) {
+            v = F.f[2][c];
+            v += F.f[3][c];
+            v *= 0.5f;
+        } else if (ty == 1) {
+            v = F.f[2][c];
+        } else {
+            v = F.f[ty - 2][c];
+        }
+        A.seis[(long long)t * A.rec_total + r] = v;
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ int shot_block(const Acq &A) { return (blockIdx.z * A.nby + blockIdx.y) * A.nbx + blockIdx.x; }
+
 // ---------------------------------------------------------------------------
 // elastic P-SV
 // ---------------------------------------------------------------------------
-template <int H>
-__global__ __launch_bounds__(NTHREADS) void el2d_velocity(Layout2D L, Fields2D F, Coef2D C, Taps T, Cpml2D P)
+template <int H, int TY>
+__global__ __launch_bounds__(NTHREADS) void el2d_velocity(Layout2D L, Fields2D F, Coef2D C, Taps T, Cpml2D P, Acq A,
+                                                          int tofs, int pend)
 {
-    using TL = Tile<H>;
+    using TL = Tile<H, TY>;
+    constexpr int R = TY / BLOCK_Y;
     __shared__ float s_xx[TL::RH][TL::RW];
     __shared__ float s_yy[TL::RH][TL::RW];
     __shared__ float s_xy[TL::RH][TL::RW];
     const int shot = blockIdx.z;
     const long long so = (long long)shot * L.plane;
-    const int x0 = blockIdx.x * TILE_X, y0 = blockIdx.y * TILE_Y;
-    load_tile<H>(s_xx, F.f[2] + so, L, x0, y0);
-    load_tile<H>(s_yy, F.f[3] + so, L, x0, y0);
-    load_tile<H>(s_xy, F.f[4] + so, L, x0, y0);
-    __syncthreads();
+    const int x0 = blockIdx.x * TILE_X, y0 = blockIdx.y * TY;
+    const int tid = threadIdx.y * TILE_X + threadIdx.x, tx = threadIdx.x;
+    if (pend) block_acquisition<ELASTIC>(A, F, so, shot_block(A), A.tbase[0] + tofs - 1, true);
 
-    const int tx = threadIdx.x;
+    TileRegs<H, TY> rxx, ryy, rxy;
+    rxx.load(F.f[2] + so, L, x0, y0, tid);
+    ryy.load(F.f[3] + so, L, x0, y0, tid);
+    rxy.load(F.f[4] + so, L, x0, y0, tid);
+
+    float *__restrict__ vx = F.f[0] + so;
+    float *__restrict__ vy = F.f[1] + so;
+    const float *__restrict__ irx = C.c[2];
+    const float *__restrict__ iry = C.c[3];
     const Col col = column(L, P, x0 + tx);
     float cxf[2 * H + 1], cxb[2 * H + 1];
     load_taps<H>(cxf, T.xf, col.x);
@@ -128,284 +216,431 @@ __global__ __launch_bounds__(NTHREADS) void el2d_velocity(Layout2D L, Fields2D F
     const int W2 = 2 * P.W;
     const int wy = __builtin_amdgcn_readfirstlane(threadIdx.y);
 
-    for (int r = 0; r < TILE_Y / BLOCK_Y; r++) {
+    float ovx[R], ovy[R], crx[R], cry[R], p0[R], p1[R], q0[R], q1[R];
+    int yi[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int y = y0 + wy + r * BLOCK_Y;
+        const long long o = L.at(col.x, y);
+        ovx[r] = vx[o];
+        ovy[r] = vy[o];
+        crx[r] = irx[o];
+        cry[r] = iry[o];
+        p0[r] = p1[r] = q0[r] = q1[r] = 0.f;
+        yi[r] = P.on ? P.yi[y] : -1;
+        if (col.xi >= 0 && y < L.NY) {
+            const long long pi = ((long long)shot * L.NY + y) * W2 + col.xi;
+            p0[r] = P.psx[0][pi];
+            p1[r] = P.psx[1][pi];
+        }
+        if (yi[r] >= 0 && col.in) {
+            const long long pi = ((long long)shot * W2 + yi[r]) * L.NX + col.x;
+            q0[r] = P.psy[0][pi];
+            q1[r] = P.psy[1][pi];
+        }
+    }
+    rxx.store(s_xx, tid);
+    ryy.store(s_yy, tid);
+    rxy.store(s_xy, tid);
+    __syncthreads();
+
+#pragma unroll
+    for (int r = 0; r < R; r++) {
         const int ly = wy + r * BLOCK_Y;
         const int y = y0 + ly;
         if (y >= L.NY) break;
-        const int yi = P.on ? P.yi[y] : -1;
         const float *cyb = T.yb_v + (long long)y * (2 * H + 1);
         const float *cyf = T.yf_v + (long long)y * (2 * H + 1);
-        const long long o = L.at(col.x, y);
-        const int R = ly + H;
-
+        const int Rr = ly + H;
         // vx += 1/rho_x * (CPML(Dxf sxx) + CPML(Dyb sxy))
-        float u = dx<H>(s_xx, R, tx, cxf);
-        float t = dy<H>(s_xy, R, tx, cyb);
+        float u = dx<H>(s_xx, Rr, tx, cxf);
+        float t = dy<H>(s_xy, Rr, tx, cyb);
         // vy += 1/rho_y * (CPML(Dxb sxy) + CPML(Dyf syy))
-        float u2 = dx<H>(s_xy, R, tx, cxb);
-        float t2 = dy<H>(s_yy, R, tx, cyf);
+        float u2 = dx<H>(s_xy, Rr, tx, cxb);
+        float t2 = dy<H>(s_yy, Rr, tx, cyf);
         if (!col.in) continue;
         if (col.xi >= 0) {
             const long long pi = ((long long)shot * L.NY + y) * W2 + col.xi;
-            cpml(u, P.psx[0] + pi, col.axh, col.bxh);
-            cpml(u2, P.psx[1] + pi, col.ax, col.bx);
+            P.psx[0][pi] = cpml(u, p0[r], col.axh, col.bxh);
+            P.psx[1][pi] = cpml(u2, p1[r], col.ax, col.bx);
         }
-        if (yi >= 0) {
-            const long long pi = ((long long)shot * W2 + yi) * L.NX + col.x;
-            cpml(t, P.psy[0] + pi, P.ay[y], P.by[y]);
-            cpml(t2, P.psy[1] + pi, P.ayh[y], P.byh[y]);
+        if (yi[r] >= 0) {
+            const long long pi = ((long long)shot * W2 + yi[r]) * L.NX + col.x;
+            P.psy[0][pi] = cpml(t, q0[r], P.ay[y], P.by[y]);
+            P.psy[1][pi] = cpml(t2, q1[r], P.ayh[y], P.byh[y]);
         }
+        const long long o = L.at(col.x, y);
         u += t;
-        u *= C.c[2][o];
-        F.f[0][so + o] = F.f[0][so + o] + u;
+        u *= crx[r];
+        vx[o] = ovx[r] + u;
         u2 += t2;
-        u2 *= C.c[3][o];
-        F.f[1][so + o] = F.f[1][so + o] + u2;
+        u2 *= cry[r];
+        vy[o] = ovy[r] + u2;
     }
 }
 
-template <int H>
-__global__ __launch_bounds__(NTHREADS) void el2d_stress(Layout2D L, Fields2D F, Coef2D C, Taps T, Cpml2D P)
+template <int H, int TY>
+__global__ __launch_bounds__(NTHREADS) void el2d_stress(Layout2D L, Fields2D F, Coef2D C, Taps T, Cpml2D P, Acq A,
+                                                        int tofs, int pend)
 {
-    using TL = Tile<H>;
+    (void)pend;
+    using TL = Tile<H, TY>;
+    constexpr int R = TY / BLOCK_Y;
     __shared__ float s_vx[TL::RH][TL::RW];
     __shared__ float s_vy[TL::RH][TL::RW];
     const int shot = blockIdx.z;
     const long long so = (long long)shot * L.plane;
-    const int x0 = blockIdx.x * TILE_X, y0 = blockIdx.y * TILE_Y;
-    load_tile<H>(s_vx, F.f[0] + so, L, x0, y0);
-    load_tile<H>(s_vy, F.f[1] + so, L, x0, y0);
-    __syncthreads();
+    const int x0 = blockIdx.x * TILE_X, y0 = blockIdx.y * TY;
+    const int tid = threadIdx.y * TILE_X + threadIdx.x, tx = threadIdx.x;
 
-    const int tx = threadIdx.x;
+    TileRegs<H, TY> rvx, rvy;
+    rvx.load(F.f[0] + so, L, x0, y0, tid);
+    rvy.load(F.f[1] + so, L, x0, y0, tid);
+
+    float *__restrict__ sxx = F.f[2] + so;
+    float *__restrict__ syy = F.f[3] + so;
+    float *__restrict__ sxy = F.f[4] + so;
     const Col col = column(L, P, x0 + tx);
     float cxf[2 * H + 1], cxb[2 * H + 1];
     load_taps<H>(cxf, T.xf, col.x);
     load_taps<H>(cxb, T.xb, col.x);
     const int W2 = 2 * P.W;
     const int wy = __builtin_amdgcn_readfirstlane(threadIdx.y);
-    const bool fs = C.free_surface != 0;
-    const bool abs_on = C.damp != nullptr;
 
-    for (int r = 0; r < TILE_Y / BLOCK_Y; r++) {
+    float oxx[R], oyy[R], oxy[R], cM[R], cmu[R], cmxy[R], p2[R], p3[R], q2[R], q3[R], dmp[R];
+    int yi[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int y = y0 + wy + r * BLOCK_Y;
+        const long long o = L.at(col.x, y);
+        oxx[r] = sxx[o];
+        oyy[r] = syy[o];
+        oxy[r] = sxy[o];
+        cM[r] = C.c[0][o];
+        cmu[r] = C.c[1][o];
+        cmxy[r] = C.c[4][o];
+        dmp[r] = C.damp ? C.damp[o] : 1.f;
+        p2[r] = p3[r] = q2[r] = q3[r] = 0.f;
+        yi[r] = P.on ? P.yi[y] : -1;
+        if (col.xi >= 0 && y < L.NY) {
+            const long long pi = ((long long)shot * L.NY + y) * W2 + col.xi;
+            p2[r] = P.psx[2][pi];
+            p3[r] = P.psx[3][pi];
+        }
+        if (yi[r] >= 0 && col.in) {
+            const long long pi = ((long long)shot * W2 + yi[r]) * L.NX + col.x;
+            q2[r] = P.psy[2][pi];
+            q3[r] = P.psy[3][pi];
+        }
+    }
+    rvx.store(s_vx, tid);
+    rvy.store(s_vy, tid);
+    __syncthreads();
+
+    const bool fs = C.free_surface != 0;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
         const int ly = wy + r * BLOCK_Y;
         const int y = y0 + ly;
         if (y >= L.NY) break;
-        const int yi = P.on ? P.yi[y] : -1;
         const float *cyb = T.yb + (long long)y * (2 * H + 1);
         const float *cyf = T.yf + (long long)y * (2 * H + 1);
-        const long long o = L.at(col.x, y);
-        const int R = ly + H;
-
-        float vxx = dx<H>(s_vx, R, tx, cxb);
-        float vyy = dy<H>(s_vy, R, tx, cyb);
-        float u2 = dy<H>(s_vx, R, tx, cyf);
-        float t2 = dx<H>(s_vy, R, tx, cxf);
+        const int Rr = ly + H;
+        float vxx = dx<H>(s_vx, Rr, tx, cxb);
+        float vyy = dy<H>(s_vy, Rr, tx, cyb);
+        float u2 = dy<H>(s_vx, Rr, tx, cyf);
+        float t2 = dx<H>(s_vy, Rr, tx, cxf);
         if (!col.in) continue;
         if (col.xi >= 0) {
             const long long pi = ((long long)shot * L.NY + y) * W2 + col.xi;
-            cpml(vxx, P.psx[2] + pi, col.ax, col.bx);
-            cpml(t2, P.psx[3] + pi, col.axh, col.bxh);
+            P.psx[2][pi] = cpml(vxx, p2[r], col.ax, col.bx);
+            P.psx[3][pi] = cpml(t2, p3[r], col.axh, col.bxh);
         }
-        if (yi >= 0) {
-            const long long pi = ((long long)shot * W2 + yi) * L.NX + col.x;
-            cpml(vyy, P.psy[2] + pi, P.ay[y], P.by[y]);
-            cpml(u2, P.psy[3] + pi, P.ayh[y], P.byh[y]);
+        if (yi[r] >= 0) {
+            const long long pi = ((long long)shot * W2 + yi[r]) * L.NX + col.x;
+            P.psy[2][pi] = cpml(vyy, q2[r], P.ay[y], P.by[y]);
+            P.psy[3][pi] = cpml(u2, q3[r], P.ayh[y], P.byh[y]);
         }
-        const float M = C.c[0][o], mu = C.c[1][o];
-        float sxx = F.f[2][so + o], syy = F.f[3][so + o], sxy = F.f[4][so + o];
+        float xx = oxx[r], yy = oyy[r], xy = oxy[r];
         float u = vxx + vyy;
-        u *= M;
-        sxx += u;
-        syy += u;
-        sxx -= 2.f * (vyy * mu);
-        syy -= 2.f * (vxx * mu);
+        u *= cM[r];
+        xx += u;
+        yy += u;
+        xx -= 2.f * (vyy * cmu[r]);
+        yy -= 2.f * (vxx * cmu[r]);
         u2 += t2;
-        u2 *= C.c[4][o];
-        sxy += u2;
+        u2 *= cmxy[r];
+        xy += u2;
         if (fs && y == 0) {
             // FreeSurface2Delastic.cpp:14-45, FreeSurface.cpp:14-21
-            sxx += C.fsH[col.x] * vxx;
-            sxx -= C.fsV[col.x] * vyy;
-            syy *= 0.f;
+            xx += C.fsH[col.x] * vxx;
+            xx -= C.fsV[col.x] * vyy;
+            yy *= 0.f;
         }
-        if (abs_on) {
-            const float d = C.damp[o];
-            sxx *= d;
-            syy *= d;
-            sxy *= d;
-        }
-        F.f[2][so + o] = sxx;
-        F.f[3][so + o] = syy;
-        F.f[4][so + o] = sxy;
+        xx *= dmp[r];
+        yy *= dmp[r];
+        xy *= dmp[r];
+        const long long o = L.at(col.x, y);
+        sxx[o] = xx;
+        syy[o] = yy;
+        sxy[o] = xy;
     }
+    block_acquisition<ELASTIC>(A, F, so, shot_block(A), A.tbase[0] + tofs, false);
 }
 
 // ---------------------------------------------------------------------------
 // acoustic
 // ---------------------------------------------------------------------------
-template <int H>
-__global__ __launch_bounds__(NTHREADS) void ac2d_velocity(Layout2D L, Fields2D F, Coef2D C, Taps T, Cpml2D P)
+template <int H, int TY>
+__global__ __launch_bounds__(NTHREADS) void ac2d_velocity(Layout2D L, Fields2D F, Coef2D C, Taps T, Cpml2D P, Acq A,
+                                                          int tofs, int pend)
 {
-    using TL = Tile<H>;
+    using TL = Tile<H, TY>;
+    constexpr int R = TY / BLOCK_Y;
     __shared__ float s_p[TL::RH][TL::RW];
     const int shot = blockIdx.z;
     const long long so = (long long)shot * L.plane;
-    const int x0 = blockIdx.x * TILE_X, y0 = blockIdx.y * TILE_Y;
-    load_tile<H>(s_p, F.f[2] + so, L, x0, y0);
-    __syncthreads();
-    const int tx = threadIdx.x;
+    const int x0 = blockIdx.x * TILE_X, y0 = blockIdx.y * TY;
+    const int tid = threadIdx.y * TILE_X + threadIdx.x, tx = threadIdx.x;
+    if (pend) block_acquisition<ACOUSTIC>(A, F, so, shot_block(A), A.tbase[0] + tofs - 1, true);
+
+    TileRegs<H, TY> rp;
+    rp.load(F.f[2] + so, L, x0, y0, tid);
+    float *__restrict__ vx = F.f[0] + so;
+    float *__restrict__ vy = F.f[1] + so;
     const Col col = column(L, P, x0 + tx);
     float cxf[2 * H + 1];
     load_taps<H>(cxf, T.xf, col.x);
     const int W2 = 2 * P.W;
     const int wy = __builtin_amdgcn_readfirstlane(threadIdx.y);
-    for (int r = 0; r < TILE_Y / BLOCK_Y; r++) {
+    float ovx[R], ovy[R], crx[R], cry[R], p0[R], q0[R];
+    int yi[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int y = y0 + wy + r * BLOCK_Y;
+        const long long o = L.at(col.x, y);
+        ovx[r] = vx[o];
+        ovy[r] = vy[o];
+        crx[r] = C.c[1][o];
+        cry[r] = C.c[2][o];
+        p0[r] = q0[r] = 0.f;
+        yi[r] = P.on ? P.yi[y] : -1;
+        if (col.xi >= 0 && y < L.NY) p0[r] = P.psx[0][((long long)shot * L.NY + y) * W2 + col.xi];
+        if (yi[r] >= 0 && col.in) q0[r] = P.psy[0][((long long)shot * W2 + yi[r]) * L.NX + col.x];
+    }
+    rp.store(s_p, tid);
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < R; r++) {
         const int ly = wy + r * BLOCK_Y;
         const int y = y0 + ly;
         if (y >= L.NY) break;
-        const int yi = P.on ? P.yi[y] : -1;
         const float *cyf = T.yf_v + (long long)y * (2 * H + 1);
-        const long long o = L.at(col.x, y);
-        const int R = ly + H;
-        float u = dx<H>(s_p, R, tx, cxf);
-        float t = dy<H>(s_p, R, tx, cyf);
+        const int Rr = ly + H;
+        float u = dx<H>(s_p, Rr, tx, cxf);
+        float t = dy<H>(s_p, Rr, tx, cyf);
         if (!col.in) continue;
-        if (col.xi >= 0) cpml(u, P.psx[0] + ((long long)shot * L.NY + y) * W2 + col.xi, col.axh, col.bxh);
-        if (yi >= 0) cpml(t, P.psy[0] + ((long long)shot * W2 + yi) * L.NX + col.x, P.ayh[y], P.byh[y]);
-        u *= C.c[1][o];
-        F.f[0][so + o] = F.f[0][so + o] + u;
-        t *= C.c[2][o];
-        F.f[1][so + o] = F.f[1][so + o] + t;
+        if (col.xi >= 0) P.psx[0][((long long)shot * L.NY + y) * W2 + col.xi] = cpml(u, p0[r], col.axh, col.bxh);
+        if (yi[r] >= 0) P.psy[0][((long long)shot * W2 + yi[r]) * L.NX + col.x] = cpml(t, q0[r], P.ayh[y], P.byh[y]);
+        const long long o = L.at(col.x, y);
+        u *= crx[r];
+        vx[o] = ovx[r] + u;
+        t *= cry[r];
+        vy[o] = ovy[r] + t;
     }
 }
 
-template <int H>
-__global__ __launch_bounds__(NTHREADS) void ac2d_pressure(Layout2D L, Fields2D F, Coef2D C, Taps T, Cpml2D P)
+template <int H, int TY>
+__global__ __launch_bounds__(NTHREADS) void ac2d_pressure(Layout2D L, Fields2D F, Coef2D C, Taps T, Cpml2D P, Acq A,
+                                                          int tofs, int pend)
 {
-    using TL = Tile<H>;
+    (void)pend;
+    using TL = Tile<H, TY>;
+    constexpr int R = TY / BLOCK_Y;
     __shared__ float s_vx[TL::RH][TL::RW];
     __shared__ float s_vy[TL::RH][TL::RW];
     const int shot = blockIdx.z;
     const long long so = (long long)shot * L.plane;
-    const int x0 = blockIdx.x * TILE_X, y0 = blockIdx.y * TILE_Y;
-    load_tile<H>(s_vx, F.f[0] + so, L, x0, y0);
-    load_tile<H>(s_vy, F.f[1] + so, L, x0, y0);
-    __syncthreads();
-    const int tx = threadIdx.x;
+    const int x0 = blockIdx.x * TILE_X, y0 = blockIdx.y * TY;
+    const int tid = threadIdx.y * TILE_X + threadIdx.x, tx = threadIdx.x;
+    TileRegs<H, TY> rvx, rvy;
+    rvx.load(F.f[0] + so, L, x0, y0, tid);
+    rvy.load(F.f[1] + so, L, x0, y0, tid);
+    float *__restrict__ pr = F.f[2] + so;
     const Col col = column(L, P, x0 + tx);
     float cxb[2 * H + 1];
     load_taps<H>(cxb, T.xb, col.x);
     const int W2 = 2 * P.W;
     const int wy = __builtin_amdgcn_readfirstlane(threadIdx.y);
+    float op[R], cM[R], p1[R], q1[R], dmp[R];
+    int yi[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int y = y0 + wy + r * BLOCK_Y;
+        const long long o = L.at(col.x, y);
+        op[r] = pr[o];
+        cM[r] = C.c[0][o];
+        dmp[r] = C.damp ? C.damp[o] : 1.f;
+        p1[r] = q1[r] = 0.f;
+        yi[r] = P.on ? P.yi[y] : -1;
+        if (col.xi >= 0 && y < L.NY) p1[r] = P.psx[1][((long long)shot * L.NY + y) * W2 + col.xi];
+        if (yi[r] >= 0 && col.in) q1[r] = P.psy[1][((long long)shot * W2 + yi[r]) * L.NX + col.x];
+    }
+    rvx.store(s_vx, tid);
+    rvy.store(s_vy, tid);
+    __syncthreads();
     const bool fs = C.free_surface != 0;
-    for (int r = 0; r < TILE_Y / BLOCK_Y; r++) {
+#pragma unroll
+    for (int r = 0; r < R; r++) {
         const int ly = wy + r * BLOCK_Y;
         const int y = y0 + ly;
         if (y >= L.NY) break;
-        const int yi = P.on ? P.yi[y] : -1;
         const float *cyb = T.yb + (long long)y * (2 * H + 1);
-        const long long o = L.at(col.x, y);
-        const int R = ly + H;
-        float u = dx<H>(s_vx, R, tx, cxb);
-        float t = dy<H>(s_vy, R, tx, cyb);
+        const int Rr = ly + H;
+        float u = dx<H>(s_vx, Rr, tx, cxb);
+        float t = dy<H>(s_vy, Rr, tx, cyb);
         if (!col.in) continue;
-        if (col.xi >= 0) cpml(u, P.psx[1] + ((long long)shot * L.NY + y) * W2 + col.xi, col.ax, col.bx);
-        if (yi >= 0) cpml(t, P.psy[1] + ((long long)shot * W2 + yi) * L.NX + col.x, P.ay[y], P.by[y]);
+        if (col.xi >= 0) P.psx[1][((long long)shot * L.NY + y) * W2 + col.xi] = cpml(u, p1[r], col.ax, col.bx);
+        if (yi[r] >= 0) P.psy[1][((long long)shot * W2 + yi[r]) * L.NX + col.x] = cpml(t, q1[r], P.ay[y], P.by[y]);
         u += t;
-        u *= C.c[0][o];
-        float p = F.f[2][so + o] + u;
-        if (C.damp) p *= C.damp[o];
+        u *= cM[r];
+        float p = op[r] + u;
+        p *= dmp[r];
         if (fs && y == 0) p *= 0.f;
-        F.f[2][so + o] = p;
+        pr[L.at(col.x, y)] = p;
     }
+    block_acquisition<ACOUSTIC>(A, F, so, shot_block(A), A.tbase[0] + tofs, false);
 }
 
 // ---------------------------------------------------------------------------
 // TM-mode EM
 // ---------------------------------------------------------------------------
-template <int H>
-__global__ __launch_bounds__(NTHREADS) void tm2d_h(Layout2D L, Fields2D F, Coef2D C, Taps T, Cpml2D P)
+template <int H, int TY>
+__global__ __launch_bounds__(NTHREADS) void tm2d_h(Layout2D L, Fields2D F, Coef2D C, Taps T, Cpml2D P, Acq A,
+                                                   int tofs, int pend)
 {
-    using TL = Tile<H>;
+    (void)A;
+    (void)tofs;
+    (void)pend;
+    using TL = Tile<H, TY>;
+    constexpr int R = TY / BLOCK_Y;
     __shared__ float s_ez[TL::RH][TL::RW];
     const int shot = blockIdx.z;
     const long long so = (long long)shot * L.plane;
-    const int x0 = blockIdx.x * TILE_X, y0 = blockIdx.y * TILE_Y;
-    load_tile<H>(s_ez, F.f[2] + so, L, x0, y0);
-    __syncthreads();
-    const int tx = threadIdx.x;
+    const int x0 = blockIdx.x * TILE_X, y0 = blockIdx.y * TY;
+    const int tid = threadIdx.y * TILE_X + threadIdx.x, tx = threadIdx.x;
+    TileRegs<H, TY> rez;
+    rez.load(F.f[2] + so, L, x0, y0, tid);
+    float *__restrict__ hx = F.f[0] + so;
+    float *__restrict__ hy = F.f[1] + so;
     const Col col = column(L, P, x0 + tx);
     float cxf[2 * H + 1];
     load_taps<H>(cxf, T.xf, col.x);
     const int W2 = 2 * P.W;
     const int wy = __builtin_amdgcn_readfirstlane(threadIdx.y);
-    for (int r = 0; r < TILE_Y / BLOCK_Y; r++) {
+    float ohx[R], ohy[R], cxz[R], cyz[R], p0[R], q0[R];
+    int yi[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int y = y0 + wy + r * BLOCK_Y;
+        const long long o = L.at(col.x, y);
+        ohx[r] = hx[o];
+        ohy[r] = hy[o];
+        cxz[r] = C.c[2][o];
+        cyz[r] = C.c[3][o];
+        p0[r] = q0[r] = 0.f;
+        yi[r] = P.on ? P.yi[y] : -1;
+        if (col.xi >= 0 && y < L.NY) p0[r] = P.psx[0][((long long)shot * L.NY + y) * W2 + col.xi];
+        if (yi[r] >= 0 && col.in) q0[r] = P.psy[0][((long long)shot * W2 + yi[r]) * L.NX + col.x];
+    }
+    rez.store(s_ez, tid);
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < R; r++) {
         const int ly = wy + r * BLOCK_Y;
         const int y = y0 + ly;
         if (y >= L.NY) break;
-        const int yi = P.on ? P.yi[y] : -1;
         const float *cyf = T.yf + (long long)y * (2 * H + 1);
-        const long long o = L.at(col.x, y);
-        const int R = ly + H;
-        float u = dy<H>(s_ez, R, tx, cyf);
-        float u2 = dx<H>(s_ez, R, tx, cxf);
+        const int Rr = ly + H;
+        float u = dy<H>(s_ez, Rr, tx, cyf);
+        float u2 = dx<H>(s_ez, Rr, tx, cxf);
         if (!col.in) continue;
-        if (yi >= 0) cpml(u, P.psy[0] + ((long long)shot * W2 + yi) * L.NX + col.x, P.ayh[y], P.byh[y]);
-        if (col.xi >= 0) cpml(u2, P.psx[0] + ((long long)shot * L.NY + y) * W2 + col.xi, col.axh, col.bxh);
+        if (yi[r] >= 0) P.psy[0][((long long)shot * W2 + yi[r]) * L.NX + col.x] = cpml(u, q0[r], P.ayh[y], P.byh[y]);
+        if (col.xi >= 0) P.psx[0][((long long)shot * L.NY + y) * W2 + col.xi] = cpml(u2, p0[r], col.axh, col.bxh);
         // hx -= 1/mu_yz * CPML(Dyf ez) ; hy -= 1/mu_xz * (-CPML(Dxf ez))
-        u *= C.c[3][o];
-        F.f[0][so + o] = F.f[0][so + o] - u;
+        const long long o = L.at(col.x, y);
+        u *= cyz[r];
+        hx[o] = ohx[r] - u;
         float w = -u2;
-        w *= C.c[2][o];
-        F.f[1][so + o] = F.f[1][so + o] - w;
+        w *= cxz[r];
+        hy[o] = ohy[r] - w;
     }
 }
 
-template <int H>
-__global__ __launch_bounds__(NTHREADS) void tm2d_e(Layout2D L, Fields2D F, Coef2D C, Taps T, Cpml2D P)
+template <int H, int TY>
+__global__ __launch_bounds__(NTHREADS) void tm2d_e(Layout2D L, Fields2D F, Coef2D C, Taps T, Cpml2D P, Acq A,
+                                                   int tofs, int pend)
 {
-    using TL = Tile<H>;
+    (void)pend;
+    using TL = Tile<H, TY>;
+    constexpr int R = TY / BLOCK_Y;
     __shared__ float s_hx[TL::RH][TL::RW];
     __shared__ float s_hy[TL::RH][TL::RW];
     const int shot = blockIdx.z;
     const long long so = (long long)shot * L.plane;
-    const int x0 = blockIdx.x * TILE_X, y0 = blockIdx.y * TILE_Y;
-    load_tile<H>(s_hx, F.f[0] + so, L, x0, y0);
-    load_tile<H>(s_hy, F.f[1] + so, L, x0, y0);
-    __syncthreads();
-    const int tx = threadIdx.x;
+    const int x0 = blockIdx.x * TILE_X, y0 = blockIdx.y * TY;
+    const int tid = threadIdx.y * TILE_X + threadIdx.x, tx = threadIdx.x;
+    TileRegs<H, TY> rhx, rhy;
+    rhx.load(F.f[0] + so, L, x0, y0, tid);
+    rhy.load(F.f[1] + so, L, x0, y0, tid);
+    float *__restrict__ ez = F.f[2] + so;
     const Col col = column(L, P, x0 + tx);
     float cxb[2 * H + 1];
     load_taps<H>(cxb, T.xb, col.x);
     const int W2 = 2 * P.W;
     const int wy = __builtin_amdgcn_readfirstlane(threadIdx.y);
-    for (int r = 0; r < TILE_Y / BLOCK_Y; r++) {
+    float oez[R], ca[R], cb[R], p1[R], q1[R], dmp[R];
+    int yi[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int y = y0 + wy + r * BLOCK_Y;
+        const long long o = L.at(col.x, y);
+        oez[r] = ez[o];
+        ca[r] = C.c[0][o];
+        cb[r] = C.c[1][o];
+        dmp[r] = C.damp ? C.damp[o] : 1.f;
+        p1[r] = q1[r] = 0.f;
+        yi[r] = P.on ? P.yi[y] : -1;
+        if (col.xi >= 0 && y < L.NY) p1[r] = P.psx[1][((long long)shot * L.NY + y) * W2 + col.xi];
+        if (yi[r] >= 0 && col.in) q1[r] = P.psy[1][((long long)shot * W2 + yi[r]) * L.NX + col.x];
+    }
+    rhx.store(s_hx, tid);
+    rhy.store(s_hy, tid);
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < R; r++) {
         const int ly = wy + r * BLOCK_Y;
         const int y = y0 + ly;
         if (y >= L.NY) break;
-        const int yi = P.on ? P.yi[y] : -1;
         const float *cyb = T.yb + (long long)y * (2 * H + 1);
-        const long long o = L.at(col.x, y);
-        const int R = ly + H;
-        float u = dx<H>(s_hy, R, tx, cxb);
-        float t = dy<H>(s_hx, R, tx, cyb);
+        const int Rr = ly + H;
+        float u = dx<H>(s_hy, Rr, tx, cxb);
+        float t = dy<H>(s_hx, Rr, tx, cyb);
         if (!col.in) continue;
-        if (col.xi >= 0) cpml(u, P.psx[1] + ((long long)shot * L.NY + y) * W2 + col.xi, col.ax, col.bx);
-        if (yi >= 0) cpml(t, P.psy[1] + ((long long)shot * W2 + yi) * L.NX + col.x, P.ay[y], P.by[y]);
+        if (col.xi >= 0) P.psx[1][((long long)shot * L.NY + y) * W2 + col.xi] = cpml(u, p1[r], col.ax, col.bx);
+        if (yi[r] >= 0) P.psy[1][((long long)shot * W2 + yi[r]) * L.NX + col.x] = cpml(t, q1[r], P.ay[y], P.by[y]);
         u -= t;
-        u *= C.c[1][o];
-        float ez = C.c[0][o] * F.f[2][so + o] + u;
-        if (C.damp) ez *= C.damp[o];
-        F.f[2][so + o] = ez;
+        u *= cb[r];
+        float e = ca[r] * oez[r] + u;
+        e *= dmp[r];
+        ez[L.at(col.x, y)] = e;
     }
+    block_acquisition<TMEM>(A, F, so, shot_block(A), A.tbase[0] + tofs, false);
 }
 
-// ---------------------------------------------------------------------------
-// sources and receivers (SourceReceiverImplSeismic.cpp:34-105, FDTD2Delastic.cpp:12-49)
-// ---------------------------------------------------------------------------
+// velocity-type sources / receivers of step t for every shot (end of a run segment)
 template <int PHYS>
-__global__ __launch_bounds__(256) void epilogue2d(Layout2D L, Fields2D F, Acq A, int tofs)
+__global__ __launch_bounds__(256) void flush2d(Layout2D L, Fields2D F, Acq A, int tofs)
 {
     const int shot = blockIdx.x;
     const long long so = (long long)shot * L.plane;
@@ -413,49 +648,18 @@ __global__ __launch_bounds__(256) void epilogue2d(Layout2D L, Fields2D F, Acq A,
     if (t < 0 || t >= A.NT) return;
     if (threadIdx.x == 0) {
         const int s0 = A.src_ptr[shot], s1 = A.src_ptr[shot + 1];
-        for (int ty = 1; ty <= 4; ty++)
+        for (int ty = 2; ty <= 3; ty++)
             for (int s = s0; s < s1; s++) {
                 if (A.src_type[s] != ty) continue;
-                const float v = A.src_sig[(long long)s * A.NT + t];
-                const long long c = so + A.src_cell[s];
-                if (PHYS == ELASTIC) {
-                    if (ty == 1) {
-                        F.f[2][c] += v;
-                        F.f[3][c] += v;
-                    } else if (ty == 2) {
-                        F.f[0][c] += v;
-                    } else if (ty == 3) {
-                        F.f[1][c] += v;
-                    }
-                } else if (PHYS == ACOUSTIC) {
-                    if (ty == 1) F.f[2][c] += v;
-                    else if (ty == 2) F.f[0][c] += v;
-                    else if (ty == 3) F.f[1][c] += v;
-                } else {
-                    if (ty == 1) F.f[2][c] += v;
-                }
+                F.f[ty - 2][so + A.src_cell[s]] += A.src_sig[(long long)s * A.NT + t];
             }
     }
     __syncthreads();
     const int r0 = A.rec_ptr[shot], r1 = A.rec_ptr[shot + 1];
     for (int r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
-        const long long c = so + A.rec_cell[r];
         const int ty = A.rec_type[r];
-        float v;
-        if (PHYS == ELASTIC) {
-            if (ty == 1) {
-                v = F.f[2][c];
-                v += F.f[3][c];
-                v *= 0.5f;
-            } else {
-                v = ty == 2 ? F.f[0][c] : F.f[1][c];
-            }
-        } else if (PHYS == ACOUSTIC) {
-            v = ty == 1 ? F.f[2][c] : (ty == 2 ? F.f[0][c] : F.f[1][c]);
-        } else {
-            v = F.f[2][c];
-        }
-        A.seis[(long long)t * A.rec_total + r] = v;
+        if (!velocity_type(ty)) continue;
+        A.seis[(long long)t * A.rec_total + r] = F.f[ty - 2][so + A.rec_cell[r]];
     }
 }
 
@@ -473,54 +677,57 @@ __global__ void advance_kernel(int *tbase, int k)
     if (threadIdx.x == 0 && blockIdx.x == 0) tbase[0] += k;
 }
 
-template <int H>
-void launch_h(int physics, int which, dim3 grid, const Layout2D &L, const Fields2D &F, const Coef2D &C,
-              const Taps &T, const Cpml2D &P, hipStream_t st)
+__global__ void set_kernel(int *p, int v)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) p[0] = v;
+}
+
+template <int H, int TY>
+void launch_h(int physics, int which, dim3 grid, const Layout2D &L, const Fields2D &F, const Coef2D &C, const Taps &T,
+              const Cpml2D &P, const Acq &A, int tofs, int pend, hipStream_t st)
 {
     dim3 block(TILE_X, BLOCK_Y);
     if (physics == ELASTIC) {
         if (which == 0)
-            el2d_velocity<H><<<grid, block, 0, st>>>(L, F, C, T, P);
+            el2d_velocity<H, TY><<<grid, block, 0, st>>>(L, F, C, T, P, A, tofs, pend);
         else
-            el2d_stress<H><<<grid, block, 0, st>>>(L, F, C, T, P);
+            el2d_stress<H, TY><<<grid, block, 0, st>>>(L, F, C, T, P, A, tofs, pend);
     } else if (physics == ACOUSTIC) {
         if (which == 0)
-            ac2d_velocity<H><<<grid, block, 0, st>>>(L, F, C, T, P);
+            ac2d_velocity<H, TY><<<grid, block, 0, st>>>(L, F, C, T, P, A, tofs, pend);
         else
-            ac2d_pressure<H><<<grid, block, 0, st>>>(L, F, C, T, P);
+            ac2d_pressure<H, TY><<<grid, block, 0, st>>>(L, F, C, T, P, A, tofs, pend);
     } else {
         if (which == 0)
-            tm2d_h<H><<<grid, block, 0, st>>>(L, F, C, T, P);
+            tm2d_h<H, TY><<<grid, block, 0, st>>>(L, F, C, T, P, A, tofs, pend);
         else
-            tm2d_e<H><<<grid, block, 0, st>>>(L, F, C, T, P);
+            tm2d_e<H, TY><<<grid, block, 0, st>>>(L, F, C, T, P, A, tofs, pend);
     }
 }
 
 }  // namespace
 
 void launch_step2d(int physics, int order, const Layout2D &L, int nshots, const Fields2D &F, const Coef2D &C,
-                   const Taps &T, const Cpml2D &P, hipStream_t stream, int which)
+                   const Taps &T, const Cpml2D &P, const Acq &A, int tofs, int pend, hipStream_t stream, int which)
 {
     dim3 grid((L.NX + TILE_X - 1) / TILE_X, (L.NY + TILE_Y - 1) / TILE_Y, nshots);
     switch (order / 2) {
-    case 1: launch_h<1>(physics, which, grid, L, F, C, T, P, stream); break;
-    case 2: launch_h<2>(physics, which, grid, L, F, C, T, P, stream); break;
-    case 3: launch_h<3>(physics, which, grid, L, F, C, T, P, stream); break;
-    case 4: launch_h<4>(physics, which, grid, L, F, C, T, P, stream); break;
-    case 5: launch_h<5>(physics, which, grid, L, F, C, T, P, stream); break;
-    case 6: launch_h<6>(physics, which, grid, L, F, C, T, P, stream); break;
+    case 1: launch_h<1, TILE_Y>(physics, which, grid, L, F, C, T, P, A, tofs, pend, stream); break;
+    case 2: launch_h<2, TILE_Y>(physics, which, grid, L, F, C, T, P, A, tofs, pend, stream); break;
+    case 3: launch_h<3, TILE_Y>(physics, which, grid, L, F, C, T, P, A, tofs, pend, stream); break;
+    case 4: launch_h<4, TILE_Y>(physics, which, grid, L, F, C, T, P, A, tofs, pend, stream); break;
+    case 5: launch_h<5, TILE_Y>(physics, which, grid, L, F, C, T, P, A, tofs, pend, stream); break;
+    case 6: launch_h<6, TILE_Y>(physics, which, grid, L, F, C, T, P, A, tofs, pend, stream); break;
     default: break;
     }
 }
 
-void launch_epilogue2d(int physics, const Layout2D &L, const Fields2D &F, const Acq &A, int tofs, hipStream_t stream)
+void launch_flush2d(int physics, const Layout2D &L, const Fields2D &F, const Acq &A, int tofs, hipStream_t stream)
 {
     if (physics == ELASTIC)
-        epilogue2d<ELASTIC><<<A.nshots, 256, 0, stream>>>(L, F, A, tofs);
+        flush2d<ELASTIC><<<A.nshots, 256, 0, stream>>>(L, F, A, tofs);
     else if (physics == ACOUSTIC)
-        epilogue2d<ACOUSTIC><<<A.nshots, 256, 0, stream>>>(L, F, A, tofs);
-    else
-        epilogue2d<TMEM><<<A.nshots, 256, 0, stream>>>(L, F, A, tofs);
+        flush2d<ACOUSTIC><<<A.nshots, 256, 0, stream>>>(L, F, A, tofs);
 }
 
 void launch_abs_velocity2d(int physics, const Layout2D &L, int nshots, const Fields2D &F, const Coef2D &C,
@@ -532,5 +739,6 @@ void launch_abs_velocity2d(int physics, const Layout2D &L, int nshots, const Fie
 }
 
 void launch_advance(int *tbase, int k, hipStream_t stream) { advance_kernel<<<1, 64, 0, stream>>>(tbase, k); }
+void launch_set(int *p, int v, hipStream_t stream) { set_kernel<<<1, 64, 0, stream>>>(p, v); }
 
 }  // namespace wv

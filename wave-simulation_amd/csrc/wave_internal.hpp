@@ -13,8 +13,8 @@ enum Physics { ACOUSTIC = 0, ELASTIC = 1, TMEM = 2 };
 
 // Maximum half-width of a derivative stencil window (order 12 -> 6).
 constexpr int MAX_H = 6;
-// Tile of the 2D fused kernels: 64 columns (one wavefront) x TILE_Y rows,
-// 256 threads per block (4 wavefronts stacked in y).
+// Tile of the 2D kernels: 64 columns (one wavefront) x TILE_Y rows, 256
+// threads per block (4 wavefronts stacked in y, each thread TILE_Y/4 rows).
 constexpr int TILE_X = 64;
 constexpr int TILE_Y = 16;
 constexpr int BLOCK_Y = 4;
@@ -26,23 +26,21 @@ constexpr int XPAD = 32;
 // so stencil taps that leave the grid read zeros.
 struct Layout2D {
     int NX, NY;
-    int H;          // taps reach [-H, H]
-    int ypad;       // rows of zero padding above the grid (= H)
-    int pitch;      // floats per row
-    int rows;       // rows per field plane
+    int H;            // taps reach [-H, H]
+    int ypad;         // rows of zero padding above the grid (= H)
+    int pitch;        // floats per row
+    int rows;         // rows per field plane
     long long plane;  // floats per plane (one field of one shot)
-    __host__ __device__ long long at(int x, int y) const {
-        return (long long)(y + ypad) * pitch + (x + XPAD);
-    }
+    __host__ __device__ long long at(int x, int y) const { return (long long)(y + ypad) * pitch + (x + XPAD); }
 };
 
 // Tap tables: for every position along an axis, 2H+1 coefficients for the
 // offsets -H..H (already multiplied by DT/DH).  Rows beyond the grid are zero.
 struct Taps {
-    const float *xf, *xb;       // [NXpad][2H+1]
-    const float *yf, *yb;       // [NYpad][2H+1] plain operators
-    const float *yf_v, *yb_v;   // y operators used in the velocity-type update
-                                // (free-surface image versions when FreeSurface=1)
+    const float *xf, *xb;      // [NXpad][2H+1]
+    const float *yf, *yb;      // [NYpad][2H+1] plain operators
+    const float *yf_v, *yb_v;  // y operators used in the velocity-type update
+                               // (free-surface image versions when FreeSurface=1)
 };
 
 // CPML coefficients along each axis (zero outside the frame) and the compact
@@ -68,34 +66,49 @@ struct Coef2D {
     // elastic : c0 = M, c1 = mu, c2 = 1/rho_x, c3 = 1/rho_y, c4 = mu_xy
     // tmem    : c0 = Ca, c1 = Cb, c2 = 1/mu_xz, c3 = 1/mu_yz
     const float *c[5];
-    const float *damp;   // ABS damping factors (padded layout) or nullptr
-    const float *fsH;    // [NXpad] free-surface horizontal scale at y = 0 (elastic) or nullptr
-    const float *fsV;    // [NXpad] free-surface vertical scale at y = 0 (elastic)
+    const float *damp;  // ABS damping factors (padded layout) or nullptr
+    const float *fsH;   // [NXpad] free-surface horizontal scale at y = 0 (elastic) or nullptr
+    const float *fsV;   // [NXpad] free-surface vertical scale at y = 0 (elastic)
     int free_surface;
 };
 
+// Sources and receivers of all shots.  Sources of the stress-type fields (P,
+// EZ) are injected and stress-type receivers gathered by the block that owns
+// the cell, at the end of the stress-type kernel of step t.  Velocity-type
+// sources (VX, VY) of step t are injected, and velocity-type receivers of
+// step t gathered, by the owning block at the start of the velocity-type
+// kernel of step t+1 (before anything reads the cell), or by the flush kernel
+// after the last step of a run segment.
 struct Acq {
     int nshots;
     int NT;
-    const int *src_ptr;     // [nshots+1]
-    const long long *src_cell;  // field offsets (layout) of each source
-    const int *src_type;    // 1..4
-    const float *src_sig;   // [nsrc_total][NT]
-    const int *rec_ptr;     // [nshots+1]
+    int nbx, nby;               // blocks per shot of the tiled kernels
+    const int *src_ptr;         // [nshots+1] sources of each shot
+    const long long *src_cell;  // field offset (layout) of each source
+    const int *src_type;        // 1..4
+    const float *src_sig;       // [nsrc_total][NT]
+    const int *rec_ptr;         // [nshots+1]
     const long long *rec_cell;
     const int *rec_type;
     int rec_total;
-    float *seis;            // time-major [NT][rec_total]
-    const int *tbase;       // device step counter base
+    float *seis;                // time-major [NT][rec_total]
+    const int *bsrc_ptr;        // [nshots*nbx*nby+1] sources per (shot, block), in source order
+    const int *bsrc;            // global source ids
+    const int *brec_ptr;        // [nshots*nbx*nby+1]
+    const int *brec;            // global receiver ids
+    const int *tbase;           // device step counter base
 };
 
 // Launchers (kernels2d.hip).  All launches go to `stream`.
+// which = 0: velocity-type phase (pend = apply the velocity-type sources /
+// receivers of the previous step first); which = 1: stress-type phase.
 void launch_step2d(int physics, int order, const Layout2D &L, int nshots, const Fields2D &F, const Coef2D &C,
-                   const Taps &T, const Cpml2D &P, hipStream_t stream, int which);
-void launch_epilogue2d(int physics, const Layout2D &L, const Fields2D &F, const Acq &A, int tofs,
-                       hipStream_t stream);
+                   const Taps &T, const Cpml2D &P, const Acq &A, int tofs, int pend, hipStream_t stream, int which);
+// velocity-type sources / receivers of step tbase + tofs (end of a run segment)
+void launch_flush2d(int physics, const Layout2D &L, const Fields2D &F, const Acq &A, int tofs, hipStream_t stream);
 void launch_abs_velocity2d(int physics, const Layout2D &L, int nshots, const Fields2D &F, const Coef2D &C,
                            hipStream_t stream);
 void launch_advance(int *tbase, int k, hipStream_t stream);
+void launch_set(int *p, int v, hipStream_t stream);
 
 }  // namespace wv
