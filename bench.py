@@ -7,8 +7,9 @@ perturbations (seeds 2 / 102), one 30 Hz Ricker VY force source, 500 VX + 500
 VY receivers.  Synthetic data, FP32 arithmetic, inputs resident in HBM before
 the timed region.
 
-Printed JSON line: metric = grid-cell updates/s of the fused batched path;
-roofline of the dominant kernel (HIP-event timing on the solver's stream);
+Printed JSON line: metric = grid-cell updates/s of the tiled path (kernel
+variant chosen by the autotuner); roofline of the dominant kernel (start/stop
+events attached to each dispatch on the solver's stream);
 the same workload on the reference-sequence GPU path ("plain", one kernel per
 LAMA operation) for the batched-vs-plain ratio; and the CPU baseline: the
 oracle (C restatement with the reference's operator sequence, OpenMP) on a
@@ -39,7 +40,12 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--plain-steps", type=int, default=400)
-    ap.add_argument("--cpu-steps", type=int, default=150)
+    ap.add_argument("--cpu-steps", type=int, default=3000,
+                    help="time steps of the CPU baseline sample (about 10-15 s on 16 cores)")
+    ap.add_argument("--tile-y", type=int, default=0, help="rows per tile (0 = autotune over all kernel variants)")
+    ap.add_argument("--path", default="fused", choices=["fused", "phased"])
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="also run a 256 MiB device copy (calibrates the profiler's byte counters)")
     return ap.parse_args()
 
 
@@ -80,7 +86,8 @@ def main():
 
     case = cases.elastic_c2()
     NT = case.settings["NT"]
-    sol = cases.make_solver(case, steps_per_graph=args.graph)
+    sol = cases.make_solver(case, steps_per_graph=args.graph, tile_y=args.tile_y, path=args.path)
+    choice = sol.tune(200) if (args.tile_y == 0 and args.path == "fused") else f"{args.path}-{args.tile_y}"
 
     for _ in range(args.warmup):
         sol.reset()
@@ -141,7 +148,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": "2D elastic P-SV order 8, 1000x1000, dx 1 m, dt 1e-4 s, 4000 steps, 30-cell CPML, "
                                "10 layers + 3% Gaussian perturbation, 1 VY Ricker 30 Hz shot, 1000 receivers",
-                   "path": "fused", "steps_per_graph": args.graph, "shots_per_step": 1},
+                   "path": args.path, "kernel": choice, "steps_per_graph": args.graph, "shots_per_step": 1},
         "hbm_frac_step": round(step_bytes * args.steps / wall / 1e9 / HBM_PEAK_GBS, 4),
         "gpu_event_ms_per_step": round(gpu_ms / args.steps, 3),
         "kernels": [{k: (round(v, 5) if isinstance(v, float) else v) for k, v in d.items()} for d in kernels],
@@ -164,6 +171,15 @@ def main():
 
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(case, args.cpu_steps)
+    if args.copy_calib:
+        import torch
+
+        a = torch.ones(64 * 1024 * 1024, device="cuda", dtype=torch.float32)
+        bb = torch.empty_like(a)
+        for _ in range(5):
+            bb.copy_(a)
+        torch.cuda.synchronize()
+        out["copy_calib_bytes"] = 2 * a.numel() * 4
     print(json.dumps(out))
 
 

@@ -46,8 +46,10 @@ extern "C" {
 #define WV_OPS_STENCIL 1 /* LAMA StencilMatrix: points outside the grid dropped (FDTD2D.cpp:156-185) */
 
 /* execution path */
-#define WV_PATH_FUSED 0  /* hand-written tiled stencil kernels, all shots per launch */
+#define WV_PATH_FUSED 0  /* hand-written kernels, all shots per launch; one launch per time step
+                            where a single-pass kernel exists (2D elastic), else WV_PATH_PHASED */
 #define WV_PATH_PLAIN 1  /* reference operator sequence: CSR SpMV + vector passes, shot by shot */
+#define WV_PATH_PHASED 2 /* hand-written kernels, velocity-type and stress-type update as two launches */
 
 typedef struct wv_solver wv_solver;
 
@@ -66,7 +68,10 @@ typedef struct {
     double vmax_cpml;      /* VMaxCPML                                             */
     int nshots;            /* shots simulated together (>= 1)                      */
     int path;              /* WV_PATH_FUSED or WV_PATH_PLAIN                      */
-    int steps_per_graph;   /* time steps per captured hipGraph segment (0 = none) */
+    int steps_per_graph;   /* time steps per captured hipGraph segment (0 = none; rounded up to even) */
+    int tile_y;            /* rows per tile (0 = default 16): single-pass elastic kernel 8, 16, 24,
+                              negative selects direct global->LDS loads for the tile;
+                              two-launch kernels 8, 16, 32 */
 } wv_config;
 
 /* dimension "2D"/"3D", type "acoustic"/"elastic"/"tmem" (case-insensitive) */
@@ -102,11 +107,22 @@ int wv_get_seismograms(wv_solver *s, int shot, float *out);
 int wv_get_wavefield(wv_solver *s, int shot, int field, float *out);
 int wv_set_wavefield(wv_solver *s, int shot, int field, const float *in);
 
+/* Autotuner: times `nsteps` steps of every kernel variant that can run this
+ * configuration (the two-launch kernels with 8-, 16- and 32-row tiles; for 2D
+ * elastic on WV_PATH_FUSED also the single-pass kernel with 8-, 16- and 24-row
+ * tiles) on the handle's own model and acquisition, keeps the fastest and
+ * resets the wavefields and seismograms.  *choice receives the index of the
+ * variant kept (0..2 = two-launch with 8/16/32 rows, 3..5 = single pass with
+ * 8/16/24 rows, -1 = the handle has a single variant: WV_PATH_PLAIN).
+ * No counterpart in the reference. */
+int wv_tune(wv_solver *s, int nsteps, int *choice);
+
 /* elapsed GPU time [ms] of the last wv_run (hipEvents on the handle's stream) */
 int wv_last_run_ms(wv_solver *s, double *ms);
 
 /* Per-kernel timing.  With wv_set_kernel_timing(s, 1), wv_run launches every
- * kernel eagerly between hipEvents on the handle's stream and accumulates the
+ * kernel eagerly with start/stop hipEvents attached to the dispatch itself
+ * (hipExtLaunchKernelGGL) on the handle's stream and accumulates the
  * elapsed time per kernel kind; wv_kernel_stats then reports, for kind
  * 0 = velocity-type phase, 1 = stress-type phase, 2 = end-of-segment source/receiver flush:
  * the kernel name, its average duration [ms] per launch, the number of

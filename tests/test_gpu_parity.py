@@ -163,3 +163,50 @@ def test_invalid_arguments_raise():
         sol.set_acquisition(0, [40 * 40], [3], np.zeros((1, 5)), [1], [3])  # source outside the grid
     with pytest.raises(wavesim.WaveError):
         sol.set_acquisition(0, [10], [4], np.zeros((1, 5)), [1], [3])  # VZ in 2D
+
+
+@pytest.mark.parametrize("path,tile_y", [("fused", 8), ("fused", 16), ("fused", 24), ("phased", 0), ("phased", 8),
+                                         ("phased", 32)])
+def test_elastic_kernel_variants(path, tile_y):
+    """Single-pass kernel at every tile height and the two-launch kernels, CPML + VY and P sources."""
+    c = cases.elastic_c2(N=150, NT=600, W=16)
+    sh = c.shots[0]
+    sh.src_index = np.array([75 + 75 * 150, 40 + 100 * 150])
+    sh.src_type = np.array([3, 1])
+    sh.src_signal = np.concatenate([sh.src_signal, 0.5 * sh.src_signal])
+    sh.rec_type = sh.rec_type.copy()
+    sh.rec_type[::5] = 1
+    _check_case(c, path=path, tile_y=tile_y)
+
+
+@pytest.mark.parametrize("order", [2, 4, 12])
+def test_single_pass_orders_free_surface(order):
+    c = cases.elastic_c2(N=90, NT=400, W=12)
+    c.settings.update(spatial_fd_order=order, DT=5e-5, free_surface=1)
+    sh = c.shots[0]
+    sh.src_index = np.array([45 + 2 * 90])
+    sh.src_type = np.array([2])
+    sh.src_signal = wavesim.ricker(400, 5e-5, 30.0, 1.0)[None]
+    _check_case(c, tile_y=8)
+
+
+@pytest.mark.parametrize("tile_y", [8, 32])
+def test_two_launch_tile_heights_acoustic_tmem(tile_y):
+    _check_case(cases.acoustic_c1(N=100, NT=500), tile_y=tile_y)
+    _check_case(cases.tmem_c3(NX=150, NY=90, NT=500, nshots=2, block=10, W=10), tile_y=tile_y)
+
+
+def test_single_pass_batched_shots_with_abs():
+    c = cases.elastic_c4(N=110, NT=350, nshots=3, W=12)
+    c.settings.update(damping_boundary=1, damping_coeff=8.0)
+    _check_case(c)
+
+
+def test_autotuner_keeps_parity():
+    c = cases.elastic_c2(N=130, NT=500, W=14)
+    sol = cases.make_solver(c)
+    choice = sol.tune(20)
+    assert choice in sol.TUNE_CHOICES
+    sol.run(0, 500)
+    o32, _ = oracle_shot(c, c.shots[0], "f32")
+    assert rel_l2(sol.seismograms(0), o32) <= TOL

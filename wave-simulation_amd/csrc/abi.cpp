@@ -26,11 +26,11 @@ int fail(int code, const std::string &msg)
     return code;
 }
 
-#define HIP_TRY(expr)                                                                            \
-    do {                                                                                         \
-        hipError_t e_ = (expr);                                                                  \
-        if (e_ != hipSuccess)                                                                    \
-            return fail(WV_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));          \
+#define HIP_TRY(expr)                                                                   \
+    do {                                                                                \
+        hipError_t e_ = (expr);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return fail(WV_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
 std::string lower(const char *s)
@@ -42,6 +42,11 @@ std::string lower(const char *s)
 
 int round_up(int v, int m) { return (v + m - 1) / m * m; }
 
+// Position-indexed tables (taps, CPML coefficients, frame indices) are stored
+// for positions [-ORG, n_padded + ORG) so kernels may index halo positions
+// outside the grid; the pointer handed to a kernel points at position 0.
+constexpr int ORG = 64;
+
 struct ShotAcq {
     std::vector<long long> sidx;
     std::vector<int> stype;
@@ -49,6 +54,8 @@ struct ShotAcq {
     std::vector<long long> ridx;
     std::vector<int> rtype;
 };
+
+enum Variant { V_SINGLE = 0, V_PHASED = 1 };
 
 }  // namespace
 
@@ -62,19 +69,24 @@ struct wv_solver {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool ran = false;
 
+    int variant = V_PHASED;
+    int tile_x = wv::TILE_X, tile_y = wv::TILE_Y;  // tiling of the kernels in use
+    int tile_load = 1;  // single-pass kernel: -1 direct global->LDS loads, 1 register-staged loads
     wv::Layout2D L{};
     int H = 0;
     int nfields = 0;
+    int cur = 0;  // buffer holding the current state (single-pass kernel ping-pongs)
     std::vector<void *> allocs;
 
     // device buffers
-    float *field[9] = {};
+    float *field[2][5] = {};
     float *coef[5] = {};
     float *damp = nullptr, *fsH = nullptr, *fsV = nullptr;
-    float *tap[6] = {};  // xf xb yf yb yf_v yb_v
+    float *tap[6] = {};  // position-0 pointers: xf xb yf yb yf_v yb_v
     float *cpx[4] = {}, *cpy[4] = {};
     int *xi = nullptr, *yi = nullptr;
-    float *psx[4] = {}, *psy[4] = {};
+    float *psv[2][4] = {};  // ping-pong: velocity-derivative memory variables psx0 psx1 psy0 psy1
+    float *pss[4] = {};     // stress-derivative memory variables psx2 psx3 psy2 psy3
     size_t psx_n = 0, psy_n = 0;
 
     // acquisition
@@ -83,6 +95,7 @@ struct wv_solver {
     std::vector<void *> acq_allocs;
     int *d_src_ptr = nullptr, *d_src_type = nullptr, *d_rec_ptr = nullptr, *d_rec_type = nullptr;
     int *d_bsrc_ptr = nullptr, *d_bsrc = nullptr, *d_brec_ptr = nullptr, *d_brec = nullptr;
+    int *d_vsrc_ptr = nullptr, *d_vsrc = nullptr;
     int nbx = 0, nby = 0;
     long long *d_src_cell = nullptr, *d_rec_cell = nullptr;
     float *d_src_sig = nullptr, *d_seis = nullptr;
@@ -90,7 +103,7 @@ struct wv_solver {
     std::vector<int> rec_ptr_h;
     int *d_tbase = nullptr;
 
-    // graph of steps_per_graph time steps
+    // graph of steps_per_graph time steps (starts and ends in buffer 0)
     hipGraphExec_t gexec = nullptr;
     int gsteps = 0;
 
@@ -99,7 +112,7 @@ struct wv_solver {
 
     // per-kernel timing (wv_set_kernel_timing)
     bool ktiming = false;
-    std::vector<hipEvent_t> kev;  // 4 events per step of a batch
+    std::vector<hipEvent_t> kev;
     double kms[3] = {0, 0, 0};
     long long klaunch[3] = {0, 0, 0};
 
@@ -124,6 +137,18 @@ struct wv_solver {
             return fail(WV_EDEVICE, "hipMemcpy failed");
         return 0;
     }
+    // upload a position-indexed table with ORG positions of `fill` on both sides
+    template <typename T>
+    int upload_pos(T **p0, const std::vector<T> &h, int per_pos, T fill)
+    {
+        std::vector<T> ext((size_t)(h.size() / per_pos + 2 * ORG) * per_pos, fill);
+        std::copy(h.begin(), h.end(), ext.begin() + (size_t)ORG * per_pos);
+        T *base = nullptr;
+        int rc = upload(&base, ext, allocs);
+        if (rc) return rc;
+        *p0 = base + (size_t)ORG * per_pos;
+        return 0;
+    }
     void free_list(std::vector<void *> &list)
     {
         for (void *p : list) (void)hipFree(p);
@@ -135,10 +160,10 @@ struct wv_solver {
         gexec = nullptr;
         gsteps = 0;
     }
-    wv::Fields2D fields2d() const
+    wv::Fields2D fields2d(int b) const
     {
         wv::Fields2D F{};
-        for (int k = 0; k < 5; k++) F.f[k] = field[k];
+        for (int k = 0; k < 5; k++) F.f[k] = field[b][k];
         return F;
     }
     wv::Coef2D coef2d() const
@@ -162,7 +187,7 @@ struct wv_solver {
         T.yb_v = tap[5];
         return T;
     }
-    wv::Cpml2D cpml2d() const
+    wv::Cpml2D cpml2d(int b) const
     {
         wv::Cpml2D P{};
         P.on = cfg.damping_boundary == 2;
@@ -177,11 +202,24 @@ struct wv_solver {
         P.byh = cpy[3];
         P.xi = xi;
         P.yi = yi;
-        for (int k = 0; k < 4; k++) {
-            P.psx[k] = psx[k];
-            P.psy[k] = psy[k];
-        }
+        P.psx[0] = psv[b][0];
+        P.psx[1] = psv[b][1];
+        P.psy[0] = psv[b][2];
+        P.psy[1] = psv[b][3];
+        P.psx[2] = pss[0];
+        P.psx[3] = pss[1];
+        P.psy[2] = pss[2];
+        P.psy[3] = pss[3];
         return P;
+    }
+    wv::PsiNew2D psinew(int b) const
+    {
+        wv::PsiNew2D N{};
+        N.psx[0] = psv[b][0];
+        N.psx[1] = psv[b][1];
+        N.psy[0] = psv[b][2];
+        N.psy[1] = psv[b][3];
+        return N;
     }
     wv::Acq acq() const
     {
@@ -204,10 +242,11 @@ struct wv_solver {
         A.bsrc = d_bsrc;
         A.brec_ptr = d_brec_ptr;
         A.brec = d_brec;
+        A.vsrc_ptr = d_vsrc_ptr;
+        A.vsrc = d_vsrc;
         return A;
     }
-    int enqueue_steps(int nsteps);
-    int enqueue_steps_timed(int nsteps);
+    int enqueue_steps(int nsteps, bool timed);
     int sync_acquisition();
     double kernel_bytes(int kind) const;
 };
@@ -237,50 +276,64 @@ int wv_solver::sync_acquisition()
     for (auto &s : shots) {
         for (size_t k = 0; k < s.sidx.size(); k++) {
             const long long g = s.sidx[k];
-            const int x = (int)(g % L.NX), y = (int)(g / L.NX);
-            sc.push_back(L.at(x, y));
+            sc.push_back(L.at((int)(g % L.NX), (int)(g / L.NX)));
             stp.push_back(s.stype[k]);
         }
         sig.insert(sig.end(), s.ssig.begin(), s.ssig.end());
         sp.push_back((int)sc.size());
         for (size_t k = 0; k < s.ridx.size(); k++) {
             const long long g = s.ridx[k];
-            const int x = (int)(g % L.NX), y = (int)(g / L.NX);
-            rc.push_back(L.at(x, y));
+            rc.push_back(L.at((int)(g % L.NX), (int)(g / L.NX)));
             rtp.push_back(s.rtype[k]);
         }
         rp.push_back((int)rc.size());
     }
     rec_total = (int)rc.size();
     rec_ptr_h = rp;
-    // per (shot, block) lists of the sources / receivers a tile owns, in list order
-    nbx = (L.NX + wv::TILE_X - 1) / wv::TILE_X;
-    nby = (L.NY + wv::TILE_Y - 1) / wv::TILE_Y;
+    // per (shot, tile) lists of the sources / receivers each tile owns, in
+    // list order; for the single-pass kernel also the velocity-type sources
+    // inside each tile's velocity region (tile + H halo)
+    nbx = (L.NX + tile_x - 1) / tile_x;
+    nby = (L.NY + tile_y - 1) / tile_y;
     const size_t nsb = (size_t)cfg.nshots * nbx * nby;
-    std::vector<std::vector<int>> bs(nsb), br(nsb);
+    std::vector<std::vector<int>> bs(nsb), br(nsb), bv(nsb);
     {
         int gs = 0, gr = 0;
         for (int sh = 0; sh < (int)shots.size(); sh++) {
-            auto blk = [&](long long g) {
+            auto blk = [&](int bxx, int byy) { return ((size_t)sh * nby + byy) * nbx + bxx; };
+            for (size_t k = 0; k < shots[sh].sidx.size(); k++, gs++) {
+                const long long g = shots[sh].sidx[k];
                 const int x = (int)(g % L.NX), y = (int)(g / L.NX);
-                return ((size_t)sh * nby + y / wv::TILE_Y) * nbx + x / wv::TILE_X;
-            };
-            for (long long g : shots[sh].sidx) bs[blk(g)].push_back(gs++);
-            for (long long g : shots[sh].ridx) br[blk(g)].push_back(gr++);
+                bs[blk(x / tile_x, y / tile_y)].push_back(gs);
+                if (variant == V_SINGLE && (shots[sh].stype[k] == 2 || shots[sh].stype[k] == 3)) {
+                    for (int byy = 0; byy < nby; byy++)
+                        for (int bxx = 0; bxx < nbx; bxx++) {
+                            const int xlo = bxx * tile_x - H, xhi = bxx * tile_x + tile_x + H;
+                            const int ylo = byy * tile_y - H, yhi = byy * tile_y + tile_y + H;
+                            if (x >= xlo && x < xhi && y >= ylo && y < yhi) bv[blk(bxx, byy)].push_back(gs);
+                        }
+                }
+            }
+            for (long long g : shots[sh].ridx) {
+                const int x = (int)(g % L.NX), y = (int)(g / L.NX);
+                br[blk(x / tile_x, y / tile_y)].push_back(gr++);
+            }
         }
     }
-    std::vector<int> bsp(1, 0), bsl, brp(1, 0), brl;
+    std::vector<int> bsp(1, 0), bsl, brp(1, 0), brl, bvp(1, 0), bvl;
     for (size_t b = 0; b < nsb; b++) {
         bsl.insert(bsl.end(), bs[b].begin(), bs[b].end());
         bsp.push_back((int)bsl.size());
         brl.insert(brl.end(), br[b].begin(), br[b].end());
         brp.push_back((int)brl.size());
+        bvl.insert(bvl.end(), bv[b].begin(), bv[b].end());
+        bvp.push_back((int)bvl.size());
     }
     int r;
     if ((r = upload(&d_bsrc_ptr, bsp, acq_allocs)) || (r = upload(&d_bsrc, bsl, acq_allocs)) ||
-        (r = upload(&d_brec_ptr, brp, acq_allocs)) || (r = upload(&d_brec, brl, acq_allocs)))
-        return r;
-    if ((r = upload(&d_src_ptr, sp, acq_allocs)) || (r = upload(&d_rec_ptr, rp, acq_allocs)) ||
+        (r = upload(&d_brec_ptr, brp, acq_allocs)) || (r = upload(&d_brec, brl, acq_allocs)) ||
+        (r = upload(&d_vsrc_ptr, bvp, acq_allocs)) || (r = upload(&d_vsrc, bvl, acq_allocs)) ||
+        (r = upload(&d_src_ptr, sp, acq_allocs)) || (r = upload(&d_rec_ptr, rp, acq_allocs)) ||
         (r = upload(&d_src_cell, sc, acq_allocs)) || (r = upload(&d_rec_cell, rc, acq_allocs)) ||
         (r = upload(&d_src_type, stp, acq_allocs)) || (r = upload(&d_rec_type, rtp, acq_allocs)) ||
         (r = upload(&d_src_sig, sig, acq_allocs)) ||
@@ -291,63 +344,74 @@ int wv_solver::sync_acquisition()
     return 0;
 }
 
-// Enqueue `nsteps` time steps starting at the device step counter.
-int wv_solver::enqueue_steps(int nsteps)
+// Enqueue `nsteps` time steps starting at the device step counter: one run
+// segment (velocity-type sources of its last step are applied by the final
+// flush, so the fields in memory are the reference state afterwards).  With
+// `timed`, every kernel is bracketed by hipEvents and its time accumulated.
+int wv_solver::enqueue_steps(int nsteps, bool timed)
 {
-    const wv::Fields2D F = fields2d();
     const wv::Coef2D C = coef2d();
     const wv::Taps T = taps();
-    const wv::Cpml2D P = cpml2d();
     const wv::Acq A = acq();
-    for (int j = 0; j < nsteps; j++) {
-        wv::launch_step2d(physics, cfg.spatial_fd_order, L, cfg.nshots, F, C, T, P, A, j, j > 0, st, 0);
-        wv::launch_step2d(physics, cfg.spatial_fd_order, L, cfg.nshots, F, C, T, P, A, j, 0, st, 1);
-        if (cfg.damping_boundary == 1) wv::launch_abs_velocity2d(physics, L, cfg.nshots, F, C, st);
-    }
-    if (nsteps > 0) wv::launch_flush2d(physics, L, F, A, nsteps - 1, st);
-    wv::launch_advance(d_tbase, nsteps, st);
-    HIP_TRY(hipGetLastError());
-    return 0;
-}
-
-// Same launches as enqueue_steps, each kernel bracketed by hipEvents on the
-// handle's stream; elapsed times are accumulated per kernel kind.
-int wv_solver::enqueue_steps_timed(int nsteps)
-{
-    const wv::Fields2D F = fields2d();
-    const wv::Coef2D C = coef2d();
-    const wv::Taps T = taps();
-    const wv::Cpml2D P = cpml2d();
-    const wv::Acq A = acq();
+    const int order = cfg.spatial_fd_order;
     constexpr int BATCH = 32;
-    if (kev.empty()) {
-        kev.resize(4 * BATCH);
+    // timed: every kernel is launched with its own start/stop events (the
+    // events are written by the dispatch itself, not by separate commands)
+    if (timed && kev.empty()) {
+        kev.resize(4 * BATCH + 2);
         for (auto &e : kev) HIP_TRY(hipEventCreate(&e));
     }
-    for (int j0 = 0; j0 < nsteps; j0 += BATCH) {
-        const int nb = std::min(BATCH, nsteps - j0);
-        for (int b = 0; b < nb; b++) {
-            const int j = j0 + b;
-            hipEvent_t *e = &kev[4 * b];
-            HIP_TRY(hipEventRecord(e[0], st));
-            wv::launch_step2d(physics, cfg.spatial_fd_order, L, cfg.nshots, F, C, T, P, A, j, j > 0, st, 0);
-            HIP_TRY(hipEventRecord(e[1], st));
-            wv::launch_step2d(physics, cfg.spatial_fd_order, L, cfg.nshots, F, C, T, P, A, j, 0, st, 1);
-            HIP_TRY(hipEventRecord(e[2], st));
-            if (cfg.damping_boundary == 1) wv::launch_abs_velocity2d(physics, L, cfg.nshots, F, C, st);
-            if (j == nsteps - 1) wv::launch_flush2d(physics, L, F, A, j, st);
-            HIP_TRY(hipEventRecord(e[3], st));
-        }
-        HIP_TRY(hipEventSynchronize(kev[4 * (nb - 1) + 3]));
-        for (int b = 0; b < nb; b++) {
-            for (int k = 0; k < 3; k++) {
-                if (k == 2 && j0 + b != nsteps - 1) continue;
+    const int nk = variant == V_SINGLE ? 1 : 2;
+    int pending = 0;
+    auto collect = [&](int n) -> int {
+        HIP_TRY(hipEventSynchronize(kev[4 * (n - 1) + 2 * nk - 1]));
+        for (int b = 0; b < n; b++)
+            for (int k = 0; k < nk; k++) {
                 float ms = 0;
-                HIP_TRY(hipEventElapsedTime(&ms, kev[4 * b + k], kev[4 * b + k + 1]));
+                HIP_TRY(hipEventElapsedTime(&ms, kev[4 * b + 2 * k], kev[4 * b + 2 * k + 1]));
                 kms[k] += ms;
                 klaunch[k] += 1;
             }
+        return 0;
+    };
+    for (int j = 0; j < nsteps; j++) {
+        hipEvent_t *e = timed ? &kev[4 * pending] : nullptr;
+        hipEvent_t e0 = e ? e[0] : nullptr, e1 = e ? e[1] : nullptr, e2 = e ? e[2] : nullptr, e3 = e ? e[3] : nullptr;
+        if (variant == V_SINGLE) {
+            wv::launch_fused2d(order, tile_y * tile_load, L, cfg.nshots, fields2d(cur), fields2d(cur ^ 1), C, T, cpml2d(cur),
+                               psinew(cur ^ 1), A, j, j > 0, st, e0, e1);
+            cur ^= 1;
+        } else {
+            const wv::Fields2D F = fields2d(cur);
+            const wv::Cpml2D P = cpml2d(cur);
+            wv::launch_step2d(physics, order, tile_y, L, cfg.nshots, F, C, T, P, A, j, j > 0, st, 0, e0, e1);
+            wv::launch_step2d(physics, order, tile_y, L, cfg.nshots, F, C, T, P, A, j, 0, st, 1, e2, e3);
+            if (cfg.damping_boundary == 1) wv::launch_abs_velocity2d(physics, L, cfg.nshots, F, C, st);
         }
+        if (timed && ++pending == BATCH) {
+            int rc = collect(pending);
+            if (rc) return rc;
+            pending = 0;
+        }
+    }
+    if (timed && pending) {
+        int rc = collect(pending);
+        if (rc) return rc;
+    }
+    if (timed) HIP_TRY(hipEventRecord(kev[4 * BATCH], st));
+    if (nsteps > 0) {
+        const wv::Fields2D F = fields2d(cur);
+        // the single-pass kernel defers the ABS damping of the velocities to the next read
+        if (variant == V_SINGLE && cfg.damping_boundary == 1) wv::launch_abs_velocity2d(physics, L, cfg.nshots, F, C, st);
+        wv::launch_flush2d(physics, L, F, A, nsteps - 1, st);
+    }
+    if (timed) {
+        HIP_TRY(hipEventRecord(kev[4 * BATCH + 1], st));
+        HIP_TRY(hipEventSynchronize(kev[4 * BATCH + 1]));
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, kev[4 * BATCH], kev[4 * BATCH + 1]));
+        kms[2] += ms;
+        klaunch[2] += 1;
     }
     wv::launch_advance(d_tbase, nsteps, st);
     HIP_TRY(hipGetLastError());
@@ -360,25 +424,27 @@ int wv_solver::enqueue_steps_timed(int nsteps)
 double wv_solver::kernel_bytes(int kind) const
 {
     const double cells = (double)cfg.NX * cfg.NY, B = cfg.nshots;
-    // {field reads+writes per shot-cell, coefficient reads per cell, psi per x-frame cell, psi per y-frame cell}
-    static const int tab[3][2][4] = {
-        {{5, 2, 1, 1}, {4, 1, 1, 1}},   // acoustic: velocity, pressure
-        {{7, 2, 2, 2}, {8, 3, 2, 2}},   // elastic: velocity, stress
-        {{5, 2, 1, 1}, {4, 2, 1, 1}},   // tmem: H, E
-    };
     if (kind == 2) {
         double nsrc = 0;
         for (auto &sh : shots) nsrc += sh.sidx.size();
         return 12.0 * nsrc + 8.0 * rec_total;
     }
-    const int *t = tab[physics][kind];
+    // {field reads+writes per shot-cell, coefficient reads per cell, psi per x-frame cell, psi per y-frame cell}
+    static const int phased[3][2][4] = {
+        {{5, 2, 1, 1}, {4, 1, 1, 1}},  // acoustic: velocity, pressure
+        {{7, 2, 2, 2}, {8, 3, 2, 2}},  // elastic: velocity, stress
+        {{5, 2, 1, 1}, {4, 2, 1, 1}},  // tmem: H, E
+    };
+    static const int single[4] = {10, 5, 4, 4};  // elastic single pass
+    if (variant == V_SINGLE && kind == 1) return 0.0;
+    const int *t = variant == V_SINGLE ? single : phased[physics][kind];
     double bytes = 4.0 * (cells * B * t[0] + cells * t[1]);
     if (cfg.damping_boundary == 2) {
         const double W = cfg.boundary_width;
         const double xf = (double)cfg.NY * 2 * W, yf = (double)cfg.NX * (cfg.free_surface ? W : 2 * W);
         bytes += 8.0 * B * (xf * t[2] + yf * t[3]);
     }
-    if (cfg.damping_boundary == 1 && kind == 1) bytes += 4.0 * cells;
+    if (cfg.damping_boundary == 1 && (kind == 1 || variant == V_SINGLE)) bytes += 4.0 * cells;
     return bytes;
 }
 
@@ -438,12 +504,19 @@ int wv_init(wv_solver *s, const wv_config *c)
     if (k.damping_boundary == 1 && !(k.damping_coeff > 0 && k.damping_coeff < 100))
         return fail(WV_EINVAL, "DampingCoeff must be in (0,100)");
     if (k.nshots < 1) return fail(WV_EINVAL, "nshots must be >= 1");
-    if (k.path != WV_PATH_FUSED && k.path != WV_PATH_PLAIN) return fail(WV_EINVAL, "bad path");
+    if (k.path != WV_PATH_FUSED && k.path != WV_PATH_PLAIN && k.path != WV_PATH_PHASED)
+        return fail(WV_EINVAL, "bad path");
     if (k.path == WV_PATH_PLAIN && k.nshots != 1) return fail(WV_EINVAL, "the plain path runs one shot per handle");
+    const bool single = k.path == WV_PATH_FUSED && s->physics == wv::ELASTIC;
+    if (single && k.tile_y != 0 && std::abs(k.tile_y) != 8 && std::abs(k.tile_y) != 16 && std::abs(k.tile_y) != 24)
+        return fail(WV_EINVAL, "tile_y of the single-pass kernel must be 0 (default), +-8, +-16 or +-24");
+    if (!single && k.tile_y != 0 && k.tile_y != 8 && k.tile_y != 16 && k.tile_y != 32)
+        return fail(WV_EINVAL, "tile_y of the two-launch kernels must be 0 (default), 8, 16 or 32");
     if (s->inited) return fail(WV_ESTATE, "solver already initialised");
 
     s->cfg = k;
     if (s->cfg.steps_per_graph < 0) s->cfg.steps_per_graph = 0;
+    s->cfg.steps_per_graph += s->cfg.steps_per_graph & 1;  // graphs start and end in buffer 0
     HIP_TRY(hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking));
     HIP_TRY(hipEventCreate(&s->ev0));
     HIP_TRY(hipEventCreate(&s->ev1));
@@ -457,23 +530,40 @@ int wv_init(wv_solver *s, const wv_config *c)
         return WV_OK;
     }
 
-    wv::Layout2D &L = s->L;
+    s->variant = single ? V_SINGLE : V_PHASED;
     s->H = N / 2;
+    if (s->variant == V_SINGLE) {
+        s->tile_x = wv::fused2d_tile_x(N);
+        s->tile_y = k.tile_y ? std::abs(k.tile_y) : 16;
+        s->tile_load = k.tile_y < 0 ? -1 : 1;
+    } else {
+        s->tile_x = wv::TILE_X;
+        s->tile_y = k.tile_y ? k.tile_y : wv::TILE_Y;
+    }
+    wv::Layout2D &L = s->L;
     L.NX = k.NX;
     L.NY = k.NY;
     L.H = s->H;
-    L.ypad = s->H;
-    L.pitch = wv::XPAD + round_up(k.NX, wv::TILE_X) + wv::XPAD;
-    L.rows = s->H + round_up(k.NY, wv::TILE_Y) + s->H;
+    L.ypad = 2 * s->H;
+    L.pitch = wv::XPAD + round_up(k.NX + 64, 64) + wv::XPAD;
+    L.rows = L.ypad + round_up(k.NY, 32) + 32 + 2 * s->H;
     L.plane = (long long)L.pitch * L.rows;
     s->nfields = s->physics == wv::ELASTIC ? 5 : 3;
+    // kernels index inside one field plane and one shot's CPML frame with 32-bit offsets
+    if (L.plane >= (1LL << 31) || (long long)k.NY * k.NX >= (1LL << 30))
+        return fail(WV_EINVAL, "grid too large for one field plane");
 
     int rc;
-    for (int f = 0; f < s->nfields; f++)
-        if ((rc = s->dalloc(&s->field[f], (size_t)L.plane * k.nshots, s->allocs))) return rc;
+    // elastic keeps two field buffers so the tuner may switch between the
+    // single-pass (ping-pong) and the two-launch kernels
+    const int nbuf = s->physics == wv::ELASTIC ? 2 : 1;
+    for (int b = 0; b < nbuf; b++)
+        for (int f = 0; f < s->nfields; f++)
+            if ((rc = s->dalloc(&s->field[b][f], (size_t)L.plane * k.nshots, s->allocs))) return rc;
 
-    // derivative taps
-    const int NXp = round_up(k.NX, wv::TILE_X), NYp = round_up(k.NY, wv::TILE_Y);
+    // derivative taps, positions [0, NXp) / [0, NYp) plus ORG on each side
+    const int NXp = round_up(k.NX, 64) + 64, NYp = round_up(k.NY, 32) + 32;
+    const int KT = 2 * s->H + 1;
     const bool stencil = k.ops_mode == WV_OPS_STENCIL;
     const float DH = (float)k.DH, DT = (float)k.DT;
     std::vector<float> t_xf, t_xb, t_yf, t_yb, t_yfv, t_ybv;
@@ -492,10 +582,9 @@ int wv_init(wv_solver *s, const wv_config *c)
         t_yfv = t_yf;
         t_ybv = t_yb;
     }
-    if ((rc = s->upload(&s->tap[0], t_xf, s->allocs)) || (rc = s->upload(&s->tap[1], t_xb, s->allocs)) ||
-        (rc = s->upload(&s->tap[2], t_yf, s->allocs)) || (rc = s->upload(&s->tap[3], t_yb, s->allocs)) ||
-        (rc = s->upload(&s->tap[4], t_yfv, s->allocs)) || (rc = s->upload(&s->tap[5], t_ybv, s->allocs)))
-        return rc;
+    const std::vector<float> *tt[6] = {&t_xf, &t_xb, &t_yf, &t_yb, &t_yfv, &t_ybv};
+    for (int q = 0; q < 6; q++)
+        if ((rc = s->upload_pos(&s->tap[q], *tt[q], KT, 0.f))) return rc;
 
     // CPML
     if (k.damping_boundary == 2) {
@@ -503,22 +592,36 @@ int wv_init(wv_solver *s, const wv_config *c)
         std::vector<float> a, b, ah, bh;
         std::vector<int> idx;
         wv::cpml_axis(k.NX, NXp, W, false, k.npower, k.fc_cpml, k.vmax_cpml, k.DT, k.DH, a, b, ah, bh, idx);
-        if ((rc = s->upload(&s->cpx[0], a, s->allocs)) || (rc = s->upload(&s->cpx[1], b, s->allocs)) ||
-            (rc = s->upload(&s->cpx[2], ah, s->allocs)) || (rc = s->upload(&s->cpx[3], bh, s->allocs)) ||
-            (rc = s->upload(&s->xi, idx, s->allocs)))
+        if ((rc = s->upload_pos(&s->cpx[0], a, 1, 0.f)) || (rc = s->upload_pos(&s->cpx[1], b, 1, 0.f)) ||
+            (rc = s->upload_pos(&s->cpx[2], ah, 1, 0.f)) || (rc = s->upload_pos(&s->cpx[3], bh, 1, 0.f)) ||
+            (rc = s->upload_pos(&s->xi, idx, 1, -1)))
             return rc;
         wv::cpml_axis(k.NY, NYp, W, k.free_surface != 0, k.npower, k.fc_cpml, k.vmax_cpml, k.DT, k.DH, a, b, ah, bh,
                       idx);
-        if ((rc = s->upload(&s->cpy[0], a, s->allocs)) || (rc = s->upload(&s->cpy[1], b, s->allocs)) ||
-            (rc = s->upload(&s->cpy[2], ah, s->allocs)) || (rc = s->upload(&s->cpy[3], bh, s->allocs)) ||
-            (rc = s->upload(&s->yi, idx, s->allocs)))
+        if ((rc = s->upload_pos(&s->cpy[0], a, 1, 0.f)) || (rc = s->upload_pos(&s->cpy[1], b, 1, 0.f)) ||
+            (rc = s->upload_pos(&s->cpy[2], ah, 1, 0.f)) || (rc = s->upload_pos(&s->cpy[3], bh, 1, 0.f)) ||
+            (rc = s->upload_pos(&s->yi, idx, 1, -1)))
             return rc;
         s->psx_n = (size_t)k.nshots * k.NY * 2 * W;
         s->psy_n = (size_t)k.nshots * 2 * W * k.NX;
-        const int npsi = s->physics == wv::ELASTIC ? 4 : 2;
-        for (int q = 0; q < npsi; q++)
-            if ((rc = s->dalloc(&s->psx[q], s->psx_n, s->allocs)) || (rc = s->dalloc(&s->psy[q], s->psy_n, s->allocs)))
+        // memory variables of the velocity-type update (ping-ponged by the single-pass kernel)
+        // and of the stress-type update; acoustic/tmem need one of each per axis
+        const int nv = s->physics == wv::ELASTIC ? 2 : 1;
+        for (int b2 = 0; b2 < nbuf; b2++)
+            for (int q = 0; q < nv; q++)
+                if ((rc = s->dalloc(&s->psv[b2][q], s->psx_n, s->allocs)) ||
+                    (rc = s->dalloc(&s->psv[b2][2 + q], s->psy_n, s->allocs)))
+                    return rc;
+        for (int q = 0; q < nv; q++)
+            if ((rc = s->dalloc(&s->pss[q], s->psx_n, s->allocs)) ||
+                (rc = s->dalloc(&s->pss[2 + q], s->psy_n, s->allocs)))
                 return rc;
+        if (s->physics != wv::ELASTIC) {
+            // the phased acoustic/tmem kernels use psx[0]/psy[0] for the velocity-type
+            // update and psx[1]/psy[1] for the stress-type update
+            s->psv[0][1] = s->pss[0];
+            s->psv[0][3] = s->pss[2];
+        }
     }
 
     // ABS
@@ -643,7 +746,7 @@ int wv_set_model(wv_solver *s, const float *m0, const float *m1, const float *m2
     }
     if (s->physics == wv::ELASTIC && s->cfg.free_surface == 1) {
         // FreeSurfaceElastic.cpp:11-47: scales of the horizontal/vertical updates at y = 0
-        const int NXp = round_up(NX, wv::TILE_X);
+        const int NXp = round_up(NX, 64) + 64;
         std::vector<float> h((size_t)NXp, 0.f), v((size_t)NXp, 0.f);
         for (int x = 0; x < NX; x++) {
             const double Mx = M[gi(x, 0)], t = Mx - 2.0 * mu[gi(x, 0)];
@@ -701,11 +804,14 @@ int wv_reset(wv_solver *s)
         return rc ? fail(rc, s->plain->error) : WV_OK;
     }
     const size_t fn = (size_t)s->L.plane * s->cfg.nshots;
-    for (int f = 0; f < s->nfields; f++) HIP_TRY(hipMemsetAsync(s->field[f], 0, fn * sizeof(float), s->st));
-    for (int q = 0; q < 4; q++) {
-        if (s->psx[q]) HIP_TRY(hipMemsetAsync(s->psx[q], 0, s->psx_n * sizeof(float), s->st));
-        if (s->psy[q]) HIP_TRY(hipMemsetAsync(s->psy[q], 0, s->psy_n * sizeof(float), s->st));
-    }
+    for (int b = 0; b < 2; b++)
+        for (int f = 0; f < s->nfields; f++)
+            if (s->field[b][f]) HIP_TRY(hipMemsetAsync(s->field[b][f], 0, fn * sizeof(float), s->st));
+    for (int b = 0; b < 2; b++)
+        for (int q = 0; q < 4; q++)
+            if (s->psv[b][q]) HIP_TRY(hipMemsetAsync(s->psv[b][q], 0, (q < 2 ? s->psx_n : s->psy_n) * 4, s->st));
+    for (int q = 0; q < 4; q++)
+        if (s->pss[q]) HIP_TRY(hipMemsetAsync(s->pss[q], 0, (q < 2 ? s->psx_n : s->psy_n) * 4, s->st));
     if (s->d_seis) HIP_TRY(hipMemsetAsync(s->d_seis, 0, (size_t)s->cfg.NT * std::max(s->rec_total, 1) * 4, s->st));
     return WV_OK;
 }
@@ -723,28 +829,26 @@ int wv_run(wv_solver *s, int t_start, int t_end)
         s->ran = true;
         return WV_OK;
     }
-    const bool fresh = s->acq_dirty;
     int rc = s->sync_acquisition();
     if (rc) return rc;
-    if (fresh) {
-        // seismogram buffer was reallocated (zeroed)
-    }
     HIP_TRY(hipEventRecord(s->ev0, s->st));
-    wv::launch_set(s->d_tbase, t_start, s->st);
-    int remaining = t_end - t_start;
+    int t = t_start;
+    auto eager = [&](int n) -> int {
+        wv::launch_set(s->d_tbase, t, s->st);
+        int r2 = s->enqueue_steps(n, s->ktiming);
+        t += n;
+        return r2;
+    };
     const int K = s->ktiming ? 0 : s->cfg.steps_per_graph;
-    if (s->ktiming) {
-        rc = s->enqueue_steps_timed(remaining);
-        if (rc) return rc;
-        remaining = 0;
-    }
-    if (K > 0 && remaining >= K) {
+    if (K > 0 && t_end - t >= K + s->cur) {
+        if (s->cur && (rc = eager(1))) return rc;  // graphs start in buffer 0
         if (!s->gexec || s->gsteps != K) {
             s->drop_graph();
             hipGraph_t g;
             HIP_TRY(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
-            rc = s->enqueue_steps(K);
+            rc = s->enqueue_steps(K, false);
             hipError_t e = hipStreamEndCapture(s->st, &g);
+            s->cur = 0;
             if (rc) return rc;
             if (e != hipSuccess) return fail(WV_EDEVICE, std::string("graph capture: ") + hipGetErrorString(e));
             e = hipGraphInstantiate(&s->gexec, g, nullptr, nullptr, 0);
@@ -752,15 +856,14 @@ int wv_run(wv_solver *s, int t_start, int t_end)
             if (e != hipSuccess) return fail(WV_EDEVICE, std::string("graph instantiate: ") + hipGetErrorString(e));
             s->gsteps = K;
         }
-        while (remaining >= K) {
+        // the graph advances the device step counter itself
+        wv::launch_set(s->d_tbase, t, s->st);
+        while (t_end - t >= K) {
             HIP_TRY(hipGraphLaunch(s->gexec, s->st));
-            remaining -= K;
+            t += K;
         }
     }
-    if (remaining > 0) {
-        rc = s->enqueue_steps(remaining);
-        if (rc) return rc;
-    }
+    if (t_end > t && (rc = eager(t_end - t))) return rc;
     HIP_TRY(hipEventRecord(s->ev1, s->st));
     s->ran = true;
     return WV_OK;
@@ -818,7 +921,7 @@ static int field_io(wv_solver *s, int shot, int field, float *out, const float *
     if (field < 0 || field >= s->nfields) return fail(WV_EINVAL, "bad field index");
     const wv::Layout2D &L = s->L;
     std::vector<float> h((size_t)L.plane);
-    float *d = s->field[field] + (size_t)shot * L.plane;
+    float *d = s->field[s->cur][field] + (size_t)shot * L.plane;
     HIP_TRY(hipStreamSynchronize(s->st));
     if (out) {
         HIP_TRY(hipMemcpy(h.data(), d, h.size() * sizeof(float), hipMemcpyDeviceToHost));
@@ -845,10 +948,57 @@ int wv_set_wavefield(wv_solver *s, int shot, int field, const float *in)
     return field_io(s, shot, field, nullptr, in);
 }
 
+int wv_tune(wv_solver *s, int nsteps, int *choice)
+{
+    if (!s || !s->inited || !s->model_ok) return fail(WV_ESTATE, "wv_tune needs an initialised solver with a model");
+    if (nsteps < 2 || nsteps > s->cfg.NT) return fail(WV_EINVAL, "nsteps must be in [2, NT]");
+    if (choice) *choice = -1;
+    if (s->plain) return WV_OK;  // one candidate
+    // candidates: two-launch kernels with 8/16/32-row tiles; for elastic P-SV
+    // also the single-pass kernel with register-staged tiles of 8/16/24 rows
+    const int cand_var[6] = {V_PHASED, V_PHASED, V_PHASED, V_SINGLE, V_SINGLE, V_SINGLE};
+    const int cand_ty[6] = {8, 16, 32, 8, 16, 24};
+    const int ncand = (s->physics == wv::ELASTIC && s->cfg.path == WV_PATH_FUSED) ? 6 : 3;
+    float best = 1e30f;
+    int best_k = 0;
+    for (int k = 0; k < ncand; k++) {
+        s->variant = cand_var[k];
+        s->tile_x = s->variant == V_SINGLE ? wv::fused2d_tile_x(s->cfg.spatial_fd_order) : wv::TILE_X;
+        s->tile_y = cand_ty[k];
+        s->tile_load = 1;
+        s->acq_dirty = true;
+        s->cur = 0;
+        int rc = wv_reset(s);
+        if (rc) return rc;
+        if ((rc = s->sync_acquisition())) return rc;
+        wv::launch_set(s->d_tbase, 0, s->st);
+        if ((rc = s->enqueue_steps(2, false))) return rc;  // warm-up
+        wv::launch_set(s->d_tbase, 0, s->st);
+        HIP_TRY(hipEventRecord(s->ev0, s->st));
+        if ((rc = s->enqueue_steps(nsteps, false))) return rc;
+        HIP_TRY(hipEventRecord(s->ev1, s->st));
+        HIP_TRY(hipEventSynchronize(s->ev1));
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+        if (ms < best) {
+            best = ms;
+            best_k = k;
+        }
+    }
+    s->variant = cand_var[best_k];
+    s->tile_x = s->variant == V_SINGLE ? wv::fused2d_tile_x(s->cfg.spatial_fd_order) : wv::TILE_X;
+    s->tile_y = cand_ty[best_k];
+    s->acq_dirty = true;
+    s->cur = 0;
+    s->drop_graph();
+    if (choice) *choice = best_k;
+    return wv_reset(s);
+}
+
 int wv_set_kernel_timing(wv_solver *s, int on)
 {
     if (!s || !s->inited) return fail(WV_ESTATE, "not initialised");
-    if (s->plain) return fail(WV_EINVAL, "kernel timing is provided by the fused path");
+    if (s->plain) return fail(WV_EINVAL, "kernel timing is provided by the tiled kernels");
     s->ktiming = on != 0;
     for (int k = 0; k < 3; k++) {
         s->kms[k] = 0;
@@ -862,12 +1012,12 @@ int wv_kernel_stats(wv_solver *s, int kind, const char **name, double *avg_ms, l
 {
     if (!s || !s->inited) return fail(WV_ESTATE, "not initialised");
     if (kind < 0 || kind > 2) return fail(WV_EINVAL, "unknown kernel kind");
-    if (s->plain) return fail(WV_EINVAL, "kernel timing is provided by the fused path");
+    if (s->plain) return fail(WV_EINVAL, "kernel timing is provided by the tiled kernels");
     static const char *names[3][3] = {{"ac2d_velocity", "ac2d_pressure", "flush2d"},
                                       {"el2d_velocity", "el2d_stress", "flush2d"},
                                       {"tm2d_h", "tm2d_e", "flush2d"}};
     HIP_TRY(hipStreamSynchronize(s->st));
-    if (name) *name = names[s->physics][kind];
+    if (name) *name = (s->variant == V_SINGLE && kind < 2) ? (kind == 0 ? "el2d_fused" : "-") : names[s->physics][kind];
     if (launches) *launches = s->klaunch[kind];
     if (avg_ms) *avg_ms = s->klaunch[kind] ? s->kms[kind] / s->klaunch[kind] : 0.0;
     if (bytes_per_launch) *bytes_per_launch = s->kernel_bytes(kind);

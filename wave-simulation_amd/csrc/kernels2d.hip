@@ -682,42 +682,66 @@ __global__ void set_kernel(int *p, int v)
     if (threadIdx.x == 0 && blockIdx.x == 0) p[0] = v;
 }
 
+// launch `k`, bracketed by e0/e1 at the kernel's own start and end when given
+template <typename Kern, typename... Args>
+void go(Kern k, dim3 grid, dim3 block, hipStream_t st, hipEvent_t e0, hipEvent_t e1, Args... args)
+{
+    if (e0)
+        hipExtLaunchKernelGGL(k, grid, block, 0, st, e0, e1, 0, args...);
+    else
+        hipLaunchKernelGGL(k, grid, block, 0, st, args...);
+}
+
 template <int H, int TY>
 void launch_h(int physics, int which, dim3 grid, const Layout2D &L, const Fields2D &F, const Coef2D &C, const Taps &T,
-              const Cpml2D &P, const Acq &A, int tofs, int pend, hipStream_t st)
+              const Cpml2D &P, const Acq &A, int tofs, int pend, hipStream_t st, hipEvent_t e0, hipEvent_t e1)
 {
     dim3 block(TILE_X, BLOCK_Y);
     if (physics == ELASTIC) {
         if (which == 0)
-            el2d_velocity<H, TY><<<grid, block, 0, st>>>(L, F, C, T, P, A, tofs, pend);
+            go(el2d_velocity<H, TY>, grid, block, st, e0, e1, L, F, C, T, P, A, tofs, pend);
         else
-            el2d_stress<H, TY><<<grid, block, 0, st>>>(L, F, C, T, P, A, tofs, pend);
+            go(el2d_stress<H, TY>, grid, block, st, e0, e1, L, F, C, T, P, A, tofs, pend);
     } else if (physics == ACOUSTIC) {
         if (which == 0)
-            ac2d_velocity<H, TY><<<grid, block, 0, st>>>(L, F, C, T, P, A, tofs, pend);
+            go(ac2d_velocity<H, TY>, grid, block, st, e0, e1, L, F, C, T, P, A, tofs, pend);
         else
-            ac2d_pressure<H, TY><<<grid, block, 0, st>>>(L, F, C, T, P, A, tofs, pend);
+            go(ac2d_pressure<H, TY>, grid, block, st, e0, e1, L, F, C, T, P, A, tofs, pend);
     } else {
         if (which == 0)
-            tm2d_h<H, TY><<<grid, block, 0, st>>>(L, F, C, T, P, A, tofs, pend);
+            go(tm2d_h<H, TY>, grid, block, st, e0, e1, L, F, C, T, P, A, tofs, pend);
         else
-            tm2d_e<H, TY><<<grid, block, 0, st>>>(L, F, C, T, P, A, tofs, pend);
+            go(tm2d_e<H, TY>, grid, block, st, e0, e1, L, F, C, T, P, A, tofs, pend);
     }
+}
+
+template <int H>
+void launch_ty2(int ty, int physics, int which, const Layout2D &L, int nshots, const Fields2D &F, const Coef2D &C,
+                const Taps &T, const Cpml2D &P, const Acq &A, int tofs, int pend, hipStream_t st, hipEvent_t e0,
+                hipEvent_t e1)
+{
+    dim3 grid((L.NX + TILE_X - 1) / TILE_X, (L.NY + ty - 1) / ty, nshots);
+    if (ty == 8)
+        launch_h<H, 8>(physics, which, grid, L, F, C, T, P, A, tofs, pend, st, e0, e1);
+    else if (ty == 32)
+        launch_h<H, 32>(physics, which, grid, L, F, C, T, P, A, tofs, pend, st, e0, e1);
+    else
+        launch_h<H, 16>(physics, which, grid, L, F, C, T, P, A, tofs, pend, st, e0, e1);
 }
 
 }  // namespace
 
-void launch_step2d(int physics, int order, const Layout2D &L, int nshots, const Fields2D &F, const Coef2D &C,
-                   const Taps &T, const Cpml2D &P, const Acq &A, int tofs, int pend, hipStream_t stream, int which)
+void launch_step2d(int physics, int order, int tile_y, const Layout2D &L, int nshots, const Fields2D &F,
+                   const Coef2D &C, const Taps &T, const Cpml2D &P, const Acq &A, int tofs, int pend,
+                   hipStream_t stream, int which, hipEvent_t e0, hipEvent_t e1)
 {
-    dim3 grid((L.NX + TILE_X - 1) / TILE_X, (L.NY + TILE_Y - 1) / TILE_Y, nshots);
     switch (order / 2) {
-    case 1: launch_h<1, TILE_Y>(physics, which, grid, L, F, C, T, P, A, tofs, pend, stream); break;
-    case 2: launch_h<2, TILE_Y>(physics, which, grid, L, F, C, T, P, A, tofs, pend, stream); break;
-    case 3: launch_h<3, TILE_Y>(physics, which, grid, L, F, C, T, P, A, tofs, pend, stream); break;
-    case 4: launch_h<4, TILE_Y>(physics, which, grid, L, F, C, T, P, A, tofs, pend, stream); break;
-    case 5: launch_h<5, TILE_Y>(physics, which, grid, L, F, C, T, P, A, tofs, pend, stream); break;
-    case 6: launch_h<6, TILE_Y>(physics, which, grid, L, F, C, T, P, A, tofs, pend, stream); break;
+    case 1: launch_ty2<1>(tile_y, physics, which, L, nshots, F, C, T, P, A, tofs, pend, stream, e0, e1); break;
+    case 2: launch_ty2<2>(tile_y, physics, which, L, nshots, F, C, T, P, A, tofs, pend, stream, e0, e1); break;
+    case 3: launch_ty2<3>(tile_y, physics, which, L, nshots, F, C, T, P, A, tofs, pend, stream, e0, e1); break;
+    case 4: launch_ty2<4>(tile_y, physics, which, L, nshots, F, C, T, P, A, tofs, pend, stream, e0, e1); break;
+    case 5: launch_ty2<5>(tile_y, physics, which, L, nshots, F, C, T, P, A, tofs, pend, stream, e0, e1); break;
+    case 6: launch_ty2<6>(tile_y, physics, which, L, nshots, F, C, T, P, A, tofs, pend, stream, e0, e1); break;
     default: break;
     }
 }

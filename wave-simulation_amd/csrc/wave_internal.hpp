@@ -3,6 +3,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <cstddef>
 #include <cstdint>
@@ -13,8 +14,9 @@ enum Physics { ACOUSTIC = 0, ELASTIC = 1, TMEM = 2 };
 
 // Maximum half-width of a derivative stencil window (order 12 -> 6).
 constexpr int MAX_H = 6;
-// Tile of the 2D kernels: 64 columns (one wavefront) x TILE_Y rows, 256
-// threads per block (4 wavefronts stacked in y, each thread TILE_Y/4 rows).
+// Tile of the 2D kernels: 64 columns (one wavefront) x tile_y rows, 256
+// threads per block (4 wavefronts stacked in y, each thread tile_y/4 rows);
+// TILE_Y is the default tile height of the two-launch kernels.
 constexpr int TILE_X = 64;
 constexpr int TILE_Y = 16;
 constexpr int BLOCK_Y = 4;
@@ -61,6 +63,13 @@ struct Fields2D {
     float *f[5];  // acoustic: vx vy p ; elastic: vx vy sxx syy sxy ; tmem: hx hy ez
 };
 
+// write buffers of the CPML memory variables of the velocity derivatives
+// (single-pass kernel: read from Cpml2D::psx/psy[0..1], write here)
+struct PsiNew2D {
+    float *psx[2];
+    float *psy[2];
+};
+
 struct Coef2D {
     // acoustic: c0 = M, c1 = 1/rho_x, c2 = 1/rho_y
     // elastic : c0 = M, c1 = mu, c2 = 1/rho_x, c3 = 1/rho_y, c4 = mu_xy
@@ -96,18 +105,29 @@ struct Acq {
     const int *bsrc;            // global source ids
     const int *brec_ptr;        // [nshots*nbx*nby+1]
     const int *brec;            // global receiver ids
+    const int *vsrc_ptr;        // single-pass kernel: velocity-type sources per (shot, block)
+    const int *vsrc;            //   covering the block's velocity region (tile + H halo)
     const int *tbase;           // device step counter base
 };
 
 // Launchers (kernels2d.hip).  All launches go to `stream`.
 // which = 0: velocity-type phase (pend = apply the velocity-type sources /
 // receivers of the previous step first); which = 1: stress-type phase.
-void launch_step2d(int physics, int order, const Layout2D &L, int nshots, const Fields2D &F, const Coef2D &C,
-                   const Taps &T, const Cpml2D &P, const Acq &A, int tofs, int pend, hipStream_t stream, int which);
+// tile_y in {8, 16, 32}.  e0/e1 (optional) receive the kernel's own start/end timestamps.
+void launch_step2d(int physics, int order, int tile_y, const Layout2D &L, int nshots, const Fields2D &F,
+                   const Coef2D &C, const Taps &T, const Cpml2D &P, const Acq &A, int tofs, int pend,
+                   hipStream_t stream, int which, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 // velocity-type sources / receivers of step tbase + tofs (end of a run segment)
 void launch_flush2d(int physics, const Layout2D &L, const Fields2D &F, const Acq &A, int tofs, hipStream_t stream);
 void launch_abs_velocity2d(int physics, const Layout2D &L, int nshots, const Fields2D &F, const Coef2D &C,
                            hipStream_t stream);
+// Single-pass 2D elastic step (fused2d.hip): reads step t-1 from F, writes step t to Fn.
+// Output tiles are (64 - order) x |tile_y| cells, |tile_y| in {8, 16, 24}; tile_y < 0 loads the
+// tile with direct global->LDS transfers, tile_y > 0 through registers.
+int fused2d_tile_x(int order);
+void launch_fused2d(int order, int tile_y, const Layout2D &L, int nshots, const Fields2D &F, const Fields2D &Fn,
+                    const Coef2D &C, const Taps &T, const Cpml2D &P, const PsiNew2D &PN, const Acq &A, int tofs,
+                    int pend, hipStream_t stream, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 void launch_advance(int *tbase, int k, hipStream_t stream);
 void launch_set(int *p, int v, hipStream_t stream);
 

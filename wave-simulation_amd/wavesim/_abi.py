@@ -16,12 +16,13 @@ WV_OPS_SPARSE = 0
 WV_OPS_STENCIL = 1
 WV_PATH_FUSED = 0
 WV_PATH_PLAIN = 1
+WV_PATH_PHASED = 2
 
 # every entry point of include/wave_abi.h
 EXPORTS = (
     "wv_create", "wv_destroy", "wv_init", "wv_set_model", "wv_set_acquisition", "wv_reset", "wv_run",
     "wv_synchronize", "wv_get_seismograms", "wv_get_wavefield", "wv_set_wavefield", "wv_last_run_ms",
-    "wv_set_kernel_timing", "wv_kernel_stats", "wv_last_error",
+    "wv_tune", "wv_set_kernel_timing", "wv_kernel_stats", "wv_last_error",
 )
 
 
@@ -33,7 +34,7 @@ class Config(C.Structure):
         ("spatial_fd_order", C.c_int), ("ops_mode", C.c_int), ("free_surface", C.c_int),
         ("damping_boundary", C.c_int), ("boundary_width", C.c_int),
         ("damping_coeff", C.c_double), ("npower", C.c_double), ("fc_cpml", C.c_double), ("vmax_cpml", C.c_double),
-        ("nshots", C.c_int), ("path", C.c_int), ("steps_per_graph", C.c_int),
+        ("nshots", C.c_int), ("path", C.c_int), ("steps_per_graph", C.c_int), ("tile_y", C.c_int),
     ]
 
 
@@ -67,6 +68,7 @@ def load():
         L.wv_set_wavefield.argtypes = [C.c_void_p, C.c_int, C.c_int, fp]
         L.wv_last_run_ms.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
         L.wv_set_kernel_timing.argtypes = [C.c_void_p, C.c_int]
+        L.wv_tune.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int)]
         L.wv_kernel_stats.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_double),
                                       C.POINTER(C.c_longlong), C.POINTER(C.c_double)]
         _lib = L

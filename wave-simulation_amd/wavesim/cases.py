@@ -163,7 +163,7 @@ def elastic_c4(N=600, NT=3000, DT=1e-4, nshots=16, seed=4, W=30, anomaly=True):
     return c
 
 
-def make_solver(case, nshots=None, path="fused", ops_mode="sparse", steps_per_graph=0, shots=None):
+def make_solver(case, nshots=None, path="fused", ops_mode="sparse", steps_per_graph=0, shots=None, tile_y=0):
     """Create and fully prepare a solver for `case` (the given shots, default all)."""
     from .solver import create
 
@@ -171,7 +171,8 @@ def make_solver(case, nshots=None, path="fused", ops_mode="sparse", steps_per_gr
     n = len(shots) if nshots is None else nshots
     s = dict(case.settings)
     sol = create(case.dimension, case.type)
-    sol.init_forward_solver(nshots=n, path=path, ops_mode=ops_mode, steps_per_graph=steps_per_graph, **s)
+    sol.init_forward_solver(nshots=n, path=path, ops_mode=ops_mode, steps_per_graph=steps_per_graph, tile_y=tile_y,
+                            **s)
     sol.prepare_for_modelling(case.m0, case.m1, case.m2)
     for k, sh in enumerate(shots[:n]):
         sol.set_acquisition(k, sh.src_index, sh.src_type, sh.src_signal, sh.rec_index, sh.rec_type)

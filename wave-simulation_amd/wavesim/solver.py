@@ -47,7 +47,7 @@ class ForwardSolver:
     # -- reference-named lifecycle -------------------------------------------------
     def init_forward_solver(self, NX, NY, NT, DH, DT, spatial_fd_order=8, NZ=1, ops_mode="sparse", free_surface=0,
                             damping_boundary=0, boundary_width=0, damping_coeff=8.0, npower=4.0, fc_cpml=5.0,
-                            vmax_cpml=3500.0, nshots=1, path="fused", steps_per_graph=0):
+                            vmax_cpml=3500.0, nshots=1, path="fused", steps_per_graph=0, tile_y=0):
         cfg = _abi.Config()
         cfg.NX, cfg.NY, cfg.NZ, cfg.NT = NX, NY, NZ, NT
         cfg.DH, cfg.DT = DH, DT
@@ -58,8 +58,9 @@ class ForwardSolver:
         cfg.boundary_width = boundary_width
         cfg.damping_coeff, cfg.npower, cfg.fc_cpml, cfg.vmax_cpml = damping_coeff, npower, fc_cpml, vmax_cpml
         cfg.nshots = nshots
-        cfg.path = {"fused": _abi.WV_PATH_FUSED, "plain": _abi.WV_PATH_PLAIN}[path]
+        cfg.path = {"fused": _abi.WV_PATH_FUSED, "plain": _abi.WV_PATH_PLAIN, "phased": _abi.WV_PATH_PHASED}[path]
         cfg.steps_per_graph = steps_per_graph
+        cfg.tile_y = tile_y
         _abi.check(_abi.load().wv_init(self._h, C.byref(cfg)))
         self.cfg = cfg
         return self
@@ -100,6 +101,15 @@ class ForwardSolver:
         if t_end is None:
             t_end = t_start + 1
         _abi.check(_abi.load().wv_run(self._h, int(t_start), int(t_end)))
+
+    TUNE_CHOICES = ("phased-8", "phased-16", "phased-32", "fused-8", "fused-16", "fused-24")
+
+    def tune(self, nsteps=100):
+        """Time every kernel variant on this problem, keep the fastest (resets the wavefields)."""
+        ch = C.c_int()
+        _abi.check(_abi.load().wv_tune(self._h, int(nsteps), C.byref(ch)))
+        self.kernel_choice = self.TUNE_CHOICES[ch.value] if ch.value >= 0 else "plain"
+        return self.kernel_choice
 
     def synchronize(self):
         _abi.check(_abi.load().wv_synchronize(self._h))
