@@ -71,7 +71,8 @@ typedef struct {
     int steps_per_graph;   /* time steps per captured hipGraph segment (0 = none; rounded up to even) */
     int tile_y;            /* rows per tile (0 = default 16): single-pass elastic kernel 8, 16, 24,
                               negative selects direct global->LDS loads for the tile;
-                              two-launch kernels 8, 16, 32 */
+                              two-launch kernels 8, 16, 32; 3D elastic: rows of the
+                              x-z tile 4, 8, 16 (default 8) */
 } wv_config;
 
 /* dimension "2D"/"3D", type "acoustic"/"elastic"/"tmem" (case-insensitive) */
@@ -113,7 +114,8 @@ int wv_set_wavefield(wv_solver *s, int shot, int field, const float *in);
  * tiles) on the handle's own model and acquisition, keeps the fastest and
  * resets the wavefields and seismograms.  *choice receives the index of the
  * variant kept (0..2 = two-launch with 8/16/32 rows, 3..5 = single pass with
- * 8/16/24 rows, -1 = the handle has a single variant: WV_PATH_PLAIN).
+ * 8/16/24 rows, 6..8 = 3D elastic with 4/8/16-row x-z tiles, -1 = the handle
+ * has a single variant: WV_PATH_PLAIN).
  * No counterpart in the reference. */
 int wv_tune(wv_solver *s, int nsteps, int *choice);
 

@@ -12,26 +12,11 @@
 #include <string>
 #include <vector>
 
-#include "plain.hpp"
+#include "handle.hpp"
 #include "prep.hpp"
-#include "wave_internal.hpp"
 
 namespace {
-
 thread_local std::string g_err;
-
-int fail(int code, const std::string &msg)
-{
-    g_err = msg;
-    return code;
-}
-
-#define HIP_TRY(expr)                                                                   \
-    do {                                                                                \
-        hipError_t e_ = (expr);                                                         \
-        if (e_ != hipSuccess)                                                           \
-            return fail(WV_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
-    } while (0)
 
 std::string lower(const char *s)
 {
@@ -39,217 +24,13 @@ std::string lower(const char *s)
     for (auto &c : r) c = (char)std::tolower((unsigned char)c);
     return r;
 }
-
-int round_up(int v, int m) { return (v + m - 1) / m * m; }
-
-// Position-indexed tables (taps, CPML coefficients, frame indices) are stored
-// for positions [-ORG, n_padded + ORG) so kernels may index halo positions
-// outside the grid; the pointer handed to a kernel points at position 0.
-constexpr int ORG = 64;
-
-struct ShotAcq {
-    std::vector<long long> sidx;
-    std::vector<int> stype;
-    std::vector<float> ssig;
-    std::vector<long long> ridx;
-    std::vector<int> rtype;
-};
-
-enum Variant { V_SINGLE = 0, V_PHASED = 1 };
-
 }  // namespace
 
-struct wv_solver {
-    int dim = 2;
-    int physics = wv::ELASTIC;
-    wv_config cfg{};
-    bool inited = false;
-    bool model_ok = false;
-    hipStream_t st = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    bool ran = false;
-
-    int variant = V_PHASED;
-    int tile_x = wv::TILE_X, tile_y = wv::TILE_Y;  // tiling of the kernels in use
-    int tile_load = 1;  // single-pass kernel: -1 direct global->LDS loads, 1 register-staged loads
-    wv::Layout2D L{};
-    int H = 0;
-    int nfields = 0;
-    int cur = 0;  // buffer holding the current state (single-pass kernel ping-pongs)
-    std::vector<void *> allocs;
-
-    // device buffers
-    float *field[2][5] = {};
-    float *coef[5] = {};
-    float *damp = nullptr, *fsH = nullptr, *fsV = nullptr;
-    float *tap[6] = {};  // position-0 pointers: xf xb yf yb yf_v yb_v
-    float *cpx[4] = {}, *cpy[4] = {};
-    int *xi = nullptr, *yi = nullptr;
-    float *psv[2][4] = {};  // ping-pong: velocity-derivative memory variables psx0 psx1 psy0 psy1
-    float *pss[4] = {};     // stress-derivative memory variables psx2 psx3 psy2 psy3
-    size_t psx_n = 0, psy_n = 0;
-
-    // acquisition
-    std::vector<ShotAcq> shots;
-    bool acq_dirty = true;
-    std::vector<void *> acq_allocs;
-    int *d_src_ptr = nullptr, *d_src_type = nullptr, *d_rec_ptr = nullptr, *d_rec_type = nullptr;
-    int *d_bsrc_ptr = nullptr, *d_bsrc = nullptr, *d_brec_ptr = nullptr, *d_brec = nullptr;
-    int *d_vsrc_ptr = nullptr, *d_vsrc = nullptr;
-    int nbx = 0, nby = 0;
-    long long *d_src_cell = nullptr, *d_rec_cell = nullptr;
-    float *d_src_sig = nullptr, *d_seis = nullptr;
-    int rec_total = 0;
-    std::vector<int> rec_ptr_h;
-    int *d_tbase = nullptr;
-
-    // graph of steps_per_graph time steps (starts and ends in buffer 0)
-    hipGraphExec_t gexec = nullptr;
-    int gsteps = 0;
-
-    // reference-sequence path
-    wv::PlainSolver *plain = nullptr;
-
-    // per-kernel timing (wv_set_kernel_timing)
-    bool ktiming = false;
-    std::vector<hipEvent_t> kev;
-    double kms[3] = {0, 0, 0};
-    long long klaunch[3] = {0, 0, 0};
-
-    ~wv_solver();
-    template <typename T>
-    int dalloc(T **p, size_t n, std::vector<void *> &list)
-    {
-        void *q = nullptr;
-        if (n == 0) n = 1;
-        if (hipMalloc(&q, n * sizeof(T)) != hipSuccess) return fail(WV_ENOMEM, "hipMalloc failed");
-        if (hipMemsetAsync(q, 0, n * sizeof(T), st) != hipSuccess) return fail(WV_EDEVICE, "hipMemset failed");
-        list.push_back(q);
-        *p = (T *)q;
-        return 0;
-    }
-    template <typename T>
-    int upload(T **p, const std::vector<T> &h, std::vector<void *> &list)
-    {
-        int rc = dalloc(p, h.size(), list);
-        if (rc) return rc;
-        if (!h.empty() && hipMemcpyAsync(*p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, st) != hipSuccess)
-            return fail(WV_EDEVICE, "hipMemcpy failed");
-        return 0;
-    }
-    // upload a position-indexed table with ORG positions of `fill` on both sides
-    template <typename T>
-    int upload_pos(T **p0, const std::vector<T> &h, int per_pos, T fill)
-    {
-        std::vector<T> ext((size_t)(h.size() / per_pos + 2 * ORG) * per_pos, fill);
-        std::copy(h.begin(), h.end(), ext.begin() + (size_t)ORG * per_pos);
-        T *base = nullptr;
-        int rc = upload(&base, ext, allocs);
-        if (rc) return rc;
-        *p0 = base + (size_t)ORG * per_pos;
-        return 0;
-    }
-    void free_list(std::vector<void *> &list)
-    {
-        for (void *p : list) (void)hipFree(p);
-        list.clear();
-    }
-    void drop_graph()
-    {
-        if (gexec) (void)hipGraphExecDestroy(gexec);
-        gexec = nullptr;
-        gsteps = 0;
-    }
-    wv::Fields2D fields2d(int b) const
-    {
-        wv::Fields2D F{};
-        for (int k = 0; k < 5; k++) F.f[k] = field[b][k];
-        return F;
-    }
-    wv::Coef2D coef2d() const
-    {
-        wv::Coef2D C{};
-        for (int k = 0; k < 5; k++) C.c[k] = coef[k];
-        C.damp = cfg.damping_boundary == 1 ? damp : nullptr;
-        C.fsH = fsH;
-        C.fsV = fsV;
-        C.free_surface = cfg.free_surface == 1;
-        return C;
-    }
-    wv::Taps taps() const
-    {
-        wv::Taps T{};
-        T.xf = tap[0];
-        T.xb = tap[1];
-        T.yf = tap[2];
-        T.yb = tap[3];
-        T.yf_v = tap[4];
-        T.yb_v = tap[5];
-        return T;
-    }
-    wv::Cpml2D cpml2d(int b) const
-    {
-        wv::Cpml2D P{};
-        P.on = cfg.damping_boundary == 2;
-        P.W = cfg.boundary_width;
-        P.ax = cpx[0];
-        P.bx = cpx[1];
-        P.axh = cpx[2];
-        P.bxh = cpx[3];
-        P.ay = cpy[0];
-        P.by = cpy[1];
-        P.ayh = cpy[2];
-        P.byh = cpy[3];
-        P.xi = xi;
-        P.yi = yi;
-        P.psx[0] = psv[b][0];
-        P.psx[1] = psv[b][1];
-        P.psy[0] = psv[b][2];
-        P.psy[1] = psv[b][3];
-        P.psx[2] = pss[0];
-        P.psx[3] = pss[1];
-        P.psy[2] = pss[2];
-        P.psy[3] = pss[3];
-        return P;
-    }
-    wv::PsiNew2D psinew(int b) const
-    {
-        wv::PsiNew2D N{};
-        N.psx[0] = psv[b][0];
-        N.psx[1] = psv[b][1];
-        N.psy[0] = psv[b][2];
-        N.psy[1] = psv[b][3];
-        return N;
-    }
-    wv::Acq acq() const
-    {
-        wv::Acq A{};
-        A.nshots = cfg.nshots;
-        A.NT = cfg.NT;
-        A.src_ptr = d_src_ptr;
-        A.src_cell = d_src_cell;
-        A.src_type = d_src_type;
-        A.src_sig = d_src_sig;
-        A.rec_ptr = d_rec_ptr;
-        A.rec_cell = d_rec_cell;
-        A.rec_type = d_rec_type;
-        A.rec_total = rec_total;
-        A.seis = d_seis;
-        A.tbase = d_tbase;
-        A.nbx = nbx;
-        A.nby = nby;
-        A.bsrc_ptr = d_bsrc_ptr;
-        A.bsrc = d_bsrc;
-        A.brec_ptr = d_brec_ptr;
-        A.brec = d_brec;
-        A.vsrc_ptr = d_vsrc_ptr;
-        A.vsrc = d_vsrc;
-        return A;
-    }
-    int enqueue_steps(int nsteps, bool timed);
-    int sync_acquisition();
-    double kernel_bytes(int kind) const;
-};
+int wv_fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
 
 wv_solver::~wv_solver()
 {
@@ -257,6 +38,7 @@ wv_solver::~wv_solver()
     for (hipEvent_t e : kev) (void)hipEventDestroy(e);
     drop_graph();
     delete plain;
+    delete e3;
     free_list(acq_allocs);
     free_list(allocs);
     if (ev0) (void)hipEventDestroy(ev0);
@@ -472,7 +254,7 @@ int wv_create(const char *dimension, const char *type, wv_solver **out)
         phys = wv::TMEM;
     else
         return fail(WV_EINVAL, "equationType " + t + " is not provided by this solver");
-    if (d == "3d") return fail(WV_EINVAL, "3D " + t + " is not provided by this build");
+    if (d == "3d" && phys != wv::ELASTIC) return fail(WV_EINVAL, "3D " + t + " is not provided by this build");
     wv_solver *s = new wv_solver();
     s->dim = d == "3d" ? 3 : 2;
     s->physics = phys;
@@ -490,7 +272,8 @@ int wv_init(wv_solver *s, const wv_config *c)
 {
     if (!s || !c) return fail(WV_EINVAL, "null argument");
     const wv_config &k = *c;
-    if (k.NX < 2 || k.NY < 2 || k.NZ != 1) return fail(WV_EINVAL, "2D grid needs NX, NY >= 2 and NZ = 1");
+    if (s->dim == 2 && (k.NX < 2 || k.NY < 2 || k.NZ != 1))
+        return fail(WV_EINVAL, "2D grid needs NX, NY >= 2 and NZ = 1");
     if (k.NT < 1) return fail(WV_EINVAL, "NT must be positive");
     if (!(k.DH > 0) || !(k.DT > 0)) return fail(WV_EINVAL, "DH and DT must be positive");
     const int N = k.spatial_fd_order;
@@ -507,10 +290,11 @@ int wv_init(wv_solver *s, const wv_config *c)
     if (k.path != WV_PATH_FUSED && k.path != WV_PATH_PLAIN && k.path != WV_PATH_PHASED)
         return fail(WV_EINVAL, "bad path");
     if (k.path == WV_PATH_PLAIN && k.nshots != 1) return fail(WV_EINVAL, "the plain path runs one shot per handle");
-    const bool single = k.path == WV_PATH_FUSED && s->physics == wv::ELASTIC;
-    if (single && k.tile_y != 0 && std::abs(k.tile_y) != 8 && std::abs(k.tile_y) != 16 && std::abs(k.tile_y) != 24)
+    if (s->dim == 3 && k.path == WV_PATH_PLAIN) return fail(WV_EINVAL, "the plain path is 2D only");
+    const bool single = s->dim == 2 && k.path == WV_PATH_FUSED && s->physics == wv::ELASTIC;
+    if (s->dim == 2 && single && k.tile_y != 0 && std::abs(k.tile_y) != 8 && std::abs(k.tile_y) != 16 && std::abs(k.tile_y) != 24)
         return fail(WV_EINVAL, "tile_y of the single-pass kernel must be 0 (default), +-8, +-16 or +-24");
-    if (!single && k.tile_y != 0 && k.tile_y != 8 && k.tile_y != 16 && k.tile_y != 32)
+    if (s->dim == 2 && !single && k.tile_y != 0 && k.tile_y != 8 && k.tile_y != 16 && k.tile_y != 32)
         return fail(WV_EINVAL, "tile_y of the two-launch kernels must be 0 (default), 8, 16 or 32");
     if (s->inited) return fail(WV_ESTATE, "solver already initialised");
 
@@ -520,6 +304,15 @@ int wv_init(wv_solver *s, const wv_config *c)
     HIP_TRY(hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking));
     HIP_TRY(hipEventCreate(&s->ev0));
     HIP_TRY(hipEventCreate(&s->ev1));
+
+    if (s->dim == 3) {
+        s->e3 = new wv::Elastic3D();
+        int rc = s->e3->init(s->cfg, s->st);
+        if (rc) return fail(rc, s->e3->error);
+        s->shots.assign(k.nshots, ShotAcq());
+        s->inited = true;
+        return WV_OK;
+    }
 
     if (k.path == WV_PATH_PLAIN) {
         s->plain = new wv::PlainSolver();
@@ -662,6 +455,12 @@ int wv_set_model(wv_solver *s, const float *m0, const float *m1, const float *m2
         s->model_ok = true;
         return WV_OK;
     }
+    if (s->e3) {
+        int rc = s->e3->set_model(m0, m1, m2);
+        if (rc) return fail(rc, s->e3->error);
+        s->model_ok = true;
+        return WV_OK;
+    }
     const wv::Layout2D &L = s->L;
     const int NX = L.NX, NY = L.NY;
     auto gi = [&](int x, int y) { return (size_t)x + (size_t)y * NX; };
@@ -772,7 +571,7 @@ int wv_set_acquisition(wv_solver *s, int shot, int nsrc, const long long *src_in
     if ((nsrc && (!src_index || !src_type || !src_signal)) || (nrec && (!rec_index || !rec_type)))
         return fail(WV_EINVAL, "missing acquisition array");
     const long long n = (long long)s->cfg.NX * s->cfg.NY * s->cfg.NZ;
-    const int maxtype = s->physics == wv::TMEM ? 1 : 3;
+    const int maxtype = s->physics == wv::TMEM ? 1 : (s->dim == 3 ? 4 : 3);
     for (int k = 0; k < nsrc; k++) {
         if (src_index[k] < 0 || src_index[k] >= n) return fail(WV_EINVAL, "source outside the grid");
         if (src_type[k] < 1 || src_type[k] > maxtype) return fail(WV_EINVAL, "unsupported source type");
@@ -791,6 +590,10 @@ int wv_set_acquisition(wv_solver *s, int shot, int nsrc, const long long *src_in
         int rc = s->plain->set_acquisition(a.sidx, a.stype, a.ssig, a.ridx, a.rtype);
         if (rc) return fail(rc, s->plain->error);
     }
+    if (s->e3) {
+        int rc = s->e3->set_acquisition(shot, a.sidx, a.stype, a.ssig, a.ridx, a.rtype);
+        if (rc) return fail(rc, s->e3->error);
+    }
     s->shots[shot] = std::move(a);
     s->acq_dirty = true;
     return WV_OK;
@@ -802,6 +605,10 @@ int wv_reset(wv_solver *s)
     if (s->plain) {
         int rc = s->plain->reset();
         return rc ? fail(rc, s->plain->error) : WV_OK;
+    }
+    if (s->e3) {
+        int rc = s->e3->reset();
+        return rc ? fail(rc, s->e3->error) : WV_OK;
     }
     const size_t fn = (size_t)s->L.plane * s->cfg.nshots;
     for (int b = 0; b < 2; b++)
@@ -825,6 +632,14 @@ int wv_run(wv_solver *s, int t_start, int t_end)
         HIP_TRY(hipEventRecord(s->ev0, s->st));
         int rc = s->plain->run(t_start, t_end);
         if (rc) return fail(rc, s->plain->error);
+        HIP_TRY(hipEventRecord(s->ev1, s->st));
+        s->ran = true;
+        return WV_OK;
+    }
+    if (s->e3) {
+        HIP_TRY(hipEventRecord(s->ev0, s->st));
+        int rc = s->e3->run(t_start, t_end, s->ktiming);
+        if (rc) return fail(rc, s->e3->error);
         HIP_TRY(hipEventRecord(s->ev1, s->st));
         s->ran = true;
         return WV_OK;
@@ -894,6 +709,10 @@ int wv_get_seismograms(wv_solver *s, int shot, float *out)
         int rc = s->plain->get_seismograms(out);
         return rc ? fail(rc, s->plain->error) : WV_OK;
     }
+    if (s->e3) {
+        int rc = s->e3->get_seismograms(shot, out);
+        return rc ? fail(rc, s->e3->error) : WV_OK;
+    }
     const int NT = s->cfg.NT;
     const int nrec = (int)s->shots[shot].ridx.size();
     if (nrec == 0) return WV_OK;
@@ -917,6 +736,10 @@ static int field_io(wv_solver *s, int shot, int field, float *out, const float *
     if (s->plain) {
         int rc = s->plain->field_io(field, out, in);
         return rc ? fail(rc, s->plain->error) : WV_OK;
+    }
+    if (s->e3) {
+        int rc = s->e3->field_io(shot, field, out, in);
+        return rc ? fail(rc, s->e3->error) : WV_OK;
     }
     if (field < 0 || field >= s->nfields) return fail(WV_EINVAL, "bad field index");
     const wv::Layout2D &L = s->L;
@@ -954,6 +777,12 @@ int wv_tune(wv_solver *s, int nsteps, int *choice)
     if (nsteps < 2 || nsteps > s->cfg.NT) return fail(WV_EINVAL, "nsteps must be in [2, NT]");
     if (choice) *choice = -1;
     if (s->plain) return WV_OK;  // one candidate
+    if (s->e3) {
+        int rc = s->e3->tune(nsteps, choice);
+        if (rc) return fail(rc, s->e3->error);
+        if (choice) *choice += 6;
+        return WV_OK;
+    }
     // candidates: two-launch kernels with 8/16/32-row tiles; for elastic P-SV
     // also the single-pass kernel with register-staged tiles of 8/16/24 rows
     const int cand_var[6] = {V_PHASED, V_PHASED, V_PHASED, V_SINGLE, V_SINGLE, V_SINGLE};
@@ -1003,6 +832,10 @@ int wv_set_kernel_timing(wv_solver *s, int on)
     for (int k = 0; k < 3; k++) {
         s->kms[k] = 0;
         s->klaunch[k] = 0;
+        if (s->e3) {
+            s->e3->kms[k] = 0;
+            s->e3->klaunch[k] = 0;
+        }
     }
     return WV_OK;
 }
@@ -1017,6 +850,13 @@ int wv_kernel_stats(wv_solver *s, int kind, const char **name, double *avg_ms, l
                                       {"el2d_velocity", "el2d_stress", "flush2d"},
                                       {"tm2d_h", "tm2d_e", "flush2d"}};
     HIP_TRY(hipStreamSynchronize(s->st));
+    if (s->e3) {
+        if (name) *name = s->e3->kernel_name(kind);
+        if (launches) *launches = s->e3->klaunch[kind];
+        if (avg_ms) *avg_ms = s->e3->klaunch[kind] ? s->e3->kms[kind] / s->e3->klaunch[kind] : 0.0;
+        if (bytes_per_launch) *bytes_per_launch = s->e3->kernel_bytes(kind);
+        return WV_OK;
+    }
     if (name) *name = (s->variant == V_SINGLE && kind < 2) ? (kind == 0 ? "el2d_fused" : "-") : names[s->physics][kind];
     if (launches) *launches = s->klaunch[kind];
     if (avg_ms) *avg_ms = s->klaunch[kind] ? s->kms[kind] / s->klaunch[kind] : 0.0;

@@ -1,0 +1,146 @@
+// 3D elastic staggered-grid solver (ForwardSolver3Delastic.cpp:120-416):
+// device layout, kernel launchers and the host-side driver used by the C ABI.
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "../../include/wave_abi.h"
+#include "wave_internal.hpp"
+
+namespace wv {
+
+// Padded 3D layout of one field of one shot.  Index order as the reference
+// (x fastest, then z, then depth y slowest: Coordinates.cpp:687):
+// element (x, y, z) lives at (y + H) * slab + (z + H) * pitch + x + XPAD.
+// Padding is zero and never written; a tile of 64 columns plus an H-wide halo
+// and up to 16 + H rows in z and H planes in y around the grid stay inside.
+struct Layout3D {
+    int NX, NY, NZ;
+    int H;
+    int pitch;       // floats per x row
+    int zrows;       // rows per y plane (H + roundup(NZ, 16) + H)
+    long long slab;  // floats per y plane
+    long long vol;   // floats per field of one shot
+    __host__ __device__ long long at(int x, int y, int z) const
+    {
+        return (long long)(y + H) * slab + (long long)(z + H) * pitch + (x + XPAD);
+    }
+};
+
+struct Fields3D {
+    float *f[9];  // vx vy vz sxx syy szz syz sxz sxy
+};
+
+struct Coef3D {
+    // 1/rho_x 1/rho_y 1/rho_z M mu mu_xy mu_xz mu_yz (shared by the shots of a batch)
+    const float *irx, *iry, *irz, *M, *mu, *muxy, *muxz, *muyz;
+    const float *damp;      // ABS factors (layout of one field) or nullptr
+    const float *fsH, *fsV;  // free-surface scales on the y = 0 plane ((z + H) * pitch + x + XPAD) or nullptr
+};
+
+struct Taps3D {
+    const float *xf, *xb, *yf, *yb, *yf_v, *yb_v, *zf, *zb;  // [pos][2H+1], see Taps
+};
+
+// CPML coefficients per axis and compact frame indices (as Cpml2D).  Memory
+// variables: x frame [shot][NY][NZ][2W], y frame [shot][2W][NZ][NX],
+// z frame [shot][NY][2W][NX].  Index k of ps[] follows the derivative terms
+// of the reference's run(): 0 sxx_x 1 sxy_y 2 sxz_z 3 sxy_x 4 syy_y 5 syz_z
+// 6 sxz_x 7 syz_y 8 szz_z | 9 vx_x 10 vy_y 11 vz_z 12 vx_y 13 vy_x 14 vx_z
+// 15 vz_x 16 vy_z 17 vz_y.
+struct Cpml3D {
+    int on;
+    int W;
+    const float *ax, *bx, *axh, *bxh;
+    const float *ay, *by, *ayh, *byh;
+    const float *az, *bz, *azh, *bzh;
+    const int *xi, *yi, *zi;
+    float *ps[18];
+};
+
+// One time step = velocity launch + stress launch (+ ABS velocity damping).
+// rz = rows in z per thread (tile height 4 * rz), nyc = chunks along y.
+void launch_el3d(int which, int order, int rz, const Layout3D &L, int nshots, int nyc, const Fields3D &F,
+                 const Coef3D &C, const Taps3D &T, const Cpml3D &P, const Acq &A, int tofs, int pend,
+                 hipStream_t stream, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
+void launch_abs_velocity3d(const Layout3D &L, int nshots, const Fields3D &F, const Coef3D &C, hipStream_t stream);
+void launch_flush3d(const Layout3D &L, const Fields3D &F, const Acq &A, int tofs, hipStream_t stream);
+
+// Host driver of the 3D elastic solver (owned by a wv_solver handle).
+class Elastic3D {
+  public:
+    ~Elastic3D();
+    int init(const wv_config &cfg, hipStream_t stream);
+    int set_model(const float *vp, const float *vs, const float *rho);
+    int set_acquisition(int shot, const std::vector<long long> &sidx, const std::vector<int> &stype,
+                        const std::vector<float> &ssig, const std::vector<long long> &ridx,
+                        const std::vector<int> &rtype);
+    int reset();
+    int run(int t_start, int t_end, bool timed);
+    int get_seismograms(int shot, float *out);  // [nrec][NT] in receiver input order
+    int field_io(int shot, int field, float *out, const float *in);
+    int tune(int nsteps, int *choice);
+    int set_rz(int rz);
+    int nrec(int shot) const;
+    double kernel_bytes(int kind) const;
+    const char *kernel_name(int kind) const;
+    double kms[3] = {0, 0, 0};
+    long long klaunch[3] = {0, 0, 0};
+    std::string error;
+
+  private:
+    int fail(int code, const std::string &m);
+    int dalloc(float **p, size_t n);
+    template <typename T>
+    int upload(T **p, const std::vector<T> &h, std::vector<void *> &list);
+    template <typename T>
+    int upload_pos(T **p0, const std::vector<T> &h, int per_pos, T fill);
+    int sync_acquisition();
+    int enqueue(int nsteps, bool timed);
+    void drop_graph();
+    void free_list(std::vector<void *> &l);
+    Fields3D fields() const;
+    Coef3D coefs() const;
+    Taps3D taps() const;
+    Cpml3D cpml() const;
+    Acq acq() const;
+
+    hipStream_t st = nullptr;
+    int NX = 0, NY = 0, NZ = 0, NT = 0, H = 0, order = 0, nshots = 1;
+    int damping = 0, free_surface = 0, W = 0, steps_per_graph = 0;
+    double DT = 0, DH = 0;
+    int rz = 2, nyc = 1, nbx = 0, nbz = 0;
+    Layout3D L{};
+    std::vector<void *> allocs, acq_allocs;
+    float *F[9] = {};
+    float *coef[8] = {};
+    float *damp = nullptr, *fsH = nullptr, *fsV = nullptr;
+    float *tap[8] = {};
+    float *cp[3][4] = {};
+    int *idx[3] = {};
+    float *ps[18] = {};
+    size_t ps_n[3] = {0, 0, 0};
+    bool model_ok = false;
+
+    struct Shot {
+        std::vector<long long> sidx, ridx;
+        std::vector<int> stype, rtype;
+        std::vector<float> ssig;
+    };
+    std::vector<Shot> shots;
+    bool acq_dirty = true;
+    int *d_src_ptr = nullptr, *d_src_type = nullptr, *d_rec_ptr = nullptr, *d_rec_type = nullptr;
+    int *d_bsrc_ptr = nullptr, *d_bsrc = nullptr, *d_brec_ptr = nullptr, *d_brec = nullptr;
+    long long *d_src_cell = nullptr, *d_rec_cell = nullptr;
+    float *d_src_sig = nullptr, *d_seis = nullptr;
+    int rec_total = 0;
+    std::vector<int> rec_ptr_h;
+    int *d_tbase = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int gsteps = 0;
+    std::vector<hipEvent_t> kev;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+}  // namespace wv

@@ -16,6 +16,9 @@ smaller grids / fewer steps of the same recipe.
                 sources with 20..300 EZ receivers each
   elastic_c4    2D elastic, order 8, 600x600, 3000 steps, layered + 80 m-radius vp/vs anomaly (+10 %),
                 16 shots, VX/VY receivers every 2 m
+  elastic_c5    3D elastic, order 8, 320^3, dx 2 m, dt 2e-4 s, 2500 steps, 24-cell CPML, 6 depth layers
+                vp 1800-3800, vs = vp/1.8, rho 1800-2500, x(1 + 0.03 g) (10-cell correlation), 8 shots of
+                15 Hz Ricker VY forces at depth 26, 40x40 receiver grids (spacing 4) of vx, vy, vz
 """
 from __future__ import annotations
 
@@ -160,6 +163,41 @@ def elastic_c4(N=600, NT=3000, DT=1e-4, nshots=16, seed=4, W=30, anomaly=True):
     for _ in range(nshots):
         sx = int(rng.integers(W + 10, NX - W - 10))
         c.shots.append(Shot(np.array([sx + y * NX]), np.array([3]), w[None, :], rec, rty))
+    return c
+
+
+def elastic_c5(N=320, NT=2500, DT=2e-4, nshots=8, seed=5, W=24, nrec_side=40, spacing=4, src_depth=26):
+    NX = NY = NZ = N
+    s = dict(NX=NX, NY=NY, NZ=NZ, NT=NT, DH=2.0, DT=DT, spatial_fd_order=8, damping_boundary=2, boundary_width=W,
+             npower=4.0, fc_cpml=15.0, vmax_cpml=4400.0)
+    # model arrays in the reference's index order: x fastest, then z, then depth y
+    rows = np.arange(NY) * 6 // NY
+    vp_y = 1800.0 + 2000.0 * rows / 5
+    rho_y = 1800.0 + 700.0 * rows / 5
+    g = gaussian_field((NY, NZ, NX), 10.0, seed)
+    gr = gaussian_field((NY, NZ, NX), 10.0, seed + 100)
+    vp = vp_y[:, None, None] * (1.0 + 0.03 * g)
+    rho = rho_y[:, None, None] * (1.0 + 0.03 * gr)
+    vs = vp / 1.8
+    c = Case("elastic_c5", "3D", "elastic", s, vp.astype(np.float32).ravel(), vs.astype(np.float32).ravel(),
+             rho.astype(np.float32).ravel())
+    rng = np.random.Generator(np.random.PCG64(seed))
+    lo = W + max(2, min(16, (N - 2 * W) // 4))
+    hi = N - lo
+    w = ricker(NT, DT, 15.0, 1.0)
+    y = min(src_depth, NY - W - 2)
+    for _ in range(nshots):
+        sx, sz = (int(v) for v in rng.integers(lo, hi, 2))
+        off = (np.arange(nrec_side) - nrec_side // 2) * spacing
+        rx = sx + off
+        rz = sz + off
+        rx = rx[(rx >= W) & (rx < NX - W)]
+        rz = rz[(rz >= W) & (rz < NZ - W)]
+        gx, gz = np.meshgrid(rx, rz)
+        cells = (gx + gz * NX + (W + 2) * NX * NZ).ravel()
+        rec = np.concatenate([cells, cells, cells])
+        rty = np.repeat(np.array([2, 3, 4], np.int32), len(cells))
+        c.shots.append(Shot(np.array([sx + sz * NX + y * NX * NZ]), np.array([3]), w[None, :], rec, rty))
     return c
 
 

@@ -27,6 +27,7 @@ class ForwardSolver:
         ("2D", "acoustic"): ("vx", "vy", "p"),
         ("2D", "elastic"): ("vx", "vy", "sxx", "syy", "sxy"),
         ("2D", "tmem"): ("hx", "hy", "ez"),
+        ("3D", "elastic"): ("vx", "vy", "vz", "sxx", "syy", "szz", "syz", "sxz", "sxy"),
     }
 
     def __init__(self, dimension: str, type_: str):
@@ -102,7 +103,8 @@ class ForwardSolver:
             t_end = t_start + 1
         _abi.check(_abi.load().wv_run(self._h, int(t_start), int(t_end)))
 
-    TUNE_CHOICES = ("phased-8", "phased-16", "phased-32", "fused-8", "fused-16", "fused-24")
+    TUNE_CHOICES = ("phased-8", "phased-16", "phased-32", "fused-8", "fused-16", "fused-24", "3d-z4", "3d-z8",
+                    "3d-z16")
 
     def tune(self, nsteps=100):
         """Time every kernel variant on this problem, keep the fastest (resets the wavefields)."""
