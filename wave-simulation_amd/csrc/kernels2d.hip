@@ -16,115 +16,11 @@
 // kernel streams at memory-level parallelism instead of serialised round trips.
 // Derivative taps along x sit in registers (one column per lane), taps along y
 // in scalar registers (one row per wavefront).
-#include "wave_internal.hpp"
+#include "tile2d.hpp"
 
 namespace wv {
 
 namespace {
-
-constexpr int NTHREADS = TILE_X * BLOCK_Y;
-
-template <int H, int TY>
-struct Tile {
-    static constexpr int RW = TILE_X + 2 * H;
-    static constexpr int RH = TY + 2 * H;
-    static constexpr int TOT = RW * RH;
-    static constexpr int IT = (TOT + NTHREADS - 1) / NTHREADS;
-};
-
-// Registers holding one thread's share of the region
-// [x0-H, x0+64+H) x [y0-H, y0+TY+H) of one field plane.
-template <int H, int TY>
-struct TileRegs {
-    using T = Tile<H, TY>;
-    float v[T::IT];
-    __device__ __forceinline__ void load(const float *__restrict__ g, const Layout2D &L, int x0, int y0, int tid)
-    {
-        const long long base = L.at(x0 - H, y0 - H);
-#pragma unroll
-        for (int k = 0; k < T::IT; k++) {
-            const int i = tid + k * NTHREADS;
-            if (T::TOT % NTHREADS == 0 || i < T::TOT) {
-                const int r = i / T::RW;
-                const int c = i - r * T::RW;
-                v[k] = g[base + (long long)r * L.pitch + c];
-            }
-        }
-    }
-    __device__ __forceinline__ void store(float (*S)[T::RW], int tid) const
-    {
-#pragma unroll
-        for (int k = 0; k < T::IT; k++) {
-            const int i = tid + k * NTHREADS;
-            if (T::TOT % NTHREADS == 0 || i < T::TOT) {
-                const int r = i / T::RW;
-                const int c = i - r * T::RW;
-                S[r][c] = v[k];
-            }
-        }
-    }
-};
-
-// derivative along x at LDS row r, column tx (taps for offsets -H..H)
-template <int H, int RW>
-__device__ __forceinline__ float dx(const float (*S)[RW], int r, int tx, const float *c)
-{
-    float s = 0.f;
-#pragma unroll
-    for (int k = 0; k <= 2 * H; k++) s += c[k] * S[r][tx + k];
-    return s;
-}
-
-// derivative along y at LDS row r (center), column tx; c is wavefront-uniform
-template <int H, int RW>
-__device__ __forceinline__ float dy(const float (*S)[RW], int r, int tx, const float *__restrict__ c)
-{
-    float s = 0.f;
-#pragma unroll
-    for (int k = 0; k <= 2 * H; k++) s += c[k] * S[r + k - H][tx + H];
-    return s;
-}
-
-// psi = b*psi + a*d ; d += psi   (CPML.cpp:84-95)
-__device__ __forceinline__ float cpml(float &d, float psi, float a, float b)
-{
-    float p = b * psi;
-    p = p + a * d;
-    d = d + p;
-    return p;
-}
-
-struct Col {
-    int x, xi;
-    bool in;
-    float ax, bx, axh, bxh;
-};
-
-__device__ __forceinline__ Col column(const Layout2D &L, const Cpml2D &P, int x)
-{
-    Col c;
-    c.x = x;
-    c.in = x < L.NX;
-    c.xi = -1;
-    c.ax = c.bx = c.axh = c.bxh = 0.f;
-    if (P.on) {
-        c.xi = c.in ? P.xi[x] : -1;
-        if (c.xi >= 0) {
-            c.ax = P.ax[x];
-            c.bx = P.bx[x];
-            c.axh = P.axh[x];
-            c.bxh = P.bxh[x];
-        }
-    }
-    return c;
-}
-
-template <int H>
-__device__ __forceinline__ void load_taps(float *dst, const float *__restrict__ tab, int pos)
-{
-#pragma unroll
-    for (int k = 0; k <= 2 * H; k++) dst[k] = tab[(long long)pos * (2 * H + 1) + k];
-}
 
 // velocity-type components: VX (2) and VY (3); stress-type: P / EZ (1)
 __device__ __forceinline__ bool velocity_type(int ty) { return ty == 2 || ty == 3; }
