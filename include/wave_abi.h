@@ -138,6 +138,32 @@ int wv_kernel_stats(wv_solver *s, int kind, const char **name, double *avg_ms, l
 
 const char *wv_last_error(void);
 
+/* ---- adjoint-state gradient (2D elastic; CPML or no absorbing frame, no free
+ * surface).  No counterpart in the reference: WAVE-Simulation has no adjoint
+ * or gradient code (only inert flags, Modelparameter.hpp:94-95, 289); these
+ * entry points serve the WAVE-Inversion-style caller that needs dJ/dm.      */
+
+/* Checkpoint every `every` steps during the following wv_run calls (0 = off);
+ * fp16 != 0 stores the wavefields in FP16 with one scale per field and shot,
+ * otherwise FP32; CPML memory variables are kept in FP32.  Switches the handle
+ * to the two-launch kernels, uses graph segments of `every` steps and resets
+ * the wavefields. */
+int wv_set_checkpointing(wv_solver *s, int every, int fp16);
+
+/* adjoint source of one shot: dJ/d(seismogram), [nrec][NT] in receiver order */
+int wv_set_adjoint_source(wv_solver *s, int shot, const float *adj);
+
+/* Time-reversed adjoint pass over [0, NT) after a checkpointed wv_run(0, NT):
+ * recomputes each checkpoint segment and correlates with the adjoint fields;
+ * the gradient is summed over all shots of the handle in FP64.  Asynchronous
+ * on the handle's stream; wv_last_run_ms reports its duration. */
+int wv_run_adjoint(wv_solver *s);
+
+/* gradient [NY][NX] (x fastest): param 0 vp, 1 vs, 2 rho (chain rule through
+ * the averaging rules of wv_set_model), 3..7 w.r.t. the coefficients
+ * 1/rho_x, 1/rho_y, M, mu, mu_xy */
+int wv_get_gradient(wv_solver *s, int param, double *out);
+
 #ifdef __cplusplus
 }
 #endif

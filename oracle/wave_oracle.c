@@ -1145,3 +1145,338 @@ int oracle_run(const oracle_params *p, const real *m0, const real *m1, const rea
     ops_free(&o);
     return 0;
 }
+
+/* ------------------------------------------------------------------------- */
+/* Adjoint-state gradient of the 2D elastic time loop (no reference code:     */
+/* the reference has no adjoint).  Exact discrete adjoint of the loop above    */
+/* (elastic 2D branch, CPML or no boundary, no free surface / ABS): the        */
+/* forward pass records, per step, the factors that multiply each model        */
+/* coefficient; the backward pass runs the transposed step in reverse order.   */
+/* ------------------------------------------------------------------------- */
+static csr_t csr_transpose(const csr_t *A)
+{
+    csr_t T;
+    long n = A->n, nnz = A->rowptr[n];
+    T.n = n;
+    T.rowptr = (long *)calloc((size_t)n + 1, sizeof(long));
+    T.col = (long *)malloc(sizeof(long) * (size_t)(nnz + 1));
+    T.val = (real *)malloc(sizeof(real) * (size_t)(nnz + 1));
+    for (long k = 0; k < nnz; k++) T.rowptr[A->col[k] + 1]++;
+    for (long i = 0; i < n; i++) T.rowptr[i + 1] += T.rowptr[i];
+    long *fill = (long *)malloc(sizeof(long) * (size_t)(n + 1));
+    memcpy(fill, T.rowptr, sizeof(long) * (size_t)(n + 1));
+    for (long i = 0; i < n; i++)
+        for (long k = A->rowptr[i]; k < A->rowptr[i + 1]; k++) {
+            long d = fill[A->col[k]]++;
+            T.col[d] = i;
+            T.val[d] = A->val[k];
+        }
+    free(fill);
+    return T;
+}
+
+/* y += A x */
+static void spmv_add(const csr_t *A, const real *x, real *y)
+{
+    long n = A->n;
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n; i++) {
+        real t = 0;
+        for (long k = A->rowptr[i]; k < A->rowptr[i + 1]; k++) t += A->val[k] * x[A->col[k]];
+        y[i] += t;
+    }
+}
+
+/* adjoint of  psi' = b psi + a g ; h = g + psi' :  given lh and mu (adjoint of psi'),
+ * lg = (1 + a) lh + a mu ; mu <- b (mu + lh).  Without CPML lg = lh. */
+static void cpml_adjoint(long n, const real *lh, real *lg, real *mu, const real *a, const real *b)
+{
+    if (!a) {
+        memcpy(lg, lh, sizeof(real) * (size_t)n);
+        return;
+    }
+    VLOOP(n)
+    {
+        real m = mu[i] + lh[i];
+        lg[i] = lh[i] + a[i] * m;
+        mu[i] = b[i] * m;
+    }
+}
+
+/*
+ * 2D elastic forward + adjoint.  Model: vp, vs, rho.  Misfit
+ * J = 1/2 sum_{r,t} (seis - obs)^2 with the receivers of oracle_run.
+ * Outputs: seis [nrec][NT], *misfit, gcoef [5][n] = dJ/d(1/rho_x, 1/rho_y, M,
+ * mu, mu_xy) and gmodel [3][n] = dJ/d(vp, vs, rho).  Returns 0, or -2 for an
+ * unsupported configuration (free surface, ABS, not 2D elastic).
+ */
+int oracle_adjoint_elastic2d(const oracle_params *p, const real *vp, const real *vs, const real *rho, int nsrc,
+                             const long *src_idx, const int *src_type, const real *src_sig, int nrec,
+                             const long *rec_idx, const int *rec_type, const real *obs, real *seis, double *misfit,
+                             double *gcoef, double *gmodel)
+{
+#ifdef _OPENMP
+    if (p->nthreads > 0) omp_set_num_threads(p->nthreads);
+#endif
+    if (p->dim != 2 || p->physics != PH_ELASTIC || p->free_surface || p->damping == 1) return -2;
+    grid_t g = {p->NX, p->NY, 1, (long)p->NX * p->NY};
+    long n = g.n;
+    int NT = p->NT;
+    ops_t o;
+    ops_build(&g, p, &o);
+    csr_t Dxf_t = csr_transpose(&o.Dxf), Dxb_t = csr_transpose(&o.Dxb);
+    csr_t Dyf_t = csr_transpose(&o.Dyf), Dyb_t = csr_transpose(&o.Dyb);
+
+    real *M = vnew(n), *mu = vnew(n), *irx = vnew(n), *iry = vnew(n), *muxy = vnew(n);
+    for (long i = 0; i < n; i++) {
+        M[i] = rho[i] * vp[i] * vp[i];
+        mu[i] = rho[i] * vs[i] * vs[i];
+    }
+    inverse_average_2pt(&g, 0, rho, irx);
+    inverse_average_2pt(&g, 1, rho, iry);
+    shear_average(&g, mu, 0, 1, muxy); /* clamps mu in place */
+
+    cpml_coef_t cp;
+    int use_cpml = p->damping == 2;
+    if (use_cpml)
+        cpml_layout(&g, 2, p->width, 0, (real)p->npower, (real)p->fc_cpml, (real)p->vmax_cpml, (real)p->DT,
+                    (real)p->DH, &cp);
+    real *ax = use_cpml ? cp.a[0] : NULL, *bx = use_cpml ? cp.b[0] : NULL;
+    real *axh = use_cpml ? cp.ah[0] : NULL, *bxh = use_cpml ? cp.bh[0] : NULL;
+    real *ay = use_cpml ? cp.a[1] : NULL, *by = use_cpml ? cp.b[1] : NULL;
+    real *ayh = use_cpml ? cp.ah[1] : NULL, *byh = use_cpml ? cp.bh[1] : NULL;
+
+    real *vx = vnew(n), *vy = vnew(n), *sxx = vnew(n), *syy = vnew(n), *sxy = vnew(n);
+    real *upd = vnew(n), *tmp = vnew(n), *vxx = vnew(n), *vyy = vnew(n);
+    real *psi[8];
+    for (int k = 0; k < 8; k++) psi[k] = vnew(n);
+    /* per-step factors: Gx = h0+h1, Gy = h2+h3, h4, h5, Gs = h6+h7 */
+    real *rec = (real *)calloc((size_t)5 * n * NT, sizeof(real));
+    if (!rec) return -3;
+
+    for (int t = 0; t < NT; t++) {
+        real *Gx = rec + ((size_t)t * 5 + 0) * n, *Gy = rec + ((size_t)t * 5 + 1) * n;
+        real *H4 = rec + ((size_t)t * 5 + 2) * n, *H5 = rec + ((size_t)t * 5 + 3) * n;
+        real *Gs = rec + ((size_t)t * 5 + 4) * n;
+        spmv(&o.Dxf, sxx, upd);
+        if (use_cpml) cpml_apply(n, upd, psi[0], axh, bxh);
+        spmv(&o.Dyb, sxy, tmp);
+        if (use_cpml) cpml_apply(n, tmp, psi[1], ay, by);
+        v_add(n, upd, tmp);
+        v_copy(n, Gx, upd);
+        v_mul(n, upd, irx);
+        v_add(n, vx, upd);
+
+        spmv(&o.Dxb, sxy, upd);
+        if (use_cpml) cpml_apply(n, upd, psi[2], ax, bx);
+        spmv(&o.Dyf, syy, tmp);
+        if (use_cpml) cpml_apply(n, tmp, psi[3], ayh, byh);
+        v_add(n, upd, tmp);
+        v_copy(n, Gy, upd);
+        v_mul(n, upd, iry);
+        v_add(n, vy, upd);
+
+        spmv(&o.Dxb, vx, vxx);
+        spmv(&o.Dyb, vy, vyy);
+        if (use_cpml) {
+            cpml_apply(n, vxx, psi[4], ax, bx);
+            cpml_apply(n, vyy, psi[5], ay, by);
+        }
+        v_copy(n, H4, vxx);
+        v_copy(n, H5, vyy);
+        v_copy(n, upd, vxx);
+        v_add(n, upd, vyy);
+        v_mul(n, upd, M);
+        v_add(n, sxx, upd);
+        v_add(n, syy, upd);
+        v_copy(n, upd, vyy);
+        v_mul(n, upd, mu);
+        v_sub2(n, sxx, upd);
+        v_copy(n, upd, vxx);
+        v_mul(n, upd, mu);
+        v_sub2(n, syy, upd);
+
+        spmv(&o.Dyf, vx, upd);
+        if (use_cpml) cpml_apply(n, upd, psi[6], ayh, byh);
+        spmv(&o.Dxf, vy, tmp);
+        if (use_cpml) cpml_apply(n, tmp, psi[7], axh, bxh);
+        v_add(n, upd, tmp);
+        v_copy(n, Gs, upd);
+        v_mul(n, upd, muxy);
+        v_add(n, sxy, upd);
+
+        for (int ty = 1; ty <= 3; ty++)
+            for (int s = 0; s < nsrc; s++) {
+                if (src_type[s] != ty) continue;
+                real v = src_sig[(long)s * NT + t];
+                long i = src_idx[s];
+                if (ty == 1) {
+                    sxx[i] += v;
+                    syy[i] += v;
+                }
+                if (ty == 2) vx[i] += v;
+                if (ty == 3) vy[i] += v;
+            }
+        for (int r = 0; r < nrec; r++) {
+            long i = rec_idx[r];
+            int ty = rec_type[r];
+            real v;
+            if (ty == 1) {
+                v = sxx[i];
+                v += syy[i];
+                v *= (real)0.5;
+            } else
+                v = ty == 2 ? vx[i] : vy[i];
+            seis[(long)r * NT + t] = v;
+        }
+    }
+
+    double J = 0;
+    for (long k = 0; k < (long)nrec * NT; k++) {
+        double d = (double)seis[k] - (double)obs[k];
+        J += 0.5 * d * d;
+    }
+    *misfit = J;
+
+    /* adjoint state */
+    real *lvx = vnew(n), *lvy = vnew(n), *lsxx = vnew(n), *lsyy = vnew(n), *lsxy = vnew(n);
+    real *lh = vnew(n), *lg = vnew(n), *lh2 = vnew(n), *lg2 = vnew(n);
+    real *mu_[8];
+    for (int k = 0; k < 8; k++) mu_[k] = vnew(n);
+    double *G = gcoef;
+    memset(G, 0, sizeof(double) * 5 * (size_t)n);
+
+    for (int t = NT - 1; t >= 0; t--) {
+        const real *Gx = rec + ((size_t)t * 5 + 0) * n, *Gy = rec + ((size_t)t * 5 + 1) * n;
+        const real *H4 = rec + ((size_t)t * 5 + 2) * n, *H5 = rec + ((size_t)t * 5 + 3) * n;
+        const real *Gs = rec + ((size_t)t * 5 + 4) * n;
+        /* receivers of step t: residual into the adjoint fields */
+        for (int r = 0; r < nrec; r++) {
+            long i = rec_idx[r];
+            real d = seis[(long)r * NT + t] - obs[(long)r * NT + t];
+            if (rec_type[r] == 1) {
+                lsxx[i] += (real)0.5 * d;
+                lsyy[i] += (real)0.5 * d;
+            } else if (rec_type[r] == 2)
+                lvx[i] += d;
+            else
+                lvy[i] += d;
+        }
+        /* transpose of the stress update */
+        for (long i = 0; i < n; i++) {
+            G[2 * n + i] += (double)(lsxx[i] + lsyy[i]) * ((double)H4[i] + (double)H5[i]);
+            G[3 * n + i] += -2.0 * (double)lsxx[i] * H5[i] - 2.0 * (double)lsyy[i] * H4[i];
+            G[4 * n + i] += (double)lsxy[i] * Gs[i];
+        }
+        /* h4 (Dxb vx, psi4: ax bx) and h5 (Dyb vy, psi5: ay by) */
+        VLOOP(n) lh[i] = M[i] * (lsxx[i] + lsyy[i]) - (real)2 * mu[i] * lsyy[i];
+        cpml_adjoint(n, lh, lg, mu_[4], ax, bx);
+        spmv_add(&Dxb_t, lg, lvx);
+        VLOOP(n) lh[i] = M[i] * (lsxx[i] + lsyy[i]) - (real)2 * mu[i] * lsxx[i];
+        cpml_adjoint(n, lh, lg, mu_[5], ay, by);
+        spmv_add(&Dyb_t, lg, lvy);
+        /* h6 (Dyf vx, psi6: ayh byh) and h7 (Dxf vy, psi7: axh bxh) */
+        VLOOP(n) lh[i] = muxy[i] * lsxy[i];
+        cpml_adjoint(n, lh, lg, mu_[6], ayh, byh);
+        spmv_add(&Dyf_t, lg, lvx);
+        cpml_adjoint(n, lh, lg2, mu_[7], axh, bxh);
+        spmv_add(&Dxf_t, lg2, lvy);
+
+        /* transpose of the velocity update */
+        for (long i = 0; i < n; i++) {
+            G[0 * n + i] += (double)lvx[i] * Gx[i];
+            G[1 * n + i] += (double)lvy[i] * Gy[i];
+        }
+        VLOOP(n) lh[i] = irx[i] * lvx[i];
+        cpml_adjoint(n, lh, lg, mu_[0], axh, bxh); /* h0 = Dxf sxx */
+        spmv_add(&Dxf_t, lg, lsxx);
+        cpml_adjoint(n, lh, lg, mu_[1], ay, by); /* h1 = Dyb sxy */
+        spmv_add(&Dyb_t, lg, lsxy);
+        VLOOP(n) lh2[i] = iry[i] * lvy[i];
+        cpml_adjoint(n, lh2, lg, mu_[2], ax, bx); /* h2 = Dxb sxy */
+        spmv_add(&Dxb_t, lg, lsxy);
+        cpml_adjoint(n, lh2, lg, mu_[3], ayh, byh); /* h3 = Dyf syy */
+        spmv_add(&Dyf_t, lg, lsyy);
+    }
+
+    /* chain rule to vp, vs, rho (Modelparameter.cpp:336-639 averaging rules) */
+    double *gvp = gmodel, *gvs = gmodel + n, *grho = gmodel + 2 * n;
+    memset(gmodel, 0, sizeof(double) * 3 * (size_t)n);
+    double *gmu_tot = (double *)calloc((size_t)n, sizeof(double));
+    for (long i = 0; i < n; i++) gmu_tot[i] = G[3 * n + i];
+    for (long j = 0; j < n; j++) {
+        long x, y, z;
+        gcoord(&g, j, &x, &y, &z);
+        double mxy = muxy[j];
+        if (mxy == 0) continue;
+        int inx = x + 1 < g.NX, iny = y + 1 < g.NY;
+        long pts[4];
+        int np = 0;
+        double w;
+        pts[np++] = j;
+        if (inx && iny) {
+            pts[np++] = gidx(&g, x + 1, y, 0);
+            pts[np++] = gidx(&g, x, y + 1, 0);
+            pts[np++] = gidx(&g, x + 1, y + 1, 0);
+            w = 0.25;
+        } else if (inx) {
+            pts[np++] = gidx(&g, x + 1, y, 0);
+            w = 0.5;
+        } else if (iny) {
+            pts[np++] = gidx(&g, x, y + 1, 0);
+            w = 0.5;
+        } else
+            w = 1.0;
+        for (int k = 0; k < np; k++) {
+            double m = mu[pts[k]];
+            gmu_tot[pts[k]] += G[4 * n + j] * mxy * mxy * w / (m * m);
+        }
+    }
+    for (long i = 0; i < n; i++) {
+        long x, y, z;
+        gcoord(&g, i, &x, &y, &z);
+        double r = rho[i], a = vp[i], b = vs[i];
+        gvp[i] = G[2 * n + i] * 2.0 * r * a;
+        grho[i] += G[2 * n + i] * a * a;
+        if (r * b * b >= 1.0) { /* mu not clamped */
+            gvs[i] = gmu_tot[i] * 2.0 * r * b;
+            grho[i] += gmu_tot[i] * b * b;
+        }
+        /* 1/rho_x at i and at its left neighbour, 1/rho_y at i and above */
+        double ix = irx[i];
+        if (x + 1 < g.NX) {
+            grho[i] += G[0 * n + i] * (-0.5 * ix * ix);
+            grho[gidx(&g, x + 1, y, 0)] += G[0 * n + i] * (-0.5 * ix * ix);
+        } else
+            grho[i] += G[0 * n + i] * (-ix * ix);
+        double iy = iry[i];
+        if (y + 1 < g.NY) {
+            grho[i] += G[1 * n + i] * (-0.5 * iy * iy);
+            grho[gidx(&g, x, y + 1, 0)] += G[1 * n + i] * (-0.5 * iy * iy);
+        } else
+            grho[i] += G[1 * n + i] * (-iy * iy);
+    }
+    free(gmu_tot);
+
+    free(rec);
+    real *frees[] = {M, mu, irx, iry, muxy, vx, vy, sxx, syy, sxy, upd, tmp, vxx, vyy, lvx, lvy, lsxx, lsyy, lsxy,
+                     lh, lg, lh2, lg2};
+    for (size_t k = 0; k < sizeof(frees) / sizeof(frees[0]); k++) free(frees[k]);
+    for (int k = 0; k < 8; k++) {
+        free(psi[k]);
+        free(mu_[k]);
+    }
+    if (use_cpml)
+        for (int a2 = 0; a2 < 3; a2++) {
+            free(cp.a[a2]);
+            free(cp.b[a2]);
+            free(cp.ah[a2]);
+            free(cp.bh[a2]);
+        }
+    csr_free(&Dxf_t);
+    csr_free(&Dxb_t);
+    csr_free(&Dyf_t);
+    csr_free(&Dyb_t);
+    ops_free(&o);
+    return 0;
+}

@@ -39,6 +39,7 @@ wv_solver::~wv_solver()
     drop_graph();
     delete plain;
     delete e3;
+    adjoint_destroy(adj);
     free_list(acq_allocs);
     free_list(allocs);
     if (ev0) (void)hipEventDestroy(ev0);
@@ -464,6 +465,12 @@ int wv_set_model(wv_solver *s, const float *m0, const float *m1, const float *m2
     const wv::Layout2D &L = s->L;
     const int NX = L.NX, NY = L.NY;
     auto gi = [&](int x, int y) { return (size_t)x + (size_t)y * NX; };
+    if (s->physics == wv::ELASTIC) {
+        // kept for the chain rule of the adjoint gradient
+        s->model_h[0].assign(m0, m0 + (size_t)NX * NY);
+        s->model_h[1].assign(m1, m1 + (size_t)NX * NY);
+        s->model_h[2].assign(m2, m2 + (size_t)NX * NY);
+    }
     auto li = [&](int x, int y) { return (size_t)L.at(x, y); };
     std::vector<std::vector<float>> c(5, std::vector<float>((size_t)L.plane, 0.f));
     auto inv_avg = [&](const std::vector<double> &m, std::vector<float> &out, int ax) {
@@ -655,30 +662,41 @@ int wv_run(wv_solver *s, int t_start, int t_end)
         return r2;
     };
     const int K = s->ktiming ? 0 : s->cfg.steps_per_graph;
-    if (K > 0 && t_end - t >= K + s->cur) {
-        if (s->cur && (rc = eager(1))) return rc;  // graphs start in buffer 0
-        if (!s->gexec || s->gsteps != K) {
-            s->drop_graph();
-            hipGraph_t g;
-            HIP_TRY(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
-            rc = s->enqueue_steps(K, false);
-            hipError_t e = hipStreamEndCapture(s->st, &g);
-            s->cur = 0;
-            if (rc) return rc;
-            if (e != hipSuccess) return fail(WV_EDEVICE, std::string("graph capture: ") + hipGetErrorString(e));
-            e = hipGraphInstantiate(&s->gexec, g, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(g);
-            if (e != hipSuccess) return fail(WV_EDEVICE, std::string("graph instantiate: ") + hipGetErrorString(e));
-            s->gsteps = K;
+    const int ck = adjoint_every(s);  // checkpoint interval of the adjoint (0 = off)
+    adjoint_run_begin(s, t_start);
+    while (t < t_end) {
+        if (ck && t % ck == 0 && (rc = adjoint_checkpoint(s, t))) return rc;
+        int n = t_end - t;
+        if (ck) n = std::min(n, ck - t % ck);
+        if (K > 0 && n >= K + s->cur) {
+            if (s->cur) {  // graphs start in buffer 0
+                if ((rc = eager(1))) return rc;
+                continue;
+            }
+            if (!s->gexec || s->gsteps != K) {
+                s->drop_graph();
+                hipGraph_t g;
+                HIP_TRY(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
+                rc = s->enqueue_steps(K, false);
+                hipError_t e = hipStreamEndCapture(s->st, &g);
+                s->cur = 0;
+                if (rc) return rc;
+                if (e != hipSuccess) return fail(WV_EDEVICE, std::string("graph capture: ") + hipGetErrorString(e));
+                e = hipGraphInstantiate(&s->gexec, g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                if (e != hipSuccess) return fail(WV_EDEVICE, std::string("graph instantiate: ") + hipGetErrorString(e));
+                s->gsteps = K;
+            }
+            // the graph advances the device step counter itself
+            wv::launch_set(s->d_tbase, t, s->st);
+            const int reps = ck ? 1 : n / K;
+            for (int q = 0; q < reps; q++) HIP_TRY(hipGraphLaunch(s->gexec, s->st));
+            t += reps * K;
+            continue;
         }
-        // the graph advances the device step counter itself
-        wv::launch_set(s->d_tbase, t, s->st);
-        while (t_end - t >= K) {
-            HIP_TRY(hipGraphLaunch(s->gexec, s->st));
-            t += K;
-        }
+        if ((rc = eager(n))) return rc;
     }
-    if (t_end > t && (rc = eager(t_end - t))) return rc;
+    adjoint_run_done(s, t_end);
     HIP_TRY(hipEventRecord(s->ev1, s->st));
     s->ran = true;
     return WV_OK;

@@ -1,0 +1,58 @@
+// Adjoint (time-reversed) 2D elastic step and FP16 checkpoints: device
+// structures and launchers (adjoint2d.hip).  The reference has no adjoint;
+// this is the exact discrete adjoint of the forward step of kernels2d.hip.
+#pragma once
+
+#include "wave_internal.hpp"
+
+namespace wv {
+
+// adjoint wavefields (same padded layout and shot stride as the forward fields)
+struct AdjFields {
+    float *lvx, *lvy, *lsxx, *lsyy, *lsxy;
+};
+
+// adjoint of the CPML memory variables, same compact layout as Cpml2D::psx/psy.
+// A kernel reads mx/my (values before its step) on its whole tile + halo and
+// writes the updated values of its own cells to mxn/myn (ping-pong, so no
+// block reads a value another block has already advanced).
+struct AdjPsi {
+    const float *mx[4], *my[4];
+    float *mxn[4], *myn[4];
+};
+
+// forward quantities of one step read by the adjoint kernels:
+// stress phase: f0 = vx, f1 = vy after the velocity update of the step;
+// velocity phase: f0 = sxx, f1 = syy, f2 = sxy at the start of the step;
+// px/py: CPML memory variables at the start of the step.
+struct FwdSnap {
+    const float *f0, *f1, *f2;
+    const float *px[4], *py[4];
+};
+
+// transposed derivative tables: (D^T w)[j] = sum_k t[j][k] w[j + k - H]
+struct TapsT {
+    const float *xf, *xb, *yf, *yb;
+};
+
+// gradients w.r.t. 1/rho_x, 1/rho_y, M, mu, mu_xy (FP64, one plane shared by all shots)
+struct Grad {
+    double *g[5];
+};
+
+void launch_adj_stress(int order, int tile_y, const Layout2D &L, int nshots, const AdjFields &A, const AdjPsi &M,
+                       const FwdSnap &S, const Coef2D &C, const Taps &T, const TapsT &TT, const Cpml2D &P,
+                       const Grad &G, hipStream_t st);
+void launch_adj_velocity(int order, int tile_y, const Layout2D &L, int nshots, const AdjFields &A, const AdjPsi &M,
+                         const FwdSnap &S, const Coef2D &C, const Taps &T, const TapsT &TT, const Cpml2D &P,
+                         const Grad &G, hipStream_t st);
+// adjoint sources of step t (time-major [NT][rec_total]) into the adjoint fields
+void launch_adj_inject(const Layout2D &L, const AdjFields &A, const Acq &Q, const float *adj, int t, hipStream_t st);
+
+// checkpoints: per (field, shot) scale = max|x| / 32768, values stored as FP16
+void launch_ckpt_pack(const float *f, unsigned short *h, unsigned *maxbits, long long per_shot, int nshots,
+                      hipStream_t st);
+void launch_ckpt_unpack(const unsigned short *h, float *f, const unsigned *maxbits, long long per_shot, int nshots,
+                        hipStream_t st);
+
+}  // namespace wv

@@ -245,6 +245,7 @@ struct wv_solver {
 void adjoint_destroy(AdjointState *a);
 // checkpoint interval (0 = checkpointing off)
 int adjoint_every(const wv_solver *s);
-// called by wv_run: before step t (t a multiple of the interval) and after a run
+// called by wv_run: at its start, before step t (t a multiple of the interval), at its end
+void adjoint_run_begin(wv_solver *s, int t_start);
 int adjoint_checkpoint(wv_solver *s, int t);
-void adjoint_run_done(wv_solver *s, int t_start, int t_end);
+void adjoint_run_done(wv_solver *s, int t_end);

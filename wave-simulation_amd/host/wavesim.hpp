@@ -1,0 +1,93 @@
+// C++ host face of the MI355X time loop: the reference's ForwardSolver
+// interface (src/ForwardSolver/ForwardSolver.hpp:26-70, factory
+// ForwardSolverFactory.cpp:4-66) over the C ABI of libwavehip.so, plus the
+// pieces of the reference's input/output the time loop needs: the KEY=VALUE
+// configuration file (Configuration.cpp:45-87), source / receiver text files
+// (AcquisitionSettings.hpp:24-100, 238-270), MatrixMarket vectors and
+// seismograms (IO.hpp:21-164, Seismogram.cpp:82-156), the Ricker wavelet
+// (Ricker.cpp:29-53) and the seismogram resampling (Common.hpp:202-233).
+// Errors throw std::runtime_error, as the reference throws.
+#pragma once
+
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/wave_abi.h"
+
+namespace wavesim {
+
+class Configuration {
+  public:
+    Configuration() = default;
+    explicit Configuration(const std::string &filename) { readFromFile(filename, true); }
+    void readFromFile(const std::string &filename, bool overwrite);
+    void add(const std::string &key, const std::string &value, bool overwrite = true);
+    bool has(const std::string &key) const;
+    std::string getString(const std::string &key) const;
+    double getDouble(const std::string &key) const;
+    long getInt(const std::string &key) const;
+    double getDouble(const std::string &key, double fallback) const { return has(key) ? getDouble(key) : fallback; }
+    long getInt(const std::string &key, long fallback) const { return has(key) ? getInt(key) : fallback; }
+
+  private:
+    std::map<std::string, std::string> map_;
+};
+
+struct SourceRow {
+    int shot, x, y, z, type, wavelet, shape;
+    double fc, amp, tshift;
+};
+struct ReceiverRow {
+    int x, y, z, type;
+};
+
+std::vector<SourceRow> readSources(const std::string &path);
+std::vector<ReceiverRow> readReceivers(const std::string &path);
+std::vector<float> ricker(int NT, double DT, double fc, double amp, double tshift);
+
+// MatrixMarket "array real general" (values column-major)
+std::vector<float> readVectorMtx(const std::string &path, size_t expected);
+void writeMatrixMtx(const std::string &path, const std::vector<float> &rowMajor, int rows, int cols);
+// linear resampling of traces [ntr][NT] from DT to seismoDT
+std::vector<float> resampleTraces(const std::vector<float> &data, int ntr, int NT, double ratio, int &NTout);
+
+// One solver handle: a grid, a model and nshots acquisitions on the GPU.
+class ForwardSolver {
+  public:
+    static std::unique_ptr<ForwardSolver> create(const std::string &dimension, const std::string &type);
+    ~ForwardSolver();
+    ForwardSolver(const ForwardSolver &) = delete;
+    ForwardSolver &operator=(const ForwardSolver &) = delete;
+
+    // initForwardSolver: grid, DT, derivative order, boundaries from the configuration
+    void initForwardSolver(const Configuration &config, int nshots, int stepsPerGraph = 50);
+    // prepareForModelling: model vectors in grid order (x fastest, then z, then depth y)
+    void prepareForModelling(const std::vector<float> &m0, const std::vector<float> &m1, const std::vector<float> &m2);
+    void setAcquisition(int shot, const std::vector<long long> &srcIndex, const std::vector<int> &srcType,
+                        const std::vector<float> &srcSignal, const std::vector<long long> &recIndex,
+                        const std::vector<int> &recType);
+    void resetCPML();
+    void resetWavefields() { resetCPML(); }
+    void run(int t) { run(t, t + 1); }
+    void run(int tStart, int tEnd);
+    void synchronize();
+    double lastRunMs();
+    std::vector<float> seismograms(int shot, int nrec);  // [nrec][NT]
+    int NT() const { return cfg_.NT; }
+    int NX() const { return cfg_.NX; }
+    int NY() const { return cfg_.NY; }
+    int NZ() const { return cfg_.NZ; }
+    wv_solver *handle() { return h_; }
+
+  private:
+    ForwardSolver() = default;
+    wv_solver *h_ = nullptr;
+    wv_config cfg_{};
+};
+
+void check(int rc);
+
+}  // namespace wavesim

@@ -121,6 +121,27 @@ class ForwardSolver:
         _abi.check(_abi.load().wv_last_run_ms(self._h, C.byref(ms)))
         return ms.value
 
+    # -- adjoint (2D elastic) -------------------------------------------------------
+    def set_checkpointing(self, every=50, fp16=True):
+        """Store checkpoints every `every` steps in the following forward runs (resets the wavefields)."""
+        _abi.check(_abi.load().wv_set_checkpointing(self._h, int(every), 1 if fp16 else 0))
+        return self
+
+    def set_adjoint_source(self, shot, adj):
+        a = np.ascontiguousarray(adj, dtype=np.float32).reshape(self._nrec.get(shot, 0), self.cfg.NT)
+        _abi.check(_abi.load().wv_set_adjoint_source(self._h, shot, _fp(a)))
+
+    def run_adjoint(self):
+        _abi.check(_abi.load().wv_run_adjoint(self._h))
+
+    GRADIENTS = ("vp", "vs", "rho", "inv_rho_x", "inv_rho_y", "M", "mu", "mu_xy")
+
+    def gradient(self, param="vp"):
+        k = self.GRADIENTS.index(param) if isinstance(param, str) else int(param)
+        out = np.zeros(self.cfg.NX * self.cfg.NY, dtype=np.float64)
+        _abi.check(_abi.load().wv_get_gradient(self._h, k, out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out
+
     # -- results ------------------------------------------------------------------
     def seismograms(self, shot=0):
         out = np.zeros((self._nrec.get(shot, 0), self.cfg.NT), dtype=np.float32)
