@@ -46,7 +46,129 @@ def parse():
     ap.add_argument("--path", default="fused", choices=["fused", "phased"])
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run a 256 MiB device copy (calibrates the profiler's byte counters)")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the batched TM-EM (64 shots), adjoint (16 shots) and 3D elastic lines")
     return ap.parse_args()
+
+
+def _progress(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def _kernel_stats(sol, L, wavesim):
+    import ctypes as C
+
+    out = []
+    for kind in range(3):
+        name = C.c_char_p()
+        avg = C.c_double()
+        n = C.c_longlong()
+        b = C.c_double()
+        wavesim._abi.check(L.wv_kernel_stats(sol._h, kind, C.byref(name), C.byref(avg), C.byref(n), C.byref(b)))
+        out.append(dict(name=name.value.decode(), avg_ms=avg.value, launches=n.value, bytes=b.value))
+    return out
+
+
+def _roofline(kernels):
+    dom = max(kernels, key=lambda k: k["avg_ms"] * k["launches"])
+    achieved = dom["bytes"] / (dom["avg_ms"] * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom["name"],
+            "kernel_avg_us": round(dom["avg_ms"] * 1e3, 3), "bytes_per_launch": dom["bytes"]}
+
+
+def bench_batched_tmem():
+    """64 TM-EM shots of uneven receiver counts in one handle against the reference-sequence GPU path."""
+    from wavesim import cases
+
+    case = cases.tmem_c3()
+    NT = case.settings["NT"]
+    sol = cases.make_solver(case, steps_per_graph=50)
+    choice = sol.tune(100)
+    sol.reset()
+    sol.run(0, NT)
+    sol.synchronize()
+    t0 = time.perf_counter()
+    sol.reset()
+    sol.run(0, NT)
+    sol.synchronize()
+    tb = time.perf_counter() - t0
+    del sol
+    plain = cases.make_solver(case, path="plain", shots=[case.shots[0]])
+    ps = 300
+    plain.run(0, 20)
+    plain.synchronize()
+    t1 = time.perf_counter()
+    plain.run(20, 20 + ps)
+    plain.synchronize()
+    shot_plain = (time.perf_counter() - t1) * NT / ps
+    n = len(case.shots)
+    return {"workload": "2D TM-EM order 8, 800x400, 3000 steps, 64 shots (20-300 receivers each), one handle",
+            "kernel": choice, "shots_per_s": round(n / tb, 3), "plain_shots_per_s": round(1.0 / shot_plain, 4),
+            "ratio_vs_plain": round(n / tb * shot_plain, 1),
+            "grid_cell_updates_per_s": round(case.cells * NT * n / tb, 1),
+            "plain_sample": f"{ps} steps of one shot, scaled to {NT}"}
+
+
+def bench_adjoint():
+    """Forward + adjoint gradient of 16 batched 2D elastic shots (600x600, 3000 steps)."""
+    import numpy as np
+    from wavesim import cases
+
+    case = cases.elastic_c4()
+    NT = case.settings["NT"]
+    sol = cases.make_solver(case, steps_per_graph=50)
+    choice = sol.tune(100)
+    sol.reset()
+    sol.run(0, NT)
+    sol.synchronize()
+    sol.reset()
+    sol.run(0, NT)
+    t_fwd = sol.last_run_ms()
+    sol.set_checkpointing(50, True)
+    sol.run(0, NT)
+    sol.synchronize()  # first checkpointed run allocates and captures
+    sol.reset()
+    sol.run(0, NT)
+    t_ck = sol.last_run_ms()
+    for k in range(len(case.shots)):
+        sol.set_adjoint_source(k, sol.seismograms(k))  # J = 1/2 ||d||^2
+    sol.run_adjoint()
+    t_adj = sol.last_run_ms()
+    g = sol.gradient("vp")
+    assert np.isfinite(g).all()
+    n = len(case.shots)
+    return {"workload": "2D elastic order 8, 600x600, 3000 steps, 16 shots: forward + adjoint gradient "
+                        "(checkpoints every 50 steps in FP16, FP64 accumulation)",
+            "forward_kernel": choice, "forward_ms": round(t_fwd, 2), "checkpointed_forward_ms": round(t_ck, 2),
+            "adjoint_ms": round(t_adj, 2), "gradient_cost_vs_forward": round((t_ck + t_adj) / t_fwd, 3),
+            "gradient_shots_per_s": round(n / ((t_ck + t_adj) * 1e-3), 3),
+            "forward_shots_per_s": round(n / (t_fwd * 1e-3), 3)}
+
+
+def bench_3d(wavesim, L):
+    """3D elastic, 320^3, 8 shots batched: 200 timed steps of the 2500-step configuration."""
+    from wavesim import cases
+
+    case = cases.elastic_c5()
+    steps = 200
+    sol = cases.make_solver(case, steps_per_graph=50)
+    choice = sol.tune(20)
+    sol.run(0, 50)
+    sol.synchronize()
+    sol.reset()
+    sol.run(0, steps)
+    ms = sol.last_run_ms()
+    wavesim._abi.check(L.wv_set_kernel_timing(sol._h, 1))
+    sol.reset()
+    sol.run(0, 20)
+    sol.synchronize()
+    kern = _kernel_stats(sol, L, wavesim)
+    n = len(case.shots)
+    return {"workload": "3D elastic order 8, 320^3, dx 2 m, 24-cell CPML, 8 shots batched (4320 receivers each); "
+                        f"{steps} of 2500 steps timed",
+            "kernel": choice, "grid_cell_updates_per_s": round(case.cells * steps * n / (ms * 1e-3), 1),
+            "ms_per_step": round(ms / steps, 3), "roofline": _roofline(kern[:2])}
 
 
 def cpu_baseline(case, nsteps):
@@ -109,29 +231,15 @@ def main():
     ms_per_step = wall * 1e3 / args.steps
     step_bytes = BYTES_PER_CELL_UPDATE * case.cells * NT
 
-    # per-kernel timing pass (eager launches between hipEvents on the solver stream)
-    import ctypes as C
-
+    # per-kernel timing pass: eager launches with start/stop events attached to each dispatch
     L = wavesim.load()
     wavesim._abi.check(L.wv_set_kernel_timing(sol._h, 1))
     sol.reset()
     sol.run(0, NT)
     sol.synchronize()
-    kernels = []
-    for kind in range(3):
-        name = C.c_char_p()
-        avg = C.c_double()
-        n = C.c_longlong()
-        b = C.c_double()
-        wavesim._abi.check(L.wv_kernel_stats(sol._h, kind, C.byref(name), C.byref(avg), C.byref(n), C.byref(b)))
-        kernels.append(dict(name=name.value.decode(), avg_ms=avg.value, launches=n.value, bytes=b.value))
+    kernels = _kernel_stats(sol, L, wavesim)
     wavesim._abi.check(L.wv_set_kernel_timing(sol._h, 0))
-    dom = max(kernels, key=lambda k: k["avg_ms"] * k["launches"])
-    achieved = dom["bytes"] / (dom["avg_ms"] * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom["name"],
-                "kernel_avg_us": round(dom["avg_ms"] * 1e3, 3),
-                "bytes_per_launch": dom["bytes"]}
+    roofline = _roofline(kernels)
 
     out = {
         "metric": "grid-cell updates/s (2D elastic order-8 fused, 1000x1000, 4000 steps)",
@@ -169,8 +277,19 @@ def main():
                         "ratio": round((args.steps / wall) * plain_shot_s, 2),
                         "sample": f"{ps} time steps, scaled to {NT}"}
 
+        del plain
+    del sol
+    _progress(f"C2: {value:.4g} cell-updates/s, {ms_per_step:.1f} ms per shot")
+    if not args.no_extras:
+        out["batched_tmem_64_shots"] = bench_batched_tmem()
+        _progress(f"batched TM-EM: {out['batched_tmem_64_shots']['shots_per_s']} shots/s")
+        out["adjoint_16_shots"] = bench_adjoint()
+        _progress(f"adjoint: cost {out['adjoint_16_shots']['gradient_cost_vs_forward']} x forward")
+        out["elastic3d_8_shots"] = bench_3d(wavesim, L)
+        _progress(f"3D: {out['elastic3d_8_shots']['grid_cell_updates_per_s']:.4g} cell-updates/s")
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(case, args.cpu_steps)
+        _progress("cpu baseline done")
     if args.copy_calib:
         import torch
 

@@ -359,25 +359,32 @@ int wv_run_adjoint(wv_solver *s)
             }
         }
         wv::launch_set(s->d_tbase, t0, st);
-        // recompute the segment, keeping what the adjoint steps read
+        // recompute the segment; the forward kernels record what the adjoint steps read
         for (int j = 0; j < n; j++) {
+            wv::Snap2D snap{};
             float *S3 = a->segS + (size_t)j * 3 * a->fld_n;
-            for (int f = 0; f < 3; f++)
-                HIP_TRY(hipMemcpyAsync(S3 + f * a->fld_n, s->field[0][2 + f], a->fld_n * sizeof(float),
-                                       hipMemcpyDeviceToDevice, st));
+            float *V2 = a->segV + (size_t)j * 2 * a->fld_n;
+            for (int f = 0; f < 3; f++) snap.f[f] = S3 + f * a->fld_n;
+            for (int f = 0; f < 2; f++) snap.f[3 + f] = V2 + f * a->fld_n;
             if (np) {
                 float *d = a->segP + (size_t)j * np;
+                float *pp[8];
                 for (int q = 0; q < 8; q++) {
-                    HIP_TRY(hipMemcpyAsync(d, psi[q], pn[q] * sizeof(float), hipMemcpyDeviceToDevice, st));
+                    pp[q] = d;
                     d += pn[q];
                 }
+                // psi order psx0 psx1 psy0 psy1 psx2 psx3 psy2 psy3
+                for (int q = 0; q < 2; q++) {
+                    snap.px[q] = pp[q];
+                    snap.py[q] = pp[2 + q];
+                    snap.px[2 + q] = pp[4 + q];
+                    snap.py[2 + q] = pp[6 + q];
+                }
             }
-            wv::launch_step2d(wv::ELASTIC, order, s->tile_y, L, ns, F, C, T, P, Arec, j, j > 0, st, 0);
-            float *V2 = a->segV + (size_t)j * 2 * a->fld_n;
-            for (int f = 0; f < 2; f++)
-                HIP_TRY(hipMemcpyAsync(V2 + f * a->fld_n, s->field[0][f], a->fld_n * sizeof(float),
-                                       hipMemcpyDeviceToDevice, st));
-            wv::launch_step2d(wv::ELASTIC, order, s->tile_y, L, ns, F, C, T, P, Arec, j, 0, st, 1);
+            wv::launch_step2d(wv::ELASTIC, order, s->tile_y, L, ns, F, C, T, P, Arec, j, j > 0, st, 0, nullptr,
+                              nullptr, &snap);
+            wv::launch_step2d(wv::ELASTIC, order, s->tile_y, L, ns, F, C, T, P, Arec, j, 0, st, 1, nullptr, nullptr,
+                              &snap);
         }
         // adjoint steps of the segment, last to first
         for (int j = n - 1; j >= 0; j--) {

@@ -83,7 +83,7 @@ __device__ __forceinline__ int shot_block(const Acq &A) { return (blockIdx.z * A
 // ---------------------------------------------------------------------------
 template <int H, int TY>
 __global__ __launch_bounds__(NTHREADS) void el2d_velocity(Layout2D L, Fields2D F, Coef2D C, Taps T, Cpml2D P, Acq A,
-                                                          int tofs, int pend)
+                                                          int tofs, int pend, Snap2D S)
 {
     using TL = Tile<H, TY>;
     constexpr int R = TY / BLOCK_Y;
@@ -155,6 +155,22 @@ __global__ __launch_bounds__(NTHREADS) void el2d_velocity(Layout2D L, Fields2D F
         float u2 = dx<H>(s_xy, Rr, tx, cxb);
         float t2 = dy<H>(s_yy, Rr, tx, cyf);
         if (!col.in) continue;
+        if (S.f[0]) {
+            const long long o = so + L.at(col.x, y);
+            S.f[0][o] = s_xx[Rr][tx + H];
+            S.f[1][o] = s_yy[Rr][tx + H];
+            S.f[2][o] = s_xy[Rr][tx + H];
+            if (col.xi >= 0) {
+                const long long pi = ((long long)shot * L.NY + y) * W2 + col.xi;
+                S.px[0][pi] = p0[r];
+                S.px[1][pi] = p1[r];
+            }
+            if (yi[r] >= 0) {
+                const long long pi = ((long long)shot * W2 + yi[r]) * L.NX + col.x;
+                S.py[0][pi] = q0[r];
+                S.py[1][pi] = q1[r];
+            }
+        }
         if (col.xi >= 0) {
             const long long pi = ((long long)shot * L.NY + y) * W2 + col.xi;
             P.psx[0][pi] = cpml(u, p0[r], col.axh, col.bxh);
@@ -172,12 +188,16 @@ __global__ __launch_bounds__(NTHREADS) void el2d_velocity(Layout2D L, Fields2D F
         u2 += t2;
         u2 *= cry[r];
         vy[o] = ovy[r] + u2;
+        if (S.f[3]) {
+            S.f[3][so + o] = vx[o];
+            S.f[4][so + o] = vy[o];
+        }
     }
 }
 
 template <int H, int TY>
 __global__ __launch_bounds__(NTHREADS) void el2d_stress(Layout2D L, Fields2D F, Coef2D C, Taps T, Cpml2D P, Acq A,
-                                                        int tofs, int pend)
+                                                        int tofs, int pend, Snap2D S)
 {
     (void)pend;
     using TL = Tile<H, TY>;
@@ -247,6 +267,18 @@ __global__ __launch_bounds__(NTHREADS) void el2d_stress(Layout2D L, Fields2D F, 
         float u2 = dy<H>(s_vx, Rr, tx, cyf);
         float t2 = dx<H>(s_vy, Rr, tx, cxf);
         if (!col.in) continue;
+        if (S.px[2]) {
+            if (col.xi >= 0) {
+                const long long pi = ((long long)shot * L.NY + y) * W2 + col.xi;
+                S.px[2][pi] = p2[r];
+                S.px[3][pi] = p3[r];
+            }
+            if (yi[r] >= 0) {
+                const long long pi = ((long long)shot * W2 + yi[r]) * L.NX + col.x;
+                S.py[2][pi] = q2[r];
+                S.py[3][pi] = q3[r];
+            }
+        }
         if (col.xi >= 0) {
             const long long pi = ((long long)shot * L.NY + y) * W2 + col.xi;
             P.psx[2][pi] = cpml(vxx, p2[r], col.ax, col.bx);
@@ -590,14 +622,15 @@ void go(Kern k, dim3 grid, dim3 block, hipStream_t st, hipEvent_t e0, hipEvent_t
 
 template <int H, int TY>
 void launch_h(int physics, int which, dim3 grid, const Layout2D &L, const Fields2D &F, const Coef2D &C, const Taps &T,
-              const Cpml2D &P, const Acq &A, int tofs, int pend, hipStream_t st, hipEvent_t e0, hipEvent_t e1)
+              const Cpml2D &P, const Acq &A, int tofs, int pend, hipStream_t st, hipEvent_t e0, hipEvent_t e1,
+              const Snap2D &S)
 {
     dim3 block(TILE_X, BLOCK_Y);
     if (physics == ELASTIC) {
         if (which == 0)
-            go(el2d_velocity<H, TY>, grid, block, st, e0, e1, L, F, C, T, P, A, tofs, pend);
+            go(el2d_velocity<H, TY>, grid, block, st, e0, e1, L, F, C, T, P, A, tofs, pend, S);
         else
-            go(el2d_stress<H, TY>, grid, block, st, e0, e1, L, F, C, T, P, A, tofs, pend);
+            go(el2d_stress<H, TY>, grid, block, st, e0, e1, L, F, C, T, P, A, tofs, pend, S);
     } else if (physics == ACOUSTIC) {
         if (which == 0)
             go(ac2d_velocity<H, TY>, grid, block, st, e0, e1, L, F, C, T, P, A, tofs, pend);
@@ -614,30 +647,31 @@ void launch_h(int physics, int which, dim3 grid, const Layout2D &L, const Fields
 template <int H>
 void launch_ty2(int ty, int physics, int which, const Layout2D &L, int nshots, const Fields2D &F, const Coef2D &C,
                 const Taps &T, const Cpml2D &P, const Acq &A, int tofs, int pend, hipStream_t st, hipEvent_t e0,
-                hipEvent_t e1)
+                hipEvent_t e1, const Snap2D &S)
 {
     dim3 grid((L.NX + TILE_X - 1) / TILE_X, (L.NY + ty - 1) / ty, nshots);
     if (ty == 8)
-        launch_h<H, 8>(physics, which, grid, L, F, C, T, P, A, tofs, pend, st, e0, e1);
+        launch_h<H, 8>(physics, which, grid, L, F, C, T, P, A, tofs, pend, st, e0, e1, S);
     else if (ty == 32)
-        launch_h<H, 32>(physics, which, grid, L, F, C, T, P, A, tofs, pend, st, e0, e1);
+        launch_h<H, 32>(physics, which, grid, L, F, C, T, P, A, tofs, pend, st, e0, e1, S);
     else
-        launch_h<H, 16>(physics, which, grid, L, F, C, T, P, A, tofs, pend, st, e0, e1);
+        launch_h<H, 16>(physics, which, grid, L, F, C, T, P, A, tofs, pend, st, e0, e1, S);
 }
 
 }  // namespace
 
 void launch_step2d(int physics, int order, int tile_y, const Layout2D &L, int nshots, const Fields2D &F,
                    const Coef2D &C, const Taps &T, const Cpml2D &P, const Acq &A, int tofs, int pend,
-                   hipStream_t stream, int which, hipEvent_t e0, hipEvent_t e1)
+                   hipStream_t stream, int which, hipEvent_t e0, hipEvent_t e1, const Snap2D *snap)
 {
+    const Snap2D S = snap ? *snap : Snap2D{};
     switch (order / 2) {
-    case 1: launch_ty2<1>(tile_y, physics, which, L, nshots, F, C, T, P, A, tofs, pend, stream, e0, e1); break;
-    case 2: launch_ty2<2>(tile_y, physics, which, L, nshots, F, C, T, P, A, tofs, pend, stream, e0, e1); break;
-    case 3: launch_ty2<3>(tile_y, physics, which, L, nshots, F, C, T, P, A, tofs, pend, stream, e0, e1); break;
-    case 4: launch_ty2<4>(tile_y, physics, which, L, nshots, F, C, T, P, A, tofs, pend, stream, e0, e1); break;
-    case 5: launch_ty2<5>(tile_y, physics, which, L, nshots, F, C, T, P, A, tofs, pend, stream, e0, e1); break;
-    case 6: launch_ty2<6>(tile_y, physics, which, L, nshots, F, C, T, P, A, tofs, pend, stream, e0, e1); break;
+    case 1: launch_ty2<1>(tile_y, physics, which, L, nshots, F, C, T, P, A, tofs, pend, stream, e0, e1, S); break;
+    case 2: launch_ty2<2>(tile_y, physics, which, L, nshots, F, C, T, P, A, tofs, pend, stream, e0, e1, S); break;
+    case 3: launch_ty2<3>(tile_y, physics, which, L, nshots, F, C, T, P, A, tofs, pend, stream, e0, e1, S); break;
+    case 4: launch_ty2<4>(tile_y, physics, which, L, nshots, F, C, T, P, A, tofs, pend, stream, e0, e1, S); break;
+    case 5: launch_ty2<5>(tile_y, physics, which, L, nshots, F, C, T, P, A, tofs, pend, stream, e0, e1, S); break;
+    case 6: launch_ty2<6>(tile_y, physics, which, L, nshots, F, C, T, P, A, tofs, pend, stream, e0, e1, S); break;
     default: break;
     }
 }

@@ -110,13 +110,24 @@ struct Acq {
     const int *tbase;           // device step counter base
 };
 
+// Forward quantities a 2D elastic step hands to the adjoint's recomputation
+// (all pointers null when not recording): stresses at the start of the step
+// (f[0..2] = sxx syy sxy, own cells), velocities after the velocity update
+// (f[3..4] = vx vy), CPML memory variables at the start of the step, same
+// layouts as the fields / Cpml2D.
+struct Snap2D {
+    float *f[5];
+    float *px[4], *py[4];
+};
+
 // Launchers (kernels2d.hip).  All launches go to `stream`.
 // which = 0: velocity-type phase (pend = apply the velocity-type sources /
 // receivers of the previous step first); which = 1: stress-type phase.
 // tile_y in {8, 16, 32}.  e0/e1 (optional) receive the kernel's own start/end timestamps.
 void launch_step2d(int physics, int order, int tile_y, const Layout2D &L, int nshots, const Fields2D &F,
                    const Coef2D &C, const Taps &T, const Cpml2D &P, const Acq &A, int tofs, int pend,
-                   hipStream_t stream, int which, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
+                   hipStream_t stream, int which, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr,
+                   const Snap2D *snap = nullptr);
 // velocity-type sources / receivers of step tbase + tofs (end of a run segment)
 void launch_flush2d(int physics, const Layout2D &L, const Fields2D &F, const Acq &A, int tofs, hipStream_t stream);
 void launch_abs_velocity2d(int physics, const Layout2D &L, int nshots, const Fields2D &F, const Coef2D &C,
