@@ -55,11 +55,11 @@ def _progress(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def _kernel_stats(sol, L, wavesim):
+def _kernel_stats(sol, L, wavesim, kinds=(0, 1, 2)):
     import ctypes as C
 
     out = []
-    for kind in range(3):
+    for kind in kinds:
         name = C.c_char_p()
         avg = C.c_double()
         n = C.c_longlong()
@@ -138,12 +138,23 @@ def bench_adjoint():
     g = sol.gradient("vp")
     assert np.isfinite(g).all()
     n = len(case.shots)
+    # per-kernel split of the adjoint pass (dispatch-attached events, eager)
+    import wavesim
+
+    L = wavesim.load()
+    wavesim._abi.check(L.wv_set_kernel_timing(sol._h, 1))
+    sol.run_adjoint()
+    sol.synchronize()
+    kern = _kernel_stats(sol, L, wavesim, kinds=(3, 4, 5, 6))
+    wavesim._abi.check(L.wv_set_kernel_timing(sol._h, 0))
     return {"workload": "2D elastic order 8, 600x600, 3000 steps, 16 shots: forward + adjoint gradient "
                         "(checkpoints every 50 steps in FP16, FP64 accumulation)",
             "forward_kernel": choice, "forward_ms": round(t_fwd, 2), "checkpointed_forward_ms": round(t_ck, 2),
             "adjoint_ms": round(t_adj, 2), "gradient_cost_vs_forward": round((t_ck + t_adj) / t_fwd, 3),
             "gradient_shots_per_s": round(n / ((t_ck + t_adj) * 1e-3), 3),
-            "forward_shots_per_s": round(n / (t_fwd * 1e-3), 3)}
+            "forward_shots_per_s": round(n / (t_fwd * 1e-3), 3),
+            "adjoint_kernels": [{"name": k["name"], "avg_us": round(k["avg_ms"] * 1e3, 2), "launches": k["launches"],
+                                 "GB_per_s": round(k["bytes"] / max(k["avg_ms"], 1e-12) / 1e6, 1)} for k in kern]}
 
 
 def bench_3d(wavesim, L):

@@ -126,7 +126,9 @@ int wv_last_run_ms(wv_solver *s, double *ms);
  * kernel eagerly with start/stop hipEvents attached to the dispatch itself
  * (hipExtLaunchKernelGGL) on the handle's stream and accumulates the
  * elapsed time per kernel kind; wv_kernel_stats then reports, for kind
- * 0 = velocity-type phase, 1 = stress-type phase, 2 = end-of-segment source/receiver flush:
+ * 0 = velocity-type phase, 1 = stress-type phase, 2 = end-of-segment source/receiver flush
+ * (and for the 2D elastic adjoint pass, wv_run_adjoint: 3 = recomputed velocity phase,
+ * 4 = recomputed stress phase, 5 = adjoint stress step, 6 = adjoint velocity step):
  * the kernel name, its average duration [ms] per launch, the number of
  * launches, and the algorithmic HBM bytes one launch must move (every field
  * and coefficient of the batch read once, updated fields written once, CPML

@@ -212,6 +212,16 @@ double wv_solver::kernel_bytes(int kind) const
         for (auto &sh : shots) nsrc += sh.sidx.size();
         return 12.0 * nsrc + 8.0 * rec_total;
     }
+    if (kind >= 3) {
+        // adjoint pass (2D elastic): per shot-cell field bytes, per cell coefficient bytes
+        //   3 recomputed velocity phase: forward velocity phase + 5 snapshot fields written
+        //   4 recomputed stress phase  : forward stress phase
+        //   5 adjoint stress step      : r lsxx lsyy lsxy vx vy, r+w lvx lvy, 3 FP64 gradient updates (16 B)
+        //   6 adjoint velocity step    : r lvx lvy sxx syy sxy, r+w lsxx lsyy lsxy, 2 FP64 gradient updates
+        static const double per_shot[4] = {4.0 * (7 + 5), 4.0 * 8, 4.0 * 9 + 48, 4.0 * 11 + 32};
+        static const double per_cell[4] = {4.0 * 2, 4.0 * 3, 4.0 * 3, 4.0 * 2};
+        return cells * (B * per_shot[kind - 3] + per_cell[kind - 3]);
+    }
     // {field reads+writes per shot-cell, coefficient reads per cell, psi per x-frame cell, psi per y-frame cell}
     static const int phased[3][2][4] = {
         {{5, 2, 1, 1}, {4, 1, 1, 1}},  // acoustic: velocity, pressure
@@ -847,9 +857,11 @@ int wv_set_kernel_timing(wv_solver *s, int on)
     if (!s || !s->inited) return fail(WV_ESTATE, "not initialised");
     if (s->plain) return fail(WV_EINVAL, "kernel timing is provided by the tiled kernels");
     s->ktiming = on != 0;
-    for (int k = 0; k < 3; k++) {
+    for (int k = 0; k < 7; k++) {
         s->kms[k] = 0;
         s->klaunch[k] = 0;
+    }
+    for (int k = 0; k < 3; k++) {
         if (s->e3) {
             s->e3->kms[k] = 0;
             s->e3->klaunch[k] = 0;
@@ -862,11 +874,13 @@ int wv_kernel_stats(wv_solver *s, int kind, const char **name, double *avg_ms, l
                     double *bytes_per_launch)
 {
     if (!s || !s->inited) return fail(WV_ESTATE, "not initialised");
-    if (kind < 0 || kind > 2) return fail(WV_EINVAL, "unknown kernel kind");
+    if (kind < 0 || kind > 6 || (s->e3 && kind > 2)) return fail(WV_EINVAL, "unknown kernel kind");
     if (s->plain) return fail(WV_EINVAL, "kernel timing is provided by the tiled kernels");
     static const char *names[3][3] = {{"ac2d_velocity", "ac2d_pressure", "flush2d"},
                                       {"el2d_velocity", "el2d_stress", "flush2d"},
                                       {"tm2d_h", "tm2d_e", "flush2d"}};
+    static const char *adj_names[4] = {"el2d_velocity (recompute)", "el2d_stress (recompute)", "adj_el2d_stress",
+                                       "adj_el2d_velocity"};
     HIP_TRY(hipStreamSynchronize(s->st));
     if (s->e3) {
         if (name) *name = s->e3->kernel_name(kind);
@@ -875,7 +889,9 @@ int wv_kernel_stats(wv_solver *s, int kind, const char **name, double *avg_ms, l
         if (bytes_per_launch) *bytes_per_launch = s->e3->kernel_bytes(kind);
         return WV_OK;
     }
-    if (name) *name = (s->variant == V_SINGLE && kind < 2) ? (kind == 0 ? "el2d_fused" : "-") : names[s->physics][kind];
+    if (name)
+        *name = kind >= 3 ? adj_names[kind - 3]
+                          : ((s->variant == V_SINGLE && kind < 2) ? (kind == 0 ? "el2d_fused" : "-") : names[s->physics][kind]);
     if (launches) *launches = s->klaunch[kind];
     if (avg_ms) *avg_ms = s->klaunch[kind] ? s->kms[kind] / s->klaunch[kind] : 0.0;
     if (bytes_per_launch) *bytes_per_launch = s->kernel_bytes(kind);

@@ -95,6 +95,54 @@ std::vector<float> transpose_taps(const std::vector<float> &t, int H)
     return r;
 }
 
+// start/stop events attached to timed dispatches, collected in batches
+struct KernelTimer {
+    wv_solver *s;
+    std::vector<hipEvent_t> ev;
+    std::vector<int> kind;
+    size_t used = 0;
+    explicit KernelTimer(wv_solver *sv) : s(sv) {}
+    ~KernelTimer()
+    {
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    }
+    int init(size_t pairs)
+    {
+        ev.resize(2 * pairs);
+        kind.resize(pairs);
+        for (auto &e : ev) HIP_TRY(hipEventCreate(&e));
+        return 0;
+    }
+    int flush()
+    {
+        if (!used) return 0;
+        HIP_TRY(hipEventSynchronize(ev[2 * used - 1]));
+        for (size_t i = 0; i < used; i++) {
+            float ms = 0;
+            HIP_TRY(hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]));
+            s->kms[kind[i]] += ms;
+            s->klaunch[kind[i]] += 1;
+        }
+        used = 0;
+        return 0;
+    }
+    // events for the next dispatch of kind k (nullptr when timing is off)
+    int next(int k, hipEvent_t *e0, hipEvent_t *e1)
+    {
+        *e0 = *e1 = nullptr;
+        if (ev.empty()) return 0;
+        if (used == kind.size()) {
+            int rc = flush();
+            if (rc) return rc;
+        }
+        kind[used] = k;
+        *e0 = ev[2 * used];
+        *e1 = ev[2 * used + 1];
+        used++;
+        return 0;
+    }
+};
+
 }  // namespace
 
 void adjoint_destroy(AdjointState *a)
@@ -338,6 +386,9 @@ int wv_run_adjoint(wv_solver *s)
     for (int f = 0; f < 5; f++) G.g[f] = a->grad[f];
     int cv = 0, cs = 0;  // ping-pong buffers of the velocity-set / stress-set adjoint memory variables
     const int order = s->cfg.spatial_fd_order;
+    KernelTimer kt(s);
+    if (s->ktiming && (rc = kt.init(64))) return rc;
+    hipEvent_t e0, e1;
     HIP_TRY(hipEventRecord(s->ev0, st));
 
     for (int k = a->nck - 1; k >= 0; k--) {
@@ -381,10 +432,10 @@ int wv_run_adjoint(wv_solver *s)
                     snap.py[2 + q] = pp[6 + q];
                 }
             }
-            wv::launch_step2d(wv::ELASTIC, order, s->tile_y, L, ns, F, C, T, P, Arec, j, j > 0, st, 0, nullptr,
-                              nullptr, &snap);
-            wv::launch_step2d(wv::ELASTIC, order, s->tile_y, L, ns, F, C, T, P, Arec, j, 0, st, 1, nullptr, nullptr,
-                              &snap);
+            if ((rc = kt.next(3, &e0, &e1))) return rc;
+            wv::launch_step2d(wv::ELASTIC, order, s->tile_y, L, ns, F, C, T, P, Arec, j, j > 0, st, 0, e0, e1, &snap);
+            if ((rc = kt.next(4, &e0, &e1))) return rc;
+            wv::launch_step2d(wv::ELASTIC, order, s->tile_y, L, ns, F, C, T, P, Arec, j, 0, st, 1, e0, e1, &snap);
         }
         // adjoint steps of the segment, last to first
         for (int j = n - 1; j >= 0; j--) {
@@ -425,7 +476,8 @@ int wv_run_adjoint(wv_solver *s)
                 M.mxn[q] = np ? a->mu[bx ^ 1][sx] : nullptr;
                 M.myn[q] = np ? a->mu[by ^ 1][sy] : nullptr;
             }
-            wv::launch_adj_stress(order, s->tile_y == 16 ? 16 : 8, L, ns, AF, M, SV, C, T, TT, P, G, st);
+            if ((rc = kt.next(5, &e0, &e1))) return rc;
+            wv::launch_adj_stress(order, s->tile_y == 16 ? 16 : 8, L, ns, AF, M, SV, C, T, TT, P, G, st, e0, e1);
             cs ^= 1;
             for (int q = 2; q < 4; q++) {
                 const int sx = 2 + q, sy = 4 + q;
@@ -434,11 +486,13 @@ int wv_run_adjoint(wv_solver *s)
                 M.mxn[q] = np ? a->mu[cs ^ 1][sx] : nullptr;
                 M.myn[q] = np ? a->mu[cs ^ 1][sy] : nullptr;
             }
-            wv::launch_adj_velocity(order, s->tile_y == 16 ? 16 : 8, L, ns, AF, M, SS, C, T, TT, P, G, st);
+            if ((rc = kt.next(6, &e0, &e1))) return rc;
+            wv::launch_adj_velocity(order, s->tile_y == 16 ? 16 : 8, L, ns, AF, M, SS, C, T, TT, P, G, st, e0, e1);
             cv ^= 1;
         }
     }
     HIP_TRY(hipGetLastError());
+    if ((rc = kt.flush())) return rc;
     HIP_TRY(hipEventRecord(s->ev1, st));
     s->ran = true;
     a->grad_ready = true;

@@ -40,12 +40,13 @@ struct Grad {
     double *g[5];
 };
 
+// e0/e1 (optional) receive the kernel's own start/end timestamps
 void launch_adj_stress(int order, int tile_y, const Layout2D &L, int nshots, const AdjFields &A, const AdjPsi &M,
                        const FwdSnap &S, const Coef2D &C, const Taps &T, const TapsT &TT, const Cpml2D &P,
-                       const Grad &G, hipStream_t st);
+                       const Grad &G, hipStream_t st, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 void launch_adj_velocity(int order, int tile_y, const Layout2D &L, int nshots, const AdjFields &A, const AdjPsi &M,
                          const FwdSnap &S, const Coef2D &C, const Taps &T, const TapsT &TT, const Cpml2D &P,
-                         const Grad &G, hipStream_t st);
+                         const Grad &G, hipStream_t st, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 // adjoint sources of step t (time-major [NT][rec_total]) into the adjoint fields
 void launch_adj_inject(const Layout2D &L, const AdjFields &A, const Acq &Q, const float *adj, int t, hipStream_t st);
 

@@ -52,9 +52,10 @@ __global__ __launch_bounds__(NTHREADS) void adj_el2d_stress(Layout2D L, AdjField
     __shared__ float s_g7[TL::RH][TL::RW];
     __shared__ float s_vx[TL::RH][TL::RW];
     __shared__ float s_vy[TL::RH][TL::RW];
-    const int shot = blockIdx.z;
+    const Blk B = xcd_block();
+    const int shot = B.shot;
     const long long so = (long long)shot * L.plane;
-    const int x0 = blockIdx.x * TILE_X, y0 = blockIdx.y * TY;
+    const int x0 = B.bx * TILE_X, y0 = B.by * TY;
     const int tid = threadIdx.y * TILE_X + threadIdx.x, tx = threadIdx.x;
     const int W2 = 2 * P.W;
 
@@ -170,9 +171,10 @@ __global__ __launch_bounds__(NTHREADS) void adj_el2d_velocity(Layout2D L, AdjFie
     __shared__ float s_xx[TL::RH][TL::RW];
     __shared__ float s_yy[TL::RH][TL::RW];
     __shared__ float s_xy[TL::RH][TL::RW];
-    const int shot = blockIdx.z;
+    const Blk B = xcd_block();
+    const int shot = B.shot;
     const long long so = (long long)shot * L.plane;
-    const int x0 = blockIdx.x * TILE_X, y0 = blockIdx.y * TY;
+    const int x0 = B.bx * TILE_X, y0 = B.by * TY;
     const int tid = threadIdx.y * TILE_X + threadIdx.x, tx = threadIdx.x;
     const int W2 = 2 * P.W;
 
@@ -328,38 +330,57 @@ __global__ void ckpt_unpack(const __half *__restrict__ h, float *f, const unsign
     }
 }
 
+template <typename Kern, typename... Args>
+void go(Kern k, dim3 grid, dim3 block, hipStream_t st, hipEvent_t e0, hipEvent_t e1, Args... args)
+{
+    if (e0)
+        hipExtLaunchKernelGGL(k, grid, block, 0, st, e0, e1, 0, args...);
+    else
+        hipLaunchKernelGGL(k, grid, block, 0, st, args...);
+}
+
+struct AdjArgs {
+    const AdjFields &A;
+    const AdjPsi &M;
+    const FwdSnap &S;
+    const Coef2D &C;
+    const Taps &T;
+    const TapsT &TT;
+    const Cpml2D &P;
+    const Grad &G;
+};
+
 template <int H, int TY>
-void adj_launch(int which, const Layout2D &L, int nshots, const AdjFields &A, const AdjPsi &M, const FwdSnap &S,
-                const Coef2D &C, const Taps &T, const TapsT &TT, const Cpml2D &P, const Grad &G, hipStream_t st)
+void adj_launch(int which, const Layout2D &L, int nshots, const AdjArgs &a, hipStream_t st, hipEvent_t e0,
+                hipEvent_t e1)
 {
     dim3 grid((L.NX + TILE_X - 1) / TILE_X, (L.NY + TY - 1) / TY, nshots), block(TILE_X, BLOCK_Y);
     if (which == 0)
-        adj_el2d_stress<H, TY><<<grid, block, 0, st>>>(L, A, M, S, C, T, TT, P, G);
+        go(adj_el2d_stress<H, TY>, grid, block, st, e0, e1, L, a.A, a.M, a.S, a.C, a.T, a.TT, a.P, a.G);
     else
-        adj_el2d_velocity<H, TY><<<grid, block, 0, st>>>(L, A, M, S, C, T, TT, P, G);
+        go(adj_el2d_velocity<H, TY>, grid, block, st, e0, e1, L, a.A, a.M, a.S, a.C, a.T, a.TT, a.P, a.G);
 }
 
 template <int H>
-void adj_ty(int which, int ty, const Layout2D &L, int nshots, const AdjFields &A, const AdjPsi &M, const FwdSnap &S,
-            const Coef2D &C, const Taps &T, const TapsT &TT, const Cpml2D &P, const Grad &G, hipStream_t st)
+void adj_ty(int which, int ty, const Layout2D &L, int nshots, const AdjArgs &a, hipStream_t st, hipEvent_t e0,
+            hipEvent_t e1)
 {
     if (ty == 16)
-        adj_launch<H, 16>(which, L, nshots, A, M, S, C, T, TT, P, G, st);
+        adj_launch<H, 16>(which, L, nshots, a, st, e0, e1);
     else
-        adj_launch<H, 8>(which, L, nshots, A, M, S, C, T, TT, P, G, st);
+        adj_launch<H, 8>(which, L, nshots, a, st, e0, e1);
 }
 
-void adj_dispatch(int which, int order, int ty, const Layout2D &L, int nshots, const AdjFields &A, const AdjPsi &M,
-                  const FwdSnap &S, const Coef2D &C, const Taps &T, const TapsT &TT, const Cpml2D &P, const Grad &G,
-                  hipStream_t st)
+void adj_dispatch(int which, int order, int ty, const Layout2D &L, int nshots, const AdjArgs &a, hipStream_t st,
+                  hipEvent_t e0, hipEvent_t e1)
 {
     switch (order / 2) {
-    case 1: adj_ty<1>(which, ty, L, nshots, A, M, S, C, T, TT, P, G, st); break;
-    case 2: adj_ty<2>(which, ty, L, nshots, A, M, S, C, T, TT, P, G, st); break;
-    case 3: adj_ty<3>(which, ty, L, nshots, A, M, S, C, T, TT, P, G, st); break;
-    case 4: adj_ty<4>(which, ty, L, nshots, A, M, S, C, T, TT, P, G, st); break;
-    case 5: adj_ty<5>(which, ty, L, nshots, A, M, S, C, T, TT, P, G, st); break;
-    case 6: adj_ty<6>(which, ty, L, nshots, A, M, S, C, T, TT, P, G, st); break;
+    case 1: adj_ty<1>(which, ty, L, nshots, a, st, e0, e1); break;
+    case 2: adj_ty<2>(which, ty, L, nshots, a, st, e0, e1); break;
+    case 3: adj_ty<3>(which, ty, L, nshots, a, st, e0, e1); break;
+    case 4: adj_ty<4>(which, ty, L, nshots, a, st, e0, e1); break;
+    case 5: adj_ty<5>(which, ty, L, nshots, a, st, e0, e1); break;
+    case 6: adj_ty<6>(which, ty, L, nshots, a, st, e0, e1); break;
     default: break;
     }
 }
@@ -368,16 +389,16 @@ void adj_dispatch(int which, int order, int ty, const Layout2D &L, int nshots, c
 
 void launch_adj_stress(int order, int tile_y, const Layout2D &L, int nshots, const AdjFields &A, const AdjPsi &M,
                        const FwdSnap &S, const Coef2D &C, const Taps &T, const TapsT &TT, const Cpml2D &P,
-                       const Grad &G, hipStream_t st)
+                       const Grad &G, hipStream_t st, hipEvent_t e0, hipEvent_t e1)
 {
-    adj_dispatch(0, order, tile_y, L, nshots, A, M, S, C, T, TT, P, G, st);
+    adj_dispatch(0, order, tile_y, L, nshots, AdjArgs{A, M, S, C, T, TT, P, G}, st, e0, e1);
 }
 
 void launch_adj_velocity(int order, int tile_y, const Layout2D &L, int nshots, const AdjFields &A, const AdjPsi &M,
                          const FwdSnap &S, const Coef2D &C, const Taps &T, const TapsT &TT, const Cpml2D &P,
-                         const Grad &G, hipStream_t st)
+                         const Grad &G, hipStream_t st, hipEvent_t e0, hipEvent_t e1)
 {
-    adj_dispatch(1, order, tile_y, L, nshots, A, M, S, C, T, TT, P, G, st);
+    adj_dispatch(1, order, tile_y, L, nshots, AdjArgs{A, M, S, C, T, TT, P, G}, st, e0, e1);
 }
 
 void launch_adj_inject(const Layout2D &L, const AdjFields &A, const Acq &Q, const float *adj, int t, hipStream_t st)

@@ -156,16 +156,25 @@ struct Pos {
     int shot, x0, z0, ys, ye, sb;
 };
 
+// Block position.  Workgroups are dealt to the 8 XCDs round robin (block b
+// on XCD b mod 8); renumbering them so that each XCD works on one contiguous
+// range of (x tile, z tile, y chunk, shot) keeps shared halos in its L2.
 __device__ __forceinline__ Pos position(const Layout3D &L, int TZ, int nyc, int LY, const Acq &A)
 {
+    const int nbx = gridDim.x, nbz = gridDim.y;
+    const int nblk = nbx * nbz * gridDim.z;
+    const int b = blockIdx.x + nbx * (blockIdx.y + nbz * blockIdx.z);
+    const int xcd = b & 7, k = b >> 3, q = nblk >> 3, rem = nblk & 7;
+    const int t = xcd < rem ? xcd * (q + 1) + k : rem * (q + 1) + (xcd - rem) * q + k;
+    const int bx = t % nbx, u = t / nbx, bz = u % nbz, bzz = u / nbz;
     Pos p;
-    p.shot = blockIdx.z / nyc;
-    const int yc = blockIdx.z - p.shot * nyc;
-    p.x0 = blockIdx.x * TILE_X;
-    p.z0 = blockIdx.y * TZ;
+    p.shot = bzz / nyc;
+    const int yc = bzz - p.shot * nyc;
+    p.x0 = bx * TILE_X;
+    p.z0 = bz * TZ;
     p.ys = yc * LY;
     p.ye = min(L.NY, p.ys + LY);
-    p.sb = (blockIdx.z * A.nby + blockIdx.y) * A.nbx + blockIdx.x;
+    p.sb = (bzz * A.nby + bz) * A.nbx + bx;
     return p;
 }
 

@@ -103,8 +103,10 @@ struct wv_solver {
     // per-kernel timing (wv_set_kernel_timing)
     bool ktiming = false;
     std::vector<hipEvent_t> kev;
-    double kms[3] = {0, 0, 0};
-    long long klaunch[3] = {0, 0, 0};
+    // kinds: 0 velocity phase, 1 stress phase, 2 flush; adjoint pass: 3 recomputed
+    // velocity phase, 4 recomputed stress phase, 5 adjoint stress step, 6 adjoint velocity step
+    double kms[7] = {0, 0, 0, 0, 0, 0, 0};
+    long long klaunch[7] = {0, 0, 0, 0, 0, 0, 0};
 
     ~wv_solver();
     template <typename T>

@@ -2,7 +2,7 @@
 //
 // One time step of the reference (ForwardSolver2Dacoustic.cpp:91-194,
 // ForwardSolver2Delastic.cpp:117-294, ForwardSolver2Dtmem.cpp:108-171) is
-// executed as two launches covering every shot of the batch (blockIdx.z):
+// executed as two launches covering every shot of the batch:
 //   phase 0: velocity-type update (vx, vy | hx, hy) from the stress-type fields;
 //            first injects the velocity-type sources / gathers the
 //            velocity-type receivers left pending by the previous step
@@ -76,7 +76,7 @@ __device__ void block_acquisition(const Acq &A, const Fields2D &F, long long so,
     __syncthreads();
 }
 
-__device__ __forceinline__ int shot_block(const Acq &A) { return (blockIdx.z * A.nby + blockIdx.y) * A.nbx + blockIdx.x; }
+__device__ __forceinline__ int shot_block(const Acq &A, const Blk &B) { return (B.shot * A.nby + B.by) * A.nbx + B.bx; }
 
 // ---------------------------------------------------------------------------
 // elastic P-SV
@@ -90,11 +90,12 @@ __global__ __launch_bounds__(NTHREADS) void el2d_velocity(Layout2D L, Fields2D F
     __shared__ float s_xx[TL::RH][TL::RW];
     __shared__ float s_yy[TL::RH][TL::RW];
     __shared__ float s_xy[TL::RH][TL::RW];
-    const int shot = blockIdx.z;
+    const Blk B = xcd_block();
+    const int shot = B.shot;
     const long long so = (long long)shot * L.plane;
-    const int x0 = blockIdx.x * TILE_X, y0 = blockIdx.y * TY;
+    const int x0 = B.bx * TILE_X, y0 = B.by * TY;
     const int tid = threadIdx.y * TILE_X + threadIdx.x, tx = threadIdx.x;
-    if (pend) block_acquisition<ELASTIC>(A, F, so, shot_block(A), A.tbase[0] + tofs - 1, true);
+    if (pend) block_acquisition<ELASTIC>(A, F, so, shot_block(A, B), A.tbase[0] + tofs - 1, true);
 
     TileRegs<H, TY> rxx, ryy, rxy;
     rxx.load(F.f[2] + so, L, x0, y0, tid);
@@ -204,9 +205,10 @@ __global__ __launch_bounds__(NTHREADS) void el2d_stress(Layout2D L, Fields2D F, 
     constexpr int R = TY / BLOCK_Y;
     __shared__ float s_vx[TL::RH][TL::RW];
     __shared__ float s_vy[TL::RH][TL::RW];
-    const int shot = blockIdx.z;
+    const Blk B = xcd_block();
+    const int shot = B.shot;
     const long long so = (long long)shot * L.plane;
-    const int x0 = blockIdx.x * TILE_X, y0 = blockIdx.y * TY;
+    const int x0 = B.bx * TILE_X, y0 = B.by * TY;
     const int tid = threadIdx.y * TILE_X + threadIdx.x, tx = threadIdx.x;
 
     TileRegs<H, TY> rvx, rvy;
@@ -313,7 +315,7 @@ __global__ __launch_bounds__(NTHREADS) void el2d_stress(Layout2D L, Fields2D F, 
         syy[o] = yy;
         sxy[o] = xy;
     }
-    block_acquisition<ELASTIC>(A, F, so, shot_block(A), A.tbase[0] + tofs, false);
+    block_acquisition<ELASTIC>(A, F, so, shot_block(A, B), A.tbase[0] + tofs, false);
 }
 
 // ---------------------------------------------------------------------------
@@ -326,11 +328,12 @@ __global__ __launch_bounds__(NTHREADS) void ac2d_velocity(Layout2D L, Fields2D F
     using TL = Tile<H, TY>;
     constexpr int R = TY / BLOCK_Y;
     __shared__ float s_p[TL::RH][TL::RW];
-    const int shot = blockIdx.z;
+    const Blk B = xcd_block();
+    const int shot = B.shot;
     const long long so = (long long)shot * L.plane;
-    const int x0 = blockIdx.x * TILE_X, y0 = blockIdx.y * TY;
+    const int x0 = B.bx * TILE_X, y0 = B.by * TY;
     const int tid = threadIdx.y * TILE_X + threadIdx.x, tx = threadIdx.x;
-    if (pend) block_acquisition<ACOUSTIC>(A, F, so, shot_block(A), A.tbase[0] + tofs - 1, true);
+    if (pend) block_acquisition<ACOUSTIC>(A, F, so, shot_block(A, B), A.tbase[0] + tofs - 1, true);
 
     TileRegs<H, TY> rp;
     rp.load(F.f[2] + so, L, x0, y0, tid);
@@ -387,9 +390,10 @@ __global__ __launch_bounds__(NTHREADS) void ac2d_pressure(Layout2D L, Fields2D F
     constexpr int R = TY / BLOCK_Y;
     __shared__ float s_vx[TL::RH][TL::RW];
     __shared__ float s_vy[TL::RH][TL::RW];
-    const int shot = blockIdx.z;
+    const Blk B = xcd_block();
+    const int shot = B.shot;
     const long long so = (long long)shot * L.plane;
-    const int x0 = blockIdx.x * TILE_X, y0 = blockIdx.y * TY;
+    const int x0 = B.bx * TILE_X, y0 = B.by * TY;
     const int tid = threadIdx.y * TILE_X + threadIdx.x, tx = threadIdx.x;
     TileRegs<H, TY> rvx, rvy;
     rvx.load(F.f[0] + so, L, x0, y0, tid);
@@ -437,7 +441,7 @@ __global__ __launch_bounds__(NTHREADS) void ac2d_pressure(Layout2D L, Fields2D F
         if (fs && y == 0) p *= 0.f;
         pr[L.at(col.x, y)] = p;
     }
-    block_acquisition<ACOUSTIC>(A, F, so, shot_block(A), A.tbase[0] + tofs, false);
+    block_acquisition<ACOUSTIC>(A, F, so, shot_block(A, B), A.tbase[0] + tofs, false);
 }
 
 // ---------------------------------------------------------------------------
@@ -453,9 +457,10 @@ __global__ __launch_bounds__(NTHREADS) void tm2d_h(Layout2D L, Fields2D F, Coef2
     using TL = Tile<H, TY>;
     constexpr int R = TY / BLOCK_Y;
     __shared__ float s_ez[TL::RH][TL::RW];
-    const int shot = blockIdx.z;
+    const Blk B = xcd_block();
+    const int shot = B.shot;
     const long long so = (long long)shot * L.plane;
-    const int x0 = blockIdx.x * TILE_X, y0 = blockIdx.y * TY;
+    const int x0 = B.bx * TILE_X, y0 = B.by * TY;
     const int tid = threadIdx.y * TILE_X + threadIdx.x, tx = threadIdx.x;
     TileRegs<H, TY> rez;
     rez.load(F.f[2] + so, L, x0, y0, tid);
@@ -514,9 +519,10 @@ __global__ __launch_bounds__(NTHREADS) void tm2d_e(Layout2D L, Fields2D F, Coef2
     constexpr int R = TY / BLOCK_Y;
     __shared__ float s_hx[TL::RH][TL::RW];
     __shared__ float s_hy[TL::RH][TL::RW];
-    const int shot = blockIdx.z;
+    const Blk B = xcd_block();
+    const int shot = B.shot;
     const long long so = (long long)shot * L.plane;
-    const int x0 = blockIdx.x * TILE_X, y0 = blockIdx.y * TY;
+    const int x0 = B.bx * TILE_X, y0 = B.by * TY;
     const int tid = threadIdx.y * TILE_X + threadIdx.x, tx = threadIdx.x;
     TileRegs<H, TY> rhx, rhy;
     rhx.load(F.f[0] + so, L, x0, y0, tid);
@@ -563,7 +569,7 @@ __global__ __launch_bounds__(NTHREADS) void tm2d_e(Layout2D L, Fields2D F, Coef2
         e *= dmp[r];
         ez[L.at(col.x, y)] = e;
     }
-    block_acquisition<TMEM>(A, F, so, shot_block(A), A.tbase[0] + tofs, false);
+    block_acquisition<TMEM>(A, F, so, shot_block(A, B), A.tbase[0] + tofs, false);
 }
 
 // velocity-type sources / receivers of step t for every shot (end of a run segment)

@@ -111,5 +111,28 @@ __device__ __forceinline__ void load_taps(float *dst, const float *__restrict__ 
     for (int k = 0; k <= 2 * H; k++) dst[k] = tab[(long long)pos * (2 * H + 1) + k];
 }
 
+// Tile and shot of this block.  Workgroups are dealt to the 8 XCDs round
+// robin (block b runs on XCD b mod 8); renumbering them so that each XCD gets
+// one contiguous range of (tile x, tile y, shot) keeps the halo rows and
+// columns a block shares with its neighbours in that XCD's L2.
+struct Blk {
+    int bx, by, shot;
+};
+
+__device__ __forceinline__ Blk xcd_block()
+{
+    const int nbx = gridDim.x, nby = gridDim.y;
+    const int nblk = nbx * nby * gridDim.z;
+    const int b = blockIdx.x + nbx * (blockIdx.y + nby * blockIdx.z);
+    const int xcd = b & 7, k = b >> 3, q = nblk >> 3, rem = nblk & 7;
+    const int t = xcd < rem ? xcd * (q + 1) + k : rem * (q + 1) + (xcd - rem) * q + k;
+    Blk r;
+    r.bx = t % nbx;
+    const int u = t / nbx;
+    r.by = u % nby;
+    r.shot = u / nby;
+    return r;
+}
+
 }  // namespace
 }  // namespace wv

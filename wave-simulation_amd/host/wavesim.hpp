@@ -63,7 +63,7 @@ class ForwardSolver {
     ForwardSolver &operator=(const ForwardSolver &) = delete;
 
     // initForwardSolver: grid, DT, derivative order, boundaries from the configuration
-    void initForwardSolver(const Configuration &config, int nshots, int stepsPerGraph = 50);
+    void initForwardSolver(const Configuration &config, int nshots, int stepsPerGraph = 50, int tileY = 0);
     // prepareForModelling: model vectors in grid order (x fastest, then z, then depth y)
     void prepareForModelling(const std::vector<float> &m0, const std::vector<float> &m1, const std::vector<float> &m2);
     void setAcquisition(int shot, const std::vector<long long> &srcIndex, const std::vector<int> &srcType,
@@ -75,6 +75,8 @@ class ForwardSolver {
     void run(int tStart, int tEnd);
     void synchronize();
     double lastRunMs();
+    // time every kernel variant for nsteps steps, keep the fastest (resets the wavefields)
+    int tune(int nsteps);
     std::vector<float> seismograms(int shot, int nrec);  // [nrec][NT]
     int NT() const { return cfg_.NT; }
     int NX() const { return cfg_.NX; }

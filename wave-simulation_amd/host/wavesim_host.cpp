@@ -198,7 +198,7 @@ ForwardSolver::~ForwardSolver()
     if (h_) wv_destroy(h_);
 }
 
-void ForwardSolver::initForwardSolver(const Configuration &c, int nshots, int stepsPerGraph)
+void ForwardSolver::initForwardSolver(const Configuration &c, int nshots, int stepsPerGraph, int tileY)
 {
     wv_config k{};
     k.NX = (int)c.getInt("NX");
@@ -218,9 +218,18 @@ void ForwardSolver::initForwardSolver(const Configuration &c, int nshots, int st
     k.fc_cpml = c.getDouble("CenterFrequencyCPML", 5.0);
     k.vmax_cpml = c.getDouble("VMaxCPML", 3500.0);
     k.nshots = nshots;
-    k.path = WV_PATH_FUSED;
+    // kernel family: fused (single-pass where it exists, default), phased (two launches), plain (reference sequence)
+    const std::string path = c.has("solverPath") ? c.getString("solverPath") : "fused";
+    if (path == "phased")
+        k.path = WV_PATH_PHASED;
+    else if (path == "plain")
+        k.path = WV_PATH_PLAIN;
+    else if (path == "fused")
+        k.path = WV_PATH_FUSED;
+    else
+        throw std::runtime_error("solverPath must be fused, phased or plain");
     k.steps_per_graph = stepsPerGraph;
-    k.tile_y = 0;
+    k.tile_y = tileY;
     check(wv_init(h_, &k));
     cfg_ = k;
 }
@@ -242,6 +251,13 @@ void ForwardSolver::setAcquisition(int shot, const std::vector<long long> &srcIn
 void ForwardSolver::resetCPML() { check(wv_reset(h_)); }
 void ForwardSolver::run(int tStart, int tEnd) { check(wv_run(h_, tStart, tEnd)); }
 void ForwardSolver::synchronize() { check(wv_synchronize(h_)); }
+
+int ForwardSolver::tune(int nsteps)
+{
+    int choice = -1;
+    check(wv_tune(h_, nsteps, &choice));
+    return choice;
+}
 
 double ForwardSolver::lastRunMs()
 {

@@ -10,7 +10,10 @@
 // receiver files (<ReceiverFilename>.txt, or .shot_<n>.txt with
 // useReceiversPerShot=1); runs every shot on the GPU in one batched handle and
 // writes <SeismogramFilename>.shot_<n>.<component>.mtx per shot and recorded
-// component, resampled to seismoDT (Seismogram.cpp:82-156).
+// component, resampled to seismoDT (Seismogram.cpp:82-156).  Keys of this
+// driver beyond the reference's: stepsPerGraph (time steps per captured graph,
+// default 50), tileY (fixed tile height; default: autotuned) and solverPath
+// (fused | phased | plain).
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -75,7 +78,8 @@ int main(int argc, char **argv)
                 shotNumbers.push_back(s.shot);
         const int nshots = (int)shotNumbers.size();
         auto solver = ForwardSolver::create(dim, type);
-        solver->initForwardSolver(config, nshots, (int)config.getInt("stepsPerGraph", 50));
+        solver->initForwardSolver(config, nshots, (int)config.getInt("stepsPerGraph", 50),
+                                  (int)config.getInt("tileY", 0));
         const int NT = solver->NT();
         const double DT = config.getDouble("DT");
         solver->prepareForModelling(m0, m1, m2);
@@ -104,6 +108,8 @@ int main(int argc, char **argv)
             solver->setAcquisition(k, si, st, sig, ri, rt);
         }
 
+        // kernel variant: autotuned on this problem unless tileY is given
+        if (!config.has("tileY")) solver->tune(std::min(NT, 100));
         solver->resetCPML();
         const auto w0 = std::chrono::steady_clock::now();
         solver->run(0, NT);
