@@ -85,60 +85,58 @@ template <int H, int TY>
 __global__ __launch_bounds__(NTHREADS) void el2d_velocity(Layout2D L, Fields2D F, Coef2D C, Taps T, Cpml2D P, Acq A,
                                                           int tofs, int pend, Snap2D S)
 {
-    using TL = Tile<H, TY>;
-    constexpr int R = TY / BLOCK_Y;
-    __shared__ float s_xx[TL::RH][TL::RW];
-    __shared__ float s_yy[TL::RH][TL::RW];
-    __shared__ float s_xy[TL::RH][TL::RW];
+    using RT = RowTile<H, TY>;
+    constexpr int R = TY / BLOCK_Y, K = 2 * H + 1;
+    __shared__ float s_xx[RT::RH][RT::RW];
+    __shared__ float s_yy[RT::RH][RT::RW];
+    __shared__ float s_xy[RT::RH][RT::RW];
     const Blk B = xcd_block();
     const int shot = B.shot;
     const long long so = (long long)shot * L.plane;
     const int x0 = B.bx * TILE_X, y0 = B.by * TY;
-    const int tid = threadIdx.y * TILE_X + threadIdx.x, tx = threadIdx.x;
+    const int tx = threadIdx.x;
+    const unsigned ux = threadIdx.x;
+    const int wy = __builtin_amdgcn_readfirstlane(threadIdx.y);
     if (pend) block_acquisition<ELASTIC>(A, F, so, shot_block(A, B), A.tbase[0] + tofs - 1, true);
 
-    TileRegs<H, TY> rxx, ryy, rxy;
-    rxx.load(F.f[2] + so, L, x0, y0, tid);
-    ryy.load(F.f[3] + so, L, x0, y0, tid);
-    rxy.load(F.f[4] + so, L, x0, y0, tid);
+    const long long corner = so + L.at(x0 - H, y0 - H);
+    RT rxx, ryy, rxy;
+    rxx.load(F.f[2] + corner, L.pitch, wy, tx);
+    ryy.load(F.f[3] + corner, L.pitch, wy, tx);
+    rxy.load(F.f[4] + corner, L.pitch, wy, tx);
 
-    float *__restrict__ vx = F.f[0] + so;
-    float *__restrict__ vy = F.f[1] + so;
-    const float *__restrict__ irx = C.c[2];
-    const float *__restrict__ iry = C.c[3];
     const Col col = column(L, P, x0 + tx);
-    float cxf[2 * H + 1], cxb[2 * H + 1];
+    float cxf[K], cxb[K];
     load_taps<H>(cxf, T.xf, col.x);
     load_taps<H>(cxb, T.xb, col.x);
     const int W2 = 2 * P.W;
-    const int wy = __builtin_amdgcn_readfirstlane(threadIdx.y);
 
     float ovx[R], ovy[R], crx[R], cry[R], p0[R], p1[R], q0[R], q1[R];
     int yi[R];
 #pragma unroll
     for (int r = 0; r < R; r++) {
         const int y = y0 + wy + r * BLOCK_Y;
-        const long long o = L.at(col.x, y);
-        ovx[r] = vx[o];
-        ovy[r] = vy[o];
-        crx[r] = irx[o];
-        cry[r] = iry[o];
+        const long long row = L.at(x0, y);  // wavefront-uniform
+        ovx[r] = (F.f[0] + (so + row))[ux];
+        ovy[r] = (F.f[1] + (so + row))[ux];
+        crx[r] = (C.c[2] + row)[ux];
+        cry[r] = (C.c[3] + row)[ux];
         p0[r] = p1[r] = q0[r] = q1[r] = 0.f;
-        yi[r] = P.on ? P.yi[y] : -1;
+        yi[r] = (P.on && y < L.NY) ? P.yi[y] : -1;
         if (col.xi >= 0 && y < L.NY) {
-            const long long pi = ((long long)shot * L.NY + y) * W2 + col.xi;
-            p0[r] = P.psx[0][pi];
-            p1[r] = P.psx[1][pi];
+            const long long pr = ((long long)shot * L.NY + y) * W2;
+            p0[r] = (P.psx[0] + pr)[(unsigned)col.xi];
+            p1[r] = (P.psx[1] + pr)[(unsigned)col.xi];
         }
         if (yi[r] >= 0 && col.in) {
-            const long long pi = ((long long)shot * W2 + yi[r]) * L.NX + col.x;
-            q0[r] = P.psy[0][pi];
-            q1[r] = P.psy[1][pi];
+            const long long pr = ((long long)shot * W2 + yi[r]) * L.NX + x0;
+            q0[r] = (P.psy[0] + pr)[ux];
+            q1[r] = (P.psy[1] + pr)[ux];
         }
     }
-    rxx.store(s_xx, tid);
-    ryy.store(s_yy, tid);
-    rxy.store(s_xy, tid);
+    rxx.store(s_xx, wy, tx);
+    ryy.store(s_yy, wy, tx);
+    rxy.store(s_xy, wy, tx);
     __syncthreads();
 
 #pragma unroll
@@ -146,8 +144,8 @@ __global__ __launch_bounds__(NTHREADS) void el2d_velocity(Layout2D L, Fields2D F
         const int ly = wy + r * BLOCK_Y;
         const int y = y0 + ly;
         if (y >= L.NY) break;
-        const float *cyb = T.yb_v + (long long)y * (2 * H + 1);
-        const float *cyf = T.yf_v + (long long)y * (2 * H + 1);
+        const float *cyb = T.yb_v + (long long)y * K;
+        const float *cyf = T.yf_v + (long long)y * K;
         const int Rr = ly + H;
         // vx += 1/rho_x * (CPML(Dxf sxx) + CPML(Dyb sxy))
         float u = dx<H>(s_xx, Rr, tx, cxf);
@@ -156,42 +154,43 @@ __global__ __launch_bounds__(NTHREADS) void el2d_velocity(Layout2D L, Fields2D F
         float u2 = dx<H>(s_xy, Rr, tx, cxb);
         float t2 = dy<H>(s_yy, Rr, tx, cyf);
         if (!col.in) continue;
+        const long long row = so + L.at(x0, y);
         if (S.f[0]) {
-            const long long o = so + L.at(col.x, y);
-            S.f[0][o] = s_xx[Rr][tx + H];
-            S.f[1][o] = s_yy[Rr][tx + H];
-            S.f[2][o] = s_xy[Rr][tx + H];
+            (S.f[0] + row)[ux] = s_xx[Rr][tx + H];
+            (S.f[1] + row)[ux] = s_yy[Rr][tx + H];
+            (S.f[2] + row)[ux] = s_xy[Rr][tx + H];
             if (col.xi >= 0) {
-                const long long pi = ((long long)shot * L.NY + y) * W2 + col.xi;
-                S.px[0][pi] = p0[r];
-                S.px[1][pi] = p1[r];
+                const long long pr = ((long long)shot * L.NY + y) * W2;
+                S.px[0][pr + col.xi] = p0[r];
+                S.px[1][pr + col.xi] = p1[r];
             }
             if (yi[r] >= 0) {
-                const long long pi = ((long long)shot * W2 + yi[r]) * L.NX + col.x;
-                S.py[0][pi] = q0[r];
-                S.py[1][pi] = q1[r];
+                const long long pr = ((long long)shot * W2 + yi[r]) * L.NX + x0;
+                S.py[0][pr + tx] = q0[r];
+                S.py[1][pr + tx] = q1[r];
             }
         }
         if (col.xi >= 0) {
-            const long long pi = ((long long)shot * L.NY + y) * W2 + col.xi;
-            P.psx[0][pi] = cpml(u, p0[r], col.axh, col.bxh);
-            P.psx[1][pi] = cpml(u2, p1[r], col.ax, col.bx);
+            const long long pr = ((long long)shot * L.NY + y) * W2;
+            (P.psx[0] + pr)[(unsigned)col.xi] = cpml(u, p0[r], col.axh, col.bxh);
+            (P.psx[1] + pr)[(unsigned)col.xi] = cpml(u2, p1[r], col.ax, col.bx);
         }
         if (yi[r] >= 0) {
-            const long long pi = ((long long)shot * W2 + yi[r]) * L.NX + col.x;
-            P.psy[0][pi] = cpml(t, q0[r], P.ay[y], P.by[y]);
-            P.psy[1][pi] = cpml(t2, q1[r], P.ayh[y], P.byh[y]);
+            const long long pr = ((long long)shot * W2 + yi[r]) * L.NX + x0;
+            (P.psy[0] + pr)[ux] = cpml(t, q0[r], P.ay[y], P.by[y]);
+            (P.psy[1] + pr)[ux] = cpml(t2, q1[r], P.ayh[y], P.byh[y]);
         }
-        const long long o = L.at(col.x, y);
         u += t;
         u *= crx[r];
-        vx[o] = ovx[r] + u;
+        const float nvx = ovx[r] + u;
         u2 += t2;
         u2 *= cry[r];
-        vy[o] = ovy[r] + u2;
+        const float nvy = ovy[r] + u2;
+        (F.f[0] + row)[ux] = nvx;
+        (F.f[1] + row)[ux] = nvy;
         if (S.f[3]) {
-            S.f[3][so + o] = vx[o];
-            S.f[4][so + o] = vy[o];
+            (S.f[3] + row)[ux] = nvx;
+            (S.f[4] + row)[ux] = nvy;
         }
     }
 }
@@ -201,58 +200,57 @@ __global__ __launch_bounds__(NTHREADS) void el2d_stress(Layout2D L, Fields2D F, 
                                                         int tofs, int pend, Snap2D S)
 {
     (void)pend;
-    using TL = Tile<H, TY>;
-    constexpr int R = TY / BLOCK_Y;
-    __shared__ float s_vx[TL::RH][TL::RW];
-    __shared__ float s_vy[TL::RH][TL::RW];
+    using RT = RowTile<H, TY>;
+    constexpr int R = TY / BLOCK_Y, K = 2 * H + 1;
+    __shared__ float s_vx[RT::RH][RT::RW];
+    __shared__ float s_vy[RT::RH][RT::RW];
     const Blk B = xcd_block();
     const int shot = B.shot;
     const long long so = (long long)shot * L.plane;
     const int x0 = B.bx * TILE_X, y0 = B.by * TY;
-    const int tid = threadIdx.y * TILE_X + threadIdx.x, tx = threadIdx.x;
+    const int tx = threadIdx.x;
+    const unsigned ux = threadIdx.x;
+    const int wy = __builtin_amdgcn_readfirstlane(threadIdx.y);
 
-    TileRegs<H, TY> rvx, rvy;
-    rvx.load(F.f[0] + so, L, x0, y0, tid);
-    rvy.load(F.f[1] + so, L, x0, y0, tid);
+    const long long corner = so + L.at(x0 - H, y0 - H);
+    RT rvx, rvy;
+    rvx.load(F.f[0] + corner, L.pitch, wy, tx);
+    rvy.load(F.f[1] + corner, L.pitch, wy, tx);
 
-    float *__restrict__ sxx = F.f[2] + so;
-    float *__restrict__ syy = F.f[3] + so;
-    float *__restrict__ sxy = F.f[4] + so;
     const Col col = column(L, P, x0 + tx);
-    float cxf[2 * H + 1], cxb[2 * H + 1];
+    float cxf[K], cxb[K];
     load_taps<H>(cxf, T.xf, col.x);
     load_taps<H>(cxb, T.xb, col.x);
     const int W2 = 2 * P.W;
-    const int wy = __builtin_amdgcn_readfirstlane(threadIdx.y);
 
     float oxx[R], oyy[R], oxy[R], cM[R], cmu[R], cmxy[R], p2[R], p3[R], q2[R], q3[R], dmp[R];
     int yi[R];
 #pragma unroll
     for (int r = 0; r < R; r++) {
         const int y = y0 + wy + r * BLOCK_Y;
-        const long long o = L.at(col.x, y);
-        oxx[r] = sxx[o];
-        oyy[r] = syy[o];
-        oxy[r] = sxy[o];
-        cM[r] = C.c[0][o];
-        cmu[r] = C.c[1][o];
-        cmxy[r] = C.c[4][o];
-        dmp[r] = C.damp ? C.damp[o] : 1.f;
+        const long long row = L.at(x0, y);  // wavefront-uniform
+        oxx[r] = (F.f[2] + (so + row))[ux];
+        oyy[r] = (F.f[3] + (so + row))[ux];
+        oxy[r] = (F.f[4] + (so + row))[ux];
+        cM[r] = (C.c[0] + row)[ux];
+        cmu[r] = (C.c[1] + row)[ux];
+        cmxy[r] = (C.c[4] + row)[ux];
+        dmp[r] = C.damp ? (C.damp + row)[ux] : 1.f;
         p2[r] = p3[r] = q2[r] = q3[r] = 0.f;
-        yi[r] = P.on ? P.yi[y] : -1;
+        yi[r] = (P.on && y < L.NY) ? P.yi[y] : -1;
         if (col.xi >= 0 && y < L.NY) {
-            const long long pi = ((long long)shot * L.NY + y) * W2 + col.xi;
-            p2[r] = P.psx[2][pi];
-            p3[r] = P.psx[3][pi];
+            const long long pr = ((long long)shot * L.NY + y) * W2;
+            p2[r] = (P.psx[2] + pr)[(unsigned)col.xi];
+            p3[r] = (P.psx[3] + pr)[(unsigned)col.xi];
         }
         if (yi[r] >= 0 && col.in) {
-            const long long pi = ((long long)shot * W2 + yi[r]) * L.NX + col.x;
-            q2[r] = P.psy[2][pi];
-            q3[r] = P.psy[3][pi];
+            const long long pr = ((long long)shot * W2 + yi[r]) * L.NX + x0;
+            q2[r] = (P.psy[2] + pr)[ux];
+            q3[r] = (P.psy[3] + pr)[ux];
         }
     }
-    rvx.store(s_vx, tid);
-    rvy.store(s_vy, tid);
+    rvx.store(s_vx, wy, tx);
+    rvy.store(s_vy, wy, tx);
     __syncthreads();
 
     const bool fs = C.free_surface != 0;
@@ -261,8 +259,8 @@ __global__ __launch_bounds__(NTHREADS) void el2d_stress(Layout2D L, Fields2D F, 
         const int ly = wy + r * BLOCK_Y;
         const int y = y0 + ly;
         if (y >= L.NY) break;
-        const float *cyb = T.yb + (long long)y * (2 * H + 1);
-        const float *cyf = T.yf + (long long)y * (2 * H + 1);
+        const float *cyb = T.yb + (long long)y * K;
+        const float *cyf = T.yf + (long long)y * K;
         const int Rr = ly + H;
         float vxx = dx<H>(s_vx, Rr, tx, cxb);
         float vyy = dy<H>(s_vy, Rr, tx, cyb);
@@ -271,25 +269,25 @@ __global__ __launch_bounds__(NTHREADS) void el2d_stress(Layout2D L, Fields2D F, 
         if (!col.in) continue;
         if (S.px[2]) {
             if (col.xi >= 0) {
-                const long long pi = ((long long)shot * L.NY + y) * W2 + col.xi;
-                S.px[2][pi] = p2[r];
-                S.px[3][pi] = p3[r];
+                const long long pr = ((long long)shot * L.NY + y) * W2;
+                S.px[2][pr + col.xi] = p2[r];
+                S.px[3][pr + col.xi] = p3[r];
             }
             if (yi[r] >= 0) {
-                const long long pi = ((long long)shot * W2 + yi[r]) * L.NX + col.x;
-                S.py[2][pi] = q2[r];
-                S.py[3][pi] = q3[r];
+                const long long pr = ((long long)shot * W2 + yi[r]) * L.NX + x0;
+                S.py[2][pr + tx] = q2[r];
+                S.py[3][pr + tx] = q3[r];
             }
         }
         if (col.xi >= 0) {
-            const long long pi = ((long long)shot * L.NY + y) * W2 + col.xi;
-            P.psx[2][pi] = cpml(vxx, p2[r], col.ax, col.bx);
-            P.psx[3][pi] = cpml(t2, p3[r], col.axh, col.bxh);
+            const long long pr = ((long long)shot * L.NY + y) * W2;
+            (P.psx[2] + pr)[(unsigned)col.xi] = cpml(vxx, p2[r], col.ax, col.bx);
+            (P.psx[3] + pr)[(unsigned)col.xi] = cpml(t2, p3[r], col.axh, col.bxh);
         }
         if (yi[r] >= 0) {
-            const long long pi = ((long long)shot * W2 + yi[r]) * L.NX + col.x;
-            P.psy[2][pi] = cpml(vyy, q2[r], P.ay[y], P.by[y]);
-            P.psy[3][pi] = cpml(u2, q3[r], P.ayh[y], P.byh[y]);
+            const long long pr = ((long long)shot * W2 + yi[r]) * L.NX + x0;
+            (P.psy[2] + pr)[ux] = cpml(vyy, q2[r], P.ay[y], P.by[y]);
+            (P.psy[3] + pr)[ux] = cpml(u2, q3[r], P.ayh[y], P.byh[y]);
         }
         float xx = oxx[r], yy = oyy[r], xy = oxy[r];
         float u = vxx + vyy;
@@ -310,10 +308,10 @@ __global__ __launch_bounds__(NTHREADS) void el2d_stress(Layout2D L, Fields2D F, 
         xx *= dmp[r];
         yy *= dmp[r];
         xy *= dmp[r];
-        const long long o = L.at(col.x, y);
-        sxx[o] = xx;
-        syy[o] = yy;
-        sxy[o] = xy;
+        const long long row = so + L.at(x0, y);
+        (F.f[2] + row)[ux] = xx;
+        (F.f[3] + row)[ux] = yy;
+        (F.f[4] + row)[ux] = xy;
     }
     block_acquisition<ELASTIC>(A, F, so, shot_block(A, B), A.tbase[0] + tofs, false);
 }

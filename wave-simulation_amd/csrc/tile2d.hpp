@@ -50,6 +50,43 @@ struct TileRegs {
     }
 };
 
+// Rows [y0-H, y0+TY+H) x columns [x0-H, x0+64+H) of one field plane, loaded
+// row by row: wavefront wy takes rows wy, wy+4, ...; lane l holds column l of
+// the row and lanes < 2H also column 64+l.  Every address is a wavefront-
+// uniform row pointer plus the lane, so the loads need no per-element index
+// arithmetic.
+template <int H, int TY>
+struct RowTile {
+    static constexpr int RW = TILE_X + 2 * H;
+    static constexpr int RH = TY + 2 * H;
+    static constexpr int NR = (RH + BLOCK_Y - 1) / BLOCK_Y;
+    float a[NR], b[NR];
+    // g: element (x0-H, y0-H) of the plane
+    __device__ __forceinline__ void load(const float *__restrict__ g, int pitch, int wy, int lane)
+    {
+#pragma unroll
+        for (int k = 0; k < NR; k++) {
+            const int r = wy + k * BLOCK_Y;
+            if (RH % BLOCK_Y == 0 || r < RH) {
+                const float *__restrict__ rp = g + (long long)r * pitch;
+                a[k] = rp[(unsigned)lane];
+                if (lane < 2 * H) b[k] = rp[TILE_X + (unsigned)lane];
+            }
+        }
+    }
+    __device__ __forceinline__ void store(float (*S)[RW], int wy, int lane) const
+    {
+#pragma unroll
+        for (int k = 0; k < NR; k++) {
+            const int r = wy + k * BLOCK_Y;
+            if (RH % BLOCK_Y == 0 || r < RH) {
+                S[r][lane] = a[k];
+                if (lane < 2 * H) S[r][TILE_X + lane] = b[k];
+            }
+        }
+    }
+};
+
 // derivative along x at LDS row r, column tx (taps for offsets -H..H)
 template <int H, int RW>
 __device__ __forceinline__ float dx(const float (*S)[RW], int r, int tx, const float *c)
