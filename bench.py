@@ -43,9 +43,7 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=3000,
                     help="time steps of the CPU baseline sample (about 10-15 s on 16 cores)")
     ap.add_argument("--tile-y", type=int, default=0, help="rows per tile (0 = autotune over all kernel variants)")
-    ap.add_argument("--path", default="fused", choices=["fused", "phased"])
-    ap.add_argument("--copy-calib", action="store_true",
-                    help="also run a 256 MiB device copy (calibrates the profiler's byte counters)")
+    ap.add_argument("--path", default="fused", choices=["fused", "phased", "strip"])
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the batched TM-EM (64 shots), adjoint (16 shots) and 3D elastic lines")
     return ap.parse_args()
@@ -301,15 +299,6 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(case, args.cpu_steps)
         _progress("cpu baseline done")
-    if args.copy_calib:
-        import torch
-
-        a = torch.ones(64 * 1024 * 1024, device="cuda", dtype=torch.float32)
-        bb = torch.empty_like(a)
-        for _ in range(5):
-            bb.copy_(a)
-        torch.cuda.synchronize()
-        out["copy_calib_bytes"] = 2 * a.numel() * 4
     print(json.dumps(out))
 
 

@@ -50,6 +50,8 @@ extern "C" {
                             where a single-pass kernel exists (2D elastic), else WV_PATH_PHASED */
 #define WV_PATH_PLAIN 1  /* reference operator sequence: CSR SpMV + vector passes, shot by shot */
 #define WV_PATH_PHASED 2 /* hand-written kernels, velocity-type and stress-type update as two launches */
+#define WV_PATH_STRIP 3  /* 2D elastic only: two launches of row-strip kernels (each wavefront owns
+                            tile_y/4 whole rows of 64 columns; y derivatives from registers) */
 
 typedef struct wv_solver wv_solver;
 
@@ -71,8 +73,8 @@ typedef struct {
     int steps_per_graph;   /* time steps per captured hipGraph segment (0 = none; rounded up to even) */
     int tile_y;            /* rows per tile (0 = default 16): single-pass elastic kernel 8, 16, 24,
                               negative selects direct global->LDS loads for the tile;
-                              two-launch kernels 8, 16, 32; 3D elastic: rows of the
-                              x-z tile 4, 8, 16 (default 8) */
+                              two-launch kernels 8, 16, 32; row-strip kernels 8, 16, 32;
+                              3D elastic: rows of the x-z tile 4, 8, 16 (default 8) */
 } wv_config;
 
 /* dimension "2D"/"3D", type "acoustic"/"elastic"/"tmem" (case-insensitive) */
@@ -111,11 +113,13 @@ int wv_set_wavefield(wv_solver *s, int shot, int field, const float *in);
 /* Autotuner: times `nsteps` steps of every kernel variant that can run this
  * configuration (the two-launch kernels with 8-, 16- and 32-row tiles; for 2D
  * elastic on WV_PATH_FUSED also the single-pass kernel with 8-, 16- and 24-row
- * tiles) on the handle's own model and acquisition, keeps the fastest and
+ * tiles and the row-strip kernels with 8-, 16- and 32-row blocks) on the handle's
+ * own model and acquisition, keeps the fastest and
  * resets the wavefields and seismograms.  *choice receives the index of the
  * variant kept (0..2 = two-launch with 8/16/32 rows, 3..5 = single pass with
- * 8/16/24 rows, 6..8 = 3D elastic with 4/8/16-row x-z tiles, -1 = the handle
- * has a single variant: WV_PATH_PLAIN).
+ * 8/16/24 rows, 6..8 = 3D elastic with 4/8/16-row x-z tiles, 9..11 = 2D
+ * elastic row-strip kernels with 8/16/32-row blocks, -1 = the handle has a
+ * single variant: WV_PATH_PLAIN).
  * No counterpart in the reference. */
 int wv_tune(wv_solver *s, int nsteps, int *choice);
 

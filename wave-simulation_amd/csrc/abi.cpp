@@ -167,8 +167,13 @@ int wv_solver::enqueue_steps(int nsteps, bool timed)
         } else {
             const wv::Fields2D F = fields2d(cur);
             const wv::Cpml2D P = cpml2d(cur);
-            wv::launch_step2d(physics, order, tile_y, L, cfg.nshots, F, C, T, P, A, j, j > 0, st, 0, e0, e1);
-            wv::launch_step2d(physics, order, tile_y, L, cfg.nshots, F, C, T, P, A, j, 0, st, 1, e2, e3);
+            if (variant == V_STRIP) {
+                wv::launch_strip2d(order, tile_y, L, cfg.nshots, F, C, T, P, A, j, j > 0, st, 0, e0, e1);
+                wv::launch_strip2d(order, tile_y, L, cfg.nshots, F, C, T, P, A, j, 0, st, 1, e2, e3);
+            } else {
+                wv::launch_step2d(physics, order, tile_y, L, cfg.nshots, F, C, T, P, A, j, j > 0, st, 0, e0, e1);
+                wv::launch_step2d(physics, order, tile_y, L, cfg.nshots, F, C, T, P, A, j, 0, st, 1, e2, e3);
+            }
             if (cfg.damping_boundary == 1) wv::launch_abs_velocity2d(physics, L, cfg.nshots, F, C, st);
         }
         if (timed && ++pending == BATCH) {
@@ -298,14 +303,19 @@ int wv_init(wv_solver *s, const wv_config *c)
     if (k.damping_boundary == 1 && !(k.damping_coeff > 0 && k.damping_coeff < 100))
         return fail(WV_EINVAL, "DampingCoeff must be in (0,100)");
     if (k.nshots < 1) return fail(WV_EINVAL, "nshots must be >= 1");
-    if (k.path != WV_PATH_FUSED && k.path != WV_PATH_PLAIN && k.path != WV_PATH_PHASED)
+    if (k.path != WV_PATH_FUSED && k.path != WV_PATH_PLAIN && k.path != WV_PATH_PHASED && k.path != WV_PATH_STRIP)
         return fail(WV_EINVAL, "bad path");
+    if (k.path == WV_PATH_STRIP && (s->dim != 2 || s->physics != wv::ELASTIC))
+        return fail(WV_EINVAL, "the row-strip kernels are 2D elastic only");
+    if (k.path == WV_PATH_STRIP && k.tile_y != 0 && k.tile_y != 8 && k.tile_y != 16 && k.tile_y != 32)
+        return fail(WV_EINVAL, "tile_y of the row-strip kernels must be 0 (default), 8, 16 or 32");
     if (k.path == WV_PATH_PLAIN && k.nshots != 1) return fail(WV_EINVAL, "the plain path runs one shot per handle");
     if (s->dim == 3 && k.path == WV_PATH_PLAIN) return fail(WV_EINVAL, "the plain path is 2D only");
     const bool single = s->dim == 2 && k.path == WV_PATH_FUSED && s->physics == wv::ELASTIC;
     if (s->dim == 2 && single && k.tile_y != 0 && std::abs(k.tile_y) != 8 && std::abs(k.tile_y) != 16 && std::abs(k.tile_y) != 24)
         return fail(WV_EINVAL, "tile_y of the single-pass kernel must be 0 (default), +-8, +-16 or +-24");
-    if (s->dim == 2 && !single && k.tile_y != 0 && k.tile_y != 8 && k.tile_y != 16 && k.tile_y != 32)
+    if (s->dim == 2 && !single && k.path != WV_PATH_STRIP && k.tile_y != 0 && k.tile_y != 8 && k.tile_y != 16 &&
+        k.tile_y != 32)
         return fail(WV_EINVAL, "tile_y of the two-launch kernels must be 0 (default), 8, 16 or 32");
     if (s->inited) return fail(WV_ESTATE, "solver already initialised");
 
@@ -334,12 +344,15 @@ int wv_init(wv_solver *s, const wv_config *c)
         return WV_OK;
     }
 
-    s->variant = single ? V_SINGLE : V_PHASED;
+    s->variant = single ? V_SINGLE : (k.path == WV_PATH_STRIP ? V_STRIP : V_PHASED);
     s->H = N / 2;
     if (s->variant == V_SINGLE) {
         s->tile_x = wv::fused2d_tile_x(N);
         s->tile_y = k.tile_y ? std::abs(k.tile_y) : 16;
         s->tile_load = k.tile_y < 0 ? -1 : 1;
+    } else if (s->variant == V_STRIP) {
+        s->tile_x = wv::TILE_X;
+        s->tile_y = k.tile_y ? k.tile_y : 16;
     } else {
         s->tile_x = wv::TILE_X;
         s->tile_y = k.tile_y ? k.tile_y : wv::TILE_Y;
@@ -356,6 +369,8 @@ int wv_init(wv_solver *s, const wv_config *c)
     // kernels index inside one field plane and one shot's CPML frame with 32-bit offsets
     if (L.plane >= (1LL << 31) || (long long)k.NY * k.NX >= (1LL << 30))
         return fail(WV_EINVAL, "grid too large for one field plane");
+    if (s->variant == V_STRIP && !wv::strip2d_fits(L))
+        return fail(WV_EINVAL, "grid too large for the row-strip kernels (field plane of 2 GiB or more)");
 
     int rc;
     // elastic keeps two field buffers so the tuner may switch between the
@@ -813,12 +828,16 @@ int wv_tune(wv_solver *s, int nsteps, int *choice)
     }
     // candidates: two-launch kernels with 8/16/32-row tiles; for elastic P-SV
     // also the single-pass kernel with register-staged tiles of 8/16/24 rows
-    const int cand_var[6] = {V_PHASED, V_PHASED, V_PHASED, V_SINGLE, V_SINGLE, V_SINGLE};
-    const int cand_ty[6] = {8, 16, 32, 8, 16, 24};
-    const int ncand = (s->physics == wv::ELASTIC && s->cfg.path == WV_PATH_FUSED) ? 6 : 3;
+    // and the row-strip kernels with 8/16/32-row blocks
+    const int cand_var[9] = {V_PHASED, V_PHASED, V_PHASED, V_SINGLE, V_SINGLE, V_SINGLE, V_STRIP, V_STRIP, V_STRIP};
+    const int cand_ty[9] = {8, 16, 32, 8, 16, 24, 8, 16, 32};
+    const int cand_id[9] = {0, 1, 2, 3, 4, 5, 9, 10, 11};
+    const bool all2d = s->physics == wv::ELASTIC && s->cfg.path == WV_PATH_FUSED;
+    const int ncand = all2d ? (wv::strip2d_fits(s->L) ? 9 : 6) : 3;
     float best = 1e30f;
-    int best_k = 0;
-    for (int k = 0; k < ncand; k++) {
+    const int k0 = s->cfg.path == WV_PATH_STRIP ? 6 : 0, k1 = s->cfg.path == WV_PATH_STRIP ? 9 : ncand;
+    int best_k = k0;
+    for (int k = k0; k < k1; k++) {
         s->variant = cand_var[k];
         s->tile_x = s->variant == V_SINGLE ? wv::fused2d_tile_x(s->cfg.spatial_fd_order) : wv::TILE_X;
         s->tile_y = cand_ty[k];
@@ -848,7 +867,7 @@ int wv_tune(wv_solver *s, int nsteps, int *choice)
     s->acq_dirty = true;
     s->cur = 0;
     s->drop_graph();
-    if (choice) *choice = best_k;
+    if (choice) *choice = cand_id[best_k];
     return wv_reset(s);
 }
 
@@ -891,7 +910,9 @@ int wv_kernel_stats(wv_solver *s, int kind, const char **name, double *avg_ms, l
     }
     if (name)
         *name = kind >= 3 ? adj_names[kind - 3]
-                          : ((s->variant == V_SINGLE && kind < 2) ? (kind == 0 ? "el2d_fused" : "-") : names[s->physics][kind]);
+                          : ((s->variant == V_SINGLE && kind < 2)   ? (kind == 0 ? "el2d_fused" : "-")
+                             : (s->variant == V_STRIP && kind < 2) ? (kind == 0 ? "el2d_velocity_strip" : "el2d_stress_strip")
+                                                                   : names[s->physics][kind]);
     if (launches) *launches = s->klaunch[kind];
     if (avg_ms) *avg_ms = s->klaunch[kind] ? s->kms[kind] / s->klaunch[kind] : 0.0;
     if (bytes_per_launch) *bytes_per_launch = s->kernel_bytes(kind);

@@ -129,6 +129,14 @@ void launch_step2d(int physics, int order, int tile_y, const Layout2D &L, int ns
                    hipStream_t stream, int which, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr,
                    const Snap2D *snap = nullptr);
 // velocity-type sources / receivers of step tbase + tofs (end of a run segment)
+// Row-strip 2D elastic kernels (strip2d.hip): blocks of 64 columns x tile_y rows,
+// tile_y in {8, 16, 32} (each of the 4 wavefronts owns tile_y/4 rows); same
+// phases and acquisition lists as launch_step2d (no adjoint snapshots).  Needs
+// planes below 2 GiB (strip2d_fits).
+bool strip2d_fits(const Layout2D &L);
+void launch_strip2d(int order, int tile_y, const Layout2D &L, int nshots, const Fields2D &F, const Coef2D &C,
+                    const Taps &T, const Cpml2D &P, const Acq &A, int tofs, int pend, hipStream_t stream, int which,
+                    hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 void launch_flush2d(int physics, const Layout2D &L, const Fields2D &F, const Acq &A, int tofs, hipStream_t stream);
 void launch_abs_velocity2d(int physics, const Layout2D &L, int nshots, const Fields2D &F, const Coef2D &C,
                            hipStream_t stream);

@@ -218,16 +218,19 @@ void ForwardSolver::initForwardSolver(const Configuration &c, int nshots, int st
     k.fc_cpml = c.getDouble("CenterFrequencyCPML", 5.0);
     k.vmax_cpml = c.getDouble("VMaxCPML", 3500.0);
     k.nshots = nshots;
-    // kernel family: fused (single-pass where it exists, default), phased (two launches), plain (reference sequence)
+    // kernel family: fused (single-pass where it exists, default), phased (two launches), strip (2D elastic
+    // row-strip kernels), plain (reference sequence)
     const std::string path = c.has("solverPath") ? c.getString("solverPath") : "fused";
     if (path == "phased")
         k.path = WV_PATH_PHASED;
+    else if (path == "strip")
+        k.path = WV_PATH_STRIP;
     else if (path == "plain")
         k.path = WV_PATH_PLAIN;
     else if (path == "fused")
         k.path = WV_PATH_FUSED;
     else
-        throw std::runtime_error("solverPath must be fused, phased or plain");
+        throw std::runtime_error("solverPath must be fused, phased, strip or plain");
     k.steps_per_graph = stepsPerGraph;
     k.tile_y = tileY;
     check(wv_init(h_, &k));

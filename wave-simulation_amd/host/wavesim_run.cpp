@@ -13,7 +13,7 @@
 // component, resampled to seismoDT (Seismogram.cpp:82-156).  Keys of this
 // driver beyond the reference's: stepsPerGraph (time steps per captured graph,
 // default 50), tileY (fixed tile height; default: autotuned) and solverPath
-// (fused | phased | plain).
+// (fused | phased | strip | plain).
 #include <algorithm>
 #include <chrono>
 #include <cmath>

@@ -17,6 +17,7 @@ WV_OPS_STENCIL = 1
 WV_PATH_FUSED = 0
 WV_PATH_PLAIN = 1
 WV_PATH_PHASED = 2
+WV_PATH_STRIP = 3
 
 # every entry point of include/wave_abi.h
 EXPORTS = (
