@@ -67,11 +67,26 @@ def _kernel_stats(sol, L, wavesim, kinds=(0, 1, 2)):
     return out
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "el2d_strip_c2_1000x1000_hbm_bytes.md")
+
+
+def _traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed FETCH_SIZE/WRITE_SIZE summary (calibrated), or None."""
+    try:
+        for line in open(TRAFFIC_FILE):
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            if len(cells) >= 6 and cells[0] == kernel:
+                return float(cells[5]) * 1e6
+    except OSError:
+        pass
+    return None
+
+
 def _roofline(kernels):
     dom = max(kernels, key=lambda k: k["avg_ms"] * k["launches"])
     achieved = dom["bytes"] / (dom["avg_ms"] * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom["name"],
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": _traffic(dom["name"]), "kernel": dom["name"],
             "kernel_avg_us": round(dom["avg_ms"] * 1e3, 3), "bytes_per_launch": dom["bytes"]}
 
 
