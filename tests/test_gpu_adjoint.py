@@ -84,6 +84,18 @@ def test_gradient_matches_fp64_oracle(fp16):
         assert e <= GTOL, (name, e)
 
 
+@pytest.mark.parametrize("tile_y", [8, 16])
+def test_gradient_with_row_strip_recomputation(tile_y):
+    """The recomputation runs on the row-strip kernels (they record the derivative sums too)."""
+    c, obs = _c4()
+    _, gg, _, resid = _gpu_gradient(c, obs, every=40, fp16=True, path="strip", tile_y=tile_y)
+    _, go = _oracle_gradient(c, obs, resid=resid)
+    for k, name in enumerate(("vp", "vs", "rho")):
+        e = rel_l2(gg[k], go[k])
+        print(f"strip-{tile_y} {name}: relL2 vs FP64 adjoint {e:.2e}")
+        assert e <= GTOL, (name, e)
+
+
 def test_gradient_uneven_segments_and_stencil_operators():
     """Checkpoint interval not dividing NT, no CPML, stencil-mode operators."""
     c, obs = _c4(N=56, NT=230, nshots=1)

@@ -219,11 +219,11 @@ double wv_solver::kernel_bytes(int kind) const
     }
     if (kind >= 3) {
         // adjoint pass (2D elastic): per shot-cell field bytes, per cell coefficient bytes
-        //   3 recomputed velocity phase: forward velocity phase + 5 snapshot fields written
-        //   4 recomputed stress phase  : forward stress phase
-        //   5 adjoint stress step      : r lsxx lsyy lsxy vx vy, r+w lvx lvy, 3 FP64 gradient updates (16 B)
-        //   6 adjoint velocity step    : r lvx lvy sxx syy sxy, r+w lsxx lsyy lsxy, 2 FP64 gradient updates
-        static const double per_shot[4] = {4.0 * (7 + 5), 4.0 * 8, 4.0 * 9 + 48, 4.0 * 11 + 32};
+        //   3 recomputed velocity phase: forward velocity phase + 2 recorded derivative sums written
+        //   4 recomputed stress phase  : forward stress phase + 3 recorded derivative sums written
+        //   5 adjoint stress step      : r lsxx lsyy lsxy h4 h5 h67, r+w lvx lvy, 3 FP64 gradient updates (16 B)
+        //   6 adjoint velocity step    : r lvx lvy h01 h23, r+w lsxx lsyy lsxy, 2 FP64 gradient updates
+        static const double per_shot[4] = {4.0 * (7 + 2), 4.0 * (8 + 3), 4.0 * 10 + 48, 4.0 * 10 + 32};
         static const double per_cell[4] = {4.0 * 2, 4.0 * 3, 4.0 * 3, 4.0 * 2};
         return cells * (B * per_shot[kind - 3] + per_cell[kind - 3]);
     }
@@ -898,7 +898,7 @@ int wv_kernel_stats(wv_solver *s, int kind, const char **name, double *avg_ms, l
     static const char *names[3][3] = {{"ac2d_velocity", "ac2d_pressure", "flush2d"},
                                       {"el2d_velocity", "el2d_stress", "flush2d"},
                                       {"tm2d_h", "tm2d_e", "flush2d"}};
-    static const char *adj_names[4] = {"el2d_velocity (recompute)", "el2d_stress (recompute)", "adj_el2d_stress",
+    static const char *adj_names[4] = {"velocity phase (recompute)", "stress phase (recompute)", "adj_el2d_stress",
                                        "adj_el2d_velocity"};
     HIP_TRY(hipStreamSynchronize(s->st));
     if (s->e3) {

@@ -31,7 +31,7 @@ struct AdjointState {
     std::vector<void *> ck_fields;
     std::vector<unsigned *> ck_max;
     std::vector<float *> ck_psi;
-    float *segS = nullptr, *segV = nullptr, *segP = nullptr;
+    float *segH = nullptr;  // recorded derivative sums of a segment: [step][5][fld_n]
     int seg_slots = 0;
     float *lam[5] = {};
     float *mu[2][8] = {};
@@ -218,7 +218,7 @@ int wv_set_checkpointing(wv_solver *s, int every, int fp16)
     if (!s->adj) s->adj = new AdjointState();
     AdjointState *a = s->adj;
     HIP_TRY(hipStreamSynchronize(s->st));
-    if (every && every == a->every && (fp16 != 0) == a->fp16 && s->variant == V_PHASED) {
+    if (every && every == a->every && (fp16 != 0) == a->fp16 && s->variant != V_SINGLE) {
         // same layout: keep the checkpoint buffers
         a->forward_done = a->grad_ready = false;
         a->next_t = -1;
@@ -235,8 +235,9 @@ int wv_set_checkpointing(wv_solver *s, int every, int fp16)
     a->grad_ready = false;
     a->next_t = -1;
     if (!every) return WV_OK;
-    // the adjoint follows the two-launch kernels: switch to them, segments of `every` steps
-    if (s->variant != V_PHASED) {
+    // the recomputation needs a two-launch kernel pair (tiled or row-strip: both record
+    // the derivative sums); the single-pass kernel is replaced by the tiled pair
+    if (s->variant == V_SINGLE) {
         s->variant = V_PHASED;
         s->tile_x = wv::TILE_X;
         s->tile_y = wv::TILE_Y;
@@ -333,10 +334,7 @@ int wv_run_adjoint(wv_solver *s)
         }
     }
     if (a->seg_slots < K) {
-        if ((rc = adalloc(a->allocs, &a->segS, (size_t)K * 3 * a->fld_n, st)) ||
-            (rc = adalloc(a->allocs, &a->segV, (size_t)K * 2 * a->fld_n, st)) ||
-            (np && (rc = adalloc(a->allocs, &a->segP, (size_t)K * np, st))))
-            return rc;
+        if ((rc = adalloc(a->allocs, &a->segH, (size_t)K * 5 * a->fld_n, st))) return rc;
         a->seg_slots = K;
     }
     const size_t nsb = (size_t)ns * s->nbx * s->nby + 1;
@@ -413,57 +411,24 @@ int wv_run_adjoint(wv_solver *s)
         // recompute the segment; the forward kernels record what the adjoint steps read
         for (int j = 0; j < n; j++) {
             wv::Snap2D snap{};
-            float *S3 = a->segS + (size_t)j * 3 * a->fld_n;
-            float *V2 = a->segV + (size_t)j * 2 * a->fld_n;
-            for (int f = 0; f < 3; f++) snap.f[f] = S3 + f * a->fld_n;
-            for (int f = 0; f < 2; f++) snap.f[3 + f] = V2 + f * a->fld_n;
-            if (np) {
-                float *d = a->segP + (size_t)j * np;
-                float *pp[8];
-                for (int q = 0; q < 8; q++) {
-                    pp[q] = d;
-                    d += pn[q];
-                }
-                // psi order psx0 psx1 psy0 psy1 psx2 psx3 psy2 psy3
-                for (int q = 0; q < 2; q++) {
-                    snap.px[q] = pp[q];
-                    snap.py[q] = pp[2 + q];
-                    snap.px[2 + q] = pp[4 + q];
-                    snap.py[2 + q] = pp[6 + q];
-                }
-            }
+            for (int f = 0; f < 5; f++) snap.h[f] = a->segH + ((size_t)j * 5 + f) * a->fld_n;
+            const bool strip = s->variant == V_STRIP;
             if ((rc = kt.next(3, &e0, &e1))) return rc;
-            wv::launch_step2d(wv::ELASTIC, order, s->tile_y, L, ns, F, C, T, P, Arec, j, j > 0, st, 0, e0, e1, &snap);
+            if (strip)
+                wv::launch_strip2d(order, s->tile_y, L, ns, F, C, T, P, Arec, j, j > 0, st, 0, e0, e1, &snap);
+            else
+                wv::launch_step2d(wv::ELASTIC, order, s->tile_y, L, ns, F, C, T, P, Arec, j, j > 0, st, 0, e0, e1, &snap);
             if ((rc = kt.next(4, &e0, &e1))) return rc;
-            wv::launch_step2d(wv::ELASTIC, order, s->tile_y, L, ns, F, C, T, P, Arec, j, 0, st, 1, e0, e1, &snap);
+            if (strip)
+                wv::launch_strip2d(order, s->tile_y, L, ns, F, C, T, P, Arec, j, 0, st, 1, e0, e1, &snap);
+            else
+                wv::launch_step2d(wv::ELASTIC, order, s->tile_y, L, ns, F, C, T, P, Arec, j, 0, st, 1, e0, e1, &snap);
         }
         // adjoint steps of the segment, last to first
         for (int j = n - 1; j >= 0; j--) {
             const int t = t0 + j;
-            wv::FwdSnap SV{}, SS{};
-            const float *Pj = np ? a->segP + (size_t)j * np : nullptr;
-            const float *pp[8] = {};
-            if (np) {
-                const float *q0 = Pj;
-                for (int q = 0; q < 8; q++) {
-                    pp[q] = q0;
-                    q0 += pn[q];
-                }
-            }
-            // psi order: psx0 psx1 psy0 psy1 psx2 psx3 psy2 psy3
-            for (int q = 0; q < 2; q++) {
-                SS.px[q] = pp[q];
-                SS.py[q] = pp[2 + q];
-                SV.px[2 + q] = pp[4 + q];
-                SV.py[2 + q] = pp[6 + q];
-            }
-            const float *V2 = a->segV + (size_t)j * 2 * a->fld_n;
-            const float *S3 = a->segS + (size_t)j * 3 * a->fld_n;
-            SV.f0 = V2;
-            SV.f1 = V2 + a->fld_n;
-            SS.f0 = S3;
-            SS.f1 = S3 + a->fld_n;
-            SS.f2 = S3 + 2 * a->fld_n;
+            wv::FwdSnap SV{}, SS{};  // stress-phase and velocity-phase sums of step t
+            for (int f = 0; f < 5; f++) SV.h[f] = SS.h[f] = a->segH + ((size_t)j * 5 + f) * a->fld_n;
             wv::launch_adj_inject(L, AF, Aq, a->adj_src, t, st);
             wv::AdjPsi M{};
             for (int q = 0; q < 4; q++) {

@@ -21,13 +21,11 @@ struct AdjPsi {
     float *mxn[4], *myn[4];
 };
 
-// forward quantities of one step read by the adjoint kernels:
-// stress phase: f0 = vx, f1 = vy after the velocity update of the step;
-// velocity phase: f0 = sxx, f1 = syy, f2 = sxy at the start of the step;
-// px/py: CPML memory variables at the start of the step.
+// forward quantities of one step read by the adjoint kernels (recorded by the
+// forward kernels, Snap2D): the CPML-modified derivative sums of the step
+// h[0] = h0 + h1, h[1] = h2 + h3, h[2] = h4, h[3] = h5, h[4] = h6 + h7.
 struct FwdSnap {
-    const float *f0, *f1, *f2;
-    const float *px[4], *py[4];
+    const float *h[5];
 };
 
 // transposed derivative tables: (D^T w)[j] = sum_k t[j][k] w[j + k - H]

@@ -13,9 +13,12 @@
 // Backward step t: adjoint sources of step t, then adj_stress (transpose of
 // the stress update; adds D^T lg to the adjoint velocities), then
 // adj_velocity (transpose of the velocity update; adds D^T lg to the adjoint
-// stresses).  Each kernel evaluates lg on its tile plus halo into LDS and
+// stresses).  Each kernel evaluates lg on its cells plus an H-wide halo and
 // applies the transposed stencils (position-dependent tables, so the
-// edge-reduced rows transpose exactly).
+// edge-reduced rows transpose exactly).  The gradient terms multiply the
+// adjoint fields by the forward step's CPML-modified derivative sums, which
+// the forward kernels recorded (FwdSnap), so no forward derivative is
+// evaluated here.
 #include <hip/hip_fp16.h>
 
 #include "adjoint.hpp"
@@ -25,252 +28,453 @@ namespace wv {
 
 namespace {
 
-// walk the region elements a thread owns (as TileRegs)
-template <int H, int TY, typename Fn>
-__device__ __forceinline__ void region_each(int tid, Fn fn)
+// ---------------------------------------------------------------------------
+// Strip form of the two adjoint steps.  A block
+// covers 64 columns x S rows for W shots: wavefront w works on shot
+// (group * W + w) over the block's cells and needs nothing from the other
+// wavefronts until the gradient, so the W shots' contributions to a cell are
+// summed in LDS (FP64) and reach the gradient planes as one FP64 atomic per
+// cell and block instead of one per shot.  Within a wavefront: the
+// y-transposed stencils read a register column of lg over rows y-H .. y+S+H-1
+// (lg evaluated by the lane itself), the x-transposed ones a wavefront-private
+// LDS strip of lg over columns x-H .. x+63+H (the halo columns evaluated by
+// lanes < 2H); plane arrays are raw buffer accesses with scalar row offsets.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int aboff(const Layout2D &L, int x, int y) { return (int)(L.at(x, y) * 4); }
+
+// x-frame CPML quantities at column x (lane-level): frame column, a, a(half), b, b(half)
+struct XFrame {
+    int xi;
+    float a, ah, b, bh;
+};
+__device__ __forceinline__ XFrame xframe(const Layout2D &L, const Cpml2D &P, int x)
 {
-    using TL = Tile<H, TY>;
-#pragma unroll
-    for (int k = 0; k < TL::IT; k++) {
-        const int i = tid + k * NTHREADS;
-        if (TL::TOT % NTHREADS == 0 || i < TL::TOT) {
-            const int r = i / TL::RW;
-            fn(k, r, i - r * TL::RW);
+    XFrame f{-1, 0.f, 0.f, 0.f, 0.f};
+    if (P.on && x >= 0 && x < L.NX) {
+        f.xi = P.xi[x];
+        if (f.xi >= 0) {
+            f.a = P.ax[x];
+            f.ah = P.axh[x];
+            f.b = P.bx[x];
+            f.bh = P.bxh[x];
         }
     }
+    return f;
 }
 
-template <int H, int TY>
-__global__ __launch_bounds__(NTHREADS) void adj_el2d_stress(Layout2D L, AdjFields A, AdjPsi M, FwdSnap S, Coef2D C,
-                                                            Taps T, TapsT TT, Cpml2D P, Grad G)
-{
-    using TL = Tile<H, TY>;
-    constexpr int R = TY / BLOCK_Y;
-    __shared__ float s_g4[TL::RH][TL::RW];
-    __shared__ float s_g5[TL::RH][TL::RW];
-    __shared__ float s_g6[TL::RH][TL::RW];
-    __shared__ float s_g7[TL::RH][TL::RW];
-    __shared__ float s_vx[TL::RH][TL::RW];
-    __shared__ float s_vy[TL::RH][TL::RW];
-    const Blk B = xcd_block();
-    const int shot = B.shot;
-    const long long so = (long long)shot * L.plane;
-    const int x0 = B.bx * TILE_X, y0 = B.by * TY;
-    const int tid = threadIdx.y * TILE_X + threadIdx.x, tx = threadIdx.x;
-    const int W2 = 2 * P.W;
-
-    TileRegs<H, TY> lxx, lyy, lxy, fvx, fvy, cM, cmu, cmxy;
-    lxx.load(A.lsxx + so, L, x0, y0, tid);
-    lyy.load(A.lsyy + so, L, x0, y0, tid);
-    lxy.load(A.lsxy + so, L, x0, y0, tid);
-    fvx.load(S.f0 + so, L, x0, y0, tid);
-    fvy.load(S.f1 + so, L, x0, y0, tid);
-    cM.load(C.c[0], L, x0, y0, tid);
-    cmu.load(C.c[1], L, x0, y0, tid);
-    cmxy.load(C.c[4], L, x0, y0, tid);
-    // adjoint CPML terms on the whole region
-    region_each<H, TY>(tid, [&](int k, int r, int c) {
-        const int x = x0 - H + c, y = y0 - H + r;
-        const float ls = lxx.v[k] + lyy.v[k];
-        const float h4 = cM.v[k] * ls - 2.f * cmu.v[k] * lyy.v[k];
-        const float h5 = cM.v[k] * ls - 2.f * cmu.v[k] * lxx.v[k];
-        const float h6 = cmxy.v[k] * lxy.v[k];
-        float g4 = h4, g5 = h5, g6 = h6, g7 = h6;
-        const bool in = x >= 0 && x < L.NX && y >= 0 && y < L.NY;
-        if (P.on && in) {
-            const int xi = P.xi[x], yi = P.yi[y];
-            if (xi >= 0) {
-                const long long pi = ((long long)shot * L.NY + y) * W2 + xi;
-                g4 = h4 + P.ax[x] * (M.mx[2][pi] + h4);
-                g7 = h6 + P.axh[x] * (M.mx[3][pi] + h6);
-            }
-            if (yi >= 0) {
-                const long long pi = ((long long)shot * W2 + yi) * L.NX + x;
-                g5 = h5 + P.ay[y] * (M.my[2][pi] + h5);
-                g6 = h6 + P.ayh[y] * (M.my[3][pi] + h6);
-            }
-        }
-        s_g4[r][c] = g4;
-        s_g5[r][c] = g5;
-        s_g6[r][c] = g6;
-        s_g7[r][c] = g7;
-    });
-    fvx.store(s_vx, tid);
-    fvy.store(s_vy, tid);
-
-    const Col col = column(L, P, x0 + tx);
-    float cxf[2 * H + 1], cxb[2 * H + 1], txf[2 * H + 1], txb[2 * H + 1];
-    load_taps<H>(cxf, T.xf, col.x);
-    load_taps<H>(cxb, T.xb, col.x);
-    load_taps<H>(txf, TT.xf, col.x);
-    load_taps<H>(txb, TT.xb, col.x);
-    const int wy = __builtin_amdgcn_readfirstlane(threadIdx.y);
-    __syncthreads();
-
+// Sum over the block's W wavefronts of per-cell FP64 contributions of NF
+// gradient planes: each wavefront adds its values into LDS accumulators (LDS
+// FP64 atomics) and counts itself in; the last wavefront to arrive adds the
+// sums to the gradient planes (one global FP64 atomic per cell and plane), so
+// no wavefront waits for the others.  init() runs before any wavefront adds.
+template <int NF, int S, int W>
+struct GradSum {
+    double acc[NF][S][TILE_X];
+    int arrived;
+    __device__ __forceinline__ void init(int wy, int lane)
+    {
+        if (wy == 0) {
 #pragma unroll
-    for (int r = 0; r < R; r++) {
-        const int ly = wy + r * BLOCK_Y;
-        const int y = y0 + ly;
-        if (y >= L.NY) break;
-        if (!col.in) continue;
-        const int Rr = ly + H;
-        const long long o = L.at(col.x, y);
-        const float sxx = A.lsxx[so + o], syy = A.lsyy[so + o], sxy = A.lsxy[so + o];
-        const float Mo = C.c[0][o], muo = C.c[1][o], mxyo = C.c[4][o];
-        const float ls = sxx + syy;
-        const float h4 = Mo * ls - 2.f * muo * syy;
-        const float h5 = Mo * ls - 2.f * muo * sxx;
-        const float h6 = mxyo * sxy;
-        // forward CPML-modified derivatives of step t (el2d_stress)
-        const float *cyb = T.yb + (long long)y * (2 * H + 1);
-        const float *cyf = T.yf + (long long)y * (2 * H + 1);
-        float f4 = dx<H>(s_vx, Rr, tx, cxb);
-        float f5 = dy<H>(s_vy, Rr, tx, cyb);
-        float f6 = dy<H>(s_vx, Rr, tx, cyf);
-        float f7 = dx<H>(s_vy, Rr, tx, cxf);
-        const int yi = P.on ? P.yi[y] : -1;
-        if (col.xi >= 0) {
-            const long long pi = ((long long)shot * L.NY + y) * W2 + col.xi;
-            (void)cpml(f4, S.px[2][pi], col.ax, col.bx);
-            (void)cpml(f7, S.px[3][pi], col.axh, col.bxh);
-            M.mxn[2][pi] = col.bx * (M.mx[2][pi] + h4);
-            M.mxn[3][pi] = col.bxh * (M.mx[3][pi] + h6);
+            for (int f = 0; f < NF; f++)
+#pragma unroll
+                for (int r = 0; r < S; r++) acc[f][r][lane] = 0.0;
+            if (lane == 0) arrived = 0;
         }
-        if (yi >= 0) {
-            const long long pi = ((long long)shot * W2 + yi) * L.NX + col.x;
-            (void)cpml(f5, S.py[2][pi], P.ay[y], P.by[y]);
-            (void)cpml(f6, S.py[3][pi], P.ayh[y], P.byh[y]);
-            M.myn[2][pi] = P.by[y] * (M.my[2][pi] + h5);
-            M.myn[3][pi] = P.byh[y] * (M.my[3][pi] + h6);
-        }
-        atomicAdd(&G.g[2][o], (double)ls * ((double)f4 + (double)f5));
-        atomicAdd(&G.g[3][o], -2.0 * (double)sxx * f5 - 2.0 * (double)syy * f4);
-        atomicAdd(&G.g[4][o], (double)sxy * ((double)f6 + (double)f7));
-        // lvx += Dxb^T lg4 + Dyf^T lg6 ; lvy += Dyb^T lg5 + Dxf^T lg7
-        const float *tyf = TT.yf + (long long)y * (2 * H + 1);
-        const float *tyb = TT.yb + (long long)y * (2 * H + 1);
-        float dvx = dx<H>(s_g4, Rr, tx, txb);
-        dvx += dy<H>(s_g6, Rr, tx, tyf);
-        float dvy = dy<H>(s_g5, Rr, tx, tyb);
-        dvy += dx<H>(s_g7, Rr, tx, txf);
-        A.lvx[so + o] += dvx;
-        A.lvy[so + o] += dvy;
+        __syncthreads();
     }
+    __device__ __forceinline__ void add(const double (&c)[NF][S], bool act, double *const (&g)[NF],
+                                        const Layout2D &L, int x0, int ya, int lane)
+    {
+        if (act && W > 1) {
+#pragma unroll
+            for (int f = 0; f < NF; f++)
+#pragma unroll
+                for (int r = 0; r < S; r++) atomicAdd(&acc[f][r][lane], c[f][r]);
+        }
+        const int x = x0 + lane;
+        if (W == 1) {
+            if (!act || x >= L.NX) return;
+#pragma unroll
+            for (int f = 0; f < NF; f++)
+#pragma unroll
+                for (int r = 0; r < S; r++)
+                    if (ya + r < L.NY) atomicAdd(g[f] + L.at(x, ya + r), c[f][r]);
+            return;
+        }
+        __threadfence_block();
+        int last = 0;
+        if (lane == 0) last = atomicAdd(&arrived, 1) == W - 1;
+        last = __builtin_amdgcn_readfirstlane(last);
+        if (!last) return;
+        __threadfence_block();
+        if (x >= L.NX) return;
+#pragma unroll
+        for (int f = 0; f < NF; f++)
+#pragma unroll
+            for (int r = 0; r < S; r++)
+                if (ya + r < L.NY) atomicAdd(g[f] + L.at(x, ya + r), acc[f][r][lane]);
+    }
+};
+
+// Loads are issued branch-free before any arithmetic (CPML memory variables at
+// clamped indices, the x-halo columns through an out-of-range offset on the
+// lanes that have none, which the buffer unit answers with 0 without a memory
+// access), so a wavefront waits for few memory round trips.
+constexpr unsigned NO_LANE = 0x7ffffff0u;
+
+template <int H, int S, int W>
+__global__ __launch_bounds__(TILE_X * W) void adj_el2d_stress_strip(Layout2D L, int nshots, AdjFields A, AdjPsi M,
+                                                                   FwdSnap F, Coef2D C, TapsT TT, Cpml2D P, Grad G)
+{
+    constexpr int K = 2 * H + 1, N = S + 2 * H, RW = TILE_X + 2 * H;
+    __shared__ float s_g4[W][S][RW];
+    __shared__ float s_g7[W][S][RW];
+    __shared__ float s_ty[W][2][S * K];
+    __shared__ GradSum<3, S, W> s_sum;
+    const Blk B = xcd_block();
+    const int wy = __builtin_amdgcn_readfirstlane(threadIdx.y), lane = threadIdx.x;
+    const int shot = B.shot * W + wy;
+    const bool act = shot < nshots;
+    const int x0 = B.bx * TILE_X, ya = B.by * S, xl = x0 - H, x = x0 + lane;
+    const long long so = (long long)(act ? shot : 0) * L.plane;
+    const int W2 = 2 * P.W;
+    s_sum.init(wy, lane);
+    double c[3][S];
+#pragma unroll
+    for (int r = 0; r < S; r++) c[0][r] = c[1][r] = c[2][r] = 0.0;
+
+    if (act) {
+        const unsigned pb = (unsigned)(L.plane * 4);
+        const auto bxx = brsrc(A.lsxx + so, pb), byy = brsrc(A.lsyy + so, pb), bxy = brsrc(A.lsxy + so, pb);
+        const auto bM = brsrc(C.c[0], pb), bmu = brsrc(C.c[1], pb), bmxy = brsrc(C.c[4], pb);
+        const unsigned vc = (unsigned)(H + lane) * 4u;
+        const bool inx = x < L.NX;
+        const int xc = min(x, L.NX - 1);
+        const int hc = lane < H ? lane : TILE_X + lane;
+        const unsigned vh = lane < 2 * H ? (unsigned)hc * 4u : NO_LANE;
+        const XFrame fx = xframe(L, P, x), fh = xframe(L, P, xl + hc);
+
+        // window of the lane's column (rows ya-H .. ya+S+H-1) and the x-halo columns of the strip's rows
+        float lxx[N], lyy[N], lxy[N], cM[N], cmu[N], cmxy[N];
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+            const int ro = aboff(L, xl, ya - H + i);
+            lxx[i] = bload(bxx, vc, ro);
+            lyy[i] = bload(byy, vc, ro);
+            lxy[i] = bload(bxy, vc, ro);
+            cM[i] = bload(bM, vc, ro);
+            cmu[i] = bload(bmu, vc, ro);
+            cmxy[i] = bload(bmxy, vc, ro);
+        }
+        float hxx[S], hyy[S], hxy[S], hM[S], hmu[S], hmxy[S];
+#pragma unroll
+        for (int r = 0; r < S; r++) {
+            const int ro = aboff(L, xl, ya + r);
+            hxx[r] = bload(bxx, vh, ro);
+            hyy[r] = bload(byy, vh, ro);
+            hxy[r] = bload(bxy, vh, ro);
+            hM[r] = bload(bM, vh, ro);
+            hmu[r] = bload(bmu, vh, ro);
+            hmxy[r] = bload(bmxy, vh, ro);
+        }
+        // adjoint memory variables (read buffers): y frame on the window rows, x frame on the strip's rows
+        float my2[N], my3[N], mx2[S], mx3[S], hx2[S], hx3[S];
+        int yiw[N];
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+            const int y = ya - H + i;
+            yiw[i] = (P.on && y >= 0 && y < L.NY) ? P.yi[y] : -1;
+            my2[i] = my3[i] = 0.f;
+        }
+#pragma unroll
+        for (int r = 0; r < S; r++) mx2[r] = mx3[r] = hx2[r] = hx3[r] = 0.f;
+        if (P.on) {
+#pragma unroll
+            for (int i = 0; i < N; i++) {
+                const long long pi = ((long long)shot * W2 + max(yiw[i], 0)) * L.NX + xc;
+                my2[i] = M.my[2][pi];
+                my3[i] = M.my[3][pi];
+            }
+#pragma unroll
+            for (int r = 0; r < S; r++) {
+                const long long row = ((long long)shot * L.NY + min(ya + r, L.NY - 1)) * W2;
+                mx2[r] = M.mx[2][row + max(fx.xi, 0)];
+                mx3[r] = M.mx[3][row + max(fx.xi, 0)];
+                hx2[r] = M.mx[2][row + max(fh.xi, 0)];
+                hx3[r] = M.mx[3][row + max(fh.xi, 0)];
+            }
+        }
+        // own cells: recorded forward derivative sums
+        float f4[S], f5[S], f67[S];
+#pragma unroll
+        for (int r = 0; r < S; r++) {
+            const long long o = so + L.at(x, min(ya + r, L.NY - 1));
+            f4[r] = F.h[2][o];
+            f5[r] = F.h[3][o];
+            f67[r] = F.h[4][o];
+        }
+
+        // lg5, lg6 on the window (registers); lg4, lg7 on the strip (LDS)
+        float w5[N], w6[N];
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+            const float ls = lxx[i] + lyy[i];
+            const float h5 = cM[i] * ls - 2.f * cmu[i] * lxx[i], h6 = cmxy[i] * lxy[i];
+            float g5 = h5, g6 = h6;
+            if (yiw[i] >= 0 && inx) {
+                const int y = ya - H + i;
+                g5 = h5 + P.ay[y] * (my2[i] + h5);
+                g6 = h6 + P.ayh[y] * (my3[i] + h6);
+            }
+            w5[i] = g5;
+            w6[i] = g6;
+        }
+#pragma unroll
+        for (int r = 0; r < S; r++) {
+            const bool iny = ya + r < L.NY;
+            const int i = r + H;
+            const float ls = lxx[i] + lyy[i];
+            const float h4 = cM[i] * ls - 2.f * cmu[i] * lyy[i], h6 = cmxy[i] * lxy[i];
+            const bool fxo = fx.xi >= 0 && iny;
+            s_g4[wy][r][H + lane] = fxo ? h4 + fx.a * (mx2[r] + h4) : h4;
+            s_g7[wy][r][H + lane] = fxo ? h6 + fx.ah * (mx3[r] + h6) : h6;
+            const float hls = hxx[r] + hyy[r];
+            const float hh4 = hM[r] * hls - 2.f * hmu[r] * hyy[r], hh6 = hmxy[r] * hxy[r];
+            const bool fho = fh.xi >= 0 && iny;
+            if (lane < 2 * H) {
+                s_g4[wy][r][hc] = fho ? hh4 + fh.a * (hx2[r] + hh4) : hh4;
+                s_g7[wy][r][hc] = fho ? hh6 + fh.ah * (hx3[r] + hh6) : hh6;
+            }
+        }
+        {
+            const float *__restrict__ ta = TT.yf + (long long)ya * K;
+            const float *__restrict__ tb = TT.yb + (long long)ya * K;
+            for (int e = lane; e < S * K; e += TILE_X) {
+                s_ty[wy][0][e] = ta[e];
+                s_ty[wy][1][e] = tb[e];
+            }
+        }
+        float txf[K], txb[K];
+        load_taps<H>(txf, TT.xf, x);
+        load_taps<H>(txb, TT.xb, x);
+        wave_sync();
+
+        const auto bvx = brsrc(A.lvx + so, pb), bvy = brsrc(A.lvy + so, pb);
+#pragma unroll
+        for (int r = 0; r < S; r++) {
+            const int y = ya + r, i = r + H;
+            // lvx += Dxb^T lg4 + Dyf^T lg6 ; lvy += Dyb^T lg5 + Dxf^T lg7
+            float dvx = 0.f, dvy = 0.f;
+#pragma unroll
+            for (int k = 0; k < K; k++) dvx += txb[k] * s_g4[wy][r][lane + k];
+            {
+                float s = 0.f;
+#pragma unroll
+                for (int k = 0; k < K; k++) s += s_ty[wy][0][r * K + k] * w6[r + k];
+                dvx += s;
+            }
+#pragma unroll
+            for (int k = 0; k < K; k++) dvy += s_ty[wy][1][r * K + k] * w5[r + k];
+            {
+                float s = 0.f;
+#pragma unroll
+                for (int k = 0; k < K; k++) s += txf[k] * s_g7[wy][r][lane + k];
+                dvy += s;
+            }
+            if (y >= L.NY || !inx) continue;
+            const int ro = aboff(L, xl, y);
+            bstore(bload(bvx, vc, ro) + dvx, bvx, vc, ro);
+            bstore(bload(bvy, vc, ro) + dvy, bvy, vc, ro);
+            // adjoint memory variables of the own cell (write buffers)
+            const float ls = lxx[i] + lyy[i];
+            const float h4 = cM[i] * ls - 2.f * cmu[i] * lyy[i], h5 = cM[i] * ls - 2.f * cmu[i] * lxx[i];
+            const float h6 = cmxy[i] * lxy[i];
+            if (fx.xi >= 0) {
+                const long long pi = ((long long)shot * L.NY + y) * W2 + fx.xi;
+                M.mxn[2][pi] = fx.b * (mx2[r] + h4);
+                M.mxn[3][pi] = fx.bh * (mx3[r] + h6);
+            }
+            if (yiw[i] >= 0) {
+                const long long pi = ((long long)shot * W2 + yiw[i]) * L.NX + x;
+                M.myn[2][pi] = P.by[y] * (my2[i] + h5);
+                M.myn[3][pi] = P.byh[y] * (my3[i] + h6);
+            }
+            // gradient terms with the forward step's recorded derivative sums
+            c[0][r] = (double)ls * ((double)f4[r] + (double)f5[r]);
+            c[1][r] = -2.0 * (double)lxx[i] * f5[r] - 2.0 * (double)lyy[i] * f4[r];
+            c[2][r] = (double)lxy[i] * (double)f67[r];
+        }
+    }
+    double *const g[3] = {G.g[2], G.g[3], G.g[4]};
+    s_sum.add(c, act, g, L, x0, ya, lane);
 }
 
-template <int H, int TY>
-__global__ __launch_bounds__(NTHREADS) void adj_el2d_velocity(Layout2D L, AdjFields A, AdjPsi M, FwdSnap S, Coef2D C,
-                                                              Taps T, TapsT TT, Cpml2D P, Grad G)
+template <int H, int S, int W>
+__global__ __launch_bounds__(TILE_X * W) void adj_el2d_velocity_strip(Layout2D L, int nshots, AdjFields A, AdjPsi M,
+                                                                     FwdSnap F, Coef2D C, TapsT TT, Cpml2D P, Grad G)
 {
-    using TL = Tile<H, TY>;
-    constexpr int R = TY / BLOCK_Y;
-    __shared__ float s_g0[TL::RH][TL::RW];
-    __shared__ float s_g1[TL::RH][TL::RW];
-    __shared__ float s_g2[TL::RH][TL::RW];
-    __shared__ float s_g3[TL::RH][TL::RW];
-    __shared__ float s_xx[TL::RH][TL::RW];
-    __shared__ float s_yy[TL::RH][TL::RW];
-    __shared__ float s_xy[TL::RH][TL::RW];
+    constexpr int K = 2 * H + 1, N = S + 2 * H, RW = TILE_X + 2 * H;
+    __shared__ float s_g0[W][S][RW];
+    __shared__ float s_g2[W][S][RW];
+    __shared__ float s_ty[W][2][S * K];
+    __shared__ GradSum<2, S, W> s_sum;
     const Blk B = xcd_block();
-    const int shot = B.shot;
-    const long long so = (long long)shot * L.plane;
-    const int x0 = B.bx * TILE_X, y0 = B.by * TY;
-    const int tid = threadIdx.y * TILE_X + threadIdx.x, tx = threadIdx.x;
+    const int wy = __builtin_amdgcn_readfirstlane(threadIdx.y), lane = threadIdx.x;
+    const int shot = B.shot * W + wy;
+    const bool act = shot < nshots;
+    const int x0 = B.bx * TILE_X, ya = B.by * S, xl = x0 - H, x = x0 + lane;
+    const long long so = (long long)(act ? shot : 0) * L.plane;
     const int W2 = 2 * P.W;
-
-    TileRegs<H, TY> lvx, lvy, crx, cry, fxx, fyy, fxy;
-    lvx.load(A.lvx + so, L, x0, y0, tid);
-    lvy.load(A.lvy + so, L, x0, y0, tid);
-    crx.load(C.c[2], L, x0, y0, tid);
-    cry.load(C.c[3], L, x0, y0, tid);
-    fxx.load(S.f0 + so, L, x0, y0, tid);
-    fyy.load(S.f1 + so, L, x0, y0, tid);
-    fxy.load(S.f2 + so, L, x0, y0, tid);
-    region_each<H, TY>(tid, [&](int k, int r, int c) {
-        const int x = x0 - H + c, y = y0 - H + r;
-        const float h0 = crx.v[k] * lvx.v[k];
-        const float h2 = cry.v[k] * lvy.v[k];
-        float g0 = h0, g1 = h0, g2 = h2, g3 = h2;
-        const bool in = x >= 0 && x < L.NX && y >= 0 && y < L.NY;
-        if (P.on && in) {
-            const int xi = P.xi[x], yi = P.yi[y];
-            if (xi >= 0) {
-                const long long pi = ((long long)shot * L.NY + y) * W2 + xi;
-                g0 = h0 + P.axh[x] * (M.mx[0][pi] + h0);
-                g2 = h2 + P.ax[x] * (M.mx[1][pi] + h2);
-            }
-            if (yi >= 0) {
-                const long long pi = ((long long)shot * W2 + yi) * L.NX + x;
-                g1 = h0 + P.ay[y] * (M.my[0][pi] + h0);
-                g3 = h2 + P.ayh[y] * (M.my[1][pi] + h2);
-            }
-        }
-        s_g0[r][c] = g0;
-        s_g1[r][c] = g1;
-        s_g2[r][c] = g2;
-        s_g3[r][c] = g3;
-    });
-    fxx.store(s_xx, tid);
-    fyy.store(s_yy, tid);
-    fxy.store(s_xy, tid);
-
-    const Col col = column(L, P, x0 + tx);
-    float cxf[2 * H + 1], cxb[2 * H + 1], txf[2 * H + 1], txb[2 * H + 1];
-    load_taps<H>(cxf, T.xf, col.x);
-    load_taps<H>(cxb, T.xb, col.x);
-    load_taps<H>(txf, TT.xf, col.x);
-    load_taps<H>(txb, TT.xb, col.x);
-    const int wy = __builtin_amdgcn_readfirstlane(threadIdx.y);
-    __syncthreads();
-
+    s_sum.init(wy, lane);
+    double c[2][S];
 #pragma unroll
-    for (int r = 0; r < R; r++) {
-        const int ly = wy + r * BLOCK_Y;
-        const int y = y0 + ly;
-        if (y >= L.NY) break;
-        if (!col.in) continue;
-        const int Rr = ly + H;
-        const long long o = L.at(col.x, y);
-        const float vx = A.lvx[so + o], vy = A.lvy[so + o];
-        const float h0 = C.c[2][o] * vx;
-        const float h2 = C.c[3][o] * vy;
-        // forward CPML-modified derivatives of step t (el2d_velocity)
-        const float *cyb = T.yb_v + (long long)y * (2 * H + 1);
-        const float *cyf = T.yf_v + (long long)y * (2 * H + 1);
-        float f0 = dx<H>(s_xx, Rr, tx, cxf);
-        float f1 = dy<H>(s_xy, Rr, tx, cyb);
-        float f2 = dx<H>(s_xy, Rr, tx, cxb);
-        float f3 = dy<H>(s_yy, Rr, tx, cyf);
-        const int yi = P.on ? P.yi[y] : -1;
-        if (col.xi >= 0) {
-            const long long pi = ((long long)shot * L.NY + y) * W2 + col.xi;
-            (void)cpml(f0, S.px[0][pi], col.axh, col.bxh);
-            (void)cpml(f2, S.px[1][pi], col.ax, col.bx);
-            M.mxn[0][pi] = col.bxh * (M.mx[0][pi] + h0);
-            M.mxn[1][pi] = col.bx * (M.mx[1][pi] + h2);
+    for (int r = 0; r < S; r++) c[0][r] = c[1][r] = 0.0;
+
+    if (act) {
+        const unsigned pb = (unsigned)(L.plane * 4);
+        const auto bvx = brsrc(A.lvx + so, pb), bvy = brsrc(A.lvy + so, pb);
+        const auto bcx = brsrc(C.c[2], pb), bcy = brsrc(C.c[3], pb);
+        const unsigned vc = (unsigned)(H + lane) * 4u;
+        const bool inx = x < L.NX;
+        const int xc = min(x, L.NX - 1);
+        const int hc = lane < H ? lane : TILE_X + lane;
+        const unsigned vh = lane < 2 * H ? (unsigned)hc * 4u : NO_LANE;
+        const XFrame fx = xframe(L, P, x), fh = xframe(L, P, xl + hc);
+
+        float vx[N], vy[N], cx[N], cy[N];
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+            const int ro = aboff(L, xl, ya - H + i);
+            vx[i] = bload(bvx, vc, ro);
+            vy[i] = bload(bvy, vc, ro);
+            cx[i] = bload(bcx, vc, ro);
+            cy[i] = bload(bcy, vc, ro);
         }
-        if (yi >= 0) {
-            const long long pi = ((long long)shot * W2 + yi) * L.NX + col.x;
-            (void)cpml(f1, S.py[0][pi], P.ay[y], P.by[y]);
-            (void)cpml(f3, S.py[1][pi], P.ayh[y], P.byh[y]);
-            M.myn[0][pi] = P.by[y] * (M.my[0][pi] + h0);
-            M.myn[1][pi] = P.byh[y] * (M.my[1][pi] + h2);
+        float hvx[S], hvy[S], hcx[S], hcy[S];
+#pragma unroll
+        for (int r = 0; r < S; r++) {
+            const int ro = aboff(L, xl, ya + r);
+            hvx[r] = bload(bvx, vh, ro);
+            hvy[r] = bload(bvy, vh, ro);
+            hcx[r] = bload(bcx, vh, ro);
+            hcy[r] = bload(bcy, vh, ro);
         }
-        f0 += f1;
-        f2 += f3;
-        atomicAdd(&G.g[0][o], (double)vx * f0);
-        atomicAdd(&G.g[1][o], (double)vy * f2);
-        // lsxx += Dxf^T lg0 ; lsxy += Dyb^T lg1 + Dxb^T lg2 ; lsyy += Dyf^T lg3
-        const float *tyf = TT.yf + (long long)y * (2 * H + 1);
-        const float *tyb = TT.yb + (long long)y * (2 * H + 1);
-        const float dxx = dx<H>(s_g0, Rr, tx, txf);
-        float dxy = dy<H>(s_g1, Rr, tx, tyb);
-        dxy += dx<H>(s_g2, Rr, tx, txb);
-        const float dyy = dy<H>(s_g3, Rr, tx, tyf);
-        A.lsxx[so + o] += dxx;
-        A.lsxy[so + o] += dxy;
-        A.lsyy[so + o] += dyy;
+        float my0[N], my1[N], mx0[S], mx1[S], hx0[S], hx1[S];
+        int yiw[N];
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+            const int y = ya - H + i;
+            yiw[i] = (P.on && y >= 0 && y < L.NY) ? P.yi[y] : -1;
+            my0[i] = my1[i] = 0.f;
+        }
+#pragma unroll
+        for (int r = 0; r < S; r++) mx0[r] = mx1[r] = hx0[r] = hx1[r] = 0.f;
+        if (P.on) {
+#pragma unroll
+            for (int i = 0; i < N; i++) {
+                const long long pi = ((long long)shot * W2 + max(yiw[i], 0)) * L.NX + xc;
+                my0[i] = M.my[0][pi];
+                my1[i] = M.my[1][pi];
+            }
+#pragma unroll
+            for (int r = 0; r < S; r++) {
+                const long long row = ((long long)shot * L.NY + min(ya + r, L.NY - 1)) * W2;
+                mx0[r] = M.mx[0][row + max(fx.xi, 0)];
+                mx1[r] = M.mx[1][row + max(fx.xi, 0)];
+                hx0[r] = M.mx[0][row + max(fh.xi, 0)];
+                hx1[r] = M.mx[1][row + max(fh.xi, 0)];
+            }
+        }
+        float h01[S], h23[S];
+#pragma unroll
+        for (int r = 0; r < S; r++) {
+            const long long o = so + L.at(x, min(ya + r, L.NY - 1));
+            h01[r] = F.h[0][o];
+            h23[r] = F.h[1][o];
+        }
+
+        // lg1, lg3 on the window (registers); lg0, lg2 on the strip (LDS)
+        float w1[N], w3[N];
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+            const float h0 = cx[i] * vx[i], h2 = cy[i] * vy[i];
+            float g1 = h0, g3 = h2;
+            if (yiw[i] >= 0 && inx) {
+                const int y = ya - H + i;
+                g1 = h0 + P.ay[y] * (my0[i] + h0);
+                g3 = h2 + P.ayh[y] * (my1[i] + h2);
+            }
+            w1[i] = g1;
+            w3[i] = g3;
+        }
+#pragma unroll
+        for (int r = 0; r < S; r++) {
+            const bool iny = ya + r < L.NY;
+            const int i = r + H;
+            const float h0 = cx[i] * vx[i], h2 = cy[i] * vy[i];
+            const bool fxo = fx.xi >= 0 && iny;
+            s_g0[wy][r][H + lane] = fxo ? h0 + fx.ah * (mx0[r] + h0) : h0;
+            s_g2[wy][r][H + lane] = fxo ? h2 + fx.a * (mx1[r] + h2) : h2;
+            const float hh0 = hcx[r] * hvx[r], hh2 = hcy[r] * hvy[r];
+            const bool fho = fh.xi >= 0 && iny;
+            if (lane < 2 * H) {
+                s_g0[wy][r][hc] = fho ? hh0 + fh.ah * (hx0[r] + hh0) : hh0;
+                s_g2[wy][r][hc] = fho ? hh2 + fh.a * (hx1[r] + hh2) : hh2;
+            }
+        }
+        {
+            const float *__restrict__ tb = TT.yb + (long long)ya * K;
+            const float *__restrict__ ta = TT.yf + (long long)ya * K;
+            for (int e = lane; e < S * K; e += TILE_X) {
+                s_ty[wy][0][e] = tb[e];
+                s_ty[wy][1][e] = ta[e];
+            }
+        }
+        float txf[K], txb[K];
+        load_taps<H>(txf, TT.xf, x);
+        load_taps<H>(txb, TT.xb, x);
+        wave_sync();
+
+        const auto bxx = brsrc(A.lsxx + so, pb), byy = brsrc(A.lsyy + so, pb), bxy = brsrc(A.lsxy + so, pb);
+#pragma unroll
+        for (int r = 0; r < S; r++) {
+            const int y = ya + r, i = r + H;
+            // lsxx += Dxf^T lg0 ; lsxy += Dyb^T lg1 + Dxb^T lg2 ; lsyy += Dyf^T lg3
+            float dxx = 0.f, dxy = 0.f, dyy = 0.f;
+#pragma unroll
+            for (int k = 0; k < K; k++) dxx += txf[k] * s_g0[wy][r][lane + k];
+#pragma unroll
+            for (int k = 0; k < K; k++) dxy += s_ty[wy][0][r * K + k] * w1[r + k];
+            {
+                float s = 0.f;
+#pragma unroll
+                for (int k = 0; k < K; k++) s += txb[k] * s_g2[wy][r][lane + k];
+                dxy += s;
+            }
+#pragma unroll
+            for (int k = 0; k < K; k++) dyy += s_ty[wy][1][r * K + k] * w3[r + k];
+            if (y >= L.NY || !inx) continue;
+            const int ro = aboff(L, xl, y);
+            bstore(bload(bxx, vc, ro) + dxx, bxx, vc, ro);
+            bstore(bload(bxy, vc, ro) + dxy, bxy, vc, ro);
+            bstore(bload(byy, vc, ro) + dyy, byy, vc, ro);
+            const float h0 = cx[i] * vx[i], h2 = cy[i] * vy[i];
+            if (fx.xi >= 0) {
+                const long long pi = ((long long)shot * L.NY + y) * W2 + fx.xi;
+                M.mxn[0][pi] = fx.bh * (mx0[r] + h0);
+                M.mxn[1][pi] = fx.b * (mx1[r] + h2);
+            }
+            if (yiw[i] >= 0) {
+                const long long pi = ((long long)shot * W2 + yiw[i]) * L.NX + x;
+                M.myn[0][pi] = P.by[y] * (my0[i] + h0);
+                M.myn[1][pi] = P.byh[y] * (my1[i] + h2);
+            }
+            c[0][r] = (double)vx[i] * h01[r];
+            c[1][r] = (double)vy[i] * h23[r];
+        }
     }
+    double *const g[2] = {G.g[0], G.g[1]};
+    s_sum.add(c, act, g, L, x0, ya, lane);
 }
 
 __global__ __launch_bounds__(256) void adj_inject(Layout2D L, AdjFields A, Acq Q, const float *__restrict__ adj, int t)
@@ -293,17 +497,20 @@ __global__ __launch_bounds__(256) void adj_inject(Layout2D L, AdjFields A, Acq Q
     }
 }
 
-__global__ void ckpt_maxabs(const float *__restrict__ f, long long per_shot, unsigned *maxbits)
+__global__ __launch_bounds__(256) void ckpt_maxabs(const float *__restrict__ f, long long per_shot, unsigned *maxbits)
 {
+    __shared__ float s_m[4];
     const int shot = blockIdx.y;
     const float *p = f + (long long)shot * per_shot;
     float m = 0.f;
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < per_shot;
          i += (long long)gridDim.x * blockDim.x)
         m = fmaxf(m, fabsf(p[i]));
-    // wavefront maximum, then one atomic per wavefront (non-negative floats order like their bits)
+    // wavefront maximum, block maximum, then one atomic per block (non-negative floats order like their bits)
     for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
-    if ((threadIdx.x & 63) == 0) atomicMax(maxbits + shot, __float_as_uint(m));
+    if ((threadIdx.x & 63) == 0) s_m[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicMax(maxbits + shot, __float_as_uint(fmaxf(fmaxf(s_m[0], s_m[1]), fmaxf(s_m[2], s_m[3]))));
 }
 
 __device__ __forceinline__ float ckpt_scale(unsigned bits)
@@ -350,25 +557,32 @@ struct AdjArgs {
     const Grad &G;
 };
 
-template <int H, int TY>
-void adj_launch(int which, const Layout2D &L, int nshots, const AdjArgs &a, hipStream_t st, hipEvent_t e0,
-                hipEvent_t e1)
+template <int H, int S, int W>
+void adj_launch_sw(int which, const Layout2D &L, int nshots, const AdjArgs &a, hipStream_t st, hipEvent_t e0,
+                   hipEvent_t e1)
 {
-    dim3 grid((L.NX + TILE_X - 1) / TILE_X, (L.NY + TY - 1) / TY, nshots), block(TILE_X, BLOCK_Y);
+    const dim3 grid((L.NX + TILE_X - 1) / TILE_X, (L.NY + S - 1) / S, (nshots + W - 1) / W), block(TILE_X, W);
     if (which == 0)
-        go(adj_el2d_stress<H, TY>, grid, block, st, e0, e1, L, a.A, a.M, a.S, a.C, a.T, a.TT, a.P, a.G);
+        go(adj_el2d_stress_strip<H, S, W>, grid, block, st, e0, e1, L, nshots, a.A, a.M, a.S, a.C, a.TT, a.P, a.G);
     else
-        go(adj_el2d_velocity<H, TY>, grid, block, st, e0, e1, L, a.A, a.M, a.S, a.C, a.T, a.TT, a.P, a.G);
+        go(adj_el2d_velocity_strip<H, S, W>, grid, block, st, e0, e1, L, nshots, a.A, a.M, a.S, a.C, a.TT, a.P,
+           a.G);
 }
 
+// W shots per block: up to 8 (one wavefront each)
 template <int H>
 void adj_ty(int which, int ty, const Layout2D &L, int nshots, const AdjArgs &a, hipStream_t st, hipEvent_t e0,
             hipEvent_t e1)
 {
-    if (ty == 16)
-        adj_launch<H, 16>(which, L, nshots, a, st, e0, e1);
+    (void)ty;
+    if (nshots >= 8)
+        adj_launch_sw<H, 4, 8>(which, L, nshots, a, st, e0, e1);
+    else if (nshots >= 4)
+        adj_launch_sw<H, 4, 4>(which, L, nshots, a, st, e0, e1);
+    else if (nshots >= 2)
+        adj_launch_sw<H, 4, 2>(which, L, nshots, a, st, e0, e1);
     else
-        adj_launch<H, 8>(which, L, nshots, a, st, e0, e1);
+        adj_launch_sw<H, 4, 1>(which, L, nshots, a, st, e0, e1);
 }
 
 void adj_dispatch(int which, int order, int ty, const Layout2D &L, int nshots, const AdjArgs &a, hipStream_t st,
@@ -410,7 +624,7 @@ void launch_ckpt_pack(const float *f, unsigned short *h, unsigned *maxbits, long
                       hipStream_t st)
 {
     (void)hipMemsetAsync(maxbits, 0, sizeof(unsigned) * nshots, st);
-    ckpt_maxabs<<<dim3(256, nshots), 256, 0, st>>>(f, per_shot, maxbits);
+    ckpt_maxabs<<<dim3(64, nshots), 256, 0, st>>>(f, per_shot, maxbits);
     ckpt_pack<<<2048, 256, 0, st>>>(f, (__half *)h, maxbits, per_shot, per_shot * nshots);
 }
 

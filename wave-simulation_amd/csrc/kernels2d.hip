@@ -100,21 +100,6 @@ __global__ __launch_bounds__(NTHREADS) void el2d_velocity(Layout2D L, Fields2D F
         float t2 = dy<H>(s_yy, Rr, tx, cyf);
         if (!col.in) continue;
         const long long row = so + L.at(x0, y);
-        if (S.f[0]) {
-            (S.f[0] + row)[ux] = s_xx[Rr][tx + H];
-            (S.f[1] + row)[ux] = s_yy[Rr][tx + H];
-            (S.f[2] + row)[ux] = s_xy[Rr][tx + H];
-            if (col.xi >= 0) {
-                const long long pr = ((long long)shot * L.NY + y) * W2;
-                S.px[0][pr + col.xi] = p0[r];
-                S.px[1][pr + col.xi] = p1[r];
-            }
-            if (yi[r] >= 0) {
-                const long long pr = ((long long)shot * W2 + yi[r]) * L.NX + x0;
-                S.py[0][pr + tx] = q0[r];
-                S.py[1][pr + tx] = q1[r];
-            }
-        }
         if (col.xi >= 0) {
             const long long pr = ((long long)shot * L.NY + y) * W2;
             (P.psx[0] + pr)[(unsigned)col.xi] = cpml(u, p0[r], col.axh, col.bxh);
@@ -126,17 +111,17 @@ __global__ __launch_bounds__(NTHREADS) void el2d_velocity(Layout2D L, Fields2D F
             (P.psy[1] + pr)[ux] = cpml(t2, q1[r], P.ayh[y], P.byh[y]);
         }
         u += t;
+        u2 += t2;
+        if (S.h[0]) {
+            (S.h[0] + row)[ux] = u;
+            (S.h[1] + row)[ux] = u2;
+        }
         u *= crx[r];
         const float nvx = ovx[r] + u;
-        u2 += t2;
         u2 *= cry[r];
         const float nvy = ovy[r] + u2;
         (F.f[0] + row)[ux] = nvx;
         (F.f[1] + row)[ux] = nvy;
-        if (S.f[3]) {
-            (S.f[3] + row)[ux] = nvx;
-            (S.f[4] + row)[ux] = nvy;
-        }
     }
 }
 
@@ -212,18 +197,6 @@ __global__ __launch_bounds__(NTHREADS) void el2d_stress(Layout2D L, Fields2D F, 
         float u2 = dy<H>(s_vx, Rr, tx, cyf);
         float t2 = dx<H>(s_vy, Rr, tx, cxf);
         if (!col.in) continue;
-        if (S.px[2]) {
-            if (col.xi >= 0) {
-                const long long pr = ((long long)shot * L.NY + y) * W2;
-                S.px[2][pr + col.xi] = p2[r];
-                S.px[3][pr + col.xi] = p3[r];
-            }
-            if (yi[r] >= 0) {
-                const long long pr = ((long long)shot * W2 + yi[r]) * L.NX + x0;
-                S.py[2][pr + tx] = q2[r];
-                S.py[3][pr + tx] = q3[r];
-            }
-        }
         if (col.xi >= 0) {
             const long long pr = ((long long)shot * L.NY + y) * W2;
             (P.psx[2] + pr)[(unsigned)col.xi] = cpml(vxx, p2[r], col.ax, col.bx);
@@ -242,6 +215,12 @@ __global__ __launch_bounds__(NTHREADS) void el2d_stress(Layout2D L, Fields2D F, 
         xx -= 2.f * (vyy * cmu[r]);
         yy -= 2.f * (vxx * cmu[r]);
         u2 += t2;
+        if (S.h[2]) {
+            const long long hr = so + L.at(x0, y);
+            (S.h[2] + hr)[ux] = vxx;
+            (S.h[3] + hr)[ux] = vyy;
+            (S.h[4] + hr)[ux] = u2;
+        }
         u2 *= cmxy[r];
         xy += u2;
         if (fs && y == 0) {

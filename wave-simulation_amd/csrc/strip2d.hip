@@ -97,7 +97,7 @@ __device__ __forceinline__ void stage_ytaps(float (*dst)[S * (2 * H + 1)], const
 // vx += 1/rho_x * (CPML(Dxf sxx) + CPML(Dyb sxy)); vy += 1/rho_y * (CPML(Dxb sxy) + CPML(Dyf syy))
 template <int H, int S, int U>
 __global__ __launch_bounds__(NTHREADS) void el2d_velocity_strip(Layout2D L, Fields2D F, Coef2D C, Taps T, Cpml2D P,
-                                                                Acq A, int tofs, int pend)
+                                                                Acq A, int tofs, int pend, Snap2D Sn)
 {
     constexpr int G = S / U, K = 2 * H + 1, N = S + 2 * H, RW = TILE_X + 2 * H;
     static_assert(S % U == 0, "rows per wavefront must be a multiple of the group");
@@ -201,10 +201,14 @@ __global__ __launch_bounds__(NTHREADS) void el2d_velocity_strip(Layout2D L, Fiel
                 py1[yi * L.NX + c.x0 + lane] = cpml(tt2, q1[r], P.ayh[y], P.byh[y]);
             }
             u += tt;
-            u *= crx[r];
             u2 += tt2;
-            u2 *= cry[r];
             const int ro = boff(L, c.xl, y);
+            if (Sn.h[0]) {
+                Sn.h[0][c.so + L.at(c.x0, y) + lane] = u;
+                Sn.h[1][c.so + L.at(c.x0, y) + lane] = u2;
+            }
+            u *= crx[r];
+            u2 *= cry[r];
             bstore(ovx[r] + u, bvx, c.vc, ro);
             bstore(ovy[r] + u2, bvy, c.vc, ro);
         }
@@ -215,7 +219,7 @@ __global__ __launch_bounds__(NTHREADS) void el2d_velocity_strip(Layout2D L, Fiel
 // stress update of one row (ForwardSolver2Delastic.cpp:204-294)
 template <int H, int S, int U>
 __global__ __launch_bounds__(NTHREADS) void el2d_stress_strip(Layout2D L, Fields2D F, Coef2D C, Taps T, Cpml2D P,
-                                                              Acq A, int tofs, int pend)
+                                                              Acq A, int tofs, int pend, Snap2D Sn)
 {
     (void)pend;
     constexpr int G = S / U, K = 2 * H + 1, N = S + 2 * H, RW = TILE_X + 2 * H;
@@ -340,6 +344,12 @@ __global__ __launch_bounds__(NTHREADS) void el2d_stress_strip(Layout2D L, Fields
             xx -= 2.f * (a3 * cmu[r]);
             yy -= 2.f * (a1 * cmu[r]);
             u2 += a2;
+            if (Sn.h[2]) {
+                const long long hr = c.so + L.at(c.x0, y) + lane;
+                Sn.h[2][hr] = a1;
+                Sn.h[3][hr] = a3;
+                Sn.h[4][hr] = u2;
+            }
             u2 *= cmxy[r];
             xy += u2;
             if (fs && y == 0) {
@@ -372,27 +382,29 @@ void go(Kern k, dim3 grid, dim3 block, hipStream_t st, hipEvent_t e0, hipEvent_t
 
 template <int H, int S, int U>
 void launch_hs(int which, const Layout2D &L, int nshots, const Fields2D &F, const Coef2D &C, const Taps &T,
-               const Cpml2D &P, const Acq &A, int tofs, int pend, hipStream_t st, hipEvent_t e0, hipEvent_t e1)
+               const Cpml2D &P, const Acq &A, int tofs, int pend, hipStream_t st, hipEvent_t e0, hipEvent_t e1,
+               const Snap2D &Sn)
 {
     constexpr int TY = S * BLOCK_Y;
     const dim3 grid((L.NX + TILE_X - 1) / TILE_X, (L.NY + TY - 1) / TY, nshots), block(TILE_X, BLOCK_Y);
     if (which == 0)
-        go(el2d_velocity_strip<H, S, U>, grid, block, st, e0, e1, L, F, C, T, P, A, tofs, pend);
+        go(el2d_velocity_strip<H, S, U>, grid, block, st, e0, e1, L, F, C, T, P, A, tofs, pend, Sn);
     else
-        go(el2d_stress_strip<H, S, U>, grid, block, st, e0, e1, L, F, C, T, P, A, tofs, pend);
+        go(el2d_stress_strip<H, S, U>, grid, block, st, e0, e1, L, F, C, T, P, A, tofs, pend, Sn);
 }
 
 template <int H>
 void launch_s(int tile_y, int which, const Layout2D &L, int nshots, const Fields2D &F, const Coef2D &C, const Taps &T,
-              const Cpml2D &P, const Acq &A, int tofs, int pend, hipStream_t st, hipEvent_t e0, hipEvent_t e1)
+              const Cpml2D &P, const Acq &A, int tofs, int pend, hipStream_t st, hipEvent_t e0, hipEvent_t e1,
+              const Snap2D &Sn)
 {
     // one group per wavefront: measured faster than marching with prefetch on gfx950
     if (tile_y == 8)
-        launch_hs<H, 2, 2>(which, L, nshots, F, C, T, P, A, tofs, pend, st, e0, e1);
+        launch_hs<H, 2, 2>(which, L, nshots, F, C, T, P, A, tofs, pend, st, e0, e1, Sn);
     else if (tile_y == 32)
-        launch_hs<H, 8, 8>(which, L, nshots, F, C, T, P, A, tofs, pend, st, e0, e1);
+        launch_hs<H, 8, 8>(which, L, nshots, F, C, T, P, A, tofs, pend, st, e0, e1, Sn);
     else
-        launch_hs<H, 4, 4>(which, L, nshots, F, C, T, P, A, tofs, pend, st, e0, e1);
+        launch_hs<H, 4, 4>(which, L, nshots, F, C, T, P, A, tofs, pend, st, e0, e1, Sn);
 }
 
 }  // namespace
@@ -401,15 +413,16 @@ bool strip2d_fits(const Layout2D &L) { return L.plane * 4 < (1LL << 31); }
 
 void launch_strip2d(int order, int tile_y, const Layout2D &L, int nshots, const Fields2D &F, const Coef2D &C,
                     const Taps &T, const Cpml2D &P, const Acq &A, int tofs, int pend, hipStream_t stream, int which,
-                    hipEvent_t e0, hipEvent_t e1)
+                    hipEvent_t e0, hipEvent_t e1, const Snap2D *snap)
 {
+    const Snap2D Sn = snap ? *snap : Snap2D{};
     switch (order / 2) {
-    case 1: launch_s<1>(tile_y, which, L, nshots, F, C, T, P, A, tofs, pend, stream, e0, e1); break;
-    case 2: launch_s<2>(tile_y, which, L, nshots, F, C, T, P, A, tofs, pend, stream, e0, e1); break;
-    case 3: launch_s<3>(tile_y, which, L, nshots, F, C, T, P, A, tofs, pend, stream, e0, e1); break;
-    case 4: launch_s<4>(tile_y, which, L, nshots, F, C, T, P, A, tofs, pend, stream, e0, e1); break;
-    case 5: launch_s<5>(tile_y, which, L, nshots, F, C, T, P, A, tofs, pend, stream, e0, e1); break;
-    case 6: launch_s<6>(tile_y, which, L, nshots, F, C, T, P, A, tofs, pend, stream, e0, e1); break;
+    case 1: launch_s<1>(tile_y, which, L, nshots, F, C, T, P, A, tofs, pend, stream, e0, e1, Sn); break;
+    case 2: launch_s<2>(tile_y, which, L, nshots, F, C, T, P, A, tofs, pend, stream, e0, e1, Sn); break;
+    case 3: launch_s<3>(tile_y, which, L, nshots, F, C, T, P, A, tofs, pend, stream, e0, e1, Sn); break;
+    case 4: launch_s<4>(tile_y, which, L, nshots, F, C, T, P, A, tofs, pend, stream, e0, e1, Sn); break;
+    case 5: launch_s<5>(tile_y, which, L, nshots, F, C, T, P, A, tofs, pend, stream, e0, e1, Sn); break;
+    case 6: launch_s<6>(tile_y, which, L, nshots, F, C, T, P, A, tofs, pend, stream, e0, e1, Sn); break;
     default: break;
     }
 }

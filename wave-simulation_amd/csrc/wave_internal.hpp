@@ -110,14 +110,14 @@ struct Acq {
     const int *tbase;           // device step counter base
 };
 
-// Forward quantities a 2D elastic step hands to the adjoint's recomputation
-// (all pointers null when not recording): stresses at the start of the step
-// (f[0..2] = sxx syy sxy, own cells), velocities after the velocity update
-// (f[3..4] = vx vy), CPML memory variables at the start of the step, same
-// layouts as the fields / Cpml2D.
+// Forward quantities a 2D elastic step hands to the adjoint's gradient (all
+// pointers null when not recording), per shot cell, same layout as the fields:
+// the CPML-modified derivative sums of the step, h = g + (b psi + a g):
+//   h[0] = h0 + h1 (vx update / (1/rho_x)),  h[1] = h2 + h3 (vy update / (1/rho_y)),
+//   h[2] = h4 = CPML(Dxb vx),  h[3] = h5 = CPML(Dyb vy),  h[4] = h6 + h7 (sxy update / mu_xy).
+// The adjoint step needs nothing else from the forward run.
 struct Snap2D {
-    float *f[5];
-    float *px[4], *py[4];
+    float *h[5];
 };
 
 // Launchers (kernels2d.hip).  All launches go to `stream`.
@@ -131,12 +131,12 @@ void launch_step2d(int physics, int order, int tile_y, const Layout2D &L, int ns
 // velocity-type sources / receivers of step tbase + tofs (end of a run segment)
 // Row-strip 2D elastic kernels (strip2d.hip): blocks of 64 columns x tile_y rows,
 // tile_y in {8, 16, 32} (each of the 4 wavefronts owns tile_y/4 rows); same
-// phases and acquisition lists as launch_step2d (no adjoint snapshots).  Needs
-// planes below 2 GiB (strip2d_fits).
+// phases, acquisition lists and recorded quantities (snap) as launch_step2d.
+// Needs planes below 2 GiB (strip2d_fits).
 bool strip2d_fits(const Layout2D &L);
 void launch_strip2d(int order, int tile_y, const Layout2D &L, int nshots, const Fields2D &F, const Coef2D &C,
                     const Taps &T, const Cpml2D &P, const Acq &A, int tofs, int pend, hipStream_t stream, int which,
-                    hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
+                    hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr, const Snap2D *snap = nullptr);
 void launch_flush2d(int physics, const Layout2D &L, const Fields2D &F, const Acq &A, int tofs, hipStream_t stream);
 void launch_abs_velocity2d(int physics, const Layout2D &L, int nshots, const Fields2D &F, const Coef2D &C,
                            hipStream_t stream);
