@@ -124,13 +124,18 @@ def bench_batched_tmem():
 
 
 def bench_adjoint():
-    """Forward + adjoint gradient of 16 batched 2D elastic shots (600x600, 3000 steps)."""
+    """Forward + adjoint gradient of 16 batched 2D elastic shots (600x600, 3000 steps), in the two memory
+    plans: the FP16 record of every step (when it fits the device memory) and FP16 checkpoints every 50 steps
+    with recomputation."""
     import numpy as np
     from wavesim import cases
+    import wavesim
 
     case = cases.elastic_c4()
     NT = case.settings["NT"]
-    sol = cases.make_solver(case, steps_per_graph=50)
+    n = len(case.shots)
+    # the per-step record is written by the row-strip kernels: tune among their block heights
+    sol = cases.make_solver(case, steps_per_graph=50, path="strip")
     choice = sol.tune(100)
     sol.reset()
     sol.run(0, NT)
@@ -138,36 +143,42 @@ def bench_adjoint():
     sol.reset()
     sol.run(0, NT)
     t_fwd = sol.last_run_ms()
-    sol.set_checkpointing(50, True)
-    sol.run(0, NT)
-    sol.synchronize()  # first checkpointed run allocates and captures
-    sol.reset()
-    sol.run(0, NT)
-    t_ck = sol.last_run_ms()
-    for k in range(len(case.shots)):
-        sol.set_adjoint_source(k, sol.seismograms(k))  # J = 1/2 ||d||^2
-    sol.run_adjoint()
-    t_adj = sol.last_run_ms()
-    g = sol.gradient("vp")
-    assert np.isfinite(g).all()
-    n = len(case.shots)
-    # per-kernel split of the adjoint pass (dispatch-attached events, eager)
-    import wavesim
-
     L = wavesim.load()
-    wavesim._abi.check(L.wv_set_kernel_timing(sol._h, 1))
-    sol.run_adjoint()
-    sol.synchronize()
-    kern = _kernel_stats(sol, L, wavesim, kinds=(3, 4, 5, 6))
-    wavesim._abi.check(L.wv_set_kernel_timing(sol._h, 0))
-    return {"workload": "2D elastic order 8, 600x600, 3000 steps, 16 shots: forward + adjoint gradient "
-                        "(checkpoints every 50 steps in FP16, FP64 accumulation)",
-            "forward_kernel": choice, "forward_ms": round(t_fwd, 2), "checkpointed_forward_ms": round(t_ck, 2),
-            "adjoint_ms": round(t_adj, 2), "gradient_cost_vs_forward": round((t_ck + t_adj) / t_fwd, 3),
-            "gradient_shots_per_s": round(n / ((t_ck + t_adj) * 1e-3), 3),
-            "forward_shots_per_s": round(n / (t_fwd * 1e-3), 3),
-            "adjoint_kernels": [{"name": k["name"], "avg_us": round(k["avg_ms"] * 1e3, 2), "launches": k["launches"],
-                                 "GB_per_s": round(k["bytes"] / max(k["avg_ms"], 1e-12) / 1e6, 1)} for k in kern]}
+    out = {"workload": "2D elastic order 8, 600x600, 3000 steps, 16 shots: forward + adjoint gradient "
+                       "(FP64 gradient accumulation)",
+           "forward_kernel": choice, "forward_ms": round(t_fwd, 2),
+           "forward_shots_per_s": round(n / (t_fwd * 1e-3), 3)}
+    for label, store in (("recorded", -1.0), ("checkpointed", 0.0)):
+        sol.set_adjoint_store(store)
+        sol.set_checkpointing(50, True)
+        sol.run(0, NT)
+        sol.synchronize()  # first run of the plan allocates and captures
+        sol.reset()
+        sol.run(0, NT)
+        t_ck = sol.last_run_ms()
+        for k in range(n):
+            sol.set_adjoint_source(k, sol.seismograms(k))  # J = 1/2 ||d||^2
+        sol.run_adjoint()
+        t_adj = sol.last_run_ms()
+        g = sol.gradient("vp")
+        assert np.isfinite(g).all()
+        # per-kernel split of the adjoint pass (dispatch-attached events, eager)
+        wavesim._abi.check(L.wv_set_kernel_timing(sol._h, 1))
+        sol.run_adjoint()
+        sol.synchronize()
+        kern = _kernel_stats(sol, L, wavesim, kinds=(3, 4, 5, 6))
+        wavesim._abi.check(L.wv_set_kernel_timing(sol._h, 0))
+        plan = ("FP16 record of every step's derivative sums, no recomputation" if kern[0]["launches"] == 0
+                else "FP16 checkpoints every 50 steps, segment recomputation")
+        out[label] = {"plan": plan, "forward_with_plan_ms": round(t_ck, 2), "adjoint_ms": round(t_adj, 2),
+                      "gradient_cost_vs_forward": round((t_ck + t_adj) / t_fwd, 3),
+                      "gradient_shots_per_s": round(n / ((t_ck + t_adj) * 1e-3), 3),
+                      "adjoint_kernels": [{"name": k["name"], "avg_us": round(k["avg_ms"] * 1e3, 2),
+                                           "launches": k["launches"],
+                                           "GB_per_s": round(k["bytes"] / max(k["avg_ms"], 1e-12) / 1e6, 1)}
+                                          for k in kern if k["launches"]]}
+    out["gradient_cost_vs_forward"] = out["recorded"]["gradient_cost_vs_forward"]
+    return out
 
 
 def bench_3d(wavesim, L):
@@ -308,7 +319,8 @@ def main():
         out["batched_tmem_64_shots"] = bench_batched_tmem()
         _progress(f"batched TM-EM: {out['batched_tmem_64_shots']['shots_per_s']} shots/s")
         out["adjoint_16_shots"] = bench_adjoint()
-        _progress(f"adjoint: cost {out['adjoint_16_shots']['gradient_cost_vs_forward']} x forward")
+        _progress(f"adjoint: cost {out['adjoint_16_shots']['recorded']['gradient_cost_vs_forward']} x forward "
+                  f"(recorded), {out['adjoint_16_shots']['checkpointed']['gradient_cost_vs_forward']} x (checkpointed)")
         out["elastic3d_8_shots"] = bench_3d(wavesim, L)
         _progress(f"3D: {out['elastic3d_8_shots']['grid_cell_updates_per_s']:.4g} cell-updates/s")
     if not args.no_cpu:

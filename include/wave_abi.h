@@ -151,16 +151,26 @@ const char *wv_last_error(void);
 
 /* Checkpoint every `every` steps during the following wv_run calls (0 = off);
  * fp16 != 0 stores the wavefields in FP16 with one scale per field and shot,
- * otherwise FP32; CPML memory variables are kept in FP32.  Switches the handle
- * to the two-launch kernels, uses graph segments of `every` steps and resets
- * the wavefields. */
+ * otherwise FP32; CPML memory variables are kept in FP32.  Switches a handle
+ * on the single-pass kernel to the two-launch kernels, uses graph segments of
+ * `every` steps and resets the wavefields. */
 int wv_set_checkpointing(wv_solver *s, int every, int fp16);
+
+/* Device-memory budget [GiB] for recording, during the checkpointed forward
+ * run, the five derivative sums of every step (FP16, one scale per row of 64
+ * cells, field and shot: about 10 B per cell, shot and step) instead of
+ * checkpointing and recomputing; used when the handle runs the row-strip
+ * kernels and the record fits.  < 0 (default): 3/4 of the free device memory
+ * at wv_set_checkpointing; 0: always checkpoint and recompute.  Takes effect
+ * at the next wv_set_checkpointing.  No counterpart in the reference. */
+int wv_set_adjoint_store(wv_solver *s, double max_gib);
 
 /* adjoint source of one shot: dJ/d(seismogram), [nrec][NT] in receiver order */
 int wv_set_adjoint_source(wv_solver *s, int shot, const float *adj);
 
 /* Time-reversed adjoint pass over [0, NT) after a checkpointed wv_run(0, NT):
- * recomputes each checkpoint segment and correlates with the adjoint fields;
+ * recomputes each checkpoint segment (or reads the per-step record) and
+ * correlates with the adjoint fields;
  * the gradient is summed over all shots of the handle in FP64.  Asynchronous
  * on the handle's stream; wv_last_run_ms reports its duration. */
 int wv_run_adjoint(wv_solver *s);

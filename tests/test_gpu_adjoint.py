@@ -50,8 +50,10 @@ def _oracle_gradient(c, obs, resid=None):
     return J, g
 
 
-def _gpu_gradient(c, obs, every=50, fp16=True, **kw):
+def _gpu_gradient(c, obs, every=50, fp16=True, store=0.0, **kw):
+    """store: device-memory budget [GiB] for recording every step (0: checkpoints and recomputation)."""
     sol = cases.make_solver(c, **kw)
+    sol.set_adjoint_store(store)
     sol.set_checkpointing(every, fp16)
     sol.run(0, c.settings["NT"])
     J = 0.0
@@ -89,6 +91,32 @@ def test_gradient_with_row_strip_recomputation(tile_y):
     """The recomputation runs on the row-strip kernels (they record the derivative sums too)."""
     c, obs = _c4()
     _, gg, _, resid = _gpu_gradient(c, obs, every=40, fp16=True, path="strip", tile_y=tile_y)
+    _, go = _oracle_gradient(c, obs, resid=resid)
+    for k, name in enumerate(("vp", "vs", "rho")):
+        e = rel_l2(gg[k], go[k])
+        print(f"strip-{tile_y} {name}: relL2 vs FP64 adjoint {e:.2e}")
+        assert e <= GTOL, (name, e)
+
+
+@pytest.mark.parametrize("tile_y", [8, 16, 32])
+def test_gradient_from_recorded_run(tile_y):
+    """Every step's derivative sums recorded in FP16 during the forward run (no
+    checkpoints, no recomputation), uneven shots per block (3 shots)."""
+    c, obs = _c4(N=70, NT=260, nshots=3)
+    _, gg, sol, resid = _gpu_gradient(c, obs, every=40, fp16=True, store=-1, path="strip", tile_y=tile_y)
+    # the adjoint pass read the record: no recomputation launches
+    import ctypes as C
+    import wavesim
+
+    L = wavesim.load()
+    wavesim._abi.check(L.wv_set_kernel_timing(sol._h, 1))
+    sol.run_adjoint()
+    sol.synchronize()
+    name, avg, n, b = C.c_char_p(), C.c_double(), C.c_longlong(), C.c_double()
+    wavesim._abi.check(L.wv_kernel_stats(sol._h, 3, C.byref(name), C.byref(avg), C.byref(n), C.byref(b)))
+    assert n.value == 0
+    wavesim._abi.check(L.wv_kernel_stats(sol._h, 5, C.byref(name), C.byref(avg), C.byref(n), C.byref(b)))
+    assert n.value == c.settings["NT"]
     _, go = _oracle_gradient(c, obs, resid=resid)
     for k, name in enumerate(("vp", "vs", "rho")):
         e = rel_l2(gg[k], go[k])
@@ -173,20 +201,22 @@ def test_torch_autograd_function():
         assert rel_l2(t.grad.numpy().ravel(), g) <= GTOL
 
 
-def test_adjoint_cost_ratio():
-    """Time of (checkpointed forward + adjoint) over the plain forward run, printed."""
+@pytest.mark.parametrize("store", [0.0, -1.0])
+def test_adjoint_cost_ratio(store):
+    """Time of (checkpointed or recording forward run + adjoint) over the forward run, printed."""
     c, obs = _c4(N=200, NT=400, nshots=4, W=20)
-    sol = cases.make_solver(c, steps_per_graph=50)
+    sol = cases.make_solver(c, steps_per_graph=50, path="strip", tile_y=16)
     sol.run(0, 400)
     sol.synchronize()
     sol.reset()
     sol.run(0, 400)
     t_fwd = sol.last_run_ms()
-    _, _, sadj, _ = _gpu_gradient(c, obs, every=50, fp16=True)
+    _, _, sadj, _ = _gpu_gradient(c, obs, every=50, fp16=True, store=store, path="strip", tile_y=16)
     t_adj = sadj.last_run_ms()
     sadj.reset()
     sadj.run(0, 400)
     t_ck = sadj.last_run_ms()
     ratio = (t_ck + t_adj) / t_fwd
-    print(f"forward {t_fwd:.2f} ms, checkpointed forward {t_ck:.2f} ms, adjoint {t_adj:.2f} ms, ratio {ratio:.2f}")
+    print(f"store={store}: forward {t_fwd:.2f} ms, forward with checkpoints/record {t_ck:.2f} ms, "
+          f"adjoint {t_adj:.2f} ms, ratio {ratio:.2f}")
     assert ratio < 20

@@ -168,8 +168,9 @@ int wv_solver::enqueue_steps(int nsteps, bool timed)
             const wv::Fields2D F = fields2d(cur);
             const wv::Cpml2D P = cpml2d(cur);
             if (variant == V_STRIP) {
-                wv::launch_strip2d(order, tile_y, L, cfg.nshots, F, C, T, P, A, j, j > 0, st, 0, e0, e1);
-                wv::launch_strip2d(order, tile_y, L, cfg.nshots, F, C, T, P, A, j, 0, st, 1, e2, e3);
+                const wv::Rec2D *rec = adjoint_record(this);
+                wv::launch_strip2d(order, tile_y, L, cfg.nshots, F, C, T, P, A, j, j > 0, st, 0, e0, e1, nullptr, rec);
+                wv::launch_strip2d(order, tile_y, L, cfg.nshots, F, C, T, P, A, j, 0, st, 1, e2, e3, nullptr, rec);
             } else {
                 wv::launch_step2d(physics, order, tile_y, L, cfg.nshots, F, C, T, P, A, j, j > 0, st, 0, e0, e1);
                 wv::launch_step2d(physics, order, tile_y, L, cfg.nshots, F, C, T, P, A, j, 0, st, 1, e2, e3);

@@ -5,12 +5,17 @@
 // five wavefields, FP16 with one scale per field and shot, or FP32, plus the
 // CPML memory variables in FP32) is stored in HBM.  The backward pass walks
 // the checkpoints from the last to the first: it restores one, recomputes its
-// segment forward while keeping the per-step quantities the gradient needs
-// (stresses at the start of each step, velocities after each velocity
-// update, CPML memory variables), then runs the adjoint steps of the segment
-// in reverse order.  Gradients with respect to 1/rho_x, 1/rho_y, M, mu, mu_xy
-// accumulate in FP64 on the GPU; the chain rule to vp, vs, rho follows the
-// averaging rules of the coefficient preparation (host, FP64).
+// segment forward while the forward kernels record the per-step quantities
+// the gradient needs (the five CPML-modified derivative sums), then runs the
+// adjoint steps of the segment in reverse order.
+// When the FP16 record of every step of the run fits in the device-memory
+// budget (wv_set_adjoint_store) and the handle runs the row-strip kernels,
+// the forward run records the sums of every step directly (FP16, one scale
+// per row of 64 cells, field and shot) and the backward pass needs neither
+// checkpoints nor recomputation.
+// Gradients with respect to 1/rho_x, 1/rho_y, M, mu, mu_xy accumulate in FP64
+// on the GPU; the chain rule to vp, vs, rho follows the averaging rules of the
+// coefficient preparation (host, FP64).
 #include <cmath>
 #include <cstring>
 
@@ -43,6 +48,9 @@ struct AdjointState {
     size_t zero_n = 0;
     float *tapT[4] = {};
     std::vector<void *> allocs, ck_allocs;
+    double store_gib = -1.0;  // budget of the per-step record (< 0: 3/4 of the free device memory)
+    bool store = false;       // this checkpointing setup records every step instead of checkpointing
+    wv::Rec2D rec{};
 };
 
 namespace {
@@ -155,10 +163,20 @@ void adjoint_destroy(AdjointState *a)
 
 int adjoint_every(const wv_solver *s) { return s->adj ? s->adj->every : 0; }
 
+const wv::Rec2D *adjoint_record(const wv_solver *s)
+{
+    const AdjointState *a = s->adj;
+    return (a && a->every && a->store) ? &a->rec : nullptr;
+}
+
 int adjoint_checkpoint(wv_solver *s, int t)
 {
     AdjointState *a = s->adj;
     if (!a || !a->every || t % a->every || !a->valid_run) return 0;
+    if (a->store) {
+        a->have[t / a->every] = 1;  // the record replaces the checkpoint
+        return 0;
+    }
     const int k = t / a->every;
     if (k >= a->nck) return 0;
     for (int f = 0; f < 5; f++) {
@@ -229,6 +247,8 @@ int wv_set_checkpointing(wv_solver *s, int every, int fp16)
     a->ck_fields.clear();
     a->ck_max.clear();
     a->ck_psi.clear();
+    a->store = false;
+    a->rec = wv::Rec2D{};
     a->every = every;
     a->fp16 = fp16 != 0;
     a->forward_done = false;
@@ -249,6 +269,34 @@ int wv_set_checkpointing(wv_solver *s, int every, int fp16)
     a->fld_n = (size_t)s->L.plane * s->cfg.nshots;
     a->nck = (s->cfg.NT + every - 1) / every;
     a->have.assign(a->nck, 0);
+    // record every step instead when it fits (row-strip kernels only)
+    if (s->variant == V_STRIP) {
+        const int nbx = (s->L.NX + wv::TILE_X - 1) / wv::TILE_X, srows = s->tile_y / 4;
+        const int nty = (s->L.NY + srows - 1) / srows;
+        const size_t fss = (size_t)s->cfg.NT * 5 * s->cfg.nshots;  // (step, field, shot) planes
+        const size_t rows = fss * s->L.NY, strips = fss * nty * nbx;
+        const double need = (double)rows * s->L.NX * 2 + (double)strips * 4;
+        size_t free_b = 0, total_b = 0;
+        HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+        const double budget = a->store_gib < 0 ? 0.75 * (double)free_b : a->store_gib * 1073741824.0;
+        if (need <= budget) {
+            // every value is written by the forward run before the adjoint reads it: no clearing
+            void *h = nullptr, *sc = nullptr;
+            if (hipMalloc(&h, rows * s->L.NX * 2) == hipSuccess) a->ck_allocs.push_back(h);
+            if (h && hipMalloc(&sc, strips * 4) == hipSuccess) a->ck_allocs.push_back(sc);
+            if (h && sc) {
+                a->store = true;
+                a->rec.h = (unsigned short *)h;
+                a->rec.scale = (float *)sc;
+                a->rec.nbx = nbx;
+                a->rec.nty = nty;
+                a->rec.srows = srows;
+                return wv_reset(s);
+            }
+            (void)hipGetLastError();  // did not fit after all: checkpoints and recomputation
+            free_all(a->ck_allocs);
+        }
+    }
     const size_t np = psi_total(s);
     for (int k = 0; k < a->nck; k++) {
         int rc;
@@ -273,6 +321,16 @@ int wv_set_checkpointing(wv_solver *s, int every, int fp16)
     return wv_reset(s);
 }
 
+int wv_set_adjoint_store(wv_solver *s, double max_gib)
+{
+    if (!s || !s->inited) return fail(WV_ESTATE, "wv_set_adjoint_store before wv_init");
+    if (!(max_gib == max_gib)) return fail(WV_EINVAL, "bad budget");
+    if (!s->adj) s->adj = new AdjointState();
+    s->adj->store_gib = max_gib;
+    s->adj->every = 0;  // the next wv_set_checkpointing chooses the memory plan again
+    return WV_OK;
+}
+
 int wv_set_adjoint_source(wv_solver *s, int shot, const float *adj)
 {
     if (!s || !s->inited || !s->adj || !s->adj->every) return fail(WV_ESTATE, "checkpointing is not enabled");
@@ -291,6 +349,8 @@ int wv_run_adjoint(wv_solver *s)
     if (!a->forward_done) return fail(WV_ESTATE, "wv_run_adjoint needs a checkpointed forward run over [0, NT)");
     for (int k = 0; k < a->nck; k++)
         if (!a->have[k]) return fail(WV_ESTATE, "missing checkpoint");
+    if (a->store && (s->variant != V_STRIP || s->tile_y / 4 != a->rec.srows))
+        return fail(WV_ESTATE, "the kernel variant changed after wv_set_checkpointing: repeat it and the forward run");
     int rc;
     if ((rc = s->sync_acquisition())) return rc;
     const wv::Layout2D &L = s->L;
@@ -333,7 +393,7 @@ int wv_run_adjoint(wv_solver *s)
             a->tapT[q] = base + (size_t)ORG * (2 * H + 1);
         }
     }
-    if (a->seg_slots < K) {
+    if (!a->store && a->seg_slots < K) {
         if ((rc = adalloc(a->allocs, &a->segH, (size_t)K * 5 * a->fld_n, st))) return rc;
         a->seg_slots = K;
     }
@@ -391,44 +451,56 @@ int wv_run_adjoint(wv_solver *s)
 
     for (int k = a->nck - 1; k >= 0; k--) {
         const int t0 = k * K, n = std::min(K, NT - t0);
-        // restore the checkpoint
-        for (int f = 0; f < 5; f++) {
-            if (a->fp16)
-                wv::launch_ckpt_unpack((const unsigned short *)a->ck_fields[k] + (size_t)f * a->fld_n,
-                                       s->field[0][f], a->ck_max[k] + (size_t)f * ns, L.plane, ns, st);
-            else
-                HIP_TRY(hipMemcpyAsync(s->field[0][f], (const float *)a->ck_fields[k] + (size_t)f * a->fld_n,
-                                       a->fld_n * sizeof(float), hipMemcpyDeviceToDevice, st));
-        }
-        if (np) {
-            const float *src = a->ck_psi[k];
-            for (int q = 0; q < 8; q++) {
-                HIP_TRY(hipMemcpyAsync(psi[q], src, pn[q] * sizeof(float), hipMemcpyDeviceToDevice, st));
-                src += pn[q];
+        if (!a->store) {
+            // restore the checkpoint
+            for (int f = 0; f < 5; f++) {
+                if (a->fp16)
+                    wv::launch_ckpt_unpack((const unsigned short *)a->ck_fields[k] + (size_t)f * a->fld_n,
+                                           s->field[0][f], a->ck_max[k] + (size_t)f * ns, L.plane, ns, st);
+                else
+                    HIP_TRY(hipMemcpyAsync(s->field[0][f], (const float *)a->ck_fields[k] + (size_t)f * a->fld_n,
+                                           a->fld_n * sizeof(float), hipMemcpyDeviceToDevice, st));
             }
-        }
-        wv::launch_set(s->d_tbase, t0, st);
-        // recompute the segment; the forward kernels record what the adjoint steps read
-        for (int j = 0; j < n; j++) {
-            wv::Snap2D snap{};
-            for (int f = 0; f < 5; f++) snap.h[f] = a->segH + ((size_t)j * 5 + f) * a->fld_n;
-            const bool strip = s->variant == V_STRIP;
-            if ((rc = kt.next(3, &e0, &e1))) return rc;
-            if (strip)
-                wv::launch_strip2d(order, s->tile_y, L, ns, F, C, T, P, Arec, j, j > 0, st, 0, e0, e1, &snap);
-            else
-                wv::launch_step2d(wv::ELASTIC, order, s->tile_y, L, ns, F, C, T, P, Arec, j, j > 0, st, 0, e0, e1, &snap);
-            if ((rc = kt.next(4, &e0, &e1))) return rc;
-            if (strip)
-                wv::launch_strip2d(order, s->tile_y, L, ns, F, C, T, P, Arec, j, 0, st, 1, e0, e1, &snap);
-            else
-                wv::launch_step2d(wv::ELASTIC, order, s->tile_y, L, ns, F, C, T, P, Arec, j, 0, st, 1, e0, e1, &snap);
+            if (np) {
+                const float *src = a->ck_psi[k];
+                for (int q = 0; q < 8; q++) {
+                    HIP_TRY(hipMemcpyAsync(psi[q], src, pn[q] * sizeof(float), hipMemcpyDeviceToDevice, st));
+                    src += pn[q];
+                }
+            }
+            wv::launch_set(s->d_tbase, t0, st);
+            // recompute the segment; the forward kernels record what the adjoint steps read
+            for (int j = 0; j < n; j++) {
+                wv::Snap2D snap{};
+                for (int f = 0; f < 5; f++) snap.h[f] = a->segH + ((size_t)j * 5 + f) * a->fld_n;
+                const bool strip = s->variant == V_STRIP;
+                if ((rc = kt.next(3, &e0, &e1))) return rc;
+                if (strip)
+                    wv::launch_strip2d(order, s->tile_y, L, ns, F, C, T, P, Arec, j, j > 0, st, 0, e0, e1, &snap);
+                else
+                    wv::launch_step2d(wv::ELASTIC, order, s->tile_y, L, ns, F, C, T, P, Arec, j, j > 0, st, 0, e0, e1, &snap);
+                if ((rc = kt.next(4, &e0, &e1))) return rc;
+                if (strip)
+                    wv::launch_strip2d(order, s->tile_y, L, ns, F, C, T, P, Arec, j, 0, st, 1, e0, e1, &snap);
+                else
+                    wv::launch_step2d(wv::ELASTIC, order, s->tile_y, L, ns, F, C, T, P, Arec, j, 0, st, 1, e0, e1, &snap);
+            }
         }
         // adjoint steps of the segment, last to first
         for (int j = n - 1; j >= 0; j--) {
             const int t = t0 + j;
             wv::FwdSnap SV{}, SS{};  // stress-phase and velocity-phase sums of step t
-            for (int f = 0; f < 5; f++) SV.h[f] = SS.h[f] = a->segH + ((size_t)j * 5 + f) * a->fld_n;
+            if (a->store) {
+                SV.hs = SS.hs = a->rec.h;
+                SV.ss = SS.ss = a->rec.scale;
+                SV.t = SS.t = t;
+                SV.nbx = SS.nbx = a->rec.nbx;
+                SV.nty = SS.nty = a->rec.nty;
+                SV.srows = SS.srows = a->rec.srows;
+                SV.nshots = SS.nshots = ns;
+            } else {
+                for (int f = 0; f < 5; f++) SV.h[f] = SS.h[f] = a->segH + ((size_t)j * 5 + f) * a->fld_n;
+            }
             wv::launch_adj_inject(L, AF, Aq, a->adj_src, t, st);
             wv::AdjPsi M{};
             for (int q = 0; q < 4; q++) {

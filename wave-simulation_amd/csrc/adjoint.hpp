@@ -26,6 +26,10 @@ struct AdjPsi {
 // h[0] = h0 + h1, h[1] = h2 + h3, h[2] = h4, h[3] = h5, h[4] = h6 + h7.
 struct FwdSnap {
     const float *h[5];
+    // or, when hs != nullptr, the FP16 record of the whole run (Rec2D layout) at step t
+    const unsigned short *hs;
+    const float *ss;
+    int t, nbx, nty, srows, nshots;
 };
 
 // transposed derivative tables: (D^T w)[j] = sum_k t[j][k] w[j + k - H]

@@ -116,13 +116,25 @@ struct GradSum {
     }
 };
 
+// forward derivative sum f of (shot, x, y) at padded offset o = shot plane + at(x, y)
+template <bool REC>
+__device__ __forceinline__ float fwd_sum(const FwdSnap &F, const Layout2D &L, int f, int shot, int x, int y,
+                                         long long o)
+{
+    if (!REC) return F.h[f][o];
+    const long long fs = ((long long)F.t * 5 + f) * F.nshots + shot;
+    const int xc = min(x, L.NX - 1);
+    return __half2float(__ushort_as_half(F.hs[(fs * L.NY + y) * L.NX + xc])) *
+           F.ss[(fs * F.nty + y / F.srows) * F.nbx + xc / TILE_X];
+}
+
 // Loads are issued branch-free before any arithmetic (CPML memory variables at
 // clamped indices, the x-halo columns through an out-of-range offset on the
 // lanes that have none, which the buffer unit answers with 0 without a memory
 // access), so a wavefront waits for few memory round trips.
 constexpr unsigned NO_LANE = 0x7ffffff0u;
 
-template <int H, int S, int W>
+template <int H, int S, int W, bool REC>
 __global__ __launch_bounds__(TILE_X * W) void adj_el2d_stress_strip(Layout2D L, int nshots, AdjFields A, AdjPsi M,
                                                                    FwdSnap F, Coef2D C, TapsT TT, Cpml2D P, Grad G)
 {
@@ -208,10 +220,11 @@ __global__ __launch_bounds__(TILE_X * W) void adj_el2d_stress_strip(Layout2D L, 
         float f4[S], f5[S], f67[S];
 #pragma unroll
         for (int r = 0; r < S; r++) {
-            const long long o = so + L.at(x, min(ya + r, L.NY - 1));
-            f4[r] = F.h[2][o];
-            f5[r] = F.h[3][o];
-            f67[r] = F.h[4][o];
+            const int yc = min(ya + r, L.NY - 1);
+            const long long o = so + L.at(x, yc);
+            f4[r] = fwd_sum<REC>(F, L, 2, shot, x, yc, o);
+            f5[r] = fwd_sum<REC>(F, L, 3, shot, x, yc, o);
+            f67[r] = fwd_sum<REC>(F, L, 4, shot, x, yc, o);
         }
 
         // lg5, lg6 on the window (registers); lg4, lg7 on the strip (LDS)
@@ -309,7 +322,7 @@ __global__ __launch_bounds__(TILE_X * W) void adj_el2d_stress_strip(Layout2D L, 
     s_sum.add(c, act, g, L, x0, ya, lane);
 }
 
-template <int H, int S, int W>
+template <int H, int S, int W, bool REC>
 __global__ __launch_bounds__(TILE_X * W) void adj_el2d_velocity_strip(Layout2D L, int nshots, AdjFields A, AdjPsi M,
                                                                      FwdSnap F, Coef2D C, TapsT TT, Cpml2D P, Grad G)
 {
@@ -388,9 +401,10 @@ __global__ __launch_bounds__(TILE_X * W) void adj_el2d_velocity_strip(Layout2D L
         float h01[S], h23[S];
 #pragma unroll
         for (int r = 0; r < S; r++) {
-            const long long o = so + L.at(x, min(ya + r, L.NY - 1));
-            h01[r] = F.h[0][o];
-            h23[r] = F.h[1][o];
+            const int yc = min(ya + r, L.NY - 1);
+            const long long o = so + L.at(x, yc);
+            h01[r] = fwd_sum<REC>(F, L, 0, shot, x, yc, o);
+            h23[r] = fwd_sum<REC>(F, L, 1, shot, x, yc, o);
         }
 
         // lg1, lg3 on the window (registers); lg0, lg2 on the strip (LDS)
@@ -516,7 +530,7 @@ __global__ __launch_bounds__(256) void ckpt_maxabs(const float *__restrict__ f, 
 __device__ __forceinline__ float ckpt_scale(unsigned bits)
 {
     const float m = __uint_as_float(bits);
-    return m > 0.f ? m * (1.f / 32768.f) : 1.f;
+    return m > 1e-30f ? m * (1.f / 32768.f) : 1.f;  // normal scale: x / scale stays finite
 }
 
 __global__ void ckpt_pack(const float *__restrict__ f, __half *h, const unsigned *__restrict__ maxbits,
@@ -557,16 +571,27 @@ struct AdjArgs {
     const Grad &G;
 };
 
+template <int H, int S, int W, bool REC>
+void adj_launch_swr(int which, const Layout2D &L, int nshots, const AdjArgs &a, hipStream_t st, hipEvent_t e0,
+                    hipEvent_t e1)
+{
+    const dim3 grid((L.NX + TILE_X - 1) / TILE_X, (L.NY + S - 1) / S, (nshots + W - 1) / W), block(TILE_X, W);
+    if (which == 0)
+        go(adj_el2d_stress_strip<H, S, W, REC>, grid, block, st, e0, e1, L, nshots, a.A, a.M, a.S, a.C, a.TT, a.P,
+           a.G);
+    else
+        go(adj_el2d_velocity_strip<H, S, W, REC>, grid, block, st, e0, e1, L, nshots, a.A, a.M, a.S, a.C, a.TT, a.P,
+           a.G);
+}
+
 template <int H, int S, int W>
 void adj_launch_sw(int which, const Layout2D &L, int nshots, const AdjArgs &a, hipStream_t st, hipEvent_t e0,
                    hipEvent_t e1)
 {
-    const dim3 grid((L.NX + TILE_X - 1) / TILE_X, (L.NY + S - 1) / S, (nshots + W - 1) / W), block(TILE_X, W);
-    if (which == 0)
-        go(adj_el2d_stress_strip<H, S, W>, grid, block, st, e0, e1, L, nshots, a.A, a.M, a.S, a.C, a.TT, a.P, a.G);
+    if (a.S.hs)
+        adj_launch_swr<H, S, W, true>(which, L, nshots, a, st, e0, e1);
     else
-        go(adj_el2d_velocity_strip<H, S, W>, grid, block, st, e0, e1, L, nshots, a.A, a.M, a.S, a.C, a.TT, a.P,
-           a.G);
+        adj_launch_swr<H, S, W, false>(which, L, nshots, a, st, e0, e1);
 }
 
 // W shots per block: up to 8 (one wavefront each)

@@ -247,6 +247,8 @@ struct wv_solver {
 void adjoint_destroy(AdjointState *a);
 // checkpoint interval (0 = checkpointing off)
 int adjoint_every(const wv_solver *s);
+// the per-step record the row-strip kernels write during a checkpointed run, or nullptr
+const wv::Rec2D *adjoint_record(const wv_solver *s);
 // called by wv_run: at its start, before step t (t a multiple of the interval), at its end
 void adjoint_run_begin(wv_solver *s, int t_start);
 int adjoint_checkpoint(wv_solver *s, int t);

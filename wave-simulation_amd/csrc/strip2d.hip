@@ -22,6 +22,8 @@
 //     the planes carry no vector address arithmetic.
 // Sources and receivers use the per-(shot, block) lists of the tiled kernels
 // with 4S-row blocks.
+#include <hip/hip_fp16.h>
+
 #include "acq2d.hpp"
 
 namespace wv {
@@ -69,6 +71,34 @@ __device__ __forceinline__ StripCtx strip_ctx(const Layout2D &L, const Cpml2D &P
     return c;
 }
 
+// Record one field of the wavefront's strip (S rows x 64 columns) for the
+// adjoint pass (Rec2D): the maximum over the strip sets its FP16 scale.
+template <int S>
+__device__ __forceinline__ void record_strip(const Rec2D &R, const Layout2D &L, int nshots, int t, int f, int shot,
+                                             int x0, int ya, int lane, const float (&h)[S])
+{
+    const int x = x0 + lane;
+    const bool in = x < L.NX;
+    float m = 0.f;
+#pragma unroll
+    for (int r = 0; r < S; r++)
+        if (in && ya + r < L.NY) m = fmaxf(m, fabsf(h[r]));
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+    // strips whose values are all below 1e-30 are stored as zeros (scale 1); above it the
+    // scale stays a normal float, so h / scale is finite
+    const float s = m > 1e-30f ? m * (1.f / 32768.f) : 1.f, is = 1.f / s;
+    const long long fs = ((long long)t * 5 + f) * nshots + shot;
+    if (ya >= L.NY) return;  // a strip wholly below the grid (last block row) records nothing
+    if (lane == 0) R.scale[(fs * R.nty + ya / S) * R.nbx + x0 / TILE_X] = s;
+#pragma unroll
+    for (int r = 0; r < S; r++) {
+        const int y = ya + r;
+        if (y >= L.NY) break;
+        if (in) R.h[(fs * L.NY + y) * L.NX + x] = __half_as_ushort(__float2half_rn(h[r] * is));
+    }
+}
+
 // y derivative at window row i (own row) with the row's table taps c (offsets -H..H)
 template <int H, int N>
 __device__ __forceinline__ float ydiff(const float (&w)[N], int i, const float *c)
@@ -97,7 +127,7 @@ __device__ __forceinline__ void stage_ytaps(float (*dst)[S * (2 * H + 1)], const
 // vx += 1/rho_x * (CPML(Dxf sxx) + CPML(Dyb sxy)); vy += 1/rho_y * (CPML(Dxb sxy) + CPML(Dyf syy))
 template <int H, int S, int U>
 __global__ __launch_bounds__(NTHREADS) void el2d_velocity_strip(Layout2D L, Fields2D F, Coef2D C, Taps T, Cpml2D P,
-                                                                Acq A, int tofs, int pend, Snap2D Sn)
+                                                                Acq A, int tofs, int pend, Snap2D Sn, Rec2D Rc)
 {
     constexpr int G = S / U, K = 2 * H + 1, N = S + 2 * H, RW = TILE_X + 2 * H;
     static_assert(S % U == 0, "rows per wavefront must be a multiple of the group");
@@ -166,6 +196,9 @@ __global__ __launch_bounds__(NTHREADS) void el2d_velocity_strip(Layout2D L, Fiel
     load_taps<H>(cxf, T.xf, col.x);
     load_taps<H>(cxb, T.xb, col.x);
     stage_ytaps<H, S>(s_ty[wy], T.yb_v, T.yf_v, c.ya, lane);
+    float rh0[S], rh1[S];
+#pragma unroll
+    for (int r = 0; r < S; r++) rh0[r] = rh1[r] = 0.f;
 
 #pragma unroll
     for (int g = 0; g < G; g++) {
@@ -202,6 +235,8 @@ __global__ __launch_bounds__(NTHREADS) void el2d_velocity_strip(Layout2D L, Fiel
             }
             u += tt;
             u2 += tt2;
+            rh0[r] = u;
+            rh1[r] = u2;
             const int ro = boff(L, c.xl, y);
             if (Sn.h[0]) {
                 Sn.h[0][c.so + L.at(c.x0, y) + lane] = u;
@@ -214,12 +249,17 @@ __global__ __launch_bounds__(NTHREADS) void el2d_velocity_strip(Layout2D L, Fiel
         }
         wave_sync();
     }
+    if (Rc.h) {
+        const int ts = A.tbase[0] + tofs;
+        record_strip<S>(Rc, L, A.nshots, ts, 0, c.shot, c.x0, c.ya, lane, rh0);
+        record_strip<S>(Rc, L, A.nshots, ts, 1, c.shot, c.x0, c.ya, lane, rh1);
+    }
 }
 
 // stress update of one row (ForwardSolver2Delastic.cpp:204-294)
 template <int H, int S, int U>
 __global__ __launch_bounds__(NTHREADS) void el2d_stress_strip(Layout2D L, Fields2D F, Coef2D C, Taps T, Cpml2D P,
-                                                              Acq A, int tofs, int pend, Snap2D Sn)
+                                                              Acq A, int tofs, int pend, Snap2D Sn, Rec2D Rc)
 {
     (void)pend;
     constexpr int G = S / U, K = 2 * H + 1, N = S + 2 * H, RW = TILE_X + 2 * H;
@@ -301,6 +341,9 @@ __global__ __launch_bounds__(NTHREADS) void el2d_stress_strip(Layout2D L, Fields
         s_cy[wy][lane][4] = yi >= 0 ? P.byh[y] : 0.f;
     }
     const bool fs = C.free_surface != 0;
+    float rh2[S], rh3[S], rh4[S];
+#pragma unroll
+    for (int r = 0; r < S; r++) rh2[r] = rh3[r] = rh4[r] = 0.f;
 
 #pragma unroll
     for (int g = 0; g < G; g++) {
@@ -344,6 +387,9 @@ __global__ __launch_bounds__(NTHREADS) void el2d_stress_strip(Layout2D L, Fields
             xx -= 2.f * (a3 * cmu[r]);
             yy -= 2.f * (a1 * cmu[r]);
             u2 += a2;
+            rh2[r] = a1;
+            rh3[r] = a3;
+            rh4[r] = u2;
             if (Sn.h[2]) {
                 const long long hr = c.so + L.at(c.x0, y) + lane;
                 Sn.h[2][hr] = a1;
@@ -368,6 +414,12 @@ __global__ __launch_bounds__(NTHREADS) void el2d_stress_strip(Layout2D L, Fields
         }
         wave_sync();
     }
+    if (Rc.h) {
+        const int ts = A.tbase[0] + tofs;
+        record_strip<S>(Rc, L, A.nshots, ts, 2, c.shot, c.x0, c.ya, lane, rh2);
+        record_strip<S>(Rc, L, A.nshots, ts, 3, c.shot, c.x0, c.ya, lane, rh3);
+        record_strip<S>(Rc, L, A.nshots, ts, 4, c.shot, c.x0, c.ya, lane, rh4);
+    }
     block_acquisition<ELASTIC>(A, F, c.so, shot_block(A, B), A.tbase[0] + tofs, false);
 }
 
@@ -383,28 +435,28 @@ void go(Kern k, dim3 grid, dim3 block, hipStream_t st, hipEvent_t e0, hipEvent_t
 template <int H, int S, int U>
 void launch_hs(int which, const Layout2D &L, int nshots, const Fields2D &F, const Coef2D &C, const Taps &T,
                const Cpml2D &P, const Acq &A, int tofs, int pend, hipStream_t st, hipEvent_t e0, hipEvent_t e1,
-               const Snap2D &Sn)
+               const Snap2D &Sn, const Rec2D &Rc)
 {
     constexpr int TY = S * BLOCK_Y;
     const dim3 grid((L.NX + TILE_X - 1) / TILE_X, (L.NY + TY - 1) / TY, nshots), block(TILE_X, BLOCK_Y);
     if (which == 0)
-        go(el2d_velocity_strip<H, S, U>, grid, block, st, e0, e1, L, F, C, T, P, A, tofs, pend, Sn);
+        go(el2d_velocity_strip<H, S, U>, grid, block, st, e0, e1, L, F, C, T, P, A, tofs, pend, Sn, Rc);
     else
-        go(el2d_stress_strip<H, S, U>, grid, block, st, e0, e1, L, F, C, T, P, A, tofs, pend, Sn);
+        go(el2d_stress_strip<H, S, U>, grid, block, st, e0, e1, L, F, C, T, P, A, tofs, pend, Sn, Rc);
 }
 
 template <int H>
 void launch_s(int tile_y, int which, const Layout2D &L, int nshots, const Fields2D &F, const Coef2D &C, const Taps &T,
               const Cpml2D &P, const Acq &A, int tofs, int pend, hipStream_t st, hipEvent_t e0, hipEvent_t e1,
-              const Snap2D &Sn)
+              const Snap2D &Sn, const Rec2D &Rc)
 {
     // one group per wavefront: measured faster than marching with prefetch on gfx950
     if (tile_y == 8)
-        launch_hs<H, 2, 2>(which, L, nshots, F, C, T, P, A, tofs, pend, st, e0, e1, Sn);
+        launch_hs<H, 2, 2>(which, L, nshots, F, C, T, P, A, tofs, pend, st, e0, e1, Sn, Rc);
     else if (tile_y == 32)
-        launch_hs<H, 8, 8>(which, L, nshots, F, C, T, P, A, tofs, pend, st, e0, e1, Sn);
+        launch_hs<H, 8, 8>(which, L, nshots, F, C, T, P, A, tofs, pend, st, e0, e1, Sn, Rc);
     else
-        launch_hs<H, 4, 4>(which, L, nshots, F, C, T, P, A, tofs, pend, st, e0, e1, Sn);
+        launch_hs<H, 4, 4>(which, L, nshots, F, C, T, P, A, tofs, pend, st, e0, e1, Sn, Rc);
 }
 
 }  // namespace
@@ -413,16 +465,17 @@ bool strip2d_fits(const Layout2D &L) { return L.plane * 4 < (1LL << 31); }
 
 void launch_strip2d(int order, int tile_y, const Layout2D &L, int nshots, const Fields2D &F, const Coef2D &C,
                     const Taps &T, const Cpml2D &P, const Acq &A, int tofs, int pend, hipStream_t stream, int which,
-                    hipEvent_t e0, hipEvent_t e1, const Snap2D *snap)
+                    hipEvent_t e0, hipEvent_t e1, const Snap2D *snap, const Rec2D *rec)
 {
     const Snap2D Sn = snap ? *snap : Snap2D{};
+    const Rec2D Rc = rec ? *rec : Rec2D{};
     switch (order / 2) {
-    case 1: launch_s<1>(tile_y, which, L, nshots, F, C, T, P, A, tofs, pend, stream, e0, e1, Sn); break;
-    case 2: launch_s<2>(tile_y, which, L, nshots, F, C, T, P, A, tofs, pend, stream, e0, e1, Sn); break;
-    case 3: launch_s<3>(tile_y, which, L, nshots, F, C, T, P, A, tofs, pend, stream, e0, e1, Sn); break;
-    case 4: launch_s<4>(tile_y, which, L, nshots, F, C, T, P, A, tofs, pend, stream, e0, e1, Sn); break;
-    case 5: launch_s<5>(tile_y, which, L, nshots, F, C, T, P, A, tofs, pend, stream, e0, e1, Sn); break;
-    case 6: launch_s<6>(tile_y, which, L, nshots, F, C, T, P, A, tofs, pend, stream, e0, e1, Sn); break;
+    case 1: launch_s<1>(tile_y, which, L, nshots, F, C, T, P, A, tofs, pend, stream, e0, e1, Sn, Rc); break;
+    case 2: launch_s<2>(tile_y, which, L, nshots, F, C, T, P, A, tofs, pend, stream, e0, e1, Sn, Rc); break;
+    case 3: launch_s<3>(tile_y, which, L, nshots, F, C, T, P, A, tofs, pend, stream, e0, e1, Sn, Rc); break;
+    case 4: launch_s<4>(tile_y, which, L, nshots, F, C, T, P, A, tofs, pend, stream, e0, e1, Sn, Rc); break;
+    case 5: launch_s<5>(tile_y, which, L, nshots, F, C, T, P, A, tofs, pend, stream, e0, e1, Sn, Rc); break;
+    case 6: launch_s<6>(tile_y, which, L, nshots, F, C, T, P, A, tofs, pend, stream, e0, e1, Sn, Rc); break;
     default: break;
     }
 }

@@ -120,6 +120,19 @@ struct Snap2D {
     float *h[5];
 };
 
+// The same five sums for every step of a run, kept for the adjoint pass
+// instead of recomputing them (row-strip kernels only): FP16 values
+// h[(((t * 5 + f) * nshots + shot) * NY + y) * NX + x] with one FP32 scale per
+// step, field, shot and strip of 64 columns x srows rows (the cells one
+// wavefront of the row-strip kernel updates),
+// scale[(((t * 5 + f) * nshots + shot) * nty + y / srows) * nbx + x / 64] = max|h| / 32768
+// over the strip (1 when all are below 1e-30).  h == nullptr: not recording.
+struct Rec2D {
+    unsigned short *h;
+    float *scale;
+    int nbx, nty, srows;
+};
+
 // Launchers (kernels2d.hip).  All launches go to `stream`.
 // which = 0: velocity-type phase (pend = apply the velocity-type sources /
 // receivers of the previous step first); which = 1: stress-type phase.
@@ -136,7 +149,8 @@ void launch_step2d(int physics, int order, int tile_y, const Layout2D &L, int ns
 bool strip2d_fits(const Layout2D &L);
 void launch_strip2d(int order, int tile_y, const Layout2D &L, int nshots, const Fields2D &F, const Coef2D &C,
                     const Taps &T, const Cpml2D &P, const Acq &A, int tofs, int pend, hipStream_t stream, int which,
-                    hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr, const Snap2D *snap = nullptr);
+                    hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr, const Snap2D *snap = nullptr,
+                    const Rec2D *rec = nullptr);
 void launch_flush2d(int physics, const Layout2D &L, const Fields2D &F, const Acq &A, int tofs, hipStream_t stream);
 void launch_abs_velocity2d(int physics, const Layout2D &L, int nshots, const Fields2D &F, const Coef2D &C,
                            hipStream_t stream);

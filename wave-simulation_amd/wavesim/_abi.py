@@ -24,7 +24,7 @@ EXPORTS = (
     "wv_create", "wv_destroy", "wv_init", "wv_set_model", "wv_set_acquisition", "wv_reset", "wv_run",
     "wv_synchronize", "wv_get_seismograms", "wv_get_wavefield", "wv_set_wavefield", "wv_last_run_ms",
     "wv_tune", "wv_set_kernel_timing", "wv_kernel_stats", "wv_last_error", "wv_set_checkpointing",
-    "wv_set_adjoint_source", "wv_run_adjoint", "wv_get_gradient",
+    "wv_set_adjoint_source", "wv_run_adjoint", "wv_get_gradient", "wv_set_adjoint_store",
 )
 
 
@@ -75,6 +75,7 @@ def load():
                                       C.POINTER(C.c_longlong), C.POINTER(C.c_double)]
         L.wv_set_checkpointing.argtypes = [C.c_void_p, C.c_int, C.c_int]
         L.wv_set_adjoint_source.argtypes = [C.c_void_p, C.c_int, fp]
+        L.wv_set_adjoint_store.argtypes = [C.c_void_p, C.c_double]
         L.wv_run_adjoint.argtypes = [C.c_void_p]
         L.wv_get_gradient.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double)]
         _lib = L

@@ -123,6 +123,10 @@ class ForwardSolver:
         return ms.value
 
     # -- adjoint (2D elastic) -------------------------------------------------------
+    def set_adjoint_store(self, max_gib):
+        """Device-memory budget for recording every step instead of checkpointing (< 0 auto, 0 never)."""
+        _abi.check(_abi.load().wv_set_adjoint_store(self._h, float(max_gib)))
+
     def set_checkpointing(self, every=50, fp16=True):
         """Store checkpoints every `every` steps in the following forward runs (resets the wavefields)."""
         _abi.check(_abi.load().wv_set_checkpointing(self._h, int(every), 1 if fp16 else 0))
