@@ -116,16 +116,46 @@ struct GradSum {
     }
 };
 
-// forward derivative sum f of (shot, x, y) at padded offset o = shot plane + at(x, y)
-template <bool REC>
-__device__ __forceinline__ float fwd_sum(const FwdSnap &F, const Layout2D &L, int f, int shot, int x, int y,
-                                         long long o)
+// Recorded forward derivative sums of the strip's cells: FP32 segment buffers
+// (recomputation) or the FP16 record of the run with one scale per record
+// strip (srows rows x 64 columns), field and shot.  The NF scales a wavefront
+// needs are fetched by one vector load (lane j: sum f0 + j / S, record strip
+// r0 + j % S) and broadcast per row with readlane.
+template <int NF, int S, bool REC>
+struct FwdSums {
+    float sc;
+    int r0;
+    __device__ __forceinline__ void init(const FwdSnap &F, const Layout2D &L, int f0, int shot, int x0, int ya,
+                                         int lane)
+    {
+        if (!REC) return;
+        r0 = ya / F.srows;
+        const int rl = min(ya + S - 1, L.NY - 1) / F.srows;
+        const int f = lane / S, q = lane - (lane / S) * S;
+        sc = 1.f;
+        if (lane < NF * S && r0 + q <= rl) {
+            const long long fs = ((long long)F.t * 5 + f0 + f) * F.nshots + shot;
+            sc = F.ss[(fs * F.nty + r0 + q) * F.nbx + x0 / TILE_X];
+        }
+    }
+    // sum f0 + f at (shot, xc, yc) (yc wavefront-uniform, xc < NX), o = shot plane offset + at(xc, yc)
+    __device__ __forceinline__ float get(const FwdSnap &F, const Layout2D &L, int f0, int f, int shot, int xc,
+                                         int yc, long long o) const
+    {
+        if (!REC) return F.h[f0 + f][o];
+        const long long fs = ((long long)F.t * 5 + f0 + f) * F.nshots + shot;
+        const float h = __half2float(__ushort_as_half(F.hs[(fs * L.NY + yc) * L.NX + xc]));
+        return h * __int_as_float(__builtin_amdgcn_readlane(__float_as_int(sc), f * S + yc / F.srows - r0));
+    }
+};
+
+// Whether the strip's window (rows ya-H .. ya+S+H-1, columns x0-H .. x0+63+H)
+// touches a CPML frame; wavefront-uniform, so interior strips skip the
+// memory-variable loads and stores entirely.
+template <int H, int S>
+__device__ __forceinline__ bool strip_in_frame(const Layout2D &L, const Cpml2D &P, int x0, int ya)
 {
-    if (!REC) return F.h[f][o];
-    const long long fs = ((long long)F.t * 5 + f) * F.nshots + shot;
-    const int xc = min(x, L.NX - 1);
-    return __half2float(__ushort_as_half(F.hs[(fs * L.NY + y) * L.NX + xc])) *
-           F.ss[(fs * F.nty + y / F.srows) * F.nbx + xc / TILE_X];
+    return P.on && (x0 - H < P.W || x0 + TILE_X + H > L.NX - P.W || ya - H < P.W || ya + S + H > L.NY - P.W);
 }
 
 // Loads are issued branch-free before any arithmetic (CPML memory variables at
@@ -164,7 +194,21 @@ __global__ __launch_bounds__(TILE_X * W) void adj_el2d_stress_strip(Layout2D L, 
         const int xc = min(x, L.NX - 1);
         const int hc = lane < H ? lane : TILE_X + lane;
         const unsigned vh = lane < 2 * H ? (unsigned)hc * 4u : NO_LANE;
-        const XFrame fx = xframe(L, P, x), fh = xframe(L, P, xl + hc);
+        const bool fr = strip_in_frame<H, S>(L, P, x0, ya);
+        XFrame fx{-1, 0.f, 0.f, 0.f, 0.f}, fh = fx;
+        if (fr) {
+            fx = xframe(L, P, x);
+            fh = xframe(L, P, xl + hc);
+        }
+        const auto bvx = brsrc(A.lvx + so, pb), bvy = brsrc(A.lvy + so, pb);
+        // own cells of the adjoint velocities (read-modify-write at the end)
+        float ovx[S], ovy[S];
+#pragma unroll
+        for (int r = 0; r < S; r++) {
+            const int ro = aboff(L, xl, min(ya + r, L.NY - 1));
+            ovx[r] = bload(bvx, vc, ro);
+            ovy[r] = bload(bvy, vc, ro);
+        }
 
         // window of the lane's column (rows ya-H .. ya+S+H-1) and the x-halo columns of the strip's rows
         float lxx[N], lyy[N], lxy[N], cM[N], cmu[N], cmxy[N];
@@ -195,12 +239,12 @@ __global__ __launch_bounds__(TILE_X * W) void adj_el2d_stress_strip(Layout2D L, 
 #pragma unroll
         for (int i = 0; i < N; i++) {
             const int y = ya - H + i;
-            yiw[i] = (P.on && y >= 0 && y < L.NY) ? P.yi[y] : -1;
+            yiw[i] = (fr && y >= 0 && y < L.NY) ? P.yi[y] : -1;
             my2[i] = my3[i] = 0.f;
         }
 #pragma unroll
         for (int r = 0; r < S; r++) mx2[r] = mx3[r] = hx2[r] = hx3[r] = 0.f;
-        if (P.on) {
+        if (fr) {
 #pragma unroll
             for (int i = 0; i < N; i++) {
                 const long long pi = ((long long)shot * W2 + max(yiw[i], 0)) * L.NX + xc;
@@ -218,13 +262,17 @@ __global__ __launch_bounds__(TILE_X * W) void adj_el2d_stress_strip(Layout2D L, 
         }
         // own cells: recorded forward derivative sums
         float f4[S], f5[S], f67[S];
+        {
+            FwdSums<3, S, REC> fs;
+            fs.init(F, L, 2, shot, x0, ya, lane);
 #pragma unroll
-        for (int r = 0; r < S; r++) {
-            const int yc = min(ya + r, L.NY - 1);
-            const long long o = so + L.at(x, yc);
-            f4[r] = fwd_sum<REC>(F, L, 2, shot, x, yc, o);
-            f5[r] = fwd_sum<REC>(F, L, 3, shot, x, yc, o);
-            f67[r] = fwd_sum<REC>(F, L, 4, shot, x, yc, o);
+            for (int r = 0; r < S; r++) {
+                const int yc = min(ya + r, L.NY - 1);
+                const long long o = so + L.at(xc, yc);
+                f4[r] = fs.get(F, L, 2, 0, shot, xc, yc, o);
+                f5[r] = fs.get(F, L, 2, 1, shot, xc, yc, o);
+                f67[r] = fs.get(F, L, 2, 2, shot, xc, yc, o);
+            }
         }
 
         // lg5, lg6 on the window (registers); lg4, lg7 on the strip (LDS)
@@ -272,7 +320,6 @@ __global__ __launch_bounds__(TILE_X * W) void adj_el2d_stress_strip(Layout2D L, 
         load_taps<H>(txb, TT.xb, x);
         wave_sync();
 
-        const auto bvx = brsrc(A.lvx + so, pb), bvy = brsrc(A.lvy + so, pb);
 #pragma unroll
         for (int r = 0; r < S; r++) {
             const int y = ya + r, i = r + H;
@@ -296,8 +343,8 @@ __global__ __launch_bounds__(TILE_X * W) void adj_el2d_stress_strip(Layout2D L, 
             }
             if (y >= L.NY || !inx) continue;
             const int ro = aboff(L, xl, y);
-            bstore(bload(bvx, vc, ro) + dvx, bvx, vc, ro);
-            bstore(bload(bvy, vc, ro) + dvy, bvy, vc, ro);
+            bstore(ovx[r] + dvx, bvx, vc, ro);
+            bstore(ovy[r] + dvy, bvy, vc, ro);
             // adjoint memory variables of the own cell (write buffers)
             const float ls = lxx[i] + lyy[i];
             const float h4 = cM[i] * ls - 2.f * cmu[i] * lyy[i], h5 = cM[i] * ls - 2.f * cmu[i] * lxx[i];
@@ -352,7 +399,22 @@ __global__ __launch_bounds__(TILE_X * W) void adj_el2d_velocity_strip(Layout2D L
         const int xc = min(x, L.NX - 1);
         const int hc = lane < H ? lane : TILE_X + lane;
         const unsigned vh = lane < 2 * H ? (unsigned)hc * 4u : NO_LANE;
-        const XFrame fx = xframe(L, P, x), fh = xframe(L, P, xl + hc);
+        const bool fr = strip_in_frame<H, S>(L, P, x0, ya);
+        XFrame fx{-1, 0.f, 0.f, 0.f, 0.f}, fh = fx;
+        if (fr) {
+            fx = xframe(L, P, x);
+            fh = xframe(L, P, xl + hc);
+        }
+        const auto bxx = brsrc(A.lsxx + so, pb), byy = brsrc(A.lsyy + so, pb), bxy = brsrc(A.lsxy + so, pb);
+        // own cells of the adjoint stresses (read-modify-write at the end)
+        float oxx[S], oyy[S], oxy[S];
+#pragma unroll
+        for (int r = 0; r < S; r++) {
+            const int ro = aboff(L, xl, min(ya + r, L.NY - 1));
+            oxx[r] = bload(bxx, vc, ro);
+            oyy[r] = bload(byy, vc, ro);
+            oxy[r] = bload(bxy, vc, ro);
+        }
 
         float vx[N], vy[N], cx[N], cy[N];
 #pragma unroll
@@ -377,12 +439,12 @@ __global__ __launch_bounds__(TILE_X * W) void adj_el2d_velocity_strip(Layout2D L
 #pragma unroll
         for (int i = 0; i < N; i++) {
             const int y = ya - H + i;
-            yiw[i] = (P.on && y >= 0 && y < L.NY) ? P.yi[y] : -1;
+            yiw[i] = (fr && y >= 0 && y < L.NY) ? P.yi[y] : -1;
             my0[i] = my1[i] = 0.f;
         }
 #pragma unroll
         for (int r = 0; r < S; r++) mx0[r] = mx1[r] = hx0[r] = hx1[r] = 0.f;
-        if (P.on) {
+        if (fr) {
 #pragma unroll
             for (int i = 0; i < N; i++) {
                 const long long pi = ((long long)shot * W2 + max(yiw[i], 0)) * L.NX + xc;
@@ -399,12 +461,16 @@ __global__ __launch_bounds__(TILE_X * W) void adj_el2d_velocity_strip(Layout2D L
             }
         }
         float h01[S], h23[S];
+        {
+            FwdSums<2, S, REC> fs;
+            fs.init(F, L, 0, shot, x0, ya, lane);
 #pragma unroll
-        for (int r = 0; r < S; r++) {
-            const int yc = min(ya + r, L.NY - 1);
-            const long long o = so + L.at(x, yc);
-            h01[r] = fwd_sum<REC>(F, L, 0, shot, x, yc, o);
-            h23[r] = fwd_sum<REC>(F, L, 1, shot, x, yc, o);
+            for (int r = 0; r < S; r++) {
+                const int yc = min(ya + r, L.NY - 1);
+                const long long o = so + L.at(xc, yc);
+                h01[r] = fs.get(F, L, 0, 0, shot, xc, yc, o);
+                h23[r] = fs.get(F, L, 0, 1, shot, xc, yc, o);
+            }
         }
 
         // lg1, lg3 on the window (registers); lg0, lg2 on the strip (LDS)
@@ -449,7 +515,6 @@ __global__ __launch_bounds__(TILE_X * W) void adj_el2d_velocity_strip(Layout2D L
         load_taps<H>(txb, TT.xb, x);
         wave_sync();
 
-        const auto bxx = brsrc(A.lsxx + so, pb), byy = brsrc(A.lsyy + so, pb), bxy = brsrc(A.lsxy + so, pb);
 #pragma unroll
         for (int r = 0; r < S; r++) {
             const int y = ya + r, i = r + H;
@@ -469,9 +534,9 @@ __global__ __launch_bounds__(TILE_X * W) void adj_el2d_velocity_strip(Layout2D L
             for (int k = 0; k < K; k++) dyy += s_ty[wy][1][r * K + k] * w3[r + k];
             if (y >= L.NY || !inx) continue;
             const int ro = aboff(L, xl, y);
-            bstore(bload(bxx, vc, ro) + dxx, bxx, vc, ro);
-            bstore(bload(bxy, vc, ro) + dxy, bxy, vc, ro);
-            bstore(bload(byy, vc, ro) + dyy, byy, vc, ro);
+            bstore(oxx[r] + dxx, bxx, vc, ro);
+            bstore(oxy[r] + dxy, bxy, vc, ro);
+            bstore(oyy[r] + dyy, byy, vc, ro);
             const float h0 = cx[i] * vx[i], h2 = cy[i] * vy[i];
             if (fx.xi >= 0) {
                 const long long pi = ((long long)shot * L.NY + y) * W2 + fx.xi;
