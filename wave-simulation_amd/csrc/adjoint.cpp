@@ -166,7 +166,9 @@ int adjoint_every(const wv_solver *s) { return s->adj ? s->adj->every : 0; }
 const wv::Rec2D *adjoint_record(const wv_solver *s)
 {
     const AdjointState *a = s->adj;
-    return (a && a->every && a->store) ? &a->rec : nullptr;
+    // only for the strip height the record was laid out for (wv_run_adjoint rejects a mismatch)
+    if (!a || !a->every || !a->store || s->variant != V_STRIP || s->tile_y / 4 != a->rec.srows) return nullptr;
+    return &a->rec;
 }
 
 int adjoint_checkpoint(wv_solver *s, int t)
@@ -274,19 +276,19 @@ int wv_set_checkpointing(wv_solver *s, int every, int fp16)
         const int nbx = (s->L.NX + wv::TILE_X - 1) / wv::TILE_X, srows = s->tile_y / 4;
         const int nty = (s->L.NY + srows - 1) / srows;
         const size_t fss = (size_t)s->cfg.NT * 5 * s->cfg.nshots;  // (step, field, shot) planes
-        const size_t rows = fss * s->L.NY, strips = fss * nty * nbx;
-        const double need = (double)rows * s->L.NX * 2 + (double)strips * 4;
+        const size_t strips = fss * nty * nbx, words = strips * (srows / 2) * wv::TILE_X;
+        const double need = (double)words * 4 + (double)strips * 4;
         size_t free_b = 0, total_b = 0;
         HIP_TRY(hipMemGetInfo(&free_b, &total_b));
         const double budget = a->store_gib < 0 ? 0.75 * (double)free_b : a->store_gib * 1073741824.0;
         if (need <= budget) {
             // every value is written by the forward run before the adjoint reads it: no clearing
             void *h = nullptr, *sc = nullptr;
-            if (hipMalloc(&h, rows * s->L.NX * 2) == hipSuccess) a->ck_allocs.push_back(h);
+            if (hipMalloc(&h, words * 4) == hipSuccess) a->ck_allocs.push_back(h);
             if (h && hipMalloc(&sc, strips * 4) == hipSuccess) a->ck_allocs.push_back(sc);
             if (h && sc) {
                 a->store = true;
-                a->rec.h = (unsigned short *)h;
+                a->rec.h = (unsigned *)h;
                 a->rec.scale = (float *)sc;
                 a->rec.nbx = nbx;
                 a->rec.nty = nty;

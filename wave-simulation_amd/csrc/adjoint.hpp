@@ -27,7 +27,7 @@ struct AdjPsi {
 struct FwdSnap {
     const float *h[5];
     // or, when hs != nullptr, the FP16 record of the whole run (Rec2D layout) at step t
-    const unsigned short *hs;
+    const unsigned *hs;
     const float *ss;
     int t, nbx, nty, srows, nshots;
 };

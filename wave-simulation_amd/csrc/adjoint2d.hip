@@ -29,12 +29,12 @@ namespace wv {
 namespace {
 
 // ---------------------------------------------------------------------------
-// Strip form of the two adjoint steps.  A block
-// covers 64 columns x S rows for W shots: wavefront w works on shot
-// (group * W + w) over the block's cells and needs nothing from the other
-// wavefronts until the gradient, so the W shots' contributions to a cell are
+// Strip form of the two adjoint steps.  A block covers 64 columns x S rows
+// for every shot: its W wavefronts take W shots at a time (wavefront w: shots
+// w, w + W, ...), so the shot-independent windows (coefficients, taps) are
+// loaded once per block and the shots' gradient contributions to a cell are
 // summed in LDS (FP64) and reach the gradient planes as one FP64 atomic per
-// cell and block instead of one per shot.  Within a wavefront: the
+// cell, plane and launch.  Within a wavefront: the
 // y-transposed stencils read a register column of lg over rows y-H .. y+S+H-1
 // (lg evaluated by the lane itself), the x-transposed ones a wavefront-private
 // LDS strip of lg over columns x-H .. x+63+H (the halo columns evaluated by
@@ -62,67 +62,47 @@ __device__ __forceinline__ XFrame xframe(const Layout2D &L, const Cpml2D &P, int
     return f;
 }
 
-// Sum over the block's W wavefronts of per-cell FP64 contributions of NF
-// gradient planes: each wavefront adds its values into LDS accumulators (LDS
-// FP64 atomics) and counts itself in; the last wavefront to arrive adds the
-// sums to the gradient planes (one global FP64 atomic per cell and plane), so
-// no wavefront waits for the others.  init() runs before any wavefront adds.
+// Sum of per-cell FP64 gradient contributions of NF planes over every shot a
+// block handles (its W wavefronts, each over several shot groups): LDS
+// accumulators (FP64 LDS atomics), then, after a block barrier, one global
+// FP64 atomic per cell and plane, the cells' rows and planes spread over the
+// wavefronts.
 template <int NF, int S, int W>
 struct GradSum {
     double acc[NF][S][TILE_X];
-    int arrived;
-    __device__ __forceinline__ void init(int wy, int lane)
+    // before any wavefront adds (a block barrier must follow)
+    __device__ __forceinline__ void zero(int wy, int lane)
     {
-        if (wy == 0) {
-#pragma unroll
-            for (int f = 0; f < NF; f++)
-#pragma unroll
-                for (int r = 0; r < S; r++) acc[f][r][lane] = 0.0;
-            if (lane == 0) arrived = 0;
-        }
-        __syncthreads();
+        for (int j = wy; j < NF * S; j += W) acc[j / S][j % S][lane] = 0.0;
     }
-    __device__ __forceinline__ void add(const double (&c)[NF][S], bool act, double *const (&g)[NF],
-                                        const Layout2D &L, int x0, int ya, int lane)
+    __device__ __forceinline__ void add(const double (&c)[NF][S], int lane)
     {
-        if (act && W > 1) {
-#pragma unroll
-            for (int f = 0; f < NF; f++)
-#pragma unroll
-                for (int r = 0; r < S; r++) atomicAdd(&acc[f][r][lane], c[f][r]);
-        }
-        const int x = x0 + lane;
-        if (W == 1) {
-            if (!act || x >= L.NX) return;
-#pragma unroll
-            for (int f = 0; f < NF; f++)
-#pragma unroll
-                for (int r = 0; r < S; r++)
-                    if (ya + r < L.NY) atomicAdd(g[f] + L.at(x, ya + r), c[f][r]);
-            return;
-        }
-        __threadfence_block();
-        int last = 0;
-        if (lane == 0) last = atomicAdd(&arrived, 1) == W - 1;
-        last = __builtin_amdgcn_readfirstlane(last);
-        if (!last) return;
-        __threadfence_block();
-        if (x >= L.NX) return;
 #pragma unroll
         for (int f = 0; f < NF; f++)
 #pragma unroll
-            for (int r = 0; r < S; r++)
-                if (ya + r < L.NY) atomicAdd(g[f] + L.at(x, ya + r), acc[f][r][lane]);
+            for (int r = 0; r < S; r++) atomicAdd(&acc[f][r][lane], c[f][r]);
+    }
+    // after a block barrier that follows every add
+    __device__ __forceinline__ void flush(double *const (&g)[NF], const Layout2D &L, int x0, int ya, int wy,
+                                          int lane) const
+    {
+        const int x = x0 + lane;
+        if (x >= L.NX) return;
+        for (int j = wy; j < NF * S; j += W) {
+            const int f = j / S, r = j % S;
+            if (ya + r < L.NY) atomicAdd(g[f] + L.at(x, ya + r), acc[f][r][lane]);
+        }
     }
 };
 
 // Recorded forward derivative sums of the strip's cells: FP32 segment buffers
-// (recomputation) or the FP16 record of the run with one scale per record
-// strip (srows rows x 64 columns), field and shot.  The NF scales a wavefront
-// needs are fetched by one vector load (lane j: sum f0 + j / S, record strip
-// r0 + j % S) and broadcast per row with readlane.
+// (recomputation) or the FP16 record of the run (Rec2D layout: row pairs of
+// srows x 64 strips, one scale per strip, field and shot).  The NF scales a
+// wavefront needs are fetched by one vector load (lane j: sum f0 + j / S,
+// record strip r0 + j % S) and broadcast per row with readlane.
 template <int NF, int S, bool REC>
 struct FwdSums {
+    static_assert(S % 2 == 0, "the record is read in row pairs");
     float sc;
     int r0;
     __device__ __forceinline__ void init(const FwdSnap &F, const Layout2D &L, int f0, int shot, int x0, int ya,
@@ -138,14 +118,36 @@ struct FwdSums {
             sc = F.ss[(fs * F.nty + r0 + q) * F.nbx + x0 / TILE_X];
         }
     }
-    // sum f0 + f at (shot, xc, yc) (yc wavefront-uniform, xc < NX), o = shot plane offset + at(xc, yc)
-    __device__ __forceinline__ float get(const FwdSnap &F, const Layout2D &L, int f0, int f, int shot, int xc,
-                                         int yc, long long o) const
+    // sums f0 .. f0+NF-1 of rows ya .. ya+S-1 (ya even) of column xc = min(x, NX-1) of the
+    // shot whose plane starts at so; rows past the grid get values that are not used
+    __device__ __forceinline__ void fetch(const FwdSnap &F, const Layout2D &L, int f0, int shot, int x0, int xc,
+                                          int lane, int ya, long long so, float (&out)[NF][S]) const
     {
-        if (!REC) return F.h[f0 + f][o];
-        const long long fs = ((long long)F.t * 5 + f0 + f) * F.nshots + shot;
-        const float h = __half2float(__ushort_as_half(F.hs[(fs * L.NY + yc) * L.NX + xc]));
-        return h * __int_as_float(__builtin_amdgcn_readlane(__float_as_int(sc), f * S + yc / F.srows - r0));
+        if (!REC) {
+#pragma unroll
+            for (int f = 0; f < NF; f++)
+#pragma unroll
+                for (int r = 0; r < S; r++) out[f][r] = F.h[f0 + f][so + L.at(xc, min(ya + r, L.NY - 1))];
+            return;
+        }
+        const int sp = F.srows / 2;
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+            const long long fs = ((long long)F.t * 5 + f0 + f) * F.nshots + shot;
+#pragma unroll
+            for (int p = 0; p < S / 2; p++) {
+                const int yp = min(ya + 2 * p, (L.NY - 1) & ~1);  // even row of a pair inside the grid
+                const int q = yp / F.srows;
+                const long long wi =
+                    (((fs * F.nty + q) * F.nbx + x0 / TILE_X) * sp + (yp - q * F.srows) / 2) * TILE_X + lane;
+                const unsigned u = F.hs[wi];
+                const int y0 = min(ya + 2 * p, L.NY - 1), y1 = min(ya + 2 * p + 1, L.NY - 1);
+                const float s0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(sc), f * S + y0 / F.srows - r0));
+                const float s1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(sc), f * S + y1 / F.srows - r0));
+                out[f][2 * p] = __half2float(__ushort_as_half((unsigned short)(u & 0xffffu))) * s0;
+                out[f][2 * p + 1] = __half2float(__ushort_as_half((unsigned short)(u >> 16))) * s1;
+            }
+        }
     }
 };
 
@@ -158,48 +160,127 @@ __device__ __forceinline__ bool strip_in_frame(const Layout2D &L, const Cpml2D &
     return P.on && (x0 - H < P.W || x0 + TILE_X + H > L.NX - P.W || ya - H < P.W || ya + S + H > L.NY - P.W);
 }
 
+// Block barrier that orders LDS accesses only: global loads issued before it
+// stay in flight.
+__device__ __forceinline__ void block_sync_lds()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Window rows y0 .. y0+N-1, columns xl .. xl+RW-1 of NC coefficient planes,
+// which are the same for every shot of a block: the block's T threads load it
+// once and store it column-major in LDS, so a lane reads its column's window
+// with wide LDS reads.
+template <int NC, int N, int RW, int T>
+struct CoefWin {
+    static constexpr int TOT = N * RW, IT = (TOT + T - 1) / T;
+    float v[NC][IT];
+    __device__ __forceinline__ void load(const float *const (&c)[NC], const Layout2D &L, int xl, int y0, int tid,
+                                         unsigned pb)
+    {
+#pragma unroll
+        for (int a = 0; a < NC; a++) {
+            const auto r = brsrc(c[a], pb);
+#pragma unroll
+            for (int k = 0; k < IT; k++) {
+                const int e = min(tid + k * T, TOT - 1);
+                const int i = e / RW, col = e - i * RW;
+                v[a][k] = bload(r, (unsigned)(L.at(xl + col, y0 + i) * 4), 0);
+            }
+        }
+    }
+    __device__ __forceinline__ void store(float (*s)[RW][N], int tid) const
+    {
+#pragma unroll
+        for (int a = 0; a < NC; a++)
+#pragma unroll
+            for (int k = 0; k < IT; k++) {
+                const int e = tid + k * T;
+                if (TOT % T == 0 || e < TOT) {
+                    const int i = e / RW, col = e - i * RW;
+                    s[a][col][i] = v[a][k];
+                }
+            }
+    }
+};
+
+// the y taps of the strip's S rows (two tables) into LDS, by the block's first S*K threads
+template <int S, int K>
+__device__ __forceinline__ void stage_ytaps_block(float (*dst)[S * K], const float *t0, const float *t1, int ya,
+                                                  int tid)
+{
+    if (tid < S * K) {
+        dst[0][tid] = t0[(long long)ya * K + tid];
+        dst[1][tid] = t1[(long long)ya * K + tid];
+    }
+}
+
 // Loads are issued branch-free before any arithmetic (CPML memory variables at
 // clamped indices, the x-halo columns through an out-of-range offset on the
 // lanes that have none, which the buffer unit answers with 0 without a memory
 // access), so a wavefront waits for few memory round trips.
 constexpr unsigned NO_LANE = 0x7ffffff0u;
 
-template <int H, int S, int W, bool REC>
+template <int H, int S, int W, bool REC, bool LOOP>
 __global__ __launch_bounds__(TILE_X * W) void adj_el2d_stress_strip(Layout2D L, int nshots, AdjFields A, AdjPsi M,
                                                                    FwdSnap F, Coef2D C, TapsT TT, Cpml2D P, Grad G)
 {
     constexpr int K = 2 * H + 1, N = S + 2 * H, RW = TILE_X + 2 * H;
     __shared__ float s_g4[W][S][RW];
     __shared__ float s_g7[W][S][RW];
-    __shared__ float s_ty[W][2][S * K];
+    __shared__ float s_ty[2][S * K];
+    __shared__ __align__(16) float s_cf[3][RW][N];
     __shared__ GradSum<3, S, W> s_sum;
     const Blk B = xcd_block();
-    const int wy = __builtin_amdgcn_readfirstlane(threadIdx.y), lane = threadIdx.x;
-    const int shot = B.shot * W + wy;
-    const bool act = shot < nshots;
+    const int wy = __builtin_amdgcn_readfirstlane(threadIdx.y), lane = threadIdx.x, tid = wy * TILE_X + lane;
     const int x0 = B.bx * TILE_X, ya = B.by * S, xl = x0 - H, x = x0 + lane;
-    const long long so = (long long)(act ? shot : 0) * L.plane;
     const int W2 = 2 * P.W;
-    s_sum.init(wy, lane);
-    double c[3][S];
-#pragma unroll
-    for (int r = 0; r < S; r++) c[0][r] = c[1][r] = c[2][r] = 0.0;
+    const unsigned pb = (unsigned)(L.plane * 4);
 
-    if (act) {
-        const unsigned pb = (unsigned)(L.plane * 4);
+    // shot-independent: coefficient windows (M, mu, mu_xy), taps, the CPML frame of the strip
+    CoefWin<3, N, RW, TILE_X * W> cw;
+    {
+        const float *const cp[3] = {C.c[0], C.c[1], C.c[4]};
+        cw.load(cp, L, xl, ya - H, tid, pb);
+    }
+    stage_ytaps_block<S, K>(s_ty, TT.yf, TT.yb, ya, tid);
+    const unsigned vc = (unsigned)(H + lane) * 4u;
+    const bool inx = x < L.NX;
+    const int xc = min(x, L.NX - 1);
+    const int hc = lane < H ? lane : TILE_X + lane;
+    const int hcl = lane < 2 * H ? hc : 0;
+    const unsigned vh = lane < 2 * H ? (unsigned)hc * 4u : NO_LANE;
+    const bool fr = strip_in_frame<H, S>(L, P, x0, ya);
+    XFrame fx{-1, 0.f, 0.f, 0.f, 0.f}, fh = fx;
+    if (fr) {
+        fx = xframe(L, P, x);
+        fh = xframe(L, P, xl + hc);
+    }
+    int yiw[N];
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        const int y = ya - H + i;
+        yiw[i] = (fr && y >= 0 && y < L.NY) ? P.yi[y] : -1;
+    }
+    float txf[K], txb[K];
+    load_taps<H>(txf, TT.xf, x);
+    load_taps<H>(txb, TT.xb, x);
+    cw.store(s_cf, tid);
+    s_sum.zero(wy, lane);
+    block_sync_lds();
+
+    // LOOP: the block takes every shot, wavefront wy shots wy, wy + W, ... (one shot group per
+    // pass); otherwise the block takes shot group blockIdx.z only (fewer registers, more atomics)
+    const int gbeg = LOOP ? 0 : B.shot * W, gend = LOOP ? nshots : gbeg + 1;
+#pragma unroll 1
+    for (int g0 = gbeg; g0 < gend; g0 += W) {
+        const int shot = g0 + wy;
+        const bool act = shot < nshots;
+        const int sh = act ? shot : 0;  // wavefronts past the last shot load shot 0 and store nothing
+        const long long so = (long long)sh * L.plane;
         const auto bxx = brsrc(A.lsxx + so, pb), byy = brsrc(A.lsyy + so, pb), bxy = brsrc(A.lsxy + so, pb);
-        const auto bM = brsrc(C.c[0], pb), bmu = brsrc(C.c[1], pb), bmxy = brsrc(C.c[4], pb);
-        const unsigned vc = (unsigned)(H + lane) * 4u;
-        const bool inx = x < L.NX;
-        const int xc = min(x, L.NX - 1);
-        const int hc = lane < H ? lane : TILE_X + lane;
-        const unsigned vh = lane < 2 * H ? (unsigned)hc * 4u : NO_LANE;
-        const bool fr = strip_in_frame<H, S>(L, P, x0, ya);
-        XFrame fx{-1, 0.f, 0.f, 0.f, 0.f}, fh = fx;
-        if (fr) {
-            fx = xframe(L, P, x);
-            fh = xframe(L, P, xl + hc);
-        }
         const auto bvx = brsrc(A.lvx + so, pb), bvy = brsrc(A.lvy + so, pb);
         // own cells of the adjoint velocities (read-modify-write at the end)
         float ovx[S], ovy[S];
@@ -209,51 +290,39 @@ __global__ __launch_bounds__(TILE_X * W) void adj_el2d_stress_strip(Layout2D L, 
             ovx[r] = bload(bvx, vc, ro);
             ovy[r] = bload(bvy, vc, ro);
         }
-
         // window of the lane's column (rows ya-H .. ya+S+H-1) and the x-halo columns of the strip's rows
-        float lxx[N], lyy[N], lxy[N], cM[N], cmu[N], cmxy[N];
+        float lxx[N], lyy[N], lxy[N];
 #pragma unroll
         for (int i = 0; i < N; i++) {
             const int ro = aboff(L, xl, ya - H + i);
             lxx[i] = bload(bxx, vc, ro);
             lyy[i] = bload(byy, vc, ro);
             lxy[i] = bload(bxy, vc, ro);
-            cM[i] = bload(bM, vc, ro);
-            cmu[i] = bload(bmu, vc, ro);
-            cmxy[i] = bload(bmxy, vc, ro);
         }
-        float hxx[S], hyy[S], hxy[S], hM[S], hmu[S], hmxy[S];
+        float hxx[S], hyy[S], hxy[S];
 #pragma unroll
         for (int r = 0; r < S; r++) {
             const int ro = aboff(L, xl, ya + r);
             hxx[r] = bload(bxx, vh, ro);
             hyy[r] = bload(byy, vh, ro);
             hxy[r] = bload(bxy, vh, ro);
-            hM[r] = bload(bM, vh, ro);
-            hmu[r] = bload(bmu, vh, ro);
-            hmxy[r] = bload(bmxy, vh, ro);
         }
         // adjoint memory variables (read buffers): y frame on the window rows, x frame on the strip's rows
         float my2[N], my3[N], mx2[S], mx3[S], hx2[S], hx3[S];
-        int yiw[N];
 #pragma unroll
-        for (int i = 0; i < N; i++) {
-            const int y = ya - H + i;
-            yiw[i] = (fr && y >= 0 && y < L.NY) ? P.yi[y] : -1;
-            my2[i] = my3[i] = 0.f;
-        }
+        for (int i = 0; i < N; i++) my2[i] = my3[i] = 0.f;
 #pragma unroll
         for (int r = 0; r < S; r++) mx2[r] = mx3[r] = hx2[r] = hx3[r] = 0.f;
         if (fr) {
 #pragma unroll
             for (int i = 0; i < N; i++) {
-                const long long pi = ((long long)shot * W2 + max(yiw[i], 0)) * L.NX + xc;
+                const long long pi = ((long long)sh * W2 + max(yiw[i], 0)) * L.NX + xc;
                 my2[i] = M.my[2][pi];
                 my3[i] = M.my[3][pi];
             }
 #pragma unroll
             for (int r = 0; r < S; r++) {
-                const long long row = ((long long)shot * L.NY + min(ya + r, L.NY - 1)) * W2;
+                const long long row = ((long long)sh * L.NY + min(ya + r, L.NY - 1)) * W2;
                 mx2[r] = M.mx[2][row + max(fx.xi, 0)];
                 mx3[r] = M.mx[3][row + max(fx.xi, 0)];
                 hx2[r] = M.mx[2][row + max(fh.xi, 0)];
@@ -261,18 +330,24 @@ __global__ __launch_bounds__(TILE_X * W) void adj_el2d_stress_strip(Layout2D L, 
             }
         }
         // own cells: recorded forward derivative sums
-        float f4[S], f5[S], f67[S];
+        float rs[3][S];  // h4, h5, h6 + h7
         {
             FwdSums<3, S, REC> fs;
-            fs.init(F, L, 2, shot, x0, ya, lane);
+            fs.init(F, L, 2, sh, x0, ya, lane);
+            fs.fetch(F, L, 2, sh, x0, xc, lane, ya, so, rs);
+        }
+        float cM[N], cmu[N], cmxy[N], hM[S], hmu[S], hmxy[S];
 #pragma unroll
-            for (int r = 0; r < S; r++) {
-                const int yc = min(ya + r, L.NY - 1);
-                const long long o = so + L.at(xc, yc);
-                f4[r] = fs.get(F, L, 2, 0, shot, xc, yc, o);
-                f5[r] = fs.get(F, L, 2, 1, shot, xc, yc, o);
-                f67[r] = fs.get(F, L, 2, 2, shot, xc, yc, o);
-            }
+        for (int i = 0; i < N; i++) {
+            cM[i] = s_cf[0][H + lane][i];
+            cmu[i] = s_cf[1][H + lane][i];
+            cmxy[i] = s_cf[2][H + lane][i];
+        }
+#pragma unroll
+        for (int r = 0; r < S; r++) {
+            hM[r] = s_cf[0][hcl][H + r];
+            hmu[r] = s_cf[1][hcl][H + r];
+            hmxy[r] = s_cf[2][hcl][H + r];
         }
 
         // lg5, lg6 on the window (registers); lg4, lg7 on the strip (LDS)
@@ -307,19 +382,11 @@ __global__ __launch_bounds__(TILE_X * W) void adj_el2d_stress_strip(Layout2D L, 
                 s_g7[wy][r][hc] = fho ? hh6 + fh.ah * (hx3[r] + hh6) : hh6;
             }
         }
-        {
-            const float *__restrict__ ta = TT.yf + (long long)ya * K;
-            const float *__restrict__ tb = TT.yb + (long long)ya * K;
-            for (int e = lane; e < S * K; e += TILE_X) {
-                s_ty[wy][0][e] = ta[e];
-                s_ty[wy][1][e] = tb[e];
-            }
-        }
-        float txf[K], txb[K];
-        load_taps<H>(txf, TT.xf, x);
-        load_taps<H>(txb, TT.xb, x);
         wave_sync();
 
+        double c[3][S];
+#pragma unroll
+        for (int r = 0; r < S; r++) c[0][r] = c[1][r] = c[2][r] = 0.0;
 #pragma unroll
         for (int r = 0; r < S; r++) {
             const int y = ya + r, i = r + H;
@@ -330,18 +397,18 @@ __global__ __launch_bounds__(TILE_X * W) void adj_el2d_stress_strip(Layout2D L, 
             {
                 float s = 0.f;
 #pragma unroll
-                for (int k = 0; k < K; k++) s += s_ty[wy][0][r * K + k] * w6[r + k];
+                for (int k = 0; k < K; k++) s += s_ty[0][r * K + k] * w6[r + k];
                 dvx += s;
             }
 #pragma unroll
-            for (int k = 0; k < K; k++) dvy += s_ty[wy][1][r * K + k] * w5[r + k];
+            for (int k = 0; k < K; k++) dvy += s_ty[1][r * K + k] * w5[r + k];
             {
                 float s = 0.f;
 #pragma unroll
                 for (int k = 0; k < K; k++) s += txf[k] * s_g7[wy][r][lane + k];
                 dvy += s;
             }
-            if (y >= L.NY || !inx) continue;
+            if (!act || y >= L.NY || !inx) continue;
             const int ro = aboff(L, xl, y);
             bstore(ovx[r] + dvx, bvx, vc, ro);
             bstore(ovy[r] + dvy, bvy, vc, ro);
@@ -360,51 +427,76 @@ __global__ __launch_bounds__(TILE_X * W) void adj_el2d_stress_strip(Layout2D L, 
                 M.myn[3][pi] = P.byh[y] * (my3[i] + h6);
             }
             // gradient terms with the forward step's recorded derivative sums
-            c[0][r] = (double)ls * ((double)f4[r] + (double)f5[r]);
-            c[1][r] = -2.0 * (double)lxx[i] * f5[r] - 2.0 * (double)lyy[i] * f4[r];
-            c[2][r] = (double)lxy[i] * (double)f67[r];
+            c[0][r] = (double)ls * ((double)rs[0][r] + (double)rs[1][r]);
+            c[1][r] = -2.0 * (double)lxx[i] * rs[1][r] - 2.0 * (double)lyy[i] * rs[0][r];
+            c[2][r] = (double)lxy[i] * (double)rs[2][r];
         }
+        if (act) s_sum.add(c, lane);
+        wave_sync();  // the next shot group rewrites this wavefront's LDS strips
     }
+    block_sync_lds();
     double *const g[3] = {G.g[2], G.g[3], G.g[4]};
-    s_sum.add(c, act, g, L, x0, ya, lane);
+    s_sum.flush(g, L, x0, ya, wy, lane);
 }
 
-template <int H, int S, int W, bool REC>
+template <int H, int S, int W, bool REC, bool LOOP>
 __global__ __launch_bounds__(TILE_X * W) void adj_el2d_velocity_strip(Layout2D L, int nshots, AdjFields A, AdjPsi M,
                                                                      FwdSnap F, Coef2D C, TapsT TT, Cpml2D P, Grad G)
 {
     constexpr int K = 2 * H + 1, N = S + 2 * H, RW = TILE_X + 2 * H;
     __shared__ float s_g0[W][S][RW];
     __shared__ float s_g2[W][S][RW];
-    __shared__ float s_ty[W][2][S * K];
+    __shared__ float s_ty[2][S * K];
+    __shared__ __align__(16) float s_cf[2][RW][N];
     __shared__ GradSum<2, S, W> s_sum;
     const Blk B = xcd_block();
-    const int wy = __builtin_amdgcn_readfirstlane(threadIdx.y), lane = threadIdx.x;
-    const int shot = B.shot * W + wy;
-    const bool act = shot < nshots;
+    const int wy = __builtin_amdgcn_readfirstlane(threadIdx.y), lane = threadIdx.x, tid = wy * TILE_X + lane;
     const int x0 = B.bx * TILE_X, ya = B.by * S, xl = x0 - H, x = x0 + lane;
-    const long long so = (long long)(act ? shot : 0) * L.plane;
     const int W2 = 2 * P.W;
-    s_sum.init(wy, lane);
-    double c[2][S];
-#pragma unroll
-    for (int r = 0; r < S; r++) c[0][r] = c[1][r] = 0.0;
+    const unsigned pb = (unsigned)(L.plane * 4);
 
-    if (act) {
-        const unsigned pb = (unsigned)(L.plane * 4);
+    // shot-independent: coefficient windows (1/rho_x, 1/rho_y), taps, the CPML frame of the strip
+    CoefWin<2, N, RW, TILE_X * W> cw;
+    {
+        const float *const cp[2] = {C.c[2], C.c[3]};
+        cw.load(cp, L, xl, ya - H, tid, pb);
+    }
+    stage_ytaps_block<S, K>(s_ty, TT.yb, TT.yf, ya, tid);
+    const unsigned vc = (unsigned)(H + lane) * 4u;
+    const bool inx = x < L.NX;
+    const int xc = min(x, L.NX - 1);
+    const int hc = lane < H ? lane : TILE_X + lane;
+    const int hcl = lane < 2 * H ? hc : 0;
+    const unsigned vh = lane < 2 * H ? (unsigned)hc * 4u : NO_LANE;
+    const bool fr = strip_in_frame<H, S>(L, P, x0, ya);
+    XFrame fx{-1, 0.f, 0.f, 0.f, 0.f}, fh = fx;
+    if (fr) {
+        fx = xframe(L, P, x);
+        fh = xframe(L, P, xl + hc);
+    }
+    int yiw[N];
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        const int y = ya - H + i;
+        yiw[i] = (fr && y >= 0 && y < L.NY) ? P.yi[y] : -1;
+    }
+    float txf[K], txb[K];
+    load_taps<H>(txf, TT.xf, x);
+    load_taps<H>(txb, TT.xb, x);
+    cw.store(s_cf, tid);
+    s_sum.zero(wy, lane);
+    block_sync_lds();
+
+    // LOOP: the block takes every shot, wavefront wy shots wy, wy + W, ... (one shot group per
+    // pass); otherwise the block takes shot group blockIdx.z only (fewer registers, more atomics)
+    const int gbeg = LOOP ? 0 : B.shot * W, gend = LOOP ? nshots : gbeg + 1;
+#pragma unroll 1
+    for (int g0 = gbeg; g0 < gend; g0 += W) {
+        const int shot = g0 + wy;
+        const bool act = shot < nshots;
+        const int sh = act ? shot : 0;  // wavefronts past the last shot load shot 0 and store nothing
+        const long long so = (long long)sh * L.plane;
         const auto bvx = brsrc(A.lvx + so, pb), bvy = brsrc(A.lvy + so, pb);
-        const auto bcx = brsrc(C.c[2], pb), bcy = brsrc(C.c[3], pb);
-        const unsigned vc = (unsigned)(H + lane) * 4u;
-        const bool inx = x < L.NX;
-        const int xc = min(x, L.NX - 1);
-        const int hc = lane < H ? lane : TILE_X + lane;
-        const unsigned vh = lane < 2 * H ? (unsigned)hc * 4u : NO_LANE;
-        const bool fr = strip_in_frame<H, S>(L, P, x0, ya);
-        XFrame fx{-1, 0.f, 0.f, 0.f, 0.f}, fh = fx;
-        if (fr) {
-            fx = xframe(L, P, x);
-            fh = xframe(L, P, xl + hc);
-        }
         const auto bxx = brsrc(A.lsxx + so, pb), byy = brsrc(A.lsyy + so, pb), bxy = brsrc(A.lsxy + so, pb);
         // own cells of the adjoint stresses (read-modify-write at the end)
         float oxx[S], oyy[S], oxy[S];
@@ -415,62 +507,57 @@ __global__ __launch_bounds__(TILE_X * W) void adj_el2d_velocity_strip(Layout2D L
             oyy[r] = bload(byy, vc, ro);
             oxy[r] = bload(bxy, vc, ro);
         }
-
-        float vx[N], vy[N], cx[N], cy[N];
+        float vx[N], vy[N];
 #pragma unroll
         for (int i = 0; i < N; i++) {
             const int ro = aboff(L, xl, ya - H + i);
             vx[i] = bload(bvx, vc, ro);
             vy[i] = bload(bvy, vc, ro);
-            cx[i] = bload(bcx, vc, ro);
-            cy[i] = bload(bcy, vc, ro);
         }
-        float hvx[S], hvy[S], hcx[S], hcy[S];
+        float hvx[S], hvy[S];
 #pragma unroll
         for (int r = 0; r < S; r++) {
             const int ro = aboff(L, xl, ya + r);
             hvx[r] = bload(bvx, vh, ro);
             hvy[r] = bload(bvy, vh, ro);
-            hcx[r] = bload(bcx, vh, ro);
-            hcy[r] = bload(bcy, vh, ro);
         }
         float my0[N], my1[N], mx0[S], mx1[S], hx0[S], hx1[S];
-        int yiw[N];
 #pragma unroll
-        for (int i = 0; i < N; i++) {
-            const int y = ya - H + i;
-            yiw[i] = (fr && y >= 0 && y < L.NY) ? P.yi[y] : -1;
-            my0[i] = my1[i] = 0.f;
-        }
+        for (int i = 0; i < N; i++) my0[i] = my1[i] = 0.f;
 #pragma unroll
         for (int r = 0; r < S; r++) mx0[r] = mx1[r] = hx0[r] = hx1[r] = 0.f;
         if (fr) {
 #pragma unroll
             for (int i = 0; i < N; i++) {
-                const long long pi = ((long long)shot * W2 + max(yiw[i], 0)) * L.NX + xc;
+                const long long pi = ((long long)sh * W2 + max(yiw[i], 0)) * L.NX + xc;
                 my0[i] = M.my[0][pi];
                 my1[i] = M.my[1][pi];
             }
 #pragma unroll
             for (int r = 0; r < S; r++) {
-                const long long row = ((long long)shot * L.NY + min(ya + r, L.NY - 1)) * W2;
+                const long long row = ((long long)sh * L.NY + min(ya + r, L.NY - 1)) * W2;
                 mx0[r] = M.mx[0][row + max(fx.xi, 0)];
                 mx1[r] = M.mx[1][row + max(fx.xi, 0)];
                 hx0[r] = M.mx[0][row + max(fh.xi, 0)];
                 hx1[r] = M.mx[1][row + max(fh.xi, 0)];
             }
         }
-        float h01[S], h23[S];
+        float rs[2][S];  // recorded h0 + h1, h2 + h3
         {
             FwdSums<2, S, REC> fs;
-            fs.init(F, L, 0, shot, x0, ya, lane);
+            fs.init(F, L, 0, sh, x0, ya, lane);
+            fs.fetch(F, L, 0, sh, x0, xc, lane, ya, so, rs);
+        }
+        float cx[N], cy[N], hcx[S], hcy[S];
 #pragma unroll
-            for (int r = 0; r < S; r++) {
-                const int yc = min(ya + r, L.NY - 1);
-                const long long o = so + L.at(xc, yc);
-                h01[r] = fs.get(F, L, 0, 0, shot, xc, yc, o);
-                h23[r] = fs.get(F, L, 0, 1, shot, xc, yc, o);
-            }
+        for (int i = 0; i < N; i++) {
+            cx[i] = s_cf[0][H + lane][i];
+            cy[i] = s_cf[1][H + lane][i];
+        }
+#pragma unroll
+        for (int r = 0; r < S; r++) {
+            hcx[r] = s_cf[0][hcl][H + r];
+            hcy[r] = s_cf[1][hcl][H + r];
         }
 
         // lg1, lg3 on the window (registers); lg0, lg2 on the strip (LDS)
@@ -502,19 +589,11 @@ __global__ __launch_bounds__(TILE_X * W) void adj_el2d_velocity_strip(Layout2D L
                 s_g2[wy][r][hc] = fho ? hh2 + fh.a * (hx1[r] + hh2) : hh2;
             }
         }
-        {
-            const float *__restrict__ tb = TT.yb + (long long)ya * K;
-            const float *__restrict__ ta = TT.yf + (long long)ya * K;
-            for (int e = lane; e < S * K; e += TILE_X) {
-                s_ty[wy][0][e] = tb[e];
-                s_ty[wy][1][e] = ta[e];
-            }
-        }
-        float txf[K], txb[K];
-        load_taps<H>(txf, TT.xf, x);
-        load_taps<H>(txb, TT.xb, x);
         wave_sync();
 
+        double c[2][S];
+#pragma unroll
+        for (int r = 0; r < S; r++) c[0][r] = c[1][r] = 0.0;
 #pragma unroll
         for (int r = 0; r < S; r++) {
             const int y = ya + r, i = r + H;
@@ -523,7 +602,7 @@ __global__ __launch_bounds__(TILE_X * W) void adj_el2d_velocity_strip(Layout2D L
 #pragma unroll
             for (int k = 0; k < K; k++) dxx += txf[k] * s_g0[wy][r][lane + k];
 #pragma unroll
-            for (int k = 0; k < K; k++) dxy += s_ty[wy][0][r * K + k] * w1[r + k];
+            for (int k = 0; k < K; k++) dxy += s_ty[0][r * K + k] * w1[r + k];
             {
                 float s = 0.f;
 #pragma unroll
@@ -531,8 +610,8 @@ __global__ __launch_bounds__(TILE_X * W) void adj_el2d_velocity_strip(Layout2D L
                 dxy += s;
             }
 #pragma unroll
-            for (int k = 0; k < K; k++) dyy += s_ty[wy][1][r * K + k] * w3[r + k];
-            if (y >= L.NY || !inx) continue;
+            for (int k = 0; k < K; k++) dyy += s_ty[1][r * K + k] * w3[r + k];
+            if (!act || y >= L.NY || !inx) continue;
             const int ro = aboff(L, xl, y);
             bstore(oxx[r] + dxx, bxx, vc, ro);
             bstore(oxy[r] + dxy, bxy, vc, ro);
@@ -548,12 +627,15 @@ __global__ __launch_bounds__(TILE_X * W) void adj_el2d_velocity_strip(Layout2D L
                 M.myn[0][pi] = P.by[y] * (my0[i] + h0);
                 M.myn[1][pi] = P.byh[y] * (my1[i] + h2);
             }
-            c[0][r] = (double)vx[i] * h01[r];
-            c[1][r] = (double)vy[i] * h23[r];
+            c[0][r] = (double)vx[i] * rs[0][r];
+            c[1][r] = (double)vy[i] * rs[1][r];
         }
+        if (act) s_sum.add(c, lane);
+        wave_sync();  // the next shot group rewrites this wavefront's LDS strips
     }
+    block_sync_lds();
     double *const g[2] = {G.g[0], G.g[1]};
-    s_sum.add(c, act, g, L, x0, ya, lane);
+    s_sum.flush(g, L, x0, ya, wy, lane);
 }
 
 __global__ __launch_bounds__(256) void adj_inject(Layout2D L, AdjFields A, Acq Q, const float *__restrict__ adj, int t)
@@ -636,17 +718,21 @@ struct AdjArgs {
     const Grad &G;
 };
 
+// The stress kernel walks all shot groups inside each block (one gradient atomic per cell
+// and launch); the velocity kernel, whose loop form needs more registers than its
+// occupancy allows, takes one shot group per block (grid z).
 template <int H, int S, int W, bool REC>
 void adj_launch_swr(int which, const Layout2D &L, int nshots, const AdjArgs &a, hipStream_t st, hipEvent_t e0,
                     hipEvent_t e1)
 {
-    const dim3 grid((L.NX + TILE_X - 1) / TILE_X, (L.NY + S - 1) / S, (nshots + W - 1) / W), block(TILE_X, W);
+    const int nbx = (L.NX + TILE_X - 1) / TILE_X, nby = (L.NY + S - 1) / S;
+    const dim3 block(TILE_X, W);
     if (which == 0)
-        go(adj_el2d_stress_strip<H, S, W, REC>, grid, block, st, e0, e1, L, nshots, a.A, a.M, a.S, a.C, a.TT, a.P,
-           a.G);
+        go(adj_el2d_stress_strip<H, S, W, REC, true>, dim3(nbx, nby), block, st, e0, e1, L, nshots, a.A, a.M, a.S,
+           a.C, a.TT, a.P, a.G);
     else
-        go(adj_el2d_velocity_strip<H, S, W, REC>, grid, block, st, e0, e1, L, nshots, a.A, a.M, a.S, a.C, a.TT, a.P,
-           a.G);
+        go(adj_el2d_velocity_strip<H, S, W, REC, false>, dim3(nbx, nby, (nshots + W - 1) / W), block, st, e0, e1, L,
+           nshots, a.A, a.M, a.S, a.C, a.TT, a.P, a.G);
 }
 
 template <int H, int S, int W>
@@ -659,15 +745,13 @@ void adj_launch_sw(int which, const Layout2D &L, int nshots, const AdjArgs &a, h
         adj_launch_swr<H, S, W, false>(which, L, nshots, a, st, e0, e1);
 }
 
-// W shots per block: up to 8 (one wavefront each)
+// W shots at a time per block (one wavefront each; a block walks all shots of its cells in groups of W)
 template <int H>
 void adj_ty(int which, int ty, const Layout2D &L, int nshots, const AdjArgs &a, hipStream_t st, hipEvent_t e0,
             hipEvent_t e1)
 {
     (void)ty;
-    if (nshots >= 8)
-        adj_launch_sw<H, 4, 8>(which, L, nshots, a, st, e0, e1);
-    else if (nshots >= 4)
+    if (nshots >= 4)
         adj_launch_sw<H, 4, 4>(which, L, nshots, a, st, e0, e1);
     else if (nshots >= 2)
         adj_launch_sw<H, 4, 2>(which, L, nshots, a, st, e0, e1);

@@ -72,13 +72,14 @@ __device__ __forceinline__ StripCtx strip_ctx(const Layout2D &L, const Cpml2D &P
 }
 
 // Record one field of the wavefront's strip (S rows x 64 columns) for the
-// adjoint pass (Rec2D): the maximum over the strip sets its FP16 scale.
+// adjoint pass (Rec2D): the maximum over the strip sets its FP16 scale; rows
+// are stored in pairs, one 32-bit word per lane and pair.
 template <int S>
 __device__ __forceinline__ void record_strip(const Rec2D &R, const Layout2D &L, int nshots, int t, int f, int shot,
                                              int x0, int ya, int lane, const float (&h)[S])
 {
-    const int x = x0 + lane;
-    const bool in = x < L.NX;
+    static_assert(S % 2 == 0, "rows are recorded in pairs");
+    const bool in = x0 + lane < L.NX;
     float m = 0.f;
 #pragma unroll
     for (int r = 0; r < S; r++)
@@ -90,12 +91,16 @@ __device__ __forceinline__ void record_strip(const Rec2D &R, const Layout2D &L, 
     const float s = m > 1e-30f ? m * (1.f / 32768.f) : 1.f, is = 1.f / s;
     const long long fs = ((long long)t * 5 + f) * nshots + shot;
     if (ya >= L.NY) return;  // a strip wholly below the grid (last block row) records nothing
-    if (lane == 0) R.scale[(fs * R.nty + ya / S) * R.nbx + x0 / TILE_X] = s;
+    const long long q = (fs * R.nty + ya / S) * R.nbx + x0 / TILE_X;
+    if (lane == 0) R.scale[q] = s;
+    unsigned *w = R.h + q * (S / 2) * TILE_X + lane;
 #pragma unroll
-    for (int r = 0; r < S; r++) {
-        const int y = ya + r;
-        if (y >= L.NY) break;
-        if (in) R.h[(fs * L.NY + y) * L.NX + x] = __half_as_ushort(__float2half_rn(h[r] * is));
+    for (int p = 0; p < S / 2; p++) {
+        // cells outside the grid store zeros
+        const float a = (in && ya + 2 * p < L.NY) ? h[2 * p] * is : 0.f;
+        const float b = (in && ya + 2 * p + 1 < L.NY) ? h[2 * p + 1] * is : 0.f;
+        w[p * TILE_X] = (unsigned)__half_as_ushort(__float2half_rn(a)) |
+                        ((unsigned)__half_as_ushort(__float2half_rn(b)) << 16);
     }
 }
 

@@ -121,14 +121,17 @@ struct Snap2D {
 };
 
 // The same five sums for every step of a run, kept for the adjoint pass
-// instead of recomputing them (row-strip kernels only): FP16 values
-// h[(((t * 5 + f) * nshots + shot) * NY + y) * NX + x] with one FP32 scale per
-// step, field, shot and strip of 64 columns x srows rows (the cells one
-// wavefront of the row-strip kernel updates),
-// scale[(((t * 5 + f) * nshots + shot) * nty + y / srows) * nbx + x / 64] = max|h| / 32768
-// over the strip (1 when all are below 1e-30).  h == nullptr: not recording.
+// instead of recomputing them (row-strip kernels only), in FP16 with one FP32
+// scale per step, field, shot and strip of 64 columns x srows rows (the cells
+// one wavefront of the row-strip kernel updates).  With
+// fs = (t * 5 + f) * nshots + shot and q = (fs * nty + y / srows) * nbx + x / 64:
+//   scale[q] = max|h| / 32768 over the strip (1 when all are below 1e-30)
+//   h[(q * (srows / 2) + (y % srows) / 2) * 64 + x % 64] = FP16 pair (row y even:
+//   low half, y + 1: high half) of h / scale,
+// so a wavefront writes and reads aligned runs of 256 bytes.  h == nullptr:
+// not recording.
 struct Rec2D {
-    unsigned short *h;
+    unsigned *h;
     float *scale;
     int nbx, nty, srows;
 };
