@@ -131,16 +131,18 @@ struct FwdSums {
             return;
         }
         const int sp = F.srows / 2;
+        const long long pw = (long long)F.nty * F.nbx * sp * TILE_X;  // words of one (step, sum, shot) plane
 #pragma unroll
         for (int f = 0; f < NF; f++) {
             const long long fs = ((long long)F.t * 5 + f0 + f) * F.nshots + shot;
+            const auto rb = brsrc(F.hs + fs * pw, (unsigned)(pw * 4));
 #pragma unroll
             for (int p = 0; p < S / 2; p++) {
                 const int yp = min(ya + 2 * p, (L.NY - 1) & ~1);  // even row of a pair inside the grid
                 const int q = yp / F.srows;
-                const long long wi =
-                    (((fs * F.nty + q) * F.nbx + x0 / TILE_X) * sp + (yp - q * F.srows) / 2) * TILE_X + lane;
-                const unsigned u = F.hs[wi];
+                const int wo = ((q * F.nbx + x0 / TILE_X) * sp + (yp - q * F.srows) / 2) * TILE_X * 4;
+                // read once: streamed past the caches
+                const unsigned u = __builtin_amdgcn_raw_buffer_load_b32(rb, lane * 4, wo, 2);
                 const int y0 = min(ya + 2 * p, L.NY - 1), y1 = min(ya + 2 * p + 1, L.NY - 1);
                 const float s0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(sc), f * S + y0 / F.srows - r0));
                 const float s1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(sc), f * S + y1 / F.srows - r0));
@@ -238,6 +240,8 @@ __global__ __launch_bounds__(TILE_X * W) void adj_el2d_stress_strip(Layout2D L, 
     const int x0 = B.bx * TILE_X, ya = B.by * S, xl = x0 - H, x = x0 + lane;
     const int W2 = 2 * P.W;
     const unsigned pb = (unsigned)(L.plane * 4);
+    // bytes of one shot's CPML memory variables: x frame [NY][2W], y frame [2W][NX]
+    const unsigned pxb = (unsigned)(L.NY * W2 * 4), pyb = (unsigned)(W2 * L.NX * 4);
 
     // shot-independent: coefficient windows (M, mu, mu_xy), taps, the CPML frame of the strip
     CoefWin<3, N, RW, TILE_X * W> cw;
@@ -249,6 +253,7 @@ __global__ __launch_bounds__(TILE_X * W) void adj_el2d_stress_strip(Layout2D L, 
     const unsigned vc = (unsigned)(H + lane) * 4u;
     const bool inx = x < L.NX;
     const int xc = min(x, L.NX - 1);
+    const unsigned vxc = (unsigned)xc * 4u;
     const int hc = lane < H ? lane : TILE_X + lane;
     const int hcl = lane < 2 * H ? hc : 0;
     const unsigned vh = lane < 2 * H ? (unsigned)hc * 4u : NO_LANE;
@@ -314,19 +319,25 @@ __global__ __launch_bounds__(TILE_X * W) void adj_el2d_stress_strip(Layout2D L, 
 #pragma unroll
         for (int r = 0; r < S; r++) mx2[r] = mx3[r] = hx2[r] = hx3[r] = 0.f;
         if (fr) {
+            // buffer accesses: per-shot base, scalar row offset, one lane offset per frame
+            const auto y2 = brsrc(M.my[2] + (long long)sh * W2 * L.NX, pyb);
+            const auto y3 = brsrc(M.my[3] + (long long)sh * W2 * L.NX, pyb);
+            const auto x2 = brsrc(M.mx[2] + (long long)sh * L.NY * W2, pxb);
+            const auto x3 = brsrc(M.mx[3] + (long long)sh * L.NY * W2, pxb);
+            const unsigned vfx = (unsigned)max(fx.xi, 0) * 4u, vfh = (unsigned)max(fh.xi, 0) * 4u;
 #pragma unroll
             for (int i = 0; i < N; i++) {
-                const long long pi = ((long long)sh * W2 + max(yiw[i], 0)) * L.NX + xc;
-                my2[i] = M.my[2][pi];
-                my3[i] = M.my[3][pi];
+                const int ro = max(yiw[i], 0) * L.NX * 4;
+                my2[i] = bload(y2, vxc, ro);
+                my3[i] = bload(y3, vxc, ro);
             }
 #pragma unroll
             for (int r = 0; r < S; r++) {
-                const long long row = ((long long)sh * L.NY + min(ya + r, L.NY - 1)) * W2;
-                mx2[r] = M.mx[2][row + max(fx.xi, 0)];
-                mx3[r] = M.mx[3][row + max(fx.xi, 0)];
-                hx2[r] = M.mx[2][row + max(fh.xi, 0)];
-                hx3[r] = M.mx[3][row + max(fh.xi, 0)];
+                const int ro = min(ya + r, L.NY - 1) * W2 * 4;
+                mx2[r] = bload(x2, vfx, ro);
+                mx3[r] = bload(x3, vfx, ro);
+                hx2[r] = bload(x2, vfh, ro);
+                hx3[r] = bload(x3, vfh, ro);
             }
         }
         // own cells: recorded forward derivative sums
@@ -417,14 +428,14 @@ __global__ __launch_bounds__(TILE_X * W) void adj_el2d_stress_strip(Layout2D L, 
             const float h4 = cM[i] * ls - 2.f * cmu[i] * lyy[i], h5 = cM[i] * ls - 2.f * cmu[i] * lxx[i];
             const float h6 = cmxy[i] * lxy[i];
             if (fx.xi >= 0) {
-                const long long pi = ((long long)shot * L.NY + y) * W2 + fx.xi;
-                M.mxn[2][pi] = fx.b * (mx2[r] + h4);
-                M.mxn[3][pi] = fx.bh * (mx3[r] + h6);
+                const int ro = y * W2 * 4;
+                bstore(fx.b * (mx2[r] + h4), brsrc(M.mxn[2] + (long long)shot * L.NY * W2, pxb), fx.xi * 4, ro);
+                bstore(fx.bh * (mx3[r] + h6), brsrc(M.mxn[3] + (long long)shot * L.NY * W2, pxb), fx.xi * 4, ro);
             }
             if (yiw[i] >= 0) {
-                const long long pi = ((long long)shot * W2 + yiw[i]) * L.NX + x;
-                M.myn[2][pi] = P.by[y] * (my2[i] + h5);
-                M.myn[3][pi] = P.byh[y] * (my3[i] + h6);
+                const int ro = yiw[i] * L.NX * 4;
+                bstore(P.by[y] * (my2[i] + h5), brsrc(M.myn[2] + (long long)shot * W2 * L.NX, pyb), vxc, ro);
+                bstore(P.byh[y] * (my3[i] + h6), brsrc(M.myn[3] + (long long)shot * W2 * L.NX, pyb), vxc, ro);
             }
             // gradient terms with the forward step's recorded derivative sums
             c[0][r] = (double)ls * ((double)rs[0][r] + (double)rs[1][r]);
@@ -454,6 +465,8 @@ __global__ __launch_bounds__(TILE_X * W) void adj_el2d_velocity_strip(Layout2D L
     const int x0 = B.bx * TILE_X, ya = B.by * S, xl = x0 - H, x = x0 + lane;
     const int W2 = 2 * P.W;
     const unsigned pb = (unsigned)(L.plane * 4);
+    // bytes of one shot's CPML memory variables: x frame [NY][2W], y frame [2W][NX]
+    const unsigned pxb = (unsigned)(L.NY * W2 * 4), pyb = (unsigned)(W2 * L.NX * 4);
 
     // shot-independent: coefficient windows (1/rho_x, 1/rho_y), taps, the CPML frame of the strip
     CoefWin<2, N, RW, TILE_X * W> cw;
@@ -465,6 +478,7 @@ __global__ __launch_bounds__(TILE_X * W) void adj_el2d_velocity_strip(Layout2D L
     const unsigned vc = (unsigned)(H + lane) * 4u;
     const bool inx = x < L.NX;
     const int xc = min(x, L.NX - 1);
+    const unsigned vxc = (unsigned)xc * 4u;
     const int hc = lane < H ? lane : TILE_X + lane;
     const int hcl = lane < 2 * H ? hc : 0;
     const unsigned vh = lane < 2 * H ? (unsigned)hc * 4u : NO_LANE;
@@ -527,19 +541,25 @@ __global__ __launch_bounds__(TILE_X * W) void adj_el2d_velocity_strip(Layout2D L
 #pragma unroll
         for (int r = 0; r < S; r++) mx0[r] = mx1[r] = hx0[r] = hx1[r] = 0.f;
         if (fr) {
+            // buffer accesses: per-shot base, scalar row offset, one lane offset per frame
+            const auto y2 = brsrc(M.my[0] + (long long)sh * W2 * L.NX, pyb);
+            const auto y3 = brsrc(M.my[1] + (long long)sh * W2 * L.NX, pyb);
+            const auto x2 = brsrc(M.mx[0] + (long long)sh * L.NY * W2, pxb);
+            const auto x3 = brsrc(M.mx[1] + (long long)sh * L.NY * W2, pxb);
+            const unsigned vfx = (unsigned)max(fx.xi, 0) * 4u, vfh = (unsigned)max(fh.xi, 0) * 4u;
 #pragma unroll
             for (int i = 0; i < N; i++) {
-                const long long pi = ((long long)sh * W2 + max(yiw[i], 0)) * L.NX + xc;
-                my0[i] = M.my[0][pi];
-                my1[i] = M.my[1][pi];
+                const int ro = max(yiw[i], 0) * L.NX * 4;
+                my0[i] = bload(y2, vxc, ro);
+                my1[i] = bload(y3, vxc, ro);
             }
 #pragma unroll
             for (int r = 0; r < S; r++) {
-                const long long row = ((long long)sh * L.NY + min(ya + r, L.NY - 1)) * W2;
-                mx0[r] = M.mx[0][row + max(fx.xi, 0)];
-                mx1[r] = M.mx[1][row + max(fx.xi, 0)];
-                hx0[r] = M.mx[0][row + max(fh.xi, 0)];
-                hx1[r] = M.mx[1][row + max(fh.xi, 0)];
+                const int ro = min(ya + r, L.NY - 1) * W2 * 4;
+                mx0[r] = bload(x2, vfx, ro);
+                mx1[r] = bload(x3, vfx, ro);
+                hx0[r] = bload(x2, vfh, ro);
+                hx1[r] = bload(x3, vfh, ro);
             }
         }
         float rs[2][S];  // recorded h0 + h1, h2 + h3
@@ -618,14 +638,14 @@ __global__ __launch_bounds__(TILE_X * W) void adj_el2d_velocity_strip(Layout2D L
             bstore(oyy[r] + dyy, byy, vc, ro);
             const float h0 = cx[i] * vx[i], h2 = cy[i] * vy[i];
             if (fx.xi >= 0) {
-                const long long pi = ((long long)shot * L.NY + y) * W2 + fx.xi;
-                M.mxn[0][pi] = fx.bh * (mx0[r] + h0);
-                M.mxn[1][pi] = fx.b * (mx1[r] + h2);
+                const int ro = y * W2 * 4;
+                bstore(fx.bh * (mx0[r] + h0), brsrc(M.mxn[0] + (long long)shot * L.NY * W2, pxb), fx.xi * 4, ro);
+                bstore(fx.b * (mx1[r] + h2), brsrc(M.mxn[1] + (long long)shot * L.NY * W2, pxb), fx.xi * 4, ro);
             }
             if (yiw[i] >= 0) {
-                const long long pi = ((long long)shot * W2 + yiw[i]) * L.NX + x;
-                M.myn[0][pi] = P.by[y] * (my0[i] + h0);
-                M.myn[1][pi] = P.byh[y] * (my1[i] + h2);
+                const int ro = yiw[i] * L.NX * 4;
+                bstore(P.by[y] * (my0[i] + h0), brsrc(M.myn[0] + (long long)shot * W2 * L.NX, pyb), vxc, ro);
+                bstore(P.byh[y] * (my1[i] + h2), brsrc(M.myn[1] + (long long)shot * W2 * L.NX, pyb), vxc, ro);
             }
             c[0][r] = (double)vx[i] * rs[0][r];
             c[1][r] = (double)vy[i] * rs[1][r];

@@ -99,8 +99,10 @@ __device__ __forceinline__ void record_strip(const Rec2D &R, const Layout2D &L, 
         // cells outside the grid store zeros
         const float a = (in && ya + 2 * p < L.NY) ? h[2 * p] * is : 0.f;
         const float b = (in && ya + 2 * p + 1 < L.NY) ? h[2 * p + 1] * is : 0.f;
-        w[p * TILE_X] = (unsigned)__half_as_ushort(__float2half_rn(a)) |
-                        ((unsigned)__half_as_ushort(__float2half_rn(b)) << 16);
+        // written once, read once much later: streamed past the caches
+        __builtin_nontemporal_store((unsigned)__half_as_ushort(__float2half_rn(a)) |
+                                        ((unsigned)__half_as_ushort(__float2half_rn(b)) << 16),
+                                    w + p * TILE_X);
     }
 }
 
