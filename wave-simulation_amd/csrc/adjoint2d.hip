@@ -75,12 +75,11 @@ struct GradSum {
     {
         for (int j = wy; j < NF * S; j += W) acc[j / S][j % S][lane] = 0.0;
     }
-    __device__ __forceinline__ void add(const double (&c)[NF][S], int lane)
+    // the contributions of one row r of the strip
+    __device__ __forceinline__ void add_row(const double (&c)[NF], int r, int lane)
     {
 #pragma unroll
-        for (int f = 0; f < NF; f++)
-#pragma unroll
-            for (int r = 0; r < S; r++) atomicAdd(&acc[f][r][lane], c[f][r]);
+        for (int f = 0; f < NF; f++) atomicAdd(&acc[f][r][lane], c[f]);
     }
     // after a block barrier that follows every add
     __device__ __forceinline__ void flush(double *const (&g)[NF], const Layout2D &L, int x0, int ya, int wy,
@@ -234,7 +233,7 @@ __global__ __launch_bounds__(TILE_X * W) void adj_el2d_stress_strip(Layout2D L, 
     __shared__ float s_g7[W][S][RW];
     __shared__ float s_ty[2][S * K];
     __shared__ __align__(16) float s_cf[3][RW][N];
-    __shared__ GradSum<3, S, W> s_sum;
+    __shared__ GradSum<5, S, W> s_sum;
     const Blk B = xcd_block();
     const int wy = __builtin_amdgcn_readfirstlane(threadIdx.y), lane = threadIdx.x, tid = wy * TILE_X + lane;
     const int x0 = B.bx * TILE_X, ya = B.by * S, xl = x0 - H, x = x0 + lane;
@@ -340,12 +339,13 @@ __global__ __launch_bounds__(TILE_X * W) void adj_el2d_stress_strip(Layout2D L, 
                 hx3[r] = bload(x3, vfh, ro);
             }
         }
-        // own cells: recorded forward derivative sums
-        float rs[3][S];  // h4, h5, h6 + h7
+        // own cells: the forward step's recorded derivative sums h0 + h1, h2 + h3 (velocity update),
+        // h4, h5, h6 + h7 (stress update)
+        float rs[5][S];
         {
-            FwdSums<3, S, REC> fs;
-            fs.init(F, L, 2, sh, x0, ya, lane);
-            fs.fetch(F, L, 2, sh, x0, xc, lane, ya, so, rs);
+            FwdSums<5, S, REC> fs;
+            fs.init(F, L, 0, sh, x0, ya, lane);
+            fs.fetch(F, L, 0, sh, x0, xc, lane, ya, so, rs);
         }
         float cM[N], cmu[N], cmxy[N], hM[S], hmu[S], hmxy[S];
 #pragma unroll
@@ -395,9 +395,6 @@ __global__ __launch_bounds__(TILE_X * W) void adj_el2d_stress_strip(Layout2D L, 
         }
         wave_sync();
 
-        double c[3][S];
-#pragma unroll
-        for (int r = 0; r < S; r++) c[0][r] = c[1][r] = c[2][r] = 0.0;
 #pragma unroll
         for (int r = 0; r < S; r++) {
             const int y = ya + r, i = r + H;
@@ -421,8 +418,9 @@ __global__ __launch_bounds__(TILE_X * W) void adj_el2d_stress_strip(Layout2D L, 
             }
             if (!act || y >= L.NY || !inx) continue;
             const int ro = aboff(L, xl, y);
-            bstore(ovx[r] + dvx, bvx, vc, ro);
-            bstore(ovy[r] + dvy, bvy, vc, ro);
+            const float nvx = ovx[r] + dvx, nvy = ovy[r] + dvy;
+            bstore(nvx, bvx, vc, ro);
+            bstore(nvy, bvy, vc, ro);
             // adjoint memory variables of the own cell (write buffers)
             const float ls = lxx[i] + lyy[i];
             const float h4 = cM[i] * ls - 2.f * cmu[i] * lyy[i], h5 = cM[i] * ls - 2.f * cmu[i] * lxx[i];
@@ -437,16 +435,19 @@ __global__ __launch_bounds__(TILE_X * W) void adj_el2d_stress_strip(Layout2D L, 
                 bstore(P.by[y] * (my2[i] + h5), brsrc(M.myn[2] + (long long)shot * W2 * L.NX, pyb), vxc, ro);
                 bstore(P.byh[y] * (my3[i] + h6), brsrc(M.myn[3] + (long long)shot * W2 * L.NX, pyb), vxc, ro);
             }
-            // gradient terms with the forward step's recorded derivative sums
-            c[0][r] = (double)ls * ((double)rs[0][r] + (double)rs[1][r]);
-            c[1][r] = -2.0 * (double)lxx[i] * rs[1][r] - 2.0 * (double)lyy[i] * rs[0][r];
-            c[2][r] = (double)lxy[i] * (double)rs[2][r];
+            // gradient terms with the forward step's recorded derivative sums: the velocity
+            // update's (1/rho_x, 1/rho_y) take the adjoint velocities just formed, which the
+            // adjoint velocity step of the same time step reads
+            const double c[5] = {(double)nvx * rs[0][r], (double)nvy * rs[1][r],
+                                 (double)ls * ((double)rs[2][r] + (double)rs[3][r]),
+                                 -2.0 * (double)lxx[i] * rs[3][r] - 2.0 * (double)lyy[i] * rs[2][r],
+                                 (double)lxy[i] * (double)rs[4][r]};
+            s_sum.add_row(c, r, lane);
         }
-        if (act) s_sum.add(c, lane);
         wave_sync();  // the next shot group rewrites this wavefront's LDS strips
     }
     block_sync_lds();
-    double *const g[3] = {G.g[2], G.g[3], G.g[4]};
+    double *const g[5] = {G.g[0], G.g[1], G.g[2], G.g[3], G.g[4]};
     s_sum.flush(g, L, x0, ya, wy, lane);
 }
 
@@ -454,12 +455,13 @@ template <int H, int S, int W, bool REC, bool LOOP>
 __global__ __launch_bounds__(TILE_X * W) void adj_el2d_velocity_strip(Layout2D L, int nshots, AdjFields A, AdjPsi M,
                                                                      FwdSnap F, Coef2D C, TapsT TT, Cpml2D P, Grad G)
 {
+    (void)F;  // the gradient terms of this step are summed by the adjoint stress step
+    (void)G;
     constexpr int K = 2 * H + 1, N = S + 2 * H, RW = TILE_X + 2 * H;
     __shared__ float s_g0[W][S][RW];
     __shared__ float s_g2[W][S][RW];
     __shared__ float s_ty[2][S * K];
     __shared__ __align__(16) float s_cf[2][RW][N];
-    __shared__ GradSum<2, S, W> s_sum;
     const Blk B = xcd_block();
     const int wy = __builtin_amdgcn_readfirstlane(threadIdx.y), lane = threadIdx.x, tid = wy * TILE_X + lane;
     const int x0 = B.bx * TILE_X, ya = B.by * S, xl = x0 - H, x = x0 + lane;
@@ -498,7 +500,6 @@ __global__ __launch_bounds__(TILE_X * W) void adj_el2d_velocity_strip(Layout2D L
     load_taps<H>(txf, TT.xf, x);
     load_taps<H>(txb, TT.xb, x);
     cw.store(s_cf, tid);
-    s_sum.zero(wy, lane);
     block_sync_lds();
 
     // LOOP: the block takes every shot, wavefront wy shots wy, wy + W, ... (one shot group per
@@ -562,12 +563,6 @@ __global__ __launch_bounds__(TILE_X * W) void adj_el2d_velocity_strip(Layout2D L
                 hx1[r] = bload(x3, vfh, ro);
             }
         }
-        float rs[2][S];  // recorded h0 + h1, h2 + h3
-        {
-            FwdSums<2, S, REC> fs;
-            fs.init(F, L, 0, sh, x0, ya, lane);
-            fs.fetch(F, L, 0, sh, x0, xc, lane, ya, so, rs);
-        }
         float cx[N], cy[N], hcx[S], hcy[S];
 #pragma unroll
         for (int i = 0; i < N; i++) {
@@ -611,9 +606,6 @@ __global__ __launch_bounds__(TILE_X * W) void adj_el2d_velocity_strip(Layout2D L
         }
         wave_sync();
 
-        double c[2][S];
-#pragma unroll
-        for (int r = 0; r < S; r++) c[0][r] = c[1][r] = 0.0;
 #pragma unroll
         for (int r = 0; r < S; r++) {
             const int y = ya + r, i = r + H;
@@ -647,15 +639,9 @@ __global__ __launch_bounds__(TILE_X * W) void adj_el2d_velocity_strip(Layout2D L
                 bstore(P.by[y] * (my0[i] + h0), brsrc(M.myn[0] + (long long)shot * W2 * L.NX, pyb), vxc, ro);
                 bstore(P.byh[y] * (my1[i] + h2), brsrc(M.myn[1] + (long long)shot * W2 * L.NX, pyb), vxc, ro);
             }
-            c[0][r] = (double)vx[i] * rs[0][r];
-            c[1][r] = (double)vy[i] * rs[1][r];
         }
-        if (act) s_sum.add(c, lane);
         wave_sync();  // the next shot group rewrites this wavefront's LDS strips
     }
-    block_sync_lds();
-    double *const g[2] = {G.g[0], G.g[1]};
-    s_sum.flush(g, L, x0, ya, wy, lane);
 }
 
 __global__ __launch_bounds__(256) void adj_inject(Layout2D L, AdjFields A, Acq Q, const float *__restrict__ adj, int t)
