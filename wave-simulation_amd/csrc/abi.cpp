@@ -222,10 +222,13 @@ double wv_solver::kernel_bytes(int kind) const
         // adjoint pass (2D elastic): per shot-cell field bytes, per cell coefficient bytes
         //   3 recomputed velocity phase: forward velocity phase + 2 recorded derivative sums written
         //   4 recomputed stress phase  : forward stress phase + 3 recorded derivative sums written
-        //   5 adjoint stress step      : r lsxx lsyy lsxy h4 h5 h67, r+w lvx lvy, 3 FP64 gradient updates (16 B)
-        //   6 adjoint velocity step    : r lvx lvy h01 h23, r+w lsxx lsyy lsxy, 2 FP64 gradient updates
-        static const double per_shot[4] = {4.0 * (7 + 2), 4.0 * (8 + 3), 4.0 * 10 + 48, 4.0 * 10 + 32};
-        static const double per_cell[4] = {4.0 * 2, 4.0 * 3, 4.0 * 3, 4.0 * 2};
+        //   5 adjoint stress step      : r lsxx lsyy lsxy, r+w lvx lvy, the 5 recorded sums (FP16 record or
+        //                                FP32 segment buffers); per cell 3 coefficients and 5 FP64 gradient
+        //                                updates (one per cell: the shots are summed in the block)
+        //   6 adjoint velocity step    : r lvx lvy, r+w lsxx lsyy lsxy; per cell 2 coefficients
+        const double hb = adjoint_record(this) ? 2.0 : 4.0;
+        const double per_shot[4] = {4.0 * (7 + 2), 4.0 * (8 + 3), 4.0 * 7 + 5 * hb, 4.0 * 8};
+        const double per_cell[4] = {4.0 * 2, 4.0 * 3, 4.0 * 3 + 8.0 * 5, 4.0 * 2};
         return cells * (B * per_shot[kind - 3] + per_cell[kind - 3]);
     }
     // {field reads+writes per shot-cell, coefficient reads per cell, psi per x-frame cell, psi per y-frame cell}
