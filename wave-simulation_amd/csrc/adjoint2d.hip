@@ -161,15 +161,6 @@ __device__ __forceinline__ bool strip_in_frame(const Layout2D &L, const Cpml2D &
     return P.on && (x0 - H < P.W || x0 + TILE_X + H > L.NX - P.W || ya - H < P.W || ya + S + H > L.NY - P.W);
 }
 
-// Block barrier that orders LDS accesses only: global loads issued before it
-// stay in flight.
-__device__ __forceinline__ void block_sync_lds()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-    __builtin_amdgcn_s_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-}
-
 // Window rows y0 .. y0+N-1, columns xl .. xl+RW-1 of NC coefficient planes,
 // which are the same for every shot of a block: the block's T threads load it
 // once and store it column-major in LDS, so a lane reads its column's window

@@ -11,6 +11,9 @@
 // tile plus its halo is staged in LDS for the x and z derivatives; the y
 // derivatives come from a register queue of the 2H+1 planes around the
 // current one, so every plane of a field is read from HBM once per block.
+#include <type_traits>
+
+#include "buffer.hpp"
 #include "elastic3d.hpp"
 
 namespace wv {
@@ -53,6 +56,115 @@ struct Region {
         }
     }
 };
+
+// Byte offsets, within one y plane, of a thread's share of a Region (the same
+// for every plane, so they are computed once per block).
+template <int HX, int HZ, int TZ>
+struct RegionOff {
+    using R = Region<HX, HZ, TZ>;
+    unsigned o[R::IT];
+    __device__ __forceinline__ void init(const Layout3D &L, int x0, int z0, int tid)
+    {
+#pragma unroll
+        for (int k = 0; k < R::IT; k++) {
+            const int i = min(tid + k * NTH, R::TOT - 1);  // surplus threads reload the last element
+            const int r = i / R::RW, c = i - r * R::RW;
+            o[k] = (unsigned)(((z0 - HZ + r + L.H) * L.pitch + (x0 - HX + c + XPAD)) * 4);
+        }
+    }
+};
+
+// One y plane of a field as a buffer resource (plane offsets stay 32-bit for any grid).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const float *f, const Layout3D &L, int y)
+{
+    return brsrc(f + (long long)(y + L.H) * L.slab, (unsigned)(L.slab * 4));
+}
+
+// Buffer offset that the hardware answers with 0 (loads) or drops (stores):
+// the CPML accesses of lanes outside a frame, without a branch.
+constexpr unsigned NO_LANE = 0x7ffffff0u;
+
+// CPML memory variables of one shot as buffers: x frame [NY][NZ][2W], y frame
+// [2W][NZ][NX], z frame [NY][2W][NX] (per shot, so offsets stay 32-bit).
+// Built once per kernel, before the plane loop.
+struct PsiRes3D {
+    __amdgpu_buffer_rsrc_t x[3], y[3], z[3];
+    __device__ __forceinline__ void init(const Cpml3D &P, const Layout3D &L, int shot, int kx0, int kx1, int kx2,
+                                         int ky0, int ky1, int ky2, int kz0, int kz1, int kz2)
+    {
+        const long long nx = (long long)L.NY * L.NZ * 2 * P.W, ny = (long long)2 * P.W * L.NZ * L.NX,
+                        nz = (long long)L.NY * 2 * P.W * L.NX;
+        const int kx[3] = {kx0, kx1, kx2}, ky[3] = {ky0, ky1, ky2}, kz[3] = {kz0, kz1, kz2};
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            x[j] = brsrc(P.ps[kx[j]] + shot * nx, (unsigned)(nx * 4));
+            y[j] = brsrc(P.ps[ky[j]] + shot * ny, (unsigned)(ny * 4));
+            z[j] = brsrc(P.ps[kz[j]] + shot * nz, (unsigned)(nz * 4));
+        }
+    }
+};
+// lane byte offsets of cell (x, y, z) in the three frames (NO_LANE outside)
+__device__ __forceinline__ unsigned psi_ox(const Layout3D &L, int W2, int xi, int y, int z, bool in)
+{
+    return (in && xi >= 0) ? (unsigned)(((y * L.NZ + z) * W2 + xi) * 4) : NO_LANE;
+}
+__device__ __forceinline__ unsigned psi_oy(const Layout3D &L, int yi, int x, int z, bool in)
+{
+    return (in && yi >= 0) ? (unsigned)(((yi * L.NZ + z) * L.NX + x) * 4) : NO_LANE;
+}
+__device__ __forceinline__ unsigned psi_oz(const Layout3D &L, int W2, int zi, int x, int y, bool in)
+{
+    return (in && zi >= 0) ? (unsigned)(((y * W2 + zi) * L.NX + x) * 4) : NO_LANE;
+}
+
+// This thread's element of plane y's y table (Taps3D::ys / yv layout, 2K + 5 entries): the
+// tables are staged in LDS a plane ahead, one element per thread.
+template <int K>
+__device__ __forceinline__ float ytab_elem(const float *tab, int y, int tid)
+{
+    return tab[(long long)y * (2 * K + 5) + min(tid, 2 * K + 4)];
+}
+
+// z tables of the tile's TZ rows staged in LDS before the plane loop: taps of the backward and
+// forward z operators, CPML z index and a_z, b_z, a_z(half), b_z(half) (0 outside the frame)
+template <int TZ, int K>
+struct ZTab {
+    float t[2][TZ][K];
+    float c[TZ][4];
+    int zi[TZ];
+    __device__ __forceinline__ void fill(const Taps3D &T, const Cpml3D &P, const Layout3D &L, int z0, int tid)
+    {
+        for (int e = tid; e < TZ * K; e += NTH) {
+            const int r = e / K, k = e - r * K;
+            t[0][r][k] = T.zb[(long long)(z0 + r) * K + k];
+            t[1][r][k] = T.zf[(long long)(z0 + r) * K + k];
+        }
+        if (tid < TZ) {
+            const int z = z0 + tid;
+            const int zi_ = (P.on && z < L.NZ) ? P.zi[z] : -1;
+            zi[tid] = zi_;
+            c[tid][0] = zi_ >= 0 ? P.az[z] : 0.f;
+            c[tid][1] = zi_ >= 0 ? P.bz[z] : 0.f;
+            c[tid][2] = zi_ >= 0 ? P.azh[z] : 0.f;
+            c[tid][3] = zi_ >= 0 ? P.bzh[z] : 0.f;
+        }
+    }
+};
+
+// Whether any cell of the block's x-z tile lies in the x or z CPML frame (block-uniform).
+__device__ __forceinline__ bool tile_in_frame(const Layout3D &L, const Cpml3D &P, int x0, int z0, int TZ)
+{
+    return P.on && (x0 < P.W || x0 + TILE_X > L.NX - P.W || z0 < P.W || z0 + TZ > L.NZ - P.W);
+}
+
+template <int HX, int HZ, int TZ>
+__device__ __forceinline__ void load_region(Region<HX, HZ, TZ> &g, const RegionOff<HX, HZ, TZ> &o, const float *f,
+                                            const Layout3D &L, int y)
+{
+    const auto b = plane_rsrc(f, L, y);
+#pragma unroll
+    for (int k = 0; k < Region<HX, HZ, TZ>::IT; k++) g.v[k] = bload(b, o.o[k], 0);
+}
 
 // derivative along x at LDS row r (array with an H-wide x halo), column tx
 template <int H, int RW>
@@ -191,21 +303,17 @@ __global__ __launch_bounds__(NTH) void el3d_velocity(Layout3D L, Fields3D F, Coe
     __shared__ float s_xz[RXZ::RH][RXZ::RW];
     __shared__ float s_yz[RZZ::RH][RZZ::RW];
     __shared__ float s_zz[RZZ::RH][RZZ::RW];
+    __shared__ ZTab<TZ, K> s_zt;
+    __shared__ float s_yt[2 * K + 5];
     const Pos q = position(L, TZ, nyc, LY, A);
     const long long so = (long long)q.shot * L.vol;
     const int tid = threadIdx.y * TILE_X + threadIdx.x, tx = threadIdx.x;
     const int wz = __builtin_amdgcn_readfirstlane(threadIdx.y);
     if (pend) acquisition3d(A, F, so, q.sb, A.tbase[0] + tofs - 1, true);
 
-    const float *__restrict__ sxx = F.f[3] + so;
-    const float *__restrict__ syy = F.f[4] + so;
-    const float *__restrict__ szz = F.f[5] + so;
-    const float *__restrict__ syz = F.f[6] + so;
-    const float *__restrict__ sxz = F.f[7] + so;
-    const float *__restrict__ sxy = F.f[8] + so;
-    float *__restrict__ vx = F.f[0] + so;
-    float *__restrict__ vy = F.f[1] + so;
-    float *__restrict__ vz = F.f[2] + so;
+    const float *sxx = F.f[3] + so, *syy = F.f[4] + so, *szz = F.f[5] + so;
+    const float *syz = F.f[6] + so, *sxz = F.f[7] + so, *sxy = F.f[8] + so;
+    float *vx = F.f[0] + so, *vy = F.f[1] + so, *vz = F.f[2] + so;
 
     const int x = q.x0 + tx;
     const bool xin = x < L.NX;
@@ -224,106 +332,139 @@ __global__ __launch_bounds__(NTH) void el3d_velocity(Layout3D L, Fields3D F, Coe
         bxh = P.bxh[x];
     }
     const int W2 = 2 * P.W;
+    // byte offsets within a plane: the thread's own cells and its shares of the regions
+    unsigned oc[RZ];
+#pragma unroll
+    for (int r = 0; r < RZ; r++) oc[r] = (unsigned)(((q.z0 + wz + r * BLOCK_Y + L.H) * L.pitch + x + XPAD) * 4);
+    RegionOff<H, 0, TZ> ox;
+    RegionOff<H, H, TZ> oxz;
+    RegionOff<0, H, TZ> ozz;
+    ox.init(L, q.x0, q.z0, tid);
+    oxz.init(L, q.x0, q.z0, tid);
+    ozz.init(L, q.x0, q.z0, tid);
 
     // y queues of the stresses differentiated along y: sxy (Dyb), syy (Dyf), syz (Dyb)
     float qxy[K][RZ], qyy[K][RZ], qyz[K][RZ];
 #pragma unroll
-    for (int k = 0; k < 2 * H; k++)
+    for (int k = 0; k < 2 * H; k++) {
+        const auto b0 = plane_rsrc(sxy, L, q.ys - H + k), b1 = plane_rsrc(syy, L, q.ys - H + k),
+                   b2 = plane_rsrc(syz, L, q.ys - H + k);
 #pragma unroll
         for (int r = 0; r < RZ; r++) {
-            const long long o = L.at(x, q.ys - H + k, q.z0 + wz + r * BLOCK_Y);
-            qxy[k + 1][r] = sxy[o];
-            qyy[k + 1][r] = syy[o];
-            qyz[k + 1][r] = syz[o];
+            qxy[k + 1][r] = bload(b0, oc[r], 0);
+            qyy[k + 1][r] = bload(b1, oc[r], 0);
+            qyz[k + 1][r] = bload(b2, oc[r], 0);
         }
+    }
+
+    // everything a plane reads besides the CPML memory variables; the next plane's set is
+    // loaded while the current one is computed
+    struct PlaneIn {
+        float nxy[RZ], nyy[RZ], nyz[RZ];  // queue entries at y + H
+        float o[3][RZ];                   // vx vy vz at y
+        float c[3][RZ];                   // 1/rho_x 1/rho_y 1/rho_z at y
+        float yt;                         // this thread's element of the plane's y tables
+    };
+    const float *const cr[3] = {C.irx, C.iry, C.irz};
+    const float *const vv[3] = {vx, vy, vz};
+    RX rxx, rxy;
+    RXZ rxz;
+    RZZ ryz, rzz;
+    auto load_plane = [&](PlaneIn &pi, int y) {
+        load_region(rxx, ox, sxx, L, y);
+        load_region(rxy, ox, sxy, L, y);
+        load_region(rxz, oxz, sxz, L, y);
+        load_region(ryz, ozz, syz, L, y);
+        load_region(rzz, ozz, szz, L, y);
+        {
+            const auto b0 = plane_rsrc(sxy, L, y + H), b1 = plane_rsrc(syy, L, y + H), b2 = plane_rsrc(syz, L, y + H);
+#pragma unroll
+            for (int r = 0; r < RZ; r++) {
+                pi.nxy[r] = bload(b0, oc[r], 0);
+                pi.nyy[r] = bload(b1, oc[r], 0);
+                pi.nyz[r] = bload(b2, oc[r], 0);
+            }
+        }
+        pi.yt = ytab_elem<K>(T.yv, y, tid);
+#pragma unroll
+        for (int f = 0; f < 3; f++) {
+            const auto b = plane_rsrc(vv[f], L, y), bc = plane_rsrc(cr[f], L, y);
+#pragma unroll
+            for (int r = 0; r < RZ; r++) {
+                pi.o[f][r] = bload(b, oc[r], 0);
+                pi.c[f][r] = bload(bc, oc[r], 0);
+            }
+        }
+    };
+
+    PlaneIn cur;
+    load_plane(cur, q.ys);
+    const bool tfr = tile_in_frame(L, P, q.x0, q.z0, TZ);
+    s_zt.fill(T, P, L, q.z0, tid);  // ordered before its first use by the first plane's barrier
+    PsiRes3D pr;  // 0 sxx_x, 3 sxy_x, 6 sxz_x | 1 sxy_y, 4 syy_y, 7 syz_y | 2 sxz_z, 5 syz_z, 8 szz_z
+    pr.init(P, L, q.shot, 0, 3, 6, 1, 4, 7, 2, 5, 8);
 
     for (int y = q.ys; y < q.ye; y++) {
         shift<H, RZ>(qxy);
         shift<H, RZ>(qyy);
         shift<H, RZ>(qyz);
-        float ovx[RZ], ovy[RZ], ovz[RZ], crx[RZ], cry[RZ], crz[RZ];
 #pragma unroll
         for (int r = 0; r < RZ; r++) {
-            const int z = q.z0 + wz + r * BLOCK_Y;
-            const long long on = L.at(x, y + H, z);
-            qxy[2 * H][r] = sxy[on];
-            qyy[2 * H][r] = syy[on];
-            qyz[2 * H][r] = syz[on];
-            const long long o = L.at(x, y, z);
-            ovx[r] = vx[o];
-            ovy[r] = vy[o];
-            ovz[r] = vz[o];
-            crx[r] = C.irx[o];
-            cry[r] = C.iry[o];
-            crz[r] = C.irz[o];
-        }
-        RX rxx, rxy;
-        RXZ rxz;
-        RZZ ryz, rzz;
-        rxx.load(sxx, L, q.x0, y, q.z0, tid);
-        rxy.load(sxy, L, q.x0, y, q.z0, tid);
-        rxz.load(sxz, L, q.x0, y, q.z0, tid);
-        ryz.load(syz, L, q.x0, y, q.z0, tid);
-        rzz.load(szz, L, q.x0, y, q.z0, tid);
-        const int yi = P.on ? P.yi[y] : -1;
-        // CPML memory variables of this plane
-        float p0[RZ], p3[RZ], p6[RZ], p1[RZ], p4[RZ], p7[RZ], p2[RZ], p5[RZ], p8[RZ];
-#pragma unroll
-        for (int r = 0; r < RZ; r++) {
-            const int z = q.z0 + wz + r * BLOCK_Y;
-            p0[r] = p3[r] = p6[r] = p1[r] = p4[r] = p7[r] = p2[r] = p5[r] = p8[r] = 0.f;
-            if (z >= L.NZ) continue;
-            if (xi >= 0) {
-                const long long pi = (((long long)q.shot * L.NY + y) * L.NZ + z) * W2 + xi;
-                p0[r] = P.ps[0][pi];
-                p3[r] = P.ps[3][pi];
-                p6[r] = P.ps[6][pi];
-            }
-            if (yi >= 0 && xin) {
-                const long long pi = (((long long)q.shot * W2 + yi) * L.NZ + z) * L.NX + x;
-                p1[r] = P.ps[1][pi];
-                p4[r] = P.ps[4][pi];
-                p7[r] = P.ps[7][pi];
-            }
-            const int zi = P.on ? P.zi[z] : -1;
-            if (zi >= 0 && xin) {
-                const long long pi = (((long long)q.shot * L.NY + y) * W2 + zi) * L.NX + x;
-                p2[r] = P.ps[2][pi];
-                p5[r] = P.ps[5][pi];
-                p8[r] = P.ps[8][pi];
-            }
+            qxy[2 * H][r] = cur.nxy[r];
+            qyy[2 * H][r] = cur.nyy[r];
+            qyz[2 * H][r] = cur.nyz[r];
         }
         rxx.store(s_xx, tid);
         rxy.store(s_xy, tid);
         rxz.store(s_xz, tid);
         ryz.store(s_yz, tid);
         rzz.store(s_zz, tid);
-        __syncthreads();
+        if (tid < 2 * K + 5) s_yt[tid] = cur.yt;
+        block_sync_lds();
 
-        const float *cyb = T.yb_v + (long long)y * K;
-        const float *cyf = T.yf_v + (long long)y * K;
-        float ay = 0.f, by = 0.f, ayh = 0.f, byh = 0.f;
-        if (yi >= 0) {
-            ay = P.ay[y];
-            by = P.by[y];
-            ayh = P.ayh[y];
-            byh = P.byh[y];
+        // CPML memory variables: tiles and planes outside every frame skip them (a uniform
+        // branch); lanes outside a frame read 0 and drop their stores (no divergent branches, so
+        // the next plane's loads stay in flight through the plane's arithmetic)
+        const int yi = __builtin_amdgcn_readfirstlane(P.on ? __float_as_int(s_yt[2 * K]) : -1);
+        // (readfirstlane: a branch the compiler sees as uniform, so y and the plane resources stay scalar)
+        const bool fr = __builtin_amdgcn_readfirstlane((int)(tfr || yi >= 0)) != 0;
+        float p0[RZ], p3[RZ], p6[RZ], p1[RZ], p4[RZ], p7[RZ], p2[RZ], p5[RZ], p8[RZ];
+        unsigned pox[RZ], poy[RZ], poz[RZ];
+#pragma unroll
+        for (int r = 0; r < RZ; r++) {
+            const int z = q.z0 + wz + r * BLOCK_Y;
+            const bool in = z < L.NZ && xin;
+            p0[r] = p3[r] = p6[r] = p1[r] = p4[r] = p7[r] = p2[r] = p5[r] = p8[r] = 0.f;
+            if (fr) {
+                pox[r] = psi_ox(L, W2, xi, y, z, in);
+                poy[r] = psi_oy(L, yi, x, z, in);
+                poz[r] = psi_oz(L, W2, s_zt.zi[wz + r * BLOCK_Y], x, y, in);
+                p0[r] = bload(pr.x[0], pox[r], 0);
+                p3[r] = bload(pr.x[1], pox[r], 0);
+                p6[r] = bload(pr.x[2], pox[r], 0);
+                p1[r] = bload(pr.y[0], poy[r], 0);
+                p4[r] = bload(pr.y[1], poy[r], 0);
+                p7[r] = bload(pr.y[2], poy[r], 0);
+                p2[r] = bload(pr.z[0], poz[r], 0);
+                p5[r] = bload(pr.z[1], poz[r], 0);
+                p8[r] = bload(pr.z[2], poz[r], 0);
+            }
         }
+        // (also after the last plane: plane ye and its queue plane ye + H lie inside the padded
+        // volume, and a branch here would make the compiler wait for every load at the join)
+        PlaneIn nxt;
+        load_plane(nxt, y + 1);
+
+        const float *cyb = s_yt, *cyf = s_yt + K;
+        const bool yf = yi >= 0;
+        const float ay = yf ? s_yt[2 * K + 1] : 0.f, by = yf ? s_yt[2 * K + 2] : 0.f;
+        const float ayh = yf ? s_yt[2 * K + 3] : 0.f, byh = yf ? s_yt[2 * K + 4] : 0.f;
 #pragma unroll
         for (int r = 0; r < RZ; r++) {
             const int lz = wz + r * BLOCK_Y;
             const int z = q.z0 + lz;
             if (z >= L.NZ || !xin) continue;
-            const float *czb = T.zb + (long long)z * K;
-            const float *czf = T.zf + (long long)z * K;
-            const int zi = P.on ? P.zi[z] : -1;
-            float az = 0.f, bz = 0.f, azh = 0.f, bzh = 0.f;
-            if (zi >= 0) {
-                az = P.az[z];
-                bz = P.bz[z];
-                azh = P.azh[z];
-                bzh = P.bzh[z];
-            }
+            const float *czb = s_zt.t[0][lz], *czf = s_zt.t[1][lz];
             // vx += 1/rho_x (Dxf sxx + Dyb sxy + Dzb sxz)
             float u = ddx<H>(s_xx, lz, tx, cxf);
             float t = ddy<H, RZ>(qxy, r, cyb);
@@ -336,39 +477,34 @@ __global__ __launch_bounds__(NTH) void el3d_velocity(Layout3D L, Fields3D F, Coe
             float u3 = ddx<H>(s_xz, lz + H, tx, cxb);
             float t3 = ddy<H, RZ>(qyz, r, cyb);
             float w3 = ddz<H>(s_zz, lz, tx, czf);
-            if (xi >= 0) {
-                const long long pi = (((long long)q.shot * L.NY + y) * L.NZ + z) * W2 + xi;
-                P.ps[0][pi] = cpml(u, p0[r], axh, bxh);
-                P.ps[3][pi] = cpml(u2, p3[r], ax, bx);
-                P.ps[6][pi] = cpml(u3, p6[r], ax, bx);
+            if (fr) {
+                const float az = s_zt.c[lz][0], bz = s_zt.c[lz][1], azh = s_zt.c[lz][2], bzh = s_zt.c[lz][3];
+                // outside a frame a = b = 0 and psi = 0: the derivative is unchanged
+                bstore(cpml(u, p0[r], axh, bxh), pr.x[0], pox[r], 0);
+                bstore(cpml(u2, p3[r], ax, bx), pr.x[1], pox[r], 0);
+                bstore(cpml(u3, p6[r], ax, bx), pr.x[2], pox[r], 0);
+                bstore(cpml(t, p1[r], ay, by), pr.y[0], poy[r], 0);
+                bstore(cpml(t2, p4[r], ayh, byh), pr.y[1], poy[r], 0);
+                bstore(cpml(t3, p7[r], ay, by), pr.y[2], poy[r], 0);
+                bstore(cpml(w, p2[r], az, bz), pr.z[0], poz[r], 0);
+                bstore(cpml(w2, p5[r], az, bz), pr.z[1], poz[r], 0);
+                bstore(cpml(w3, p8[r], azh, bzh), pr.z[2], poz[r], 0);
             }
-            if (yi >= 0) {
-                const long long pi = (((long long)q.shot * W2 + yi) * L.NZ + z) * L.NX + x;
-                P.ps[1][pi] = cpml(t, p1[r], ay, by);
-                P.ps[4][pi] = cpml(t2, p4[r], ayh, byh);
-                P.ps[7][pi] = cpml(t3, p7[r], ay, by);
-            }
-            if (zi >= 0) {
-                const long long pi = (((long long)q.shot * L.NY + y) * W2 + zi) * L.NX + x;
-                P.ps[2][pi] = cpml(w, p2[r], az, bz);
-                P.ps[5][pi] = cpml(w2, p5[r], az, bz);
-                P.ps[8][pi] = cpml(w3, p8[r], azh, bzh);
-            }
-            const long long o = L.at(x, y, z);
             u += t;
             u += w;
-            u *= crx[r];
-            vx[o] = ovx[r] + u;
+            u *= cur.c[0][r];
+            bstore(cur.o[0][r] + u, plane_rsrc(vx, L, y), oc[r], 0);
             u2 += t2;
             u2 += w2;
-            u2 *= cry[r];
-            vy[o] = ovy[r] + u2;
+            u2 *= cur.c[1][r];
+            bstore(cur.o[1][r] + u2, plane_rsrc(vy, L, y), oc[r], 0);
             u3 += t3;
             u3 += w3;
-            u3 *= crz[r];
-            vz[o] = ovz[r] + u3;
+            u3 *= cur.c[2][r];
+            bstore(cur.o[2][r] + u3, plane_rsrc(vz, L, y), oc[r], 0);
         }
-        __syncthreads();
+        block_sync_lds();  // the next plane rewrites the LDS regions
+        cur = nxt;
     }
 }
 
@@ -381,20 +517,16 @@ __global__ __launch_bounds__(NTH) void el3d_stress(Layout3D L, Fields3D F, Coef3
     __shared__ float s_vx[RXZ::RH][RXZ::RW];
     __shared__ float s_vy[RXZ::RH][RXZ::RW];
     __shared__ float s_vz[RXZ::RH][RXZ::RW];
+    __shared__ ZTab<TZ, K> s_zt;
+    __shared__ float s_yt[2 * K + 5];
     const Pos q = position(L, TZ, nyc, LY, A);
     const long long so = (long long)q.shot * L.vol;
     const int tid = threadIdx.y * TILE_X + threadIdx.x, tx = threadIdx.x;
     const int wz = __builtin_amdgcn_readfirstlane(threadIdx.y);
 
-    const float *__restrict__ vx = F.f[0] + so;
-    const float *__restrict__ vy = F.f[1] + so;
-    const float *__restrict__ vz = F.f[2] + so;
-    float *__restrict__ sxx = F.f[3] + so;
-    float *__restrict__ syy = F.f[4] + so;
-    float *__restrict__ szz = F.f[5] + so;
-    float *__restrict__ syz = F.f[6] + so;
-    float *__restrict__ sxz = F.f[7] + so;
-    float *__restrict__ sxy = F.f[8] + so;
+    const float *vx = F.f[0] + so, *vy = F.f[1] + so, *vz = F.f[2] + so;
+    float *sxx = F.f[3] + so, *syy = F.f[4] + so, *szz = F.f[5] + so;
+    float *syz = F.f[6] + so, *sxz = F.f[7] + so, *sxy = F.f[8] + so;
 
     const int x = q.x0 + tx;
     const bool xin = x < L.NX;
@@ -412,180 +544,213 @@ __global__ __launch_bounds__(NTH) void el3d_stress(Layout3D L, Fields3D F, Coef3
         axh = P.axh[x];
         bxh = P.bxh[x];
     }
-    const int W2 = 2 * P.W;
+    // byte offsets within a plane: the thread's own cells and its share of the x-z region
+    unsigned oc[RZ];
+#pragma unroll
+    for (int r = 0; r < RZ; r++) oc[r] = (unsigned)(((q.z0 + wz + r * BLOCK_Y + L.H) * L.pitch + x + XPAD) * 4);
+    RegionOff<H, H, TZ> ro;
+    ro.init(L, q.x0, q.z0, tid);
 
     // y queues: vx (Dyf), vy (Dyb), vz (Dyf)
     float qvx[K][RZ], qvy[K][RZ], qvz[K][RZ];
 #pragma unroll
-    for (int k = 0; k < 2 * H; k++)
+    for (int k = 0; k < 2 * H; k++) {
+        const auto b0 = plane_rsrc(vx, L, q.ys - H + k), b1 = plane_rsrc(vy, L, q.ys - H + k),
+                   b2 = plane_rsrc(vz, L, q.ys - H + k);
 #pragma unroll
         for (int r = 0; r < RZ; r++) {
-            const long long o = L.at(x, q.ys - H + k, q.z0 + wz + r * BLOCK_Y);
-            qvx[k + 1][r] = vx[o];
-            qvy[k + 1][r] = vy[o];
-            qvz[k + 1][r] = vz[o];
+            qvx[k + 1][r] = bload(b0, oc[r], 0);
+            qvy[k + 1][r] = bload(b1, oc[r], 0);
+            qvz[k + 1][r] = bload(b2, oc[r], 0);
         }
+    }
+
+    // everything a plane reads besides the CPML memory variables; the next plane's set is
+    // loaded while the current one is computed
+    struct PlaneIn {
+        float nvx[RZ], nvy[RZ], nvz[RZ];  // queue entries at y + H
+        float o[6][RZ];                   // sxx syy szz syz sxz sxy at y
+        float c[6][RZ];                   // M mu mu_xy mu_xz mu_yz damp at y
+        float yt;                         // this thread's element of the plane's y tables
+    };
+    const float *const cf[5] = {C.M, C.mu, C.muxy, C.muxz, C.muyz};
+    const float *const st[6] = {sxx, syy, szz, syz, sxz, sxy};
+    auto load_plane = [&](PlaneIn &pi, RXZ &gx, RXZ &gy, RXZ &gz, int y) {
+        load_region(gx, ro, vx, L, y);
+        load_region(gy, ro, vy, L, y);
+        load_region(gz, ro, vz, L, y);
+        {
+            const auto b0 = plane_rsrc(vx, L, y + H), b1 = plane_rsrc(vy, L, y + H), b2 = plane_rsrc(vz, L, y + H);
+#pragma unroll
+            for (int r = 0; r < RZ; r++) {
+                pi.nvx[r] = bload(b0, oc[r], 0);
+                pi.nvy[r] = bload(b1, oc[r], 0);
+                pi.nvz[r] = bload(b2, oc[r], 0);
+            }
+        }
+#pragma unroll
+        for (int f = 0; f < 6; f++) {
+            const auto b = plane_rsrc(st[f], L, y);
+#pragma unroll
+            for (int r = 0; r < RZ; r++) pi.o[f][r] = bload(b, oc[r], 0);
+        }
+#pragma unroll
+        for (int f = 0; f < 5; f++) {
+            const auto b = plane_rsrc(cf[f], L, y);
+#pragma unroll
+            for (int r = 0; r < RZ; r++) pi.c[f][r] = bload(b, oc[r], 0);
+        }
+        pi.yt = ytab_elem<K>(T.ys, y, tid);
+        if (C.damp) {
+            const auto b = plane_rsrc(C.damp, L, y);
+#pragma unroll
+            for (int r = 0; r < RZ; r++) pi.c[5][r] = bload(b, oc[r], 0);
+        } else {
+#pragma unroll
+            for (int r = 0; r < RZ; r++) pi.c[5][r] = 1.f;
+        }
+    };
+
+    PlaneIn cur;
+    RXZ rvx, rvy, rvz;
+    load_plane(cur, rvx, rvy, rvz, q.ys);
+    const bool tfr = tile_in_frame(L, P, q.x0, q.z0, TZ);
+    s_zt.fill(T, P, L, q.z0, tid);  // ordered before its first use by the first plane's barrier
+    PsiRes3D pr;  // 9 vx_x, 13 vy_x, 15 vz_x | 10 vy_y, 12 vx_y, 17 vz_y | 11 vz_z, 14 vx_z, 16 vy_z
+    pr.init(P, L, q.shot, 9, 13, 15, 10, 12, 17, 11, 14, 16);
+    const int W2 = 2 * P.W;
 
     for (int y = q.ys; y < q.ye; y++) {
         shift<H, RZ>(qvx);
         shift<H, RZ>(qvy);
         shift<H, RZ>(qvz);
-        float o_xx[RZ], o_yy[RZ], o_zz[RZ], o_yz[RZ], o_xz[RZ], o_xy[RZ];
-        float cM[RZ], cmu[RZ], cxy[RZ], cxz[RZ], cyz[RZ], dmp[RZ];
 #pragma unroll
         for (int r = 0; r < RZ; r++) {
-            const int z = q.z0 + wz + r * BLOCK_Y;
-            const long long on = L.at(x, y + H, z);
-            qvx[2 * H][r] = vx[on];
-            qvy[2 * H][r] = vy[on];
-            qvz[2 * H][r] = vz[on];
-            const long long o = L.at(x, y, z);
-            o_xx[r] = sxx[o];
-            o_yy[r] = syy[o];
-            o_zz[r] = szz[o];
-            o_yz[r] = syz[o];
-            o_xz[r] = sxz[o];
-            o_xy[r] = sxy[o];
-            cM[r] = C.M[o];
-            cmu[r] = C.mu[o];
-            cxy[r] = C.muxy[o];
-            cxz[r] = C.muxz[o];
-            cyz[r] = C.muyz[o];
-            dmp[r] = C.damp ? C.damp[o] : 1.f;
-        }
-        RXZ rvx, rvy, rvz;
-        rvx.load(vx, L, q.x0, y, q.z0, tid);
-        rvy.load(vy, L, q.x0, y, q.z0, tid);
-        rvz.load(vz, L, q.x0, y, q.z0, tid);
-        const int yi = P.on ? P.yi[y] : -1;
-        float p9[RZ], p13[RZ], p15[RZ], p10[RZ], p12[RZ], p17[RZ], p11[RZ], p14[RZ], p16[RZ];
-#pragma unroll
-        for (int r = 0; r < RZ; r++) {
-            const int z = q.z0 + wz + r * BLOCK_Y;
-            p9[r] = p13[r] = p15[r] = p10[r] = p12[r] = p17[r] = p11[r] = p14[r] = p16[r] = 0.f;
-            if (z >= L.NZ) continue;
-            if (xi >= 0) {
-                const long long pi = (((long long)q.shot * L.NY + y) * L.NZ + z) * W2 + xi;
-                p9[r] = P.ps[9][pi];
-                p13[r] = P.ps[13][pi];
-                p15[r] = P.ps[15][pi];
-            }
-            if (yi >= 0 && xin) {
-                const long long pi = (((long long)q.shot * W2 + yi) * L.NZ + z) * L.NX + x;
-                p10[r] = P.ps[10][pi];
-                p12[r] = P.ps[12][pi];
-                p17[r] = P.ps[17][pi];
-            }
-            const int zi = P.on ? P.zi[z] : -1;
-            if (zi >= 0 && xin) {
-                const long long pi = (((long long)q.shot * L.NY + y) * W2 + zi) * L.NX + x;
-                p11[r] = P.ps[11][pi];
-                p14[r] = P.ps[14][pi];
-                p16[r] = P.ps[16][pi];
-            }
+            qvx[2 * H][r] = cur.nvx[r];
+            qvy[2 * H][r] = cur.nvy[r];
+            qvz[2 * H][r] = cur.nvz[r];
         }
         rvx.store(s_vx, tid);
         rvy.store(s_vy, tid);
         rvz.store(s_vz, tid);
-        __syncthreads();
+        if (tid < 2 * K + 5) s_yt[tid] = cur.yt;
+        block_sync_lds();
 
-        const float *cyb = T.yb + (long long)y * K;
-        const float *cyf = T.yf + (long long)y * K;
-        float ay = 0.f, by = 0.f, ayh = 0.f, byh = 0.f;
-        if (yi >= 0) {
-            ay = P.ay[y];
-            by = P.by[y];
-            ayh = P.ayh[y];
-            byh = P.byh[y];
-        }
-        const bool surface = C.fsH && y == 0;
+        // CPML memory variables: planes and tiles outside every frame take the body without them;
+        // in the other, lanes outside a frame read 0 and drop their stores (no branches, so the
+        // next plane's loads stay in flight through the plane's arithmetic)
+        const int yi = __builtin_amdgcn_readfirstlane(P.on ? __float_as_int(s_yt[2 * K]) : -1);
+        // (readfirstlane: a branch the compiler sees as uniform, so y and the plane resources stay scalar)
+        const bool fr = __builtin_amdgcn_readfirstlane((int)(tfr || yi >= 0)) != 0;
+        {
+            float p9[RZ], p13[RZ], p15[RZ], p10[RZ], p12[RZ], p17[RZ], p11[RZ], p14[RZ], p16[RZ];
+            unsigned pox[RZ], poy[RZ], poz[RZ];
+            int zir[RZ];
 #pragma unroll
-        for (int r = 0; r < RZ; r++) {
-            const int lz = wz + r * BLOCK_Y;
-            const int z = q.z0 + lz;
-            if (z >= L.NZ || !xin) continue;
-            const float *czb = T.zb + (long long)z * K;
-            const float *czf = T.zf + (long long)z * K;
-            const int zi = P.on ? P.zi[z] : -1;
-            float az = 0.f, bz = 0.f, azh = 0.f, bzh = 0.f;
-            if (zi >= 0) {
-                az = P.az[z];
-                bz = P.bz[z];
-                azh = P.azh[z];
-                bzh = P.bzh[z];
+            for (int r = 0; r < RZ; r++) {
+                const int z = q.z0 + wz + r * BLOCK_Y;
+                const bool in = z < L.NZ && xin;
+                zir[r] = s_zt.zi[wz + r * BLOCK_Y];
+                p9[r] = p13[r] = p15[r] = p10[r] = p12[r] = p17[r] = p11[r] = p14[r] = p16[r] = 0.f;
+                if (fr) {
+                    pox[r] = psi_ox(L, W2, xi, y, z, in);
+                    poy[r] = psi_oy(L, yi, x, z, in);
+                    poz[r] = psi_oz(L, W2, zir[r], x, y, in);
+                    p9[r] = bload(pr.x[0], pox[r], 0);
+                    p13[r] = bload(pr.x[1], pox[r], 0);
+                    p15[r] = bload(pr.x[2], pox[r], 0);
+                    p10[r] = bload(pr.y[0], poy[r], 0);
+                    p12[r] = bload(pr.y[1], poy[r], 0);
+                    p17[r] = bload(pr.y[2], poy[r], 0);
+                    p11[r] = bload(pr.z[0], poz[r], 0);
+                    p14[r] = bload(pr.z[1], poz[r], 0);
+                    p16[r] = bload(pr.z[2], poz[r], 0);
+                }
             }
-            float vxx = ddx<H>(s_vx, lz + H, tx, cxb);
-            float vyy = ddy<H, RZ>(qvy, r, cyb);
-            float vzz = ddz<H>(s_vz, lz, tx + H, czb);
-            float vxy = ddy<H, RZ>(qvx, r, cyf);
-            float vyx = ddx<H>(s_vy, lz + H, tx, cxf);
-            float vxz = ddz<H>(s_vx, lz, tx + H, czf);
-            float vzx = ddx<H>(s_vz, lz + H, tx, cxf);
-            float vyz = ddz<H>(s_vy, lz, tx + H, czf);
-            float vzy = ddy<H, RZ>(qvz, r, cyf);
-            if (xi >= 0) {
-                const long long pi = (((long long)q.shot * L.NY + y) * L.NZ + z) * W2 + xi;
-                P.ps[9][pi] = cpml(vxx, p9[r], ax, bx);
-                P.ps[13][pi] = cpml(vyx, p13[r], axh, bxh);
-                P.ps[15][pi] = cpml(vzx, p15[r], axh, bxh);
+            // (also after the last plane: plane ye and its queue plane ye + H lie inside the padded
+            // volume, and a branch here would make the compiler wait for every load at the join)
+            PlaneIn nxt;
+            load_plane(nxt, rvx, rvy, rvz, y + 1);
+
+            const float *cyb = s_yt, *cyf = s_yt + K;
+            const bool yf = yi >= 0;
+            const float ay = yf ? s_yt[2 * K + 1] : 0.f, by = yf ? s_yt[2 * K + 2] : 0.f;
+            const float ayh = yf ? s_yt[2 * K + 3] : 0.f, byh = yf ? s_yt[2 * K + 4] : 0.f;
+            const bool surface = C.fsH && y == 0;
+#pragma unroll
+            for (int r = 0; r < RZ; r++) {
+                const int lz = wz + r * BLOCK_Y;
+                const int z = q.z0 + lz;
+                if (z >= L.NZ || !xin) continue;
+                const float *czb = s_zt.t[0][lz], *czf = s_zt.t[1][lz];
+                float vxx = ddx<H>(s_vx, lz + H, tx, cxb);
+                float vyy = ddy<H, RZ>(qvy, r, cyb);
+                float vzz = ddz<H>(s_vz, lz, tx + H, czb);
+                float vxy = ddy<H, RZ>(qvx, r, cyf);
+                float vyx = ddx<H>(s_vy, lz + H, tx, cxf);
+                float vxz = ddz<H>(s_vx, lz, tx + H, czf);
+                float vzx = ddx<H>(s_vz, lz + H, tx, cxf);
+                float vyz = ddz<H>(s_vy, lz, tx + H, czf);
+                float vzy = ddy<H, RZ>(qvz, r, cyf);
+                if (fr) {
+                    const float az = s_zt.c[lz][0], bz = s_zt.c[lz][1], azh = s_zt.c[lz][2], bzh = s_zt.c[lz][3];
+                    // outside a frame a = b = 0 and psi = 0: the derivative is unchanged
+                    bstore(cpml(vxx, p9[r], ax, bx), pr.x[0], pox[r], 0);
+                    bstore(cpml(vyx, p13[r], axh, bxh), pr.x[1], pox[r], 0);
+                    bstore(cpml(vzx, p15[r], axh, bxh), pr.x[2], pox[r], 0);
+                    bstore(cpml(vyy, p10[r], ay, by), pr.y[0], poy[r], 0);
+                    bstore(cpml(vxy, p12[r], ayh, byh), pr.y[1], poy[r], 0);
+                    bstore(cpml(vzy, p17[r], ayh, byh), pr.y[2], poy[r], 0);
+                    bstore(cpml(vzz, p11[r], az, bz), pr.z[0], poz[r], 0);
+                    bstore(cpml(vxz, p14[r], azh, bzh), pr.z[1], poz[r], 0);
+                    bstore(cpml(vyz, p16[r], azh, bzh), pr.z[2], poz[r], 0);
+                }
+                // normal stresses (ForwardSolver3Delastic.cpp: M (vxx+vyy+vzz) - 2 mu (...))
+                float s = vxx + vyy;
+                s += vzz;
+                s *= cur.c[0][r];
+                float nxx = cur.o[0][r] + s, nyy = cur.o[1][r] + s, nzz = cur.o[2][r] + s;
+                float m = vyy + vzz;
+                m *= cur.c[1][r];
+                nxx -= 2.f * m;
+                m = vxx + vzz;
+                m *= cur.c[1][r];
+                nyy -= 2.f * m;
+                m = vxx + vyy;
+                m *= cur.c[1][r];
+                nzz -= 2.f * m;
+                float sh = vxy + vyx;
+                sh *= cur.c[2][r];
+                const float nxy = cur.o[5][r] + sh;
+                sh = vxz + vzx;
+                sh *= cur.c[3][r];
+                const float nxz = cur.o[4][r] + sh;
+                sh = vyz + vzy;
+                sh *= cur.c[4][r];
+                const float nyz = cur.o[3][r] + sh;
+                if (surface) {
+                    // FreeSurface3Delastic: Syy = 0 at the surface, then the horizontal update exchange
+                    const long long fo = (long long)(z + L.H) * L.pitch + x + XPAD;
+                    nyy *= 0.f;
+                    float hs = vxx + vzz;
+                    const float a = C.fsH[fo] * hs;
+                    nxx += a;
+                    nzz += a;
+                    const float b = C.fsV[fo] * vyy;
+                    nxx -= b;
+                    nzz -= b;
+                }
+                const float d = cur.c[5][r];
+                bstore(nxx * d, plane_rsrc(sxx, L, y), oc[r], 0);
+                bstore(nyy * d, plane_rsrc(syy, L, y), oc[r], 0);
+                bstore(nzz * d, plane_rsrc(szz, L, y), oc[r], 0);
+                bstore(nxy * d, plane_rsrc(sxy, L, y), oc[r], 0);
+                bstore(nxz * d, plane_rsrc(sxz, L, y), oc[r], 0);
+                bstore(nyz * d, plane_rsrc(syz, L, y), oc[r], 0);
             }
-            if (yi >= 0) {
-                const long long pi = (((long long)q.shot * W2 + yi) * L.NZ + z) * L.NX + x;
-                P.ps[10][pi] = cpml(vyy, p10[r], ay, by);
-                P.ps[12][pi] = cpml(vxy, p12[r], ayh, byh);
-                P.ps[17][pi] = cpml(vzy, p17[r], ayh, byh);
-            }
-            if (zi >= 0) {
-                const long long pi = (((long long)q.shot * L.NY + y) * W2 + zi) * L.NX + x;
-                P.ps[11][pi] = cpml(vzz, p11[r], az, bz);
-                P.ps[14][pi] = cpml(vxz, p14[r], azh, bzh);
-                P.ps[16][pi] = cpml(vyz, p16[r], azh, bzh);
-            }
-            // normal stresses (ForwardSolver3Delastic.cpp: M (vxx+vyy+vzz) - 2 mu (...))
-            float s = vxx + vyy;
-            s += vzz;
-            s *= cM[r];
-            float nxx = o_xx[r] + s, nyy = o_yy[r] + s, nzz = o_zz[r] + s;
-            float m = vyy + vzz;
-            m *= cmu[r];
-            nxx -= 2.f * m;
-            m = vxx + vzz;
-            m *= cmu[r];
-            nyy -= 2.f * m;
-            m = vxx + vyy;
-            m *= cmu[r];
-            nzz -= 2.f * m;
-            float sh = vxy + vyx;
-            sh *= cxy[r];
-            const float nxy = o_xy[r] + sh;
-            sh = vxz + vzx;
-            sh *= cxz[r];
-            const float nxz = o_xz[r] + sh;
-            sh = vyz + vzy;
-            sh *= cyz[r];
-            const float nyz = o_yz[r] + sh;
-            if (surface) {
-                // FreeSurface3Delastic: Syy = 0 at the surface, then the horizontal update exchange
-                const long long fo = (long long)(z + L.H) * L.pitch + x + XPAD;
-                nyy *= 0.f;
-                float hs = vxx + vzz;
-                const float a = C.fsH[fo] * hs;
-                nxx += a;
-                nzz += a;
-                const float b = C.fsV[fo] * vyy;
-                nxx -= b;
-                nzz -= b;
-            }
-            const long long o = L.at(x, y, z);
-            const float d = dmp[r];
-            sxx[o] = nxx * d;
-            syy[o] = nyy * d;
-            szz[o] = nzz * d;
-            sxy[o] = nxy * d;
-            sxz[o] = nxz * d;
-            syz[o] = nyz * d;
+            cur = nxt;
         }
-        __syncthreads();
+        block_sync_lds();  // the next plane rewrites the LDS regions
     }
     acquisition3d(A, F, so, q.sb, A.tbase[0] + tofs, false);
 }

@@ -41,6 +41,11 @@ struct Coef3D {
 
 struct Taps3D {
     const float *xf, *xb, *yf, *yb, *yf_v, *yb_v, *zf, *zb;  // [pos][2H+1], see Taps
+    // per y position, everything a plane of a kernel needs along y, [pos][2K + 5] (K = 2H+1):
+    // K taps of the backward operator, K of the forward one, the CPML y frame index (int
+    // bits, -1 outside), a_y, b_y, a_y(half), b_y(half) (0 outside the frame); ys: stress
+    // update (Dyb, Dyf), yv: velocity update (its operators, image ones under a free surface)
+    const float *ys, *yv;
 };
 
 // CPML coefficients per axis and compact frame indices (as Cpml2D).  Memory
@@ -117,6 +122,7 @@ class Elastic3D {
     float *coef[8] = {};
     float *damp = nullptr, *fsH = nullptr, *fsV = nullptr;
     float *tap[8] = {};
+    float *ytab[2] = {};  // Taps3D::ys, Taps3D::yv
     float *cp[3][4] = {};
     int *idx[3] = {};
     float *ps[18] = {};

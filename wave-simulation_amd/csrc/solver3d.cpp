@@ -1,6 +1,7 @@
 // Host driver of the 3D elastic solver: layout, coefficient preparation
 // (Modelparameter.cpp:336-639 rules, FP64 then one rounding), acquisition
 // lists, graph-captured run segments.
+#include <cstring>
 #include <algorithm>
 #include <cmath>
 #include <string>
@@ -165,6 +166,43 @@ int Elastic3D::init(const wv_config &k, hipStream_t stream)
         static const int axis_of[18] = {0, 1, 2, 0, 1, 2, 0, 1, 2, 0, 1, 2, 1, 0, 2, 0, 2, 1};
         for (int q = 0; q < 18; q++)
             if ((rc = dalloc(&ps[q], ps_n[axis_of[q]]))) return rc;
+    }
+    {
+        // per-plane y tables of the two kernels: taps, CPML y index and coefficients (see Taps3D)
+        std::vector<float> ay(NYp, 0.f), by(NYp, 0.f), ayh(NYp, 0.f), byh(NYp, 0.f);
+        std::vector<int> yi(NYp, -1);
+        if (damping == 2) {
+            std::vector<float> a, b, ah, bh;
+            std::vector<int> id;
+            cpml_axis(NY, NYp, W, free_surface != 0, k.npower, k.fc_cpml, k.vmax_cpml, DT, DH, a, b, ah, bh, id);
+            for (int y = 0; y < NYp; y++) {
+                yi[y] = id[y];
+                if (id[y] >= 0) {
+                    ay[y] = a[y];
+                    by[y] = b[y];
+                    ayh[y] = ah[y];
+                    byh[y] = bh[y];
+                }
+            }
+        }
+        const int E = 2 * K + 5;
+        for (int v = 0; v < 2; v++) {
+            const std::vector<float> &tb = v ? t[5] : t[3], &tf = v ? t[4] : t[2];
+            std::vector<float> tab((size_t)NYp * E, 0.f);
+            for (int y = 0; y < NYp; y++) {
+                float *e = &tab[(size_t)y * E];
+                for (int j = 0; j < K; j++) {
+                    e[j] = tb[(size_t)y * K + j];
+                    e[K + j] = tf[(size_t)y * K + j];
+                }
+                std::memcpy(&e[2 * K], &yi[y], sizeof(int));
+                e[2 * K + 1] = ay[y];
+                e[2 * K + 2] = by[y];
+                e[2 * K + 3] = ayh[y];
+                e[2 * K + 4] = byh[y];
+            }
+            if ((rc = upload_pos(&ytab[v], tab, E, 0.f))) return rc;
+        }
     }
     if (damping == 1) {
         // ABS3D.cpp:150-215
@@ -419,6 +457,8 @@ Taps3D Elastic3D::taps() const
     t.yb_v = tap[5];
     t.zf = tap[6];
     t.zb = tap[7];
+    t.ys = ytab[0];
+    t.yv = ytab[1];
     return t;
 }
 

@@ -2,6 +2,7 @@
 // LDS tiles with an H-wide halo, derivative taps, CPML memory update.
 #pragma once
 
+#include "buffer.hpp"
 #include "wave_internal.hpp"
 
 namespace wv {
@@ -86,24 +87,6 @@ struct RowTile {
         }
     }
 };
-
-// Raw buffer access: a 128-bit resource (base, size in bytes) in scalar
-// registers, a per-lane 32-bit byte offset and a wavefront-uniform 32-bit byte
-// offset (soffset).  Rows of a plane differ only in soffset, so a load or store
-// needs no vector address arithmetic.  Word 3 = 0x00020000 is the gfx9 raw
-// 32-bit-element resource format.
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t brsrc(const void *p, unsigned bytes)
-{
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, 0x00020000);
-}
-__device__ __forceinline__ float bload(__amdgpu_buffer_rsrc_t r, unsigned voff, int soff)
-{
-    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, soff, 0));
-}
-__device__ __forceinline__ void bstore(float v, __amdgpu_buffer_rsrc_t r, unsigned voff, int soff)
-{
-    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)voff, soff, 0);
-}
 
 // LDS written by a wavefront and read back by the same wavefront: orders the
 // accesses without a workgroup barrier.
