@@ -44,6 +44,8 @@ def parse():
                     help="time steps of the CPU baseline sample (about 10-15 s on 16 cores)")
     ap.add_argument("--tile-y", type=int, default=0, help="rows per tile (0 = autotune over all kernel variants)")
     ap.add_argument("--path", default="fused", choices=["fused", "phased", "strip"])
+    ap.add_argument("--line", choices=["tmem", "adjoint", "3d"],
+                    help="run only this extra line and print it as JSON (for profiling; no headline metric)")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the batched TM-EM (64 shots), adjoint (16 shots) and 3D elastic lines")
     return ap.parse_args()
@@ -240,6 +242,12 @@ def main():
 
     import wavesim
     from wavesim import cases
+
+    if args.line:
+        L = wavesim.load()
+        one = {"tmem": bench_batched_tmem, "adjoint": bench_adjoint, "3d": lambda: bench_3d(wavesim, L)}
+        print(json.dumps({args.line: one[args.line]()}))
+        return
 
     case = cases.elastic_c2()
     NT = case.settings["NT"]

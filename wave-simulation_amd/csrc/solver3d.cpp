@@ -13,6 +13,12 @@
 
 namespace wv {
 
+// Blocks per launch the y chunking aims at: about 48 per CU, so the last round of blocks
+// leaves little of the GPU idle (measured on 320^3 x 8 shots: 2048 -> 10.9 ms per step,
+// 12288 -> 10.1 ms; each chunk re-reads 2H planes of y halo).
+constexpr int kBlocks3D = 12288;
+
+
 namespace {
 constexpr int ORG = 64;  // margin of position-indexed tables (see abi.cpp)
 int round_up(int v, int m) { return (v + m - 1) / m * m; }
@@ -231,7 +237,7 @@ int Elastic3D::init(const wv_config &k, hipStream_t stream)
     }
     // chunks along y: enough blocks to fill the chip, chunks of at least 4H planes
     const int nxz = ((NX + TILE_X - 1) / TILE_X) * ((NZ + 4 * rz - 1) / (4 * rz)) * nshots;
-    nyc = std::max(1, std::min((2048 + nxz - 1) / nxz, NY / std::max(4 * H, 8)));
+    nyc = std::max(1, std::min((kBlocks3D + nxz - 1) / nxz, NY / std::max(4 * H, 8)));
     HIP3(hipEventCreate(&ev0));
     HIP3(hipEventCreate(&ev1));
     shots.assign(nshots, Shot());
@@ -243,7 +249,7 @@ int Elastic3D::set_rz(int r)
 {
     rz = r;
     const int nxz = ((NX + TILE_X - 1) / TILE_X) * ((NZ + 4 * rz - 1) / (4 * rz)) * nshots;
-    nyc = std::max(1, std::min((2048 + nxz - 1) / nxz, NY / std::max(4 * H, 8)));
+    nyc = std::max(1, std::min((kBlocks3D + nxz - 1) / nxz, NY / std::max(4 * H, 8)));
     acq_dirty = true;
     drop_graph();
     return 0;
