@@ -116,19 +116,48 @@ __device__ __forceinline__ float ydiff(const float (&w)[N], int i, const float *
     return s;
 }
 
-// the y taps of this wavefront's S rows (two tables) into its LDS slice
+// The y taps of this wavefront's S rows (two tables) and the rows' CPML y
+// coefficients (a_y, b_y, a_y(half), b_y(half); 0 outside the frame), loaded
+// into registers before the field loads and written to the wavefront's LDS
+// slice after them, so waiting for them does not wait for the fields.
 template <int H, int S>
-__device__ __forceinline__ void stage_ytaps(float (*dst)[S * (2 * H + 1)], const float *t0, const float *t1, int ya,
-                                            int lane)
-{
-    constexpr int M = S * (2 * H + 1);
-    const float *__restrict__ a = t0 + (long long)ya * (2 * H + 1);
-    const float *__restrict__ b = t1 + (long long)ya * (2 * H + 1);
-    for (int e = lane; e < M; e += TILE_X) {
-        dst[0][e] = a[e];
-        dst[1][e] = b[e];
+struct YStage {
+    static constexpr int K = 2 * H + 1, M = S * K, IT = (M + TILE_X - 1) / TILE_X;
+    float a[IT], b[IT], cy[4];
+    __device__ __forceinline__ void load(const float *t0, const float *t1, const Cpml2D &P, int ya, int lane)
+    {
+#pragma unroll
+        for (int it = 0; it < IT; it++) {
+            const int e = min(lane + it * TILE_X, M - 1);
+            a[it] = t0[(long long)ya * K + e];
+            b[it] = t1[(long long)ya * K + e];
+        }
+        const int y = ya + min(lane, S - 1);
+        cy[0] = cy[1] = cy[2] = cy[3] = 0.f;
+        if (P.on) {
+            cy[0] = P.ay[y];
+            cy[1] = P.by[y];
+            cy[2] = P.ayh[y];
+            cy[3] = P.byh[y];
+        }
     }
-}
+    // taps to dst[2][M]; coefficients of row r to scy[r][o .. o+3]
+    template <int NC>
+    __device__ __forceinline__ void store(float (*dst)[M], float (*scy)[NC], int o, int lane) const
+    {
+#pragma unroll
+        for (int it = 0; it < IT; it++) {
+            const int e = lane + it * TILE_X;
+            if (M % TILE_X == 0 || e < M) {
+                dst[0][e] = a[it];
+                dst[1][e] = b[it];
+            }
+        }
+        if (lane < S)
+#pragma unroll
+            for (int j = 0; j < 4; j++) scy[lane][o + j] = cy[j];
+    }
+};
 
 // velocity update of one row (ForwardSolver2Delastic.cpp:117-166):
 // vx += 1/rho_x * (CPML(Dxf sxx) + CPML(Dyb sxy)); vy += 1/rho_y * (CPML(Dxb sxy) + CPML(Dyf syy))
@@ -141,6 +170,7 @@ __global__ __launch_bounds__(NTHREADS) void el2d_velocity_strip(Layout2D L, Fiel
     __shared__ float s_xx[BLOCK_Y][U][RW];
     __shared__ float s_xy[BLOCK_Y][U][RW];
     __shared__ float s_ty[BLOCK_Y][2][S * K];
+    __shared__ float s_cy[BLOCK_Y][S][4];  // per row: a_y, b_y, a_y(half), b_y(half) (0 outside the y frame)
     const Blk B = xcd_block();
     const int wy = __builtin_amdgcn_readfirstlane(threadIdx.y);
     const Col col = column(L, P, B.bx * TILE_X + (int)threadIdx.x);
@@ -159,6 +189,8 @@ __global__ __launch_bounds__(NTHREADS) void el2d_velocity_strip(Layout2D L, Fiel
     const int xi = col.xi;
     const int hc = lane < H ? lane : TILE_X + lane;
 
+    YStage<H, S> ys;
+    ys.load(T.yb_v, T.yf_v, P, c.ya, lane);
     // column windows of the y-differentiated fields (rows ya-H .. ya+S+H-1) and per-row values
     float wxy[N], wyy[N];
     float axx[S], hxx[S], hxy[S], ovx[S], ovy[S], crx[S], cry[S], p0[S], p1[S], q0[S], q1[S];
@@ -184,7 +216,7 @@ __global__ __launch_bounds__(NTHREADS) void el2d_velocity_strip(Layout2D L, Fiel
                     p0[r] = px0[y * c.W2 + xi];
                     p1[r] = px1[y * c.W2 + xi];
                 }
-                const int yi = P.on ? P.yi[y] : -1;
+                const int yi = P.on ? frame_index(y, L.NY, P.W, C.free_surface) : -1;
                 if (yi >= 0 && col.in) {
                     q0[r] = py0[yi * L.NX + c.x0 + lane];
                     q1[r] = py1[yi * L.NX + c.x0 + lane];
@@ -202,7 +234,7 @@ __global__ __launch_bounds__(NTHREADS) void el2d_velocity_strip(Layout2D L, Fiel
     float cxf[K], cxb[K];
     load_taps<H>(cxf, T.xf, col.x);
     load_taps<H>(cxb, T.xb, col.x);
-    stage_ytaps<H, S>(s_ty[wy], T.yb_v, T.yf_v, c.ya, lane);
+    ys.store(s_ty[wy], s_cy[wy], 0, lane);
     float rh0[S], rh1[S];
 #pragma unroll
     for (int r = 0; r < S; r++) rh0[r] = rh1[r] = 0.f;
@@ -235,10 +267,11 @@ __global__ __launch_bounds__(NTHREADS) void el2d_velocity_strip(Layout2D L, Fiel
                 px0[y * c.W2 + xi] = cpml(u, p0[r], col.axh, col.bxh);
                 px1[y * c.W2 + xi] = cpml(u2, p1[r], col.ax, col.bx);
             }
-            const int yi = P.on ? P.yi[y] : -1;
+            const int yi = P.on ? frame_index(y, L.NY, P.W, C.free_surface) : -1;
             if (yi >= 0) {
-                py0[yi * L.NX + c.x0 + lane] = cpml(tt, q0[r], P.ay[y], P.by[y]);
-                py1[yi * L.NX + c.x0 + lane] = cpml(tt2, q1[r], P.ayh[y], P.byh[y]);
+                const float *cy = s_cy[wy][r];
+                py0[yi * L.NX + c.x0 + lane] = cpml(tt, q0[r], cy[0], cy[1]);
+                py1[yi * L.NX + c.x0 + lane] = cpml(tt2, q1[r], cy[2], cy[3]);
             }
             u += tt;
             u2 += tt2;
@@ -292,6 +325,8 @@ __global__ __launch_bounds__(NTHREADS) void el2d_stress_strip(Layout2D L, Fields
     const int xi = col.xi;
     const int hc = lane < H ? lane : TILE_X + lane;
 
+    YStage<H, S> ys;
+    ys.load(T.yb, T.yf, P, c.ya, lane);
     float wx[N], wyv[N];
     float hx[S], hy[S], oxx[S], oyy[S], oxy[S], cM[S], cmu[S], cmxy[S], dmp[S], p2[S], p3[S], q2[S], q3[S];
     auto load_group = [&](int g) {
@@ -319,7 +354,7 @@ __global__ __launch_bounds__(NTHREADS) void el2d_stress_strip(Layout2D L, Fields
                     p2[r] = px2[y * c.W2 + xi];
                     p3[r] = px3[y * c.W2 + xi];
                 }
-                const int yi = P.on ? P.yi[y] : -1;
+                const int yi = P.on ? frame_index(y, L.NY, P.W, C.free_surface) : -1;
                 if (yi >= 0 && col.in) {
                     q2[r] = py2[yi * L.NX + c.x0 + lane];
                     q3[r] = py3[yi * L.NX + c.x0 + lane];
@@ -337,16 +372,9 @@ __global__ __launch_bounds__(NTHREADS) void el2d_stress_strip(Layout2D L, Fields
     float cxf[K], cxb[K];
     load_taps<H>(cxf, T.xf, col.x);
     load_taps<H>(cxb, T.xb, col.x);
-    stage_ytaps<H, S>(s_ty[wy], T.yb, T.yf, c.ya, lane);
-    if (lane < S) {
-        const int y = c.ya + lane;
-        const int yi = (P.on && y < L.NY) ? P.yi[y] : -1;
-        s_cy[wy][lane][0] = (float)yi;
-        s_cy[wy][lane][1] = yi >= 0 ? P.ay[y] : 0.f;
-        s_cy[wy][lane][2] = yi >= 0 ? P.by[y] : 0.f;
-        s_cy[wy][lane][3] = yi >= 0 ? P.ayh[y] : 0.f;
-        s_cy[wy][lane][4] = yi >= 0 ? P.byh[y] : 0.f;
-    }
+    ys.store(s_ty[wy], s_cy[wy], 1, lane);
+    if (lane < S)
+        s_cy[wy][lane][0] = (float)(P.on ? frame_index(c.ya + lane, L.NY, P.W, C.free_surface) : -1);
     const bool fs = C.free_surface != 0;
     float rh2[S], rh3[S], rh4[S];
 #pragma unroll

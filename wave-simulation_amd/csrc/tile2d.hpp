@@ -132,6 +132,17 @@ struct Col {
     float ax, bx, axh, bxh;
 };
 
+// Compact CPML frame index of position p on an axis of n positions (the rule the
+// host tables follow): near side 0..W-1, far side W..2W-1, -1 outside the frame
+// or on a skipped near side (the top under a free surface).  Computed, not
+// loaded, so no load address waits for it.
+__device__ __forceinline__ int frame_index(int p, int n, int W, bool skip_near)
+{
+    if (p < 0 || p >= n) return -1;
+    if (p < W) return skip_near ? -1 : p;
+    return n - 1 - p < W ? 2 * W - n + p : -1;
+}
+
 __device__ __forceinline__ Col column(const Layout2D &L, const Cpml2D &P, int x)
 {
     Col c;
@@ -140,13 +151,12 @@ __device__ __forceinline__ Col column(const Layout2D &L, const Cpml2D &P, int x)
     c.xi = -1;
     c.ax = c.bx = c.axh = c.bxh = 0.f;
     if (P.on) {
-        c.xi = c.in ? P.xi[x] : -1;
-        if (c.xi >= 0) {
-            c.ax = P.ax[x];
-            c.bx = P.bx[x];
-            c.axh = P.axh[x];
-            c.bxh = P.bxh[x];
-        }
+        // the coefficient tables are 0 outside the frame and padded beyond the grid
+        c.xi = frame_index(x, L.NX, P.W, false);
+        c.ax = P.ax[x];
+        c.bx = P.bx[x];
+        c.axh = P.axh[x];
+        c.bxh = P.bxh[x];
     }
     return c;
 }
