@@ -379,11 +379,19 @@ __global__ __launch_bounds__(NTHREADS) void tm2d_h(Layout2D L, Fields2D F, Coef2
     using TL = Tile<H, TY>;
     constexpr int R = TY / BLOCK_Y;
     __shared__ float s_ez[TL::RH][TL::RW];
+    __shared__ float s_ty[TY][2 * H + 1];  // Dyf taps of the tile's rows
+    __shared__ float s_cy[TY][2];          // a_y(half), b_y(half) of the tile's rows (0 outside the frame)
     const Blk B = xcd_block();
     const int shot = B.shot;
     const long long so = (long long)shot * L.plane;
     const int x0 = B.bx * TILE_X, y0 = B.by * TY;
     const int tid = threadIdx.y * TILE_X + threadIdx.x, tx = threadIdx.x;
+    // per-row tables first, so waiting for them does not wait for the field loads
+    constexpr int NTY = TY * (2 * H + 1), ITY = (NTY + NTHREADS - 1) / NTHREADS;
+    float tyv[ITY], cyv = 0.f;
+#pragma unroll
+    for (int k = 0; k < ITY; k++) tyv[k] = T.yf[(long long)y0 * (2 * H + 1) + min(tid + k * NTHREADS, NTY - 1)];
+    if (tid < 2 * TY && P.on) cyv = (tid & 1 ? P.byh : P.ayh)[y0 + (tid >> 1)];
     TileRegs<H, TY> rez;
     rez.load(F.f[2] + so, L, x0, y0, tid);
     float *__restrict__ hx = F.f[0] + so;
@@ -404,23 +412,26 @@ __global__ __launch_bounds__(NTHREADS) void tm2d_h(Layout2D L, Fields2D F, Coef2
         cxz[r] = C.c[2][o];
         cyz[r] = C.c[3][o];
         p0[r] = q0[r] = 0.f;
-        yi[r] = P.on ? P.yi[y] : -1;
+        yi[r] = P.on ? frame_index(y, L.NY, P.W, C.free_surface) : -1;
         if (col.xi >= 0 && y < L.NY) p0[r] = P.psx[0][((long long)shot * L.NY + y) * W2 + col.xi];
         if (yi[r] >= 0 && col.in) q0[r] = P.psy[0][((long long)shot * W2 + yi[r]) * L.NX + col.x];
     }
     rez.store(s_ez, tid);
+#pragma unroll
+    for (int k = 0; k < ITY; k++)
+        if (tid + k * NTHREADS < NTY) (&s_ty[0][0])[tid + k * NTHREADS] = tyv[k];
+    if (tid < 2 * TY) (&s_cy[0][0])[tid] = cyv;
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < R; r++) {
         const int ly = wy + r * BLOCK_Y;
         const int y = y0 + ly;
         if (y >= L.NY) break;
-        const float *cyf = T.yf + (long long)y * (2 * H + 1);
         const int Rr = ly + H;
-        float u = dy<H>(s_ez, Rr, tx, cyf);
+        float u = dy<H>(s_ez, Rr, tx, s_ty[ly]);
         float u2 = dx<H>(s_ez, Rr, tx, cxf);
         if (!col.in) continue;
-        if (yi[r] >= 0) P.psy[0][((long long)shot * W2 + yi[r]) * L.NX + col.x] = cpml(u, q0[r], P.ayh[y], P.byh[y]);
+        if (yi[r] >= 0) P.psy[0][((long long)shot * W2 + yi[r]) * L.NX + col.x] = cpml(u, q0[r], s_cy[ly][0], s_cy[ly][1]);
         if (col.xi >= 0) P.psx[0][((long long)shot * L.NY + y) * W2 + col.xi] = cpml(u2, p0[r], col.axh, col.bxh);
         // hx -= 1/mu_yz * CPML(Dyf ez) ; hy -= 1/mu_xz * (-CPML(Dxf ez))
         const long long o = L.at(col.x, y);
@@ -441,11 +452,19 @@ __global__ __launch_bounds__(NTHREADS) void tm2d_e(Layout2D L, Fields2D F, Coef2
     constexpr int R = TY / BLOCK_Y;
     __shared__ float s_hx[TL::RH][TL::RW];
     __shared__ float s_hy[TL::RH][TL::RW];
+    __shared__ float s_ty[TY][2 * H + 1];  // Dyb taps of the tile's rows
+    __shared__ float s_cy[TY][2];          // a_y, b_y of the tile's rows (0 outside the frame)
     const Blk B = xcd_block();
     const int shot = B.shot;
     const long long so = (long long)shot * L.plane;
     const int x0 = B.bx * TILE_X, y0 = B.by * TY;
     const int tid = threadIdx.y * TILE_X + threadIdx.x, tx = threadIdx.x;
+    // per-row tables first, so waiting for them does not wait for the field loads
+    constexpr int NTY = TY * (2 * H + 1), ITY = (NTY + NTHREADS - 1) / NTHREADS;
+    float tyv[ITY], cyv = 0.f;
+#pragma unroll
+    for (int k = 0; k < ITY; k++) tyv[k] = T.yb[(long long)y0 * (2 * H + 1) + min(tid + k * NTHREADS, NTY - 1)];
+    if (tid < 2 * TY && P.on) cyv = (tid & 1 ? P.by : P.ay)[y0 + (tid >> 1)];
     TileRegs<H, TY> rhx, rhy;
     rhx.load(F.f[0] + so, L, x0, y0, tid);
     rhy.load(F.f[1] + so, L, x0, y0, tid);
@@ -466,25 +485,28 @@ __global__ __launch_bounds__(NTHREADS) void tm2d_e(Layout2D L, Fields2D F, Coef2
         cb[r] = C.c[1][o];
         dmp[r] = C.damp ? C.damp[o] : 1.f;
         p1[r] = q1[r] = 0.f;
-        yi[r] = P.on ? P.yi[y] : -1;
+        yi[r] = P.on ? frame_index(y, L.NY, P.W, C.free_surface) : -1;
         if (col.xi >= 0 && y < L.NY) p1[r] = P.psx[1][((long long)shot * L.NY + y) * W2 + col.xi];
         if (yi[r] >= 0 && col.in) q1[r] = P.psy[1][((long long)shot * W2 + yi[r]) * L.NX + col.x];
     }
     rhx.store(s_hx, tid);
     rhy.store(s_hy, tid);
+#pragma unroll
+    for (int k = 0; k < ITY; k++)
+        if (tid + k * NTHREADS < NTY) (&s_ty[0][0])[tid + k * NTHREADS] = tyv[k];
+    if (tid < 2 * TY) (&s_cy[0][0])[tid] = cyv;
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < R; r++) {
         const int ly = wy + r * BLOCK_Y;
         const int y = y0 + ly;
         if (y >= L.NY) break;
-        const float *cyb = T.yb + (long long)y * (2 * H + 1);
         const int Rr = ly + H;
         float u = dx<H>(s_hy, Rr, tx, cxb);
-        float t = dy<H>(s_hx, Rr, tx, cyb);
+        float t = dy<H>(s_hx, Rr, tx, s_ty[ly]);
         if (!col.in) continue;
         if (col.xi >= 0) P.psx[1][((long long)shot * L.NY + y) * W2 + col.xi] = cpml(u, p1[r], col.ax, col.bx);
-        if (yi[r] >= 0) P.psy[1][((long long)shot * W2 + yi[r]) * L.NX + col.x] = cpml(t, q1[r], P.ay[y], P.by[y]);
+        if (yi[r] >= 0) P.psy[1][((long long)shot * W2 + yi[r]) * L.NX + col.x] = cpml(t, q1[r], s_cy[ly][0], s_cy[ly][1]);
         u -= t;
         u *= cb[r];
         float e = ca[r] * oez[r] + u;
