@@ -111,6 +111,14 @@ def bench_batched_tmem():
     sol.run(0, NT)
     sol.synchronize()
     tb = time.perf_counter() - t0
+    import wavesim
+
+    L = wavesim.load()
+    wavesim._abi.check(L.wv_set_kernel_timing(sol._h, 1))
+    sol.reset()
+    sol.run(0, 300)
+    sol.synchronize()
+    kern = _kernel_stats(sol, L, wavesim)
     del sol
     plain = cases.make_solver(case, path="plain", shots=[case.shots[0]])
     ps = 300
@@ -125,7 +133,8 @@ def bench_batched_tmem():
             "kernel": choice, "shots_per_s": round(n / tb, 3), "plain_shots_per_s": round(1.0 / shot_plain, 4),
             "ratio_vs_plain": round(n / tb * shot_plain, 1),
             "grid_cell_updates_per_s": round(case.cells * NT * n / tb, 1),
-            "plain_sample": f"{ps} steps of one shot, scaled to {NT}"}
+            "plain_sample": f"{ps} steps of one shot, scaled to {NT}",
+            "roofline": _roofline([k for k in kern[:2] if k["launches"]])}
 
 
 def bench_adjoint():
