@@ -50,16 +50,28 @@ struct XFrame {
 __device__ __forceinline__ XFrame xframe(const Layout2D &L, const Cpml2D &P, int x)
 {
     XFrame f{-1, 0.f, 0.f, 0.f, 0.f};
-    if (P.on && x >= 0 && x < L.NX) {
-        f.xi = P.xi[x];
-        if (f.xi >= 0) {
-            f.a = P.ax[x];
-            f.ah = P.axh[x];
-            f.b = P.bx[x];
-            f.bh = P.bxh[x];
-        }
+    if (P.on) {
+        // the frame index is computed (no load for later loads to wait on); the coefficient
+        // tables are 0 outside the frame and padded around the grid
+        f.xi = frame_index(x, L.NX, P.W, false);
+        f.a = P.ax[x];
+        f.ah = P.axh[x];
+        f.b = P.bx[x];
+        f.bh = P.bxh[x];
     }
     return f;
+}
+
+// CPML y coefficients (a_y, a_y(half), b_y, b_y(half)) of the N window rows from y0, staged in LDS by
+// the block's first 4N threads before the block barrier
+template <int N>
+__device__ __forceinline__ void stage_ycpml(float (*dst)[4], const Cpml2D &P, int y0, int tid)
+{
+    if (P.on && tid < 4 * N) {
+        const int i = tid >> 2, j = tid & 3;
+        const float *t = j == 0 ? P.ay : j == 1 ? P.ayh : j == 2 ? P.by : P.byh;
+        dst[i][j] = t[y0 + i];
+    }
 }
 
 // Sum of per-cell FP64 gradient contributions of NF planes over every shot a
@@ -224,6 +236,7 @@ __global__ __launch_bounds__(TILE_X * W) void adj_el2d_stress_strip(Layout2D L, 
     __shared__ float s_g7[W][S][RW];
     __shared__ float s_ty[2][S * K];
     __shared__ __align__(16) float s_cf[3][RW][N];
+    __shared__ float s_cyw[N][4];  // CPML y coefficients of the window rows
     __shared__ GradSum<5, S, W> s_sum;
     const Blk B = xcd_block();
     const int wy = __builtin_amdgcn_readfirstlane(threadIdx.y), lane = threadIdx.x, tid = wy * TILE_X + lane;
@@ -240,6 +253,7 @@ __global__ __launch_bounds__(TILE_X * W) void adj_el2d_stress_strip(Layout2D L, 
         cw.load(cp, L, xl, ya - H, tid, pb);
     }
     stage_ytaps_block<S, K>(s_ty, TT.yf, TT.yb, ya, tid);
+    stage_ycpml<N>(s_cyw, P, ya - H, tid);
     const unsigned vc = (unsigned)(H + lane) * 4u;
     const bool inx = x < L.NX;
     const int xc = min(x, L.NX - 1);
@@ -257,7 +271,7 @@ __global__ __launch_bounds__(TILE_X * W) void adj_el2d_stress_strip(Layout2D L, 
 #pragma unroll
     for (int i = 0; i < N; i++) {
         const int y = ya - H + i;
-        yiw[i] = (fr && y >= 0 && y < L.NY) ? P.yi[y] : -1;
+        yiw[i] = fr ? frame_index(y, L.NY, P.W, C.free_surface) : -1;
     }
     float txf[K], txb[K];
     load_taps<H>(txf, TT.xf, x);
@@ -360,9 +374,8 @@ __global__ __launch_bounds__(TILE_X * W) void adj_el2d_stress_strip(Layout2D L, 
             const float h5 = cM[i] * ls - 2.f * cmu[i] * lxx[i], h6 = cmxy[i] * lxy[i];
             float g5 = h5, g6 = h6;
             if (yiw[i] >= 0 && inx) {
-                const int y = ya - H + i;
-                g5 = h5 + P.ay[y] * (my2[i] + h5);
-                g6 = h6 + P.ayh[y] * (my3[i] + h6);
+                g5 = h5 + s_cyw[i][0] * (my2[i] + h5);
+                g6 = h6 + s_cyw[i][1] * (my3[i] + h6);
             }
             w5[i] = g5;
             w6[i] = g6;
@@ -423,8 +436,8 @@ __global__ __launch_bounds__(TILE_X * W) void adj_el2d_stress_strip(Layout2D L, 
             }
             if (yiw[i] >= 0) {
                 const int ro = yiw[i] * L.NX * 4;
-                bstore(P.by[y] * (my2[i] + h5), brsrc(M.myn[2] + (long long)shot * W2 * L.NX, pyb), vxc, ro);
-                bstore(P.byh[y] * (my3[i] + h6), brsrc(M.myn[3] + (long long)shot * W2 * L.NX, pyb), vxc, ro);
+                bstore(s_cyw[i][2] * (my2[i] + h5), brsrc(M.myn[2] + (long long)shot * W2 * L.NX, pyb), vxc, ro);
+                bstore(s_cyw[i][3] * (my3[i] + h6), brsrc(M.myn[3] + (long long)shot * W2 * L.NX, pyb), vxc, ro);
             }
             // gradient terms with the forward step's recorded derivative sums: the velocity
             // update's (1/rho_x, 1/rho_y) take the adjoint velocities just formed, which the
@@ -453,6 +466,7 @@ __global__ __launch_bounds__(TILE_X * W) void adj_el2d_velocity_strip(Layout2D L
     __shared__ float s_g2[W][S][RW];
     __shared__ float s_ty[2][S * K];
     __shared__ __align__(16) float s_cf[2][RW][N];
+    __shared__ float s_cyw[N][4];  // CPML y coefficients of the window rows
     const Blk B = xcd_block();
     const int wy = __builtin_amdgcn_readfirstlane(threadIdx.y), lane = threadIdx.x, tid = wy * TILE_X + lane;
     const int x0 = B.bx * TILE_X, ya = B.by * S, xl = x0 - H, x = x0 + lane;
@@ -468,6 +482,7 @@ __global__ __launch_bounds__(TILE_X * W) void adj_el2d_velocity_strip(Layout2D L
         cw.load(cp, L, xl, ya - H, tid, pb);
     }
     stage_ytaps_block<S, K>(s_ty, TT.yb, TT.yf, ya, tid);
+    stage_ycpml<N>(s_cyw, P, ya - H, tid);
     const unsigned vc = (unsigned)(H + lane) * 4u;
     const bool inx = x < L.NX;
     const int xc = min(x, L.NX - 1);
@@ -485,7 +500,7 @@ __global__ __launch_bounds__(TILE_X * W) void adj_el2d_velocity_strip(Layout2D L
 #pragma unroll
     for (int i = 0; i < N; i++) {
         const int y = ya - H + i;
-        yiw[i] = (fr && y >= 0 && y < L.NY) ? P.yi[y] : -1;
+        yiw[i] = fr ? frame_index(y, L.NY, P.W, C.free_surface) : -1;
     }
     float txf[K], txb[K];
     load_taps<H>(txf, TT.xf, x);
@@ -573,9 +588,8 @@ __global__ __launch_bounds__(TILE_X * W) void adj_el2d_velocity_strip(Layout2D L
             const float h0 = cx[i] * vx[i], h2 = cy[i] * vy[i];
             float g1 = h0, g3 = h2;
             if (yiw[i] >= 0 && inx) {
-                const int y = ya - H + i;
-                g1 = h0 + P.ay[y] * (my0[i] + h0);
-                g3 = h2 + P.ayh[y] * (my1[i] + h2);
+                g1 = h0 + s_cyw[i][0] * (my0[i] + h0);
+                g3 = h2 + s_cyw[i][1] * (my1[i] + h2);
             }
             w1[i] = g1;
             w3[i] = g3;
@@ -627,8 +641,8 @@ __global__ __launch_bounds__(TILE_X * W) void adj_el2d_velocity_strip(Layout2D L
             }
             if (yiw[i] >= 0) {
                 const int ro = yiw[i] * L.NX * 4;
-                bstore(P.by[y] * (my0[i] + h0), brsrc(M.myn[0] + (long long)shot * W2 * L.NX, pyb), vxc, ro);
-                bstore(P.byh[y] * (my1[i] + h2), brsrc(M.myn[1] + (long long)shot * W2 * L.NX, pyb), vxc, ro);
+                bstore(s_cyw[i][2] * (my0[i] + h0), brsrc(M.myn[0] + (long long)shot * W2 * L.NX, pyb), vxc, ro);
+                bstore(s_cyw[i][3] * (my1[i] + h2), brsrc(M.myn[1] + (long long)shot * W2 * L.NX, pyb), vxc, ro);
             }
         }
         wave_sync();  // the next shot group rewrites this wavefront's LDS strips
