@@ -71,38 +71,53 @@ __device__ __forceinline__ StripCtx strip_ctx(const Layout2D &L, const Cpml2D &P
     return c;
 }
 
-// Record one field of the wavefront's strip (S rows x 64 columns) for the
-// adjoint pass (Rec2D): the maximum over the strip sets its FP16 scale; rows
-// are stored in pairs, one 32-bit word per lane and pair.
-template <int S>
-__device__ __forceinline__ void record_strip(const Rec2D &R, const Layout2D &L, int nshots, int t, int f, int shot,
-                                             int x0, int ya, int lane, const float (&h)[S])
+// Record NF fields (f0, f0+1, ...) of the wavefront's strip (S rows x 64
+// columns) for the adjoint pass (Rec2D): the maximum over the strip sets each
+// field's FP16 scale (the NF maxima are reduced together, so the shuffle steps
+// of the fields overlap); rows are stored in pairs, one 32-bit word per lane
+// and pair.
+template <int S, int NF>
+__device__ __forceinline__ void record_strips(const Rec2D &R, const Layout2D &L, int nshots, int t, int f0, int shot,
+                                              int x0, int ya, int lane, const float (&h)[NF][S])
 {
     static_assert(S % 2 == 0, "rows are recorded in pairs");
     const bool in = x0 + lane < L.NX;
-    float m = 0.f;
+    float m[NF];
 #pragma unroll
-    for (int r = 0; r < S; r++)
-        if (in && ya + r < L.NY) m = fmaxf(m, fabsf(h[r]));
+    for (int f = 0; f < NF; f++) {
+        m[f] = 0.f;
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
-    // strips whose values are all below 1e-30 are stored as zeros (scale 1); above it the
-    // scale stays a normal float, so h / scale is finite
-    const float s = m > 1e-30f ? m * (1.f / 32768.f) : 1.f, is = 1.f / s;
-    const long long fs = ((long long)t * 5 + f) * nshots + shot;
+        for (int r = 0; r < S; r++)
+            if (in && ya + r < L.NY) m[f] = fmaxf(m[f], fabsf(h[f][r]));
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        float o[NF];
+#pragma unroll
+        for (int f = 0; f < NF; f++) o[f] = __shfl_xor(m[f], off);
+#pragma unroll
+        for (int f = 0; f < NF; f++) m[f] = fmaxf(m[f], o[f]);
+    }
     if (ya >= L.NY) return;  // a strip wholly below the grid (last block row) records nothing
-    const long long q = (fs * R.nty + ya / S) * R.nbx + x0 / TILE_X;
-    if (lane == 0) R.scale[q] = s;
-    unsigned *w = R.h + q * (S / 2) * TILE_X + lane;
 #pragma unroll
-    for (int p = 0; p < S / 2; p++) {
-        // cells outside the grid store zeros
-        const float a = (in && ya + 2 * p < L.NY) ? h[2 * p] * is : 0.f;
-        const float b = (in && ya + 2 * p + 1 < L.NY) ? h[2 * p + 1] * is : 0.f;
-        // written once, read once much later: streamed past the caches
-        __builtin_nontemporal_store((unsigned)__half_as_ushort(__float2half_rn(a)) |
-                                        ((unsigned)__half_as_ushort(__float2half_rn(b)) << 16),
-                                    w + p * TILE_X);
+    for (int f = 0; f < NF; f++) {
+        // strips whose values are all below 1e-30 are stored as zeros (scale 1); above it the
+        // scale stays a normal float, so h / scale is finite
+        const float sc = m[f] > 1e-30f ? m[f] * (1.f / 32768.f) : 1.f, is = 1.f / sc;
+        const long long fs = ((long long)t * 5 + f0 + f) * nshots + shot;
+        const long long q = (fs * R.nty + ya / S) * R.nbx + x0 / TILE_X;
+        if (lane == 0) R.scale[q] = sc;
+        unsigned *w = R.h + q * (S / 2) * TILE_X + lane;
+#pragma unroll
+        for (int p = 0; p < S / 2; p++) {
+            // cells outside the grid store zeros
+            const float a = (in && ya + 2 * p < L.NY) ? h[f][2 * p] * is : 0.f;
+            const float b = (in && ya + 2 * p + 1 < L.NY) ? h[f][2 * p + 1] * is : 0.f;
+            // written once, read once much later: streamed past the caches
+            __builtin_nontemporal_store((unsigned)__half_as_ushort(__float2half_rn(a)) |
+                                            ((unsigned)__half_as_ushort(__float2half_rn(b)) << 16),
+                                        w + p * TILE_X);
+        }
     }
 }
 
@@ -235,9 +250,9 @@ __global__ __launch_bounds__(NTHREADS) void el2d_velocity_strip(Layout2D L, Fiel
     load_taps<H>(cxf, T.xf, col.x);
     load_taps<H>(cxb, T.xb, col.x);
     ys.store(s_ty[wy], s_cy[wy], 0, lane);
-    float rh0[S], rh1[S];
+    float rh[2][S];
 #pragma unroll
-    for (int r = 0; r < S; r++) rh0[r] = rh1[r] = 0.f;
+    for (int r = 0; r < S; r++) rh[0][r] = rh[1][r] = 0.f;
 
 #pragma unroll
     for (int g = 0; g < G; g++) {
@@ -275,8 +290,8 @@ __global__ __launch_bounds__(NTHREADS) void el2d_velocity_strip(Layout2D L, Fiel
             }
             u += tt;
             u2 += tt2;
-            rh0[r] = u;
-            rh1[r] = u2;
+            rh[0][r] = u;
+            rh[1][r] = u2;
             const int ro = boff(L, c.xl, y);
             if (Sn.h[0]) {
                 Sn.h[0][c.so + L.at(c.x0, y) + lane] = u;
@@ -291,8 +306,7 @@ __global__ __launch_bounds__(NTHREADS) void el2d_velocity_strip(Layout2D L, Fiel
     }
     if (Rc.h) {
         const int ts = A.tbase[0] + tofs;
-        record_strip<S>(Rc, L, A.nshots, ts, 0, c.shot, c.x0, c.ya, lane, rh0);
-        record_strip<S>(Rc, L, A.nshots, ts, 1, c.shot, c.x0, c.ya, lane, rh1);
+        record_strips<S, 2>(Rc, L, A.nshots, ts, 0, c.shot, c.x0, c.ya, lane, rh);
     }
 }
 
@@ -376,9 +390,9 @@ __global__ __launch_bounds__(NTHREADS) void el2d_stress_strip(Layout2D L, Fields
     if (lane < S)
         s_cy[wy][lane][0] = (float)(P.on ? frame_index(c.ya + lane, L.NY, P.W, C.free_surface) : -1);
     const bool fs = C.free_surface != 0;
-    float rh2[S], rh3[S], rh4[S];
+    float rh[3][S];
 #pragma unroll
-    for (int r = 0; r < S; r++) rh2[r] = rh3[r] = rh4[r] = 0.f;
+    for (int r = 0; r < S; r++) rh[0][r] = rh[1][r] = rh[2][r] = 0.f;
 
 #pragma unroll
     for (int g = 0; g < G; g++) {
@@ -422,9 +436,9 @@ __global__ __launch_bounds__(NTHREADS) void el2d_stress_strip(Layout2D L, Fields
             xx -= 2.f * (a3 * cmu[r]);
             yy -= 2.f * (a1 * cmu[r]);
             u2 += a2;
-            rh2[r] = a1;
-            rh3[r] = a3;
-            rh4[r] = u2;
+            rh[0][r] = a1;
+            rh[1][r] = a3;
+            rh[2][r] = u2;
             if (Sn.h[2]) {
                 const long long hr = c.so + L.at(c.x0, y) + lane;
                 Sn.h[2][hr] = a1;
@@ -451,9 +465,7 @@ __global__ __launch_bounds__(NTHREADS) void el2d_stress_strip(Layout2D L, Fields
     }
     if (Rc.h) {
         const int ts = A.tbase[0] + tofs;
-        record_strip<S>(Rc, L, A.nshots, ts, 2, c.shot, c.x0, c.ya, lane, rh2);
-        record_strip<S>(Rc, L, A.nshots, ts, 3, c.shot, c.x0, c.ya, lane, rh3);
-        record_strip<S>(Rc, L, A.nshots, ts, 4, c.shot, c.x0, c.ya, lane, rh4);
+        record_strips<S, 3>(Rc, L, A.nshots, ts, 2, c.shot, c.x0, c.ya, lane, rh);
     }
     block_acquisition<ELASTIC>(A, F, c.so, shot_block(A, B), A.tbase[0] + tofs, false);
 }
