@@ -1,5 +1,6 @@
-// Raw buffer loads and stores, and an LDS-only block barrier, for the gfx950
-// kernels (shared by the 2D and 3D kernel files).  Not part of the public ABI.
+// Raw buffer loads and stores, an LDS-only block barrier and a wavefront
+// maximum, for the gfx950 kernels (shared by the 2D and 3D kernel files).  Not
+// part of the public ABI.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -32,6 +33,28 @@ __device__ __forceinline__ void block_sync_lds()
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
     __builtin_amdgcn_s_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Maximum of a non-negative float over the 64 lanes, returned to every lane, with
+// DPP moves only (no LDS round trip): swap neighbours, swap pairs, mirror within 8,
+// mirror within 16 (every lane of a row holds the row maximum), then broadcast
+// lane 15 into rows 1 and 3 and lane 31 into rows 2 and 3; lane 63 holds the result.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float dpp_max_step(float x)
+{
+    const float y =
+        __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(x), __float_as_int(x), CTRL, ROWS, 0xF, false));
+    return fmaxf(x, y);
+}
+__device__ __forceinline__ float wave_max_nonneg(float x)
+{
+    x = dpp_max_step<0xB1, 0xF>(x);   // quad_perm [1,0,3,2]
+    x = dpp_max_step<0x4E, 0xF>(x);   // quad_perm [2,3,0,1]
+    x = dpp_max_step<0x141, 0xF>(x);  // row_half_mirror
+    x = dpp_max_step<0x140, 0xF>(x);  // row_mirror
+    x = dpp_max_step<0x142, 0xA>(x);  // row_bcast:15 into rows 1, 3
+    x = dpp_max_step<0x143, 0xC>(x);  // row_bcast:31 into rows 2, 3
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 63));
 }
 
 }  // namespace

@@ -72,10 +72,9 @@ __device__ __forceinline__ StripCtx strip_ctx(const Layout2D &L, const Cpml2D &P
 }
 
 // Record NF fields (f0, f0+1, ...) of the wavefront's strip (S rows x 64
-// columns) for the adjoint pass (Rec2D): the maximum over the strip sets each
-// field's FP16 scale (the NF maxima are reduced together, so the shuffle steps
-// of the fields overlap); rows are stored in pairs, one 32-bit word per lane
-// and pair.
+// columns) for the adjoint pass (Rec2D): the maximum over the strip (a DPP
+// wavefront reduction) sets each field's FP16 scale; rows are stored in pairs,
+// one 32-bit word per lane and pair.
 template <int S, int NF>
 __device__ __forceinline__ void record_strips(const Rec2D &R, const Layout2D &L, int nshots, int t, int f0, int shot,
                                               int x0, int ya, int lane, const float (&h)[NF][S])
@@ -91,13 +90,7 @@ __device__ __forceinline__ void record_strips(const Rec2D &R, const Layout2D &L,
             if (in && ya + r < L.NY) m[f] = fmaxf(m[f], fabsf(h[f][r]));
     }
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        float o[NF];
-#pragma unroll
-        for (int f = 0; f < NF; f++) o[f] = __shfl_xor(m[f], off);
-#pragma unroll
-        for (int f = 0; f < NF; f++) m[f] = fmaxf(m[f], o[f]);
-    }
+    for (int f = 0; f < NF; f++) m[f] = wave_max_nonneg(m[f]);
     if (ya >= L.NY) return;  // a strip wholly below the grid (last block row) records nothing
 #pragma unroll
     for (int f = 0; f < NF; f++) {
