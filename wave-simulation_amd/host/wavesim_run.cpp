@@ -10,20 +10,100 @@
 // receiver files (<ReceiverFilename>.txt, or .shot_<n>.txt with
 // useReceiversPerShot=1); runs every shot on the GPU in one batched handle and
 // writes <SeismogramFilename>.shot_<n>.<component>.mtx per shot and recorded
-// component, resampled to seismoDT (Seismogram.cpp:82-156).  Keys of this
+// component, resampled to seismoDT (Seismogram.cpp:82-156), after the trace
+// normalisation normalizeTraces = 1 / 2 (Seismogram.cpp:215-234), as MatrixMarket
+// (SeismogramFormat = 1) or SeismicUnix (SeismogramFormat = 4, <...>.su,
+// SUIO.hpp:156-260).  Keys of this
 // driver beyond the reference's: stepsPerGraph (time steps per captured graph,
 // default 50), tileY (fixed tile height; default: autotuned) and solverPath
 // (fused | phased | strip | plain).
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
+#include <cstring>
+#include <fstream>
 #include <iostream>
 #include <map>
 
 #include "wavesim.hpp"
 
 using namespace wavesim;
+
+namespace {
+
+// Seismogram::normalizeTrace (Seismogram.cpp:215-234): each trace [ntr][ns] divided by
+// its maximum absolute value (mode 1) or its L2 norm (mode 2); a zero norm divides by 1.
+void normalizeTraces(std::vector<float> &d, int ntr, int ns, int mode)
+{
+    if (mode <= 0) return;
+    for (int r = 0; r < ntr; r++) {
+        float *t = &d[(size_t)r * ns];
+        double nrm = 0.0;
+        if (mode == 1) {
+            for (int i = 0; i < ns; i++) nrm = std::max(nrm, (double)std::fabs(t[i]));
+        } else {
+            for (int i = 0; i < ns; i++) nrm += (double)t[i] * t[i];
+            nrm = std::sqrt(nrm);
+        }
+        float f = (float)nrm;
+        if (f == 0.f) f = 1.f;
+        for (int i = 0; i < ns; i++) t[i] /= f;
+    }
+}
+
+// SUIO::writeSU (SUIO.hpp:156-260): per trace a 240-byte SEG-Y trace header (the SU
+// layout; unset fields 0, counit 1) and ns native-endian FP32 samples.  Coordinates in
+// millimetres (scalel = scalco = -3), the source position only when the shot has a
+// single source (else the origin), angles as in the reference (reference point
+// shifted by 800 m for swdep); arithmetic in FP32 like the reference's float build.
+void writeSU(const std::string &fn, const std::vector<float> &d, int ntr, int ns, double outDT,
+             const std::vector<const ReceiverRow *> &rec, int sx, int sy, int sz, float DH)
+{
+    std::ofstream f(fn, std::ios::binary);
+    if (!f) throw std::runtime_error("cannot write " + fn);
+    const float XS = (float)sx * DH, YS = (float)sy * DH, ZS = (float)sz * DH;
+    const float xshift = 800.f, yshift = 800.f, pi = 3.1415926f;
+    const float dtus = (float)(outDT * 1000000.0);
+    unsigned char h[240];
+    auto i32 = [&](int off, int32_t v) { std::memcpy(h + off, &v, 4); };
+    auto i16 = [&](int off, int16_t v) { std::memcpy(h + off, &v, 2); };
+    auto f32 = [&](int off, float v) { std::memcpy(h + off, &v, 4); };
+    for (int r = 0; r < ntr; r++) {
+        std::memset(h, 0, sizeof(h));
+        const float xr = (float)rec[r]->x * DH, yr = (float)rec[r]->y * DH, zr = (float)rec[r]->z * DH;
+        const float x = xr - XS, y = yr - YS, z = zr - ZS;
+        i32(0, r + 1);                                                                  // tracl
+        i32(4, 1);                                                                      // tracr
+        i32(16, 1);                                                                     // ep
+        i32(20, ntr);                                                                   // cdp
+        i16(28, 1);                                                                     // trid
+        i32(36, (int32_t)std::lround(std::sqrt(x * x + y * y + z * z) * 1000.0));       // offset
+        i32(40, (int32_t)std::lround(yr * 1000.0));                                     // gelev
+        i32(48, (int32_t)std::lround(YS * 1000.0));                                     // sdepth
+        i32(52, (int32_t)std::lround(std::atan2(-y, z) * 180 * 1000.0 / pi));           // gdel
+        i32(60, (int32_t)std::lround((360.0f / (2.0f * pi)) * std::atan2(x - xshift, y - yshift) * 1000.0));  // swdep
+        i32(64, (int32_t)std::lround(std::sqrt(z * z + y * y) * 1000.0));               // gwdep
+        i16(68, -3);                                                                    // scalel
+        i16(70, -3);                                                                    // scalco
+        i32(72, (int32_t)std::lround(XS * 1000.0));                                     // sx
+        i32(76, (int32_t)std::lround(ZS * 1000.0));                                     // sy
+        i32(80, (int32_t)std::lround(xr * 1000.0));                                     // gx
+        i32(84, (int32_t)std::lround(zr * 1000.0));                                     // gy
+        i16(88, 1);                                                                     // counit
+        const uint16_t nsu = (uint16_t)ns, dtu = (uint16_t)std::lround(dtus);
+        std::memcpy(h + 114, &nsu, 2);                                                  // ns
+        std::memcpy(h + 116, &dtu, 2);                                                  // dt [us]
+        f32(180, (float)dtu * 1.0e-6f);                                                 // d1
+        i32(204, ntr);                                                                  // ntr
+        f.write((const char *)h, sizeof(h));
+        f.write((const char *)&d[(size_t)r * ns], sizeof(float) * (size_t)ns);
+    }
+    if (!f) throw std::runtime_error("write failed: " + fn);
+}
+
+}  // namespace
 
 int main(int argc, char **argv)
 {
@@ -122,23 +202,45 @@ int main(int argc, char **argv)
         const double seismoDT = config.getDouble("seismoDT", DT);
         const double ratio = seismoDT / DT;
         const std::string out = config.getString("SeismogramFilename");
+        const int format = (int)config.getInt("SeismogramFormat", 1), norm = (int)config.getInt("normalizeTraces", 0);
+        if (format != 1 && format != 4)
+            throw std::runtime_error("SeismogramFormat " + std::to_string(format) + " is not written (1 = mtx, 4 = su)");
+        if (norm < 0 || norm > 2)
+            throw std::runtime_error("normalizeTraces " + std::to_string(norm) + " is not applied (0, 1, 2)");
+        const float DHf = (float)config.getDouble("DH");
         for (int k = 0; k < nshots; k++) {
             const int nr = (int)recs[k].size();
             std::vector<float> all = solver->seismograms(k, nr);
+            // the shot's source position for the SU headers: its only source, or the origin
+            int sxi = 0, syi = 0, szi = 0, nsrc = 0;
+            for (auto &s2 : src)
+                if (s2.shot == shotNumbers[k]) {
+                    sxi = s2.x;
+                    syi = s2.y;
+                    szi = s2.z;
+                    nsrc++;
+                }
+            if (nsrc != 1) sxi = syi = szi = 0;
             for (int ty = 1; ty <= 4; ty++) {
                 std::vector<float> sel;
+                std::vector<const ReceiverRow *> selr;
                 int ntr = 0;
                 for (int r = 0; r < nr; r++) {
                     if (recs[k][r].type != ty) continue;
                     sel.insert(sel.end(), all.begin() + (size_t)r * NT, all.begin() + (size_t)(r + 1) * NT);
+                    selr.push_back(&recs[k][r]);
                     ntr++;
                 }
                 if (!ntr) continue;
+                normalizeTraces(sel, ntr, NT, norm);  // on the time-loop samples, before resampling
                 int NTo = NT;
                 if (std::fabs(ratio - 1.0) > 1e-12) sel = resampleTraces(sel, ntr, NT, ratio, NTo);
-                const std::string fn =
-                    out + ".shot_" + std::to_string(shotNumbers[k]) + "." + (em ? comp_em[ty] : comp[ty]) + ".mtx";
-                writeMatrixMtx(fn, sel, ntr, NTo);
+                const std::string fn = out + ".shot_" + std::to_string(shotNumbers[k]) + "." +
+                                       (em ? comp_em[ty] : comp[ty]) + (format == 4 ? ".su" : ".mtx");
+                if (format == 4)
+                    writeSU(fn, sel, ntr, NTo, seismoDT, selr, sxi, syi, szi, DHf);
+                else
+                    writeMatrixMtx(fn, sel, ntr, NTo);
                 std::printf("writing %s\n", fn.c_str());
             }
         }
