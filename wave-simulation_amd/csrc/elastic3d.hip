@@ -55,6 +55,20 @@ struct Region {
             }
         }
     }
+    // rows of an LDS array that every thread's share fits in (surplus threads write padding)
+    static constexpr int RHP = (IT * NTH + RW - 1) / RW;
+    // the same without a guard (S has RHP rows), so no divergent branch sits between the loads
+    // in flight and this store
+    __device__ __forceinline__ void store_padded(float (*S)[RW], int tid) const
+    {
+#pragma unroll
+        for (int k = 0; k < IT; k++) {
+            const int i = tid + k * NTH;
+            const int r = i / RW;
+            const int c = i - r * RW;
+            S[r][c] = v[k];
+        }
+    }
 };
 
 // Byte offsets, within one y plane, of a thread's share of a Region (the same
@@ -298,13 +312,13 @@ __global__ __launch_bounds__(NTH) void el3d_velocity(Layout3D L, Fields3D F, Coe
     using RX = Region<H, 0, TZ>;   // x derivatives only
     using RXZ = Region<H, H, TZ>;  // x and z derivatives
     using RZZ = Region<0, H, TZ>;  // z derivatives only
-    __shared__ float s_xx[RX::RH][RX::RW];
-    __shared__ float s_xy[RX::RH][RX::RW];
-    __shared__ float s_xz[RXZ::RH][RXZ::RW];
-    __shared__ float s_yz[RZZ::RH][RZZ::RW];
-    __shared__ float s_zz[RZZ::RH][RZZ::RW];
+    __shared__ float s_xx[RX::RHP][RX::RW];
+    __shared__ float s_xy[RX::RHP][RX::RW];
+    __shared__ float s_xz[RXZ::RHP][RXZ::RW];
+    __shared__ float s_yz[RZZ::RHP][RZZ::RW];
+    __shared__ float s_zz[RZZ::RHP][RZZ::RW];
     __shared__ ZTab<TZ, K> s_zt;
-    __shared__ float s_yt[2 * K + 5];
+    __shared__ float s_yt[NTH];  // the plane's y table in [0, 2K+5); every thread writes its slot
     const Pos q = position(L, TZ, nyc, LY, A);
     const long long so = (long long)q.shot * L.vol;
     const int tid = threadIdx.y * TILE_X + threadIdx.x, tx = threadIdx.x;
@@ -371,6 +385,8 @@ __global__ __launch_bounds__(NTH) void el3d_velocity(Layout3D L, Fields3D F, Coe
     RXZ rxz;
     RZZ ryz, rzz;
     auto load_plane = [&](PlaneIn &pi, int y) {
+        // the y table first: the LDS store that waits for it then waits for nothing issued after it
+        pi.yt = ytab_elem<K>(T.yv, y, tid);
         load_region(rxx, ox, sxx, L, y);
         load_region(rxy, ox, sxy, L, y);
         load_region(rxz, oxz, sxz, L, y);
@@ -385,7 +401,6 @@ __global__ __launch_bounds__(NTH) void el3d_velocity(Layout3D L, Fields3D F, Coe
                 pi.nyz[r] = bload(b2, oc[r], 0);
             }
         }
-        pi.yt = ytab_elem<K>(T.yv, y, tid);
 #pragma unroll
         for (int f = 0; f < 3; f++) {
             const auto b = plane_rsrc(vv[f], L, y), bc = plane_rsrc(cr[f], L, y);
@@ -414,12 +429,12 @@ __global__ __launch_bounds__(NTH) void el3d_velocity(Layout3D L, Fields3D F, Coe
             qyy[2 * H][r] = cur.nyy[r];
             qyz[2 * H][r] = cur.nyz[r];
         }
-        rxx.store(s_xx, tid);
-        rxy.store(s_xy, tid);
-        rxz.store(s_xz, tid);
-        ryz.store(s_yz, tid);
-        rzz.store(s_zz, tid);
-        if (tid < 2 * K + 5) s_yt[tid] = cur.yt;
+        rxx.store_padded(s_xx, tid);
+        rxy.store_padded(s_xy, tid);
+        rxz.store_padded(s_xz, tid);
+        ryz.store_padded(s_yz, tid);
+        rzz.store_padded(s_zz, tid);
+        s_yt[tid] = cur.yt;  // (no guard: a divergent store here would wait for every load)
         block_sync_lds();
 
         // CPML memory variables: tiles and planes outside every frame skip them (a uniform
@@ -514,11 +529,11 @@ __global__ __launch_bounds__(NTH) void el3d_stress(Layout3D L, Fields3D F, Coef3
 {
     constexpr int TZ = RZ * BLOCK_Y, K = 2 * H + 1;
     using RXZ = Region<H, H, TZ>;
-    __shared__ float s_vx[RXZ::RH][RXZ::RW];
-    __shared__ float s_vy[RXZ::RH][RXZ::RW];
-    __shared__ float s_vz[RXZ::RH][RXZ::RW];
+    __shared__ float s_vx[RXZ::RHP][RXZ::RW];
+    __shared__ float s_vy[RXZ::RHP][RXZ::RW];
+    __shared__ float s_vz[RXZ::RHP][RXZ::RW];
     __shared__ ZTab<TZ, K> s_zt;
-    __shared__ float s_yt[2 * K + 5];
+    __shared__ float s_yt[NTH];  // the plane's y table in [0, 2K+5); every thread writes its slot
     const Pos q = position(L, TZ, nyc, LY, A);
     const long long so = (long long)q.shot * L.vol;
     const int tid = threadIdx.y * TILE_X + threadIdx.x, tx = threadIdx.x;
@@ -576,6 +591,8 @@ __global__ __launch_bounds__(NTH) void el3d_stress(Layout3D L, Fields3D F, Coef3
     const float *const cf[5] = {C.M, C.mu, C.muxy, C.muxz, C.muyz};
     const float *const st[6] = {sxx, syy, szz, syz, sxz, sxy};
     auto load_plane = [&](PlaneIn &pi, RXZ &gx, RXZ &gy, RXZ &gz, int y) {
+        // the y table first: the LDS store that waits for it then waits for nothing issued after it
+        pi.yt = ytab_elem<K>(T.ys, y, tid);
         load_region(gx, ro, vx, L, y);
         load_region(gy, ro, vy, L, y);
         load_region(gz, ro, vz, L, y);
@@ -600,7 +617,6 @@ __global__ __launch_bounds__(NTH) void el3d_stress(Layout3D L, Fields3D F, Coef3
 #pragma unroll
             for (int r = 0; r < RZ; r++) pi.c[f][r] = bload(b, oc[r], 0);
         }
-        pi.yt = ytab_elem<K>(T.ys, y, tid);
         if (C.damp) {
             const auto b = plane_rsrc(C.damp, L, y);
 #pragma unroll
@@ -630,10 +646,10 @@ __global__ __launch_bounds__(NTH) void el3d_stress(Layout3D L, Fields3D F, Coef3
             qvy[2 * H][r] = cur.nvy[r];
             qvz[2 * H][r] = cur.nvz[r];
         }
-        rvx.store(s_vx, tid);
-        rvy.store(s_vy, tid);
-        rvz.store(s_vz, tid);
-        if (tid < 2 * K + 5) s_yt[tid] = cur.yt;
+        rvx.store_padded(s_vx, tid);
+        rvy.store_padded(s_vy, tid);
+        rvz.store_padded(s_vz, tid);
+        s_yt[tid] = cur.yt;  // (no guard: a divergent store here would wait for every load)
         block_sync_lds();
 
         // CPML memory variables: planes and tiles outside every frame take the body without them;
