@@ -271,7 +271,7 @@ __global__ __launch_bounds__(NTH) void el2d_fused(Layout2D L, Fields2D F, Fields
                 }
             }
             if (P.on) {
-                const int yi = P.yi[yv];
+                const int yi = frame_index(yv, L.NY, P.W, C.free_surface);
                 if (yi >= 0) {
                     const int pi = yi * L.NX + xv;
                     const float b0 = cpml_step(t, P.psy[0][pys + pi], P.ay[yv], P.by[yv]);
@@ -331,7 +331,7 @@ __global__ __launch_bounds__(NTH) void el2d_fused(Layout2D L, Fields2D F, Fields
                 P.psx[3][pxs + pi] = cpml_step(t2, P.psx[3][pxs + pi], axh, bxh);
             }
             if (P.on) {
-                const int yi = P.yi[y];
+                const int yi = frame_index(y, L.NY, P.W, C.free_surface);
                 if (yi >= 0) {
                     const int pi = yi * L.NX + xv;
                     P.psy[2][pys + pi] = cpml_step(vyy, P.psy[2][pys + pi], P.ay[y], P.by[y]);

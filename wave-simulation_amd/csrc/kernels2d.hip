@@ -67,7 +67,7 @@ __global__ __launch_bounds__(NTHREADS) void el2d_velocity(Layout2D L, Fields2D F
         crx[r] = (C.c[2] + row)[ux];
         cry[r] = (C.c[3] + row)[ux];
         p0[r] = p1[r] = q0[r] = q1[r] = 0.f;
-        yi[r] = (P.on && y < L.NY) ? P.yi[y] : -1;
+        yi[r] = P.on ? frame_index(y, L.NY, P.W, C.free_surface) : -1;
         if (col.xi >= 0 && y < L.NY) {
             const long long pr = ((long long)shot * L.NY + y) * W2;
             p0[r] = (P.psx[0] + pr)[(unsigned)col.xi];
@@ -167,7 +167,7 @@ __global__ __launch_bounds__(NTHREADS) void el2d_stress(Layout2D L, Fields2D F, 
         cmxy[r] = (C.c[4] + row)[ux];
         dmp[r] = C.damp ? (C.damp + row)[ux] : 1.f;
         p2[r] = p3[r] = q2[r] = q3[r] = 0.f;
-        yi[r] = (P.on && y < L.NY) ? P.yi[y] : -1;
+        yi[r] = P.on ? frame_index(y, L.NY, P.W, C.free_surface) : -1;
         if (col.xi >= 0 && y < L.NY) {
             const long long pr = ((long long)shot * L.NY + y) * W2;
             p2[r] = (P.psx[2] + pr)[(unsigned)col.xi];
@@ -277,7 +277,7 @@ __global__ __launch_bounds__(NTHREADS) void ac2d_velocity(Layout2D L, Fields2D F
         crx[r] = C.c[1][o];
         cry[r] = C.c[2][o];
         p0[r] = q0[r] = 0.f;
-        yi[r] = P.on ? P.yi[y] : -1;
+        yi[r] = P.on ? frame_index(y, L.NY, P.W, C.free_surface) : -1;
         if (col.xi >= 0 && y < L.NY) p0[r] = P.psx[0][((long long)shot * L.NY + y) * W2 + col.xi];
         if (yi[r] >= 0 && col.in) q0[r] = P.psy[0][((long long)shot * W2 + yi[r]) * L.NX + col.x];
     }
@@ -336,7 +336,7 @@ __global__ __launch_bounds__(NTHREADS) void ac2d_pressure(Layout2D L, Fields2D F
         cM[r] = C.c[0][o];
         dmp[r] = C.damp ? C.damp[o] : 1.f;
         p1[r] = q1[r] = 0.f;
-        yi[r] = P.on ? P.yi[y] : -1;
+        yi[r] = P.on ? frame_index(y, L.NY, P.W, C.free_surface) : -1;
         if (col.xi >= 0 && y < L.NY) p1[r] = P.psx[1][((long long)shot * L.NY + y) * W2 + col.xi];
         if (yi[r] >= 0 && col.in) q1[r] = P.psy[1][((long long)shot * W2 + yi[r]) * L.NX + col.x];
     }

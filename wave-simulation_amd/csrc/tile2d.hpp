@@ -132,17 +132,6 @@ struct Col {
     float ax, bx, axh, bxh;
 };
 
-// Compact CPML frame index of position p on an axis of n positions (the rule the
-// host tables follow): near side 0..W-1, far side W..2W-1, -1 outside the frame
-// or on a skipped near side (the top under a free surface).  Computed, not
-// loaded, so no load address waits for it.
-__device__ __forceinline__ int frame_index(int p, int n, int W, bool skip_near)
-{
-    if (p < 0 || p >= n) return -1;
-    if (p < W) return skip_near ? -1 : p;
-    return n - 1 - p < W ? 2 * W - n + p : -1;
-}
-
 __device__ __forceinline__ Col column(const Layout2D &L, const Cpml2D &P, int x)
 {
     Col c;

@@ -45,6 +45,17 @@ struct Taps {
                                // (free-surface image versions when FreeSurface=1)
 };
 
+// Compact CPML frame index of position p on an axis of n positions (the rule of
+// the host tables, cpml_axis): near side 0..W-1, far side W..2W-1, -1 outside
+// the frame or on a skipped near side (the top under a free surface).  Kernels
+// compute it rather than load it, so no load address waits for it.
+__host__ __device__ inline int frame_index(int p, int n, int W, bool skip_near)
+{
+    if (p < 0 || p >= n) return -1;
+    if (p < W) return skip_near ? -1 : p;
+    return n - 1 - p < W ? 2 * W - n + p : -1;
+}
+
 // CPML coefficients along each axis (zero outside the frame) and the compact
 // frame index of each position (-1 outside).  Memory variables of the x frame
 // are stored [shot][NY][2W], of the y frame [shot][2W][NX].
