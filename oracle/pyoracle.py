@@ -144,6 +144,39 @@ def adjoint_elastic2d(params: Params, vp, vs, rho, src_idx, src_type, src_sig, r
     return seis, J.value, gcoef, gmodel
 
 
+def adjoint_elastic3d(params: Params, vp, vs, rho, src_idx, src_type, src_sig, rec_idx, rec_type, obs,
+                      precision="f64"):
+    """3D elastic misfit J = 1/2 ||seis - obs||^2 and its adjoint-state gradient.
+
+    Returns (seis [nrec, NT], J, gcoef [8, n] w.r.t. (1/rho_x, 1/rho_y,
+    1/rho_z, M, mu, mu_xy, mu_xz, mu_yz), gmodel [3, n] w.r.t. (vp, vs,
+    rho)); gradients are FP64.  Receiver types 1 pressure, 2 vx, 3 vy, 4 vz.
+    """
+    L = lib(precision)
+    dt = dtype_of(precision)
+    real = C.c_float if precision == "f32" else C.c_double
+    n = params.NX * params.NY * params.NZ
+    a = [np.ascontiguousarray(m, dtype=dt).ravel() for m in (vp, vs, rho)]
+    src_idx = np.ascontiguousarray(src_idx, dtype=np.int64)
+    src_type = np.ascontiguousarray(src_type, dtype=np.int32)
+    src_sig = np.ascontiguousarray(src_sig, dtype=dt)
+    rec_idx = np.ascontiguousarray(rec_idx, dtype=np.int64)
+    rec_type = np.ascontiguousarray(rec_type, dtype=np.int32)
+    obs = np.ascontiguousarray(obs, dtype=dt).reshape(len(rec_idx), params.NT)
+    seis = np.zeros((len(rec_idx), params.NT), dtype=dt)
+    J = C.c_double()
+    gcoef = np.zeros((8, n), dtype=np.float64)
+    gmodel = np.zeros((3, n), dtype=np.float64)
+    rc = L.oracle_adjoint_elastic3d(C.byref(params), _ptr(a[0], real), _ptr(a[1], real), _ptr(a[2], real),
+                                    C.c_int(len(src_idx)), _ptr(src_idx, C.c_long), _ptr(src_type, C.c_int),
+                                    _ptr(src_sig, real), C.c_int(len(rec_idx)), _ptr(rec_idx, C.c_long),
+                                    _ptr(rec_type, C.c_int), _ptr(obs, real), _ptr(seis, real), C.byref(J),
+                                    _ptr(gcoef, C.c_double), _ptr(gmodel, C.c_double))
+    if rc != 0:
+        raise RuntimeError(f"oracle_adjoint_elastic3d failed with {rc}")
+    return seis, J.value, gcoef, gmodel
+
+
 def ricker(NT, DT, fc, amp, tshift=0.0, precision="f32"):
     L = lib(precision)
     dt = dtype_of(precision)

@@ -37,6 +37,10 @@
  *   source / receiver           SourceReceiverImpl.cpp:12-37, SourceReceiverImplSeismic.cpp:34-105,
  *                               FDTD2Delastic.cpp:12-49, FDTDacoustic.cpp:12-45
  *   Ricker wavelet              src/Acquisition/SourceSignal/Ricker.cpp:29-53
+ *
+ * The adjoint-state gradients (oracle_adjoint_elastic2d / _elastic3d) have no
+ * counterpart in the reference; they are the exact discrete adjoints of the
+ * loops above, pinned by FP64 central differences of this file's own misfit.
  */
 #include <math.h>
 #include <stdint.h>
@@ -1477,6 +1481,313 @@ int oracle_adjoint_elastic2d(const oracle_params *p, const real *vp, const real 
     csr_free(&Dxb_t);
     csr_free(&Dyf_t);
     csr_free(&Dyb_t);
+    ops_free(&o);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Adjoint-state gradient of the 3D elastic time loop: the same construction */
+/* as the 2D case above for ForwardSolver3Delastic.cpp:120-416 (CPML or no    */
+/* boundary, no free surface / ABS).  Each stencil term of the step is one    */
+/* row of a table (operator, field read, CPML memory variable, CPML axis,     */
+/* half-grid coefficients); the forward pass and its transpose walk the same  */
+/* rows, so the two cannot disagree on a term.                                */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+    int op;   /* 0 Dxf 1 Dxb 2 Dyf 3 Dyb 4 Dzf 5 Dzb */
+    int f;    /* field read: 0 vx 1 vy 2 vz 3 sxx 4 syy 5 szz 6 syz 7 sxz 8 sxy */
+    int psi;  /* CPML memory variable 0..17 (the order of oracle_run) */
+    int axis; /* CPML profile axis */
+    int half; /* half-grid profile (a_h, b_h) */
+} term3d_t;
+
+/* velocity component c: three terms, times 1/rho_c */
+static const term3d_t VEL3[3][3] = {
+    {{0, 3, 0, 0, 1}, {3, 8, 1, 1, 0}, {5, 7, 2, 2, 0}}, /* vx: Dxf sxx, Dyb sxy, Dzb sxz */
+    {{1, 8, 3, 0, 0}, {2, 4, 4, 1, 1}, {5, 6, 5, 2, 0}}, /* vy: Dxb sxy, Dyf syy, Dzb syz */
+    {{1, 7, 6, 0, 0}, {3, 6, 7, 1, 0}, {4, 5, 8, 2, 1}}, /* vz: Dxb sxz, Dyb syz, Dzf szz */
+};
+/* normal strains Dxb vx, Dyb vy, Dzb vz */
+static const term3d_t NRM3[3] = {{1, 0, 9, 0, 0}, {3, 1, 10, 1, 0}, {5, 2, 11, 2, 0}};
+/* shear stress k (sxy, sxz, syz): two terms, times mu_xy, mu_xz, mu_yz */
+static const term3d_t SHR3[3][2] = {
+    {{2, 0, 12, 1, 1}, {0, 1, 13, 0, 1}}, /* sxy: Dyf vx, Dxf vy */
+    {{4, 0, 14, 2, 1}, {0, 2, 15, 0, 1}}, /* sxz: Dzf vx, Dxf vz */
+    {{4, 1, 16, 2, 1}, {2, 2, 17, 1, 1}}, /* syz: Dzf vy, Dyf vz */
+};
+static const int SHR3_FIELD[3] = {8, 7, 6};
+static const int SHR3_AXES[3][2] = {{0, 1}, {0, 2}, {1, 2}};
+
+/*
+ * 3D elastic forward + adjoint.  Model: vp, vs, rho (x fastest, then z, then
+ * y).  Misfit J = 1/2 sum_{r,t} (seis - obs)^2 with the receivers of
+ * oracle_run (types 1 pressure = (sxx+syy+szz)/3, 2 vx, 3 vy, 4 vz).
+ * Outputs: seis [nrec][NT], *misfit, gcoef [8][n] = dJ/d(1/rho_x, 1/rho_y,
+ * 1/rho_z, M, mu, mu_xy, mu_xz, mu_yz), gmodel [3][n] = dJ/d(vp, vs, rho).
+ * Returns 0, -2 for an unsupported configuration (free surface, ABS, not 3D
+ * elastic), -3 when the per-step record cannot be allocated.
+ */
+int oracle_adjoint_elastic3d(const oracle_params *p, const real *vp, const real *vs, const real *rho, int nsrc,
+                             const long *src_idx, const int *src_type, const real *src_sig, int nrec,
+                             const long *rec_idx, const int *rec_type, const real *obs, real *seis, double *misfit,
+                             double *gcoef, double *gmodel)
+{
+#ifdef _OPENMP
+    if (p->nthreads > 0) omp_set_num_threads(p->nthreads);
+#endif
+    if (p->dim != 3 || p->physics != PH_ELASTIC || p->free_surface || p->damping == 1) return -2;
+    grid_t g = {p->NX, p->NY, p->NZ, (long)p->NX * p->NY * p->NZ};
+    long n = g.n;
+    int NT = p->NT;
+    /* per-step factors: 0..2 the velocity sums, 3..5 the normal strains, 6..8 the shear sums */
+    real *rec = (real *)calloc((size_t)9 * n * NT, sizeof(real));
+    if (!rec) return -3;
+    ops_t o;
+    ops_build(&g, p, &o);
+    const csr_t *D[6] = {&o.Dxf, &o.Dxb, &o.Dyf, &o.Dyb, &o.Dzf, &o.Dzb};
+    csr_t DT_[6];
+    for (int k = 0; k < 6; k++) DT_[k] = csr_transpose(D[k]);
+
+    real *M = vnew(n), *mu = vnew(n), *ir[3], *ms[3];
+    for (long i = 0; i < n; i++) {
+        M[i] = rho[i] * vp[i] * vp[i];
+        mu[i] = rho[i] * vs[i] * vs[i];
+    }
+    for (int a = 0; a < 3; a++) {
+        ir[a] = vnew(n);
+        inverse_average_2pt(&g, a, rho, ir[a]);
+    }
+    for (int k = 0; k < 3; k++) {
+        ms[k] = vnew(n);
+        shear_average(&g, mu, SHR3_AXES[k][0], SHR3_AXES[k][1], ms[k]); /* clamps mu in place */
+    }
+
+    cpml_coef_t cp;
+    int use_cpml = p->damping == 2;
+    if (use_cpml)
+        cpml_layout(&g, 3, p->width, 0, (real)p->npower, (real)p->fc_cpml, (real)p->vmax_cpml, (real)p->DT,
+                    (real)p->DH, &cp);
+#define CA(t) (use_cpml ? ((t).half ? cp.ah[(t).axis] : cp.a[(t).axis]) : NULL)
+#define CB(t) (use_cpml ? ((t).half ? cp.bh[(t).axis] : cp.b[(t).axis]) : NULL)
+
+    real *F[9], *L[9], *psi[18], *lmu[18];
+    for (int k = 0; k < 9; k++) {
+        F[k] = vnew(n);
+        L[k] = vnew(n);
+    }
+    for (int k = 0; k < 18; k++) {
+        psi[k] = vnew(n);
+        lmu[k] = vnew(n);
+    }
+    real *upd = vnew(n), *tmp = vnew(n), *lh = vnew(n), *lg = vnew(n);
+
+    for (int t = 0; t < NT; t++) {
+        real *R = rec + (size_t)t * 9 * n;
+        /* velocity update with the stresses of the previous step */
+        for (int c = 0; c < 3; c++) {
+            for (int j = 0; j < 3; j++) {
+                const term3d_t *q = &VEL3[c][j];
+                spmv(D[q->op], F[q->f], j ? tmp : upd);
+                if (use_cpml) cpml_apply(n, j ? tmp : upd, psi[q->psi], CA(*q), CB(*q));
+                if (j) v_add(n, upd, tmp);
+            }
+            v_copy(n, R + (size_t)c * n, upd);
+            v_mul(n, upd, ir[c]);
+            v_add(n, F[c], upd);
+        }
+        /* normal stresses */
+        real *e[3];
+        for (int c = 0; c < 3; c++) {
+            const term3d_t *q = &NRM3[c];
+            e[c] = R + (size_t)(3 + c) * n;
+            spmv(D[q->op], F[q->f], e[c]);
+            if (use_cpml) cpml_apply(n, e[c], psi[q->psi], CA(*q), CB(*q));
+        }
+        VLOOP(n)
+        {
+            real tr = M[i] * (e[0][i] + e[1][i] + e[2][i]);
+            F[3][i] += tr - (real)2 * mu[i] * (e[1][i] + e[2][i]);
+            F[4][i] += tr - (real)2 * mu[i] * (e[0][i] + e[2][i]);
+            F[5][i] += tr - (real)2 * mu[i] * (e[0][i] + e[1][i]);
+        }
+        /* shear stresses */
+        for (int k = 0; k < 3; k++) {
+            for (int j = 0; j < 2; j++) {
+                const term3d_t *q = &SHR3[k][j];
+                spmv(D[q->op], F[q->f], j ? tmp : upd);
+                if (use_cpml) cpml_apply(n, j ? tmp : upd, psi[q->psi], CA(*q), CB(*q));
+                if (j) v_add(n, upd, tmp);
+            }
+            v_copy(n, R + (size_t)(6 + k) * n, upd);
+            v_mul(n, upd, ms[k]);
+            v_add(n, F[SHR3_FIELD[k]], upd);
+        }
+        for (int s = 0; s < nsrc; s++) {
+            real v = src_sig[(long)s * NT + t];
+            long i = src_idx[s];
+            if (src_type[s] == 1) {
+                F[3][i] += v;
+                F[4][i] += v;
+                F[5][i] += v;
+            } else if (src_type[s] >= 2 && src_type[s] <= 4)
+                F[src_type[s] - 2][i] += v;
+        }
+        for (int r = 0; r < nrec; r++) {
+            long i = rec_idx[r];
+            real v;
+            if (rec_type[r] == 1) {
+                v = F[3][i];
+                v += F[4][i];
+                v += F[5][i];
+                v /= (real)3;
+            } else
+                v = F[rec_type[r] - 2][i];
+            seis[(long)r * NT + t] = v;
+        }
+    }
+
+    double J = 0;
+    for (long k = 0; k < (long)nrec * NT; k++) {
+        double d = (double)seis[k] - (double)obs[k];
+        J += 0.5 * d * d;
+    }
+    *misfit = J;
+
+    double *G = gcoef;
+    memset(G, 0, sizeof(double) * 8 * (size_t)n);
+    for (int t = NT - 1; t >= 0; t--) {
+        const real *R = rec + (size_t)t * 9 * n;
+        for (int r = 0; r < nrec; r++) {
+            long i = rec_idx[r];
+            real d = seis[(long)r * NT + t] - obs[(long)r * NT + t];
+            if (rec_type[r] == 1)
+                for (int c = 3; c < 6; c++) L[c][i] += d / (real)3;
+            else
+                L[rec_type[r] - 2][i] += d;
+        }
+        /* transpose of the stress update */
+        const real *e0 = R + 3 * n, *e1 = R + 4 * n, *e2 = R + 5 * n;
+        for (long i = 0; i < n; i++) {
+            double lx = L[3][i], ly = L[4][i], lz = L[5][i];
+            G[3 * n + i] += (lx + ly + lz) * ((double)e0[i] + (double)e1[i] + (double)e2[i]);
+            G[4 * n + i] += -2.0 * (lx * ((double)e1[i] + e2[i]) + ly * ((double)e0[i] + e2[i]) +
+                                    lz * ((double)e0[i] + e1[i]));
+            for (int k = 0; k < 3; k++) G[(5 + k) * n + i] += (double)L[SHR3_FIELD[k]][i] * R[(6 + k) * n + i];
+        }
+        for (int c = 0; c < 3; c++) {
+            const term3d_t *q = &NRM3[c];
+            VLOOP(n)
+            {
+                real ls = L[3][i] + L[4][i] + L[5][i];
+                lh[i] = M[i] * ls - (real)2 * mu[i] * (ls - L[3 + c][i]);
+            }
+            cpml_adjoint(n, lh, lg, lmu[q->psi], CA(*q), CB(*q));
+            spmv_add(&DT_[q->op], lg, L[q->f]);
+        }
+        for (int k = 0; k < 3; k++) {
+            VLOOP(n) lh[i] = ms[k][i] * L[SHR3_FIELD[k]][i];
+            for (int j = 0; j < 2; j++) {
+                const term3d_t *q = &SHR3[k][j];
+                cpml_adjoint(n, lh, lg, lmu[q->psi], CA(*q), CB(*q));
+                spmv_add(&DT_[q->op], lg, L[q->f]);
+            }
+        }
+        /* transpose of the velocity update */
+        for (int c = 0; c < 3; c++) {
+            for (long i = 0; i < n; i++) G[(size_t)c * n + i] += (double)L[c][i] * R[(size_t)c * n + i];
+            VLOOP(n) lh[i] = ir[c][i] * L[c][i];
+            for (int j = 0; j < 3; j++) {
+                const term3d_t *q = &VEL3[c][j];
+                cpml_adjoint(n, lh, lg, lmu[q->psi], CA(*q), CB(*q));
+                spmv_add(&DT_[q->op], lg, L[q->f]);
+            }
+        }
+    }
+#undef CA
+#undef CB
+
+    /* chain rule to vp, vs, rho (Modelparameter.cpp:336-639 averaging rules) */
+    double *gvp = gmodel, *gvs = gmodel + n, *grho = gmodel + 2 * n;
+    memset(gmodel, 0, sizeof(double) * 3 * (size_t)n);
+    double *gmu_tot = (double *)calloc((size_t)n, sizeof(double));
+    for (long i = 0; i < n; i++) gmu_tot[i] = G[4 * n + i];
+    for (int k = 0; k < 3; k++) {
+        int a0 = SHR3_AXES[k][0], a1 = SHR3_AXES[k][1];
+        long L0 = axis_len(&g, a0), L1 = axis_len(&g, a1);
+        for (long j = 0; j < n; j++) {
+            double m = ms[k][j];
+            if (m == 0) continue; /* below the cut-off: constant 0 */
+            long x, y, z;
+            gcoord(&g, j, &x, &y, &z);
+            long p0 = axis_pos(x, y, z, a0), p1 = axis_pos(x, y, z, a1);
+            int in0 = p0 + 1 < L0, in1 = p1 + 1 < L1;
+            long pts[4];
+            int np = 0;
+            pts[np++] = j;
+            if (in0) pts[np++] = axis_idx(&g, x, y, z, a0, p0 + 1);
+            if (in1) pts[np++] = axis_idx(&g, x, y, z, a1, p1 + 1);
+            if (in0 && in1) {
+                long x2 = x, y2 = y, z2 = z;
+                long *c0 = a0 == 0 ? &x2 : (a0 == 1 ? &y2 : &z2), *c1 = a1 == 0 ? &x2 : (a1 == 1 ? &y2 : &z2);
+                *c0 += 1;
+                *c1 += 1;
+                pts[np++] = gidx(&g, x2, y2, z2);
+            }
+            double w = 1.0 / np;
+            for (int q = 0; q < np; q++) {
+                double mq = mu[pts[q]];
+                gmu_tot[pts[q]] += G[(5 + k) * n + j] * m * m * w / (mq * mq);
+            }
+        }
+    }
+    for (long i = 0; i < n; i++) {
+        double r = rho[i], a = vp[i], b = vs[i];
+        gvp[i] = G[3 * n + i] * 2.0 * r * a;
+        grho[i] += G[3 * n + i] * a * a;
+        if (r * b * b >= 1.0) { /* mu not clamped */
+            gvs[i] = gmu_tot[i] * 2.0 * r * b;
+            grho[i] += gmu_tot[i] * b * b;
+        }
+    }
+    for (int a = 0; a < 3; a++) {
+        long La = axis_len(&g, a);
+        for (long i = 0; i < n; i++) {
+            long x, y, z;
+            gcoord(&g, i, &x, &y, &z);
+            long pa = axis_pos(x, y, z, a);
+            double iv = ir[a][i], gi = G[(size_t)a * n + i];
+            if (pa + 1 < La) {
+                grho[i] += gi * (-0.5 * iv * iv);
+                grho[axis_idx(&g, x, y, z, a, pa + 1)] += gi * (-0.5 * iv * iv);
+            } else
+                grho[i] += gi * (-iv * iv);
+        }
+    }
+    free(gmu_tot);
+
+    free(rec);
+    for (int k = 0; k < 9; k++) {
+        free(F[k]);
+        free(L[k]);
+    }
+    for (int k = 0; k < 18; k++) {
+        free(psi[k]);
+        free(lmu[k]);
+    }
+    for (int k = 0; k < 3; k++) {
+        free(ir[k]);
+        free(ms[k]);
+    }
+    real *frees[] = {M, mu, upd, tmp, lh, lg};
+    for (size_t k = 0; k < sizeof(frees) / sizeof(frees[0]); k++) free(frees[k]);
+    if (use_cpml)
+        for (int a2 = 0; a2 < 3; a2++) {
+            free(cp.a[a2]);
+            free(cp.b[a2]);
+            free(cp.ah[a2]);
+            free(cp.bh[a2]);
+        }
+    for (int k = 0; k < 6; k++) csr_free(&DT_[k]);
     ops_free(&o);
     return 0;
 }
