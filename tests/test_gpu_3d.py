@@ -91,6 +91,32 @@ def test_3d_orders_free_surface(order):
     _check(c)
 
 
+def _noncubic_case(NX=44, NY=36, NZ=30, NT=160, W=6, DT=2e-4):
+    """Three different axis lengths, none a multiple of the x or z tile."""
+    s = dict(NX=NX, NY=NY, NZ=NZ, NT=NT, DH=2.0, DT=DT, spatial_fd_order=8, damping_boundary=2, boundary_width=W,
+             npower=4.0, fc_cpml=60.0, vmax_cpml=3600.0)
+    g = cases.gaussian_field((NY, NZ, NX), 4.0, 21)
+    gr = cases.gaussian_field((NY, NZ, NX), 4.0, 22)
+    vp = (1800.0 + 40.0 * np.arange(NY))[:, None, None] * (1.0 + 0.03 * g)
+    rho = (1900.0 + 15.0 * np.arange(NY))[:, None, None] * (1.0 + 0.03 * gr)
+    c = cases.Case("elastic_noncubic", "3D", "elastic", s, vp.astype(np.float32).ravel(),
+                   (vp / 1.8).astype(np.float32).ravel(), rho.astype(np.float32).ravel())
+    idx = lambda x, y, z: x + z * NX + y * NX * NZ  # noqa: E731
+    w = wavesim.ricker(NT, DT, 60.0, 1.0)
+    rec = np.array([idx(x, W + 2, z) for x in range(W + 1, NX - W - 1, 5) for z in range(W + 1, NZ - W - 1, 5)])
+    rec = np.concatenate([rec, rec, rec, rec])
+    rty = np.repeat(np.array([1, 2, 3, 4], np.int32), len(rec) // 4)
+    c.shots.append(cases.Shot(np.array([idx(13, 10, 17)]), np.array([3]), w[None, :], rec, rty))
+    c.shots.append(cases.Shot(np.array([idx(30, 20, 9)]), np.array([1]), w[None, :], rec, rty))
+    return c
+
+
+@pytest.mark.parametrize("tile_y", [4, 16])
+def test_3d_noncubic_ragged_grid(tile_y):
+    """NX, NY, NZ all different, so a stride or axis mix-up cannot cancel."""
+    _check(_noncubic_case(), tile_y=tile_y)
+
+
 def test_3d_abs():
     c = _small_c5(N=36, NT=140, W=6)
     c.settings.update(damping_boundary=1, damping_coeff=8.0)
