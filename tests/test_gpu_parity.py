@@ -179,6 +179,43 @@ def test_elastic_kernel_variants(path, tile_y):
     _check_case(c, path=path, tile_y=tile_y)
 
 
+def _elastic_rect(NX=170, NY=97, NT=500, W=12):
+    """NX != NY, neither a multiple of the 64-column strips or of any tile height."""
+    s = dict(NX=NX, NY=NY, NT=NT, DH=1.0, DT=1e-4, spatial_fd_order=8, damping_boundary=2, boundary_width=W,
+             npower=4.0, fc_cpml=30.0, vmax_cpml=4000.0)
+    vp, vs, rho = cases.elastic_model(NX, NY, 7, 107, sigma=8.0)
+    c = cases.Case("elastic_rect", "2D", "elastic", s, vp, vs, rho)
+    x = np.arange(W + 2, NX - W - 2, 3)
+    rec = np.concatenate([x + (W + 4) * NX, x + (W + 4) * NX, x + (NY - W - 5) * NX])
+    rty = np.concatenate([np.full(len(x), 2), np.full(len(x), 3), np.full(len(x), 1)]).astype(np.int32)
+    w = wavesim.ricker(NT, 1e-4, 30.0, 1.0)
+    c.shots.append(cases.Shot(np.array([60 + 50 * NX, 130 + 30 * NX]), np.array([3, 1]),
+                              np.stack([w, 0.5 * w]), rec, rty))
+    return c
+
+
+@pytest.mark.parametrize("path,tile_y", [("strip", 16), ("strip", 32), ("fused", 16), ("phased", 8), ("plain", 0)])
+def test_elastic_rectangular_grid(path, tile_y):
+    """A rectangular, ragged grid through every kernel family, so an x/y mix-up cannot cancel."""
+    _check_case(_elastic_rect(), path=path, tile_y=tile_y)
+
+
+@pytest.mark.parametrize("path,tile_y", [("fused", 8), ("fused", 32), ("plain", 0)])
+def test_acoustic_rectangular_grid(path, tile_y):
+    NX, NY, NT = 150, 83, 500
+    s = dict(NX=NX, NY=NY, NT=NT, DH=1.0, DT=2e-4, spatial_fd_order=4, damping_boundary=2, boundary_width=12,
+             npower=4.0, fc_cpml=50.0, vmax_cpml=2000.0)
+    vp = (1800.0 + 4.0 * np.arange(NY))[:, None].repeat(NX, 1).astype(np.float32).ravel()
+    rho = np.full(NX * NY, 2000.0, np.float32)
+    c = cases.Case("acoustic_rect", "2D", "acoustic", s, vp, None, rho)
+    x = np.arange(15, NX - 15, 2)
+    rec = np.concatenate([x + 16 * NX, x + 16 * NX, x + 60 * NX])
+    rty = np.concatenate([np.full(len(x), 1), np.full(len(x), 2), np.full(len(x), 3)]).astype(np.int32)
+    c.shots.append(cases.Shot(np.array([100 + 40 * NX]), np.array([1]), wavesim.ricker(NT, 2e-4, 50.0, 1.0)[None],
+                              rec, rty))
+    _check_case(c, path=path, tile_y=tile_y)
+
+
 @pytest.mark.parametrize("order", [2, 4, 12])
 def test_single_pass_orders_free_surface(order):
     c = cases.elastic_c2(N=90, NT=400, W=12)
