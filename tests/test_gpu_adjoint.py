@@ -9,6 +9,7 @@ import os
 import numpy as np
 import pytest
 
+import wavesim
 from wavesim import cases
 from oracle import pyoracle as po
 
@@ -121,6 +122,47 @@ def test_gradient_from_recorded_run(tile_y):
     for k, name in enumerate(("vp", "vs", "rho")):
         e = rel_l2(gg[k], go[k])
         print(f"strip-{tile_y} {name}: relL2 vs FP64 adjoint {e:.2e}")
+        assert e <= GTOL, (name, e)
+
+
+def _rect(NX=90, NY=61, NT=260, W=10):
+    """NX != NY, ragged against the 64-column strips and the 16-row blocks; two shots."""
+    def make(anomaly):
+        s = dict(NX=NX, NY=NY, NT=NT, DH=1.0, DT=1e-4, spatial_fd_order=8, damping_boundary=2, boundary_width=W,
+                 npower=4.0, fc_cpml=30.0, vmax_cpml=4000.0)
+        vp = cases.layered(NY, NX, 10, 1500.0, 4000.0)
+        rho = cases.layered(NY, NX, 10, 1800.0, 2600.0)
+        if anomaly:
+            yy, xx = np.mgrid[0:NY, 0:NX]
+            vp = np.where((xx - 52) ** 2 + (yy - 36) ** 2 <= 100, vp * 1.10, vp)
+        c = cases.Case("elastic_rect_adjoint", "2D", "elastic", s, vp.astype(np.float32).ravel(),
+                       (vp / 1.8).astype(np.float32).ravel(), rho.astype(np.float32).ravel())
+        y = W + 4
+        xr = np.arange(W + 2, NX - W - 2, 2)
+        rec = np.concatenate([xr + y * NX, xr + y * NX])
+        rty = np.concatenate([np.full(len(xr), 2), np.full(len(xr), 3)]).astype(np.int32)
+        rty[::5] = 1
+        w = wavesim.ricker(NT, 1e-4, 30.0, 1.0)
+        for sx in (30, 65):
+            c.shots.append(cases.Shot(np.array([sx + y * NX]), np.array([3]), w[None, :], rec, rty))
+        return c
+
+    c, c0 = make(True), make(False)
+    p = oracle_params(c)
+    obs = [po.run(p, c0.m0, c0.m1, c0.m2, sh.src_index, sh.src_type, sh.src_signal, sh.rec_index, sh.rec_type,
+                  "f64")[0] for sh in c0.shots]
+    return c, obs
+
+
+@pytest.mark.parametrize("store", [-1.0, 0.0])
+def test_gradient_rectangular_grid(store):
+    """A rectangular grid, from the per-step record (store -1) and from checkpoints + recomputation (store 0)."""
+    c, obs = _rect()
+    _, gg, _, resid = _gpu_gradient(c, obs, every=40, fp16=True, store=store, path="strip", tile_y=16)
+    _, go = _oracle_gradient(c, obs, resid=resid)
+    for k, name in enumerate(("vp", "vs", "rho")):
+        e = rel_l2(gg[k], go[k])
+        print(f"rectangular store={store} {name}: relL2 vs FP64 adjoint {e:.2e}")
         assert e <= GTOL, (name, e)
 
 
