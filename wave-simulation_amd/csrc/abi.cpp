@@ -413,17 +413,15 @@ int wv_init(wv_solver *s, const wv_config *c)
     if (k.damping_boundary == 2) {
         const int W = k.boundary_width;
         std::vector<float> a, b, ah, bh;
-        std::vector<int> idx;
+        std::vector<int> idx;  // frame indices: the kernels compute them (frame_index)
         wv::cpml_axis(k.NX, NXp, W, false, k.npower, k.fc_cpml, k.vmax_cpml, k.DT, k.DH, a, b, ah, bh, idx);
         if ((rc = s->upload_pos(&s->cpx[0], a, 1, 0.f)) || (rc = s->upload_pos(&s->cpx[1], b, 1, 0.f)) ||
-            (rc = s->upload_pos(&s->cpx[2], ah, 1, 0.f)) || (rc = s->upload_pos(&s->cpx[3], bh, 1, 0.f)) ||
-            (rc = s->upload_pos(&s->xi, idx, 1, -1)))
+            (rc = s->upload_pos(&s->cpx[2], ah, 1, 0.f)) || (rc = s->upload_pos(&s->cpx[3], bh, 1, 0.f)))
             return rc;
         wv::cpml_axis(k.NY, NYp, W, k.free_surface != 0, k.npower, k.fc_cpml, k.vmax_cpml, k.DT, k.DH, a, b, ah, bh,
                       idx);
         if ((rc = s->upload_pos(&s->cpy[0], a, 1, 0.f)) || (rc = s->upload_pos(&s->cpy[1], b, 1, 0.f)) ||
-            (rc = s->upload_pos(&s->cpy[2], ah, 1, 0.f)) || (rc = s->upload_pos(&s->cpy[3], bh, 1, 0.f)) ||
-            (rc = s->upload_pos(&s->yi, idx, 1, -1)))
+            (rc = s->upload_pos(&s->cpy[2], ah, 1, 0.f)) || (rc = s->upload_pos(&s->cpy[3], bh, 1, 0.f)))
             return rc;
         s->psx_n = (size_t)k.nshots * k.NY * 2 * W;
         s->psy_n = (size_t)k.nshots * 2 * W * k.NX;

@@ -191,7 +191,7 @@ __global__ __launch_bounds__(NTH) void el2d_fused(Layout2D L, Fields2D F, Fields
     int xi = -1;
     float ax = 0.f, bxx = 0.f, axh = 0.f, bxh = 0.f;
     if (P.on && xin) {
-        xi = P.xi[xv];
+        xi = frame_index(xv, L.NX, P.W, false);
         if (xi >= 0) {
             ax = P.ax[xv];
             bxx = P.bx[xv];

@@ -68,7 +68,6 @@ struct wv_solver {
     float *damp = nullptr, *fsH = nullptr, *fsV = nullptr;
     float *tap[6] = {};  // position-0 pointers: xf xb yf yb yf_v yb_v
     float *cpx[4] = {}, *cpy[4] = {};
-    int *xi = nullptr, *yi = nullptr;
     float *psv[2][4] = {};  // ping-pong: velocity-derivative memory variables psx0 psx1 psy0 psy1
     float *pss[4] = {};     // stress-derivative memory variables psx2 psx3 psy2 psy3
     size_t psx_n = 0, psy_n = 0;
@@ -192,8 +191,6 @@ struct wv_solver {
         P.by = cpy[1];
         P.ayh = cpy[2];
         P.byh = cpy[3];
-        P.xi = xi;
-        P.yi = yi;
         P.psx[0] = psv[b][0];
         P.psx[1] = psv[b][1];
         P.psy[0] = psv[b][2];

@@ -56,16 +56,15 @@ __host__ __device__ inline int frame_index(int p, int n, int W, bool skip_near)
     return n - 1 - p < W ? 2 * W - n + p : -1;
 }
 
-// CPML coefficients along each axis (zero outside the frame) and the compact
-// frame index of each position (-1 outside).  Memory variables of the x frame
-// are stored [shot][NY][2W], of the y frame [shot][2W][NX].
+// CPML coefficients along each axis (zero outside the frame); the compact frame
+// index of a position is frame_index(), computed in the kernels.  Memory
+// variables of the x frame are stored [shot][NY][2W], of the y frame
+// [shot][2W][NX].
 struct Cpml2D {
     int on;
     int W;
     const float *ax, *bx, *axh, *bxh;  // [NXpad]
     const float *ay, *by, *ayh, *byh;  // [NYpad]
-    const int *xi;                     // [NXpad] compact x-frame column or -1
-    const int *yi;                     // [NYpad] compact y-frame row or -1
     float *psx[4];                     // x-frame memory variables
     float *psy[4];                     // y-frame memory variables
 };
