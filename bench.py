@@ -70,28 +70,30 @@ def _kernel_stats(sol, L, wavesim, kinds=(0, 1, 2)):
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "el2d_strip_c2_1000x1000_hbm_bytes.md")
+TRAFFIC_BATCHED = os.path.join(ROOT, "profiles", "el2d_strip_batched8_1000x1000_hbm_bytes.md")
+TRAFFIC_3D = os.path.join(ROOT, "profiles", "el3d_c5_320cube_8shots_hbm_bytes.md")
 
 
-def _traffic(kernel):
-    """HBM bytes per launch of `kernel` from the committed FETCH_SIZE/WRITE_SIZE summary (calibrated), or None."""
+def _traffic(kernel, path=TRAFFIC_FILE):
+    """HBM bytes per launch of `kernel` from a committed FETCH_SIZE/WRITE_SIZE summary (calibrated), or None."""
     try:
-        for line in open(TRAFFIC_FILE):
+        for line in open(path):
             cells = [c.strip() for c in line.strip().strip("|").split("|")]
             if len(cells) >= 6 and cells[0] == kernel:
-                return float(cells[5]) * 1e6
+                return float(cells[5].replace(",", "")) * 1e6
     except OSError:
         pass
     return None
 
 
-def _roofline(kernels, measured_traffic=False):
-    """Roofline of the dominant kernel; `traffic` from the committed counter summary, which covers the
-    headline workload only (null for the other lines)."""
+def _roofline(kernels, measured_traffic=False, traffic_file=TRAFFIC_FILE):
+    """Roofline of the dominant kernel; `traffic` from the committed counter summary of the same workload and
+    kernel variant (null for a line or variant that has none)."""
     dom = max(kernels, key=lambda k: k["avg_ms"] * k["launches"])
     achieved = dom["bytes"] / (dom["avg_ms"] * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": _traffic(dom["name"]) if measured_traffic else None, "kernel": dom["name"],
+            "traffic": _traffic(dom["name"], traffic_file) if measured_traffic else None, "kernel": dom["name"],
             "kernel_avg_us": round(dom["avg_ms"] * 1e3, 3), "bytes_per_launch": dom["bytes"]}
 
 
@@ -225,7 +227,8 @@ def bench_batched_elastic(wavesim, L, nshots=8):
     return {"workload": f"2D elastic order 8, 1000x1000, 4000 steps, {nshots} shots batched in one handle",
             "kernel": choice, "grid_cell_updates_per_s": round(case.cells * NT * nshots / (ms * 1e-3), 1),
             "shots_per_s": round(nshots / (ms * 1e-3), 3), "ms_per_step": round(ms / NT, 4),
-            "roofline": _roofline([k for k in kern[:2] if k["launches"]])}
+            "roofline": _roofline([k for k in kern[:2] if k["launches"]], measured_traffic=choice == "strip-16",
+                                  traffic_file=TRAFFIC_BATCHED)}
 
 
 def bench_3d(wavesim, L):
@@ -250,7 +253,8 @@ def bench_3d(wavesim, L):
     return {"workload": "3D elastic order 8, 320^3, dx 2 m, 24-cell CPML, 8 shots batched (4320 receivers each); "
                         f"{steps} of 2500 steps timed",
             "kernel": choice, "grid_cell_updates_per_s": round(case.cells * steps * n / (ms * 1e-3), 1),
-            "ms_per_step": round(ms / steps, 3), "roofline": _roofline(kern[:2])}
+            "ms_per_step": round(ms / steps, 3),
+            "roofline": _roofline(kern[:2], measured_traffic=choice == "3d-z4", traffic_file=TRAFFIC_3D)}
 
 
 def cpu_baseline(case, nsteps):
@@ -328,7 +332,7 @@ def main():
     sol.synchronize()
     kernels = _kernel_stats(sol, L, wavesim)
     wavesim._abi.check(L.wv_set_kernel_timing(sol._h, 0))
-    roofline = _roofline(kernels, measured_traffic=True)
+    roofline = _roofline(kernels, measured_traffic=choice == "strip-16")
 
     out = {
         "metric": "grid-cell updates/s (2D elastic order-8 fused, 1000x1000, 4000 steps)",
