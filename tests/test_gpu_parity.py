@@ -233,6 +233,22 @@ def test_two_launch_tile_heights_acoustic_tmem(tile_y):
     _check_case(cases.tmem_c3(NX=150, NY=90, NT=500, nshots=2, block=10, W=10), tile_y=tile_y)
 
 
+@pytest.mark.parametrize("fs,damping", [(1, 2), (1, 1), (0, 1)])
+def test_acoustic_free_surface_and_abs(fs, damping):
+    """Acoustic free surface (image operator, pressure zero at the surface) with CPML or ABS, and ABS alone."""
+    c = cases.acoustic_c1(N=100, NT=500)
+    c.settings.update(free_surface=fs, damping_boundary=damping, damping_coeff=8.0)
+    if fs:
+        c.shots[0].src_index = np.array([50 + 3 * 100])  # three cells below the surface
+    _check_case(c)
+
+
+def test_tmem_abs():
+    c = cases.tmem_c3(NX=150, NY=90, NT=500, nshots=2, block=10, W=10)
+    c.settings.update(damping_boundary=1, damping_coeff=8.0)
+    _check_case(c)
+
+
 def test_single_pass_batched_shots_with_abs():
     c = cases.elastic_c4(N=110, NT=350, nshots=3, W=12)
     c.settings.update(damping_boundary=1, damping_coeff=8.0)
