@@ -243,6 +243,20 @@ def test_acoustic_free_surface_and_abs(fs, damping):
     _check_case(c)
 
 
+@pytest.mark.parametrize("order", [2, 4, 12])
+def test_tmem_orders(order):
+    c = cases.tmem_c3(NX=150, NY=90, NT=500, nshots=1, block=10, W=10)
+    c.settings.update(spatial_fd_order=order)
+    _check_case(c)
+
+
+@pytest.mark.parametrize("order", [8, 12])
+def test_acoustic_orders(order):
+    c = cases.acoustic_c1(N=100, NT=500)
+    c.settings.update(spatial_fd_order=order)
+    _check_case(c)
+
+
 def test_tmem_abs():
     c = cases.tmem_c3(NX=150, NY=90, NT=500, nshots=2, block=10, W=10)
     c.settings.update(damping_boundary=1, damping_coeff=8.0)
