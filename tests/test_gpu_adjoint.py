@@ -20,9 +20,11 @@ pytestmark = pytest.mark.gpu
 GTOL = 1e-3
 
 
-def _c4(N=64, NT=300, nshots=2, W=10):
+def _c4(N=64, NT=300, nshots=2, W=10, **settings):
     c = cases.elastic_c4(N=N, NT=NT, nshots=nshots, W=W)
     c0 = cases.elastic_c4(N=N, NT=NT, nshots=nshots, W=W, anomaly=False)
+    c.settings.update(settings)
+    c0.settings.update(settings)
     for sh in c.shots:
         sh.rec_type = sh.rec_type.copy()
         sh.rec_type[::5] = 1  # some P receivers
@@ -163,6 +165,18 @@ def test_gradient_rectangular_grid(store):
     for k, name in enumerate(("vp", "vs", "rho")):
         e = rel_l2(gg[k], go[k])
         print(f"rectangular store={store} {name}: relL2 vs FP64 adjoint {e:.2e}")
+        assert e <= GTOL, (name, e)
+
+
+@pytest.mark.parametrize("order,store", [(2, -1.0), (4, 0.0), (12, -1.0), (12, 0.0)])
+def test_gradient_orders(order, store):
+    """Adjoint kernels at other stencil orders, from the per-step record (store -1) or checkpoints (store 0)."""
+    c, obs = _c4(N=60, NT=240, nshots=2, spatial_fd_order=order)
+    _, gg, _, resid = _gpu_gradient(c, obs, every=40, fp16=True, store=store, path="strip", tile_y=16)
+    _, go = _oracle_gradient(c, obs, resid=resid)
+    for k, name in enumerate(("vp", "vs", "rho")):
+        e = rel_l2(gg[k], go[k])
+        print(f"order {order} store={store} {name}: relL2 vs FP64 adjoint {e:.2e}")
         assert e <= GTOL, (name, e)
 
 
