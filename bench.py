@@ -72,6 +72,7 @@ def _kernel_stats(sol, L, wavesim, kinds=(0, 1, 2)):
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "el2d_strip_c2_1000x1000_hbm_bytes.md")
 TRAFFIC_BATCHED = os.path.join(ROOT, "profiles", "el2d_strip_batched8_1000x1000_hbm_bytes.md")
 TRAFFIC_3D = os.path.join(ROOT, "profiles", "el3d_c5_320cube_8shots_hbm_bytes.md")
+TRAFFIC_TMEM = os.path.join(ROOT, "profiles", "tm2d_c3_800x400_64shots_hbm_bytes.md")
 
 
 def _traffic(kernel, path=TRAFFIC_FILE):
@@ -136,7 +137,8 @@ def bench_batched_tmem():
             "ratio_vs_plain": round(n / tb * shot_plain, 1),
             "grid_cell_updates_per_s": round(case.cells * NT * n / tb, 1),
             "plain_sample": f"{ps} steps of one shot, scaled to {NT}",
-            "roofline": _roofline([k for k in kern[:2] if k["launches"]])}
+            "roofline": _roofline([k for k in kern[:2] if k["launches"]],
+                                  measured_traffic=choice == "phased-16", traffic_file=TRAFFIC_TMEM)}
 
 
 def bench_adjoint():
