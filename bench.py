@@ -7,13 +7,17 @@ perturbations (seeds 2 / 102), one 30 Hz Ricker VY force source, 500 VX + 500
 VY receivers.  Synthetic data, FP32 arithmetic, inputs resident in HBM before
 the timed region.
 
-Printed JSON line: metric = grid-cell updates/s of the tiled path (kernel
-variant chosen by the autotuner); roofline of the dominant kernel (start/stop
-events attached to each dispatch on the solver's stream);
-the same workload on the reference-sequence GPU path ("plain", one kernel per
-LAMA operation) for the batched-vs-plain ratio; and the CPU baseline: the
-oracle (C restatement with the reference's operator sequence, OpenMP) on a
-bounded sample of the same workload, timed on this host's cores.
+Printed JSON line: value = grid-cell updates/s of the hand-written kernels
+(kernel variant chosen by the autotuner); `roofline` = algorithmic HBM bytes of
+one time step (60 B per cell-update for 2D elastic plus the CPML memory
+variables of the frame cells) over the GPU time of one time step's kernels
+(start/stop events attached to each dispatch on the solver's stream); the same
+workload on the reference-sequence GPU path ("plain", one kernel per LAMA
+operation) for the shots/s ratio; `parity`: the GPU seismograms against the
+oracle's float and double runs of the whole shot; and the CPU baseline: the
+oracle (C restatement with the reference's operator sequence, OpenMP) and its
+fused-loop mode on a bounded sample of the same workload, timed on this host's
+cores (threads pinned, median of 3).
 """
 import argparse
 import json
@@ -27,8 +31,26 @@ for p in (ROOT, os.path.join(ROOT, "wave-simulation_amd")):
         sys.path.insert(0, p)
 
 HBM_PEAK_GBS = 8000.0
-# algorithmic bytes per 2D elastic cell-update: 5 fields read + written, 5 coefficients read (FP32)
-BYTES_PER_CELL_UPDATE = 4 * (2 * 5 + 5)
+METRIC = "grid-cell updates/s (% HBM peak); batched+adjoint shots/s vs plain at 1e-5"
+
+# Algorithmic HBM bytes of one time step (FP32; every field read and written once, every
+# coefficient read once and shared by the B shots of a batch; CPML memory variables read
+# and written on the frame cells only, 8 B each per axis):
+#   2D acoustic 4 (2*3 + 3/B) per cell + 2 memory variables per frame cell and axis
+#   2D elastic  4 (2*5 + 5/B)            + 4
+#   2D TM-EM    4 (2*3 + 4/B)            + 2
+#   3D elastic  4 (2*9 + 8/B)            + 6
+PHYS_BYTES = {"acoustic": (3, 3, 2), "elastic2d": (5, 5, 4), "tmem": (3, 4, 2), "elastic3d": (9, 8, 6)}
+
+
+def alg_bytes_per_step(kind, NX, NY, NZ=1, B=1, W=0):
+    nf, nc, npsi = PHYS_BYTES[kind]
+    cells = NX * NY * NZ
+    if NZ > 1:
+        frame = 2 * W * (NY * NZ + NX * NZ + NX * NY)
+    else:
+        frame = 2 * W * (NY + NX)
+    return B * cells * 4 * (2 * nf + nc / B) + B * npsi * 8 * frame
 
 
 def parse():
@@ -40,12 +62,17 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--plain-steps", type=int, default=400)
-    ap.add_argument("--cpu-steps", type=int, default=3000,
-                    help="time steps of the CPU baseline sample (about 10-15 s on 16 cores)")
+    ap.add_argument("--cpu-steps", type=int, default=1000,
+                    help="time steps of each CPU baseline sample (a few seconds on 16 cores)")
     ap.add_argument("--tile-y", type=int, default=0, help="rows per tile (0 = autotune over all kernel variants)")
     ap.add_argument("--path", default="fused", choices=["fused", "phased", "strip"])
     ap.add_argument("--line", choices=["tmem", "adjoint", "3d", "batched-elastic"],
                     help="run only this extra line and print it as JSON (for profiling; no headline metric)")
+    ap.add_argument("--variant", default="",
+                    help="with --line: kernel variant instead of the autotuner's pick (e.g. phased-16, strip-16, "
+                         "3d-z4), so a profiled run is reproducible")
+    ap.add_argument("--cpu-samples", type=int, default=3, help="CPU baseline samples (median reported)")
+    ap.add_argument("--no-parity", action="store_true", help="skip the whole-shot oracle parity runs")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the batched TM-EM (64 shots), batched elastic (8 shots), adjoint (16 shots) and 3D elastic lines")
     return ap.parse_args()
@@ -87,25 +114,53 @@ def _traffic(kernel, path=TRAFFIC_FILE):
     return None
 
 
-def _roofline(kernels, measured_traffic=False, traffic_file=TRAFFIC_FILE):
-    """Roofline of the dominant kernel; `traffic` from the committed counter summary of the same workload and
-    kernel variant (null for a line or variant that has none)."""
-    dom = max(kernels, key=lambda k: k["avg_ms"] * k["launches"])
-    achieved = dom["bytes"] / (dom["avg_ms"] * 1e-3) / 1e9
+def _step_roofline(kernels, steps, alg_bytes, traffic_file=None):
+    """Roofline of one time step: the algorithmic bytes of a step over the GPU time of the step's kernels
+    (dispatch-attached events, summed over the kernels of `steps` eager time steps); `traffic` = the HBM bytes
+    the step's kernels move, from the committed counter summary of the same workload and kernel variant (null
+    when there is none)."""
+    run = [k for k in kernels if k["launches"]]
+    step_ms = sum(k["avg_ms"] * k["launches"] for k in run) / steps
+    achieved = alg_bytes / (step_ms * 1e-3) / 1e9
+    main = [k for k in run if k["launches"] >= steps]  # the per-step kernels (not the end-of-run flush)
+    traffic = None
+    if traffic_file:
+        tr = [_traffic(k["name"], traffic_file) for k in main]
+        traffic = sum(tr) if tr and all(t is not None for t in tr) else None
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": _traffic(dom["name"], traffic_file) if measured_traffic else None, "kernel": dom["name"],
-            "kernel_avg_us": round(dom["avg_ms"] * 1e3, 3), "bytes_per_launch": dom["bytes"]}
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "kernel": " + ".join(k["name"] for k in main) + " (one time step)",
+            "step_us": round(step_ms * 1e3, 3), "alg_bytes_per_step": alg_bytes,
+            "kernel_model": [{"name": k["name"], "avg_us": round(k["avg_ms"] * 1e3, 3), "launches": k["launches"],
+                              "model_bytes_per_launch": k["bytes"]} for k in run]}
 
 
-def bench_batched_tmem():
+VARIANTS = {"phased-8": ("phased", 8), "phased-16": ("phased", 16), "phased-32": ("phased", 32),
+            "fused-8": ("fused", 8), "fused-16": ("fused", 16), "fused-24": ("fused", 24),
+            "strip-8": ("strip", 8), "strip-16": ("strip", 16), "strip-32": ("strip", 32),
+            "3d-z4": ("fused", 4), "3d-z8": ("fused", 8), "3d-z16": ("fused", 16)}
+
+
+def _solver(case, variant, tune_steps, **kw):
+    """A prepared solver on `variant` (a key of VARIANTS) or on the autotuner's pick ('')."""
+    from wavesim import cases
+
+    if variant:
+        path, ty = VARIANTS[variant]
+        kw = dict(kw, path=kw.get("path", path) if path == "fused" else path, tile_y=ty)
+        return cases.make_solver(case, **kw), variant
+    sol = cases.make_solver(case, **kw)
+    return sol, sol.tune(tune_steps)
+
+
+def bench_batched_tmem(variant="", plain_leg=True):
     """64 TM-EM shots of uneven receiver counts in one handle against the reference-sequence GPU path."""
     from wavesim import cases
+    import wavesim
 
     case = cases.tmem_c3()
     NT = case.settings["NT"]
-    sol = cases.make_solver(case, steps_per_graph=50)
-    choice = sol.tune(100)
+    sol, choice = _solver(case, variant, 100, steps_per_graph=50)
     sol.reset()
     sol.run(0, NT)
     sol.synchronize()
@@ -114,8 +169,6 @@ def bench_batched_tmem():
     sol.run(0, NT)
     sol.synchronize()
     tb = time.perf_counter() - t0
-    import wavesim
-
     L = wavesim.load()
     wavesim._abi.check(L.wv_set_kernel_timing(sol._h, 1))
     sol.reset()
@@ -123,25 +176,29 @@ def bench_batched_tmem():
     sol.synchronize()
     kern = _kernel_stats(sol, L, wavesim)
     del sol
-    plain = cases.make_solver(case, path="plain", shots=[case.shots[0]])
-    ps = 300
-    plain.run(0, 20)
-    plain.synchronize()
-    t1 = time.perf_counter()
-    plain.run(20, 20 + ps)
-    plain.synchronize()
-    shot_plain = (time.perf_counter() - t1) * NT / ps
     n = len(case.shots)
-    return {"workload": "2D TM-EM order 8, 800x400, 3000 steps, 64 shots (20-300 receivers each), one handle",
-            "kernel": choice, "shots_per_s": round(n / tb, 3), "plain_shots_per_s": round(1.0 / shot_plain, 4),
-            "ratio_vs_plain": round(n / tb * shot_plain, 1),
-            "grid_cell_updates_per_s": round(case.cells * NT * n / tb, 1),
-            "plain_sample": f"{ps} steps of one shot, scaled to {NT}",
-            "roofline": _roofline([k for k in kern[:2] if k["launches"]],
-                                  measured_traffic=choice == "phased-16", traffic_file=TRAFFIC_TMEM)}
+    s = case.settings
+    out = {"workload": "2D TM-EM order 8, 800x400, 3000 steps, 64 shots (20-300 receivers each), one handle",
+           "kernel": choice, "shots_per_s": round(n / tb, 3),
+           "grid_cell_updates_per_s": round(case.cells * NT * n / tb, 1),
+           "roofline": _step_roofline(kern, 300, alg_bytes_per_step("tmem", s["NX"], s["NY"], 1, n,
+                                                                    s["boundary_width"]),
+                                      TRAFFIC_TMEM if choice == "phased-16" else None)}
+    if plain_leg:
+        plain = cases.make_solver(case, path="plain", shots=[case.shots[0]])
+        ps = 300
+        plain.run(0, 20)
+        plain.synchronize()
+        t1 = time.perf_counter()
+        plain.run(20, 20 + ps)
+        plain.synchronize()
+        shot_plain = (time.perf_counter() - t1) * NT / ps
+        out.update({"plain_shots_per_s": round(1.0 / shot_plain, 4), "ratio_vs_plain": round(n / tb * shot_plain, 1),
+                    "plain_sample": f"{ps} steps of one shot, scaled to {NT}"})
+    return out
 
 
-def bench_adjoint():
+def bench_adjoint(variant=""):
     """Forward + adjoint gradient of 16 batched 2D elastic shots (600x600, 3000 steps), in the two memory
     plans: the FP16 record of every step (when it fits the device memory) and FP16 checkpoints every 50 steps
     with recomputation."""
@@ -153,8 +210,7 @@ def bench_adjoint():
     NT = case.settings["NT"]
     n = len(case.shots)
     # the per-step record is written by the row-strip kernels: tune among their block heights
-    sol = cases.make_solver(case, steps_per_graph=50, path="strip")
-    choice = sol.tune(100)
+    sol, choice = _solver(case, variant, 100, steps_per_graph=50, path="strip")
     sol.reset()
     sol.run(0, NT)
     sol.synchronize()
@@ -199,9 +255,9 @@ def bench_adjoint():
     return out
 
 
-def bench_batched_elastic(wavesim, L, nshots=8):
+def bench_batched_elastic(wavesim, L, nshots=8, variant=""):
     """The C2 model with 8 shots in one handle (sources 20 cells apart): the batched 2D elastic solver's
-    grid-cell updates/s and the HBM fraction of its dominant kernel (out-of-cache working set)."""
+    grid-cell updates/s and its step roofline (out-of-cache working set)."""
     import copy
 
     from wavesim import cases
@@ -214,8 +270,7 @@ def bench_batched_elastic(wavesim, L, nshots=8):
         sh = copy.deepcopy(sh0)
         sh.src_index = sh0.src_index + 20 * k
         case.shots.append(sh)
-    sol = cases.make_solver(case, steps_per_graph=50)
-    choice = sol.tune(50)
+    sol, choice = _solver(case, variant, 50, steps_per_graph=50)
     sol.run(0, 100)
     sol.synchronize()
     sol.reset()
@@ -226,21 +281,22 @@ def bench_batched_elastic(wavesim, L, nshots=8):
     sol.run(0, 200)
     sol.synchronize()
     kern = _kernel_stats(sol, L, wavesim)
+    s = case.settings
     return {"workload": f"2D elastic order 8, 1000x1000, 4000 steps, {nshots} shots batched in one handle",
             "kernel": choice, "grid_cell_updates_per_s": round(case.cells * NT * nshots / (ms * 1e-3), 1),
             "shots_per_s": round(nshots / (ms * 1e-3), 3), "ms_per_step": round(ms / NT, 4),
-            "roofline": _roofline([k for k in kern[:2] if k["launches"]], measured_traffic=choice == "strip-16",
-                                  traffic_file=TRAFFIC_BATCHED)}
+            "roofline": _step_roofline(kern, 200, alg_bytes_per_step("elastic2d", s["NX"], s["NY"], 1, nshots,
+                                                                     s["boundary_width"]),
+                                       TRAFFIC_BATCHED if choice == "strip-16" else None)}
 
 
-def bench_3d(wavesim, L):
+def bench_3d(wavesim, L, variant=""):
     """3D elastic, 320^3, 8 shots batched: 200 timed steps of the 2500-step configuration."""
     from wavesim import cases
 
     case = cases.elastic_c5()
     steps = 200
-    sol = cases.make_solver(case, steps_per_graph=50)
-    choice = sol.tune(20)
+    sol, choice = _solver(case, variant, 20, steps_per_graph=50)
     sol.run(0, 50)
     sol.synchronize()
     sol.reset()
@@ -252,35 +308,83 @@ def bench_3d(wavesim, L):
     sol.synchronize()
     kern = _kernel_stats(sol, L, wavesim)
     n = len(case.shots)
+    s = case.settings
     return {"workload": "3D elastic order 8, 320^3, dx 2 m, 24-cell CPML, 8 shots batched (4320 receivers each); "
                         f"{steps} of 2500 steps timed",
             "kernel": choice, "grid_cell_updates_per_s": round(case.cells * steps * n / (ms * 1e-3), 1),
             "ms_per_step": round(ms / steps, 3),
-            "roofline": _roofline(kern[:2], measured_traffic=choice == "3d-z4", traffic_file=TRAFFIC_3D)}
+            "roofline": _step_roofline(kern, 20, alg_bytes_per_step("elastic3d", s["NX"], s["NY"], s["NZ"], n,
+                                                                    s["boundary_width"]),
+                                       TRAFFIC_3D if choice == "3d-z4" else None)}
 
 
-def cpu_baseline(case, nsteps):
-    """Oracle (reference operator sequence, FP32, OpenMP) on a truncated run of the same workload."""
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline_leg(case, nsteps, nsamples, gpu_seis, parity=True):
+    """CPU baseline and the whole-shot parity check, both on the oracle (the only leg that runs it).
+
+    Baseline: the oracle's operator-sequence mode (the reference's run() sequence: CSR SpMV per
+    derivative, separate vector passes, CPML vectors; FP32, OpenMP) and its fused-loop mode (same
+    per-cell arithmetic, two passes per step), each timed on `nsteps` steps of the same shot with its
+    setup cancelled (a 10-step run subtracted), median of `nsamples`, threads pinned to cores.
+    Parity: the GPU seismograms against the oracle's FP32 and FP64 runs of the whole shot."""
+    import statistics
+
     import numpy as np
 
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle import pyoracle as po
-    from _cases import oracle_params
+    from _cases import oracle_params, oracle_threads
 
-    threads = min(16, os.cpu_count() or 1)
+    threads = oracle_threads()
     p = oracle_params(case, precision_threads=threads)
     sh = case.shots[0]
     args = (case.m0, case.m1, case.m2, sh.src_index, sh.src_type, sh.src_signal, sh.rec_index, sh.rec_type)
+
+    def loop_seconds(fn):
+        t0 = time.perf_counter()
+        fn(10)
+        t1 = time.perf_counter()
+        fn(10 + nsteps)
+        t2 = time.perf_counter()
+        return (t2 - t1) - (t1 - t0)
+
+    seq = statistics.median(loop_seconds(lambda te: po.run(p, *args, t_end=te)) for _ in range(nsamples))
+    fused = statistics.median(loop_seconds(lambda te: po.run_fused_elastic2d(p, *args, t_end=te))
+                              for _ in range(nsamples))
+    base = {"value": round(case.cells * nsteps / seq, 1), "unit": "grid-cell updates/s", "cores": threads,
+            "kind": "port", "cpu": _cpu_model(), "threads_pinned": os.environ.get("OMP_PROC_BIND"),
+            "sample": f"{nsteps} time steps of the same 1000x1000 order-8 CPML shot, operator-sequence mode "
+                      f"(loop only, setup subtracted), median of {nsamples}",
+            "seconds": round(seq, 3),
+            "fused_loop": {"value": round(case.cells * nsteps / fused, 1), "unit": "grid-cell updates/s",
+                           "cores": threads, "seconds": round(fused, 3),
+                           "sample": f"{nsteps} time steps, fused-loop mode, median of {nsamples}"}}
+    if not parity:
+        return base, None
+
+    def rel(a, b):
+        b = np.asarray(b, np.float64)
+        return float(np.linalg.norm(np.asarray(a, np.float64) - b) / np.linalg.norm(b))
+
     t0 = time.perf_counter()
-    po.run(p, *args, t_end=10)
-    t1 = time.perf_counter()
-    po.run(p, *args, t_end=10 + nsteps)
-    t2 = time.perf_counter()
-    loop = (t2 - t1) - (t1 - t0)  # setup and the first 10 steps cancel
-    rate = case.cells * nsteps / loop
-    return {"value": rate, "unit": "grid-cell updates/s", "cores": threads, "kind": "port",
-            "sample": f"{nsteps} time steps of the same 1000x1000 order-8 CPML shot (loop only, setup subtracted)",
-            "seconds": round(loop, 3)}
+    o32, _ = po.run(p, *args, precision="f32")
+    # FP64: the fused-loop mode (its results equal the operator-sequence mode's to the last bit)
+    o64 = po.run_fused_elastic2d(p, *args, precision="f64")
+    par = {"relL2_vs_f32_oracle": rel(gpu_seis, o32), "relL2_vs_f64_oracle": rel(gpu_seis, o64),
+           "oracle_f32_vs_f64": rel(o32, o64), "tolerance": 1e-5,
+           "sample": "the whole shot (4000 steps, 1000 traces)", "seconds": round(time.perf_counter() - t0, 1)}
+    return base, par
 
 
 def main():
@@ -296,13 +400,16 @@ def main():
 
     if args.line:
         L = wavesim.load()
-        one = {"tmem": bench_batched_tmem, "adjoint": bench_adjoint, "3d": lambda: bench_3d(wavesim, L),
-               "batched-elastic": lambda: bench_batched_elastic(wavesim, L)}
+        v = args.variant
+        one = {"tmem": lambda: bench_batched_tmem(v, plain_leg=not args.no_plain),
+               "adjoint": lambda: bench_adjoint(v), "3d": lambda: bench_3d(wavesim, L, v),
+               "batched-elastic": lambda: bench_batched_elastic(wavesim, L, variant=v)}
         print(json.dumps({args.line: one[args.line]()}))
         return
 
     case = cases.elastic_c2()
     NT = case.settings["NT"]
+    s = case.settings
     sol = cases.make_solver(case, steps_per_graph=args.graph, tile_y=args.tile_y, path=args.path)
     choice = sol.tune(200) if (args.tile_y == 0 and args.path == "fused") else f"{args.path}-{args.tile_y}"
 
@@ -319,12 +426,13 @@ def main():
     sol.synchronize()
     wall = time.perf_counter() - t0
     seis = sol.seismograms(0)
-    assert np.isfinite(seis).all() and np.abs(seis).max() > 0
+    sol.check_finite()  # Simulation.cpp:519
+    assert np.abs(seis).max() > 0
 
     updates = case.cells * NT * args.steps
     value = updates / wall
     ms_per_step = wall * 1e3 / args.steps
-    step_bytes = BYTES_PER_CELL_UPDATE * case.cells * NT
+    step_bytes = alg_bytes_per_step("elastic2d", s["NX"], s["NY"], 1, 1, s["boundary_width"])
 
     # per-kernel timing pass: eager launches with start/stop events attached to each dispatch
     L = wavesim.load()
@@ -334,10 +442,10 @@ def main():
     sol.synchronize()
     kernels = _kernel_stats(sol, L, wavesim)
     wavesim._abi.check(L.wv_set_kernel_timing(sol._h, 0))
-    roofline = _roofline(kernels, measured_traffic=choice == "strip-16")
+    roofline = _step_roofline(kernels, NT, step_bytes, TRAFFIC_FILE if choice == "strip-16" else None)
 
     out = {
-        "metric": "grid-cell updates/s (2D elastic order-8 fused, 1000x1000, 4000 steps)",
+        "metric": METRIC,
         "value": round(value, 1),
         "unit": "grid-cell updates/s",
         "n_gpus": 1,
@@ -350,11 +458,14 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": "2D elastic P-SV order 8, 1000x1000, dx 1 m, dt 1e-4 s, 4000 steps, 30-cell CPML, "
-                               "10 layers + 3% Gaussian perturbation, 1 VY Ricker 30 Hz shot, 1000 receivers",
-                   "path": args.path, "kernel": choice, "steps_per_graph": args.graph, "shots_per_step": 1},
-        "hbm_frac_step": round(step_bytes * args.steps / wall / 1e9 / HBM_PEAK_GBS, 4),
+                               "10 layers + 3% Gaussian perturbation, 1 VY Ricker 30 Hz shot, 1000 receivers "
+                               "(one benchmark step = one shot)",
+                   "path": args.path, "kernel": choice,
+                   "launches_per_time_step": 1 if choice.startswith("fused") else 2,
+                   "steps_per_graph": args.graph, "shots_per_step": 1},
+        # the step's algorithmic bytes over the whole timed loop (graphs, flushes, resets included)
+        "hbm_frac_step": round(step_bytes * NT * args.steps / wall / 1e9 / HBM_PEAK_GBS, 4),
         "gpu_event_ms_per_step": round(gpu_ms / args.steps, 3),
-        "kernels": [{k: (round(v, 5) if isinstance(v, float) else v) for k, v in d.items()} for d in kernels],
         "roofline": roofline,
     }
 
@@ -374,7 +485,7 @@ def main():
 
         del plain
     del sol
-    _progress(f"C2: {value:.4g} cell-updates/s, {ms_per_step:.1f} ms per shot")
+    _progress(f"C2: {value:.4g} cell-updates/s, {ms_per_step:.1f} ms per shot, step roofline {roofline['frac']}")
     if not args.no_extras:
         out["batched_tmem_64_shots"] = bench_batched_tmem()
         _progress(f"batched TM-EM: {out['batched_tmem_64_shots']['shots_per_s']} shots/s")
@@ -386,7 +497,12 @@ def main():
         out["elastic3d_8_shots"] = bench_3d(wavesim, L)
         _progress(f"3D: {out['elastic3d_8_shots']['grid_cell_updates_per_s']:.4g} cell-updates/s")
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(case, args.cpu_steps)
+        base, par = cpu_baseline_leg(case, args.cpu_steps, args.cpu_samples, seis, parity=not args.no_parity)
+        out["cpu_baseline"] = base
+        if par is not None:
+            out["parity"] = par
+            _progress(f"parity: relL2 vs f32 oracle {par['relL2_vs_f32_oracle']:.2e}, "
+                      f"vs f64 {par['relL2_vs_f64_oracle']:.2e}")
         _progress("cpu baseline done")
     print(json.dumps(out))
 

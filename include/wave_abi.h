@@ -40,6 +40,7 @@ extern "C" {
 #define WV_ESTATE (-2)   /* call out of order (e.g. run before init)   */
 #define WV_EDEVICE (-3)  /* HIP runtime error                           */
 #define WV_ENOMEM (-4)   /* device allocation failed                    */
+#define WV_ENONFINITE (-5) /* Inf or NaN in the wavefields or seismograms (wv_check_finite) */
 
 /* operator edge mode */
 #define WV_OPS_SPARSE 0  /* sparse matrices with edge order reduction (Derivatives.cpp:129-1234) */
@@ -143,6 +144,28 @@ int wv_kernel_stats(wv_solver *s, int kind, const char **name, double *avg_ms, l
                     double *bytes_per_launch);
 
 const char *wv_last_error(void);
+
+/* CheckParameter::checkNumericalArtefactsAndInstabilities
+ * (src/CheckParameter/CheckParameter.hpp:166-246) for a regular grid; host only, needs no
+ * GPU and no handle.  Model vectors as wv_set_model (m1 = velocityS for elastic / sh);
+ * fc_max = the largest centre frequency of the shot's sources.
+ *  - elastic: WV_EINVAL when vp/vs < sqrt(2) at a cell (the reference asserts);
+ *  - checkStabilityCriterion (:60-112): WV_EINVAL when DT > DH / (h sqrt(D) vmax), with
+ *    vmax = max vp (vs for sh, 1/sqrt(eps mu) for EM) and h = 1 for every order (the
+ *    reference's h = 7/6, 149/120, ... are integer divisions);
+ *  - checkNumericalDispersion (:121-162): *dispersion_ok = 0 when DH > vmin / (2 fc_max N),
+ *    N = 12, 8, 7, 6, 5, 4 for orders 2..12, vmin = the smallest positive vp (acoustic),
+ *    vs (elastic, sh) or EM velocity; a warning, as in the reference: the status stays
+ *    WV_OK and wv_last_error() holds the message.                                        */
+int wv_check_parameters(const char *dimension, const char *type, const wv_config *cfg, const float *m0,
+                        const float *m1, const float *m2, double fc_max, int *dispersion_ok);
+
+/* The per-shot test of Simulation.cpp:519 (WavefieldsSeismic::isFinite,
+ * SeismogramHandler::isFinite): WV_ENONFINITE, with the reference's message, when any
+ * wavefield value of any shot or any seismogram sample is Inf or NaN (the reference
+ * samples one wavefield cell and the last sample of each trace; this tests all of
+ * them).  Synchronizes the handle's stream. */
+int wv_check_finite(wv_solver *s);
 
 /* ---- adjoint-state gradient (2D elastic; CPML or no absorbing frame, no free
  * surface).  No counterpart in the reference: WAVE-Simulation has no adjoint

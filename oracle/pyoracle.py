@@ -112,6 +112,30 @@ def run(params: Params, m0, m1, m2, src_idx, src_type, src_sig, rec_idx, rec_typ
     return seis, fields
 
 
+def run_fused_elastic2d(params: Params, vp, vs, rho, src_idx, src_type, src_sig, rec_idx, rec_type, precision="f32",
+                        t_end=-1):
+    """Fused-loop CPU mode of the 2D elastic time loop (two passes per step); returns seis [nrec, NT].
+
+    Same per-cell arithmetic as run(); the CPU baseline's best-effort mode."""
+    L = lib(precision)
+    dt = dtype_of(precision)
+    real = C.c_float if precision == "f32" else C.c_double
+    a = [np.ascontiguousarray(m, dtype=dt).ravel() for m in (vp, vs, rho)]
+    src_idx = np.ascontiguousarray(src_idx, dtype=np.int64)
+    src_type = np.ascontiguousarray(src_type, dtype=np.int32)
+    src_sig = np.ascontiguousarray(src_sig, dtype=dt)
+    rec_idx = np.ascontiguousarray(rec_idx, dtype=np.int64)
+    rec_type = np.ascontiguousarray(rec_type, dtype=np.int32)
+    seis = np.zeros((len(rec_idx), params.NT), dtype=dt)
+    rc = L.oracle_run_fused_elastic2d(C.byref(params), _ptr(a[0], real), _ptr(a[1], real), _ptr(a[2], real),
+                                      C.c_int(len(src_idx)), _ptr(src_idx, C.c_long), _ptr(src_type, C.c_int),
+                                      _ptr(src_sig, real), C.c_int(len(rec_idx)), _ptr(rec_idx, C.c_long),
+                                      _ptr(rec_type, C.c_int), _ptr(seis, real), C.c_int(t_end))
+    if rc != 0:
+        raise RuntimeError(f"oracle_run_fused_elastic2d failed with {rc}")
+    return seis
+
+
 def adjoint_elastic2d(params: Params, vp, vs, rho, src_idx, src_type, src_sig, rec_idx, rec_type, obs,
                       precision="f64"):
     """2D elastic misfit J = 1/2 ||seis - obs||^2 and its adjoint-state gradient.

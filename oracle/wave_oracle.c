@@ -1214,6 +1214,130 @@ static void cpml_adjoint(long n, const real *lh, real *lg, real *mu, const real 
  * mu, mu_xy) and gmodel [3][n] = dJ/d(vp, vs, rho).  Returns 0, or -2 for an
  * unsupported configuration (free surface, ABS, not 2D elastic).
  */
+/* One forward step of the 2D elastic adjoint's forward model (the operator
+ * sequence of ForwardSolver2Delastic.cpp:117-294, as oracle_run), recording the
+ * five CPML-modified derivative sums of step t into rec5 (Gx, Gy, h4, h5, Gs)
+ * when rec5 is not NULL, and the receivers of step t into seis. */
+typedef struct {
+    long n;
+    int NT, nsrc, nrec, use_cpml;
+    const ops_t *o;
+    const real *M, *mu, *irx, *iry, *muxy;
+    const real *ax, *bx, *axh, *bxh, *ay, *by, *ayh, *byh;
+    real *vx, *vy, *sxx, *syy, *sxy, *upd, *tmp, *vxx, *vyy;
+    real *psi[8];
+    const long *src_idx, *rec_idx;
+    const int *src_type, *rec_type;
+    const real *src_sig;
+    real *seis;
+} el2d_fwd_t;
+
+static void el2d_fwd_step(el2d_fwd_t *c, int t, real *rec5)
+{
+    const long n = c->n;
+    const ops_t *o = c->o;
+    real *vx = c->vx, *vy = c->vy, *sxx = c->sxx, *syy = c->syy, *sxy = c->sxy;
+    real *upd = c->upd, *tmp = c->tmp, *vxx = c->vxx, *vyy = c->vyy, **psi = c->psi;
+    const int use_cpml = c->use_cpml;
+    real *Gx = rec5 ? rec5 + 0 * n : NULL, *Gy = rec5 ? rec5 + 1 * n : NULL;
+    real *H4 = rec5 ? rec5 + 2 * n : NULL, *H5 = rec5 ? rec5 + 3 * n : NULL, *Gs = rec5 ? rec5 + 4 * n : NULL;
+    spmv(&o->Dxf, sxx, upd);
+    if (use_cpml) cpml_apply(n, upd, psi[0], c->axh, c->bxh);
+    spmv(&o->Dyb, sxy, tmp);
+    if (use_cpml) cpml_apply(n, tmp, psi[1], c->ay, c->by);
+    v_add(n, upd, tmp);
+    if (Gx) v_copy(n, Gx, upd);
+    v_mul(n, upd, c->irx);
+    v_add(n, vx, upd);
+
+    spmv(&o->Dxb, sxy, upd);
+    if (use_cpml) cpml_apply(n, upd, psi[2], c->ax, c->bx);
+    spmv(&o->Dyf, syy, tmp);
+    if (use_cpml) cpml_apply(n, tmp, psi[3], c->ayh, c->byh);
+    v_add(n, upd, tmp);
+    if (Gy) v_copy(n, Gy, upd);
+    v_mul(n, upd, c->iry);
+    v_add(n, vy, upd);
+
+    spmv(&o->Dxb, vx, vxx);
+    spmv(&o->Dyb, vy, vyy);
+    if (use_cpml) {
+        cpml_apply(n, vxx, psi[4], c->ax, c->bx);
+        cpml_apply(n, vyy, psi[5], c->ay, c->by);
+    }
+    if (H4) v_copy(n, H4, vxx);
+    if (H5) v_copy(n, H5, vyy);
+    v_copy(n, upd, vxx);
+    v_add(n, upd, vyy);
+    v_mul(n, upd, c->M);
+    v_add(n, sxx, upd);
+    v_add(n, syy, upd);
+    v_copy(n, upd, vyy);
+    v_mul(n, upd, c->mu);
+    v_sub2(n, sxx, upd);
+    v_copy(n, upd, vxx);
+    v_mul(n, upd, c->mu);
+    v_sub2(n, syy, upd);
+
+    spmv(&o->Dyf, vx, upd);
+    if (use_cpml) cpml_apply(n, upd, psi[6], c->ayh, c->byh);
+    spmv(&o->Dxf, vy, tmp);
+    if (use_cpml) cpml_apply(n, tmp, psi[7], c->axh, c->bxh);
+    v_add(n, upd, tmp);
+    if (Gs) v_copy(n, Gs, upd);
+    v_mul(n, upd, c->muxy);
+    v_add(n, sxy, upd);
+
+    for (int ty = 1; ty <= 3; ty++)
+        for (int s = 0; s < c->nsrc; s++) {
+            if (c->src_type[s] != ty) continue;
+            real v = c->src_sig[(long)s * c->NT + t];
+            long i = c->src_idx[s];
+            if (ty == 1) {
+                sxx[i] += v;
+                syy[i] += v;
+            }
+            if (ty == 2) vx[i] += v;
+            if (ty == 3) vy[i] += v;
+        }
+    for (int r = 0; r < c->nrec; r++) {
+        long i = c->rec_idx[r];
+        int ty = c->rec_type[r];
+        real v;
+        if (ty == 1) {
+            v = sxx[i];
+            v += syy[i];
+            v *= (real)0.5;
+        } else
+            v = ty == 2 ? vx[i] : vy[i];
+        c->seis[(long)r * c->NT + t] = v;
+    }
+}
+
+/* the state a checkpoint holds: five fields and eight CPML memory variables */
+static void el2d_state_copy(el2d_fwd_t *c, real *ck, int save)
+{
+    real *f[13] = {c->vx, c->vy, c->sxx, c->syy, c->sxy, c->psi[0], c->psi[1], c->psi[2],
+                   c->psi[3], c->psi[4], c->psi[5], c->psi[6], c->psi[7]};
+    for (int k = 0; k < 13; k++) {
+        if (save)
+            memcpy(ck + (size_t)k * c->n, f[k], sizeof(real) * (size_t)c->n);
+        else
+            memcpy(f[k], ck + (size_t)k * c->n, sizeof(real) * (size_t)c->n);
+    }
+}
+
+/* segment length of the checkpointed backward pass (0: automatic) */
+static int g_adj_segment = 0;
+void oracle_adjoint_segment(int K) { g_adj_segment = K > 0 ? K : 0; }
+
+/*
+ * 2D elastic forward + adjoint.  When the per-step record of the whole run
+ * (5 x n x NT values) exceeds about 2 GiB, the forward run keeps the state every
+ * 50 steps and the backward pass recomputes each segment from its checkpoint
+ * before its adjoint steps: the same arithmetic in the same order, so the result
+ * is the same as with the full record.
+ */
 int oracle_adjoint_elastic2d(const oracle_params *p, const real *vp, const real *vs, const real *rho, int nsrc,
                              const long *src_idx, const int *src_type, const real *src_sig, int nrec,
                              const long *rec_idx, const int *rec_type, const real *obs, real *seis, double *misfit,
@@ -1254,85 +1378,20 @@ int oracle_adjoint_elastic2d(const oracle_params *p, const real *vp, const real 
     real *upd = vnew(n), *tmp = vnew(n), *vxx = vnew(n), *vyy = vnew(n);
     real *psi[8];
     for (int k = 0; k < 8; k++) psi[k] = vnew(n);
-    /* per-step factors: Gx = h0+h1, Gy = h2+h3, h4, h5, Gs = h6+h7 */
-    real *rec = (real *)calloc((size_t)5 * n * NT, sizeof(real));
-    if (!rec) return -3;
+    el2d_fwd_t fc = {n, NT, nsrc, nrec, use_cpml, &o, M, mu, irx, iry, muxy, ax, bx, axh, bxh, ay, by, ayh, byh,
+                     vx, vy, sxx, syy, sxy, upd, tmp, vxx, vyy, {psi[0], psi[1], psi[2], psi[3], psi[4], psi[5],
+                     psi[6], psi[7]}, src_idx, rec_idx, src_type, rec_type, src_sig, seis};
+    /* per-step factors: Gx = h0+h1, Gy = h2+h3, h4, h5, Gs = h6+h7 (one segment of steps) */
+    const int K = g_adj_segment > 0 ? (g_adj_segment < NT ? g_adj_segment : NT)
+                                    : ((double)5 * n * NT * sizeof(real) > 2147483648.0 ? 50 : NT);
+    const int nseg = (NT + K - 1) / K;
+    real *rec = (real *)calloc((size_t)5 * n * K, sizeof(real));
+    real *cks = nseg > 1 ? (real *)malloc(sizeof(real) * (size_t)13 * n * nseg) : NULL;
+    if (!rec || (nseg > 1 && !cks)) return -3;
 
     for (int t = 0; t < NT; t++) {
-        real *Gx = rec + ((size_t)t * 5 + 0) * n, *Gy = rec + ((size_t)t * 5 + 1) * n;
-        real *H4 = rec + ((size_t)t * 5 + 2) * n, *H5 = rec + ((size_t)t * 5 + 3) * n;
-        real *Gs = rec + ((size_t)t * 5 + 4) * n;
-        spmv(&o.Dxf, sxx, upd);
-        if (use_cpml) cpml_apply(n, upd, psi[0], axh, bxh);
-        spmv(&o.Dyb, sxy, tmp);
-        if (use_cpml) cpml_apply(n, tmp, psi[1], ay, by);
-        v_add(n, upd, tmp);
-        v_copy(n, Gx, upd);
-        v_mul(n, upd, irx);
-        v_add(n, vx, upd);
-
-        spmv(&o.Dxb, sxy, upd);
-        if (use_cpml) cpml_apply(n, upd, psi[2], ax, bx);
-        spmv(&o.Dyf, syy, tmp);
-        if (use_cpml) cpml_apply(n, tmp, psi[3], ayh, byh);
-        v_add(n, upd, tmp);
-        v_copy(n, Gy, upd);
-        v_mul(n, upd, iry);
-        v_add(n, vy, upd);
-
-        spmv(&o.Dxb, vx, vxx);
-        spmv(&o.Dyb, vy, vyy);
-        if (use_cpml) {
-            cpml_apply(n, vxx, psi[4], ax, bx);
-            cpml_apply(n, vyy, psi[5], ay, by);
-        }
-        v_copy(n, H4, vxx);
-        v_copy(n, H5, vyy);
-        v_copy(n, upd, vxx);
-        v_add(n, upd, vyy);
-        v_mul(n, upd, M);
-        v_add(n, sxx, upd);
-        v_add(n, syy, upd);
-        v_copy(n, upd, vyy);
-        v_mul(n, upd, mu);
-        v_sub2(n, sxx, upd);
-        v_copy(n, upd, vxx);
-        v_mul(n, upd, mu);
-        v_sub2(n, syy, upd);
-
-        spmv(&o.Dyf, vx, upd);
-        if (use_cpml) cpml_apply(n, upd, psi[6], ayh, byh);
-        spmv(&o.Dxf, vy, tmp);
-        if (use_cpml) cpml_apply(n, tmp, psi[7], axh, bxh);
-        v_add(n, upd, tmp);
-        v_copy(n, Gs, upd);
-        v_mul(n, upd, muxy);
-        v_add(n, sxy, upd);
-
-        for (int ty = 1; ty <= 3; ty++)
-            for (int s = 0; s < nsrc; s++) {
-                if (src_type[s] != ty) continue;
-                real v = src_sig[(long)s * NT + t];
-                long i = src_idx[s];
-                if (ty == 1) {
-                    sxx[i] += v;
-                    syy[i] += v;
-                }
-                if (ty == 2) vx[i] += v;
-                if (ty == 3) vy[i] += v;
-            }
-        for (int r = 0; r < nrec; r++) {
-            long i = rec_idx[r];
-            int ty = rec_type[r];
-            real v;
-            if (ty == 1) {
-                v = sxx[i];
-                v += syy[i];
-                v *= (real)0.5;
-            } else
-                v = ty == 2 ? vx[i] : vy[i];
-            seis[(long)r * NT + t] = v;
-        }
+        if (nseg > 1 && t % K == 0) el2d_state_copy(&fc, cks + (size_t)(t / K) * 13 * n, 1);
+        el2d_fwd_step(&fc, t, nseg > 1 ? NULL : rec + (size_t)t * 5 * n);
     }
 
     double J = 0;
@@ -1350,10 +1409,20 @@ int oracle_adjoint_elastic2d(const oracle_params *p, const real *vp, const real 
     double *G = gcoef;
     memset(G, 0, sizeof(double) * 5 * (size_t)n);
 
+    real *seis_re = nseg > 1 ? vnew((long)nrec * NT) : NULL; /* recomputed receivers (discarded) */
     for (int t = NT - 1; t >= 0; t--) {
-        const real *Gx = rec + ((size_t)t * 5 + 0) * n, *Gy = rec + ((size_t)t * 5 + 1) * n;
-        const real *H4 = rec + ((size_t)t * 5 + 2) * n, *H5 = rec + ((size_t)t * 5 + 3) * n;
-        const real *Gs = rec + ((size_t)t * 5 + 4) * n;
+        const int t0 = t / K * K;
+        if (nseg > 1 && (t == NT - 1 || t % K == K - 1)) {
+            /* recompute segment [t0, t] from its checkpoint, recording its sums */
+            el2d_state_copy(&fc, cks + (size_t)(t / K) * 13 * n, 0);
+            fc.seis = seis_re;
+            for (int q = t0; q <= t; q++) el2d_fwd_step(&fc, q, rec + (size_t)(q - t0) * 5 * n);
+            fc.seis = seis;
+        }
+        const size_t ts = (size_t)(nseg > 1 ? t - t0 : t);
+        const real *Gx = rec + (ts * 5 + 0) * n, *Gy = rec + (ts * 5 + 1) * n;
+        const real *H4 = rec + (ts * 5 + 2) * n, *H5 = rec + (ts * 5 + 3) * n;
+        const real *Gs = rec + (ts * 5 + 4) * n;
         /* receivers of step t: residual into the adjoint fields */
         for (int r = 0; r < nrec; r++) {
             long i = rec_idx[r];
@@ -1463,6 +1532,8 @@ int oracle_adjoint_elastic2d(const oracle_params *p, const real *vp, const real 
     free(gmu_tot);
 
     free(rec);
+    free(cks);
+    free(seis_re);
     real *frees[] = {M, mu, irx, iry, muxy, vx, vy, sxx, syy, sxy, upd, tmp, vxx, vyy, lvx, lvy, lsxx, lsyy, lsxy,
                      lh, lg, lh2, lg2};
     for (size_t k = 0; k < sizeof(frees) / sizeof(frees[0]); k++) free(frees[k]);
@@ -1789,5 +1860,282 @@ int oracle_adjoint_elastic3d(const oracle_params *p, const real *vp, const real 
         }
     for (int k = 0; k < 6; k++) csr_free(&DT_[k]);
     ops_free(&o);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Fused-loop CPU mode of the 2D elastic time loop (the CPU baseline's "fair  */
+/* best" next to the operator-sequence mode above).  Same arithmetic per cell */
+/* and in the same order as oracle_run (ForwardSolver2Delastic.cpp:117-294):  */
+/* each derivative sums the assembled operator row in stored order, read here */
+/* from per-position tap tables over zero-padded fields, so the results are   */
+/* those of oracle_run; but a time step is two passes over the grid (velocity, */
+/* stress), each cell's whole update done while its row is in cache.         */
+/* 2D elastic with CPML or no absorbing boundary, no free surface.            */
+/* ------------------------------------------------------------------------- */
+
+/* taps[(k)*L + pos] = coefficient of offset k - H of the row at `pos` along `axis` */
+static real *taps_from_csr(const csr_t *A, const grid_t *g, int axis, int H)
+{
+    long L = axis_len(g, axis), stride = axis == 0 ? 1 : g->NX;
+    real *t = vnew((2 * H + 1) * L);
+    for (long pos = 0; pos < L; pos++) {
+        long i = pos * stride; /* the row of this position on the first line of the other axis */
+        for (long e = A->rowptr[i]; e < A->rowptr[i + 1]; e++) {
+            long off = (A->col[e] - i) / stride;
+            t[(off + H) * L + pos] = A->val[e];
+        }
+    }
+    return t;
+}
+
+/* out[x] = sum_k t[k][x] f[x + k - H] (x derivative of one padded row) */
+__attribute__((target_clones("avx2", "default"))) static void fused_row_x(real *restrict out, const real *restrict t,
+                                                                            const real *restrict f, long NX, int K,
+                                                                            int H)
+{
+    for (long x = 0; x < NX; x++) out[x] = 0;
+    for (int k = 0; k < K; k++) {
+        const real *tk = t + k * NX, *fk = f + k - H;
+        for (long x = 0; x < NX; x++) out[x] += tk[x] * fk[x];
+    }
+}
+
+/* out[x] = sum_k t[k][y] f[x + (k - H) P] (y derivative of row y; taps of the row) */
+__attribute__((target_clones("avx2", "default"))) static void fused_row_y(real *restrict out, const real *restrict t,
+                                                                            long y, long NY, const real *restrict f,
+                                                                            long P, long NX, int K, int H)
+{
+    for (long x = 0; x < NX; x++) out[x] = 0;
+    for (int k = 0; k < K; k++) {
+        const real tk = t[k * NY + y];
+        const real *fk = f + (k - H) * P;
+        for (long x = 0; x < NX; x++) out[x] += tk * fk[x];
+    }
+}
+
+/* vx += irx (h0 + h1); vy += iry (h2 + h3) */
+__attribute__((target_clones("avx2", "default"))) static void fused_vel_update(real *restrict vx, real *restrict vy,
+                                                                                 const real *restrict h,
+                                                                                 const real *restrict irx,
+                                                                                 const real *restrict iry, long NX)
+{
+    for (long x = 0; x < NX; x++) {
+        real u = h[x] + h[NX + x];
+        u = u * irx[x];
+        vx[x] = vx[x] + u;
+        real w = h[2 * NX + x] + h[3 * NX + x];
+        w = w * iry[x];
+        vy[x] = vy[x] + w;
+    }
+}
+
+/* sxx += M (h4 + h5) - 2 mu h5; syy += M (h4 + h5) - 2 mu h4; sxy += muxy (h6 + h7) */
+__attribute__((target_clones("avx2", "default"))) static void fused_stress_update(
+    real *restrict sxx, real *restrict syy, real *restrict sxy, const real *restrict h, const real *restrict M,
+    const real *restrict mu, const real *restrict muxy, long NX)
+{
+    for (long x = 0; x < NX; x++) {
+        real u = h[x] + h[NX + x];
+        u = u * M[x];
+        real a = sxx[x] + u, b = syy[x] + u;
+        real w = h[NX + x] * mu[x];
+        a = a - (real)2.0 * w;
+        w = h[x] * mu[x];
+        b = b - (real)2.0 * w;
+        sxx[x] = a;
+        syy[x] = b;
+        real v = h[2 * NX + x] + h[3 * NX + x];
+        v = v * muxy[x];
+        sxy[x] = sxy[x] + v;
+    }
+}
+
+int oracle_run_fused_elastic2d(const oracle_params *p, const real *vp, const real *vs, const real *rho, int nsrc,
+                               const long *src_idx, const int *src_type, const real *src_sig, int nrec,
+                               const long *rec_idx, const int *rec_type, real *seis, int t_end)
+{
+#ifdef _OPENMP
+    if (p->nthreads > 0) omp_set_num_threads(p->nthreads);
+#endif
+    if (p->dim != 2 || p->physics != PH_ELASTIC || p->free_surface || p->damping == 1) return -2;
+    grid_t g = {p->NX, p->NY, 1, (long)p->NX * p->NY};
+    const long NX = g.NX, NY = g.NY, n = g.n;
+    const int NT = p->NT, H = p->order / 2, K = 2 * H + 1;
+    if (t_end < 0 || t_end > NT) t_end = NT;
+    ops_t o;
+    ops_build(&g, p, &o);
+    real *txf = taps_from_csr(&o.Dxf, &g, 0, H), *txb = taps_from_csr(&o.Dxb, &g, 0, H);
+    real *tyf = taps_from_csr(&o.Dyf, &g, 1, H), *tyb = taps_from_csr(&o.Dyb, &g, 1, H);
+    ops_free(&o);
+
+    real *M = vnew(n), *mu = vnew(n), *irx = vnew(n), *iry = vnew(n), *muxy = vnew(n);
+    for (long i = 0; i < n; i++) {
+        M[i] = rho[i] * vp[i] * vp[i];
+        mu[i] = rho[i] * vs[i] * vs[i];
+    }
+    inverse_average_2pt(&g, 0, rho, irx);
+    inverse_average_2pt(&g, 1, rho, iry);
+    shear_average(&g, mu, 0, 1, muxy);
+
+    /* CPML profiles per position (2D without free surface: x frame by column, y frame by row) */
+    const int use_cpml = p->damping == 2, W = p->width;
+    real *cx[4] = {0}, *cy[4] = {0};
+    if (use_cpml) {
+        cpml_coef_t cp;
+        cpml_layout(&g, 2, W, 0, (real)p->npower, (real)p->fc_cpml, (real)p->vmax_cpml, (real)p->DT, (real)p->DH,
+                    &cp);
+        for (int q = 0; q < 4; q++) {
+            cx[q] = vnew(NX);
+            cy[q] = vnew(NY);
+        }
+        for (long x = 0; x < NX; x++) {
+            cx[0][x] = cp.a[0][x];
+            cx[1][x] = cp.b[0][x];
+            cx[2][x] = cp.ah[0][x];
+            cx[3][x] = cp.bh[0][x];
+        }
+        for (long y = 0; y < NY; y++) {
+            cy[0][y] = cp.a[1][y * NX];
+            cy[1][y] = cp.b[1][y * NX];
+            cy[2][y] = cp.ah[1][y * NX];
+            cy[3][y] = cp.bh[1][y * NX];
+        }
+        for (int a2 = 0; a2 < 3; a2++) {
+            free(cp.a[a2]);
+            free(cp.b[a2]);
+            free(cp.ah[a2]);
+            free(cp.bh[a2]);
+        }
+    }
+    /* padded fields: H zero columns on each side, H zero rows above and below */
+    const long P = NX + 2 * H, R = NY + 2 * H;
+    real *f[5];
+    for (int k = 0; k < 5; k++) f[k] = vnew(P * R);
+    real *vx = f[0], *vy = f[1], *sxx = f[2], *syy = f[3], *sxy = f[4];
+#define AT(y, x) (((y) + H) * P + (x) + H)
+    /* memory variables: x frame [NY][2W] (terms 0 2 4 7), y frame [2W][NX] (terms 1 3 5 6) */
+    real *px[8] = {0}, *py[8] = {0};
+    if (use_cpml)
+        for (int q = 0; q < 8; q++) {
+            px[q] = vnew(NY * 2 * W);
+            py[q] = vnew(2 * W * NX);
+        }
+    /* x frame column index of x (-1 outside), y frame row index of y */
+#define XF(x) ((x) < W ? (x) : ((x) >= NX - W ? (x) - (NX - 2 * W) : -1))
+#define YF(y) ((y) < W ? (y) : ((y) >= NY - W ? (y) - (NY - 2 * W) : -1))
+#define CPML(v, ps, a, b)           \
+    do {                            \
+        real t_ = (a) * (v);        \
+        real p_ = (ps) * (b);       \
+        p_ = p_ + t_;               \
+        (ps) = p_;                  \
+        (v) = (v) + p_;             \
+    } while (0)
+
+    for (int t = 0; t < t_end; t++) {
+        /* velocity pass: per row, each derivative as k-outer / x-inner sums (vectorised
+           across the row; every cell still sums its taps in increasing offset order) */
+#pragma omp parallel
+        {
+            real *h = (real *)malloc(sizeof(real) * 4 * (size_t)NX);
+#pragma omp for schedule(static)
+            for (long y = 0; y < NY; y++) {
+                const int yf = use_cpml ? (int)YF(y) : -1;
+                fused_row_x(h + 0 * NX, txf, sxx + AT(y, 0), NX, K, H);
+                fused_row_y(h + 1 * NX, tyb, y, NY, sxy + AT(y, 0), P, NX, K, H);
+                fused_row_x(h + 2 * NX, txb, sxy + AT(y, 0), NX, K, H);
+                fused_row_y(h + 3 * NX, tyf, y, NY, syy + AT(y, 0), P, NX, K, H);
+                if (use_cpml) {
+                    for (long x = 0; x < NX; x++) {
+                        const int xf = (int)XF(x);
+                        if (xf < 0) {
+                            x = NX - W - 1;
+                            continue;
+                        }
+                        CPML(h[0 * NX + x], px[0][y * 2 * W + xf], cx[2][x], cx[3][x]);
+                        CPML(h[2 * NX + x], px[2][y * 2 * W + xf], cx[0][x], cx[1][x]);
+                    }
+                    if (yf >= 0)
+                        for (long x = 0; x < NX; x++) {
+                            CPML(h[1 * NX + x], py[1][yf * NX + x], cy[0][y], cy[1][y]);
+                            CPML(h[3 * NX + x], py[3][yf * NX + x], cy[2][y], cy[3][y]);
+                        }
+                }
+                fused_vel_update(vx + AT(y, 0), vy + AT(y, 0), h, irx + y * NX, iry + y * NX, NX);
+            }
+            free(h);
+        }
+        /* stress pass */
+#pragma omp parallel
+        {
+            real *h = (real *)malloc(sizeof(real) * 4 * (size_t)NX);
+#pragma omp for schedule(static)
+            for (long y = 0; y < NY; y++) {
+                const int yf = use_cpml ? (int)YF(y) : -1;
+                fused_row_x(h + 0 * NX, txb, vx + AT(y, 0), NX, K, H);
+                fused_row_y(h + 1 * NX, tyb, y, NY, vy + AT(y, 0), P, NX, K, H);
+                fused_row_y(h + 2 * NX, tyf, y, NY, vx + AT(y, 0), P, NX, K, H);
+                fused_row_x(h + 3 * NX, txf, vy + AT(y, 0), NX, K, H);
+                if (use_cpml) {
+                    for (long x = 0; x < NX; x++) {
+                        const int xf = (int)XF(x);
+                        if (xf < 0) {
+                            x = NX - W - 1;
+                            continue;
+                        }
+                        CPML(h[0 * NX + x], px[4][y * 2 * W + xf], cx[0][x], cx[1][x]);
+                        CPML(h[3 * NX + x], px[7][y * 2 * W + xf], cx[2][x], cx[3][x]);
+                    }
+                    if (yf >= 0)
+                        for (long x = 0; x < NX; x++) {
+                            CPML(h[1 * NX + x], py[5][yf * NX + x], cy[0][y], cy[1][y]);
+                            CPML(h[2 * NX + x], py[6][yf * NX + x], cy[2][y], cy[3][y]);
+                        }
+                }
+                fused_stress_update(sxx + AT(y, 0), syy + AT(y, 0), sxy + AT(y, 0), h, M + y * NX, mu + y * NX,
+                                    muxy + y * NX, NX);
+            }
+            free(h);
+        }
+        for (int ty = 1; ty <= 3; ty++)
+            for (int s = 0; s < nsrc; s++) {
+                if (src_type[s] != ty) continue;
+                real v = src_sig[(long)s * NT + t];
+                long i = src_idx[s], c = AT(i / NX, i % NX);
+                if (ty == 1) {
+                    sxx[c] += v;
+                    syy[c] += v;
+                }
+                if (ty == 2) vx[c] += v;
+                if (ty == 3) vy[c] += v;
+            }
+        for (int r = 0; r < nrec; r++) {
+            long i = rec_idx[r], c = AT(i / NX, i % NX);
+            int ty = rec_type[r];
+            real v;
+            if (ty == 1) {
+                v = sxx[c];
+                v += syy[c];
+                v *= (real)0.5;
+            } else
+                v = ty == 2 ? vx[c] : vy[c];
+            seis[(long)r * NT + t] = v;
+        }
+    }
+#undef AT
+#undef XF
+#undef YF
+#undef CPML
+    real *frees[] = {txf, txb, tyf, tyb, M, mu, irx, iry, muxy, f[0], f[1], f[2], f[3], f[4]};
+    for (size_t k = 0; k < sizeof(frees) / sizeof(frees[0]); k++) free(frees[k]);
+    for (int q = 0; q < 8; q++) {
+        free(px[q]);
+        free(py[q]);
+    }
+    for (int q = 0; q < 4; q++) {
+        free(cx[q]);
+        free(cy[q]);
+    }
     return 0;
 }

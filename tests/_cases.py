@@ -8,6 +8,11 @@ from oracle import pyoracle as po
 PHYS = {"acoustic": 0, "elastic": 1, "tmem": 2}
 
 
+def oracle_threads():
+    """OpenMP threads of the oracle: the host's cores, at most 16 (the share a GPU box gives one GPU)."""
+    return max(1, min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count() or 1))
+
+
 def oracle_params(case, precision_threads=None, ops_mode="sparse"):
     s = case.settings
     return po.make_params(dim=2 if case.dimension == "2D" else 3, physics=PHYS[case.type], NX=s["NX"], NY=s["NY"],
@@ -16,7 +21,7 @@ def oracle_params(case, precision_threads=None, ops_mode="sparse"):
                           damping=s.get("damping_boundary", 0), width=s.get("boundary_width", 0), DH=s["DH"],
                           DT=s["DT"], damping_coeff=s.get("damping_coeff", 8.0), npower=s.get("npower", 4.0),
                           fc_cpml=s.get("fc_cpml", 5.0), vmax_cpml=s.get("vmax_cpml", 3500.0),
-                          nthreads=precision_threads or min(8, os.cpu_count() or 1))
+                          nthreads=precision_threads or oracle_threads())
 
 
 def oracle_shot(case, shot, precision="f32", ops_mode="sparse", want_fields=False):

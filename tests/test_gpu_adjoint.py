@@ -180,6 +180,37 @@ def test_gradient_orders(order, store):
         assert e <= GTOL, (name, e)
 
 
+def test_tune_after_checkpointing_keeps_gradient():
+    """wv_tune on a handle with checkpointing on (odd interval 25, the handle starting on the
+    single-pass kernel): the tuner skips the single-pass candidates, lays the checkpoints
+    out again for the variant it keeps, and the forward seismograms and the gradient
+    still match the oracle."""
+    c, obs = _c4(N=64, NT=250, nshots=2)
+    sol = cases.make_solver(c, path="fused", tile_y=16)
+    sol.set_adjoint_store(0.0)
+    sol.set_checkpointing(25, True)
+    choice = sol.tune(20)
+    print("kept after tune:", choice)
+    assert not choice.startswith("fused")
+    sol.run(0, c.settings["NT"])
+    resid = []
+    for k, o in enumerate(obs):
+        seis = sol.seismograms(k)
+        o32, _ = po.run(oracle_params(c), c.m0, c.m1, c.m2, c.shots[k].src_index, c.shots[k].src_type,
+                        c.shots[k].src_signal, c.shots[k].rec_index, c.shots[k].rec_type, "f32")
+        assert rel_l2(seis, o32) <= 1e-5
+        r = seis.astype(np.float64) - o
+        resid.append(r.astype(np.float32).astype(np.float64))
+        sol.set_adjoint_source(k, r.astype(np.float32))
+    sol.run_adjoint()
+    gg = np.stack([sol.gradient(p) for p in ("vp", "vs", "rho")])
+    _, go = _oracle_gradient(c, obs, resid=resid)
+    for k, name in enumerate(("vp", "vs", "rho")):
+        e = rel_l2(gg[k], go[k])
+        print(f"tune after checkpointing {name}: relL2 vs FP64 adjoint {e:.2e}")
+        assert e <= GTOL, (name, e)
+
+
 def test_gradient_uneven_segments_and_stencil_operators():
     """Checkpoint interval not dividing NT, no CPML, stencil-mode operators."""
     c, obs = _c4(N=56, NT=230, nshots=1)

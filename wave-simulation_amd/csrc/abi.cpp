@@ -701,6 +701,9 @@ int wv_run(wv_solver *s, int t_start, int t_end)
                 continue;
             }
             if (!s->gexec || s->gsteps != K) {
+                // the single-pass kernel ping-pongs two buffers: a graph must hold an even step count
+                if (s->variant == V_SINGLE && (K & 1))
+                    return fail(WV_ESTATE, "graph segments of the single-pass kernel need an even step count");
                 s->drop_graph();
                 hipGraph_t g;
                 HIP_TRY(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
@@ -838,8 +841,12 @@ int wv_tune(wv_solver *s, int nsteps, int *choice)
     const int ncand = all2d ? (wv::strip2d_fits(s->L) ? 9 : 6) : 3;
     float best = 1e30f;
     const int k0 = s->cfg.path == WV_PATH_STRIP ? 6 : 0, k1 = s->cfg.path == WV_PATH_STRIP ? 9 : ncand;
-    int best_k = k0;
+    // with checkpointing on, the adjoint recomputes with a two-launch pair: the
+    // single-pass kernel (ping-pong buffers, even graph segments) is no candidate
+    const int ck_every = adjoint_every(s);
+    int best_k = ck_every && k0 == 0 ? 0 : k0;
     for (int k = k0; k < k1; k++) {
+        if (ck_every && cand_var[k] == V_SINGLE) continue;
         s->variant = cand_var[k];
         s->tile_x = s->variant == V_SINGLE ? wv::fused2d_tile_x(s->cfg.spatial_fd_order) : wv::TILE_X;
         s->tile_y = cand_ty[k];
@@ -870,6 +877,13 @@ int wv_tune(wv_solver *s, int nsteps, int *choice)
     s->cur = 0;
     s->drop_graph();
     if (choice) *choice = cand_id[best_k];
+    if (ck_every) {
+        // the checkpoint / record layout depends on the kernel variant: lay it out
+        // again for the variant kept (this also drops any earlier forward run)
+        const int fp16 = adjoint_fp16(s);
+        adjoint_invalidate_layout(s);
+        return wv_set_checkpointing(s, ck_every, fp16);
+    }
     return wv_reset(s);
 }
 

@@ -163,6 +163,16 @@ void adjoint_destroy(AdjointState *a)
 
 int adjoint_every(const wv_solver *s) { return s->adj ? s->adj->every : 0; }
 
+int adjoint_fp16(const wv_solver *s) { return s->adj && s->adj->fp16 ? 1 : 0; }
+
+void adjoint_invalidate_layout(wv_solver *s)
+{
+    if (!s->adj) return;
+    s->adj->every = 0;
+    s->adj->forward_done = s->adj->grad_ready = false;
+    s->adj->next_t = -1;
+}
+
 const wv::Rec2D *adjoint_record(const wv_solver *s)
 {
     const AdjointState *a = s->adj;
@@ -235,6 +245,8 @@ int wv_set_checkpointing(wv_solver *s, int every, int fp16)
     if (s->cfg.free_surface || s->cfg.damping_boundary == 1)
         return fail(WV_EINVAL, "the adjoint supports CPML or no absorbing boundary, without free surface");
     if (every < 0 || every > s->cfg.NT) return fail(WV_EINVAL, "checkpoint interval out of range");
+    // the adjoint kernels address one field plane with 32-bit byte offsets
+    if (!wv::strip2d_fits(s->L)) return fail(WV_EINVAL, "grid too large for the adjoint kernels (field plane of 2 GiB or more)");
     if (!s->adj) s->adj = new AdjointState();
     AdjointState *a = s->adj;
     HIP_TRY(hipStreamSynchronize(s->st));
@@ -280,7 +292,13 @@ int wv_set_checkpointing(wv_solver *s, int every, int fp16)
         const double need = (double)words * 4 + (double)strips * 4;
         size_t free_b = 0, total_b = 0;
         HIP_TRY(hipMemGetInfo(&free_b, &total_b));
-        const double budget = a->store_gib < 0 ? 0.75 * (double)free_b : a->store_gib * 1073741824.0;
+        // what wv_run_adjoint allocates later must still fit next to the record: the five
+        // adjoint fields, two sets of CPML memory variables, the FP64 gradient planes, the
+        // transposed tap tables and the time-major adjoint sources
+        const double later = (double)a->fld_n * 5 * 4 + 2.0 * (double)psi_total(s) * 4 +
+                             (double)s->L.plane * 5 * 8 + (double)s->cfg.NT * (s->rec_total + 1) * 4 + 64.0e6;
+        const double avail = std::max(0.0, (double)free_b - later);
+        const double budget = a->store_gib < 0 ? 0.75 * avail : std::min(a->store_gib * 1073741824.0, avail);
         if (need <= budget) {
             // every value is written by the forward run before the adjoint reads it: no clearing
             void *h = nullptr, *sc = nullptr;

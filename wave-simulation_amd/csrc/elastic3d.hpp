@@ -3,6 +3,7 @@
 #pragma once
 
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/wave_abi.h"
@@ -90,6 +91,8 @@ class Elastic3D {
     int nrec(int shot) const;
     double kernel_bytes(int kind) const;
     const char *kernel_name(int kind) const;
+    // device buffers of the state (wavefields of all shots, seismograms) for wv_check_finite
+    void buffers(std::vector<std::pair<const float *, long long>> &out) const;
     double kms[3] = {0, 0, 0};
     long long klaunch[3] = {0, 0, 0};
     std::string error;

@@ -85,6 +85,7 @@ struct wv_solver {
     int rec_total = 0;
     std::vector<int> rec_ptr_h;
     int *d_tbase = nullptr;
+    unsigned *d_flag = nullptr;  // wv_check_finite's result word
 
     // graph of steps_per_graph time steps (starts and ends in buffer 0)
     hipGraphExec_t gexec = nullptr;
@@ -244,6 +245,10 @@ struct wv_solver {
 void adjoint_destroy(AdjointState *a);
 // checkpoint interval (0 = checkpointing off)
 int adjoint_every(const wv_solver *s);
+// checkpoint precision of the active setup (1 = FP16)
+int adjoint_fp16(const wv_solver *s);
+// forget the checkpoint / record layout so the next wv_set_checkpointing lays it out again
+void adjoint_invalidate_layout(wv_solver *s);
 // the per-step record the row-strip kernels write during a checkpointed run, or nullptr
 const wv::Rec2D *adjoint_record(const wv_solver *s);
 // called by wv_run: at its start, before step t (t a multiple of the interval), at its end

@@ -595,6 +595,14 @@ int PlainSolver::get_seismograms(float *out)
     return 0;
 }
 
+void PlainSolver::buffers(std::vector<std::pair<const float *, long long>> &out) const
+{
+    const int nf = physics_ == ELASTIC ? 5 : 3;
+    for (int f = 0; f < nf; f++) out.emplace_back(f_[f], (long long)n_);
+    for (int c = 0; c < 5; c++)
+        if (rec_[c].n) out.emplace_back(rec_[c].seis, (long long)rec_[c].n * cfg_.NT);
+}
+
 int PlainSolver::field_io(int field, float *out, const float *in)
 {
     const int nf = physics_ == ELASTIC ? 5 : 3;

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/wave_abi.h"
@@ -40,6 +41,8 @@ class PlainSolver {
     int run(int t0, int t1);
     int get_seismograms(float *out);
     int field_io(int field, float *out, const float *in);
+    // device buffers of the state (wavefields, seismograms) for wv_check_finite
+    void buffers(std::vector<std::pair<const float *, long long>> &out) const;
 
   private:
     int physics_ = 0;

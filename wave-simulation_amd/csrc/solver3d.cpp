@@ -127,6 +127,14 @@ int Elastic3D::init(const wv_config &k, hipStream_t stream)
     L.zrows = H + round_up(NZ, 16) + H;
     L.slab = (long long)L.zrows * L.pitch;
     L.vol = (long long)(NY + 2 * H) * L.slab;
+    // the kernels address one y plane of a field and one shot's CPML frame of each
+    // axis through buffer resources with 32-bit byte offsets
+    {
+        const long long lim = 1LL << 31, w2 = damping == 2 ? 2LL * W : 0;
+        if (L.slab * 4 >= lim || (long long)NY * NZ * w2 * 4 >= lim || w2 * NZ * NX * 4 >= lim ||
+            (long long)NY * w2 * NX * 4 >= lim)
+            return fail(WV_EINVAL, "grid too large for the 3D kernels (a y plane or a CPML frame of 2 GiB or more)");
+    }
     int rc;
     for (int f = 0; f < 9; f++)
         if ((rc = dalloc(&F[f], (size_t)L.vol * nshots))) return rc;
@@ -613,6 +621,12 @@ int Elastic3D::get_seismograms(int shot, float *out)
     for (int r = 0; r < nr; r++)
         for (int t = 0; t < NT; t++) out[(size_t)r * NT + t] = h[(size_t)t * rec_total + r0 + r];
     return 0;
+}
+
+void Elastic3D::buffers(std::vector<std::pair<const float *, long long>> &out) const
+{
+    for (int f = 0; f < 9; f++) out.emplace_back(F[f], (long long)L.vol * nshots);
+    if (d_seis && !acq_dirty) out.emplace_back(d_seis, (long long)NT * rec_total);
 }
 
 int Elastic3D::field_io(int shot, int field, float *out, const float *in)

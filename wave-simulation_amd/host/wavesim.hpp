@@ -74,6 +74,9 @@ class ForwardSolver {
     void run(int t) { run(t, t + 1); }
     void run(int tStart, int tEnd);
     void synchronize();
+    // Simulation.cpp:519: throws if a wavefield value or seismogram sample is Inf or NaN
+    void checkFinite();
+    const wv_config &config() const { return cfg_; }
     double lastRunMs();
     // time every kernel variant for nsteps steps, keep the fastest (resets the wavefields)
     int tune(int nsteps);
@@ -91,5 +94,13 @@ class ForwardSolver {
 };
 
 void check(int rc);
+
+// CheckParameter::checkNumericalArtefactsAndInstabilities (CheckParameter.hpp:166-246):
+// throws when the CFL criterion or vp/vs >= sqrt(2) fails; returns the dispersion
+// warning ("" when the criterion holds), which the reference only prints
+std::string checkNumericalArtefactsAndInstabilities(const std::string &dimension, const std::string &type,
+                                                    const wv_config &cfg, const std::vector<float> &m0,
+                                                    const std::vector<float> &m1, const std::vector<float> &m2,
+                                                    double fcMax);
 
 }  // namespace wavesim

@@ -254,6 +254,18 @@ void ForwardSolver::setAcquisition(int shot, const std::vector<long long> &srcIn
 void ForwardSolver::resetCPML() { check(wv_reset(h_)); }
 void ForwardSolver::run(int tStart, int tEnd) { check(wv_run(h_, tStart, tEnd)); }
 void ForwardSolver::synchronize() { check(wv_synchronize(h_)); }
+void ForwardSolver::checkFinite() { check(wv_check_finite(h_)); }
+
+std::string checkNumericalArtefactsAndInstabilities(const std::string &dimension, const std::string &type,
+                                                    const wv_config &cfg, const std::vector<float> &m0,
+                                                    const std::vector<float> &m1, const std::vector<float> &m2,
+                                                    double fcMax)
+{
+    int ok = 1;
+    check(wv_check_parameters(dimension.c_str(), type.c_str(), &cfg, m0.empty() ? nullptr : m0.data(),
+                              m1.empty() ? nullptr : m1.data(), m2.empty() ? nullptr : m2.data(), fcMax, &ok));
+    return ok ? std::string() : std::string(wv_last_error());
+}
 
 int ForwardSolver::tune(int nsteps)
 {

@@ -169,6 +169,13 @@ int main(int argc, char **argv)
             std::vector<long long> si, ri;
             std::vector<int> st, rt;
             std::vector<float> sig;
+            // per shot, as Simulation.cpp:389: CFL and vp/vs throw, dispersion is reported
+            double fcMax = 0;
+            for (auto &s : src)
+                if (s.shot == shotNumbers[k]) fcMax = std::max(fcMax, s.fc);
+            const std::string warn =
+                checkNumericalArtefactsAndInstabilities(dim, type, solver->config(), m0, m1, m2, fcMax);
+            if (!warn.empty()) std::printf("shot %d: %s\n", shotNumbers[k], warn.c_str());
             for (auto &s : src) {
                 if (s.shot != shotNumbers[k]) continue;
                 if (s.wavelet != 1 || s.shape != 1) throw std::runtime_error("only synthetic Ricker sources are generated");
@@ -195,6 +202,7 @@ int main(int argc, char **argv)
         solver->run(0, NT);
         solver->synchronize();
         const double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - w0).count();
+        solver->checkFinite();  // Simulation.cpp:519, over every shot of the batch
 
         // seismograms per shot and component (Acquisition.hpp:28-33 names)
         static const char *comp[5] = {"", "p", "vx", "vy", "vz"};

@@ -18,6 +18,7 @@ WV_PATH_FUSED = 0
 WV_PATH_PLAIN = 1
 WV_PATH_PHASED = 2
 WV_PATH_STRIP = 3
+WV_ENONFINITE = -5
 
 # every entry point of include/wave_abi.h
 EXPORTS = (
@@ -25,6 +26,7 @@ EXPORTS = (
     "wv_synchronize", "wv_get_seismograms", "wv_get_wavefield", "wv_set_wavefield", "wv_last_run_ms",
     "wv_tune", "wv_set_kernel_timing", "wv_kernel_stats", "wv_last_error", "wv_set_checkpointing",
     "wv_set_adjoint_source", "wv_run_adjoint", "wv_get_gradient", "wv_set_adjoint_store",
+    "wv_check_parameters", "wv_check_finite",
 )
 
 
@@ -78,6 +80,8 @@ def load():
         L.wv_set_adjoint_store.argtypes = [C.c_void_p, C.c_double]
         L.wv_run_adjoint.argtypes = [C.c_void_p]
         L.wv_get_gradient.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double)]
+        L.wv_check_parameters.argtypes = [C.c_char_p, C.c_char_p, C.POINTER(Config), fp, fp, fp, C.c_double, ip]
+        L.wv_check_finite.argtypes = [C.c_void_p]
         _lib = L
     return _lib
 

@@ -117,6 +117,10 @@ class ForwardSolver:
     def synchronize(self):
         _abi.check(_abi.load().wv_synchronize(self._h))
 
+    def check_finite(self):
+        """Simulation.cpp:519: raise WaveError if any wavefield value or seismogram sample is Inf/NaN."""
+        _abi.check(_abi.load().wv_check_finite(self._h))
+
     def last_run_ms(self):
         ms = C.c_double()
         _abi.check(_abi.load().wv_last_run_ms(self._h, C.byref(ms)))
@@ -168,6 +172,30 @@ class ForwardSolver:
             idx = self.FIELDS[(self.dimension, self.type)].index(idx)
         v = np.ascontiguousarray(values, dtype=np.float32).reshape(-1)
         _abi.check(_abi.load().wv_set_wavefield(self._h, shot, idx, _fp(v)))
+
+
+def check_parameters(dimension, type_, settings, m0, m1, m2, fc_max):
+    """CheckParameter::checkNumericalArtefactsAndInstabilities (CheckParameter.hpp:166-246).
+
+    `settings` holds the keys of ``init_forward_solver`` (NX, NY, NZ, DH, DT,
+    spatial_fd_order).  Raises WaveError when the CFL criterion (with the
+    reference's h = 1 for every order) or, for elastic, vp/vs >= sqrt(2) fails;
+    returns (dispersion_ok, message) for the dispersion criterion, which the
+    reference only reports.
+    """
+    L = _abi.load()
+    cfg = _abi.Config()
+    cfg.NX, cfg.NY, cfg.NZ = settings["NX"], settings["NY"], settings.get("NZ", 1)
+    cfg.DH, cfg.DT = settings["DH"], settings["DT"]
+    cfg.spatial_fd_order = settings.get("spatial_fd_order", 8)
+    n = cfg.NX * cfg.NY * (cfg.NZ if dimension.upper() == "3D" else 1)
+    arrs = [None if m is None else np.ascontiguousarray(m, dtype=np.float32).reshape(-1)[:n].copy()
+            for m in (m0, m1, m2)]
+    ptr = [(_fp(a) if a is not None else None) for a in arrs]
+    ok = C.c_int(1)
+    _abi.check(L.wv_check_parameters(dimension.upper().encode(), type_.encode(), C.byref(cfg), *ptr,
+                                     float(fc_max), C.byref(ok)))
+    return bool(ok.value), ("" if ok.value else L.wv_last_error().decode())
 
 
 def create(dimension: str, type_: str) -> ForwardSolver:
