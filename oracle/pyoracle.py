@@ -101,7 +101,7 @@ def run(params: Params, m0, m1, m2, src_idx, src_type, src_sig, rec_idx, rec_typ
     rec_idx = np.ascontiguousarray(rec_idx, dtype=np.int64)
     rec_type = np.ascontiguousarray(rec_type, dtype=np.int32)
     seis = np.zeros((len(rec_idx), params.NT), dtype=dt)
-    nf = {0: 3, 1: 5 if params.dim == 2 else 9, 2: 3}[params.physics]
+    nf = {0: 3 if params.dim == 2 else 4, 1: 5 if params.dim == 2 else 9, 2: 3}[params.physics]
     fields = np.zeros((nf, n), dtype=dt) if want_fields else None
     rc = L.oracle_run(C.byref(params), _ptr(m0, real), _ptr(m1, real), _ptr(m2, real), C.c_int(len(src_idx)),
                       _ptr(src_idx, C.c_long), _ptr(src_type, C.c_int), _ptr(src_sig, real), C.c_int(len(rec_idx)),
@@ -199,6 +199,13 @@ def adjoint_elastic3d(params: Params, vp, vs, rho, src_idx, src_type, src_sig, r
     if rc != 0:
         raise RuntimeError(f"oracle_adjoint_elastic3d failed with {rc}")
     return seis, J.value, gcoef, gmodel
+
+
+def plant_cpml_swap(psi_index, precision="f64"):
+    """Tests only: make the oracle's elastic loops (2D and 3D) use the full-grid CPML coefficients
+    where the half-grid ones belong (and the reverse) for one memory variable, given by its index
+    in the loop (2D: 0..7, 3D: 0..17); -1 restores the correct placement."""
+    lib(precision).oracle_plant_cpml_swap(C.c_int(psi_index))
 
 
 def ricker(NT, DT, fc, amp, tshift=0.0, precision="f32"):

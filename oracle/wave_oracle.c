@@ -463,6 +463,13 @@ static void cpml_profile(int W, real NPower, real fc, real vmax, real DT, real D
     }
 }
 
+/* Tests only: swap the full-grid and half-grid CPML coefficients of one elastic memory
+ * variable (its index in the 2D or 3D elastic loop, -1 = none), so a test can show that its
+ * absorption check fails with such a mistake. */
+static int g_plant_cpml = -1;
+void oracle_plant_cpml_swap(int psi_index) { g_plant_cpml = psi_index; }
+#define PLANT(k, c, swapped) (g_plant_cpml == (k) ? (swapped) : (c))
+
 typedef struct {
     real *a[3], *b[3], *ah[3], *bh[3]; /* full-grid coefficient vectors, 0 outside frame */
 } cpml_coef_t;
@@ -602,7 +609,24 @@ static void ops_build(const grid_t *g, const oracle_params *p, ops_t *o)
     int axes = p->dim == 3 ? 3 : 2;
     csr_t *F[3] = {&o->Dxf, &o->Dyf, &o->Dzf}, *B[3] = {&o->Dxb, &o->Dyb, &o->Dzb};
     for (int ax = 0; ax < axes; ax++) {
-        if (p->stencil) {
+        /* stencil == 1: every operator is the StencilMatrix one; stencil >= 2 (tests only): bit
+         * 1 + 2*ax selects it for the forward and bit 2 + 2*ax for the backward operator of axis
+         * ax, the others keep the sparse construction */
+        int fwd_st = p->stencil == 1 || (p->stencil >= 2 && ((p->stencil >> (1 + 2 * ax)) & 1));
+        int bwd_st = p->stencil == 1 || (p->stencil >= 2 && ((p->stencil >> (2 + 2 * ax)) & 1));
+        if (fwd_st != bwd_st) {
+            if (fwd_st) {
+                *F[ax] = build_stencil(g, ax, p->order, 0);
+                csr_scale(F[ax], DT / DH);
+                *B[ax] = build_backward(g, ax, p->order, DH);
+                csr_scale(B[ax], DT);
+            } else {
+                *F[ax] = build_forward(g, ax, p->order, DH);
+                csr_scale(F[ax], DT);
+                *B[ax] = build_stencil(g, ax, p->order, 1);
+                csr_scale(B[ax], DT / DH);
+            }
+        } else if (fwd_st) {
             real s = DT / DH;
             *F[ax] = build_stencil(g, ax, p->order, 0);
             *B[ax] = build_stencil(g, ax, p->order, 1);
@@ -709,7 +733,7 @@ int oracle_abs_layout(const oracle_params *p, real *out)
  * Source / receiver type: 1 = P (EZ for tmem), 2 = VX, 3 = VY, 4 = VZ.
  * src_sig is [nsrc][NT]; seis is [nrec][NT], in receiver input order.
  * fields_out (optional) receives the final wavefields:
- *   acoustic 2D: vx vy p ; elastic 2D: vx vy sxx syy sxy ; tmem: hx hy ez ;
+ *   acoustic 2D: vx vy p ; acoustic 3D: vx vy vz p ; elastic 2D: vx vy sxx syy sxy ; tmem: hx hy ez ;
  *   elastic 3D: vx vy vz sxx syy szz syz sxz sxy
  * t_end (<= NT) allows truncated runs for timing; seis columns past it stay 0.
  */
@@ -720,7 +744,7 @@ int oracle_run(const oracle_params *p, const real *m0, const real *m1, const rea
 #ifdef _OPENMP
     if (p->nthreads > 0) omp_set_num_threads(p->nthreads);
 #endif
-    if (p->dim == 3 && p->physics != PH_ELASTIC) return -2;
+    if (p->dim == 3 && p->physics == PH_TMEM) return -2;
     grid_t g = {p->NX, p->NY, p->NZ, (long)p->NX * p->NY * p->NZ};
     long n = g.n;
     int NT = p->NT;
@@ -822,7 +846,7 @@ int oracle_run(const oracle_params *p, const real *m0, const real *m1, const rea
     if (p->damping == 1) damp = abs_layout(&g, p->dim, p->width, (real)p->damping_coeff, p->free_surface);
 
     /* wavefields and scratch */
-    int nf = p->physics == PH_ELASTIC ? (p->dim == 3 ? 9 : 5) : 3;
+    int nf = p->physics == PH_ELASTIC ? (p->dim == 3 ? 9 : 5) : (p->dim == 3 ? 4 : 3);
     real *F[9];
     for (int k = 0; k < nf; k++) F[k] = vnew(n);
     real *upd = vnew(n), *tmp = vnew(n), *vxx = vnew(n), *vyy = vnew(n), *vzz = vnew(n);
@@ -837,7 +861,39 @@ int oracle_run(const oracle_params *p, const real *m0, const real *m1, const rea
     real *azh = use_cpml ? cp.ah[2] : NULL, *bzh = use_cpml ? cp.bh[2] : NULL;
 
     for (int t = 0; t < t_end; t++) {
-        if (p->physics == PH_ACOUSTIC) {
+        if (p->physics == PH_ACOUSTIC && p->dim == 3) {
+            /* ForwardSolver3Dacoustic.cpp:95-229 ; F: vx vy vz p */
+            real *vx = F[0], *vy = F[1], *vz = F[2], *pr = F[3];
+            spmv(&o.Dxf, pr, upd);
+            if (use_cpml) cpml_apply(n, upd, psi[0], axh, bxh);
+            v_mul(n, upd, irx);
+            v_add(n, vx, upd);
+            spmv(fs ? &o.DyfFS : &o.Dyf, pr, upd);
+            if (use_cpml) cpml_apply(n, upd, psi[1], ayh, byh);
+            v_mul(n, upd, iry);
+            v_add(n, vy, upd);
+            spmv(&o.Dzf, pr, upd);
+            if (use_cpml) cpml_apply(n, upd, psi[2], azh, bzh);
+            v_mul(n, upd, irz);
+            v_add(n, vz, upd);
+            spmv(&o.Dxb, vx, upd);
+            if (use_cpml) cpml_apply(n, upd, psi[3], ax, bx);
+            spmv(&o.Dyb, vy, tmp);
+            if (use_cpml) cpml_apply(n, tmp, psi[4], ay, by);
+            v_add(n, upd, tmp);
+            spmv(&o.Dzb, vz, tmp);
+            if (use_cpml) cpml_apply(n, tmp, psi[5], az, bz);
+            v_add(n, upd, tmp);
+            v_mul(n, upd, M);
+            v_add(n, pr, upd);
+            if (damp) {
+                v_mul(n, pr, damp);
+                v_mul(n, vx, damp);
+                v_mul(n, vy, damp);
+                v_mul(n, vz, damp);
+            }
+            if (fs) v_mul(n, pr, fsZero);
+        } else if (p->physics == PH_ACOUSTIC) {
             /* ForwardSolver2Dacoustic.cpp:91-194 ; F: vx vy p */
             real *vx = F[0], *vy = F[1], *pr = F[2];
             spmv(&o.Dxf, pr, upd);
@@ -865,17 +921,17 @@ int oracle_run(const oracle_params *p, const real *m0, const real *m1, const rea
             /* ForwardSolver2Delastic.cpp:117-294 ; F: vx vy sxx syy sxy */
             real *vx = F[0], *vy = F[1], *sxx = F[2], *syy = F[3], *sxy = F[4];
             spmv(&o.Dxf, sxx, upd);
-            if (use_cpml) cpml_apply(n, upd, psi[0], axh, bxh); /* sxx_x */
+            if (use_cpml) cpml_apply(n, upd, psi[0], PLANT(0, axh, ax), PLANT(0, bxh, bx)); /* sxx_x */
             spmv(fs ? &o.DybFS : &o.Dyb, sxy, tmp);
-            if (use_cpml) cpml_apply(n, tmp, psi[1], ay, by); /* sxy_y */
+            if (use_cpml) cpml_apply(n, tmp, psi[1], PLANT(1, ay, ayh), PLANT(1, by, byh)); /* sxy_y */
             v_add(n, upd, tmp);
             v_mul(n, upd, irx);
             v_add(n, vx, upd);
 
             spmv(&o.Dxb, sxy, upd);
-            if (use_cpml) cpml_apply(n, upd, psi[2], ax, bx); /* sxy_x */
+            if (use_cpml) cpml_apply(n, upd, psi[2], PLANT(2, ax, axh), PLANT(2, bx, bxh)); /* sxy_x */
             spmv(fs ? &o.DyfFS : &o.Dyf, syy, tmp);
-            if (use_cpml) cpml_apply(n, tmp, psi[3], ayh, byh); /* syy_y */
+            if (use_cpml) cpml_apply(n, tmp, psi[3], PLANT(3, ayh, ay), PLANT(3, byh, by)); /* syy_y */
             v_add(n, upd, tmp);
             v_mul(n, upd, iry);
             v_add(n, vy, upd);
@@ -883,8 +939,8 @@ int oracle_run(const oracle_params *p, const real *m0, const real *m1, const rea
             spmv(&o.Dxb, vx, vxx);
             spmv(&o.Dyb, vy, vyy);
             if (use_cpml) {
-                cpml_apply(n, vxx, psi[4], ax, bx);
-                cpml_apply(n, vyy, psi[5], ay, by);
+                cpml_apply(n, vxx, psi[4], PLANT(4, ax, axh), PLANT(4, bx, bxh));
+                cpml_apply(n, vyy, psi[5], PLANT(5, ay, ayh), PLANT(5, by, byh));
             }
             v_copy(n, upd, vxx);
             v_add(n, upd, vyy);
@@ -899,9 +955,9 @@ int oracle_run(const oracle_params *p, const real *m0, const real *m1, const rea
             v_sub2(n, syy, upd);
 
             spmv(&o.Dyf, vx, upd);
-            if (use_cpml) cpml_apply(n, upd, psi[6], ayh, byh); /* vxy */
+            if (use_cpml) cpml_apply(n, upd, psi[6], PLANT(6, ayh, ay), PLANT(6, byh, by)); /* vxy */
             spmv(&o.Dxf, vy, tmp);
-            if (use_cpml) cpml_apply(n, tmp, psi[7], axh, bxh); /* vyx */
+            if (use_cpml) cpml_apply(n, tmp, psi[7], PLANT(7, axh, ax), PLANT(7, bxh, bx)); /* vyx */
             v_add(n, upd, tmp);
             v_mul(n, upd, muxy);
             v_add(n, sxy, upd);
@@ -928,34 +984,34 @@ int oracle_run(const oracle_params *p, const real *m0, const real *m1, const rea
             real *vx = F[0], *vy = F[1], *vz = F[2], *sxx = F[3], *syy = F[4], *szz = F[5], *syz = F[6], *sxz = F[7],
                  *sxy = F[8];
             spmv(&o.Dxf, sxx, upd);
-            if (use_cpml) cpml_apply(n, upd, psi[0], axh, bxh); /* sxx_x */
+            if (use_cpml) cpml_apply(n, upd, psi[0], PLANT(0, axh, ax), PLANT(0, bxh, bx)); /* sxx_x */
             spmv(fs ? &o.DybFS : &o.Dyb, sxy, tmp);
-            if (use_cpml) cpml_apply(n, tmp, psi[1], ay, by); /* sxy_y */
+            if (use_cpml) cpml_apply(n, tmp, psi[1], PLANT(1, ay, ayh), PLANT(1, by, byh)); /* sxy_y */
             v_add(n, upd, tmp);
             spmv(&o.Dzb, sxz, tmp);
-            if (use_cpml) cpml_apply(n, tmp, psi[2], az, bz); /* sxz_z */
+            if (use_cpml) cpml_apply(n, tmp, psi[2], PLANT(2, az, azh), PLANT(2, bz, bzh)); /* sxz_z */
             v_add(n, upd, tmp);
             v_mul(n, upd, irx);
             v_add(n, vx, upd);
 
             spmv(&o.Dxb, sxy, upd);
-            if (use_cpml) cpml_apply(n, upd, psi[3], ax, bx); /* sxy_x */
+            if (use_cpml) cpml_apply(n, upd, psi[3], PLANT(3, ax, axh), PLANT(3, bx, bxh)); /* sxy_x */
             spmv(fs ? &o.DyfFS : &o.Dyf, syy, tmp);
-            if (use_cpml) cpml_apply(n, tmp, psi[4], ayh, byh); /* syy_y */
+            if (use_cpml) cpml_apply(n, tmp, psi[4], PLANT(4, ayh, ay), PLANT(4, byh, by)); /* syy_y */
             v_add(n, upd, tmp);
             spmv(&o.Dzb, syz, tmp);
-            if (use_cpml) cpml_apply(n, tmp, psi[5], az, bz); /* syz_z */
+            if (use_cpml) cpml_apply(n, tmp, psi[5], PLANT(5, az, azh), PLANT(5, bz, bzh)); /* syz_z */
             v_add(n, upd, tmp);
             v_mul(n, upd, iry);
             v_add(n, vy, upd);
 
             spmv(&o.Dxb, sxz, upd);
-            if (use_cpml) cpml_apply(n, upd, psi[6], ax, bx); /* sxz_x */
+            if (use_cpml) cpml_apply(n, upd, psi[6], PLANT(6, ax, axh), PLANT(6, bx, bxh)); /* sxz_x */
             spmv(fs ? &o.DybFS : &o.Dyb, syz, tmp);
-            if (use_cpml) cpml_apply(n, tmp, psi[7], ay, by); /* syz_y */
+            if (use_cpml) cpml_apply(n, tmp, psi[7], PLANT(7, ay, ayh), PLANT(7, by, byh)); /* syz_y */
             v_add(n, upd, tmp);
             spmv(&o.Dzf, szz, tmp);
-            if (use_cpml) cpml_apply(n, tmp, psi[8], azh, bzh); /* szz_z */
+            if (use_cpml) cpml_apply(n, tmp, psi[8], PLANT(8, azh, az), PLANT(8, bzh, bz)); /* szz_z */
             v_add(n, upd, tmp);
             v_mul(n, upd, irz);
             v_add(n, vz, upd);
@@ -964,9 +1020,9 @@ int oracle_run(const oracle_params *p, const real *m0, const real *m1, const rea
             spmv(&o.Dyb, vy, vyy);
             spmv(&o.Dzb, vz, vzz);
             if (use_cpml) {
-                cpml_apply(n, vxx, psi[9], ax, bx);
-                cpml_apply(n, vyy, psi[10], ay, by);
-                cpml_apply(n, vzz, psi[11], az, bz);
+                cpml_apply(n, vxx, psi[9], PLANT(9, ax, axh), PLANT(9, bx, bxh));
+                cpml_apply(n, vyy, psi[10], PLANT(10, ay, ayh), PLANT(10, by, byh));
+                cpml_apply(n, vzz, psi[11], PLANT(11, az, azh), PLANT(11, bz, bzh));
             }
             v_copy(n, upd, vxx);
             v_add(n, upd, vyy);
@@ -986,25 +1042,25 @@ int oracle_run(const oracle_params *p, const real *m0, const real *m1, const rea
             v_sub2(n, szz, upd);
 
             spmv(&o.Dyf, vx, upd);
-            if (use_cpml) cpml_apply(n, upd, psi[12], ayh, byh); /* vxy */
+            if (use_cpml) cpml_apply(n, upd, psi[12], PLANT(12, ayh, ay), PLANT(12, byh, by)); /* vxy */
             spmv(&o.Dxf, vy, tmp);
-            if (use_cpml) cpml_apply(n, tmp, psi[13], axh, bxh); /* vyx */
+            if (use_cpml) cpml_apply(n, tmp, psi[13], PLANT(13, axh, ax), PLANT(13, bxh, bx)); /* vyx */
             v_add(n, upd, tmp);
             v_mul(n, upd, muxy);
             v_add(n, sxy, upd);
 
             spmv(&o.Dzf, vx, upd);
-            if (use_cpml) cpml_apply(n, upd, psi[14], azh, bzh); /* vxz */
+            if (use_cpml) cpml_apply(n, upd, psi[14], PLANT(14, azh, az), PLANT(14, bzh, bz)); /* vxz */
             spmv(&o.Dxf, vz, tmp);
-            if (use_cpml) cpml_apply(n, tmp, psi[15], axh, bxh); /* vzx */
+            if (use_cpml) cpml_apply(n, tmp, psi[15], PLANT(15, axh, ax), PLANT(15, bxh, bx)); /* vzx */
             v_add(n, upd, tmp);
             v_mul(n, upd, muxz);
             v_add(n, sxz, upd);
 
             spmv(&o.Dzf, vy, upd);
-            if (use_cpml) cpml_apply(n, upd, psi[16], azh, bzh); /* vyz */
+            if (use_cpml) cpml_apply(n, upd, psi[16], PLANT(16, azh, az), PLANT(16, bzh, bz)); /* vyz */
             spmv(&o.Dyf, vz, tmp);
-            if (use_cpml) cpml_apply(n, tmp, psi[17], ayh, byh); /* vzy */
+            if (use_cpml) cpml_apply(n, tmp, psi[17], PLANT(17, ayh, ay), PLANT(17, byh, by)); /* vzy */
             v_add(n, upd, tmp);
             v_mul(n, upd, muyz);
             v_add(n, syz, upd);
@@ -1061,9 +1117,10 @@ int oracle_run(const oracle_params *p, const real *m0, const real *m1, const rea
                 real v = src_sig[(long)s * NT + t];
                 long i = src_idx[s];
                 if (p->physics == PH_ACOUSTIC) {
-                    if (ty == 1) F[2][i] += v;
+                    if (ty == 1) F[nf - 1][i] += v;
                     if (ty == 2) F[0][i] += v;
                     if (ty == 3) F[1][i] += v;
+                    if (ty == 4 && p->dim == 3) F[2][i] += v;
                 } else if (p->physics == PH_ELASTIC && p->dim == 2) {
                     if (ty == 1) {
                         F[2][i] += v;
@@ -1090,7 +1147,7 @@ int oracle_run(const oracle_params *p, const real *m0, const real *m1, const rea
             int ty = rec_type[r];
             real v = 0;
             if (p->physics == PH_ACOUSTIC) {
-                v = ty == 1 ? F[2][i] : (ty == 2 ? F[0][i] : F[1][i]);
+                v = ty == 1 ? F[nf - 1][i] : (ty == 2 ? F[0][i] : (ty == 3 ? F[1][i] : F[2][i]));
             } else if (p->physics == PH_ELASTIC && p->dim == 2) {
                 if (ty == 1) {
                     v = F[2][i];

@@ -60,9 +60,42 @@ def test_2d_elastic_fixture_sparse_operators_differ_near_surface():
     assert 0.01 < _rel(seis, ref) < 0.2
 
 
+# per-operator construction masks of the oracle (stencil >= 2): bit 1 Dxf, 2 Dxb, 3 Dyf, 4 Dyb
+# zero-padded (StencilMatrix rows), the others edge-reduced (Derivatives.cpp:129-280, 706-843)
+_ZP = {"Dxf": 2, "Dxb": 4, "Dyf": 8, "Dyb": 16}
+
+
+@pytest.mark.parametrize("sparse_ops", [("Dxf",), ("Dxb",), ("Dyf",), ("Dyb",), ("Dxf", "Dxb"), ("Dyf", "Dyb")])
+def test_2d_elastic_fixture_needs_every_operator_zero_padded(sparse_ops):
+    """Which operators make the 2D elastic fixture fail when they take the edge-reduced rows of
+    Derivatives.cpp:166-179 (the reference's default, useStencilMatrix unset): each one does, alone or
+    by axis.  Measured misfits (bound 5e-7): Dxf 1.3e-6, Dxb 3.1e-5, Dyf 6.1e-4, Dyb 5.5e-4, x pair
+    2.5e-5, y pair 2.8e-4; every operator zero-padded 9.2e-9.  So the fixture was computed with
+    operators that have no edge order reduction in either axis (the free-surface image operators,
+    which have none in either mode, are common to both)."""
+    mask = 0
+    for k, v in _ZP.items():
+        if k not in sparse_ops:
+            mask |= v
+    seis, ref = _run_ci("2D.elastic", "vy", stencil=mask)
+    assert po.ci_misfit(seis, ref) > 1e-6
+
+
 @pytest.mark.slow
 def test_3d_elastic_fixture():
     """3D, StencilMatrix order 2, free surface, ABS, two-layer model."""
     seis, ref = _run_ci("3D.elastic", "vy")
     assert po.ci_misfit(seis, ref) <= 5e-7
     assert _rel(seis, ref) <= 1e-5
+
+
+@pytest.mark.slow
+def test_3d_acoustic_fixture_zero_padded_operators():
+    """3D acoustic, order 4, ABS, two-layer model, P source / P receiver; its configuration sets no
+    useStencilMatrix either.  Like the 2D elastic one, the fixture is reproduced with zero-padded
+    operators (misfit 1.3e-8, relL2 2.2e-6) and missed with the edge-reduced ones (6.4e-5, 1.1e-2)."""
+    seis, ref = _run_ci("3D.acoustic", "p", stencil=1)
+    assert po.ci_misfit(seis, ref) <= 5e-7
+    assert _rel(seis, ref) <= 1e-5
+    seis, _ = _run_ci("3D.acoustic", "p", stencil=0)
+    assert po.ci_misfit(seis, ref) > 1e-5
