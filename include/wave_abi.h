@@ -200,8 +200,17 @@ int wv_run_adjoint(wv_solver *s);
 
 /* gradient [NY][NX] (x fastest): param 0 vp, 1 vs, 2 rho (chain rule through
  * the averaging rules of wv_set_model), 3..7 w.r.t. the coefficients
- * 1/rho_x, 1/rho_y, M, mu, mu_xy */
+ * 1/rho_x, 1/rho_y, M, mu, mu_xy.  3D elastic: [NY][NZ][NX] in grid index order,
+ * params 3..10 = 1/rho_x, 1/rho_y, 1/rho_z, M, mu, mu_xy, mu_xz, mu_yz. */
 int wv_get_gradient(wv_solver *s, int param, double *out);
+
+/* 3D elastic: the memory plan wv_set_checkpointing chose from the budget of
+ * wv_set_adjoint_store (default 3/4 of the free device memory).  The forward run keeps
+ * checkpoints every `every` steps, on the device or (*on_host = 1) in pinned host memory;
+ * the adjoint pass recomputes each segment in sub-segments of *k2 steps (*nsub of them,
+ * FP32 state saved at the start of each when *nsub > 1), recording the derivative sums
+ * of every step of one sub-segment.  No counterpart in the reference. */
+int wv_adjoint_plan(wv_solver *s, int *k2, int *nsub, int *on_host);
 
 #ifdef __cplusplus
 }

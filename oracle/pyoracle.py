@@ -21,7 +21,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 
-PHYSICS = {"acoustic": 0, "elastic": 1, "tmem": 2}
+PHYSICS = {"acoustic": 0, "elastic": 1, "tmem": 2, "sh": 3}
 
 
 class Params(C.Structure):
@@ -101,7 +101,7 @@ def run(params: Params, m0, m1, m2, src_idx, src_type, src_sig, rec_idx, rec_typ
     rec_idx = np.ascontiguousarray(rec_idx, dtype=np.int64)
     rec_type = np.ascontiguousarray(rec_type, dtype=np.int32)
     seis = np.zeros((len(rec_idx), params.NT), dtype=dt)
-    nf = {0: 3 if params.dim == 2 else 4, 1: 5 if params.dim == 2 else 9, 2: 3}[params.physics]
+    nf = {0: 3 if params.dim == 2 else 4, 1: 5 if params.dim == 2 else 9, 2: 3, 3: 3}[params.physics]
     fields = np.zeros((nf, n), dtype=dt) if want_fields else None
     rc = L.oracle_run(C.byref(params), _ptr(m0, real), _ptr(m1, real), _ptr(m2, real), C.c_int(len(src_idx)),
                       _ptr(src_idx, C.c_long), _ptr(src_type, C.c_int), _ptr(src_sig, real), C.c_int(len(rec_idx)),
@@ -370,8 +370,9 @@ def ci_case(cfg_path):
                     stencil=stencil, free_surface=cfg_num(cfg, "FreeSurface", int),
                     damping=cfg_num(cfg, "DampingBoundary", int), width=cfg_num(cfg, "BoundaryWidth", int),
                     DH=cfg_num(cfg, "DH"), DT=DT, damping_coeff=cfg_num(cfg, "DampingCoeff"),
-                    npower=cfg_num(cfg, "NPower"), fc_cpml=cfg_num(cfg, "CenterFrequencyCPML"),
-                    vmax_cpml=cfg_num(cfg, "VMaxCPML"))
+                    npower=cfg_num(cfg, "NPower", default=4.0),
+                    fc_cpml=cfg_num(cfg, "CenterFrequencyCPML", default=5.0),
+                    vmax_cpml=cfg_num(cfg, "VMaxCPML", default=3500.0))
     if cfg_num(cfg, "ModelRead", int) == 1:
         vp, vs, rho = two_layer_model(NX, NY, NZ)
     else:

@@ -427,6 +427,33 @@ static real avg4_row(const grid_t *g, const real *v, long x, long y, long z, int
     return s;
 }
 
+/* SH shear moduli (SH.cpp:413-433 with ModelparameterSeismic.cpp:421-433): mu<1 -> 1 (in
+ * place), 1/avg(1/mu) with the 2-point average of calcAverageMatrixX/Y (Modelparameter.cpp:
+ * 336-408: half of the point and of the next one along the axis, the point itself at the
+ * last one), <4 -> 0 */
+static void shear_average_2pt(const grid_t *g, real *mu, int axis, real *out)
+{
+    for (long i = 0; i < g->n; i++)
+        if (mu[i] < (real)1.0) mu[i] = (real)1.0;
+    long L = axis_len(g, axis);
+    for (long i = 0; i < g->n; i++) {
+        long x, y, z;
+        gcoord(g, i, &x, &y, &z);
+        long p = axis_pos(x, y, z, axis);
+        real s;
+        if (p + 1 < L) {
+            real h = (real)0.5;
+            s = h * ((real)1 / mu[i]);
+            s += h * ((real)1 / mu[axis_idx(g, x, y, z, axis, p + 1)]);
+        } else {
+            s = (real)1.0 * ((real)1 / mu[i]);
+        }
+        real v = (real)1 / s;
+        if (v < (real)4.0) v = 0;
+        out[i] = v;
+    }
+}
+
 /* ModelparameterSeismic.cpp:421-433: mu<1 -> 1 (in place), 1/avg(1/mu), <4 -> 0 */
 static void shear_average(const grid_t *g, real *mu, int a0, int a1, real *out)
 {
@@ -595,7 +622,7 @@ typedef struct {
     double damping_coeff, npower, fc_cpml, vmax_cpml;
 } oracle_params;
 
-enum { PH_ACOUSTIC = 0, PH_ELASTIC = 1, PH_TMEM = 2 };
+enum { PH_ACOUSTIC = 0, PH_ELASTIC = 1, PH_TMEM = 2, PH_SH = 3 };
 
 typedef struct {
     csr_t Dxf, Dxb, Dyf, Dyb, Dzf, Dzb, DyfFS, DybFS;
@@ -643,7 +670,7 @@ static void ops_build(const grid_t *g, const oracle_params *p, ops_t *o)
         o->have_fs = 1;
         o->DyfFS = build_dyf_fs(g, p->order, DH);
         csr_scale(&o->DyfFS, DT);
-        if (p->physics == PH_ELASTIC) {
+        if (p->physics == PH_ELASTIC || p->physics == PH_SH) {
             o->DybFS = build_dyb_fs(g, p->order, DH);
             csr_scale(&o->DybFS, DT);
         }
@@ -728,12 +755,14 @@ int oracle_abs_layout(const oracle_params *p, real *out)
 /*
  * Run one shot.  Model arrays by physics:
  *   acoustic: m0 = vp, m1 unused, m2 = rho
+ *   sh      : m0 unused, m1 = vs, m2 = rho
  *   elastic : m0 = vp, m1 = vs,   m2 = rho
  *   tmem    : m0 = epsilon_r, m1 = sigma, m2 = mu_r
  * Source / receiver type: 1 = P (EZ for tmem), 2 = VX, 3 = VY, 4 = VZ.
  * src_sig is [nsrc][NT]; seis is [nrec][NT], in receiver input order.
  * fields_out (optional) receives the final wavefields:
  *   acoustic 2D: vx vy p ; acoustic 3D: vx vy vz p ; elastic 2D: vx vy sxx syy sxy ; tmem: hx hy ez ;
+ *   sh: vz sxz syz ;
  *   elastic 3D: vx vy vz sxx syy szz syz sxz sxy
  * t_end (<= NT) allows truncated runs for timing; seis columns past it stay 0.
  */
@@ -744,7 +773,7 @@ int oracle_run(const oracle_params *p, const real *m0, const real *m1, const rea
 #ifdef _OPENMP
     if (p->nthreads > 0) omp_set_num_threads(p->nthreads);
 #endif
-    if (p->dim == 3 && p->physics == PH_TMEM) return -2;
+    if (p->dim == 3 && (p->physics == PH_TMEM || p->physics == PH_SH)) return -2;
     grid_t g = {p->NX, p->NY, p->NZ, (long)p->NX * p->NY * p->NZ};
     long n = g.n;
     int NT = p->NT;
@@ -781,6 +810,18 @@ int oracle_run(const oracle_params *p, const real *m0, const real *m1, const rea
                 shear_average(&g, mu, 1, 2, muyz);
             }
         }
+    } else if (p->physics == PH_SH) {
+        /* SH: m1 = vs, m2 = rho; 1/rho at the node (ModelparameterSeismic.cpp:164-172) */
+        irx = vnew(n);
+        mu = vnew(n);
+        for (long i = 0; i < n; i++) {
+            irx[i] = (real)1 / m2[i];
+            mu[i] = m2[i] * m1[i] * m1[i];
+        }
+        muxz = vnew(n);
+        muyz = vnew(n);
+        shear_average_2pt(&g, mu, 0, muxz);
+        shear_average_2pt(&g, mu, 1, muyz);
     } else {
         real eps0 = (real)8.8541878176e-12, mu0 = (real)1.2566370614e-6;
         real *eps = vnew(n), *mup = vnew(n);
@@ -846,7 +887,7 @@ int oracle_run(const oracle_params *p, const real *m0, const real *m1, const rea
     if (p->damping == 1) damp = abs_layout(&g, p->dim, p->width, (real)p->damping_coeff, p->free_surface);
 
     /* wavefields and scratch */
-    int nf = p->physics == PH_ELASTIC ? (p->dim == 3 ? 9 : 5) : (p->dim == 3 ? 4 : 3);
+    int nf = p->physics == PH_ELASTIC ? (p->dim == 3 ? 9 : 5) : (p->dim == 3 && p->physics == PH_ACOUSTIC ? 4 : 3);
     real *F[9];
     for (int k = 0; k < nf; k++) F[k] = vnew(n);
     real *upd = vnew(n), *tmp = vnew(n), *vxx = vnew(n), *vyy = vnew(n), *vzz = vnew(n);
@@ -1082,6 +1123,31 @@ int oracle_run(const oracle_params *p, const real *m0, const real *m1, const rea
                 real *all[9] = {sxx, syy, szz, sxy, sxz, syz, vx, vy, vz};
                 for (int k = 0; k < 9; k++) v_mul(n, all[k], damp);
             }
+        } else if (p->physics == PH_SH) {
+            /* ForwardSolver2Dsh.cpp:101-190 ; F: vz sxz syz */
+            real *vz = F[0], *sxz = F[1], *syz = F[2];
+            spmv(&o.Dxb, sxz, upd);
+            spmv(fs ? &o.DybFS : &o.Dyb, syz, tmp);
+            if (use_cpml) {
+                cpml_apply(n, upd, psi[0], ax, bx); /* sxz_x */
+                cpml_apply(n, tmp, psi[1], ay, by); /* syz_y */
+            }
+            v_add(n, upd, tmp);
+            v_mul(n, upd, irx);
+            v_add(n, vz, upd);
+            spmv(&o.Dxf, vz, upd);
+            if (use_cpml) cpml_apply(n, upd, psi[2], axh, bxh); /* vzx */
+            v_mul(n, upd, muxz);
+            v_add(n, sxz, upd);
+            spmv(&o.Dyf, vz, upd);
+            if (use_cpml) cpml_apply(n, upd, psi[3], ayh, byh); /* vzy */
+            v_mul(n, upd, muyz);
+            v_add(n, syz, upd);
+            if (damp) {
+                v_mul(n, sxz, damp);
+                v_mul(n, syz, damp);
+                v_mul(n, vz, damp);
+            }
         } else {
             /* ForwardSolver2Dtmem.cpp:108-171 ; F: hx hy ez */
             real *hx = F[0], *hy = F[1], *ez = F[2];
@@ -1137,6 +1203,8 @@ int oracle_run(const oracle_params *p, const real *m0, const real *m1, const rea
                     if (ty == 2) F[0][i] += v;
                     if (ty == 3) F[1][i] += v;
                     if (ty == 4) F[2][i] += v;
+                } else if (p->physics == PH_SH) {
+                    if (ty == 4) F[0][i] += v; /* only VZ sources in SH (FDTD2Dsh.hpp:25-27) */
                 } else {
                     if (ty == 1) F[2][i] += v;
                 }
@@ -1163,6 +1231,8 @@ int oracle_run(const oracle_params *p, const real *m0, const real *m1, const rea
                     v /= (real)3;
                 } else
                     v = ty == 2 ? F[0][i] : (ty == 3 ? F[1][i] : F[2][i]);
+            } else if (p->physics == PH_SH) {
+                v = F[0][i];
             } else {
                 v = F[2][i];
             }

@@ -24,8 +24,15 @@ def _write_vec(path, v):
         np.savetxt(f, np.asarray(v, np.float64), fmt="%.9g")
 
 
-@pytest.mark.parametrize("name,comp,extra", [("2D.acoustic", "p", []), ("2D.elastic", "vy", ["useStencilMatrix=1"]),
-                                             ("3D.elastic", "vy", [])])
+@pytest.mark.parametrize("name,comp,extra", [
+    ("2D.acoustic", "p", []),
+    ("2D.elastic", "vy", ["useStencilMatrix=1"]),
+    pytest.param("2D.elastic", "vy", [], marks=pytest.mark.skip(
+        reason="the unmodified configuration selects the sparse operators with edge order reduction "
+               "(Derivatives.cpp:26-30, 166-179); they miss this fixture (misfit 2.8e-4 > 5e-7), which matches "
+               "zero-padded operators only (test_oracle_golden.py, per-operator hypotheses)")),
+    ("3D.elastic", "vy", []),
+])
 def test_cli_reproduces_golden_seismogram(tmp_path, name, comp, extra):
     cfg_path = os.path.join(CI, f"configuration_ci.{name}.txt")
     cfg = po.read_config(cfg_path)

@@ -1,0 +1,218 @@
+"""GPU parity at the full size of every benchmark configuration.
+
+The other GPU tests run reduced grids; these run each benchmark configuration
+at its stated grid, step count and shot count, through the C ABI, and compare
+with the oracle on the same seeded inputs:
+
+  C1  2D acoustic, order 4, 200 x 200, 2000 steps             vs f32 oracle (assert 1e-5), f64 reported
+  C2  2D elastic, order 8, 1000 x 1000, 4000 steps            vs f32 oracle (assert 1e-5), f64 reported,
+                                                              through the row-strip, two-launch and
+                                                              single-pass kernels
+  C3  2D TM-EM, order 8, 800 x 400, 3000 steps, 64 shots      one batched launch; 8 shots vs f32 oracle
+  C4  2D elastic adjoint, 600 x 600, 3000 steps, 16 shots     gradient of two shots' misfit vs the FP64
+                                                              oracle adjoint (full FP64 pipeline), in both
+                                                              memory plans (per-step FP16 record; FP16
+                                                              checkpoints every 50 steps)
+  C5  3D elastic, order 8, 320^3, 8 shots                     batched == single-shot bit-exact over 300
+                                                              steps; one shot vs f32 oracle over the first
+                                                              120 steps, its Ricker moved 88 ms earlier so
+                                                              the pulse (peak at step 60) lies inside the
+                                                              window (the oracle's 3D loop takes about a
+                                                              second per step at this size)
+
+Tolerances (north star): relative L2 seismogram error <= 1e-5 against the
+oracle's float build; gradient relative error <= 1e-3 against FP64.  The
+FP64 seismogram error is asserted below 1e-4 (it is reported, not a target).
+
+The oracle runs are the long part (minutes of CPU); they are all started in a
+background pool (two at a time, half the host's cores each; ctypes releases
+the GIL) when the first test of this file asks for them, so they overlap each
+other and the GPU work.  The C2 references use the oracle's fused-loop mode,
+which equals its operator-sequence mode to the last bit (test_oracle_fused).
+"""
+import concurrent.futures as cf
+import time
+
+import numpy as np
+import pytest
+
+from wavesim import cases
+from oracle import pyoracle as po
+
+from _cases import oracle_params, oracle_threads, rel_l2
+
+pytestmark = [pytest.mark.gpu, pytest.mark.slow]
+
+TOL = 1e-5
+GTOL = 1e-3
+TOL64 = 1e-4
+C3_SHOTS = (0, 9, 18, 27, 36, 45, 54, 63)  # 8 shots spread over the batch
+C3_SHOTS64 = (0, 63)
+C5_NT, C5_WIN = 300, 120
+
+
+def _c5_window_case():
+    c = cases.elastic_c5(NT=C5_WIN, nshots=1)
+    DT = c.settings["DT"]
+    sig = po.ricker(C5_WIN, DT, 15.0, 1.0, -(1.5 / 15.0 - 60 * DT), "f32")
+    sh = c.shots[0]
+    c.shots[0] = cases.Shot(sh.src_index, sh.src_type, sig[None, :].astype(np.float32), sh.rec_index, sh.rec_type)
+    return c
+
+
+def _shot_ref(case, shot, precision, nthreads, fused=False):
+    p = oracle_params(case, precision_threads=nthreads)
+    dt = np.float32 if precision == "f32" else np.float64
+    m1 = case.m1 if case.m1 is not None else np.zeros_like(case.m0)
+    args = (case.m0.astype(dt), m1.astype(dt), case.m2.astype(dt), shot.src_index, shot.src_type,
+            shot.src_signal.astype(dt), shot.rec_index, shot.rec_type)
+    if fused:
+        return po.run_fused_elastic2d(p, *args, precision=precision)
+    return po.run(p, *args, precision)[0]
+
+
+def _c4_ref(k, nthreads):
+    """Observed data (FP64 oracle, background model) and the FP64 oracle adjoint gradient of shot k's misfit."""
+    c = cases.elastic_c4()
+    c0 = cases.elastic_c4(anomaly=False)
+    p = oracle_params(c, precision_threads=nthreads)
+    sh0, sh = c0.shots[k], c.shots[k]
+    o, _ = po.run(p, c0.m0, c0.m1, c0.m2, sh0.src_index, sh0.src_type, sh0.src_signal, sh0.rec_index, sh0.rec_type,
+                  "f64")
+    _, _, _, g = po.adjoint_elastic2d(p, c.m0, c.m1, c.m2, sh.src_index, sh.src_type, sh.src_signal, sh.rec_index,
+                                      sh.rec_type, o)
+    return o, g
+
+
+@pytest.fixture(scope="module")
+def refs():
+    """Every oracle reference of this file, computed in a background pool; values are futures."""
+    nt = max(1, oracle_threads() // 2)
+    pool = cf.ThreadPoolExecutor(max_workers=2)
+    c1, c2, c3 = cases.acoustic_c1(), cases.elastic_c2(), cases.tmem_c3()
+    jobs = {}
+    # longest first, so the pool's tail is short
+    for k in (0, 1):
+        jobs[("c4", k)] = pool.submit(_c4_ref, k, nt)
+    w = _c5_window_case()
+    jobs["c5"] = pool.submit(_shot_ref, w, w.shots[0], "f32", nt)
+    for prec in ("f32", "f64"):
+        jobs[("c1", prec)] = pool.submit(_shot_ref, c1, c1.shots[0], prec, nt)
+        jobs[("c2", prec)] = pool.submit(_shot_ref, c2, c2.shots[0], prec, nt, True)
+    for k in C3_SHOTS:
+        jobs[("c3", k, "f32")] = pool.submit(_shot_ref, c3, c3.shots[k], "f32", nt)
+    for k in C3_SHOTS64:
+        jobs[("c3", k, "f64")] = pool.submit(_shot_ref, c3, c3.shots[k], "f64", nt)
+    yield jobs
+    pool.shutdown(wait=False, cancel_futures=True)
+
+
+def _get(refs, key):
+    t0 = time.time()
+    r = refs[key].result()
+    print(f"  [oracle {key}: waited {time.time() - t0:.1f} s]", flush=True)
+    return r
+
+
+def _check(tag, gpu, o32, o64=None):
+    e32 = rel_l2(gpu, o32)
+    msg = f"{tag}: relL2 vs f32 oracle {e32:.2e}"
+    e64 = None
+    if o64 is not None:
+        e64 = rel_l2(gpu, o64)
+        msg += f", vs f64 oracle {e64:.2e} (oracle f32 vs f64 {rel_l2(o32, o64):.2e})"
+    print(msg, flush=True)
+    assert np.abs(o32).max() > 0
+    assert e32 <= TOL, (tag, e32)
+    if e64 is not None:
+        assert e64 <= TOL64, (tag, e64)
+
+
+def test_c1_acoustic_full_size(refs):
+    case = cases.acoustic_c1()
+    assert (case.settings["NX"], case.settings["NY"], case.settings["NT"]) == (200, 200, 2000)
+    sol = cases.make_solver(case, steps_per_graph=50)
+    print("C1 kernel:", sol.tune(100), flush=True)
+    sol.run(0, case.settings["NT"])
+    sol.check_finite()
+    _check("C1 200x200x2000", sol.seismograms(0), _get(refs, ("c1", "f32")), _get(refs, ("c1", "f64")))
+
+
+def test_c2_elastic_full_size_every_kernel_family(refs):
+    case = cases.elastic_c2()
+    assert (case.settings["NX"], case.settings["NY"], case.settings["NT"]) == (1000, 1000, 4000)
+    runs = {}
+    for label, kw in (("strip-16", dict(path="strip", tile_y=16)), ("phased-8", dict(path="phased", tile_y=8)),
+                      ("single-pass-16", dict(path="fused", tile_y=16))):
+        sol = cases.make_solver(case, steps_per_graph=50, **kw)
+        sol.run(0, case.settings["NT"])
+        sol.check_finite()
+        runs[label] = sol.seismograms(0)
+        del sol
+    o32, o64 = _get(refs, ("c2", "f32")), _get(refs, ("c2", "f64"))
+    for label, g in runs.items():
+        _check(f"C2 1000x1000x4000 {label}", g, o32, o64)
+
+
+def test_c3_tmem_64_shots_one_launch(refs):
+    case = cases.tmem_c3()
+    assert len(case.shots) == 64 and case.settings["NT"] == 3000
+    nrec = [len(s.rec_index) for s in case.shots]
+    assert min(nrec) >= 20 and max(nrec) <= 300 and len(set(nrec)) > 10  # uneven receiver counts
+    sol = cases.make_solver(case, steps_per_graph=50)
+    print("C3 kernel:", sol.tune(100), flush=True)
+    sol.run(0, case.settings["NT"])
+    sol.check_finite()
+    for k in C3_SHOTS:
+        o64 = _get(refs, ("c3", k, "f64")) if k in C3_SHOTS64 else None
+        _check(f"C3 shot {k} ({nrec[k]} receivers)", sol.seismograms(k), _get(refs, ("c3", k, "f32")), o64)
+
+
+@pytest.mark.parametrize("plan,store", [("per-step FP16 record", -1.0), ("FP16 checkpoints every 50 steps", 0.0)])
+def test_c4_adjoint_full_size(plan, store, refs):
+    """16-shot C4 handle, forward + adjoint on the GPU; the adjoint sources of shots
+    2..15 are zero, so the gradient is the first two shots' (it is linear in the
+    adjoint source).  Compared with the all-FP64 pipeline (FP64 forward, FP64
+    residual, FP64 adjoint)."""
+    c = cases.elastic_c4()
+    assert (c.settings["NX"], c.settings["NT"], len(c.shots)) == (600, 3000, 16)
+    (o0, g0), (o1, g1) = _get(refs, ("c4", 0)), _get(refs, ("c4", 1))
+    obs, g64 = [o0, o1], g0 + g1
+    sol = cases.make_solver(c, path="strip", tile_y=16)
+    sol.set_adjoint_store(store)
+    sol.set_checkpointing(50, True)
+    sol.run(0, c.settings["NT"])
+    sol.check_finite()
+    for k in range(16):
+        r = sol.seismograms(k).astype(np.float64)
+        r = r - obs[k] if k < 2 else np.zeros_like(r)
+        sol.set_adjoint_source(k, r.astype(np.float32))
+    sol.run_adjoint()
+    for k, name in enumerate(("vp", "vs", "rho")):
+        e = rel_l2(sol.gradient(name), g64[k])
+        print(f"C4 600x600x3000, 16 shots, {plan}: d/d{name} relL2 vs FP64 pipeline {e:.2e}", flush=True)
+        assert np.abs(g64[k]).max() > 0
+        assert e <= GTOL, (plan, name, e)
+
+
+def test_c5_elastic3d_batched_equals_single_and_oracle_window(refs):
+    case = cases.elastic_c5(NT=C5_NT)
+    assert case.settings["NX"] == 320 and len(case.shots) == 8
+    sol = cases.make_solver(case, steps_per_graph=50, tile_y=4)  # 4-row x-z tiles, the benchmark's variant
+    sol.run(0, C5_NT)
+    sol.check_finite()
+    batched = [sol.seismograms(k) for k in range(8)]
+    del sol
+    for k in (0, 5):
+        single = cases.make_solver(case, shots=[case.shots[k]], steps_per_graph=50, tile_y=4)
+        single.run(0, C5_NT)
+        s = single.seismograms(0)
+        del single
+        print(f"C5 shot {k}: single-shot handle vs 8-shot batch, max |diff| {np.abs(s - batched[k]).max():.3e}",
+              flush=True)
+        assert np.array_equal(s, batched[k]), k
+    win = _c5_window_case()
+    sol = cases.make_solver(win, steps_per_graph=50, tile_y=4)
+    sol.run(0, C5_WIN)
+    sol.check_finite()
+    _check(f"C5 320^3 shot 0, first {C5_WIN} steps (Ricker peak at step 60)", sol.seismograms(0), _get(refs, "c5"))

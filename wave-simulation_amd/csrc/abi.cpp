@@ -232,10 +232,11 @@ double wv_solver::kernel_bytes(int kind) const
         return cells * (B * per_shot[kind - 3] + per_cell[kind - 3]);
     }
     // {field reads+writes per shot-cell, coefficient reads per cell, psi per x-frame cell, psi per y-frame cell}
-    static const int phased[3][2][4] = {
+    static const int phased[4][2][4] = {
         {{5, 2, 1, 1}, {4, 1, 1, 1}},  // acoustic: velocity, pressure
         {{7, 2, 2, 2}, {8, 3, 2, 2}},  // elastic: velocity, stress
         {{5, 2, 1, 1}, {4, 2, 1, 1}},  // tmem: H, E
+        {{4, 1, 1, 1}, {5, 2, 1, 1}},  // sh: vz, stresses
     };
     static const int single[4] = {10, 5, 4, 4};  // elastic single pass
     if (variant == V_SINGLE && kind == 1) return 0.0;
@@ -272,9 +273,12 @@ int wv_create(const char *dimension, const char *type, wv_solver **out)
         phys = wv::ELASTIC;
     else if (t == "tmem")
         phys = wv::TMEM;
+    else if (t == "sh")
+        phys = wv::SH;
     else
         return fail(WV_EINVAL, "equationType " + t + " is not provided by this solver");
-    if (d == "3d" && phys != wv::ELASTIC) return fail(WV_EINVAL, "3D " + t + " is not provided by this build");
+    if (d == "3d" && phys != wv::ELASTIC && phys != wv::ACOUSTIC)
+        return fail(WV_EINVAL, "3D " + t + " is not provided by this build");
     wv_solver *s = new wv_solver();
     s->dim = d == "3d" ? 3 : 2;
     s->physics = phys;
@@ -315,6 +319,8 @@ int wv_init(wv_solver *s, const wv_config *c)
         return fail(WV_EINVAL, "tile_y of the row-strip kernels must be 0 (default), 8, 16 or 32");
     if (k.path == WV_PATH_PLAIN && k.nshots != 1) return fail(WV_EINVAL, "the plain path runs one shot per handle");
     if (s->dim == 3 && k.path == WV_PATH_PLAIN) return fail(WV_EINVAL, "the plain path is 2D only");
+    if (s->physics == wv::SH && k.path == WV_PATH_PLAIN)
+        return fail(WV_EINVAL, "the plain path is provided for acoustic, elastic and tmem");
     const bool single = s->dim == 2 && k.path == WV_PATH_FUSED && s->physics == wv::ELASTIC;
     if (s->dim == 2 && single && k.tile_y != 0 && std::abs(k.tile_y) != 8 && std::abs(k.tile_y) != 16 && std::abs(k.tile_y) != 24)
         return fail(WV_EINVAL, "tile_y of the single-pass kernel must be 0 (default), +-8, +-16 or +-24");
@@ -332,7 +338,7 @@ int wv_init(wv_solver *s, const wv_config *c)
 
     if (s->dim == 3) {
         s->e3 = new wv::Elastic3D();
-        int rc = s->e3->init(s->cfg, s->st);
+        int rc = s->e3->init(s->cfg, s->st, s->physics);
         if (rc) return fail(rc, s->e3->error);
         s->shots.assign(k.nshots, ShotAcq());
         s->inited = true;
@@ -396,8 +402,9 @@ int wv_init(wv_solver *s, const wv_config *c)
     wv::build_taps(t_yb, k.NY, NYp, N, s->H, false, stencil, DH, DT);
     if (k.free_surface == 1) {
         // velocity-type updates use the image operators (ForwardSolver2Delastic.cpp:168-202)
+        // (SH: DybFreeSurface for syz, ForwardSolver2Dsh.cpp:141-146)
         wv::build_taps_free_surface(t_yfv, k.NY, NYp, N, s->H, true, DH, DT);
-        if (s->physics == wv::ELASTIC)
+        if (s->physics == wv::ELASTIC || s->physics == wv::SH)
             wv::build_taps_free_surface(t_ybv, k.NY, NYp, N, s->H, false, DH, DT);
         else
             t_ybv = t_yb;
@@ -476,7 +483,9 @@ int wv_init(wv_solver *s, const wv_config *c)
 int wv_set_model(wv_solver *s, const float *m0, const float *m1, const float *m2)
 {
     if (!s || !s->inited) return fail(WV_ESTATE, "wv_set_model before wv_init");
-    if (!m0 || !m2 || (s->physics != wv::ACOUSTIC && !m1)) return fail(WV_EINVAL, "missing model vector");
+    // (SH needs no velocityP: m0 may be null)
+    if (!m2 || (s->physics != wv::SH && !m0) || (s->physics != wv::ACOUSTIC && !m1))
+        return fail(WV_EINVAL, "missing model vector");
     if (s->plain) {
         int rc = s->plain->set_model(m0, m1, m2);
         if (rc) return fail(rc, s->plain->error);
@@ -554,18 +563,38 @@ int wv_set_model(wv_solver *s, const float *m0, const float *m1, const float *m2
                     c[4][li(x, y)] = (float)v;
                 }
         }
+    } else if (s->physics == wv::SH) {
+        // SH (SH.cpp:413-433): 1/rho at the node (ModelparameterSeismic.cpp:164-172, in float),
+        // mu_xz / mu_yz the harmonic 2-point averages of mu along x / y (calcAveragedSWaveModulus,
+        // ModelparameterSeismic.cpp:421-433, with calcAverageMatrixX/Y, Modelparameter.cpp:336-408)
+        for (int y = 0; y < NY; y++)
+            for (int x = 0; x < NX; x++) {
+                const size_t i = gi(x, y);
+                c[0][li(x, y)] = 1.f / m2[i];
+                mu[i] = std::max((double)m2[i] * m1[i] * m1[i], 1.0);
+            }
+        for (int ax = 0; ax < 2; ax++)
+            for (int y = 0; y < NY; y++)
+                for (int x = 0; x < NX; x++) {
+                    const bool last = ax == 0 ? (x + 1 >= NX) : (y + 1 >= NY);
+                    const double a = 1.0 / mu[gi(x, y)];
+                    const double avg = last ? a : 0.5 * (a + 1.0 / mu[ax == 0 ? gi(x + 1, y) : gi(x, y + 1)]);
+                    double v = 1.0 / avg;
+                    if (v < 4.0) v = 0.0;
+                    c[1 + ax][li(x, y)] = (float)v;
+                }
     } else {
-        const double eps0 = 8.8541878176e-12, mu0 = 1.2566370614e-6;
+        const double mu0 = 1.2566370614e-6;
+        const float eps0f = 8.8541878176e-12f, dtf = (float)s->cfg.DT;
         std::vector<double> perm((size_t)NX * NY);
         for (int y = 0; y < NY; y++)
             for (int x = 0; x < NX; x++) {
                 const size_t i = gi(x, y);
-                const double eps = (double)m0[i] * eps0, sig = m1[i];
                 perm[i] = (double)m2[i] * mu0;
-                // ForwardSolverEM.cpp:14-68
-                const double q = 0.5 * sig * s->cfg.DT / eps;
-                c[0][li(x, y)] = (float)((1.0 - q) / (1.0 + q));
-                c[1][li(x, y)] = (float)(1.0 / eps / (1.0 + q));
+                // Ca, Cb in the reference's float arithmetic, step by step as ForwardSolverEM.cpp:14-68
+                // (epsilon = epsilon_r * eps0, TMEM.cpp:230-241): the time loop multiplies ez by Ca
+                // every step, so a differently rounded Ca drifts from the float build's seismograms
+                wv::em_ca_cb(m0[i] * eps0f, m1[i], dtf, c[0][li(x, y)], c[1][li(x, y)]);
             }
         inv_avg(perm, c[2], 0);  // 1/mu averaged along x (TMEM.cpp:366-373)
         inv_avg(perm, c[3], 1);  // 1/mu averaged along y
@@ -605,14 +634,16 @@ int wv_set_acquisition(wv_solver *s, int shot, int nsrc, const long long *src_in
     if ((nsrc && (!src_index || !src_type || !src_signal)) || (nrec && (!rec_index || !rec_type)))
         return fail(WV_EINVAL, "missing acquisition array");
     const long long n = (long long)s->cfg.NX * s->cfg.NY * s->cfg.NZ;
-    const int maxtype = s->physics == wv::TMEM ? 1 : (s->dim == 3 ? 4 : 3);
+    const int maxtype = s->physics == wv::TMEM ? 1 : (s->dim == 3 || s->physics == wv::SH ? 4 : 3);
+    // SH has VZ sources and receivers only (FDTD2Dsh.hpp:25-27 throws for P, VX, VY)
+    const int mintype = s->physics == wv::SH ? 4 : 1;
     for (int k = 0; k < nsrc; k++) {
         if (src_index[k] < 0 || src_index[k] >= n) return fail(WV_EINVAL, "source outside the grid");
-        if (src_type[k] < 1 || src_type[k] > maxtype) return fail(WV_EINVAL, "unsupported source type");
+        if (src_type[k] < mintype || src_type[k] > maxtype) return fail(WV_EINVAL, "unsupported source type");
     }
     for (int k = 0; k < nrec; k++) {
         if (rec_index[k] < 0 || rec_index[k] >= n) return fail(WV_EINVAL, "receiver outside the grid");
-        if (rec_type[k] < 1 || rec_type[k] > maxtype) return fail(WV_EINVAL, "unsupported receiver type");
+        if (rec_type[k] < mintype || rec_type[k] > maxtype) return fail(WV_EINVAL, "unsupported receiver type");
     }
     ShotAcq a;
     a.sidx.assign(src_index, src_index + nsrc);

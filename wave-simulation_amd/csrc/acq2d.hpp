@@ -7,8 +7,19 @@
 namespace wv {
 namespace {
 
-// velocity-type components: VX (2) and VY (3); stress-type: P / EZ (1)
+// velocity-type components: VX (2) and VY (3), VZ (4) in SH; stress-type: P / EZ (1)
 __device__ __forceinline__ bool velocity_type(int ty) { return ty == 2 || ty == 3; }
+template <int PHYS>
+__device__ __forceinline__ bool velocity_type_of(int ty)
+{
+    return PHYS == SH ? ty == 4 : velocity_type(ty);
+}
+// field of a velocity-type component: vx, vy (fields 0, 1); vz is field 0 of SH
+template <int PHYS>
+__device__ __forceinline__ int velocity_field(int ty)
+{
+    return PHYS == SH ? 0 : ty - 2;
+}
 
 // Sources and receivers owned by this block (shot-block sb) at step t, for
 // either the velocity-type or the stress-type components.  Uniform per block.
@@ -23,7 +34,7 @@ __device__ void block_acquisition(const Acq &A, const Fields2D &F, long long so,
     if (tid == 0) {
         // components in the reference's order P, VX, VY, VZ; each in list order
         for (int ty = 1; ty <= 4; ty++) {
-            if (velocity_type(ty) != vel) continue;
+            if (velocity_type_of<PHYS>(ty) != vel) continue;
             for (int e = s0; e < s1; e++) {
                 const int s = A.bsrc[e];
                 if (A.src_type[s] != ty) continue;
@@ -35,7 +46,7 @@ __device__ void block_acquisition(const Acq &A, const Fields2D &F, long long so,
                 } else if (ty == 1) {
                     F.f[2][c] += v;
                 } else {
-                    F.f[ty - 2][c] += v;
+                    F.f[velocity_field<PHYS>(ty)][c] += v;
                 }
             }
         }
@@ -44,7 +55,7 @@ __device__ void block_acquisition(const Acq &A, const Fields2D &F, long long so,
     for (int e = r0 + tid; e < r1; e += NTHREADS) {
         const int r = A.brec[e];
         const int ty = A.rec_type[r];
-        if (velocity_type(ty) != vel) continue;
+        if (velocity_type_of<PHYS>(ty) != vel) continue;
         const long long c = so + A.rec_cell[r];
         float v;
         if (PHYS == ELASTIC && ty == 1) {
@@ -54,7 +65,7 @@ __device__ void block_acquisition(const Acq &A, const Fields2D &F, long long so,
         } else if (ty == 1) {
             v = F.f[2][c];
         } else {
-            v = F.f[ty - 2][c];
+            v = F.f[velocity_field<PHYS>(ty)][c];
         }
         A.seis[(long long)t * A.rec_total + r] = v;
     }

@@ -89,20 +89,6 @@ size_t psi_total(const wv_solver *s)
     return s->cfg.damping_boundary == 2 ? 4 * s->psx_n + 4 * s->psy_n : 0;
 }
 
-// (D^T w)[j] = sum_k Tt[j][k] w[j + k - H] with Tt[j][k] = T[j + k - H][2H - k]
-std::vector<float> transpose_taps(const std::vector<float> &t, int H)
-{
-    const int K = 2 * H + 1;
-    const int np = (int)(t.size() / K);
-    std::vector<float> r(t.size(), 0.f);
-    for (int j = 0; j < np; j++)
-        for (int k = 0; k < K; k++) {
-            const int i = j + k - H;
-            if (i >= 0 && i < np) r[(size_t)j * K + k] = t[(size_t)i * K + (2 * H - k)];
-        }
-    return r;
-}
-
 // start/stop events attached to timed dispatches, collected in batches
 struct KernelTimer {
     wv_solver *s;
@@ -240,6 +226,10 @@ extern "C" {
 int wv_set_checkpointing(wv_solver *s, int every, int fp16)
 {
     if (!s || !s->inited) return fail(WV_ESTATE, "wv_set_checkpointing before wv_init");
+    if (s->e3) {
+        const int rc = s->e3->set_checkpointing(every, fp16, s->adj ? s->adj->store_gib : -1.0);
+        return rc ? fail(rc, s->e3->error) : WV_OK;
+    }
     if (s->dim != 2 || s->physics != wv::ELASTIC || s->plain)
         return fail(WV_EINVAL, "the adjoint is provided for the tiled 2D elastic solver");
     if (s->cfg.free_surface || s->cfg.damping_boundary == 1)
@@ -353,6 +343,11 @@ int wv_set_adjoint_store(wv_solver *s, double max_gib)
 
 int wv_set_adjoint_source(wv_solver *s, int shot, const float *adj)
 {
+    if (s && s->inited && s->e3) {
+        if (shot < 0 || shot >= s->cfg.nshots || !adj) return fail(WV_EINVAL, "bad shot or adjoint source");
+        const int rc = s->e3->set_adjoint_source(shot, adj);
+        return rc ? fail(rc, s->e3->error) : WV_OK;
+    }
     if (!s || !s->inited || !s->adj || !s->adj->every) return fail(WV_ESTATE, "checkpointing is not enabled");
     if (shot < 0 || shot >= s->cfg.nshots || !adj) return fail(WV_EINVAL, "bad shot or adjoint source");
     AdjointState *a = s->adj;
@@ -364,6 +359,14 @@ int wv_set_adjoint_source(wv_solver *s, int shot, const float *adj)
 
 int wv_run_adjoint(wv_solver *s)
 {
+    if (s && s->inited && s->e3) {
+        HIP_TRY(hipEventRecord(s->ev0, s->st));
+        const int rc = s->e3->run_adjoint();
+        if (rc) return fail(rc, s->e3->error);
+        HIP_TRY(hipEventRecord(s->ev1, s->st));
+        s->ran = true;
+        return WV_OK;
+    }
     if (!s || !s->inited || !s->adj || !s->adj->every) return fail(WV_ESTATE, "checkpointing is not enabled");
     AdjointState *a = s->adj;
     if (!a->forward_done) return fail(WV_ESTATE, "wv_run_adjoint needs a checkpointed forward run over [0, NT)");
@@ -404,7 +407,7 @@ int wv_run_adjoint(wv_solver *s)
         wv::build_taps(t[2], s->cfg.NY, NYp, N, H, true, stencil, DH, DT);
         wv::build_taps(t[3], s->cfg.NY, NYp, N, H, false, stencil, DH, DT);
         for (int q = 0; q < 4; q++) {
-            std::vector<float> tt = transpose_taps(t[q], H);
+            std::vector<float> tt = wv::transpose_taps(t[q], H);
             std::vector<float> ext((size_t)(tt.size() / (2 * H + 1) + 2 * ORG) * (2 * H + 1), 0.f);
             std::copy(tt.begin(), tt.end(), ext.begin() + (size_t)ORG * (2 * H + 1));
             float *base;
@@ -556,8 +559,19 @@ int wv_run_adjoint(wv_solver *s)
     return WV_OK;
 }
 
+int wv_adjoint_plan(wv_solver *s, int *k2, int *nsub, int *on_host)
+{
+    if (!s || !s->inited || !s->e3) return fail(WV_EINVAL, "the memory plan is reported for 3D elastic handles");
+    s->e3->adjoint_plan(k2, nsub, on_host);
+    return WV_OK;
+}
+
 int wv_get_gradient(wv_solver *s, int param, double *out)
 {
+    if (s && s->inited && s->e3) {
+        const int rc = s->e3->get_gradient(param, out);
+        return rc ? fail(rc, s->e3->error) : WV_OK;
+    }
     if (!s || !s->inited || !s->adj || !s->adj->grad_ready) return fail(WV_ESTATE, "no gradient computed");
     if (param < 0 || param > 7 || !out) return fail(WV_EINVAL, "param must be 0..7");
     AdjointState *a = s->adj;

@@ -28,6 +28,21 @@ __global__ __launch_bounds__(256) void nonfinite_scan(const float *__restrict__ 
     if (bad) *flag = 1u;
 }
 
+// the device buffers holding the handle's state (allocates nothing)
+void state_buffers(const wv_solver *s, wv::StateBuffers &b)
+{
+    if (s->plain)
+        s->plain->buffers(b);
+    else if (s->e3)
+        s->e3->buffers(b);
+    else {
+        for (int f = 0; f < s->nfields; f++) b.add(s->field[s->cur][f], (long long)s->L.plane * s->cfg.nshots);
+        if (s->d_seis && !s->acq_dirty) b.add(s->d_seis, (long long)s->cfg.NT * s->rec_total);
+    }
+}
+
+constexpr const char *NONFINITE_MSG = "Infinite or NaN value in seismogram or/and velocity wavefield!";
+
 std::string lower(const char *s)
 {
     std::string r = s ? s : "";
@@ -131,30 +146,23 @@ int wv_check_parameters(const char *dimension, const char *type, const wv_config
 int wv_check_finite(wv_solver *s)
 {
     if (!s || !s->inited) return fail(WV_ESTATE, "not initialised");
-    std::vector<std::pair<const float *, long long>> bufs;
-    if (s->plain)
-        s->plain->buffers(bufs);
-    else if (s->e3)
-        s->e3->buffers(bufs);
-    else {
-        for (int f = 0; f < s->nfields; f++) bufs.emplace_back(s->field[s->cur][f], (long long)s->L.plane * s->cfg.nshots);
-        if (s->d_seis && !s->acq_dirty) bufs.emplace_back(s->d_seis, (long long)s->cfg.NT * s->rec_total);
-    }
+    wv::StateBuffers bufs;
+    state_buffers(s, bufs);
     if (!s->d_flag) {
         int rc = s->dalloc(&s->d_flag, 1, s->allocs);
         if (rc) return rc;
     }
     HIP_TRY(hipMemsetAsync(s->d_flag, 0, sizeof(unsigned), s->st));
-    for (auto &b : bufs) {
-        if (!b.first || b.second <= 0) continue;
-        const long long blocks = std::min<long long>((b.second + 255) / 256, 4096);
-        hipLaunchKernelGGL(nonfinite_scan, dim3((unsigned)blocks), dim3(256), 0, s->st, b.first, b.second, s->d_flag);
+    for (int k = 0; k < bufs.k; k++) {
+        const long long blocks = std::min<long long>((bufs.n[k] + 255) / 256, 4096);
+        hipLaunchKernelGGL(nonfinite_scan, dim3((unsigned)blocks), dim3(256), 0, s->st, bufs.p[k], bufs.n[k],
+                           s->d_flag);
     }
     HIP_TRY(hipGetLastError());
     unsigned h = 0;
     HIP_TRY(hipMemcpyAsync(&h, s->d_flag, sizeof(unsigned), hipMemcpyDeviceToHost, s->st));
     HIP_TRY(hipStreamSynchronize(s->st));
-    if (h) return fail(WV_ENONFINITE, "Infinite or NaN value in seismogram or/and velocity wavefield!");
+    if (h) return fail(WV_ENONFINITE, NONFINITE_MSG);
     return WV_OK;
 }
 

@@ -231,7 +231,9 @@ __device__ __forceinline__ float cpml(float &d, float psi, float a, float b)
 __device__ __forceinline__ bool velocity_type(int ty) { return ty >= 2; }
 
 // Sources / receivers owned by block sb of one shot at step t, velocity-type
-// (VX, VY, VZ) or stress-type (P).  Uniform per block.
+// (VX, VY, VZ) or stress-type (P: sxx, syy, szz in elastic, p = field 3 in acoustic).
+// Uniform per block.
+template <bool AC>
 __device__ void acquisition3d(const Acq &A, const Fields3D &F, long long so, int sb, int t, bool vel)
 {
     const int s0 = A.bsrc_ptr[sb], s1 = A.bsrc_ptr[sb + 1];
@@ -248,7 +250,9 @@ __device__ void acquisition3d(const Acq &A, const Fields3D &F, long long so, int
                 if (A.src_type[s] != ty) continue;
                 const float v = A.src_sig[(long long)s * A.NT + t];
                 const long long c = so + A.src_cell[s];
-                if (ty == 1) {
+                if (ty == 1 && AC) {
+                    F.f[3][c] += v;
+                } else if (ty == 1) {
                     F.f[3][c] += v;
                     F.f[4][c] += v;
                     F.f[5][c] += v;
@@ -265,7 +269,9 @@ __device__ void acquisition3d(const Acq &A, const Fields3D &F, long long so, int
         if (velocity_type(ty) != vel) continue;
         const long long c = so + A.rec_cell[r];
         float v;
-        if (ty == 1) {
+        if (ty == 1 && AC) {
+            v = F.f[3][c];
+        } else if (ty == 1) {
             v = F.f[3][c];
             v += F.f[4][c];
             v += F.f[5][c];
@@ -304,9 +310,9 @@ __device__ __forceinline__ Pos position(const Layout3D &L, int TZ, int nyc, int 
     return p;
 }
 
-template <int H, int RZ>
+template <int H, int RZ, bool REC>
 __global__ __launch_bounds__(NTH) void el3d_velocity(Layout3D L, Fields3D F, Coef3D C, Taps3D T, Cpml3D P, Acq A,
-                                                     int nyc, int LY, int tofs, int pend)
+                                                     int nyc, int LY, int tofs, int pend, Rec3D R)
 {
     constexpr int TZ = RZ * BLOCK_Y, K = 2 * H + 1;
     using RX = Region<H, 0, TZ>;   // x derivatives only
@@ -323,7 +329,7 @@ __global__ __launch_bounds__(NTH) void el3d_velocity(Layout3D L, Fields3D F, Coe
     const long long so = (long long)q.shot * L.vol;
     const int tid = threadIdx.y * TILE_X + threadIdx.x, tx = threadIdx.x;
     const int wz = __builtin_amdgcn_readfirstlane(threadIdx.y);
-    if (pend) acquisition3d(A, F, so, q.sb, A.tbase[0] + tofs - 1, true);
+    if (pend) acquisition3d<false>(A, F, so, q.sb, A.tbase[0] + tofs - 1, true);
 
     const float *sxx = F.f[3] + so, *syy = F.f[4] + so, *szz = F.f[5] + so;
     const float *syz = F.f[6] + so, *sxz = F.f[7] + so, *sxy = F.f[8] + so;
@@ -507,14 +513,19 @@ __global__ __launch_bounds__(NTH) void el3d_velocity(Layout3D L, Fields3D F, Coe
             }
             u += t;
             u += w;
-            u *= cur.c[0][r];
-            bstore(cur.o[0][r] + u, plane_rsrc(vx, L, y), oc[r], 0);
             u2 += t2;
             u2 += w2;
-            u2 *= cur.c[1][r];
-            bstore(cur.o[1][r] + u2, plane_rsrc(vy, L, y), oc[r], 0);
             u3 += t3;
             u3 += w3;
+            if (REC) {
+                bstore(u, plane_rsrc(R.r[0] + so, L, y), oc[r], 0);
+                bstore(u2, plane_rsrc(R.r[1] + so, L, y), oc[r], 0);
+                bstore(u3, plane_rsrc(R.r[2] + so, L, y), oc[r], 0);
+            }
+            u *= cur.c[0][r];
+            bstore(cur.o[0][r] + u, plane_rsrc(vx, L, y), oc[r], 0);
+            u2 *= cur.c[1][r];
+            bstore(cur.o[1][r] + u2, plane_rsrc(vy, L, y), oc[r], 0);
             u3 *= cur.c[2][r];
             bstore(cur.o[2][r] + u3, plane_rsrc(vz, L, y), oc[r], 0);
         }
@@ -523,9 +534,9 @@ __global__ __launch_bounds__(NTH) void el3d_velocity(Layout3D L, Fields3D F, Coe
     }
 }
 
-template <int H, int RZ>
+template <int H, int RZ, bool REC>
 __global__ __launch_bounds__(NTH) void el3d_stress(Layout3D L, Fields3D F, Coef3D C, Taps3D T, Cpml3D P, Acq A,
-                                                   int nyc, int LY, int tofs)
+                                                   int nyc, int LY, int tofs, Rec3D R)
 {
     constexpr int TZ = RZ * BLOCK_Y, K = 2 * H + 1;
     using RXZ = Region<H, H, TZ>;
@@ -721,6 +732,14 @@ __global__ __launch_bounds__(NTH) void el3d_stress(Layout3D L, Fields3D F, Coef3
                     bstore(cpml(vxz, p14[r], azh, bzh), pr.z[1], poz[r], 0);
                     bstore(cpml(vyz, p16[r], azh, bzh), pr.z[2], poz[r], 0);
                 }
+                if (REC) {
+                    bstore(vxx, plane_rsrc(R.r[3] + so, L, y), oc[r], 0);
+                    bstore(vyy, plane_rsrc(R.r[4] + so, L, y), oc[r], 0);
+                    bstore(vzz, plane_rsrc(R.r[5] + so, L, y), oc[r], 0);
+                    bstore(vxy + vyx, plane_rsrc(R.r[6] + so, L, y), oc[r], 0);
+                    bstore(vxz + vzx, plane_rsrc(R.r[7] + so, L, y), oc[r], 0);
+                    bstore(vyz + vzy, plane_rsrc(R.r[8] + so, L, y), oc[r], 0);
+                }
                 // normal stresses (ForwardSolver3Delastic.cpp: M (vxx+vyy+vzz) - 2 mu (...))
                 float s = vxx + vyy;
                 s += vzz;
@@ -768,7 +787,221 @@ __global__ __launch_bounds__(NTH) void el3d_stress(Layout3D L, Fields3D F, Coef3
         }
         block_sync_lds();  // the next plane rewrites the LDS regions
     }
-    acquisition3d(A, F, so, q.sb, A.tbase[0] + tofs, false);
+    acquisition3d<false>(A, F, so, q.sb, A.tbase[0] + tofs, false);
+}
+
+// ---------------------------------------------------------------------------
+// 3D acoustic (ForwardSolver3Dacoustic.cpp:95-229).  Fields vx vy vz p (0..3),
+// coefficients 1/rho_x 1/rho_y 1/rho_z M (Coef3D irx iry irz M), CPML memory
+// variables 0 p_x, 1 p_y, 2 p_z (half-grid coefficients), 3 vx_x, 4 vy_y,
+// 5 vz_z (full).  Same tiles and y march as the elastic kernels; a plane's
+// loads are issued together before the plane's arithmetic.
+// ---------------------------------------------------------------------------
+template <int H, int RZ>
+__global__ __launch_bounds__(NTH) void ac3d_velocity(Layout3D L, Fields3D F, Coef3D C, Taps3D T, Cpml3D P, Acq A,
+                                                     int nyc, int LY, int tofs, int pend)
+{
+    constexpr int TZ = RZ * BLOCK_Y, K = 2 * H + 1;
+    using RXZ = Region<H, H, TZ>;
+    __shared__ float s_p[RXZ::RHP][RXZ::RW];
+    __shared__ ZTab<TZ, K> s_zt;
+    __shared__ float s_yt[NTH];
+    const Pos q = position(L, TZ, nyc, LY, A);
+    const long long so = (long long)q.shot * L.vol;
+    const int tid = threadIdx.y * TILE_X + threadIdx.x, tx = threadIdx.x;
+    const int wz = __builtin_amdgcn_readfirstlane(threadIdx.y);
+    if (pend) acquisition3d<true>(A, F, so, q.sb, A.tbase[0] + tofs - 1, true);
+    const float *pr = F.f[3] + so;
+    float *const vv[3] = {F.f[0] + so, F.f[1] + so, F.f[2] + so};
+    const float *const cr[3] = {C.irx, C.iry, C.irz};
+    const int x = q.x0 + tx;
+    const bool xin = x < L.NX;
+    float cxf[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) cxf[k] = T.xf[(long long)x * K + k];
+    const int xi = (P.on && xin) ? P.xi[x] : -1;
+    const float axh = xi >= 0 ? P.axh[x] : 0.f, bxh = xi >= 0 ? P.bxh[x] : 0.f;
+    unsigned oc[RZ];
+#pragma unroll
+    for (int r = 0; r < RZ; r++) oc[r] = (unsigned)(((q.z0 + wz + r * BLOCK_Y + L.H) * L.pitch + x + XPAD) * 4);
+    RegionOff<H, H, TZ> ro;
+    ro.init(L, q.x0, q.z0, tid);
+    float qp[K][RZ];
+#pragma unroll
+    for (int k = 0; k < 2 * H; k++) {
+        const auto b = plane_rsrc(pr, L, q.ys - H + k);
+#pragma unroll
+        for (int r = 0; r < RZ; r++) qp[k + 1][r] = bload(b, oc[r], 0);
+    }
+    const bool tfr = tile_in_frame(L, P, q.x0, q.z0, TZ);
+    s_zt.fill(T, P, L, q.z0, tid);
+    PsiRes3D ps;  // x frame: 0 p_x | y frame: 1 p_y | z frame: 2 p_z
+    ps.init(P, L, q.shot, 0, 0, 0, 1, 1, 1, 2, 2, 2);
+    const int W2 = 2 * P.W;
+    RXZ rp;
+    for (int y = q.ys; y < q.ye; y++) {
+        const float yt = ytab_elem<K>(T.yv, y, tid);
+        load_region(rp, ro, pr, L, y);
+        float np[RZ], o[3][RZ], c[3][RZ];
+        {
+            const auto b = plane_rsrc(pr, L, y + H);
+#pragma unroll
+            for (int r = 0; r < RZ; r++) np[r] = bload(b, oc[r], 0);
+        }
+#pragma unroll
+        for (int f = 0; f < 3; f++) {
+            const auto b = plane_rsrc(vv[f], L, y), bc = plane_rsrc(cr[f], L, y);
+#pragma unroll
+            for (int r = 0; r < RZ; r++) {
+                o[f][r] = bload(b, oc[r], 0);
+                c[f][r] = bload(bc, oc[r], 0);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 2 * H; k++)
+#pragma unroll
+            for (int r = 0; r < RZ; r++) qp[k][r] = qp[k + 1][r];
+#pragma unroll
+        for (int r = 0; r < RZ; r++) qp[2 * H][r] = np[r];
+        rp.store_padded(s_p, tid);
+        s_yt[tid] = yt;
+        block_sync_lds();
+        const int yi = __builtin_amdgcn_readfirstlane(P.on ? __float_as_int(s_yt[2 * K]) : -1);
+        const bool fr = __builtin_amdgcn_readfirstlane((int)(tfr || yi >= 0)) != 0;
+        const float *cyf = s_yt + K;
+        const float ayh = yi >= 0 ? s_yt[2 * K + 3] : 0.f, byh = yi >= 0 ? s_yt[2 * K + 4] : 0.f;
+#pragma unroll
+        for (int r = 0; r < RZ; r++) {
+            const int lz = wz + r * BLOCK_Y;
+            const int z = q.z0 + lz;
+            if (z >= L.NZ || !xin) continue;
+            float u = ddx<H>(s_p, lz + H, tx, cxf);
+            float t = ddy<H, RZ>(qp, r, cyf);
+            float w = ddz<H>(s_p, lz, tx + H, s_zt.t[1][lz]);
+            if (fr) {
+                const unsigned ox = psi_ox(L, W2, xi, y, z, true), oy = psi_oy(L, yi, x, z, true),
+                               oz = psi_oz(L, W2, s_zt.zi[lz], x, y, true);
+                bstore(cpml(u, bload(ps.x[0], ox, 0), axh, bxh), ps.x[0], ox, 0);
+                bstore(cpml(t, bload(ps.y[0], oy, 0), ayh, byh), ps.y[0], oy, 0);
+                bstore(cpml(w, bload(ps.z[0], oz, 0), s_zt.c[lz][2], s_zt.c[lz][3]), ps.z[0], oz, 0);
+            }
+            u *= c[0][r];
+            bstore(o[0][r] + u, plane_rsrc(vv[0], L, y), oc[r], 0);
+            t *= c[1][r];
+            bstore(o[1][r] + t, plane_rsrc(vv[1], L, y), oc[r], 0);
+            w *= c[2][r];
+            bstore(o[2][r] + w, plane_rsrc(vv[2], L, y), oc[r], 0);
+        }
+        block_sync_lds();
+    }
+}
+
+template <int H, int RZ>
+__global__ __launch_bounds__(NTH) void ac3d_pressure(Layout3D L, Fields3D F, Coef3D C, Taps3D T, Cpml3D P, Acq A,
+                                                     int nyc, int LY, int tofs)
+{
+    constexpr int TZ = RZ * BLOCK_Y, K = 2 * H + 1;
+    using RX = Region<H, 0, TZ>;
+    using RZZ = Region<0, H, TZ>;
+    __shared__ float s_vx[RX::RHP][RX::RW];
+    __shared__ float s_vz[RZZ::RHP][RZZ::RW];
+    __shared__ ZTab<TZ, K> s_zt;
+    __shared__ float s_yt[NTH];
+    const Pos q = position(L, TZ, nyc, LY, A);
+    const long long so = (long long)q.shot * L.vol;
+    const int tid = threadIdx.y * TILE_X + threadIdx.x, tx = threadIdx.x;
+    const int wz = __builtin_amdgcn_readfirstlane(threadIdx.y);
+    const float *vx = F.f[0] + so, *vy = F.f[1] + so, *vz = F.f[2] + so;
+    float *pr = F.f[3] + so;
+    const int x = q.x0 + tx;
+    const bool xin = x < L.NX;
+    float cxb[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) cxb[k] = T.xb[(long long)x * K + k];
+    const int xi = (P.on && xin) ? P.xi[x] : -1;
+    const float ax = xi >= 0 ? P.ax[x] : 0.f, bx = xi >= 0 ? P.bx[x] : 0.f;
+    unsigned oc[RZ];
+#pragma unroll
+    for (int r = 0; r < RZ; r++) oc[r] = (unsigned)(((q.z0 + wz + r * BLOCK_Y + L.H) * L.pitch + x + XPAD) * 4);
+    RegionOff<H, 0, TZ> ox_;
+    RegionOff<0, H, TZ> oz_;
+    ox_.init(L, q.x0, q.z0, tid);
+    oz_.init(L, q.x0, q.z0, tid);
+    float qv[K][RZ];
+#pragma unroll
+    for (int k = 0; k < 2 * H; k++) {
+        const auto b = plane_rsrc(vy, L, q.ys - H + k);
+#pragma unroll
+        for (int r = 0; r < RZ; r++) qv[k + 1][r] = bload(b, oc[r], 0);
+    }
+    const bool tfr = tile_in_frame(L, P, q.x0, q.z0, TZ);
+    s_zt.fill(T, P, L, q.z0, tid);
+    PsiRes3D ps;  // x frame: 3 vx_x | y frame: 4 vy_y | z frame: 5 vz_z
+    ps.init(P, L, q.shot, 3, 3, 3, 4, 4, 4, 5, 5, 5);
+    const int W2 = 2 * P.W;
+    const bool surface_plane = C.fs != 0;
+    RX rvx;
+    RZZ rvz;
+    for (int y = q.ys; y < q.ye; y++) {
+        const float yt = ytab_elem<K>(T.ys, y, tid);
+        load_region(rvx, ox_, vx, L, y);
+        load_region(rvz, oz_, vz, L, y);
+        float nv[RZ], op[RZ], cm[RZ], dmp[RZ];
+        {
+            const auto b = plane_rsrc(vy, L, y + H), bp = plane_rsrc(pr, L, y), bm = plane_rsrc(C.M, L, y);
+#pragma unroll
+            for (int r = 0; r < RZ; r++) {
+                nv[r] = bload(b, oc[r], 0);
+                op[r] = bload(bp, oc[r], 0);
+                cm[r] = bload(bm, oc[r], 0);
+                dmp[r] = 1.f;
+            }
+            if (C.damp) {
+                const auto bd = plane_rsrc(C.damp, L, y);
+#pragma unroll
+                for (int r = 0; r < RZ; r++) dmp[r] = bload(bd, oc[r], 0);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 2 * H; k++)
+#pragma unroll
+            for (int r = 0; r < RZ; r++) qv[k][r] = qv[k + 1][r];
+#pragma unroll
+        for (int r = 0; r < RZ; r++) qv[2 * H][r] = nv[r];
+        rvx.store_padded(s_vx, tid);
+        rvz.store_padded(s_vz, tid);
+        s_yt[tid] = yt;
+        block_sync_lds();
+        const int yi = __builtin_amdgcn_readfirstlane(P.on ? __float_as_int(s_yt[2 * K]) : -1);
+        const bool fr = __builtin_amdgcn_readfirstlane((int)(tfr || yi >= 0)) != 0;
+        const float *cyb = s_yt;
+        const float ay = yi >= 0 ? s_yt[2 * K + 1] : 0.f, by = yi >= 0 ? s_yt[2 * K + 2] : 0.f;
+#pragma unroll
+        for (int r = 0; r < RZ; r++) {
+            const int lz = wz + r * BLOCK_Y;
+            const int z = q.z0 + lz;
+            if (z >= L.NZ || !xin) continue;
+            float u = ddx<H>(s_vx, lz, tx, cxb);
+            float t = ddy<H, RZ>(qv, r, cyb);
+            float w = ddz<H>(s_vz, lz, tx, s_zt.t[0][lz]);
+            if (fr) {
+                const unsigned ox = psi_ox(L, W2, xi, y, z, true), oy = psi_oy(L, yi, x, z, true),
+                               oz = psi_oz(L, W2, s_zt.zi[lz], x, y, true);
+                bstore(cpml(u, bload(ps.x[0], ox, 0), ax, bx), ps.x[0], ox, 0);
+                bstore(cpml(t, bload(ps.y[0], oy, 0), ay, by), ps.y[0], oy, 0);
+                bstore(cpml(w, bload(ps.z[0], oz, 0), s_zt.c[lz][0], s_zt.c[lz][1]), ps.z[0], oz, 0);
+            }
+            u += t;
+            u += w;
+            u *= cm[r];
+            float p = op[r] + u;
+            p *= dmp[r];
+            if (surface_plane && y == 0) p *= 0.f;  // FreeSurface: setSurfaceZero(p)
+            bstore(p, plane_rsrc(pr, L, y), oc[r], 0);
+        }
+        block_sync_lds();
+    }
+    acquisition3d<true>(A, F, so, q.sb, A.tbase[0] + tofs, false);
 }
 
 __global__ __launch_bounds__(256) void flush3d(Layout3D L, Fields3D F, Acq A, int tofs)
@@ -816,43 +1049,94 @@ void go(Kern k, dim3 grid, dim3 block, hipStream_t st, hipEvent_t e0, hipEvent_t
 template <int H, int RZ>
 void launch_h3(int which, const Layout3D &L, int nshots, int nyc, const Fields3D &F, const Coef3D &C,
                const Taps3D &T, const Cpml3D &P, const Acq &A, int tofs, int pend, hipStream_t st, hipEvent_t e0,
-               hipEvent_t e1)
+               hipEvent_t e1, const Rec3D *R)
 {
     constexpr int TZ = RZ * BLOCK_Y;
     const int LY = (L.NY + nyc - 1) / nyc;
     dim3 grid((L.NX + TILE_X - 1) / TILE_X, (L.NZ + TZ - 1) / TZ, nyc * nshots), block(TILE_X, BLOCK_Y);
-    if (which == 0)
-        go(el3d_velocity<H, RZ>, grid, block, st, e0, e1, L, F, C, T, P, A, nyc, LY, tofs, pend);
-    else
-        go(el3d_stress<H, RZ>, grid, block, st, e0, e1, L, F, C, T, P, A, nyc, LY, tofs);
+    const Rec3D r = R ? *R : Rec3D{};
+    if (which == 0) {
+        if (R)
+            go(el3d_velocity<H, RZ, true>, grid, block, st, e0, e1, L, F, C, T, P, A, nyc, LY, tofs, pend, r);
+        else
+            go(el3d_velocity<H, RZ, false>, grid, block, st, e0, e1, L, F, C, T, P, A, nyc, LY, tofs, pend, r);
+    } else {
+        if (R)
+            go(el3d_stress<H, RZ, true>, grid, block, st, e0, e1, L, F, C, T, P, A, nyc, LY, tofs, r);
+        else
+            go(el3d_stress<H, RZ, false>, grid, block, st, e0, e1, L, F, C, T, P, A, nyc, LY, tofs, r);
+    }
 }
 
 template <int H>
 void launch_rz(int rz, int which, const Layout3D &L, int nshots, int nyc, const Fields3D &F, const Coef3D &C,
                const Taps3D &T, const Cpml3D &P, const Acq &A, int tofs, int pend, hipStream_t st, hipEvent_t e0,
-               hipEvent_t e1)
+               hipEvent_t e1, const Rec3D *R)
 {
     if (rz == 1)
-        launch_h3<H, 1>(which, L, nshots, nyc, F, C, T, P, A, tofs, pend, st, e0, e1);
+        launch_h3<H, 1>(which, L, nshots, nyc, F, C, T, P, A, tofs, pend, st, e0, e1, R);
     else if (rz == 4)
-        launch_h3<H, 4>(which, L, nshots, nyc, F, C, T, P, A, tofs, pend, st, e0, e1);
+        launch_h3<H, 4>(which, L, nshots, nyc, F, C, T, P, A, tofs, pend, st, e0, e1, R);
     else
-        launch_h3<H, 2>(which, L, nshots, nyc, F, C, T, P, A, tofs, pend, st, e0, e1);
+        launch_h3<H, 2>(which, L, nshots, nyc, F, C, T, P, A, tofs, pend, st, e0, e1, R);
 }
 
 }  // namespace
 
 void launch_el3d(int which, int order, int rz, const Layout3D &L, int nshots, int nyc, const Fields3D &F,
                  const Coef3D &C, const Taps3D &T, const Cpml3D &P, const Acq &A, int tofs, int pend,
+                 hipStream_t stream, hipEvent_t e0, hipEvent_t e1, const Rec3D *R)
+{
+    switch (order / 2) {
+    case 1: launch_rz<1>(rz, which, L, nshots, nyc, F, C, T, P, A, tofs, pend, stream, e0, e1, R); break;
+    case 2: launch_rz<2>(rz, which, L, nshots, nyc, F, C, T, P, A, tofs, pend, stream, e0, e1, R); break;
+    case 3: launch_rz<3>(rz, which, L, nshots, nyc, F, C, T, P, A, tofs, pend, stream, e0, e1, R); break;
+    case 4: launch_rz<4>(rz, which, L, nshots, nyc, F, C, T, P, A, tofs, pend, stream, e0, e1, R); break;
+    case 5: launch_rz<5>(rz, which, L, nshots, nyc, F, C, T, P, A, tofs, pend, stream, e0, e1, R); break;
+    case 6: launch_rz<6>(rz, which, L, nshots, nyc, F, C, T, P, A, tofs, pend, stream, e0, e1, R); break;
+    default: break;
+    }
+}
+
+namespace {
+template <int H, int RZ>
+void launch_ac3d_h(int which, const Layout3D &L, int nshots, int nyc, const Fields3D &F, const Coef3D &C,
+                   const Taps3D &T, const Cpml3D &P, const Acq &A, int tofs, int pend, hipStream_t st, hipEvent_t e0,
+                   hipEvent_t e1)
+{
+    constexpr int TZ = RZ * BLOCK_Y;
+    const int LY = (L.NY + nyc - 1) / nyc;
+    dim3 grid((L.NX + TILE_X - 1) / TILE_X, (L.NZ + TZ - 1) / TZ, nyc * nshots), block(TILE_X, BLOCK_Y);
+    if (which == 0)
+        go(ac3d_velocity<H, RZ>, grid, block, st, e0, e1, L, F, C, T, P, A, nyc, LY, tofs, pend);
+    else
+        go(ac3d_pressure<H, RZ>, grid, block, st, e0, e1, L, F, C, T, P, A, nyc, LY, tofs);
+}
+template <int H>
+void launch_ac3d_rz(int rz, int which, const Layout3D &L, int nshots, int nyc, const Fields3D &F, const Coef3D &C,
+                    const Taps3D &T, const Cpml3D &P, const Acq &A, int tofs, int pend, hipStream_t st, hipEvent_t e0,
+                    hipEvent_t e1)
+{
+    if (rz == 1)
+        launch_ac3d_h<H, 1>(which, L, nshots, nyc, F, C, T, P, A, tofs, pend, st, e0, e1);
+    else if (rz == 4)
+        launch_ac3d_h<H, 4>(which, L, nshots, nyc, F, C, T, P, A, tofs, pend, st, e0, e1);
+    else
+        launch_ac3d_h<H, 2>(which, L, nshots, nyc, F, C, T, P, A, tofs, pend, st, e0, e1);
+}
+}  // namespace
+
+void launch_ac3d(int which, int order, int rz, const Layout3D &L, int nshots, int nyc, const Fields3D &F,
+                 const Coef3D &C, const Taps3D &T, const Cpml3D &P, const Acq &A, int tofs, int pend,
                  hipStream_t stream, hipEvent_t e0, hipEvent_t e1)
 {
     switch (order / 2) {
-    case 1: launch_rz<1>(rz, which, L, nshots, nyc, F, C, T, P, A, tofs, pend, stream, e0, e1); break;
-    case 2: launch_rz<2>(rz, which, L, nshots, nyc, F, C, T, P, A, tofs, pend, stream, e0, e1); break;
-    case 3: launch_rz<3>(rz, which, L, nshots, nyc, F, C, T, P, A, tofs, pend, stream, e0, e1); break;
-    case 4: launch_rz<4>(rz, which, L, nshots, nyc, F, C, T, P, A, tofs, pend, stream, e0, e1); break;
-    case 5: launch_rz<5>(rz, which, L, nshots, nyc, F, C, T, P, A, tofs, pend, stream, e0, e1); break;
-    case 6: launch_rz<6>(rz, which, L, nshots, nyc, F, C, T, P, A, tofs, pend, stream, e0, e1); break;
+    case 1: launch_ac3d_rz<1>(rz, which, L, nshots, nyc, F, C, T, P, A, tofs, pend, stream, e0, e1); break;
+    case 2: launch_ac3d_rz<2>(rz, which, L, nshots, nyc, F, C, T, P, A, tofs, pend, stream, e0, e1); break;
+    case 3: launch_ac3d_rz<3>(rz, which, L, nshots, nyc, F, C, T, P, A, tofs, pend, stream, e0, e1); break;
+    case 4: launch_ac3d_rz<4>(rz, which, L, nshots, nyc, F, C, T, P, A, tofs, pend, stream, e0, e1); break;
+    case 5: launch_ac3d_rz<5>(rz, which, L, nshots, nyc, F, C, T, P, A, tofs, pend, stream, e0, e1); break;
+    case 6: launch_ac3d_rz<6>(rz, which, L, nshots, nyc, F, C, T, P, A, tofs, pend, stream, e0, e1); break;
     default: break;
     }
 }

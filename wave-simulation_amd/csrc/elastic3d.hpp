@@ -38,6 +38,7 @@ struct Coef3D {
     const float *irx, *iry, *irz, *M, *mu, *muxy, *muxz, *muyz;
     const float *damp;      // ABS factors (layout of one field) or nullptr
     const float *fsH, *fsV;  // free-surface scales on the y = 0 plane ((z + H) * pitch + x + XPAD) or nullptr
+    int fs;                  // FreeSurface = 1 (acoustic: p = 0 on the y = 0 plane)
 };
 
 struct Taps3D {
@@ -65,9 +66,23 @@ struct Cpml3D {
     float *ps[18];
 };
 
+// What the forward kernels store, when asked, for the adjoint pass: per time step, the
+// CPML-modified derivative sums the gradient correlates with, in the layout of the fields
+// (all shots): r[0..2] the velocity-update sums of vx, vy, vz (before 1/rho), r[3..5] the
+// normal strains Dxb vx, Dyb vy, Dzb vz, r[6..8] the shear sums of sxy, sxz, syz (before
+// mu_xy, mu_xz, mu_yz).
+struct Rec3D {
+    float *r[9];
+};
+
 // One time step = velocity launch + stress launch (+ ABS velocity damping).
-// rz = rows in z per thread (tile height 4 * rz), nyc = chunks along y.
+// rz = rows in z per thread (tile height 4 * rz), nyc = chunks along y; R (optional)
+// receives this step's record.
 void launch_el3d(int which, int order, int rz, const Layout3D &L, int nshots, int nyc, const Fields3D &F,
+                 const Coef3D &C, const Taps3D &T, const Cpml3D &P, const Acq &A, int tofs, int pend,
+                 hipStream_t stream, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr, const Rec3D *R = nullptr);
+// 3D acoustic step (fields vx vy vz p, memory variables 0..5 of Cpml3D::ps): which = 0 velocity, 1 pressure
+void launch_ac3d(int which, int order, int rz, const Layout3D &L, int nshots, int nyc, const Fields3D &F,
                  const Coef3D &C, const Taps3D &T, const Cpml3D &P, const Acq &A, int tofs, int pend,
                  hipStream_t stream, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 void launch_abs_velocity3d(const Layout3D &L, int nshots, const Fields3D &F, const Coef3D &C, hipStream_t stream);
@@ -77,7 +92,7 @@ void launch_flush3d(const Layout3D &L, const Fields3D &F, const Acq &A, int tofs
 class Elastic3D {
   public:
     ~Elastic3D();
-    int init(const wv_config &cfg, hipStream_t stream);
+    int init(const wv_config &cfg, hipStream_t stream, int physics = ELASTIC);
     int set_model(const float *vp, const float *vs, const float *rho);
     int set_acquisition(int shot, const std::vector<long long> &sidx, const std::vector<int> &stype,
                         const std::vector<float> &ssig, const std::vector<long long> &ridx,
@@ -92,12 +107,36 @@ class Elastic3D {
     double kernel_bytes(int kind) const;
     const char *kernel_name(int kind) const;
     // device buffers of the state (wavefields of all shots, seismograms) for wv_check_finite
-    void buffers(std::vector<std::pair<const float *, long long>> &out) const;
+    void buffers(wv::StateBuffers &out) const;
+
+    // adjoint-state gradient (CPML or no absorbing frame, no free surface; adjoint3d_host.cpp):
+    // checkpoints every `every` steps of the following runs (FP16 or FP32 fields), kept in
+    // device memory or, when they do not fit next to the recomputation buffers, in pinned
+    // host memory; budget_gib < 0: 3/4 of the free device memory
+    int set_checkpointing(int every, int fp16, double budget_gib);
+    int set_adjoint_source(int shot, const float *adj);  // [nrec][NT]
+    int run_adjoint();
+    int get_gradient(int param, double *out);  // 0 vp 1 vs 2 rho 3..10 coefficients
+    // memory plan of the last set_checkpointing: sub-segment length, sub-checkpoints per
+    // segment, whether the checkpoints sit in host memory
+    void adjoint_plan(int *k2, int *nsub, int *on_host) const;
     double kms[3] = {0, 0, 0};
     long long klaunch[3] = {0, 0, 0};
     std::string error;
 
   private:
+    struct Adj3State;
+    Adj3State *adj3 = nullptr;
+    std::vector<float> model_h[3];  // vp, vs, rho (chain rule of the gradient)
+    int run_segment(int t_start, int t_end, bool timed);
+    int checkpoint_interval() const;
+    int enqueue_rec(int nsteps, float *rec);
+    int save_checkpoint(int k);
+    int restore_checkpoint(int k);
+    int save_state(float *dst);
+    int restore_state(const float *src);
+    size_t psi_floats() const;
+    void free_adjoint();
     int fail(int code, const std::string &m);
     int dalloc(float **p, size_t n);
     template <typename T>
@@ -117,6 +156,10 @@ class Elastic3D {
     hipStream_t st = nullptr;
     int NX = 0, NY = 0, NZ = 0, NT = 0, H = 0, order = 0, nshots = 1;
     int damping = 0, free_surface = 0, W = 0, steps_per_graph = 0;
+    bool ops_stencil = false;
+    int physics = ELASTIC;
+    int nf = 9;    // fields: elastic vx vy vz sxx syy szz syz sxz sxy, acoustic vx vy vz p
+    int nps = 18;  // CPML memory variables (acoustic: p_x p_y p_z vx_x vy_y vz_z)
     double DT = 0, DH = 0;
     int rz = 2, nyc = 1, nbx = 0, nbz = 0;
     Layout3D L{};

@@ -516,6 +516,129 @@ __global__ __launch_bounds__(NTHREADS) void tm2d_e(Layout2D L, Fields2D F, Coef2
     block_acquisition<TMEM>(A, F, so, shot_block(A, B), A.tbase[0] + tofs, false);
 }
 
+// ---------------------------------------------------------------------------
+// SH (ForwardSolver2Dsh.cpp:101-190): phase 0 vz from the stresses, phase 1 the
+// stresses from vz; VZ sources / receivers are velocity-type
+// ---------------------------------------------------------------------------
+template <int H, int TY>
+__global__ __launch_bounds__(NTHREADS) void sh2d_v(Layout2D L, Fields2D F, Coef2D C, Taps T, Cpml2D P, Acq A,
+                                                   int tofs, int pend)
+{
+    using TL = Tile<H, TY>;
+    constexpr int R = TY / BLOCK_Y;
+    __shared__ float s_xz[TL::RH][TL::RW];
+    __shared__ float s_yz[TL::RH][TL::RW];
+    const Blk B = xcd_block();
+    const int shot = B.shot;
+    const long long so = (long long)shot * L.plane;
+    const int x0 = B.bx * TILE_X, y0 = B.by * TY;
+    const int tid = threadIdx.y * TILE_X + threadIdx.x, tx = threadIdx.x;
+    if (pend) block_acquisition<SH>(A, F, so, shot_block(A, B), A.tbase[0] + tofs - 1, true);
+    TileRegs<H, TY> rxz, ryz;
+    rxz.load(F.f[1] + so, L, x0, y0, tid);
+    ryz.load(F.f[2] + so, L, x0, y0, tid);
+    float *__restrict__ vz = F.f[0] + so;
+    const Col col = column(L, P, x0 + tx);
+    float cxb[2 * H + 1];
+    load_taps<H>(cxb, T.xb, col.x);
+    const int W2 = 2 * P.W;
+    const int wy = __builtin_amdgcn_readfirstlane(threadIdx.y);
+    float ov[R], cr[R], p0[R], q0[R];
+    int yi[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int y = y0 + wy + r * BLOCK_Y;
+        const long long o = L.at(col.x, y);
+        ov[r] = vz[o];
+        cr[r] = C.c[0][o];
+        p0[r] = q0[r] = 0.f;
+        yi[r] = P.on ? frame_index(y, L.NY, P.W, C.free_surface) : -1;
+        if (col.xi >= 0 && y < L.NY) p0[r] = P.psx[0][((long long)shot * L.NY + y) * W2 + col.xi];
+        if (yi[r] >= 0 && col.in) q0[r] = P.psy[0][((long long)shot * W2 + yi[r]) * L.NX + col.x];
+    }
+    rxz.store(s_xz, tid);
+    ryz.store(s_yz, tid);
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int ly = wy + r * BLOCK_Y;
+        const int y = y0 + ly;
+        if (y >= L.NY) break;
+        // Dyb of syz (the free-surface image operator when FreeSurface = 1)
+        const float *cyb = T.yb_v + (long long)y * (2 * H + 1);
+        const int Rr = ly + H;
+        float u = dx<H>(s_xz, Rr, tx, cxb);
+        float t = dy<H>(s_yz, Rr, tx, cyb);
+        if (!col.in) continue;
+        if (col.xi >= 0) P.psx[0][((long long)shot * L.NY + y) * W2 + col.xi] = cpml(u, p0[r], col.ax, col.bx);
+        if (yi[r] >= 0) P.psy[0][((long long)shot * W2 + yi[r]) * L.NX + col.x] = cpml(t, q0[r], P.ay[y], P.by[y]);
+        u += t;
+        u *= cr[r];
+        vz[L.at(col.x, y)] = ov[r] + u;
+    }
+}
+
+template <int H, int TY>
+__global__ __launch_bounds__(NTHREADS) void sh2d_s(Layout2D L, Fields2D F, Coef2D C, Taps T, Cpml2D P, Acq A,
+                                                   int tofs, int pend)
+{
+    (void)pend;
+    using TL = Tile<H, TY>;
+    constexpr int R = TY / BLOCK_Y;
+    __shared__ float s_v[TL::RH][TL::RW];
+    const Blk B = xcd_block();
+    const int shot = B.shot;
+    const long long so = (long long)shot * L.plane;
+    const int x0 = B.bx * TILE_X, y0 = B.by * TY;
+    const int tid = threadIdx.y * TILE_X + threadIdx.x, tx = threadIdx.x;
+    TileRegs<H, TY> rv;
+    rv.load(F.f[0] + so, L, x0, y0, tid);
+    float *__restrict__ sxz = F.f[1] + so;
+    float *__restrict__ syz = F.f[2] + so;
+    const Col col = column(L, P, x0 + tx);
+    float cxf[2 * H + 1];
+    load_taps<H>(cxf, T.xf, col.x);
+    const int W2 = 2 * P.W;
+    const int wy = __builtin_amdgcn_readfirstlane(threadIdx.y);
+    float oxz[R], oyz[R], cxz[R], cyz[R], p1[R], q1[R], dmp[R];
+    int yi[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int y = y0 + wy + r * BLOCK_Y;
+        const long long o = L.at(col.x, y);
+        oxz[r] = sxz[o];
+        oyz[r] = syz[o];
+        cxz[r] = C.c[1][o];
+        cyz[r] = C.c[2][o];
+        dmp[r] = C.damp ? C.damp[o] : 1.f;
+        p1[r] = q1[r] = 0.f;
+        yi[r] = P.on ? frame_index(y, L.NY, P.W, C.free_surface) : -1;
+        if (col.xi >= 0 && y < L.NY) p1[r] = P.psx[1][((long long)shot * L.NY + y) * W2 + col.xi];
+        if (yi[r] >= 0 && col.in) q1[r] = P.psy[1][((long long)shot * W2 + yi[r]) * L.NX + col.x];
+    }
+    rv.store(s_v, tid);
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int ly = wy + r * BLOCK_Y;
+        const int y = y0 + ly;
+        if (y >= L.NY) break;
+        const float *cyf = T.yf + (long long)y * (2 * H + 1);
+        const int Rr = ly + H;
+        float u = dx<H>(s_v, Rr, tx, cxf);
+        float t = dy<H>(s_v, Rr, tx, cyf);
+        if (!col.in) continue;
+        if (col.xi >= 0) P.psx[1][((long long)shot * L.NY + y) * W2 + col.xi] = cpml(u, p1[r], col.axh, col.bxh);
+        if (yi[r] >= 0) P.psy[1][((long long)shot * W2 + yi[r]) * L.NX + col.x] = cpml(t, q1[r], P.ayh[y], P.byh[y]);
+        const long long o = L.at(col.x, y);
+        u *= cxz[r];
+        sxz[o] = (oxz[r] + u) * dmp[r];
+        t *= cyz[r];
+        syz[o] = (oyz[r] + t) * dmp[r];
+    }
+    block_acquisition<SH>(A, F, so, shot_block(A, B), A.tbase[0] + tofs, false);
+}
+
 // velocity-type sources / receivers of step t for every shot (end of a run segment)
 template <int PHYS>
 __global__ __launch_bounds__(256) void flush2d(Layout2D L, Fields2D F, Acq A, int tofs)
@@ -526,27 +649,28 @@ __global__ __launch_bounds__(256) void flush2d(Layout2D L, Fields2D F, Acq A, in
     if (t < 0 || t >= A.NT) return;
     if (threadIdx.x == 0) {
         const int s0 = A.src_ptr[shot], s1 = A.src_ptr[shot + 1];
-        for (int ty = 2; ty <= 3; ty++)
+        for (int ty = 2; ty <= 4; ty++)
             for (int s = s0; s < s1; s++) {
-                if (A.src_type[s] != ty) continue;
-                F.f[ty - 2][so + A.src_cell[s]] += A.src_sig[(long long)s * A.NT + t];
+                if (A.src_type[s] != ty || !velocity_type_of<PHYS>(ty)) continue;
+                F.f[velocity_field<PHYS>(ty)][so + A.src_cell[s]] += A.src_sig[(long long)s * A.NT + t];
             }
     }
     __syncthreads();
     const int r0 = A.rec_ptr[shot], r1 = A.rec_ptr[shot + 1];
     for (int r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
         const int ty = A.rec_type[r];
-        if (!velocity_type(ty)) continue;
-        A.seis[(long long)t * A.rec_total + r] = F.f[ty - 2][so + A.rec_cell[r]];
+        if (!velocity_type_of<PHYS>(ty)) continue;
+        A.seis[(long long)t * A.rec_total + r] = F.f[velocity_field<PHYS>(ty)][so + A.rec_cell[r]];
     }
 }
 
+// ABS of the velocity-type fields (b may be null: SH has one, vz)
 __global__ void abs_velocity(long long n, float *a, float *b, const float *__restrict__ damp, long long plane)
 {
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
         const float d = damp[i % plane];
         a[i] *= d;
-        b[i] *= d;
+        if (b) b[i] *= d;
     }
 }
 
@@ -586,6 +710,11 @@ void launch_h(int physics, int which, dim3 grid, const Layout2D &L, const Fields
             go(ac2d_velocity<H, TY>, grid, block, st, e0, e1, L, F, C, T, P, A, tofs, pend);
         else
             go(ac2d_pressure<H, TY>, grid, block, st, e0, e1, L, F, C, T, P, A, tofs, pend);
+    } else if (physics == SH) {
+        if (which == 0)
+            go(sh2d_v<H, TY>, grid, block, st, e0, e1, L, F, C, T, P, A, tofs, pend);
+        else
+            go(sh2d_s<H, TY>, grid, block, st, e0, e1, L, F, C, T, P, A, tofs, pend);
     } else {
         if (which == 0)
             go(tm2d_h<H, TY>, grid, block, st, e0, e1, L, F, C, T, P, A, tofs, pend);
@@ -632,14 +761,15 @@ void launch_flush2d(int physics, const Layout2D &L, const Fields2D &F, const Acq
         flush2d<ELASTIC><<<A.nshots, 256, 0, stream>>>(L, F, A, tofs);
     else if (physics == ACOUSTIC)
         flush2d<ACOUSTIC><<<A.nshots, 256, 0, stream>>>(L, F, A, tofs);
+    else if (physics == SH)
+        flush2d<SH><<<A.nshots, 256, 0, stream>>>(L, F, A, tofs);
 }
 
 void launch_abs_velocity2d(int physics, const Layout2D &L, int nshots, const Fields2D &F, const Coef2D &C,
                            hipStream_t stream)
 {
-    (void)physics;
     const long long n = (long long)nshots * L.plane;
-    abs_velocity<<<1024, 256, 0, stream>>>(n, F.f[0], F.f[1], C.damp, L.plane);
+    abs_velocity<<<1024, 256, 0, stream>>>(n, F.f[0], physics == SH ? nullptr : F.f[1], C.damp, L.plane);
 }
 
 void launch_advance(int *tbase, int k, hipStream_t stream) { advance_kernel<<<1, 64, 0, stream>>>(tbase, k); }

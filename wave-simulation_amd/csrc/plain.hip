@@ -320,14 +320,12 @@ int PlainSolver::set_model(const float *m0, const float *m1, const float *m2)
                 }
         }
     } else {
-        const double eps0 = 8.8541878176e-12, mu0 = 1.2566370614e-6;
+        const double mu0 = 1.2566370614e-6;
+        const float eps0f = 8.8541878176e-12f;
         std::vector<double> perm(n_);
         for (int i = 0; i < n_; i++) {
-            const double eps = (double)m0[i] * eps0, sig = m1[i];
             perm[i] = (double)m2[i] * mu0;
-            const double q = 0.5 * sig * cfg_.DT / eps;
-            c[0][i] = (float)((1.0 - q) / (1.0 + q));
-            c[1][i] = (float)(1.0 / eps / (1.0 + q));
+            em_ca_cb(m0[i] * eps0f, m1[i], (float)cfg_.DT, c[0][i], c[1][i]);
         }
         inv_avg(perm, c[2], 0);
         inv_avg(perm, c[3], 1);
@@ -595,12 +593,12 @@ int PlainSolver::get_seismograms(float *out)
     return 0;
 }
 
-void PlainSolver::buffers(std::vector<std::pair<const float *, long long>> &out) const
+void PlainSolver::buffers(wv::StateBuffers &out) const
 {
     const int nf = physics_ == ELASTIC ? 5 : 3;
-    for (int f = 0; f < nf; f++) out.emplace_back(f_[f], (long long)n_);
+    for (int f = 0; f < nf; f++) out.add(f_[f], (long long)n_);
     for (int c = 0; c < 5; c++)
-        if (rec_[c].n) out.emplace_back(rec_[c].seis, (long long)rec_[c].n * cfg_.NT);
+        if (rec_[c].n) out.add(rec_[c].seis, (long long)rec_[c].n * cfg_.NT);
 }
 
 int PlainSolver::field_io(int field, float *out, const float *in)

@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "../../include/wave_abi.h"
+#include "wave_internal.hpp"
 
 namespace wv {
 
@@ -42,7 +43,7 @@ class PlainSolver {
     int get_seismograms(float *out);
     int field_io(int field, float *out, const float *in);
     // device buffers of the state (wavefields, seismograms) for wv_check_finite
-    void buffers(std::vector<std::pair<const float *, long long>> &out) const;
+    void buffers(wv::StateBuffers &out) const;
 
   private:
     int physics_ = 0;

@@ -172,4 +172,31 @@ std::vector<double> abs_profile(int W, double damping_coeff)
     return c;
 }
 
+void em_ca_cb(float eps, float sigma, float DT, float &ca, float &cb)
+{
+    float q = 0.5f / eps;
+    q *= sigma;
+    q *= DT;
+    const float cinv = 1.f / (q + 1.f);
+    float a = 1.f - q;
+    a *= cinv;
+    float b = 1.f / eps;
+    b *= cinv;
+    ca = a;
+    cb = b;
+}
+
+std::vector<float> transpose_taps(const std::vector<float> &t, int H)
+{
+    const int K = 2 * H + 1;
+    const int np = (int)(t.size() / K);
+    std::vector<float> r(t.size(), 0.f);
+    for (int j = 0; j < np; j++)
+        for (int k = 0; k < K; k++) {
+            const int i = j + k - H;
+            if (i >= 0 && i < np) r[(size_t)j * K + k] = t[(size_t)i * K + (2 * H - k)];
+        }
+    return r;
+}
+
 }  // namespace wv

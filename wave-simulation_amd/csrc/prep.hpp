@@ -21,6 +21,10 @@ double fd_coefficient(int order, int j);
 void build_taps(std::vector<float> &out, int n, int padded_n, int order, int H, bool forward, bool stencil,
                 float DH, float DT);
 
+// Transposed tap table of a build_taps table: (D^T w)[j] = sum_k Tt[j][k] w[j + k - H] with
+// Tt[j][k] = T[j + k - H][2H - k] (rows outside the table contribute nothing).
+std::vector<float> transpose_taps(const std::vector<float> &t, int H);
+
 // Free-surface image operators along depth (Derivatives.cpp:367-526, non-hybrid):
 // forward=true -> DyfFreeSurface, false -> DybFreeSurface.
 void build_taps_free_surface(std::vector<float> &out, int n, int padded_n, int order, int H, bool forward, float DH,
@@ -37,6 +41,11 @@ void cpml_profile(int W, double npower, double fc, double vmax, double DT, doubl
 void cpml_axis(int n, int padded_n, int W, bool skip_near_side, double npower, double fc, double vmax, double DT,
                double DH, std::vector<float> &a, std::vector<float> &b, std::vector<float> &ah, std::vector<float> &bh,
                std::vector<int> &idx);
+
+// Ca = (1 - q) / (1 + q), Cb = 1 / (eps (1 + q)), q = sigma DT / (2 eps), in the float
+// arithmetic and operation order of ForwardSolverEM.cpp:14-68 (the time loop multiplies ez
+// by Ca every step, so a Ca rounded differently drifts from the reference float build).
+void em_ca_cb(float eps, float sigma, float DT, float &ca, float &cb);
 
 // ABS damping factor of one grid cell (ABS2D.cpp:110-178, ABS3D.cpp:150-215).
 std::vector<double> abs_profile(int W, double damping_coeff);

@@ -39,6 +39,7 @@ int Elastic3D::fail(int code, const std::string &m)
 Elastic3D::~Elastic3D()
 {
     if (st) (void)hipStreamSynchronize(st);
+    free_adjoint();
     drop_graph();
     for (hipEvent_t e : kev) (void)hipEventDestroy(e);
     if (ev0) (void)hipEventDestroy(ev0);
@@ -96,9 +97,12 @@ int Elastic3D::upload_pos(T **p0, const std::vector<T> &h, int per_pos, T fill)
     return 0;
 }
 
-int Elastic3D::init(const wv_config &k, hipStream_t stream)
+int Elastic3D::init(const wv_config &k, hipStream_t stream, int phys)
 {
     st = stream;
+    physics = phys;
+    nf = physics == ACOUSTIC ? 4 : 9;
+    nps = physics == ACOUSTIC ? 6 : 18;
     if (k.NX < 2 || k.NY < 2 || k.NZ < 2) return fail(WV_EINVAL, "3D grid needs NX, NY, NZ >= 2");
     if (k.nshots < 1) return fail(WV_EINVAL, "nshots must be >= 1");
     NX = k.NX;
@@ -136,10 +140,13 @@ int Elastic3D::init(const wv_config &k, hipStream_t stream)
             return fail(WV_EINVAL, "grid too large for the 3D kernels (a y plane or a CPML frame of 2 GiB or more)");
     }
     int rc;
-    for (int f = 0; f < 9; f++)
-        if ((rc = dalloc(&F[f], (size_t)L.vol * nshots))) return rc;
+    // one plane of slack after the last shot: the kernels' y march issues the loads of the plane
+    // after its chunk (2H + 1 planes beyond the last grid plane) before leaving the loop
+    for (int f = 0; f < nf; f++)
+        if ((rc = dalloc(&F[f], (size_t)L.vol * nshots + (size_t)L.slab))) return rc;
 
     const bool stencil = k.ops_mode == WV_OPS_STENCIL;
+    ops_stencil = stencil;
     const int NXp = round_up(NX, 64) + 64, NYp = NY + 64, NZp = round_up(NZ, 16) + 64;
     const int K = 2 * H + 1;
     const float fDH = (float)DH, fDT = (float)DT;
@@ -178,7 +185,7 @@ int Elastic3D::init(const wv_config &k, hipStream_t stream)
         ps_n[2] = (size_t)nshots * NY * 2 * W * NX;
         // axis of each memory variable (see Cpml3D)
         static const int axis_of[18] = {0, 1, 2, 0, 1, 2, 0, 1, 2, 0, 1, 2, 1, 0, 2, 0, 2, 1};
-        for (int q = 0; q < 18; q++)
+        for (int q = 0; q < nps; q++)
             if ((rc = dalloc(&ps[q], ps_n[axis_of[q]]))) return rc;
     }
     {
@@ -267,10 +274,15 @@ int Elastic3D::set_model(const float *vp, const float *vs, const float *rho)
 {
     const size_t n = (size_t)NX * NY * NZ;
     auto gi = [&](int x, int y, int z) { return (size_t)x + (size_t)z * NX + (size_t)y * NX * NZ; };
-    std::vector<double> r(n), M(n), mu(n), imu(n);
+    const bool ac = physics == ACOUSTIC;
+    model_h[0].assign(vp, vp + n);
+    if (!ac) model_h[1].assign(vs, vs + n);
+    model_h[2].assign(rho, rho + n);
+    std::vector<double> r(n), M(n), mu(n, 1.0), imu(n, 1.0);
     for (size_t i = 0; i < n; i++) {
         r[i] = rho[i];
         M[i] = (double)rho[i] * vp[i] * vp[i];
+        if (ac) continue;
         mu[i] = std::max((double)rho[i] * vs[i] * vs[i], 1.0);  // ModelparameterSeismic.cpp:424
         imu[i] = 1.0 / mu[i];
     }
@@ -299,7 +311,7 @@ int Elastic3D::set_model(const float *vp, const float *vs, const float *rho)
                 // harmonic 4-point averages of mu in the xy, xz, yz planes
                 // (Modelparameter.cpp:449-526, ModelparameterSeismic.cpp:421-433)
                 static const int planes[3][2] = {{0, 1}, {0, 2}, {1, 2}};
-                for (int pl = 0; pl < 3; pl++) {
+                for (int pl = 0; pl < (ac ? 0 : 3); pl++) {
                     const int a0 = planes[pl][0], a1 = planes[pl][1];
                     const bool in0 = pos[a0] + 1 < len[a0], in1 = pos[a1] + 1 < len[a1];
                     double avg;
@@ -321,11 +333,11 @@ int Elastic3D::set_model(const float *vp, const float *vs, const float *rho)
                 }
             }
     int rc;
-    for (int q = 0; q < 8; q++) {
+    for (int q = 0; q < (ac ? 4 : 8); q++) {
         if (!coef[q] && (rc = dalloc(&coef[q], (size_t)L.vol))) return rc;
         HIP3(hipMemcpyAsync(coef[q], c[q].data(), c[q].size() * sizeof(float), hipMemcpyHostToDevice, st));
     }
-    if (free_surface == 1) {
+    if (free_surface == 1 && !ac) {
         // FreeSurfaceElastic.cpp:11-47: scales of the horizontal update at y = 0
         std::vector<float> h((size_t)L.slab, 0.f), v((size_t)L.slab, 0.f);
         for (int z = 0; z < NZ; z++)
@@ -428,9 +440,9 @@ int Elastic3D::sync_acquisition()
 
 int Elastic3D::reset()
 {
-    for (int f = 0; f < 9; f++) HIP3(hipMemsetAsync(F[f], 0, (size_t)L.vol * nshots * sizeof(float), st));
+    for (int f = 0; f < nf; f++) HIP3(hipMemsetAsync(F[f], 0, (size_t)L.vol * nshots * sizeof(float), st));
     static const int axis_of[18] = {0, 1, 2, 0, 1, 2, 0, 1, 2, 0, 1, 2, 1, 0, 2, 0, 2, 1};
-    for (int q = 0; q < 18; q++)
+    for (int q = 0; q < nps; q++)
         if (ps[q]) HIP3(hipMemsetAsync(ps[q], 0, ps_n[axis_of[q]] * sizeof(float), st));
     if (d_seis) HIP3(hipMemsetAsync(d_seis, 0, (size_t)NT * std::max(rec_total, 1) * sizeof(float), st));
     return 0;
@@ -457,6 +469,7 @@ Coef3D Elastic3D::coefs() const
     c.damp = damping == 1 ? damp : nullptr;
     c.fsH = free_surface == 1 ? fsH : nullptr;
     c.fsV = free_surface == 1 ? fsV : nullptr;
+    c.fs = free_surface == 1;
     return c;
 }
 
@@ -553,9 +566,14 @@ int Elastic3D::enqueue(int nsteps, bool timed)
     };
     for (int j = 0; j < nsteps; j++) {
         hipEvent_t *e = timed ? &kev[4 * pending] : nullptr;
-        launch_el3d(0, order, rz, L, nshots, nyc, Fd, C, T, P, A, j, j > 0, st, e ? e[0] : nullptr,
-                    e ? e[1] : nullptr);
-        launch_el3d(1, order, rz, L, nshots, nyc, Fd, C, T, P, A, j, 0, st, e ? e[2] : nullptr, e ? e[3] : nullptr);
+        auto launch = physics == ACOUSTIC ? launch_ac3d : [](int w, int o, int r, const Layout3D &L_, int ns, int ny,
+                                                              const Fields3D &F_, const Coef3D &C_, const Taps3D &T_,
+                                                              const Cpml3D &P_, const Acq &A_, int tofs, int pend,
+                                                              hipStream_t s_, hipEvent_t a, hipEvent_t b) {
+            launch_el3d(w, o, r, L_, ns, ny, F_, C_, T_, P_, A_, tofs, pend, s_, a, b);
+        };
+        launch(0, order, rz, L, nshots, nyc, Fd, C, T, P, A, j, j > 0, st, e ? e[0] : nullptr, e ? e[1] : nullptr);
+        launch(1, order, rz, L, nshots, nyc, Fd, C, T, P, A, j, 0, st, e ? e[2] : nullptr, e ? e[3] : nullptr);
         if (damping == 1) launch_abs_velocity3d(L, nshots, Fd, C, st);
         if (timed && ++pending == BATCH) {
             int rc = collect();
@@ -572,11 +590,9 @@ int Elastic3D::enqueue(int nsteps, bool timed)
     return 0;
 }
 
-int Elastic3D::run(int t_start, int t_end, bool timed)
+int Elastic3D::run_segment(int t_start, int t_end, bool timed)
 {
-    if (!model_ok) return fail(WV_ESTATE, "wv_run before wv_set_model");
-    int rc = sync_acquisition();
-    if (rc) return rc;
+    int rc;
     int t = t_start;
     const int K = timed ? 0 : steps_per_graph;
     if (K > 0 && t_end - t >= K) {
@@ -606,6 +622,24 @@ int Elastic3D::run(int t_start, int t_end, bool timed)
     return 0;
 }
 
+int Elastic3D::run(int t_start, int t_end, bool timed)
+{
+    if (!model_ok) return fail(WV_ESTATE, "wv_run before wv_set_model");
+    int rc = sync_acquisition();
+    if (rc) return rc;
+    const int every = adj3 ? checkpoint_interval() : 0;
+    if (!every) return run_segment(t_start, t_end, timed);
+    // checkpointed: segments end on multiples of the interval; the state at the start of
+    // each segment (all steps before it done and flushed) is saved first
+    for (int t = t_start; t < t_end;) {
+        if (t % every == 0 && (rc = save_checkpoint(t / every))) return rc;
+        const int next = std::min(t_end, (t / every + 1) * every);
+        if ((rc = run_segment(t, next, timed))) return rc;
+        t = next;
+    }
+    return 0;
+}
+
 int Elastic3D::get_seismograms(int shot, float *out)
 {
     const int nr = nrec(shot);
@@ -623,15 +657,15 @@ int Elastic3D::get_seismograms(int shot, float *out)
     return 0;
 }
 
-void Elastic3D::buffers(std::vector<std::pair<const float *, long long>> &out) const
+void Elastic3D::buffers(wv::StateBuffers &out) const
 {
-    for (int f = 0; f < 9; f++) out.emplace_back(F[f], (long long)L.vol * nshots);
-    if (d_seis && !acq_dirty) out.emplace_back(d_seis, (long long)NT * rec_total);
+    for (int f = 0; f < nf; f++) out.add(F[f], (long long)L.vol * nshots);
+    if (d_seis && !acq_dirty) out.add(d_seis, (long long)NT * rec_total);
 }
 
 int Elastic3D::field_io(int shot, int field, float *out, const float *in)
 {
-    if (field < 0 || field >= 9) return fail(WV_EINVAL, "bad field index");
+    if (field < 0 || field >= nf) return fail(WV_EINVAL, "bad field index");
     std::vector<float> h((size_t)L.vol);
     float *d = F[field] + (size_t)shot * L.vol;
     HIP3(hipStreamSynchronize(st));
@@ -693,11 +727,16 @@ double Elastic3D::kernel_bytes(int kind) const
         for (auto &s : shots) ns += s.sidx.size();
         return 12.0 * ns + 8.0 * rec_total;
     }
-    double bytes = kind == 0 ? 4.0 * (cells * B * 12 + cells * 3) : 4.0 * (cells * B * 15 + cells * 5);
+    // per shot-cell field reads + writes, per-cell coefficient reads, memory variables per frame cell
+    // elastic: velocity 6 + 3 read, 3 written, 3 coefficients; stress 3 + 6 read, 6 written, 5 coefficients
+    // acoustic: velocity 1 + 3 read, 3 written, 3 coefficients; pressure 3 + 1 read, 1 written, M (+ ABS)
+    const bool ac = physics == ACOUSTIC;
+    const double fr = ac ? (kind == 0 ? 7 : 5) : (kind == 0 ? 12 : 15), co = ac ? (kind == 0 ? 3 : 1) : (kind == 0 ? 3 : 5);
+    double bytes = 4.0 * (cells * B * fr + cells * co);
     if (damping == 2) {
         const double fx = (double)NY * NZ * 2 * W, fy = (double)NX * NZ * (free_surface ? W : 2 * W),
                      fz = (double)NX * NY * 2 * W;
-        bytes += 8.0 * B * 3 * (fx + fy + fz);
+        bytes += 8.0 * B * (ac ? 1 : 3) * (fx + fy + fz);
     }
     return bytes;
 }
@@ -705,7 +744,8 @@ double Elastic3D::kernel_bytes(int kind) const
 const char *Elastic3D::kernel_name(int kind) const
 {
     static const char *n[3] = {"el3d_velocity", "el3d_stress", "flush3d"};
-    return n[kind];
+    static const char *a[3] = {"ac3d_velocity", "ac3d_pressure", "flush3d"};
+    return physics == ACOUSTIC ? a[kind] : n[kind];
 }
 
 }  // namespace wv

@@ -8,9 +8,34 @@
 #include <cstddef>
 #include <cstdint>
 
+#include "../../include/wave_abi.h"
+
 namespace wv {
 
-enum Physics { ACOUSTIC = 0, ELASTIC = 1, TMEM = 2 };
+// The device buffers that hold a handle's state (wavefields of every shot, seismograms):
+// what wv_check_finite scans.  Fixed capacity, so listing them
+// allocates nothing; a handle never has more (2D: 5 fields + seismograms; 3D: 9 + 1;
+// reference-sequence path: 5 fields + 5 seismogram components).
+struct StateBuffers {
+    static constexpr int CAP = 16;
+    const float *p[CAP];
+    long long n[CAP];
+    int k = 0;
+    bool overflow = false;
+    void add(const float *ptr, long long len)
+    {
+        if (!ptr || len <= 0) return;
+        if (k == CAP) {
+            overflow = true;
+            return;
+        }
+        p[k] = ptr;
+        n[k] = len;
+        k++;
+    }
+};
+
+enum Physics { ACOUSTIC = 0, ELASTIC = 1, TMEM = 2, SH = 3 };
 
 // Maximum half-width of a derivative stencil window (order 12 -> 6).
 constexpr int MAX_H = 6;
@@ -70,7 +95,7 @@ struct Cpml2D {
 };
 
 struct Fields2D {
-    float *f[5];  // acoustic: vx vy p ; elastic: vx vy sxx syy sxy ; tmem: hx hy ez
+    float *f[5];  // acoustic: vx vy p ; elastic: vx vy sxx syy sxy ; tmem: hx hy ez ; sh: vz sxz syz
 };
 
 // write buffers of the CPML memory variables of the velocity derivatives
@@ -84,6 +109,7 @@ struct Coef2D {
     // acoustic: c0 = M, c1 = 1/rho_x, c2 = 1/rho_y
     // elastic : c0 = M, c1 = mu, c2 = 1/rho_x, c3 = 1/rho_y, c4 = mu_xy
     // tmem    : c0 = Ca, c1 = Cb, c2 = 1/mu_xz, c3 = 1/mu_yz
+    // sh      : c0 = 1/rho, c1 = mu_xz, c2 = mu_yz
     const float *c[5];
     const float *damp;  // ABS damping factors (padded layout) or nullptr
     const float *fsH;   // [NXpad] free-surface horizontal scale at y = 0 (elastic) or nullptr

@@ -115,6 +115,28 @@ def elastic_c2(N=1000, NT=4000, DT=1e-4, W=30):
     return c
 
 
+def sh_2d(NX=180, NY=120, NT=600, DT=1e-4, nshots=3, order=8, W=16, damping=2, free_surface=0, seed=7):
+    """2D SH (Love waves, ForwardSolver2Dsh.cpp): the C2-style layered model with Gaussian
+    perturbations, VZ Ricker sources at seeded positions, VZ receivers on a row and a column
+    (uneven counts per shot)."""
+    s = dict(NX=NX, NY=NY, NT=NT, DH=1.0, DT=DT, spatial_fd_order=order, damping_boundary=damping,
+             boundary_width=W, npower=4.0, fc_cpml=30.0, vmax_cpml=2500.0, free_surface=free_surface,
+             damping_coeff=8.0)
+    vp, vs, rho = elastic_model(NX, NY, seed, seed + 100, sigma=8.0)
+    c = Case("sh_2d", "2D", "sh", s, vp, vs, rho)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    w = ricker(NT, DT, 60.0, 1.0)
+    lo = W + 4 if damping else 4
+    for k in range(nshots):
+        sx, sy = int(rng.integers(lo, NX - lo)), int(rng.integers(lo if not free_surface else 1, NY - lo))
+        nr = int(rng.integers(10, 40))
+        rx = np.sort(rng.choice(np.arange(lo, NX - lo), nr, replace=False))
+        ry = 0 if free_surface else lo
+        rec = np.concatenate([rx + ry * NX, (NX // 2) + np.arange(lo, NY - lo, 7) * NX])
+        c.shots.append(Shot(np.array([sx + sy * NX]), np.array([4]), w[None, :], rec, np.full(len(rec), 4, np.int32)))
+    return c
+
+
 def tmem_c3(NX=800, NY=400, NT=3000, DT=3e-11, nshots=64, seed=3, block=25, W=20):
     s = dict(NX=NX, NY=NY, NT=NT, DH=0.02, DT=DT, spatial_fd_order=8, damping_boundary=2, boundary_width=W,
              npower=4.0, fc_cpml=2e8, vmax_cpml=3e8)
@@ -198,6 +220,34 @@ def elastic_c5(N=320, NT=2500, DT=2e-4, nshots=8, seed=5, W=24, nrec_side=40, sp
         rec = np.concatenate([cells, cells, cells])
         rty = np.repeat(np.array([2, 3, 4], np.int32), len(cells))
         c.shots.append(Shot(np.array([sx + sz * NX + y * NX * NZ]), np.array([3]), w[None, :], rec, rty))
+    return c
+
+
+def acoustic_3d(NX=72, NY=60, NZ=52, NT=260, DT=2e-4, nshots=2, order=8, W=10, damping=2, free_surface=0,
+                seed=9):
+    """3D acoustic (ForwardSolver3Dacoustic.cpp): layered vp/rho with Gaussian perturbations, P and
+    VX/VY/VZ sources at seeded positions, receivers of every component on a line at depth."""
+    s = dict(NX=NX, NY=NY, NZ=NZ, NT=NT, DH=2.0, DT=DT, spatial_fd_order=order, damping_boundary=damping,
+             boundary_width=W, npower=4.0, fc_cpml=80.0, vmax_cpml=3000.0, free_surface=free_surface,
+             damping_coeff=8.0)
+    rows = np.arange(NY) * 5 // NY
+    g = gaussian_field((NY, NZ, NX), 4.0, seed)
+    vp = (1800.0 + 250.0 * rows)[:, None, None] * (1.0 + 0.03 * g)
+    rho = (1900.0 + 100.0 * rows)[:, None, None] * np.ones((1, NZ, NX))
+    c = Case("acoustic_3d", "3D", "acoustic", s, vp.astype(np.float32).ravel(), None, rho.astype(np.float32).ravel())
+    rng = np.random.Generator(np.random.PCG64(seed))
+    w = ricker(NT, DT, 80.0, 1.0)
+    lo = W + 3 if damping else 3
+    for k in range(nshots):
+        sx, sy, sz = (int(rng.integers(lo, n - lo)) for n in (NX, NY, NZ))
+        ty = 1 if k % 2 == 0 else 2 + k % 3
+        rx = np.arange(lo, NX - lo, 3)
+        yr = 0 if free_surface else lo
+        cells = rx + (NZ // 2) * NX + yr * NX * NZ
+        rec = np.concatenate([cells, cells[::2], cells[1::2], cells[::3]])
+        rty = np.concatenate([np.full(len(cells), 1), np.full(len(cells[::2]), 2), np.full(len(cells[1::2]), 3),
+                              np.full(len(cells[::3]), 4)]).astype(np.int32)
+        c.shots.append(Shot(np.array([sx + sz * NX + sy * NX * NZ]), np.array([ty]), w[None, :], rec, rty))
     return c
 
 

@@ -27,7 +27,9 @@ class ForwardSolver:
         ("2D", "acoustic"): ("vx", "vy", "p"),
         ("2D", "elastic"): ("vx", "vy", "sxx", "syy", "sxy"),
         ("2D", "tmem"): ("hx", "hy", "ez"),
+        ("2D", "sh"): ("vz", "sxz", "syz"),
         ("3D", "elastic"): ("vx", "vy", "vz", "sxx", "syy", "szz", "syz", "sxz", "sxy"),
+        ("3D", "acoustic"): ("vx", "vy", "vz", "p"),
     }
 
     def __init__(self, dimension: str, type_: str):
@@ -143,11 +145,20 @@ class ForwardSolver:
     def run_adjoint(self):
         _abi.check(_abi.load().wv_run_adjoint(self._h))
 
+    def adjoint_plan(self):
+        """3D elastic: (sub-segment steps, sub-segments per checkpoint segment, checkpoints on the host)."""
+        k2, ns, oh = C.c_int(), C.c_int(), C.c_int()
+        _abi.check(_abi.load().wv_adjoint_plan(self._h, C.byref(k2), C.byref(ns), C.byref(oh)))
+        return k2.value, ns.value, bool(oh.value)
+
     GRADIENTS = ("vp", "vs", "rho", "inv_rho_x", "inv_rho_y", "M", "mu", "mu_xy")
+    GRADIENTS3D = ("vp", "vs", "rho", "inv_rho_x", "inv_rho_y", "inv_rho_z", "M", "mu", "mu_xy", "mu_xz", "mu_yz")
 
     def gradient(self, param="vp"):
-        k = self.GRADIENTS.index(param) if isinstance(param, str) else int(param)
-        out = np.zeros(self.cfg.NX * self.cfg.NY, dtype=np.float64)
+        names = self.GRADIENTS3D if self.dimension == "3D" else self.GRADIENTS
+        k = names.index(param) if isinstance(param, str) else int(param)
+        n = self.cfg.NX * self.cfg.NY * (self.cfg.NZ if self.dimension == "3D" else 1)
+        out = np.zeros(n, dtype=np.float64)
         _abi.check(_abi.load().wv_get_gradient(self._h, k, out.ctypes.data_as(C.POINTER(C.c_double))))
         return out
 
