@@ -167,6 +167,20 @@ int wv_check_parameters(const char *dimension, const char *type, const wv_config
  * them).  Synchronizes the handle's stream. */
 int wv_check_finite(wv_solver *s);
 
+/* The same test as wv_check_finite, for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): it calls
+ * no hipMalloc / hipFree and no allocating library routine.  No counterpart in the
+ * reference.  Its scratch memory -- one 64-bit count of non-finite values per state buffer
+ * of the handle, at most WV_NOALLOC_MAX_BUFFERS buffers (128 B) -- is a static device
+ * array of the library, one copy per GPU, held until the process ends; it is cleared by
+ * a fill kernel at every call.  Calls that overlap on one GPU (from different threads or
+ * handles) share that array and race.  *nonfinite (may be NULL) receives the number of
+ * Inf/NaN values found in the wavefields of every shot (padding included) and in the
+ * seismograms.  Returns WV_ENONFINITE as wv_check_finite does, WV_EINVAL when the handle
+ * has more state buffers than the scratch holds.  Synchronizes the handle's stream. */
+#define WV_NOALLOC_MAX_BUFFERS 16
+int wv_check_finite_noalloc(wv_solver *s, long long *nonfinite);
+
 /* ---- adjoint-state gradient (2D elastic; CPML or no absorbing frame, no free
  * surface).  No counterpart in the reference: WAVE-Simulation has no adjoint
  * or gradient code (only inert flags, Modelparameter.hpp:94-95, 289); these

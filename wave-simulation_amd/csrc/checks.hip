@@ -3,10 +3,13 @@
 //    (src/CheckParameter/CheckParameter.hpp:166-246) with checkStabilityCriterion
 //    (:60-112) and checkNumericalDispersion (:121-162), on the host;
 //  * wv_check_finite: the per-shot test of Simulation.cpp:519 that wavefields and
-//    seismograms hold no Inf/NaN, as one GPU pass over the handle's buffers.
+//    seismograms hold no Inf/NaN, as one GPU pass over the handle's buffers;
+//  * wv_check_finite_noalloc: the same test with its scratch memory in a static device
+//    array (no allocation), counting the non-finite values per buffer.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <mutex>
 #include <sstream>
 #include <string>
 #include <utility>
@@ -26,6 +29,49 @@ __global__ __launch_bounds__(256) void nonfinite_scan(const float *__restrict__ 
         bad |= (u & 0x7f800000u) == 0x7f800000u;  // exponent all ones: Inf or NaN
     }
     if (bad) *flag = 1u;
+}
+
+// one count per state buffer of wv_check_finite_noalloc; each GPU holds its own copy
+__device__ __align__(256) unsigned long long scratch[WV_NOALLOC_MAX_BUFFERS];
+
+// number of non-finite values in p[0, n), added to *count (one atomic per wavefront that saw any)
+__global__ __launch_bounds__(256) void nonfinite_count(const float *__restrict__ p, long long n,
+                                                       unsigned long long *count)
+{
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    unsigned c = 0;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        c += (__float_as_uint(p[i]) & 0x7f800000u) == 0x7f800000u;
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_down(c, off, 64);
+    if ((threadIdx.x & 63) == 0 && c) atomicAdd(count, (unsigned long long)c);
+}
+
+// zero the first n counts (a kernel, not hipMemsetAsync: the call may sit in a captured graph)
+__global__ void scratch_clear(unsigned long long *p, int n)
+{
+    if ((int)threadIdx.x < n) p[threadIdx.x] = 0ull;
+}
+
+// device address of `scratch` on each GPU, looked up once per device with that device current
+constexpr int MAX_DEVICES = 64;
+std::mutex scratch_mu;
+unsigned long long *scratch_ptr[MAX_DEVICES] = {};
+
+int scratch_address(unsigned long long **out)
+{
+    int dev = -1;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= MAX_DEVICES) return fail(WV_EDEVICE, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (!scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(WV_EDEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        scratch_ptr[dev] = (unsigned long long *)p;
+    }
+    *out = scratch_ptr[dev];
+    return WV_OK;
 }
 
 // the device buffers holding the handle's state (allocates nothing)
@@ -163,6 +209,36 @@ int wv_check_finite(wv_solver *s)
     HIP_TRY(hipMemcpyAsync(&h, s->d_flag, sizeof(unsigned), hipMemcpyDeviceToHost, s->st));
     HIP_TRY(hipStreamSynchronize(s->st));
     if (h) return fail(WV_ENONFINITE, NONFINITE_MSG);
+    return WV_OK;
+}
+
+int wv_check_finite_noalloc(wv_solver *s, long long *nonfinite)
+{
+    if (nonfinite) *nonfinite = 0;
+    if (!s || !s->inited) return fail(WV_ESTATE, "not initialised");
+    wv::StateBuffers bufs;
+    state_buffers(s, bufs);
+    if (bufs.overflow)
+        return fail(WV_EINVAL, "the handle has more state buffers than the static scratch holds "
+                               "(WV_NOALLOC_MAX_BUFFERS)");
+    unsigned long long *scr = nullptr;
+    int rc = scratch_address(&scr);
+    if (rc) return rc;
+    if (bufs.k == 0) return WV_OK;
+    hipLaunchKernelGGL(scratch_clear, dim3(1), dim3(64), 0, s->st, scr, bufs.k);
+    for (int k = 0; k < bufs.k; k++) {
+        const long long blocks = std::min<long long>((bufs.n[k] + 255) / 256, 4096);
+        hipLaunchKernelGGL(nonfinite_count, dim3((unsigned)blocks), dim3(256), 0, s->st, bufs.p[k], bufs.n[k],
+                           scr + k);
+    }
+    HIP_TRY(hipGetLastError());
+    unsigned long long h[WV_NOALLOC_MAX_BUFFERS];
+    HIP_TRY(hipMemcpyAsync(h, scr, sizeof(unsigned long long) * bufs.k, hipMemcpyDeviceToHost, s->st));
+    HIP_TRY(hipStreamSynchronize(s->st));
+    unsigned long long total = 0;
+    for (int k = 0; k < bufs.k; k++) total += h[k];
+    if (nonfinite) *nonfinite = (long long)total;
+    if (total) return fail(WV_ENONFINITE, NONFINITE_MSG);
     return WV_OK;
 }
 

@@ -13,11 +13,11 @@
 namespace wv {
 
 // The device buffers that hold a handle's state (wavefields of every shot, seismograms):
-// what wv_check_finite scans.  Fixed capacity, so listing them
+// what wv_check_finite and wv_check_finite_noalloc scan.  Fixed capacity, so listing them
 // allocates nothing; a handle never has more (2D: 5 fields + seismograms; 3D: 9 + 1;
 // reference-sequence path: 5 fields + 5 seismogram components).
 struct StateBuffers {
-    static constexpr int CAP = 16;
+    static constexpr int CAP = WV_NOALLOC_MAX_BUFFERS;
     const float *p[CAP];
     long long n[CAP];
     int k = 0;

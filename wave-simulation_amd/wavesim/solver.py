@@ -123,6 +123,13 @@ class ForwardSolver:
         """Simulation.cpp:519: raise WaveError if any wavefield value or seismogram sample is Inf/NaN."""
         _abi.check(_abi.load().wv_check_finite(self._h))
 
+    def check_finite_noalloc(self):
+        """The same test through wv_check_finite_noalloc (no device allocation); returns
+        (status, number of non-finite values found) without raising."""
+        n = C.c_longlong(-1)
+        rc = _abi.load().wv_check_finite_noalloc(self._h, C.byref(n))
+        return rc, n.value
+
     def last_run_ms(self):
         ms = C.c_double()
         _abi.check(_abi.load().wv_last_run_ms(self._h, C.byref(ms)))
