@@ -307,15 +307,35 @@ def bench_3d(wavesim, L, variant=""):
     sol.run(0, 20)
     sol.synchronize()
     kern = _kernel_stats(sol, L, wavesim)
+    wavesim._abi.check(L.wv_set_kernel_timing(sol._h, 0))
     n = len(case.shots)
     s = case.settings
-    return {"workload": "3D elastic order 8, 320^3, dx 2 m, 24-cell CPML, 8 shots batched (4320 receivers each); "
-                        f"{steps} of 2500 steps timed",
-            "kernel": choice, "grid_cell_updates_per_s": round(case.cells * steps * n / (ms * 1e-3), 1),
-            "ms_per_step": round(ms / steps, 3),
-            "roofline": _step_roofline(kern, 20, alg_bytes_per_step("elastic3d", s["NX"], s["NY"], s["NZ"], n,
-                                                                    s["boundary_width"]),
-                                       TRAFFIC_3D if choice == "3d-z4" else None)}
+    out = {"workload": "3D elastic order 8, 320^3, dx 2 m, 24-cell CPML, 8 shots batched (4320 receivers each); "
+                       f"{steps} of 2500 steps timed",
+           "kernel": choice, "grid_cell_updates_per_s": round(case.cells * steps * n / (ms * 1e-3), 1),
+           "ms_per_step": round(ms / steps, 3),
+           "roofline": _step_roofline(kern, 20, alg_bytes_per_step("elastic3d", s["NX"], s["NY"], s["NZ"], n,
+                                                                   s["boundary_width"]),
+                                      TRAFFIC_3D if choice == "3d-z4" else None)}
+    # forward + adjoint gradient over the same truncated window (misfit 1/2 ||d||^2 of all 8 shots),
+    # FP16 checkpoints every 50 steps, the memory plan the budget gives (default 3/4 of free memory)
+    sol.set_checkpointing(50, True)
+    k2, nsub, host = sol.adjoint_plan()
+    sol.run(0, steps)
+    sol.synchronize()
+    t_ck = sol.last_run_ms()
+    for k in range(n):
+        sol.set_adjoint_source(k, sol.seismograms(k))
+    sol.run_adjoint()
+    t_adj = sol.last_run_ms()
+    import numpy as np
+    assert np.isfinite(sol.gradient("vp")).all()
+    out["adjoint"] = {"window": f"first {steps} steps", "plan": {"checkpoint_every": 50, "fp16": True,
+                                                                  "sub_segment_steps": k2, "sub_segments": nsub,
+                                                                  "checkpoints_in_host_memory": host},
+                      "forward_ms": round(ms, 2), "forward_with_checkpoints_ms": round(t_ck, 2),
+                      "adjoint_ms": round(t_adj, 2), "gradient_cost_vs_forward": round((t_ck + t_adj) / ms, 3)}
+    return out
 
 
 def _cpu_model():
@@ -495,7 +515,8 @@ def main():
         _progress(f"adjoint: cost {out['adjoint_16_shots']['recorded']['gradient_cost_vs_forward']} x forward "
                   f"(recorded), {out['adjoint_16_shots']['checkpointed']['gradient_cost_vs_forward']} x (checkpointed)")
         out["elastic3d_8_shots"] = bench_3d(wavesim, L)
-        _progress(f"3D: {out['elastic3d_8_shots']['grid_cell_updates_per_s']:.4g} cell-updates/s")
+        _progress(f"3D: {out['elastic3d_8_shots']['grid_cell_updates_per_s']:.4g} cell-updates/s, gradient cost "
+                  f"{out['elastic3d_8_shots']['adjoint']['gradient_cost_vs_forward']} x forward")
     if not args.no_cpu:
         base, par = cpu_baseline_leg(case, args.cpu_steps, args.cpu_samples, seis, parity=not args.no_parity)
         out["cpu_baseline"] = base

@@ -87,6 +87,7 @@ int wv_init(wv_solver *s, const wv_config *cfg);
 /* model vectors [NX*NY*NZ] in grid index order:
  *   acoustic: m0 = velocityP, m1 ignored (may be NULL), m2 = density
  *   elastic : m0 = velocityP, m1 = velocityS, m2 = density
+ *   sh      : m0 ignored (may be NULL), m1 = velocityS, m2 = density
  *   tmem    : m0 = epsilon_r, m1 = sigma [S/m], m2 = mu_r                  */
 int wv_set_model(wv_solver *s, const float *m0, const float *m1, const float *m2);
 
@@ -95,6 +96,15 @@ int wv_set_model(wv_solver *s, const float *m0, const float *m1, const float *m2
  * Receivers keep the order given; seismograms come back in that order.      */
 int wv_set_acquisition(wv_solver *s, int shot, int nsrc, const long long *src_index, const int *src_type,
                        const float *src_signal, int nrec, const long long *rec_index, const int *rec_type);
+
+/* Synthetic source signal of wavelet shape `shape` into out[0, NT), evaluated in float
+ * as the reference's float build: Sources::generateSyntheticSignal
+ * (src/Acquisition/Sources.cpp:157-220) with src/Acquisition/SourceSignal/*.cpp:
+ * 1 Ricker, 2 SinW, 3 SinThree, 4 FGaussian, 5 Spike, 6 IntgSinThree, 7 Ricker_GprMax,
+ * 8 Berlage, 9 Sin; fc [Hz], amp, tshift [s] as in the source file's columns.  Host
+ * code only (no GPU, no handle).  WV_EINVAL for an unknown shape, NT/DT/fc <= 0, or a
+ * shape with a window (2, 3, 5, 6) shifted beyond the signal, with the reference's messages. */
+int wv_source_signal(int shape, int NT, double DT, double fc, double amp, double tshift, float *out);
 
 int wv_reset(wv_solver *s);
 

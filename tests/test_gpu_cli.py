@@ -134,3 +134,85 @@ def test_cli_su_output_and_trace_normalisation(tmp_path, norm):
         assert (h["sx"], h["sy"], h["sdepth"]) == (round(XS * 1000), round(ZS * 1000), round(YS * 1000))
         dist = np.sqrt((xr - XS) ** 2 + (yr - YS) ** 2 + (zr - ZS) ** 2) * 1000
         assert abs(h["offset"] - dist) <= 1.0
+
+
+def _write_rows(path, rows):
+    """Signals [nsrc][NT] as a MatrixMarket array (values column-major), the layout the reference writes."""
+    rows = np.asarray(rows, np.float32)
+    with open(path, "w") as f:
+        f.write("%%MatrixMarket matrix array real general\n")
+        f.write(f"{rows.shape[0]} {rows.shape[1]}\n")
+        np.savetxt(f, rows.T.reshape(-1).astype(np.float64), fmt="%.9g")
+
+
+def test_cli_source_signal_shapes_and_signal_files(tmp_path):
+    """Every synthetic shape (wavelet type 1, shapes 1..9, Sources.cpp:157-220) and signals read from
+    <SourceSignalFilename>.mtx (type 2: row 0 for every source; type 3: the row of the source's line in the
+    source file; both keep only the sign of the integer part of amp, Sources.cpp:37-45, 99-144, 354-375),
+    over two shots of one batched run: the driver's pressure seismograms equal the oracle's float build
+    driven by the same signals (relative L2 <= 1e-5)."""
+    import wavesim
+    from _cases import oracle_threads, rel_l2
+
+    cfg_path = os.path.join(CI, "configuration_ci.2D.acoustic.txt")
+    case = po.ci_case(cfg_path)
+    p = case["params"]
+    p.nthreads = oracle_threads()
+    NT, DT, NX = p.NT, p.DT, p.NX
+    rows = np.stack([wavesim.ricker(NT, DT, 4.0, 1.0, 0.1), wavesim.ricker(NT, DT, 6.0, 1.0, 0.3),
+                     wavesim.source_signal(3, NT, DT, 5.0, 1.0, 0.05), wavesim.ricker(NT, DT, 3.0, 2.0, 0.0),
+                     wavesim.source_signal(9, NT, DT, 4.0, 1.0, 0.2), wavesim.ricker(NT, DT, 5.0, 1.0, 0.15),
+                     wavesim.ricker(NT, DT, 7.0, 1.0, 0.05), wavesim.ricker(NT, DT, 4.5, 0.5, 0.0),
+                     wavesim.ricker(NT, DT, 5.5, 1.0, 0.25), wavesim.ricker(NT, DT, 6.5, 1.0, 0.35)])
+    sigfile = tmp_path / "signal"
+    _write_rows(f"{sigfile}.mtx", rows)
+    # shot 0: shapes 1..9 and two type-3 rows (amp 0.5 -> integer 0 -> sign -1; amp 3 -> +1);
+    # shot 1: type 2 (row 0) with amp -2 (-1) and 1.5 (+1)
+    lines = [(0, 20, 30, 1, 1, k, 4.0 + 0.3 * k, 1.0 + 0.1 * k, 0.02 * k) for k in range(1, 10)]
+    lines += [(0, 70, 40, 1, 3, 0, 0, 0.5, 0), (0, 50, 60, 1, 3, 0, 0, 3.0, 0)]
+    lines += [(1, 30, 50, 1, 2, 0, 0, -2.0, 0), (1, 65, 45, 1, 2, 0, 0, 1.5, 0)]
+    src = tmp_path / "src"
+    with open(f"{src}.txt", "w") as f:
+        f.write("# shot x y z type wavelet shape fc amp tshift\n")
+        for k, (shot, x, y, typ, wt, shape, fc, amp, ts) in enumerate(lines):
+            f.write(f"{shot} {x + 3 * (k % 9) * (wt == 1)} {y} 0 {typ} {wt} {shape} {fc:g} {amp:g} {ts:g}\n")
+    srcs = po.read_sources(f"{src}.txt")
+    rec = os.path.join(CI, os.path.basename(case["cfg"]["receiverfilename"].split()[0]))
+    out = tmp_path / "seis"
+    args = [EXE, cfg_path, f"SourceFilename={src}", f"ReceiverFilename={rec}", f"SeismogramFilename={out}",
+            "SeismogramFormat=1", f"SourceSignalFilename={sigfile}", "normalizeTraces=0", "initSourcesFromSU=0"]
+    r = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    for shot in (0, 1):
+        idx, typ, sig = [], [], []
+        for row, s in enumerate(srcs):
+            if s["shot"] != shot:
+                continue
+            idx.append(s["x"] + s["y"] * NX)
+            typ.append(s["type"])
+            if s["wtype"] == 1:
+                sig.append(wavesim.source_signal(s["shape"], NT, DT, s["fc"], s["amp"], s["tshift"]))
+            else:
+                base = rows[0] if s["wtype"] == 2 else rows[row]
+                sig.append(base if int(s["amp"]) > 0 else -base)
+        o32, _ = po.run(p, case["vp"], case["vs"], case["rho"], idx, typ, np.stack(sig).astype(np.float32),
+                        case["rec_idx"], case["rec_type"], "f32")
+        seis = po.read_mtx_array(f"{out}.shot_{shot}.p.mtx")
+        e = rel_l2(seis, o32)
+        print(f"shot {shot}: {len(idx)} sources, relL2 vs f32 oracle {e:.2e}")
+        assert np.abs(o32).max() > 0 and e <= 1e-5
+
+
+def test_cli_rejects_unknown_wavelets(tmp_path):
+    cfg_path = os.path.join(CI, "configuration_ci.2D.acoustic.txt")
+    rec = os.path.join(CI, "receiver_ci.2D")
+    for line, msg in (("0 50 50 0 1 1 12 5 1 0", "Unknown wavelet shape"), ("0 50 50 0 1 4 1 5 1 0", "Unknown wavelet type"),
+                      ("0 50 50 0 1 2 0 0 1 0\n0 40 50 0 1 3 0 0 1 0", "Combination of wavelet type 2 and 3")):
+        with open(tmp_path / "src.txt", "w") as f:
+            f.write(line + "\n")
+        _write_rows(tmp_path / "sig.mtx", np.zeros((2, 1000), np.float32))
+        r = subprocess.run([EXE, cfg_path, f"SourceFilename={tmp_path / 'src'}", f"ReceiverFilename={rec}",
+                            f"SeismogramFilename={tmp_path / 'o'}", f"SourceSignalFilename={tmp_path / 'sig'}"],
+                           capture_output=True, text=True, timeout=60)
+        assert r.returncode == 1 and msg in r.stderr, (line, r.stderr)

@@ -12,7 +12,15 @@ with the oracle on the same seeded inputs:
   C4  2D elastic adjoint, 600 x 600, 3000 steps, 16 shots     gradient of two shots' misfit vs the FP64
                                                               oracle adjoint (full FP64 pipeline), in both
                                                               memory plans (per-step FP16 record; FP16
-                                                              checkpoints every 50 steps)
+                                                              checkpoints every 50 steps).  The misfit is
+                                                              1/2 ||d||^2 of the anomaly model's data: the
+                                                              configuration's anomaly (centre x 303, y 433)
+                                                              is out of reach of the receivers within 3000
+                                                              steps, so the misfit against the background
+                                                              model's data is 0 in exact arithmetic (FP64
+                                                              oracle: 1.7e-28) and its gradient is rounding
+                                                              noise; the anomaly misfit is checked at reduced
+                                                              sizes (test_gpu_adjoint.py)
   C5  3D elastic, order 8, 320^3, 8 shots                     batched == single-shot bit-exact over 300
                                                               steps; one shot vs f32 oracle over the first
                                                               120 steps, its Ricker moved 88 ms earlier so
@@ -72,16 +80,13 @@ def _shot_ref(case, shot, precision, nthreads, fused=False):
 
 
 def _c4_ref(k, nthreads):
-    """Observed data (FP64 oracle, background model) and the FP64 oracle adjoint gradient of shot k's misfit."""
+    """FP64 oracle adjoint gradient of shot k's misfit 1/2 ||d||^2 (observed data 0) and its J."""
     c = cases.elastic_c4()
-    c0 = cases.elastic_c4(anomaly=False)
     p = oracle_params(c, precision_threads=nthreads)
-    sh0, sh = c0.shots[k], c.shots[k]
-    o, _ = po.run(p, c0.m0, c0.m1, c0.m2, sh0.src_index, sh0.src_type, sh0.src_signal, sh0.rec_index, sh0.rec_type,
-                  "f64")
-    _, _, _, g = po.adjoint_elastic2d(p, c.m0, c.m1, c.m2, sh.src_index, sh.src_type, sh.src_signal, sh.rec_index,
-                                      sh.rec_type, o)
-    return o, g
+    sh = c.shots[k]
+    _, J, _, g = po.adjoint_elastic2d(p, c.m0, c.m1, c.m2, sh.src_index, sh.src_type, sh.src_signal, sh.rec_index,
+                                      sh.rec_type, np.zeros((len(sh.rec_index), c.settings["NT"])))
+    return J, g
 
 
 @pytest.fixture(scope="module")
@@ -176,17 +181,21 @@ def test_c4_adjoint_full_size(plan, store, refs):
     residual, FP64 adjoint)."""
     c = cases.elastic_c4()
     assert (c.settings["NX"], c.settings["NT"], len(c.shots)) == (600, 3000, 16)
-    (o0, g0), (o1, g1) = _get(refs, ("c4", 0)), _get(refs, ("c4", 1))
-    obs, g64 = [o0, o1], g0 + g1
+    (J0, g0), (J1, g1) = _get(refs, ("c4", 0)), _get(refs, ("c4", 1))
+    g64 = g0 + g1
     sol = cases.make_solver(c, path="strip", tile_y=16)
     sol.set_adjoint_store(store)
     sol.set_checkpointing(50, True)
     sol.run(0, c.settings["NT"])
     sol.check_finite()
+    J = 0.0
     for k in range(16):
-        r = sol.seismograms(k).astype(np.float64)
-        r = r - obs[k] if k < 2 else np.zeros_like(r)
-        sol.set_adjoint_source(k, r.astype(np.float32))
+        d = sol.seismograms(k)
+        if k < 2:
+            J += 0.5 * float(np.sum(d.astype(np.float64) ** 2))
+        sol.set_adjoint_source(k, d if k < 2 else np.zeros_like(d))
+    print(f"C4 misfit 1/2 ||d||^2 of shots 0, 1: GPU {J:.6e}, FP64 oracle {J0 + J1:.6e}", flush=True)
+    assert abs(J - (J0 + J1)) <= 1e-5 * (J0 + J1)
     sol.run_adjoint()
     for k, name in enumerate(("vp", "vs", "rho")):
         e = rel_l2(sol.gradient(name), g64[k])

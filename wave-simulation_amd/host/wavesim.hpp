@@ -4,8 +4,9 @@
 // pieces of the reference's input/output the time loop needs: the KEY=VALUE
 // configuration file (Configuration.cpp:45-87), source / receiver text files
 // (AcquisitionSettings.hpp:24-100, 238-270), MatrixMarket vectors and
-// seismograms (IO.hpp:21-164, Seismogram.cpp:82-156), the Ricker wavelet
-// (Ricker.cpp:29-53) and the seismogram resampling (Common.hpp:202-233).
+// seismograms (IO.hpp:21-164, Seismogram.cpp:82-156), the source signals
+// (Sources.cpp:99-220, 231-249: synthetic shapes 1..9 or rows of a signal file)
+// and the seismogram resampling (Common.hpp:202-233).
 // Errors throw std::runtime_error, as the reference throws.
 #pragma once
 
@@ -39,6 +40,7 @@ class Configuration {
 struct SourceRow {
     int shot, x, y, z, type, wavelet, shape;
     double fc, amp, tshift;
+    long row;  // data line of the source file, from 0 (AcquisitionSettings.hpp:68-102)
 };
 struct ReceiverRow {
     int x, y, z, type;
@@ -46,7 +48,12 @@ struct ReceiverRow {
 
 std::vector<SourceRow> readSources(const std::string &path);
 std::vector<ReceiverRow> readReceivers(const std::string &path);
+// synthetic signal of wavelet shape 1..9 (Sources.cpp:157-220) from the library
+std::vector<float> sourceSignal(int shape, int NT, double DT, double fc, double amp, double tshift);
 std::vector<float> ricker(int NT, double DT, double fc, double amp, double tshift);
+// row `row` of a MatrixMarket matrix with `cols` columns, "array" or "coordinate"
+// (IO.hpp:172-195, DenseStorage::readFromFile of one row)
+std::vector<float> readMatrixRowMtx(const std::string &path, long row, size_t cols);
 
 // MatrixMarket "array real general" (values column-major)
 std::vector<float> readVectorMtx(const std::string &path, size_t expected);

@@ -101,7 +101,8 @@ std::vector<SourceRow> readSources(const std::string &path)
     std::vector<SourceRow> out;
     for (auto &t : readRows(path, 10))
         out.push_back({std::stoi(t[0]), std::stoi(t[1]), std::stoi(t[2]), std::stoi(t[3]), std::stoi(t[4]),
-                       std::stoi(t[5]), std::stoi(t[6]), std::stod(t[7]), std::stod(t[8]), std::stod(t[9])});
+                       std::stoi(t[5]), std::stoi(t[6]), std::stod(t[7]), std::stod(t[8]), std::stod(t[9]),
+                       (long)out.size()});
     return out;
 }
 
@@ -112,22 +113,16 @@ std::vector<ReceiverRow> readReceivers(const std::string &path)
     return out;
 }
 
-// Ricker.cpp:29-53 in the value type (float)
+std::vector<float> sourceSignal(int shape, int NT, double DT, double fc, double amp, double tshift)
+{
+    std::vector<float> w((size_t)std::max(NT, 1));
+    check(wv_source_signal(shape, NT, DT, fc, amp, tshift, w.data()));
+    return w;
+}
+
 std::vector<float> ricker(int NT, double DT, double fc, double amp, double tshift)
 {
-    std::vector<float> w(NT);
-    const float dt = (float)DT, f = (float)fc, a = (float)amp, ts = (float)tshift;
-    const float help = (float)(1.5 / (double)f + (double)ts);
-    const float pf = (float)(M_PI * (double)f);
-    for (int k = 0; k < NT; k++) {
-        const float t = (float)k * dt;
-        const float tau = (t - help) * pf;
-        const float h2 = tau * tau;
-        const float e = (float)std::exp(-(double)h2);
-        const float v = 1.f - 2.f * h2;
-        w[k] = a * v * e;
-    }
-    return w;
+    return sourceSignal(1, NT, DT, fc, amp, tshift);
 }
 
 // ---- MatrixMarket (IO.hpp:21-164)
@@ -153,6 +148,55 @@ std::vector<float> readVectorMtx(const std::string &path, size_t expected)
     }
     if (v.size() != expected) throw std::runtime_error(path + ": expected " + std::to_string(expected) + " values");
     return v;
+}
+
+std::vector<float> readMatrixRowMtx(const std::string &path, long row, size_t cols)
+{
+    std::ifstream in(path);
+    if (!in.good()) throw std::runtime_error("cannot open " + path);
+    std::string line;
+    bool banner = false, coordinate = false, header = false;
+    long R = 0, C = 0, nnz = 0;
+    size_t k = 0;
+    std::vector<float> out;
+    while (std::getline(in, line)) {
+        if (!banner && line.rfind("%%MatrixMarket", 0) == 0) {
+            banner = true;
+            coordinate = line.find("coordinate") != std::string::npos;
+            continue;
+        }
+        if (line.empty() || line[0] == '%') continue;
+        std::istringstream is(line);
+        if (!header) {
+            is >> R >> C;
+            if (coordinate) is >> nnz;
+            if (!is || R <= 0 || C <= 0) throw std::runtime_error(path + ": bad MatrixMarket size line");
+            if (row < 0 || row >= R)
+                throw std::runtime_error(path + ": row " + std::to_string(row) + " of a matrix with " + std::to_string(R) + " rows");
+            if ((size_t)C != cols) throw std::runtime_error(path + ": Source signal has invalid length");
+            out.assign(cols, 0.f);
+            header = true;
+            continue;
+        }
+        if (coordinate) {
+            long i, j;
+            double v;
+            if (is >> i >> j >> v) {
+                if (i < 1 || i > R || j < 1 || j > C) throw std::runtime_error(path + ": entry out of range");
+                if (i - 1 == row) out[(size_t)(j - 1)] = (float)v;
+            }
+        } else {
+            double v;
+            while (is >> v) {
+                if ((long)(k % (size_t)R) == row && k / (size_t)R < cols) out[k / (size_t)R] = (float)v;
+                k++;
+            }
+        }
+    }
+    if (!header) throw std::runtime_error(path + ": no MatrixMarket size line");
+    if (!coordinate && k != (size_t)R * (size_t)C)
+        throw std::runtime_error(path + ": expected " + std::to_string((size_t)R * C) + " values, read " + std::to_string(k));
+    return out;
 }
 
 void writeMatrixMtx(const std::string &path, const std::vector<float> &rowMajor, int rows, int cols)

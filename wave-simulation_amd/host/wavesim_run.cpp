@@ -6,7 +6,8 @@
 // Reads the reference configuration (later KEY=VALUE arguments override it),
 // the model (ModelRead=1: <ModelFilename>.{vp,vs,density}.mtx, TM-EM
 // {epsilonr,sigma,mur}; ModelRead=0: the homogeneous values of the
-// configuration), the source file (<SourceFilename>.txt, Ricker wavelets) and
+// configuration), the source file (<SourceFilename>.txt: synthetic wavelets of
+// shapes 1..9, or wavelet types 2 / 3, rows of <SourceSignalFilename>.mtx) and
 // receiver files (<ReceiverFilename>.txt, or .shot_<n>.txt with
 // useReceiversPerShot=1); runs every shot on the GPU in one batched handle and
 // writes <SeismogramFilename>.shot_<n>.<component>.mtx per shot and recorded
@@ -103,6 +104,19 @@ void writeSU(const std::string &fn, const std::vector<float> &d, int ntr, int ns
     if (!f) throw std::runtime_error("write failed: " + fn);
 }
 
+// Sources::readSignalFromFile (Sources.cpp:231-249) with initSourcesFromSU = 0: row `row`
+// of <SourceSignalFilename>.mtx (SeismogramFormat 1; IO.hpp:172-195), NT samples
+std::vector<float> signalFromFile(const Configuration &config, long row, int NT)
+{
+    if (config.getInt("initSourcesFromSU", 0) != 0)
+        throw std::runtime_error("initSourcesFromSU=1 (acquisition and signals from SU files) is not read");
+    const int format = (int)config.getInt("SeismogramFormat", 1);
+    if (format == 2 || format == 3)
+        throw std::runtime_error("source signal files in the lmf/frv formats are not read (SeismogramFormat 1 = mtx)");
+    if (format != 1) throw std::runtime_error("Unexpected fileFormat option!");
+    return readMatrixRowMtx(config.getString("SourceSignalFilename") + ".mtx", row, (size_t)NT);
+}
+
 }  // namespace
 
 int main(int argc, char **argv)
@@ -176,12 +190,25 @@ int main(int argc, char **argv)
             const std::string warn =
                 checkNumericalArtefactsAndInstabilities(dim, type, solver->config(), m0, m1, m2, fcMax);
             if (!warn.empty()) std::printf("shot %d: %s\n", shotNumbers[k], warn.c_str());
+            bool fromFile2 = false, fromFile3 = false;
             for (auto &s : src) {
                 if (s.shot != shotNumbers[k]) continue;
-                if (s.wavelet != 1 || s.shape != 1) throw std::runtime_error("only synthetic Ricker sources are generated");
                 si.push_back(lin(s.x, s.y, s.z));
                 st.push_back(s.type);
-                std::vector<float> w = ricker(NT, DT, s.fc, s.amp, s.tshift);
+                std::vector<float> w;
+                if (s.wavelet == 1) {
+                    w = sourceSignal(s.shape, NT, DT, s.fc, s.amp, s.tshift);
+                } else if (s.wavelet == 2 || s.wavelet == 3) {
+                    (s.wavelet == 2 ? fromFile2 : fromFile3) = true;
+                    if (fromFile2 && fromFile3) throw std::runtime_error("Combination of wavelet type 2 and 3 not supported");
+                    w = signalFromFile(config, s.wavelet == 2 ? 0 : s.row, NT);
+                    // copySignalsToSeismogramHandler keeps only the sign of the amplitude, held there as
+                    // an integer: 0 <= amp < 1 gives -1 (Sources.cpp:37-45, 354-375)
+                    if (!((long)s.amp > 0))
+                        for (auto &v : w) v = -v;
+                } else {
+                    throw std::runtime_error("Unknown wavelet type ");
+                }
                 sig.insert(sig.end(), w.begin(), w.end());
             }
             const std::string rf = config.getString("ReceiverFilename") +

@@ -5,4 +5,4 @@ C ABI in include/wave_abi.h); this package is the thin host-side binding.
 """
 from ._abi import WaveError, load  # noqa: F401
 from .solver import ForwardSolver, create  # noqa: F401
-from .acquisition import ricker  # noqa: F401
+from .acquisition import ricker, source_signal  # noqa: F401
