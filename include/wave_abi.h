@@ -99,7 +99,7 @@ int wv_set_acquisition(wv_solver *s, int shot, int nsrc, const long long *src_in
 
 /* Synthetic source signal of wavelet shape `shape` into out[0, NT), evaluated in float
  * as the reference's float build: Sources::generateSyntheticSignal
- * (src/Acquisition/Sources.cpp:157-220) with src/Acquisition/SourceSignal/*.cpp:
+ * (src/Acquisition/Sources.cpp:157-220) with the shapes of src/Acquisition/SourceSignal/:
  * 1 Ricker, 2 SinW, 3 SinThree, 4 FGaussian, 5 Spike, 6 IntgSinThree, 7 Ricker_GprMax,
  * 8 Berlage, 9 Sin; fc [Hz], amp, tshift [s] as in the source file's columns.  Host
  * code only (no GPU, no handle).  WV_EINVAL for an unknown shape, NT/DT/fc <= 0, or a

@@ -163,7 +163,8 @@ def test_cli_source_signal_shapes_and_signal_files(tmp_path):
                      wavesim.source_signal(3, NT, DT, 5.0, 1.0, 0.05), wavesim.ricker(NT, DT, 3.0, 2.0, 0.0),
                      wavesim.source_signal(9, NT, DT, 4.0, 1.0, 0.2), wavesim.ricker(NT, DT, 5.0, 1.0, 0.15),
                      wavesim.ricker(NT, DT, 7.0, 1.0, 0.05), wavesim.ricker(NT, DT, 4.5, 0.5, 0.0),
-                     wavesim.ricker(NT, DT, 5.5, 1.0, 0.25), wavesim.ricker(NT, DT, 6.5, 1.0, 0.35)])
+                     wavesim.ricker(NT, DT, 5.5, 1.0, 0.25), wavesim.ricker(NT, DT, 6.5, 1.0, 0.35),
+                     wavesim.ricker(NT, DT, 3.5, 1.5, 0.2), wavesim.source_signal(2, NT, DT, 4.0, 1.0, 0.1)])
     sigfile = tmp_path / "signal"
     _write_rows(f"{sigfile}.mtx", rows)
     # shot 0: shapes 1..9 and two type-3 rows (amp 0.5 -> integer 0 -> sign -1; amp 3 -> +1);
@@ -213,6 +214,7 @@ def test_cli_rejects_unknown_wavelets(tmp_path):
             f.write(line + "\n")
         _write_rows(tmp_path / "sig.mtx", np.zeros((2, 1000), np.float32))
         r = subprocess.run([EXE, cfg_path, f"SourceFilename={tmp_path / 'src'}", f"ReceiverFilename={rec}",
-                            f"SeismogramFilename={tmp_path / 'o'}", f"SourceSignalFilename={tmp_path / 'sig'}"],
+                            f"SeismogramFilename={tmp_path / 'o'}", f"SourceSignalFilename={tmp_path / 'sig'}",
+                            "SeismogramFormat=1"],
                            capture_output=True, text=True, timeout=60)
         assert r.returncode == 1 and msg in r.stderr, (line, r.stderr)
