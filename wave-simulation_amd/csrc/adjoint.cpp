@@ -413,6 +413,7 @@ int wv_run_adjoint(wv_solver *s)
             float *base;
             if ((rc = adalloc(a->allocs, &base, ext.size(), st))) return rc;
             HIP_TRY(hipMemcpyAsync(base, ext.data(), ext.size() * sizeof(float), hipMemcpyHostToDevice, st));
+            HIP_TRY(hipStreamSynchronize(st));  // ext is freed at the end of this iteration
             a->tapT[q] = base + (size_t)ORG * (2 * H + 1);
         }
     }

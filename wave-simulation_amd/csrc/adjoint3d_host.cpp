@@ -271,6 +271,7 @@ int Elastic3D::set_checkpointing(int every, int fp16, double budget_gib)
             float *base;
             if ((rc = dal((void **)&base, ext.size() * 4))) return rc;
             HIPA(hipMemcpyAsync(base, ext.data(), ext.size() * 4, hipMemcpyHostToDevice, st));
+            HIPA(hipStreamSynchronize(st));  // ext is freed at the end of this iteration
             a->tapT[q] = base + (size_t)ORG * K;
         }
     }

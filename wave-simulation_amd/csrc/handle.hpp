@@ -125,7 +125,9 @@ struct wv_solver {
     {
         int rc = dalloc(p, h.size(), list);
         if (rc) return rc;
-        if (!h.empty() && hipMemcpyAsync(*p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, st) != hipSuccess)
+        // pageable source: the copy is complete before the caller may free it
+        if (!h.empty() && (hipMemcpyAsync(*p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, st) != hipSuccess ||
+                           hipStreamSynchronize(st) != hipSuccess))
             return fail(WV_EDEVICE, "hipMemcpy failed");
         return 0;
     }

@@ -389,6 +389,8 @@ int PlainSolver::set_acquisition(const std::vector<long long> &sidx, const std::
                 return rc;
             (void)hipMemcpyAsync(rec_[c].idx, ri.data(), ri.size() * 4, hipMemcpyHostToDevice, st_);
         }
+        // si, sg and ri are freed at the end of this iteration: their copies complete first
+        if (hipStreamSynchronize(st_) != hipSuccess) return WV_EDEVICE;
     }
     return hipStreamSynchronize(st_) == hipSuccess ? 0 : WV_EDEVICE;
 }

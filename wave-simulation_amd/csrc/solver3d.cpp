@@ -80,7 +80,11 @@ int Elastic3D::upload(T **p, const std::vector<T> &h, std::vector<void *> &list)
     if (hipMalloc(&q, n * sizeof(T)) != hipSuccess) return fail(WV_ENOMEM, "hipMalloc failed");
     list.push_back(q);
     HIP3(hipMemsetAsync(q, 0, n * sizeof(T), st));
-    if (!h.empty()) HIP3(hipMemcpyAsync(q, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, st));
+    if (!h.empty()) {
+        // pageable source: the copy is complete before the caller may free it
+        HIP3(hipMemcpyAsync(q, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, st));
+        HIP3(hipStreamSynchronize(st));
+    }
     *p = (T *)q;
     return 0;
 }
