@@ -78,7 +78,9 @@ typedef struct {
                               3D elastic: rows of the x-z tile 4, 8, 16 (default 8) */
 } wv_config;
 
-/* dimension "2D"/"3D", type "acoustic"/"elastic"/"tmem" (case-insensitive) */
+/* dimension "2D"/"3D", type (case-insensitive): 2D "acoustic", "elastic", "viscoelastic", "sh",
+ * "tmem"; 3D "acoustic", "elastic".  The factory's other types ("viscosh", "emem", "viscotmem",
+ * "viscoemem", 3D others) return WV_EINVAL ("not provided"), unknown ones WV_EINVAL. */
 int wv_create(const char *dimension, const char *type, wv_solver **out);
 int wv_destroy(wv_solver *s);
 
@@ -87,9 +89,21 @@ int wv_init(wv_solver *s, const wv_config *cfg);
 /* model vectors [NX*NY*NZ] in grid index order:
  *   acoustic: m0 = velocityP, m1 ignored (may be NULL), m2 = density
  *   elastic : m0 = velocityP, m1 = velocityS, m2 = density
+ *   viscoelastic: as elastic, after wv_set_relaxation
  *   sh      : m0 ignored (may be NULL), m1 = velocityS, m2 = density
  *   tmem    : m0 = epsilon_r, m1 = sigma [S/m], m2 = mu_r                  */
 int wv_set_model(wv_solver *s, const float *m0, const float *m1, const float *m2);
+
+/* Relaxation mechanisms of the 2D viscoelastic solver ("viscoelastic" in wv_create): nmech
+ * (1..4) relaxation frequencies [Hz] and tauP, tauS per cell [NX*NY] (grid order), as
+ * Modelparameter::Viscoelastic holds them (numRelaxationMechanisms, relaxationFrequency..4,
+ * the .tauP / .tauS model files; Viscoelastic.cpp:160-260, 585-596) and
+ * FD2Dviscoelastic::initForwardSolver derives its constants from them
+ * (ForwardSolver2Dviscoelastic.cpp:40-60).  Call after wv_init and before wv_set_model: the
+ * moduli velocityP / velocityS given to wv_set_model are taken at the reference frequency
+ * CenterFrequencyCPML (wv_config.fc_cpml) and scaled to the relaxed ones
+ * (Viscoelastic.cpp:650-700).  WV_EINVAL for other solvers or bad arguments. */
+int wv_set_relaxation(wv_solver *s, int nmech, const double *freqs, const float *tauP, const float *tauS);
 
 /* Acquisition of one shot.  type: 1 = P (EZ for tmem), 2 = VX, 3 = VY, 4 = VZ.
  * src_signal is [nsrc][NT] (already scaled, as the reference's source seismogram).

@@ -21,7 +21,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 
-PHYSICS = {"acoustic": 0, "elastic": 1, "tmem": 2, "sh": 3}
+PHYSICS = {"acoustic": 0, "elastic": 1, "tmem": 2, "sh": 3, "viscoelastic": 4, "visco": 4}
 
 
 class Params(C.Structure):
@@ -86,8 +86,9 @@ def make_params(**kw) -> Params:
 
 
 def run(params: Params, m0, m1, m2, src_idx, src_type, src_sig, rec_idx, rec_type, precision="f32", want_fields=False,
-        t_end=-1):
-    """Run one shot; returns (seis [nrec, NT], fields or None)."""
+        t_end=-1, visco=None):
+    """Run one shot; returns (seis [nrec, NT], fields or None).  Viscoelastic (physics 4) needs
+    visco = dict(freqs=[relaxation frequencies, one per mechanism], tauP=..., tauS=... per cell)."""
     L = lib(precision)
     dt = dtype_of(precision)
     real = C.c_float if precision == "f32" else C.c_double
@@ -101,8 +102,14 @@ def run(params: Params, m0, m1, m2, src_idx, src_type, src_sig, rec_idx, rec_typ
     rec_idx = np.ascontiguousarray(rec_idx, dtype=np.int64)
     rec_type = np.ascontiguousarray(rec_type, dtype=np.int32)
     seis = np.zeros((len(rec_idx), params.NT), dtype=dt)
-    nf = {0: 3 if params.dim == 2 else 4, 1: 5 if params.dim == 2 else 9, 2: 3, 3: 3}[params.physics]
+    nf = {0: 3 if params.dim == 2 else 4, 1: 5 if params.dim == 2 else 9, 2: 3, 3: 3, 4: 5}[params.physics]
     fields = np.zeros((nf, n), dtype=dt) if want_fields else None
+    if params.physics == 4:
+        freqs = np.ascontiguousarray(visco["freqs"], dtype=np.float64)
+        tp = np.ascontiguousarray(np.broadcast_to(visco["tauP"], (n,)), dtype=dt)
+        ts = np.ascontiguousarray(np.broadcast_to(visco["tauS"], (n,)), dtype=dt)
+        if L.oracle_set_visco(C.c_int(len(freqs)), _ptr(freqs, C.c_double), _ptr(tp, real), _ptr(ts, real)) != 0:
+            raise RuntimeError("oracle_set_visco: 1..4 mechanisms")
     rc = L.oracle_run(C.byref(params), _ptr(m0, real), _ptr(m1, real), _ptr(m2, real), C.c_int(len(src_idx)),
                       _ptr(src_idx, C.c_long), _ptr(src_type, C.c_int), _ptr(src_sig, real), C.c_int(len(rec_idx)),
                       _ptr(rec_idx, C.c_long), _ptr(rec_type, C.c_int), _ptr(seis, real),
@@ -388,7 +395,20 @@ def ci_case(cfg_path):
         return r["x"] + r["z"] * NX + r["y"] * NX * NZ
 
     ref_name = os.path.basename(cfg["seismogramfilename"]).split(".ci")[0] + ".ref"
-    return dict(cfg=cfg, params=p, vp=vp, vs=vs, rho=rho, sources=srcs, receivers=recs,
+    visco = None
+    if physics == 4:
+        # Viscoelastic.cpp:160-200: numRelaxationMechanisms, relaxationFrequency(2..4); tauP/tauS
+        # from the model files (TwoLayer.cpp:25-26, 83-86: 0.1 in both layers) or the configuration
+        nl = cfg_num(cfg, "numRelaxationMechanisms", int)
+        keys = ["relaxationFrequency", "relaxationFrequency2", "relaxationFrequency3", "relaxationFrequency4"]
+        freqs = [cfg_num(cfg, keys[l]) for l in range(nl)]
+        if cfg_num(cfg, "ModelRead", int) == 1:
+            tp = ts = np.full(NX * NY * NZ, 0.1)
+        else:
+            tp = np.full(NX * NY * NZ, cfg_num(cfg, "tauP"))
+            ts = np.full(NX * NY * NZ, cfg_num(cfg, "tauS"))
+        visco = dict(freqs=freqs, tauP=tp, tauS=ts)
+    return dict(cfg=cfg, params=p, vp=vp, vs=vs, rho=rho, sources=srcs, receivers=recs, visco=visco,
                 src_idx=[lin(s) for s in srcs], src_type=[s["type"] for s in srcs],
                 rec_idx=[lin(r) for r in recs], rec_type=[r["type"] for r in recs], ref_base=os.path.join(d, ref_name))
 

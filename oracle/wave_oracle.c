@@ -622,7 +622,23 @@ typedef struct {
     double damping_coeff, npower, fc_cpml, vmax_cpml;
 } oracle_params;
 
-enum { PH_ACOUSTIC = 0, PH_ELASTIC = 1, PH_TMEM = 2, PH_SH = 3 };
+enum { PH_ACOUSTIC = 0, PH_ELASTIC = 1, PH_TMEM = 2, PH_SH = 3, PH_VISCO = 4 };
+
+/* Viscoelastic parameters of the next oracle_run (2D viscoelastic only): L relaxation
+ * mechanisms with their relaxation frequencies [Hz], tauP and tauS per cell (grid order).
+ * Held by pointer: the caller keeps the arrays alive during the run. */
+static int g_visco_L = 0;
+static double g_visco_f[4];
+static const real *g_tauP = NULL, *g_tauS = NULL;
+int oracle_set_visco(int L, const double *freqs, const real *tauP, const real *tauS)
+{
+    if (L < 1 || L > 4 || !freqs || !tauP || !tauS) return -1;
+    g_visco_L = L;
+    for (int l = 0; l < L; l++) g_visco_f[l] = freqs[l];
+    g_tauP = tauP;
+    g_tauS = tauS;
+    return 0;
+}
 
 typedef struct {
     csr_t Dxf, Dxb, Dyf, Dyb, Dzf, Dzb, DyfFS, DybFS;
@@ -670,7 +686,7 @@ static void ops_build(const grid_t *g, const oracle_params *p, ops_t *o)
         o->have_fs = 1;
         o->DyfFS = build_dyf_fs(g, p->order, DH);
         csr_scale(&o->DyfFS, DT);
-        if (p->physics == PH_ELASTIC || p->physics == PH_SH) {
+        if (p->physics == PH_ELASTIC || p->physics == PH_SH || p->physics == PH_VISCO) {
             o->DybFS = build_dyb_fs(g, p->order, DH);
             csr_scale(&o->DybFS, DT);
         }
@@ -773,7 +789,8 @@ int oracle_run(const oracle_params *p, const real *m0, const real *m1, const rea
 #ifdef _OPENMP
     if (p->nthreads > 0) omp_set_num_threads(p->nthreads);
 #endif
-    if (p->dim == 3 && (p->physics == PH_TMEM || p->physics == PH_SH)) return -2;
+    if (p->dim == 3 && (p->physics == PH_TMEM || p->physics == PH_SH || p->physics == PH_VISCO)) return -2;
+    if (p->physics == PH_VISCO && (g_visco_L < 1 || !g_tauP || !g_tauS)) return -3;
     grid_t g = {p->NX, p->NY, p->NZ, (long)p->NX * p->NY * p->NZ};
     long n = g.n;
     int NT = p->NT;
@@ -787,7 +804,11 @@ int oracle_run(const oracle_params *p, const real *m0, const real *m1, const rea
     /* model coefficients */
     real *M = NULL, *mu = NULL, *irx = NULL, *iry = NULL, *irz = NULL, *muxy = NULL, *muxz = NULL, *muyz = NULL;
     real *Ca = NULL, *Cb = NULL, *imxz = NULL, *imyz = NULL;
-    if (p->physics == PH_ACOUSTIC || p->physics == PH_ELASTIC) {
+    /* viscoelastic: relaxed moduli, (1 + L tau), averaged tauS, per-mechanism constants */
+    int VL = p->physics == PH_VISCO ? g_visco_L : 0;
+    real *opP = NULL, *opS = NULL, *tauSxy = NULL, *Rv[12] = {0};
+    real vtau[4] = {0}, vitau[4] = {0}, vc1[4] = {0}, vc2[4] = {0}, dth = 0;
+    if (p->physics == PH_ACOUSTIC || p->physics == PH_ELASTIC || p->physics == PH_VISCO) {
         M = vnew(n);
         for (long i = 0; i < n; i++) M[i] = m2[i] * m0[i] * m0[i];
         irx = vnew(n);
@@ -797,6 +818,53 @@ int oracle_run(const oracle_params *p, const real *m0, const real *m1, const rea
         if (p->dim == 3) {
             irz = vnew(n);
             inverse_average_2pt(&g, 2, m2, irz);
+        }
+        if (p->physics == PH_VISCO) {
+            /* Viscoelastic.cpp:650-700: the moduli at the reference frequency CenterFrequencyCPML
+             * scaled to the relaxed ones, M / (1 + sum tauP), mu / (1 + sum tauS) */
+            mu = vnew(n);
+            for (long i = 0; i < n; i++) mu[i] = m2[i] * m1[i] * m1[i];
+            real w_ref = (real)(2.0 * M_PI * (double)(real)p->fc_cpml);
+            real sum = 0;
+            for (int l = 0; l < VL; l++) {
+                real ts = (real)(1.0 / (2.0 * M_PI * (double)(real)g_visco_f[l]));
+                real num = w_ref * w_ref * ts * ts;
+                sum = (real)((double)sum + (double)num / (1.0 + (double)(w_ref * w_ref * ts * ts)));
+            }
+            for (long i = 0; i < n; i++) {
+                real tp = sum * g_tauP[i];
+                tp = (real)1.0 + tp;
+                M[i] = M[i] / tp;
+                real tsv = sum * g_tauS[i];
+                tsv = (real)1.0 + tsv;
+                mu[i] = mu[i] / tsv;
+            }
+            muxy = vnew(n);
+            shear_average(&g, mu, 0, 1, muxy); /* mu < 1 -> 1 in place, as the reference */
+            /* ModelparameterSeismic averaging: tauS with the 4-point matrix of mu_xy */
+            tauSxy = vnew(n);
+            for (long i = 0; i < n; i++) {
+                long x, y, z;
+                gcoord(&g, i, &x, &y, &z);
+                tauSxy[i] = avg4_row(&g, g_tauS, x, y, z, 0, 1);
+            }
+            /* ForwardSolver2Dviscoelastic.cpp:40-60, 85-100 */
+            opP = vnew(n);
+            opS = vnew(n);
+            for (long i = 0; i < n; i++) {
+                real a = (real)VL * g_tauP[i];
+                opP[i] = (real)1.0 + a;
+                real b = (real)VL * g_tauS[i];
+                opS[i] = (real)1.0 + b;
+            }
+            for (int l = 0; l < VL; l++) {
+                vtau[l] = (real)(1.0 / (2.0 * M_PI * (double)(real)g_visco_f[l]));
+                vitau[l] = (real)(1.0 / (double)vtau[l]);
+                vc1[l] = (real)(1.0 - (double)DT / (2.0 * (double)vtau[l]));
+                vc2[l] = (real)(1.0 / (1.0 + (double)DT / (2.0 * (double)vtau[l])));
+            }
+            dth = (real)((double)DT / 2.0);
+            for (int k = 0; k < 3 * VL; k++) Rv[k] = vnew(n); /* Rxx[l], Ryy[l], Rxy[l] */
         }
         if (p->physics == PH_ELASTIC) {
             mu = vnew(n);
@@ -852,14 +920,54 @@ int oracle_run(const oracle_params *p, const real *m0, const real *m1, const rea
         free(mup);
     }
 
-    /* free surface scaling (FreeSurfaceElastic.cpp:11-47) */
-    real *fsH = NULL, *fsV = NULL, *fsZero = NULL;
+    /* free surface scaling (FreeSurfaceElastic.cpp:11-47; viscoelastic:
+     * FreeSurfaceViscoelastic.cpp:10-60, per mechanism also the relaxation updates) */
+    real *fsH = NULL, *fsV = NULL, *fsZero = NULL, *fsRH[4] = {0}, *fsRV[4] = {0};
     if (fs) {
         fsZero = vnew(n);
         for (long i = 0; i < n; i++) {
             long x, y, z;
             gcoord(&g, i, &x, &y, &z);
             fsZero[i] = (y == 0) ? (real)0 : (real)1;
+        }
+        if (p->physics == PH_VISCO) {
+            fsH = vnew(n);
+            fsV = vnew(n);
+            for (int l = 0; l < VL; l++) {
+                fsRH[l] = vnew(n);
+                fsRV[l] = vnew(n);
+            }
+            for (long i = 0; i < n; i++) {
+                long x, y, z;
+                gcoord(&g, i, &x, &y, &z);
+                if (y != 0) continue;
+                real t = (real)-2 * mu[i];
+                t *= opS[i];
+                real t2 = M[i] * opP[i];
+                t += t2;
+                t2 = (real)1 / t2;
+                fsV[i] = t;
+                real h = (real)-1 * t;
+                h *= t;
+                h *= t2;
+                fsH[i] = h;
+                for (int l = 0; l < VL; l++) {
+                    real a = (real)2 * mu[i], b = M[i];
+                    real t3 = a * g_tauS[i];
+                    t3 -= b * g_tauP[i];
+                    real rv = t3 * vc2[l];
+                    rv /= vtau[l];
+                    fsRV[l][i] = rv;
+                    a *= opS[i];
+                    b *= opP[i];
+                    a /= b;
+                    a -= (real)1;
+                    real rh = t3 * a;
+                    rh *= vc2[l];
+                    rh /= vtau[l];
+                    fsRH[l][i] = rh;
+                }
+            }
         }
         if (p->physics == PH_ELASTIC) {
             fsH = vnew(n);
@@ -887,7 +995,8 @@ int oracle_run(const oracle_params *p, const real *m0, const real *m1, const rea
     if (p->damping == 1) damp = abs_layout(&g, p->dim, p->width, (real)p->damping_coeff, p->free_surface);
 
     /* wavefields and scratch */
-    int nf = p->physics == PH_ELASTIC ? (p->dim == 3 ? 9 : 5) : (p->dim == 3 && p->physics == PH_ACOUSTIC ? 4 : 3);
+    int nf = p->physics == PH_ELASTIC ? (p->dim == 3 ? 9 : 5)
+                                      : (p->physics == PH_VISCO ? 5 : (p->dim == 3 && p->physics == PH_ACOUSTIC ? 4 : 3));
     real *F[9];
     for (int k = 0; k < nf; k++) F[k] = vnew(n);
     real *upd = vnew(n), *tmp = vnew(n), *vxx = vnew(n), *vyy = vnew(n), *vzz = vnew(n);
@@ -958,8 +1067,9 @@ int oracle_run(const oracle_params *p, const real *m0, const real *m1, const rea
                 v_mul(n, vy, damp);
             }
             if (fs) v_mul(n, pr, fsZero);
-        } else if (p->physics == PH_ELASTIC && p->dim == 2) {
-            /* ForwardSolver2Delastic.cpp:117-294 ; F: vx vy sxx syy sxy */
+        } else if ((p->physics == PH_ELASTIC || p->physics == PH_VISCO) && p->dim == 2) {
+            /* ForwardSolver2Delastic.cpp:117-294 (ForwardSolver2Dviscoelastic.cpp:140-190: the
+             * same velocity update) ; F: vx vy sxx syy sxy */
             real *vx = F[0], *vy = F[1], *sxx = F[2], *syy = F[3], *sxy = F[4];
             spmv(&o.Dxf, sxx, upd);
             if (use_cpml) cpml_apply(n, upd, psi[0], PLANT(0, axh, ax), PLANT(0, bxh, bx)); /* sxx_x */
@@ -983,42 +1093,133 @@ int oracle_run(const oracle_params *p, const real *m0, const real *m1, const rea
                 cpml_apply(n, vxx, psi[4], PLANT(4, ax, axh), PLANT(4, bx, bxh));
                 cpml_apply(n, vyy, psi[5], PLANT(5, ay, ayh), PLANT(5, by, byh));
             }
-            v_copy(n, upd, vxx);
-            v_add(n, upd, vyy);
-            v_mul(n, upd, M);
-            v_add(n, sxx, upd);
-            v_add(n, syy, upd);
-            v_copy(n, upd, vyy);
-            v_mul(n, upd, mu);
-            v_sub2(n, sxx, upd);
-            v_copy(n, upd, vxx);
-            v_mul(n, upd, mu);
-            v_sub2(n, syy, upd);
+            if (p->physics == PH_VISCO) {
+                /* ForwardSolver2Dviscoelastic.cpp:190-300: memory variables Rxx, Ryy, Rxy of
+                 * each mechanism (Crank-Nicolson in the relaxation term) */
+                real **Rxx = Rv, **Ryy = Rv + VL, **Rxy = Rv + 2 * VL;
+                v_copy(n, upd, vxx);
+                v_add(n, upd, vyy);
+                v_mul(n, upd, M);
+                for (int l = 0; l < VL; l++)
+                    VLOOP(n)
+                    {
+                        real u2 = vitau[l] * upd[i];
+                        u2 *= g_tauP[i];
+                        sxx[i] += dth * Rxx[l][i];
+                        Rxx[l][i] *= vc1[l];
+                        Rxx[l][i] -= u2;
+                        syy[i] += dth * Ryy[l][i];
+                        Ryy[l][i] *= vc1[l];
+                        Ryy[l][i] -= u2;
+                    }
+                v_mul(n, upd, opP);
+                v_add(n, sxx, upd);
+                v_add(n, syy, upd);
+                VLOOP(n) upd[i] = vyy[i] * mu[i] * (real)2.0;
+                for (int l = 0; l < VL; l++)
+                    VLOOP(n)
+                    {
+                        real u2 = vitau[l] * upd[i];
+                        u2 *= g_tauS[i];
+                        Rxx[l][i] += u2;
+                        Rxx[l][i] *= vc2[l];
+                        sxx[i] += dth * Rxx[l][i];
+                    }
+                v_mul(n, upd, opS);
+                v_sub(n, sxx, upd);
+                VLOOP(n) upd[i] = vxx[i] * mu[i] * (real)2.0;
+                for (int l = 0; l < VL; l++)
+                    VLOOP(n)
+                    {
+                        real u2 = vitau[l] * upd[i];
+                        u2 *= g_tauS[i];
+                        Ryy[l][i] += u2;
+                        Ryy[l][i] *= vc2[l];
+                        syy[i] += dth * Ryy[l][i];
+                    }
+                v_mul(n, upd, opS);
+                v_sub(n, syy, upd);
 
-            spmv(&o.Dyf, vx, upd);
-            if (use_cpml) cpml_apply(n, upd, psi[6], PLANT(6, ayh, ay), PLANT(6, byh, by)); /* vxy */
-            spmv(&o.Dxf, vy, tmp);
-            if (use_cpml) cpml_apply(n, tmp, psi[7], PLANT(7, axh, ax), PLANT(7, bxh, bx)); /* vyx */
-            v_add(n, upd, tmp);
-            v_mul(n, upd, muxy);
-            v_add(n, sxy, upd);
-
-            if (fs) {
-                /* FreeSurface2Delastic.cpp:14-45 */
-                for (long i = 0; i < g.NX; i++) {
-                    real a = fsH[i] * vxx[i];
-                    sxx[i] += a;
-                    real b = fsV[i] * vyy[i];
-                    sxx[i] -= b;
+                spmv(&o.Dyf, vx, upd);
+                if (use_cpml) cpml_apply(n, upd, psi[6], ayh, byh); /* vxy */
+                spmv(&o.Dxf, vy, tmp);
+                if (use_cpml) cpml_apply(n, tmp, psi[7], axh, bxh); /* vyx */
+                v_add(n, upd, tmp);
+                v_mul(n, upd, muxy);
+                for (int l = 0; l < VL; l++)
+                    VLOOP(n)
+                    {
+                        sxy[i] += dth * Rxy[l][i];
+                        Rxy[l][i] *= vc1[l];
+                        real u2 = vitau[l] * upd[i];
+                        u2 *= tauSxy[i];
+                        Rxy[l][i] -= u2;
+                        Rxy[l][i] *= vc2[l];
+                        sxy[i] += dth * Rxy[l][i];
+                    }
+                v_mul(n, upd, opS);
+                v_add(n, sxy, upd);
+                if (fs) {
+                    /* FreeSurface2Dviscoelastic.cpp:14-45 on the surface row, then syy and the
+                     * Ryy of every mechanism zero there */
+                    for (long i = 0; i < g.NX; i++) {
+                        for (int l = 0; l < VL; l++) sxx[i] -= dth * Rxx[l][i];
+                        sxx[i] += fsH[i] * vxx[i];
+                        sxx[i] -= fsV[i] * vyy[i];
+                        for (int l = 0; l < VL; l++) {
+                            Rxx[l][i] += fsRH[l][i] * vxx[i];
+                            Rxx[l][i] -= fsRV[l][i] * vyy[i];
+                            sxx[i] += dth * Rxx[l][i];
+                        }
+                    }
+                    v_mul(n, syy, fsZero);
+                    for (int l = 0; l < VL; l++) v_mul(n, Ryy[l], fsZero);
                 }
-                v_mul(n, syy, fsZero);
-            }
-            if (damp) {
-                v_mul(n, sxx, damp);
-                v_mul(n, syy, damp);
-                v_mul(n, sxy, damp);
-                v_mul(n, vx, damp);
-                v_mul(n, vy, damp);
+                if (damp) {
+                    v_mul(n, sxx, damp);
+                    v_mul(n, syy, damp);
+                    v_mul(n, sxy, damp);
+                    v_mul(n, vx, damp);
+                    v_mul(n, vy, damp);
+                }
+            } else {
+                v_copy(n, upd, vxx);
+                v_add(n, upd, vyy);
+                v_mul(n, upd, M);
+                v_add(n, sxx, upd);
+                v_add(n, syy, upd);
+                v_copy(n, upd, vyy);
+                v_mul(n, upd, mu);
+                v_sub2(n, sxx, upd);
+                v_copy(n, upd, vxx);
+                v_mul(n, upd, mu);
+                v_sub2(n, syy, upd);
+
+                spmv(&o.Dyf, vx, upd);
+                if (use_cpml) cpml_apply(n, upd, psi[6], PLANT(6, ayh, ay), PLANT(6, byh, by)); /* vxy */
+                spmv(&o.Dxf, vy, tmp);
+                if (use_cpml) cpml_apply(n, tmp, psi[7], PLANT(7, axh, ax), PLANT(7, bxh, bx)); /* vyx */
+                v_add(n, upd, tmp);
+                v_mul(n, upd, muxy);
+                v_add(n, sxy, upd);
+
+                if (fs) {
+                    /* FreeSurface2Delastic.cpp:14-45 */
+                    for (long i = 0; i < g.NX; i++) {
+                        real a = fsH[i] * vxx[i];
+                        sxx[i] += a;
+                        real b = fsV[i] * vyy[i];
+                        sxx[i] -= b;
+                    }
+                    v_mul(n, syy, fsZero);
+                }
+                if (damp) {
+                    v_mul(n, sxx, damp);
+                    v_mul(n, syy, damp);
+                    v_mul(n, sxy, damp);
+                    v_mul(n, vx, damp);
+                    v_mul(n, vy, damp);
+                }
             }
         } else if (p->physics == PH_ELASTIC && p->dim == 3) {
             /* ForwardSolver3Delastic.cpp:120-416 ; F: vx vy vz sxx syy szz syz sxz sxy */
@@ -1187,7 +1388,7 @@ int oracle_run(const oracle_params *p, const real *m0, const real *m1, const rea
                     if (ty == 2) F[0][i] += v;
                     if (ty == 3) F[1][i] += v;
                     if (ty == 4 && p->dim == 3) F[2][i] += v;
-                } else if (p->physics == PH_ELASTIC && p->dim == 2) {
+                } else if ((p->physics == PH_ELASTIC || p->physics == PH_VISCO) && p->dim == 2) {
                     if (ty == 1) {
                         F[2][i] += v;
                         F[3][i] += v;
@@ -1216,7 +1417,7 @@ int oracle_run(const oracle_params *p, const real *m0, const real *m1, const rea
             real v = 0;
             if (p->physics == PH_ACOUSTIC) {
                 v = ty == 1 ? F[nf - 1][i] : (ty == 2 ? F[0][i] : (ty == 3 ? F[1][i] : F[2][i]));
-            } else if (p->physics == PH_ELASTIC && p->dim == 2) {
+            } else if ((p->physics == PH_ELASTIC || p->physics == PH_VISCO) && p->dim == 2) {
                 if (ty == 1) {
                     v = F[2][i];
                     v += F[3][i];
@@ -1273,6 +1474,14 @@ int oracle_run(const oracle_params *p, const real *m0, const real *m1, const rea
     free(fsH);
     free(fsV);
     free(fsZero);
+    for (int l = 0; l < 4; l++) {
+        free(fsRH[l]);
+        free(fsRV[l]);
+    }
+    for (int k = 0; k < 12; k++) free(Rv[k]);
+    free(opP);
+    free(opS);
+    free(tauSxy);
     ops_free(&o);
     return 0;
 }

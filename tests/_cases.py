@@ -5,7 +5,7 @@ import numpy as np
 
 from oracle import pyoracle as po
 
-PHYS = {"acoustic": 0, "elastic": 1, "tmem": 2, "sh": 3}
+PHYS = {"acoustic": 0, "elastic": 1, "tmem": 2, "sh": 3, "viscoelastic": 4}
 
 
 def oracle_threads():
@@ -29,7 +29,8 @@ def oracle_shot(case, shot, precision="f32", ops_mode="sparse", want_fields=Fals
     m1 = case.m1 if case.m1 is not None else np.zeros_like(case.m0)
     dt = np.float32 if precision == "f32" else np.float64
     return po.run(p, case.m0.astype(dt), m1.astype(dt), case.m2.astype(dt), shot.src_index, shot.src_type,
-                  shot.src_signal.astype(dt), shot.rec_index, shot.rec_type, precision, want_fields=want_fields)
+                  shot.src_signal.astype(dt), shot.rec_index, shot.rec_type, precision, want_fields=want_fields,
+                  visco=case.visco)
 
 
 def rel_l2(a, b):

@@ -1,3 +1,4 @@
+import dataclasses
 """Invariants that pin the oracle's CPML and TM-EM paths (no reference fixture covers them).
 
 * CPML absorbs: a run in a small domain with CPML matches, at the receivers, a
@@ -164,3 +165,50 @@ def test_elastic3d_cpml_absorbs_and_each_term_placement_matters():
     # measured: 4.0e-5 correct, 6.6e-4 .. 7.3e-3 with any one of the 18 terms swapped
     assert e < 2e-4
     assert min(swapped) > 4e-4
+
+
+def _visco_case(L_freqs, tau, free_surface=0, damping=2):
+    from wavesim import cases
+
+    c = cases.visco_2d(NX=90, NY=70, NT=300, nshots=1, free_surface=free_surface, damping=damping, W=10,
+                       freqs=L_freqs, tau=tau)
+    return c
+
+
+def test_visco_with_zero_tau_is_the_elastic_loop():
+    """tauP = tauS = 0: the memory variables stay zero and (1 + L tau) = 1, so the viscoelastic
+    step is the elastic one to the bit (CPML, no free surface: the elastic free surface forms its
+    scales in another order)."""
+    from _cases import oracle_shot
+
+    c = _visco_case([20.0, 40.0], 0.0)
+    c.visco["tauP"] = np.zeros_like(c.visco["tauP"])
+    c.visco["tauS"] = np.zeros_like(c.visco["tauS"])
+    v, fv = oracle_shot(c, c.shots[0], "f32", want_fields=True)
+    e = dataclasses.replace(c, type="elastic", visco=None)
+    el, fe = oracle_shot(e, e.shots[0], "f32", want_fields=True)
+    assert np.abs(el).max() > 0
+    assert np.array_equal(v, el) and np.array_equal(fv, fe)
+
+
+@pytest.mark.parametrize("fs", [0, 1])
+def test_visco_mechanisms_split_at_one_frequency(fs):
+    """L mechanisms at one relaxation frequency with tau / L each equal one mechanism with tau:
+    the memory-variable recursion is linear (each R_l is R / L), 1 + L (tau / L) = 1 + tau, and
+    the relaxed moduli and free-surface scales are unchanged -- up to rounding of the split sums."""
+    from _cases import oracle_shot
+
+    one = _visco_case([25.0], 0.2, free_surface=fs)
+    s1, _ = oracle_shot(one, one.shots[0], "f64")
+    for L in (2, 3):
+        c = _visco_case([25.0] * L, 0.2, free_surface=fs)
+        c.visco["tauP"] = one.visco["tauP"] / L
+        c.visco["tauS"] = one.visco["tauS"] / L
+        sL, _ = oracle_shot(c, c.shots[0], "f64")
+        err = np.linalg.norm(sL - s1) / np.linalg.norm(s1)
+        assert err < 1e-7, (L, err)  # FP64: 0 for L = 2 (halves are exact), 1.8e-9 for L = 3
+    # and a visibly different attenuation when the split is not by L
+    c = _visco_case([25.0, 25.0], 0.2, free_surface=fs)
+    sW, _ = oracle_shot(c, c.shots[0], "f64")
+    assert np.linalg.norm(sW - s1) / np.linalg.norm(s1) > 1e-3
+

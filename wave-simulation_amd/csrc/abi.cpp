@@ -171,6 +171,9 @@ int wv_solver::enqueue_steps(int nsteps, bool timed)
                 const wv::Rec2D *rec = adjoint_record(this);
                 wv::launch_strip2d(order, tile_y, L, cfg.nshots, F, C, T, P, A, j, j > 0, st, 0, e0, e1, nullptr, rec);
                 wv::launch_strip2d(order, tile_y, L, cfg.nshots, F, C, T, P, A, j, 0, st, 1, e2, e3, nullptr, rec);
+            } else if (visco) {
+                wv::launch_step2d(physics, order, tile_y, L, cfg.nshots, F, C, T, P, A, j, j > 0, st, 0, e0, e1);
+                wv::launch_visco2d_stress(order, tile_y, L, cfg.nshots, F, C, T, P, A, j, visco2d(), st, e2, e3);
             } else {
                 wv::launch_step2d(physics, order, tile_y, L, cfg.nshots, F, C, T, P, A, j, j > 0, st, 0, e0, e1);
                 wv::launch_step2d(physics, order, tile_y, L, cfg.nshots, F, C, T, P, A, j, 0, st, 1, e2, e3);
@@ -248,6 +251,9 @@ double wv_solver::kernel_bytes(int kind) const
         bytes += 8.0 * B * (xf * t[2] + yf * t[3]);
     }
     if (cfg.damping_boundary == 1 && (kind == 1 || variant == V_SINGLE)) bytes += 4.0 * cells;
+    // viscoelastic stress phase: the memory variables (3 L read and written per shot-cell) and
+    // tauP, tauS, tauS_xy, 1 + L tauP, 1 + L tauS per cell
+    if (visco && kind == 1) bytes += 4.0 * (cells * B * 6.0 * vL + cells * 5.0);
     return bytes;
 }
 
@@ -275,6 +281,8 @@ int wv_create(const char *dimension, const char *type, wv_solver **out)
         phys = wv::TMEM;
     else if (t == "sh")
         phys = wv::SH;
+    else if (t == "viscoelastic" && d == "2d")
+        phys = wv::ELASTIC;  // the elastic step with memory variables in the stress phase
     else
         return fail(WV_EINVAL, "equationType " + t + " is not provided by this solver");
     if (d == "3d" && phys != wv::ELASTIC && phys != wv::ACOUSTIC)
@@ -282,6 +290,7 @@ int wv_create(const char *dimension, const char *type, wv_solver **out)
     wv_solver *s = new wv_solver();
     s->dim = d == "3d" ? 3 : 2;
     s->physics = phys;
+    s->visco = t == "viscoelastic";
     *out = s;
     return WV_OK;
 }
@@ -313,15 +322,17 @@ int wv_init(wv_solver *s, const wv_config *c)
     if (k.nshots < 1) return fail(WV_EINVAL, "nshots must be >= 1");
     if (k.path != WV_PATH_FUSED && k.path != WV_PATH_PLAIN && k.path != WV_PATH_PHASED && k.path != WV_PATH_STRIP)
         return fail(WV_EINVAL, "bad path");
-    if (k.path == WV_PATH_STRIP && (s->dim != 2 || s->physics != wv::ELASTIC))
+    if (k.path == WV_PATH_STRIP && (s->dim != 2 || s->physics != wv::ELASTIC || s->visco))
         return fail(WV_EINVAL, "the row-strip kernels are 2D elastic only");
+    if (s->visco && k.path == WV_PATH_PLAIN)
+        return fail(WV_EINVAL, "the plain path is provided for acoustic, elastic and tmem");
     if (k.path == WV_PATH_STRIP && k.tile_y != 0 && k.tile_y != 8 && k.tile_y != 16 && k.tile_y != 32)
         return fail(WV_EINVAL, "tile_y of the row-strip kernels must be 0 (default), 8, 16 or 32");
     if (k.path == WV_PATH_PLAIN && k.nshots != 1) return fail(WV_EINVAL, "the plain path runs one shot per handle");
     if (s->dim == 3 && k.path == WV_PATH_PLAIN) return fail(WV_EINVAL, "the plain path is 2D only");
     if (s->physics == wv::SH && k.path == WV_PATH_PLAIN)
         return fail(WV_EINVAL, "the plain path is provided for acoustic, elastic and tmem");
-    const bool single = s->dim == 2 && k.path == WV_PATH_FUSED && s->physics == wv::ELASTIC;
+    const bool single = s->dim == 2 && k.path == WV_PATH_FUSED && s->physics == wv::ELASTIC && !s->visco;
     if (s->dim == 2 && single && k.tile_y != 0 && std::abs(k.tile_y) != 8 && std::abs(k.tile_y) != 16 && std::abs(k.tile_y) != 24)
         return fail(WV_EINVAL, "tile_y of the single-pass kernel must be 0 (default), +-8, +-16 or +-24");
     if (s->dim == 2 && !single && k.path != WV_PATH_STRIP && k.tile_y != 0 && k.tile_y != 8 && k.tile_y != 16 &&
@@ -498,9 +509,22 @@ int wv_set_model(wv_solver *s, const float *m0, const float *m1, const float *m2
         s->model_ok = true;
         return WV_OK;
     }
+    if (s->visco && s->vL == 0) return fail(WV_ESTATE, "the viscoelastic model needs wv_set_relaxation first");
     const wv::Layout2D &L = s->L;
     const int NX = L.NX, NY = L.NY;
     auto gi = [&](int x, int y) { return (size_t)x + (size_t)y * NX; };
+    // viscoelastic: the moduli given at the reference frequency CenterFrequencyCPML are scaled to
+    // the relaxed ones, M / (1 + sum tauP) and mu / (1 + sum tauS) with
+    // sum = sum_l w^2 ts_l^2 / (1 + w^2 ts_l^2), w = 2 pi fc, ts_l = 1 / (2 pi f_l)
+    // (Viscoelastic.cpp:650-700; Bohlen 2002, eq. 12)
+    double vsum = 0.0;
+    if (s->visco) {
+        const double w = 2.0 * M_PI * s->cfg.fc_cpml;
+        for (int l = 0; l < s->vL; l++) {
+            const double ts = 1.0 / (2.0 * M_PI * s->vfreq[l]);
+            vsum += w * w * ts * ts / (1.0 + w * w * ts * ts);
+        }
+    }
     if (s->physics == wv::ELASTIC) {
         // kept for the chain rule of the adjoint gradient
         s->model_h[0].assign(m0, m0 + (size_t)NX * NY);
@@ -529,9 +553,12 @@ int wv_set_model(wv_solver *s, const float *m0, const float *m1, const float *m2
                 const double vp = m0[i], r = m2[i];
                 rho[i] = r;
                 M[i] = r * vp * vp;
+                if (s->visco) M[i] /= 1.0 + vsum * s->vtauP[i];
                 if (s->physics == wv::ELASTIC) {
                     const double vs = m1[i];
-                    mu[i] = std::max(r * vs * vs, 1.0);  // ModelparameterSeismic.cpp:424
+                    double m = r * vs * vs;
+                    if (s->visco) m /= 1.0 + vsum * s->vtauS[i];
+                    mu[i] = std::max(m, 1.0);  // ModelparameterSeismic.cpp:424
                 }
             }
         for (int y = 0; y < NY; y++)
@@ -606,22 +633,124 @@ int wv_set_model(wv_solver *s, const float *m0, const float *m1, const float *m2
         }
         HIP_TRY(hipMemcpyAsync(s->coef[q], c[q].data(), c[q].size() * sizeof(float), hipMemcpyHostToDevice, s->st));
     }
+    if (s->visco) {
+        // per cell tauP, tauS, tauS averaged with the 4-point matrix of mu_xy (Viscoelastic.cpp:
+        // 562-575), 1 + L tauP, 1 + L tauS (ForwardSolver2Dviscoelastic.cpp:85-100)
+        std::vector<std::vector<float>> vc(5, std::vector<float>((size_t)L.plane, 0.f));
+        const std::vector<float> &tp = s->vtauP, &ts = s->vtauS;
+        for (int y = 0; y < NY; y++)
+            for (int x = 0; x < NX; x++) {
+                const size_t i = gi(x, y), o = li(x, y);
+                const bool inx = x + 1 < NX, iny = y + 1 < NY;
+                double a;
+                if (inx && iny)
+                    a = 0.25 * ((double)ts[i] + ts[gi(x + 1, y)] + ts[gi(x, y + 1)] + ts[gi(x + 1, y + 1)]);
+                else if (inx)
+                    a = 0.5 * ((double)ts[i] + ts[gi(x + 1, y)]);
+                else if (iny)
+                    a = 0.5 * ((double)ts[i] + ts[gi(x, y + 1)]);
+                else
+                    a = ts[i];
+                vc[0][o] = tp[i];
+                vc[1][o] = ts[i];
+                vc[2][o] = (float)a;
+                vc[3][o] = (float)(1.0 + (double)s->vL * tp[i]);
+                vc[4][o] = (float)(1.0 + (double)s->vL * ts[i]);
+            }
+        for (int q = 0; q < 5; q++) {
+            if (!s->vcoef[q] && (rc = s->dalloc(&s->vcoef[q], (size_t)L.plane, s->allocs))) return rc;
+            HIP_TRY(hipMemcpyAsync(s->vcoef[q], vc[q].data(), vc[q].size() * sizeof(float), hipMemcpyHostToDevice,
+                                   s->st));
+            HIP_TRY(hipStreamSynchronize(s->st));  // vc is freed at the end of this block
+        }
+    }
     if (s->physics == wv::ELASTIC && s->cfg.free_surface == 1) {
-        // FreeSurfaceElastic.cpp:11-47: scales of the horizontal/vertical updates at y = 0
+        // FreeSurfaceElastic.cpp:11-47: scales of the horizontal/vertical updates at y = 0;
+        // viscoelastic (FreeSurfaceViscoelastic.cpp:10-60): with the unrelaxed moduli M (1 + L tauP),
+        // mu (1 + L tauS), and per mechanism the updates of Rxx
         const int NXp = round_up(NX, 64) + 64;
-        std::vector<float> h((size_t)NXp, 0.f), v((size_t)NXp, 0.f);
+        std::vector<float> h((size_t)NXp, 0.f), v((size_t)NXp, 0.f), rh((size_t)4 * NXp, 0.f),
+            rv((size_t)4 * NXp, 0.f);
         for (int x = 0; x < NX; x++) {
-            const double Mx = M[gi(x, 0)], t = Mx - 2.0 * mu[gi(x, 0)];
+            const size_t i = gi(x, 0);
+            if (!s->visco) {
+                const double Mx = M[i], t = Mx - 2.0 * mu[i];
+                v[x] = (float)t;
+                h[x] = (float)(-(t * t) / Mx);
+                continue;
+            }
+            const double oP = 1.0 + (double)s->vL * s->vtauP[i], oS = 1.0 + (double)s->vL * s->vtauS[i];
+            const double Mu = M[i] * oP, t = Mu - 2.0 * mu[i] * oS;
             v[x] = (float)t;
-            h[x] = (float)(-(t * t) / Mx);
+            h[x] = (float)(-(t * t) / Mu);
+            const double t3 = 2.0 * mu[i] * s->vtauS[i] - M[i] * s->vtauP[i];
+            const double ratio = 2.0 * mu[i] * oS / Mu - 1.0;
+            for (int l = 0; l < s->vL; l++) {
+                const double tau = 1.0 / (2.0 * M_PI * s->vfreq[l]);
+                const double c2 = 1.0 / (1.0 + s->cfg.DT / (2.0 * tau));
+                rv[(size_t)l * NXp + x] = (float)(t3 * c2 / tau);
+                rh[(size_t)l * NXp + x] = (float)(t3 * ratio * c2 / tau);
+            }
         }
         if (!s->fsH && ((rc = s->dalloc(&s->fsH, NXp, s->allocs)) || (rc = s->dalloc(&s->fsV, NXp, s->allocs))))
             return rc;
         HIP_TRY(hipMemcpyAsync(s->fsH, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice, s->st));
         HIP_TRY(hipMemcpyAsync(s->fsV, v.data(), v.size() * sizeof(float), hipMemcpyHostToDevice, s->st));
+        if (s->visco)
+            for (int l = 0; l < s->vL; l++) {
+                if (!s->vfs[l] && ((rc = s->dalloc(&s->vfs[l], NXp, s->allocs)) ||
+                                   (rc = s->dalloc(&s->vfs[4 + l], NXp, s->allocs))))
+                    return rc;
+                HIP_TRY(hipMemcpyAsync(s->vfs[l], rh.data() + (size_t)l * NXp, NXp * sizeof(float),
+                                       hipMemcpyHostToDevice, s->st));
+                HIP_TRY(hipMemcpyAsync(s->vfs[4 + l], rv.data() + (size_t)l * NXp, NXp * sizeof(float),
+                                       hipMemcpyHostToDevice, s->st));
+            }
+        HIP_TRY(hipStreamSynchronize(s->st));  // h, v, rh, rv are freed at the end of this block
     }
     HIP_TRY(hipStreamSynchronize(s->st));
     s->model_ok = true;
+    return WV_OK;
+}
+
+int wv_set_relaxation(wv_solver *s, int nmech, const double *freqs, const float *tauP, const float *tauS)
+{
+    if (!s || !s->inited) return fail(WV_ESTATE, "wv_set_relaxation before wv_init");
+    if (!s->visco) return fail(WV_EINVAL, "relaxation mechanisms belong to the viscoelastic solver");
+    // Viscoelastic.cpp:585-596
+    if (nmech < 1 || nmech > 4)
+        return fail(WV_EINVAL, "The number of relaxation mechanisms should be >0 in an viscoelastic simulation (1..4)");
+    if (!freqs || !tauP || !tauS) return fail(WV_EINVAL, "missing relaxation argument");
+    for (int l = 0; l < nmech; l++)
+        if (!(freqs[l] > 0)) return fail(WV_EINVAL, "The relaxation frequency should be >=0 in an viscoelastic simulation");
+    const size_t n = (size_t)s->L.NX * s->L.NY;
+    s->vfreq.assign(freqs, freqs + nmech);
+    s->vtauP.assign(tauP, tauP + n);
+    s->vtauS.assign(tauS, tauS + n);
+    // per-mechanism constants in the reference's value type, step by step as
+    // ForwardSolver2Dviscoelastic.cpp:50-60 (they multiply the memory variables every step)
+    const float DTf = (float)s->cfg.DT;
+    for (int l = 0; l < 4; l++) s->vitau[l] = s->vc1[l] = s->vc2[l] = 0.f;
+    for (int l = 0; l < nmech; l++) {
+        const float tau = (float)(1.0 / (2.0 * M_PI * (double)(float)freqs[l]));
+        s->vitau[l] = (float)(1.0 / (double)tau);
+        s->vc1[l] = (float)(1.0 - (double)DTf / (2.0 * (double)tau));
+        s->vc2[l] = (float)(1.0 / (1.0 + (double)DTf / (2.0 * (double)tau)));
+    }
+    s->vdth = (float)((double)DTf / 2.0);
+    const size_t fn = (size_t)s->L.plane * s->cfg.nshots;
+    for (int l = 0; l < nmech; l++)
+        for (int g = 0; g < 3; g++) {
+            float *&p = s->vR[4 * g + l];
+            if (!p) {
+                int rc = s->dalloc(&p, fn, s->allocs);
+                if (rc) return rc;
+                HIP_TRY(hipMemsetAsync(p, 0, fn * sizeof(float), s->st));
+            }
+        }
+    s->vL = nmech;
+    s->model_ok = false;  // the moduli depend on tauP, tauS: wv_set_model follows
+    HIP_TRY(hipStreamSynchronize(s->st));
     return WV_OK;
 }
 
@@ -685,6 +814,8 @@ int wv_reset(wv_solver *s)
     for (int q = 0; q < 4; q++)
         if (s->pss[q]) HIP_TRY(hipMemsetAsync(s->pss[q], 0, (q < 2 ? s->psx_n : s->psy_n) * 4, s->st));
     if (s->d_seis) HIP_TRY(hipMemsetAsync(s->d_seis, 0, (size_t)s->cfg.NT * std::max(s->rec_total, 1) * 4, s->st));
+    for (int k = 0; k < 12; k++)
+        if (s->vR[k]) HIP_TRY(hipMemsetAsync(s->vR[k], 0, fn * sizeof(float), s->st));
     return WV_OK;
 }
 
@@ -868,7 +999,7 @@ int wv_tune(wv_solver *s, int nsteps, int *choice)
     const int cand_var[9] = {V_PHASED, V_PHASED, V_PHASED, V_SINGLE, V_SINGLE, V_SINGLE, V_STRIP, V_STRIP, V_STRIP};
     const int cand_ty[9] = {8, 16, 32, 8, 16, 24, 8, 16, 32};
     const int cand_id[9] = {0, 1, 2, 3, 4, 5, 9, 10, 11};
-    const bool all2d = s->physics == wv::ELASTIC && s->cfg.path == WV_PATH_FUSED;
+    const bool all2d = s->physics == wv::ELASTIC && s->cfg.path == WV_PATH_FUSED && !s->visco;
     const int ncand = all2d ? (wv::strip2d_fits(s->L) ? 9 : 6) : 3;
     float best = 1e30f;
     const int k0 = s->cfg.path == WV_PATH_STRIP ? 6 : 0, k1 = s->cfg.path == WV_PATH_STRIP ? 9 : ncand;
@@ -959,6 +1090,7 @@ int wv_kernel_stats(wv_solver *s, int kind, const char **name, double *avg_ms, l
         *name = kind >= 3 ? adj_names[kind - 3]
                           : ((s->variant == V_SINGLE && kind < 2)   ? (kind == 0 ? "el2d_fused" : "-")
                              : (s->variant == V_STRIP && kind < 2) ? (kind == 0 ? "el2d_velocity_strip" : "el2d_stress_strip")
+                             : (s->visco && kind == 1)              ? "ve2d_stress"
                                                                    : names[s->physics][kind]);
     if (launches) *launches = s->klaunch[kind];
     if (avg_ms) *avg_ms = s->klaunch[kind] ? s->kms[kind] / s->klaunch[kind] : 0.0;

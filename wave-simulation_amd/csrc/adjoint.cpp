@@ -230,7 +230,7 @@ int wv_set_checkpointing(wv_solver *s, int every, int fp16)
         const int rc = s->e3->set_checkpointing(every, fp16, s->adj ? s->adj->store_gib : -1.0);
         return rc ? fail(rc, s->e3->error) : WV_OK;
     }
-    if (s->dim != 2 || s->physics != wv::ELASTIC || s->plain)
+    if (s->dim != 2 || s->physics != wv::ELASTIC || s->plain || s->visco)
         return fail(WV_EINVAL, "the adjoint is provided for the tiled 2D elastic solver");
     if (s->cfg.free_surface || s->cfg.damping_boundary == 1)
         return fail(WV_EINVAL, "the adjoint supports CPML or no absorbing boundary, without free surface");

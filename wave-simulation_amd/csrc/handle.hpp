@@ -67,6 +67,15 @@ struct wv_solver {
     float *coef[5] = {};
     float *damp = nullptr, *fsH = nullptr, *fsV = nullptr;
     float *tap[6] = {};  // position-0 pointers: xf xb yf yb yf_v yb_v
+    // viscoelastic (2D): relaxation mechanisms (wv_set_relaxation) and their device data
+    bool visco = false;
+    int vL = 0;
+    std::vector<double> vfreq;
+    std::vector<float> vtauP, vtauS;  // grid order
+    float *vR[12] = {};               // memory variables Rxx[4], Ryy[4], Rxy[4] (shot planes)
+    float *vcoef[5] = {};             // tauP, tauS, tauS_xy, 1 + L tauP, 1 + L tauS (padded layout)
+    float *vfs[8] = {};               // free-surface relaxation updates: horizontal[4], vertical[4]
+    float vitau[4] = {}, vc1[4] = {}, vc2[4] = {}, vdth = 0.f;
     float *cpx[4] = {}, *cpy[4] = {};
     float *psv[2][4] = {};  // ping-pong: velocity-derivative memory variables psx0 psx1 psy0 psy1
     float *pss[4] = {};     // stress-derivative memory variables psx2 psx3 psy2 psy3
@@ -180,6 +189,26 @@ struct wv_solver {
         T.yf_v = tap[4];
         T.yb_v = tap[5];
         return T;
+    }
+    wv::Visco2D visco2d() const
+    {
+        wv::Visco2D V{};
+        V.L = vL;
+        for (int k = 0; k < 12; k++) V.R[k] = vR[k];
+        V.tauP = vcoef[0];
+        V.tauS = vcoef[1];
+        V.tauSxy = vcoef[2];
+        V.opP = vcoef[3];
+        V.opS = vcoef[4];
+        for (int l = 0; l < 4; l++) {
+            V.fsRH[l] = vfs[l];
+            V.fsRV[l] = vfs[4 + l];
+            V.itau[l] = vitau[l];
+            V.c1[l] = vc1[l];
+            V.c2[l] = vc2[l];
+        }
+        V.dth = vdth;
+        return V;
     }
     wv::Cpml2D cpml2d(int b) const
     {

@@ -26,7 +26,7 @@ EXPORTS = (
     "wv_synchronize", "wv_get_seismograms", "wv_get_wavefield", "wv_set_wavefield", "wv_last_run_ms",
     "wv_tune", "wv_set_kernel_timing", "wv_kernel_stats", "wv_last_error", "wv_set_checkpointing",
     "wv_set_adjoint_source", "wv_run_adjoint", "wv_get_gradient", "wv_set_adjoint_store",
-    "wv_check_parameters", "wv_check_finite", "wv_check_finite_noalloc", "wv_adjoint_plan", "wv_source_signal",
+    "wv_check_parameters", "wv_check_finite", "wv_check_finite_noalloc", "wv_adjoint_plan", "wv_source_signal", "wv_set_relaxation",
 )
 
 
@@ -80,6 +80,7 @@ def load():
         L.wv_set_adjoint_store.argtypes = [C.c_void_p, C.c_double]
         L.wv_check_finite_noalloc.argtypes = [C.c_void_p, llp]
         L.wv_adjoint_plan.argtypes = [C.c_void_p, ip, ip, ip]
+        L.wv_set_relaxation.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), fp, fp]
         L.wv_source_signal.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, fp]
         L.wv_run_adjoint.argtypes = [C.c_void_p]
         L.wv_get_gradient.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double)]

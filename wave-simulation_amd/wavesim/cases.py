@@ -48,6 +48,7 @@ class Case:
     m1: np.ndarray | None
     m2: np.ndarray
     shots: list = field(default_factory=list)
+    visco: dict | None = None  # viscoelastic: freqs (per mechanism), tauP, tauS (per cell)
 
     @property
     def cells(self):
@@ -188,6 +189,34 @@ def elastic_c4(N=600, NT=3000, DT=1e-4, nshots=16, seed=4, W=30, anomaly=True):
     return c
 
 
+def visco_2d(NX=170, NY=97, NT=500, DT=1e-4, nshots=3, order=8, W=14, damping=2, free_surface=0, freqs=(30.0,),
+             tau=0.15, seed=11):
+    """2D viscoelastic P-SV (ForwardSolver2Dviscoelastic.cpp): the C2-style layered model with
+    Gaussian perturbations, tauP / tauS varying by layer around `tau`, VX / VY / P Ricker sources at
+    seeded positions, VX, VY and P receivers on a row and a column (uneven counts per shot)."""
+    s = dict(NX=NX, NY=NY, NT=NT, DH=1.0, DT=DT, spatial_fd_order=order, damping_boundary=damping,
+             boundary_width=W, npower=4.0, fc_cpml=30.0, vmax_cpml=4000.0, free_surface=free_surface,
+             damping_coeff=8.0)
+    vp, vs, rho = elastic_model(NX, NY, seed, seed + 100, sigma=8.0)
+    rows = np.repeat((np.arange(NY) * 4 // NY)[:, None], NX, axis=1).ravel()
+    tp = (tau * (1.0 + 0.25 * rows)).astype(np.float32)
+    ts = (tau * (1.2 - 0.15 * rows)).astype(np.float32)
+    c = Case("visco_2d", "2D", "viscoelastic", s, vp, vs, rho,
+             visco=dict(freqs=list(freqs), tauP=tp, tauS=ts))
+    rng = np.random.Generator(np.random.PCG64(seed))
+    w = ricker(NT, DT, 60.0, 1.0)
+    lo = W + 4 if damping else 4
+    for k in range(nshots):
+        sx, sy = int(rng.integers(lo, NX - lo)), int(rng.integers(lo if not free_surface else 1, NY - lo))
+        nr = int(rng.integers(10, 40))
+        rx = np.sort(rng.choice(np.arange(lo, NX - lo), nr, replace=False))
+        ry = 0 if free_surface else lo
+        rec = np.concatenate([rx + ry * NX, (NX // 2) + np.arange(lo, NY - lo, 7) * NX])
+        rty = np.array([1 + (j % 3) for j in range(len(rec))], np.int32)
+        c.shots.append(Shot(np.array([sx + sy * NX]), np.array([1 + k % 3]), w[None, :], rec, rty))
+    return c
+
+
 def elastic_c5(N=320, NT=2500, DT=2e-4, nshots=8, seed=5, W=24, nrec_side=40, spacing=4, src_depth=26):
     NX = NY = NZ = N
     s = dict(NX=NX, NY=NY, NZ=NZ, NT=NT, DH=2.0, DT=DT, spatial_fd_order=8, damping_boundary=2, boundary_width=W,
@@ -261,6 +290,8 @@ def make_solver(case, nshots=None, path="fused", ops_mode="sparse", steps_per_gr
     sol = create(case.dimension, case.type)
     sol.init_forward_solver(nshots=n, path=path, ops_mode=ops_mode, steps_per_graph=steps_per_graph, tile_y=tile_y,
                             **s)
+    if case.visco is not None:
+        sol.set_relaxation(case.visco["freqs"], case.visco["tauP"], case.visco["tauS"])
     sol.prepare_for_modelling(case.m0, case.m1, case.m2)
     for k, sh in enumerate(shots[:n]):
         sol.set_acquisition(k, sh.src_index, sh.src_type, sh.src_signal, sh.rec_index, sh.rec_type)

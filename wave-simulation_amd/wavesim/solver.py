@@ -26,6 +26,7 @@ class ForwardSolver:
     FIELDS = {
         ("2D", "acoustic"): ("vx", "vy", "p"),
         ("2D", "elastic"): ("vx", "vy", "sxx", "syy", "sxy"),
+        ("2D", "viscoelastic"): ("vx", "vy", "sxx", "syy", "sxy"),
         ("2D", "tmem"): ("hx", "hy", "ez"),
         ("2D", "sh"): ("vz", "sxz", "syz"),
         ("3D", "elastic"): ("vx", "vy", "vz", "sxx", "syy", "szz", "syz", "sxz", "sxy"),
@@ -79,6 +80,17 @@ class ForwardSolver:
                 raise ValueError("model vector size mismatch")
         ptr = [(_fp(a) if a is not None else None) for a in arrs]
         _abi.check(_abi.load().wv_set_model(self._h, *ptr))
+        return self
+
+    def set_relaxation(self, freqs, tauP, tauS):
+        """Viscoelastic: relaxation frequencies [Hz] (1..4 mechanisms) and tauP, tauS per cell (or
+        scalars), before prepare_for_modelling (Modelparameter::Viscoelastic, Viscoelastic.cpp:160-260)."""
+        n = self.cfg.NX * self.cfg.NY * self.cfg.NZ
+        f = np.ascontiguousarray(np.atleast_1d(freqs), dtype=np.float64)
+        tp = np.ascontiguousarray(np.broadcast_to(np.asarray(tauP, np.float32), (n,)))
+        ts = np.ascontiguousarray(np.broadcast_to(np.asarray(tauS, np.float32), (n,)))
+        _abi.check(_abi.load().wv_set_relaxation(self._h, len(f), f.ctypes.data_as(C.POINTER(C.c_double)), _fp(tp),
+                                                 _fp(ts)))
         return self
 
     def set_acquisition(self, shot, src_index, src_type, src_signal, rec_index, rec_type):
