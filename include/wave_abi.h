@@ -79,7 +79,7 @@ typedef struct {
 } wv_config;
 
 /* dimension "2D"/"3D", type (case-insensitive): 2D "acoustic", "elastic", "viscoelastic", "sh",
- * "tmem"; 3D "acoustic", "elastic".  The factory's other types ("viscosh", "emem", "viscotmem",
+ * "viscosh", "tmem"; 3D "acoustic", "elastic".  The factory's other types ("emem", "viscotmem",
  * "viscoemem", 3D others) return WV_EINVAL ("not provided"), unknown ones WV_EINVAL. */
 int wv_create(const char *dimension, const char *type, wv_solver **out);
 int wv_destroy(wv_solver *s);
@@ -94,7 +94,8 @@ int wv_init(wv_solver *s, const wv_config *cfg);
  *   tmem    : m0 = epsilon_r, m1 = sigma [S/m], m2 = mu_r                  */
 int wv_set_model(wv_solver *s, const float *m0, const float *m1, const float *m2);
 
-/* Relaxation mechanisms of the 2D viscoelastic solver ("viscoelastic" in wv_create): nmech
+/* Relaxation mechanisms of the 2D viscoelastic solvers ("viscoelastic", "viscosh"; tauP unused
+ * and may be NULL for visco-SH, ViscoSH.cpp:450-575): nmech
  * (1..4) relaxation frequencies [Hz] and tauP, tauS per cell [NX*NY] (grid order), as
  * Modelparameter::Viscoelastic holds them (numRelaxationMechanisms, relaxationFrequency..4,
  * the .tauP / .tauS model files; Viscoelastic.cpp:160-260, 585-596) and

@@ -21,7 +21,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 
-PHYSICS = {"acoustic": 0, "elastic": 1, "tmem": 2, "sh": 3, "viscoelastic": 4, "visco": 4}
+PHYSICS = {"acoustic": 0, "elastic": 1, "tmem": 2, "sh": 3, "viscoelastic": 4, "visco": 4, "viscosh": 5}
 
 
 class Params(C.Structure):
@@ -102,9 +102,9 @@ def run(params: Params, m0, m1, m2, src_idx, src_type, src_sig, rec_idx, rec_typ
     rec_idx = np.ascontiguousarray(rec_idx, dtype=np.int64)
     rec_type = np.ascontiguousarray(rec_type, dtype=np.int32)
     seis = np.zeros((len(rec_idx), params.NT), dtype=dt)
-    nf = {0: 3 if params.dim == 2 else 4, 1: 5 if params.dim == 2 else 9, 2: 3, 3: 3, 4: 5}[params.physics]
+    nf = {0: 3 if params.dim == 2 else 4, 1: 5 if params.dim == 2 else 9, 2: 3, 3: 3, 4: 5, 5: 3}[params.physics]
     fields = np.zeros((nf, n), dtype=dt) if want_fields else None
-    if params.physics == 4:
+    if params.physics in (4, 5):
         freqs = np.ascontiguousarray(visco["freqs"], dtype=np.float64)
         tp = np.ascontiguousarray(np.broadcast_to(visco["tauP"], (n,)), dtype=dt)
         ts = np.ascontiguousarray(np.broadcast_to(visco["tauS"], (n,)), dtype=dt)
@@ -396,7 +396,7 @@ def ci_case(cfg_path):
 
     ref_name = os.path.basename(cfg["seismogramfilename"]).split(".ci")[0] + ".ref"
     visco = None
-    if physics == 4:
+    if physics in (4, 5):
         # Viscoelastic.cpp:160-200: numRelaxationMechanisms, relaxationFrequency(2..4); tauP/tauS
         # from the model files (TwoLayer.cpp:25-26, 83-86: 0.1 in both layers) or the configuration
         nl = cfg_num(cfg, "numRelaxationMechanisms", int)

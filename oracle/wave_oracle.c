@@ -622,9 +622,9 @@ typedef struct {
     double damping_coeff, npower, fc_cpml, vmax_cpml;
 } oracle_params;
 
-enum { PH_ACOUSTIC = 0, PH_ELASTIC = 1, PH_TMEM = 2, PH_SH = 3, PH_VISCO = 4 };
+enum { PH_ACOUSTIC = 0, PH_ELASTIC = 1, PH_TMEM = 2, PH_SH = 3, PH_VISCO = 4, PH_VISCOSH = 5 };
 
-/* Viscoelastic parameters of the next oracle_run (2D viscoelastic only): L relaxation
+/* Viscoelastic parameters of the next oracle_run (2D viscoelastic and visco-SH): L relaxation
  * mechanisms with their relaxation frequencies [Hz], tauP and tauS per cell (grid order).
  * Held by pointer: the caller keeps the arrays alive during the run. */
 static int g_visco_L = 0;
@@ -686,7 +686,7 @@ static void ops_build(const grid_t *g, const oracle_params *p, ops_t *o)
         o->have_fs = 1;
         o->DyfFS = build_dyf_fs(g, p->order, DH);
         csr_scale(&o->DyfFS, DT);
-        if (p->physics == PH_ELASTIC || p->physics == PH_SH || p->physics == PH_VISCO) {
+        if (p->physics == PH_ELASTIC || p->physics == PH_SH || p->physics == PH_VISCO || p->physics == PH_VISCOSH) {
             o->DybFS = build_dyb_fs(g, p->order, DH);
             csr_scale(&o->DybFS, DT);
         }
@@ -789,8 +789,10 @@ int oracle_run(const oracle_params *p, const real *m0, const real *m1, const rea
 #ifdef _OPENMP
     if (p->nthreads > 0) omp_set_num_threads(p->nthreads);
 #endif
-    if (p->dim == 3 && (p->physics == PH_TMEM || p->physics == PH_SH || p->physics == PH_VISCO)) return -2;
-    if (p->physics == PH_VISCO && (g_visco_L < 1 || !g_tauP || !g_tauS)) return -3;
+    if (p->dim == 3 && (p->physics == PH_TMEM || p->physics == PH_SH || p->physics == PH_VISCO ||
+                        p->physics == PH_VISCOSH))
+        return -2;
+    if ((p->physics == PH_VISCO || p->physics == PH_VISCOSH) && (g_visco_L < 1 || !g_tauP || !g_tauS)) return -3;
     grid_t g = {p->NX, p->NY, p->NZ, (long)p->NX * p->NY * p->NZ};
     long n = g.n;
     int NT = p->NT;
@@ -805,7 +807,8 @@ int oracle_run(const oracle_params *p, const real *m0, const real *m1, const rea
     real *M = NULL, *mu = NULL, *irx = NULL, *iry = NULL, *irz = NULL, *muxy = NULL, *muxz = NULL, *muyz = NULL;
     real *Ca = NULL, *Cb = NULL, *imxz = NULL, *imyz = NULL;
     /* viscoelastic: relaxed moduli, (1 + L tau), averaged tauS, per-mechanism constants */
-    int VL = p->physics == PH_VISCO ? g_visco_L : 0;
+    int VL = (p->physics == PH_VISCO || p->physics == PH_VISCOSH) ? g_visco_L : 0;
+    real *tauSxz = NULL, *tauSyz = NULL;
     real *opP = NULL, *opS = NULL, *tauSxy = NULL, *Rv[12] = {0};
     real vtau[4] = {0}, vitau[4] = {0}, vc1[4] = {0}, vc2[4] = {0}, dth = 0;
     if (p->physics == PH_ACOUSTIC || p->physics == PH_ELASTIC || p->physics == PH_VISCO) {
@@ -878,13 +881,52 @@ int oracle_run(const oracle_params *p, const real *m0, const real *m1, const rea
                 shear_average(&g, mu, 1, 2, muyz);
             }
         }
-    } else if (p->physics == PH_SH) {
+    } else if (p->physics == PH_SH || p->physics == PH_VISCOSH) {
         /* SH: m1 = vs, m2 = rho; 1/rho at the node (ModelparameterSeismic.cpp:164-172) */
         irx = vnew(n);
         mu = vnew(n);
         for (long i = 0; i < n; i++) {
             irx[i] = (real)1 / m2[i];
             mu[i] = m2[i] * m1[i] * m1[i];
+        }
+        if (p->physics == PH_VISCOSH) {
+            /* ViscoSH.cpp:555-575: the relaxed shear modulus; 2-point averages of tauS along x and
+             * y (ViscoSH.cpp:450-460, the matrices of mu_xz / mu_yz); ForwardSolver2Dviscosh.cpp
+             * constants as the viscoelastic ones */
+            real w_ref = (real)(2.0 * M_PI * (double)(real)p->fc_cpml);
+            real sum = 0;
+            for (int l = 0; l < VL; l++) {
+                real ts = (real)(1.0 / (2.0 * M_PI * (double)(real)g_visco_f[l]));
+                real num = w_ref * w_ref * ts * ts;
+                sum = (real)((double)sum + (double)num / (1.0 + (double)(w_ref * w_ref * ts * ts)));
+            }
+            for (long i = 0; i < n; i++) {
+                real tsv = sum * g_tauS[i];
+                tsv = (real)1.0 + tsv;
+                mu[i] = mu[i] / tsv;
+            }
+            tauSxz = vnew(n);
+            tauSyz = vnew(n);
+            for (long i = 0; i < n; i++) {
+                long x, y, z;
+                gcoord(&g, i, &x, &y, &z);
+                real h = (real)0.5;
+                tauSxz[i] = x + 1 < g.NX ? h * g_tauS[i] + h * g_tauS[gidx(&g, x + 1, y, z)] : g_tauS[i];
+                tauSyz[i] = y + 1 < g.NY ? h * g_tauS[i] + h * g_tauS[gidx(&g, x, y + 1, z)] : g_tauS[i];
+            }
+            opS = vnew(n);
+            for (long i = 0; i < n; i++) {
+                real b = (real)VL * g_tauS[i];
+                opS[i] = (real)1.0 + b;
+            }
+            for (int l = 0; l < VL; l++) {
+                vtau[l] = (real)(1.0 / (2.0 * M_PI * (double)(real)g_visco_f[l]));
+                vitau[l] = (real)(1.0 / (double)vtau[l]);
+                vc1[l] = (real)(1.0 - (double)DT / (2.0 * (double)vtau[l]));
+                vc2[l] = (real)(1.0 / (1.0 + (double)DT / (2.0 * (double)vtau[l])));
+            }
+            dth = (real)((double)DT / 2.0);
+            for (int k = 0; k < 2 * VL; k++) Rv[k] = vnew(n); /* Rxz[l], Ryz[l] */
         }
         muxz = vnew(n);
         muyz = vnew(n);
@@ -1324,6 +1366,57 @@ int oracle_run(const oracle_params *p, const real *m0, const real *m1, const rea
                 real *all[9] = {sxx, syy, szz, sxy, sxz, syz, vx, vy, vz};
                 for (int k = 0; k < 9; k++) v_mul(n, all[k], damp);
             }
+        } else if (p->physics == PH_VISCOSH) {
+            /* ForwardSolver2Dviscosh.cpp:110-190 ; F: vz sxz syz; the SH velocity update, then
+             * the stresses with the memory variables Rxz, Ryz of each mechanism */
+            real *vz = F[0], *sxz = F[1], *syz = F[2];
+            real **Rxz = Rv, **Ryz = Rv + VL;
+            spmv(&o.Dxb, sxz, upd);
+            spmv(fs ? &o.DybFS : &o.Dyb, syz, tmp);
+            if (use_cpml) {
+                cpml_apply(n, upd, psi[0], ax, bx); /* sxz_x */
+                cpml_apply(n, tmp, psi[1], ay, by); /* syz_y */
+            }
+            v_add(n, upd, tmp);
+            v_mul(n, upd, irx);
+            v_add(n, vz, upd);
+            spmv(&o.Dxf, vz, upd);
+            if (use_cpml) cpml_apply(n, upd, psi[2], axh, bxh); /* vzx */
+            v_mul(n, upd, muxz);
+            for (int l = 0; l < VL; l++)
+                VLOOP(n)
+                {
+                    sxz[i] += dth * Rxz[l][i];
+                    Rxz[l][i] *= vc1[l];
+                    real u2 = vitau[l] * upd[i];
+                    u2 *= tauSxz[i];
+                    Rxz[l][i] -= u2;
+                    Rxz[l][i] *= vc2[l];
+                    sxz[i] += dth * Rxz[l][i];
+                }
+            v_mul(n, upd, opS);
+            v_add(n, sxz, upd);
+            spmv(&o.Dyf, vz, upd);
+            if (use_cpml) cpml_apply(n, upd, psi[3], ayh, byh); /* vzy */
+            v_mul(n, upd, muyz);
+            for (int l = 0; l < VL; l++)
+                VLOOP(n)
+                {
+                    syz[i] += dth * Ryz[l][i];
+                    Ryz[l][i] *= vc1[l];
+                    real u2 = vitau[l] * upd[i];
+                    u2 *= tauSyz[i];
+                    Ryz[l][i] -= u2;
+                    Ryz[l][i] *= vc2[l];
+                    syz[i] += dth * Ryz[l][i];
+                }
+            v_mul(n, upd, opS);
+            v_add(n, syz, upd);
+            if (damp) {
+                v_mul(n, sxz, damp);
+                v_mul(n, syz, damp);
+                v_mul(n, vz, damp);
+            }
         } else if (p->physics == PH_SH) {
             /* ForwardSolver2Dsh.cpp:101-190 ; F: vz sxz syz */
             real *vz = F[0], *sxz = F[1], *syz = F[2];
@@ -1404,7 +1497,7 @@ int oracle_run(const oracle_params *p, const real *m0, const real *m1, const rea
                     if (ty == 2) F[0][i] += v;
                     if (ty == 3) F[1][i] += v;
                     if (ty == 4) F[2][i] += v;
-                } else if (p->physics == PH_SH) {
+                } else if (p->physics == PH_SH || p->physics == PH_VISCOSH) {
                     if (ty == 4) F[0][i] += v; /* only VZ sources in SH (FDTD2Dsh.hpp:25-27) */
                 } else {
                     if (ty == 1) F[2][i] += v;
@@ -1432,7 +1525,7 @@ int oracle_run(const oracle_params *p, const real *m0, const real *m1, const rea
                     v /= (real)3;
                 } else
                     v = ty == 2 ? F[0][i] : (ty == 3 ? F[1][i] : F[2][i]);
-            } else if (p->physics == PH_SH) {
+            } else if (p->physics == PH_SH || p->physics == PH_VISCOSH) {
                 v = F[0][i];
             } else {
                 v = F[2][i];
@@ -1482,6 +1575,8 @@ int oracle_run(const oracle_params *p, const real *m0, const real *m1, const rea
     free(opP);
     free(opS);
     free(tauSxy);
+    free(tauSxz);
+    free(tauSyz);
     ops_free(&o);
     return 0;
 }

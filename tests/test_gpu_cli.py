@@ -32,6 +32,9 @@ def _write_vec(path, v):
                "(Derivatives.cpp:26-30, 166-179); they miss this fixture (misfit 2.8e-4 > 5e-7), which matches "
                "zero-padded operators only (test_oracle_golden.py, per-operator hypotheses)")),
     ("3D.elastic", "vy", []),
+    # the configuration's equationType=visco is the older name of viscoelastic; zero-padded operators
+    # as for the elastic fixture (test_oracle_golden.py)
+    ("2D.visco", "vy", ["useStencilMatrix=1", "equationType=viscoelastic"]),
 ])
 def test_cli_reproduces_golden_seismogram(tmp_path, name, comp, extra):
     cfg_path = os.path.join(CI, f"configuration_ci.{name}.txt")
@@ -46,6 +49,9 @@ def test_cli_reproduces_golden_seismogram(tmp_path, name, comp, extra):
         vp, vs, rho = po.two_layer_model(NX, NY, NZ)
         for ext, v in (("vp", vp), ("vs", vs), ("density", rho)):
             _write_vec(f"{model}.{ext}.mtx", v)
+        if "visco" in name:  # TwoLayer.cpp:25-26, 83-86: tauP = tauS = 0.1 in both layers
+            for ext in ("tauP", "tauS"):
+                _write_vec(f"{model}.{ext}.mtx", np.full(len(vp), 0.1))
         model_args = [f"ModelFilename={model}", "fileFormat=1"]
     src = os.path.join(CI, os.path.basename(cfg["sourcefilename"].split()[0]))
     rec = os.path.join(CI, os.path.basename(cfg["receiverfilename"].split()[0]))

@@ -1,4 +1,5 @@
-"""2D viscoelastic P-SV (ForwardSolver2Dviscoelastic.cpp:120-300) on the GPU, through the C ABI.
+"""2D viscoelastic P-SV (ForwardSolver2Dviscoelastic.cpp:120-300) and visco-SH
+(ForwardSolver2Dviscosh.cpp:110-190) on the GPU, through the C ABI.
 
 * The reference's CI fixture par/ci/seismogram.2D.visco.ref.vy.mtx from its own configuration
   (order 8, one mechanism at 5 Hz, free surface, ABS, the two-layer model with tauP = tauS = 0.1)
@@ -127,3 +128,49 @@ def test_visco_rejections():
     st = wavesim.create("2D", "viscoelastic")
     with pytest.raises(wavesim.WaveError, match="row-strip"):
         st.init_forward_solver(nshots=1, path="strip", **c.settings)
+
+
+# ---- 2D visco-SH (ForwardSolver2Dviscosh.cpp:110-190): the sh_2d acquisition and model with
+# tauS by layer and one to three mechanisms, against the oracle (no reference fixture exists)
+
+
+def _check_sh(case, **kw):
+    sol = cases.make_solver(case, **kw)
+    sol.run(0, case.settings["NT"])
+    sol.check_finite()
+    out = []
+    for k, sh in enumerate(case.shots):
+        o32, f32 = oracle_shot(case, sh, "f32", want_fields=True)
+        o64, _ = oracle_shot(case, sh, "f64")
+        g = sol.seismograms(k)
+        e32, e64 = rel_l2(g, o32), rel_l2(g, o64)
+        print(f"{case.name} order {case.settings['spatial_fd_order']} mechanisms {len(case.visco['freqs'])} {kw} "
+              f"shot {k}: relL2 vs f32 oracle {e32:.2e}, vs f64 {e64:.2e}", flush=True)
+        assert np.abs(o32).max() > 0
+        assert e32 <= TOL, (k, e32)
+        assert e64 <= 1e-4, (k, e64)
+        if k == len(case.shots) - 1:
+            for f, name in enumerate(("vz", "sxz", "syz")):
+                assert rel_l2(sol.wavefield(name, shot=k), f32[f]) <= TOL, name
+        out.append(g)
+    return out
+
+
+@pytest.mark.parametrize("order,freqs", [(2, (30.0,)), (8, (12.0, 45.0)), (12, (8.0, 30.0, 90.0))])
+def test_viscosh_cpml_orders_and_mechanisms(order, freqs):
+    _check_sh(cases.viscosh_2d(order=order, freqs=freqs))
+
+
+@pytest.mark.parametrize("tile_y", [8, 16, 32])
+def test_viscosh_abs_free_surface_tiles(tile_y):
+    _check_sh(cases.viscosh_2d(damping=1, free_surface=1, W=12, freqs=(20.0, 60.0)), tile_y=tile_y)
+
+
+def test_viscosh_batched_equals_single():
+    c = cases.viscosh_2d(NT=400, freqs=(25.0, 70.0))
+    batched = cases.make_solver(c, steps_per_graph=50)
+    batched.run(0, 400)
+    for k, sh in enumerate(c.shots):
+        one = cases.make_solver(c, shots=[sh])
+        one.run(0, 400)
+        assert np.array_equal(one.seismograms(0), batched.seismograms(k))

@@ -212,3 +212,35 @@ def test_visco_mechanisms_split_at_one_frequency(fs):
     sW, _ = oracle_shot(c, c.shots[0], "f64")
     assert np.linalg.norm(sW - s1) / np.linalg.norm(s1) > 1e-3
 
+
+def test_viscosh_with_zero_tau_is_the_sh_loop():
+    """visco-SH with tauS = 0 is the SH loop to the bit (no memory-variable contribution, unit
+    (1 + L tauS), unscaled modulus); with tauS > 0 it attenuates (smaller late amplitudes)."""
+    from wavesim import cases
+    from _cases import oracle_shot
+
+    c = cases.viscosh_2d(NX=90, NY=70, NT=400, nshots=1, W=10, freqs=(20.0, 50.0))
+    c.visco["tauS"] = np.zeros_like(c.visco["tauS"])
+    v, fv = oracle_shot(c, c.shots[0], "f32", want_fields=True)
+    e = dataclasses.replace(c, type="sh", visco=None)
+    sh, fe = oracle_shot(e, e.shots[0], "f32", want_fields=True)
+    assert np.abs(sh).max() > 0
+    assert np.array_equal(v, sh) and np.array_equal(fv, fe)
+    c2 = cases.viscosh_2d(NX=90, NY=70, NT=400, nshots=1, W=10, freqs=(20.0, 50.0), tau=0.3)
+    a, _ = oracle_shot(c2, c2.shots[0], "f64")
+    assert np.abs(a[:, 250:]).max() < np.abs(sh[:, 250:]).max()
+
+
+def test_viscosh_mechanisms_split_at_one_frequency():
+    """As for P-SV: L mechanisms at one frequency with tauS / L each equal one mechanism."""
+    from wavesim import cases
+    from _cases import oracle_shot
+
+    one = cases.viscosh_2d(NX=90, NY=70, NT=300, nshots=1, W=10, freqs=(25.0,), tau=0.2)
+    s1, _ = oracle_shot(one, one.shots[0], "f64")
+    for L in (2, 3):
+        c = cases.viscosh_2d(NX=90, NY=70, NT=300, nshots=1, W=10, freqs=(25.0,) * L, tau=0.2)
+        c.visco["tauS"] = one.visco["tauS"] / L
+        sL, _ = oracle_shot(c, c.shots[0], "f64")
+        assert np.linalg.norm(sL - s1) / np.linalg.norm(s1) < 1e-7, L
+

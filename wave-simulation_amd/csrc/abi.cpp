@@ -173,7 +173,8 @@ int wv_solver::enqueue_steps(int nsteps, bool timed)
                 wv::launch_strip2d(order, tile_y, L, cfg.nshots, F, C, T, P, A, j, 0, st, 1, e2, e3, nullptr, rec);
             } else if (visco) {
                 wv::launch_step2d(physics, order, tile_y, L, cfg.nshots, F, C, T, P, A, j, j > 0, st, 0, e0, e1);
-                wv::launch_visco2d_stress(order, tile_y, L, cfg.nshots, F, C, T, P, A, j, visco2d(), st, e2, e3);
+                wv::launch_visco2d_stress(physics, order, tile_y, L, cfg.nshots, F, C, T, P, A, j, visco2d(), st, e2,
+                                          e3);
             } else {
                 wv::launch_step2d(physics, order, tile_y, L, cfg.nshots, F, C, T, P, A, j, j > 0, st, 0, e0, e1);
                 wv::launch_step2d(physics, order, tile_y, L, cfg.nshots, F, C, T, P, A, j, 0, st, 1, e2, e3);
@@ -253,7 +254,8 @@ double wv_solver::kernel_bytes(int kind) const
     if (cfg.damping_boundary == 1 && (kind == 1 || variant == V_SINGLE)) bytes += 4.0 * cells;
     // viscoelastic stress phase: the memory variables (3 L read and written per shot-cell) and
     // tauP, tauS, tauS_xy, 1 + L tauP, 1 + L tauS per cell
-    if (visco && kind == 1) bytes += 4.0 * (cells * B * 6.0 * vL + cells * 5.0);
+    if (visco && kind == 1)
+        bytes += physics == wv::SH ? 4.0 * (cells * B * 4.0 * vL + cells * 3.0) : 4.0 * (cells * B * 6.0 * vL + cells * 5.0);
     return bytes;
 }
 
@@ -283,6 +285,8 @@ int wv_create(const char *dimension, const char *type, wv_solver **out)
         phys = wv::SH;
     else if (t == "viscoelastic" && d == "2d")
         phys = wv::ELASTIC;  // the elastic step with memory variables in the stress phase
+    else if (t == "viscosh" && d == "2d")
+        phys = wv::SH;  // the SH step with memory variables in the stress phase
     else
         return fail(WV_EINVAL, "equationType " + t + " is not provided by this solver");
     if (d == "3d" && phys != wv::ELASTIC && phys != wv::ACOUSTIC)
@@ -290,7 +294,7 @@ int wv_create(const char *dimension, const char *type, wv_solver **out)
     wv_solver *s = new wv_solver();
     s->dim = d == "3d" ? 3 : 2;
     s->physics = phys;
-    s->visco = t == "viscoelastic";
+    s->visco = t == "viscoelastic" || t == "viscosh";
     *out = s;
     return WV_OK;
 }
@@ -598,7 +602,9 @@ int wv_set_model(wv_solver *s, const float *m0, const float *m1, const float *m2
             for (int x = 0; x < NX; x++) {
                 const size_t i = gi(x, y);
                 c[0][li(x, y)] = 1.f / m2[i];
-                mu[i] = std::max((double)m2[i] * m1[i] * m1[i], 1.0);
+                double m = (double)m2[i] * m1[i] * m1[i];
+                if (s->visco) m /= 1.0 + vsum * s->vtauS[i];  // ViscoSH.cpp:555-575
+                mu[i] = std::max(m, 1.0);
             }
         for (int ax = 0; ax < 2; ax++)
             for (int y = 0; y < NY; y++)
@@ -633,7 +639,27 @@ int wv_set_model(wv_solver *s, const float *m0, const float *m1, const float *m2
         }
         HIP_TRY(hipMemcpyAsync(s->coef[q], c[q].data(), c[q].size() * sizeof(float), hipMemcpyHostToDevice, s->st));
     }
-    if (s->visco) {
+    if (s->visco && s->physics == wv::SH) {
+        // visco-SH: tauS averaged along x and y with the 2-point matrices of mu_xz / mu_yz
+        // (ViscoSH.cpp:450-460), 1 + L tauS (ForwardSolver2Dviscosh.cpp:70-90)
+        std::vector<std::vector<float>> vc(3, std::vector<float>((size_t)L.plane, 0.f));
+        const std::vector<float> &ts = s->vtauS;
+        for (int y = 0; y < NY; y++)
+            for (int x = 0; x < NX; x++) {
+                const size_t i = gi(x, y), o = li(x, y);
+                vc[0][o] = x + 1 < NX ? (float)(0.5 * ((double)ts[i] + ts[gi(x + 1, y)])) : ts[i];
+                vc[1][o] = y + 1 < NY ? (float)(0.5 * ((double)ts[i] + ts[gi(x, y + 1)])) : ts[i];
+                vc[2][o] = (float)(1.0 + (double)s->vL * ts[i]);
+            }
+        const int slot[3] = {5, 6, 4};
+        for (int q = 0; q < 3; q++) {
+            float *&d = s->vcoef[slot[q]];
+            if (!d && (rc = s->dalloc(&d, (size_t)L.plane, s->allocs))) return rc;
+            HIP_TRY(hipMemcpyAsync(d, vc[q].data(), vc[q].size() * sizeof(float), hipMemcpyHostToDevice, s->st));
+            HIP_TRY(hipStreamSynchronize(s->st));  // vc is freed at the end of this block
+        }
+    }
+    if (s->visco && s->physics == wv::ELASTIC) {
         // per cell tauP, tauS, tauS averaged with the 4-point matrix of mu_xy (Viscoelastic.cpp:
         // 562-575), 1 + L tauP, 1 + L tauS (ForwardSolver2Dviscoelastic.cpp:85-100)
         std::vector<std::vector<float>> vc(5, std::vector<float>((size_t)L.plane, 0.f));
@@ -720,12 +746,15 @@ int wv_set_relaxation(wv_solver *s, int nmech, const double *freqs, const float 
     // Viscoelastic.cpp:585-596
     if (nmech < 1 || nmech > 4)
         return fail(WV_EINVAL, "The number of relaxation mechanisms should be >0 in an viscoelastic simulation (1..4)");
-    if (!freqs || !tauP || !tauS) return fail(WV_EINVAL, "missing relaxation argument");
+    if (!freqs || !tauS || (!tauP && s->physics != wv::SH)) return fail(WV_EINVAL, "missing relaxation argument");
     for (int l = 0; l < nmech; l++)
         if (!(freqs[l] > 0)) return fail(WV_EINVAL, "The relaxation frequency should be >=0 in an viscoelastic simulation");
     const size_t n = (size_t)s->L.NX * s->L.NY;
     s->vfreq.assign(freqs, freqs + nmech);
-    s->vtauP.assign(tauP, tauP + n);
+    if (tauP)
+        s->vtauP.assign(tauP, tauP + n);
+    else
+        s->vtauP.assign(n, 0.f);  // visco-SH has no tauP
     s->vtauS.assign(tauS, tauS + n);
     // per-mechanism constants in the reference's value type, step by step as
     // ForwardSolver2Dviscoelastic.cpp:50-60 (they multiply the memory variables every step)
@@ -739,8 +768,9 @@ int wv_set_relaxation(wv_solver *s, int nmech, const double *freqs, const float 
     }
     s->vdth = (float)((double)DTf / 2.0);
     const size_t fn = (size_t)s->L.plane * s->cfg.nshots;
+    const int ngroups = s->physics == wv::SH ? 2 : 3;  // Rxz, Ryz | Rxx, Ryy, Rxy
     for (int l = 0; l < nmech; l++)
-        for (int g = 0; g < 3; g++) {
+        for (int g = 0; g < ngroups; g++) {
             float *&p = s->vR[4 * g + l];
             if (!p) {
                 int rc = s->dalloc(&p, fn, s->allocs);
@@ -1090,7 +1120,7 @@ int wv_kernel_stats(wv_solver *s, int kind, const char **name, double *avg_ms, l
         *name = kind >= 3 ? adj_names[kind - 3]
                           : ((s->variant == V_SINGLE && kind < 2)   ? (kind == 0 ? "el2d_fused" : "-")
                              : (s->variant == V_STRIP && kind < 2) ? (kind == 0 ? "el2d_velocity_strip" : "el2d_stress_strip")
-                             : (s->visco && kind == 1)              ? "ve2d_stress"
+                             : (s->visco && kind == 1)              ? (s->physics == wv::SH ? "vsh2d_s" : "ve2d_stress")
                                                                    : names[s->physics][kind]);
     if (launches) *launches = s->klaunch[kind];
     if (avg_ms) *avg_ms = s->klaunch[kind] ? s->kms[kind] / s->klaunch[kind] : 0.0;

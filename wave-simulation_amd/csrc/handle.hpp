@@ -73,7 +73,7 @@ struct wv_solver {
     std::vector<double> vfreq;
     std::vector<float> vtauP, vtauS;  // grid order
     float *vR[12] = {};               // memory variables Rxx[4], Ryy[4], Rxy[4] (shot planes)
-    float *vcoef[5] = {};             // tauP, tauS, tauS_xy, 1 + L tauP, 1 + L tauS (padded layout)
+    float *vcoef[7] = {};             // tauP, tauS, tauS_xy, 1 + L tauP, 1 + L tauS, tauS_xz, tauS_yz
     float *vfs[8] = {};               // free-surface relaxation updates: horizontal[4], vertical[4]
     float vitau[4] = {}, vc1[4] = {}, vc2[4] = {}, vdth = 0.f;
     float *cpx[4] = {}, *cpy[4] = {};
@@ -200,6 +200,8 @@ struct wv_solver {
         V.tauSxy = vcoef[2];
         V.opP = vcoef[3];
         V.opS = vcoef[4];
+        V.tauSxz = vcoef[5];
+        V.tauSyz = vcoef[6];
         for (int l = 0; l < 4; l++) {
             V.fsRH[l] = vfs[l];
             V.fsRV[l] = vfs[4 + l];

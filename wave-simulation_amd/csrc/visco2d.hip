@@ -1,4 +1,5 @@
-// 2D viscoelastic P-SV stress update for gfx950 (ForwardSolver2Dviscoelastic.cpp:190-300).
+// 2D viscoelastic P-SV and visco-SH stress updates for gfx950 (ForwardSolver2Dviscoelastic.cpp:
+// 190-300, ForwardSolver2Dviscosh.cpp:130-185).
 //
 // The velocity update of the viscoelastic solver is the elastic one
 // (ForwardSolver2Dviscoelastic.cpp:140-190), so a viscoelastic step is the elastic velocity
@@ -204,6 +205,104 @@ __global__ __launch_bounds__(NTHREADS) void ve2d_stress(Layout2D L, Fields2D F, 
     block_acquisition<ELASTIC>(A, F, so, shot_block(A, B), A.tbase[0] + tofs, false);
 }
 
+// visco-SH stress update (ForwardSolver2Dviscosh.cpp:130-185): sh2d_s (kernels2d.hip) with the
+// memory variables Rxz, Ryz of each mechanism, in the reference's order:
+//   u = mu_xz CPML(Dxf vz);  for l: sxz += dt/2 Rxz_l; Rxz_l = c2 (c1 Rxz_l - u tauS_xz / tau_l);
+//       sxz += dt/2 Rxz_l;  sxz += (1 + L tauS) u;  the same for syz with Dyf, mu_yz, tauS_yz;
+// then the ABS damping of the stresses.
+template <int H, int TY>
+__global__ __launch_bounds__(NTHREADS) void vsh2d_s(Layout2D L, Fields2D F, Coef2D C, Taps T, Cpml2D P, Acq A,
+                                                    int tofs, Visco2D V)
+{
+    using TL = Tile<H, TY>;
+    constexpr int R = TY / BLOCK_Y;
+    __shared__ float s_v[TL::RH][TL::RW];
+    const Blk B = xcd_block();
+    const int shot = B.shot;
+    const long long so = (long long)shot * L.plane;
+    const int x0 = B.bx * TILE_X, y0 = B.by * TY;
+    const int tid = threadIdx.y * TILE_X + threadIdx.x, tx = threadIdx.x;
+    TileRegs<H, TY> rv;
+    rv.load(F.f[0] + so, L, x0, y0, tid);
+    float *__restrict__ sxz = F.f[1] + so;
+    float *__restrict__ syz = F.f[2] + so;
+    const Col col = column(L, P, x0 + tx);
+    float cxf[2 * H + 1];
+    load_taps<H>(cxf, T.xf, col.x);
+    const int W2 = 2 * P.W;
+    const int wy = __builtin_amdgcn_readfirstlane(threadIdx.y);
+    const int NL = V.L;
+    float p1[R], q1[R];
+    int yi[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int y = y0 + wy + r * BLOCK_Y;
+        p1[r] = q1[r] = 0.f;
+        yi[r] = P.on ? frame_index(y, L.NY, P.W, C.free_surface) : -1;
+        if (col.xi >= 0 && y < L.NY) p1[r] = P.psx[1][((long long)shot * L.NY + y) * W2 + col.xi];
+        if (yi[r] >= 0 && col.in) q1[r] = P.psy[1][((long long)shot * W2 + yi[r]) * L.NX + col.x];
+    }
+    rv.store(s_v, tid);
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int ly = wy + r * BLOCK_Y;
+        const int y = y0 + ly;
+        if (y >= L.NY) break;
+        const float *cyf = T.yf + (long long)y * (2 * H + 1);
+        const int Rr = ly + H;
+        float u = dx<H>(s_v, Rr, tx, cxf);
+        float t = dy<H>(s_v, Rr, tx, cyf);
+        if (!col.in) continue;
+        if (col.xi >= 0) P.psx[1][((long long)shot * L.NY + y) * W2 + col.xi] = cpml(u, p1[r], col.axh, col.bxh);
+        if (yi[r] >= 0) P.psy[1][((long long)shot * W2 + yi[r]) * L.NX + col.x] = cpml(t, q1[r], P.ayh[y], P.byh[y]);
+        const long long c = L.at(col.x, y), o = so + c;
+        const float oS = V.opS[c], txz = V.tauSxz[c], tyz = V.tauSyz[c];
+        const float dth = V.dth;
+        float xz = F.f[1][o], yz = F.f[2][o];
+        u *= C.c[1][c];
+#pragma unroll
+        for (int l = 0; l < 4; l++)
+            if (l < NL) {
+                float rr = V.R[l][o];
+                xz += dth * rr;
+                rr *= V.c1[l];
+                float g = V.itau[l] * u;
+                g *= txz;
+                rr -= g;
+                rr *= V.c2[l];
+                xz += dth * rr;
+                V.R[l][o] = rr;
+            }
+        u *= oS;
+        xz += u;
+        t *= C.c[2][c];
+#pragma unroll
+        for (int l = 0; l < 4; l++)
+            if (l < NL) {
+                float rr = V.R[4 + l][o];
+                yz += dth * rr;
+                rr *= V.c1[l];
+                float g = V.itau[l] * t;
+                g *= tyz;
+                rr -= g;
+                rr *= V.c2[l];
+                yz += dth * rr;
+                V.R[4 + l][o] = rr;
+            }
+        t *= oS;
+        yz += t;
+        if (C.damp) {
+            const float d = C.damp[c];
+            xz *= d;
+            yz *= d;
+        }
+        sxz[c] = xz;
+        syz[c] = yz;
+    }
+    block_acquisition<SH>(A, F, so, shot_block(A, B), A.tbase[0] + tofs, false);
+}
+
 template <typename Kern, typename... Args>
 void go(Kern k, dim3 grid, dim3 block, hipStream_t st, hipEvent_t e0, hipEvent_t e1, Args... args)
 {
@@ -214,11 +313,18 @@ void go(Kern k, dim3 grid, dim3 block, hipStream_t st, hipEvent_t e0, hipEvent_t
 }
 
 template <int H>
-void launch_ty(int ty, const Layout2D &L, int nshots, const Fields2D &F, const Coef2D &C, const Taps &T,
+void launch_ty(int physics, int ty, const Layout2D &L, int nshots, const Fields2D &F, const Coef2D &C, const Taps &T,
                const Cpml2D &P, const Acq &A, int tofs, const Visco2D &V, hipStream_t st, hipEvent_t e0, hipEvent_t e1)
 {
     const dim3 grid((L.NX + TILE_X - 1) / TILE_X, (L.NY + ty - 1) / ty, nshots), block(TILE_X, BLOCK_Y);
-    if (ty == 8)
+    if (physics == SH) {
+        if (ty == 8)
+            go(vsh2d_s<H, 8>, grid, block, st, e0, e1, L, F, C, T, P, A, tofs, V);
+        else if (ty == 32)
+            go(vsh2d_s<H, 32>, grid, block, st, e0, e1, L, F, C, T, P, A, tofs, V);
+        else
+            go(vsh2d_s<H, 16>, grid, block, st, e0, e1, L, F, C, T, P, A, tofs, V);
+    } else if (ty == 8)
         go(ve2d_stress<H, 8>, grid, block, st, e0, e1, L, F, C, T, P, A, tofs, V);
     else if (ty == 32)
         go(ve2d_stress<H, 32>, grid, block, st, e0, e1, L, F, C, T, P, A, tofs, V);
@@ -228,17 +334,17 @@ void launch_ty(int ty, const Layout2D &L, int nshots, const Fields2D &F, const C
 
 }  // namespace
 
-void launch_visco2d_stress(int order, int tile_y, const Layout2D &L, int nshots, const Fields2D &F, const Coef2D &C,
-                           const Taps &T, const Cpml2D &P, const Acq &A, int tofs, const Visco2D &V,
+void launch_visco2d_stress(int physics, int order, int tile_y, const Layout2D &L, int nshots, const Fields2D &F,
+                           const Coef2D &C, const Taps &T, const Cpml2D &P, const Acq &A, int tofs, const Visco2D &V,
                            hipStream_t stream, hipEvent_t e0, hipEvent_t e1)
 {
     switch (order / 2) {
-    case 1: launch_ty<1>(tile_y, L, nshots, F, C, T, P, A, tofs, V, stream, e0, e1); break;
-    case 2: launch_ty<2>(tile_y, L, nshots, F, C, T, P, A, tofs, V, stream, e0, e1); break;
-    case 3: launch_ty<3>(tile_y, L, nshots, F, C, T, P, A, tofs, V, stream, e0, e1); break;
-    case 4: launch_ty<4>(tile_y, L, nshots, F, C, T, P, A, tofs, V, stream, e0, e1); break;
-    case 5: launch_ty<5>(tile_y, L, nshots, F, C, T, P, A, tofs, V, stream, e0, e1); break;
-    case 6: launch_ty<6>(tile_y, L, nshots, F, C, T, P, A, tofs, V, stream, e0, e1); break;
+    case 1: launch_ty<1>(physics, tile_y, L, nshots, F, C, T, P, A, tofs, V, stream, e0, e1); break;
+    case 2: launch_ty<2>(physics, tile_y, L, nshots, F, C, T, P, A, tofs, V, stream, e0, e1); break;
+    case 3: launch_ty<3>(physics, tile_y, L, nshots, F, C, T, P, A, tofs, V, stream, e0, e1); break;
+    case 4: launch_ty<4>(physics, tile_y, L, nshots, F, C, T, P, A, tofs, V, stream, e0, e1); break;
+    case 5: launch_ty<5>(physics, tile_y, L, nshots, F, C, T, P, A, tofs, V, stream, e0, e1); break;
+    case 6: launch_ty<6>(physics, tile_y, L, nshots, F, C, T, P, A, tofs, V, stream, e0, e1); break;
     default: break;
     }
 }

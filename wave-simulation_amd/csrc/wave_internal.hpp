@@ -146,15 +146,16 @@ struct Acq {
     const int *tbase;           // device step counter base
 };
 
-// 2D viscoelastic stress update (visco2d.hip): L relaxation mechanisms (1..4), their memory
-// variables Rxx[l] = R[l], Ryy[l] = R[4 + l], Rxy[l] = R[8 + l] (shot planes in the field
-// layout), per cell tauP, tauS, tauS averaged like mu_xy, 1 + L tauP, 1 + L tauS (padded
-// layout, one plane), per mechanism 1/tau_l, c1 = 1 - DT/(2 tau_l), c2 = 1/(1 + DT/(2 tau_l)),
-// DT/2, and the relaxation updates of the free surface at y = 0 ([NXpad], or null).
+// 2D viscoelastic and visco-SH stress updates (visco2d.hip): L relaxation mechanisms (1..4),
+// their memory variables (shot planes in the field layout: P-SV Rxx[l] = R[l], Ryy[l] = R[4 + l],
+// Rxy[l] = R[8 + l]; SH Rxz[l] = R[l], Ryz[l] = R[4 + l]), per cell tauP, tauS, tauS averaged
+// like mu_xy (P-SV) or like mu_xz / mu_yz (SH), 1 + L tauP, 1 + L tauS (padded layout, one
+// plane), per mechanism 1/tau_l, c1 = 1 - DT/(2 tau_l), c2 = 1/(1 + DT/(2 tau_l)), DT/2, and
+// the relaxation updates of the P-SV free surface at y = 0 ([NXpad], or null).
 struct Visco2D {
     int L;
     float *R[12];
-    const float *tauP, *tauS, *tauSxy, *opP, *opS;
+    const float *tauP, *tauS, *tauSxy, *opP, *opS, *tauSxz, *tauSyz;
     const float *fsRH[4], *fsRV[4];
     float itau[4], c1[4], c2[4], dth;
 };
@@ -213,10 +214,10 @@ int fused2d_tile_x(int order);
 void launch_fused2d(int order, int tile_y, const Layout2D &L, int nshots, const Fields2D &F, const Fields2D &Fn,
                     const Coef2D &C, const Taps &T, const Cpml2D &P, const PsiNew2D &PN, const Acq &A, int tofs,
                     int pend, hipStream_t stream, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
-// Viscoelastic stress phase (visco2d.hip); the velocity phase is launch_step2d(ELASTIC, which = 0).
-// tile_y in {8, 16, 32}.
-void launch_visco2d_stress(int order, int tile_y, const Layout2D &L, int nshots, const Fields2D &F, const Coef2D &C,
-                           const Taps &T, const Cpml2D &P, const Acq &A, int tofs, const Visco2D &V,
+// Viscoelastic stress phase (visco2d.hip); the velocity phase is launch_step2d(ELASTIC or SH,
+// which = 0).  physics ELASTIC: P-SV, SH: visco-SH.  tile_y in {8, 16, 32}.
+void launch_visco2d_stress(int physics, int order, int tile_y, const Layout2D &L, int nshots, const Fields2D &F,
+                           const Coef2D &C, const Taps &T, const Cpml2D &P, const Acq &A, int tofs, const Visco2D &V,
                            hipStream_t stream, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 void launch_advance(int *tbase, int k, hipStream_t stream);
 void launch_set(int *p, int v, hipStream_t stream);

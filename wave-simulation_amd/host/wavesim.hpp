@@ -73,6 +73,9 @@ class ForwardSolver {
     void initForwardSolver(const Configuration &config, int nshots, int stepsPerGraph = 50, int tileY = 0);
     // prepareForModelling: model vectors in grid order (x fastest, then z, then depth y)
     void prepareForModelling(const std::vector<float> &m0, const std::vector<float> &m1, const std::vector<float> &m2);
+    // viscoelastic: relaxation frequencies (1..4 mechanisms) and tauP, tauS per cell, before
+    // prepareForModelling (Modelparameter::Viscoelastic, Viscoelastic.cpp:160-260)
+    void setRelaxation(const std::vector<double> &freqs, const std::vector<float> &tauP, const std::vector<float> &tauS);
     void setAcquisition(int shot, const std::vector<long long> &srcIndex, const std::vector<int> &srcType,
                         const std::vector<float> &srcSignal, const std::vector<long long> &recIndex,
                         const std::vector<int> &recType);

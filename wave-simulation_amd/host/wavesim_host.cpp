@@ -287,6 +287,12 @@ void ForwardSolver::prepareForModelling(const std::vector<float> &m0, const std:
     check(wv_set_model(h_, m0.data(), m1.empty() ? nullptr : m1.data(), m2.data()));
 }
 
+void ForwardSolver::setRelaxation(const std::vector<double> &freqs, const std::vector<float> &tauP,
+                                  const std::vector<float> &tauS)
+{
+    check(wv_set_relaxation(h_, (int)freqs.size(), freqs.data(), tauP.data(), tauS.data()));
+}
+
 void ForwardSolver::setAcquisition(int shot, const std::vector<long long> &srcIndex, const std::vector<int> &srcType,
                                    const std::vector<float> &srcSignal, const std::vector<long long> &recIndex,
                                    const std::vector<int> &recType)

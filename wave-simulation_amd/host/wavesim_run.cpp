@@ -5,7 +5,8 @@
 //
 // Reads the reference configuration (later KEY=VALUE arguments override it),
 // the model (ModelRead=1: <ModelFilename>.{vp,vs,density}.mtx, TM-EM
-// {epsilonr,sigma,mur}; ModelRead=0: the homogeneous values of the
+// {epsilonr,sigma,mur}, viscoelastic also .tauP / .tauS with the relaxation
+// mechanisms of the configuration; ModelRead=0: the homogeneous values of the
 // configuration), the source file (<SourceFilename>.txt: synthetic wavelets of
 // shapes 1..9, or wavelet types 2 / 3, rows of <SourceSignalFilename>.mtx) and
 // receiver files (<ReceiverFilename>.txt, or .shot_<n>.txt with
@@ -163,6 +164,27 @@ int main(int argc, char **argv)
                 m[q]->assign(n, (float)config.getDouble(em ? keys_em[q] : keys[q]));
             }
         }
+        // viscoelastic: relaxation mechanisms from the configuration, tauP / tauS from the
+        // model files or the configuration (Viscoelastic.cpp:160-200, 260-290)
+        std::string tl = type;
+        for (auto &c : tl) c = (char)std::tolower((unsigned char)c);
+        const bool visco = tl == "viscoelastic";
+        std::vector<double> freqs;
+        std::vector<float> tauP, tauS;
+        if (visco) {
+            const long nm = config.getInt("numRelaxationMechanisms");
+            const char *fk[4] = {"relaxationFrequency", "relaxationFrequency2", "relaxationFrequency3",
+                                 "relaxationFrequency4"};
+            for (long l = 0; l < nm; l++) freqs.push_back(l < 4 ? config.getDouble(fk[l]) : 0.0);
+            if (config.getInt("ModelRead", 0) == 1) {
+                const std::string base = config.getString("ModelFilename");
+                tauP = readVectorMtx(base + ".tauP.mtx", n);
+                tauS = readVectorMtx(base + ".tauS.mtx", n);
+            } else {
+                tauP.assign(n, (float)config.getDouble("tauP"));
+                tauS.assign(n, (float)config.getDouble("tauS"));
+            }
+        }
 
         // acquisition: shots in order of first appearance in the source file
         const std::vector<SourceRow> src = readSources(config.getString("SourceFilename") + ".txt");
@@ -176,6 +198,7 @@ int main(int argc, char **argv)
                                   (int)config.getInt("tileY", 0));
         const int NT = solver->NT();
         const double DT = config.getDouble("DT");
+        if (visco) solver->setRelaxation(freqs, tauP, tauS);
         solver->prepareForModelling(m0, m1, m2);
         auto lin = [&](int x, int y, int z) { return (long long)x + (long long)z * NX + (long long)y * NX * NZ; };
         std::vector<std::vector<ReceiverRow>> recs(nshots);

@@ -217,6 +217,19 @@ def visco_2d(NX=170, NY=97, NT=500, DT=1e-4, nshots=3, order=8, W=14, damping=2,
     return c
 
 
+def viscosh_2d(NX=180, NY=120, NT=600, DT=1e-4, nshots=3, order=8, W=16, damping=2, free_surface=0, freqs=(30.0,),
+               tau=0.15, seed=7):
+    """2D visco-SH (ForwardSolver2Dviscosh.cpp): the sh_2d acquisition and model with tauS varying
+    by layer around `tau` and L relaxation mechanisms at `freqs`."""
+    c = sh_2d(NX=NX, NY=NY, NT=NT, DT=DT, nshots=nshots, order=order, W=W, damping=damping,
+              free_surface=free_surface, seed=seed)
+    rows = np.repeat((np.arange(NY) * 4 // NY)[:, None], NX, axis=1).ravel()
+    ts = (tau * (1.2 - 0.15 * rows)).astype(np.float32)
+    c.name, c.type = "viscosh_2d", "viscosh"
+    c.visco = dict(freqs=list(freqs), tauP=np.zeros_like(ts), tauS=ts)
+    return c
+
+
 def elastic_c5(N=320, NT=2500, DT=2e-4, nshots=8, seed=5, W=24, nrec_side=40, spacing=4, src_depth=26):
     NX = NY = NZ = N
     s = dict(NX=NX, NY=NY, NZ=NZ, NT=NT, DH=2.0, DT=DT, spatial_fd_order=8, damping_boundary=2, boundary_width=W,

@@ -29,6 +29,7 @@ class ForwardSolver:
         ("2D", "viscoelastic"): ("vx", "vy", "sxx", "syy", "sxy"),
         ("2D", "tmem"): ("hx", "hy", "ez"),
         ("2D", "sh"): ("vz", "sxz", "syz"),
+        ("2D", "viscosh"): ("vz", "sxz", "syz"),
         ("3D", "elastic"): ("vx", "vy", "vz", "sxx", "syy", "szz", "syz", "sxz", "sxy"),
         ("3D", "acoustic"): ("vx", "vy", "vz", "p"),
     }
