@@ -11,7 +11,8 @@ batched shots with P/VX/VY/VZ receivers, compared with:
   * FP64 central differences of the oracle's misfit on seeded cells.
 Each memory plan is exercised: checkpoints on the device with one recomputation
 per segment, sub-segments (a budget that fits 10 recorded steps of a 40-step
-segment), and checkpoints in pinned host memory.
+segment), checkpoints in pinned host memory, and device-sized checkpoints moved to the
+host because they leave no room for a sub-segment.
 """
 import numpy as np
 import pytest
@@ -153,11 +154,17 @@ def test_adjoint3d_memory_plans_agree():
     budget = (sz["ck"] + 10 * sz["rec"] + sz["rec"] // 2) / gib
     host, _, plan = _gpu_gradient(c, obs, 10, False, budget)
     assert plan == (10, 1, True), plan
+    # checkpoints that fit half the budget but leave no room for any sub-segment next to
+    # them (6 FP32 states against >= 8.6 states of record + sub-segment states) go to the host
+    nck = -(-NT // 20)
+    budget = 2 * nck * sz["ck"] / gib
+    moved, _, plan = _gpu_gradient(c, obs, 20, False, budget)
+    assert plan[2] and plan[0] < 20 and plan[1] > 1, plan
     for name in ("vp", "vs", "rho"):
-        e1, e2 = rel_l2(sub[name], base[name]), rel_l2(host[name], base[name])
-        print(f"d/d{name}: sub-segments vs one recomputation {e1:.1e}, host checkpoints (every 10) {e2:.1e}",
-              flush=True)
-        assert e1 <= 1e-6 and e2 <= 1e-6, (name, e1, e2)
+        e1, e2, e3 = (rel_l2(g[name], base[name]) for g in (sub, host, moved))
+        print(f"d/d{name}: sub-segments vs one recomputation {e1:.1e}, host checkpoints (every 10) {e2:.1e}, "
+              f"moved to the host (every 20, plan {plan}) {e3:.1e}", flush=True)
+        assert e1 <= 1e-6 and e2 <= 1e-6 and e3 <= 1e-6, (name, e1, e2, e3)
 
 
 def test_adjoint3d_against_central_differences():

@@ -6,8 +6,9 @@
 // Memory plan (three levels, sized at set_checkpointing from a device-memory budget):
 //   1. the forward run saves the state (9 fields in FP16 with one scale per field and
 //      shot, or FP32, plus the CPML memory variables in FP32) every `every` steps --
-//      in device memory when all checkpoints fit in half of the budget, otherwise in
-//      pinned host memory, copied out through a device staging buffer on a second
+//      in device memory when all checkpoints fit in half of the budget and leave room
+//      for a sub-segment, otherwise in pinned host memory (refused beyond 80% of the
+//      host memory available), copied out through a device staging buffer on a second
 //      stream so the time loop does not wait for the transfer;
 //   2. the adjoint pass walks the segments last to first; a segment of `every` steps is
 //      recomputed from its checkpoint in sub-segments of K2 steps, saving the FP32
@@ -21,6 +22,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -37,6 +39,41 @@ constexpr int ORG = 64;  // margin of position-indexed tables (as the forward ta
 int round_up(int v, int m) { return (v + m - 1) / m * m; }
 // axis of each CPML memory variable (Cpml3D order)
 constexpr int AXIS_OF[18] = {0, 1, 2, 0, 1, 2, 0, 1, 2, 0, 1, 2, 1, 0, 2, 0, 2, 1};
+
+// Host memory this process may still take: MemAvailable of /proc/meminfo, lowered to
+// what the cgroup's limit leaves (v2 memory.max - memory.current, or v1
+// memory.limit_in_bytes - memory.usage_in_bytes).  < 0 when nothing is readable.
+double host_bytes_available()
+{
+    double avail = -1;
+    if (FILE *f = std::fopen("/proc/meminfo", "r")) {
+        char key[64];
+        unsigned long long kb;
+        while (std::fscanf(f, "%63s %llu kB", key, &kb) == 2)
+            if (!std::strcmp(key, "MemAvailable:")) {
+                avail = (double)kb * 1024.0;
+                break;
+            }
+        std::fclose(f);
+    }
+    auto read_num = [](const char *path, double &v) {
+        FILE *f = std::fopen(path, "r");
+        if (!f) return false;
+        unsigned long long x;
+        const bool ok = std::fscanf(f, "%llu", &x) == 1;  // "max" (no limit) does not parse
+        std::fclose(f);
+        if (ok) v = (double)x;
+        return ok;
+    };
+    double lim, cur;
+    if ((read_num("/sys/fs/cgroup/memory.max", lim) && read_num("/sys/fs/cgroup/memory.current", cur)) ||
+        (read_num("/sys/fs/cgroup/memory/memory.limit_in_bytes", lim) &&
+         read_num("/sys/fs/cgroup/memory/memory.usage_in_bytes", cur))) {
+        const double left = std::max(0.0, lim - cur);
+        avail = avail < 0 ? left : std::min(avail, left);
+    }
+    return avail;
+}
 }  // namespace
 
 #define HIPA(expr)                                                                                           \
@@ -283,19 +320,32 @@ int Elastic3D::set_checkpointing(int every, int fp16, double budget_gib)
     a->ck_fld_bytes = a->fld_n * 9 * (a->fp16 ? 2 : 4);
     const double ck_bytes = (double)a->ck_fld_bytes + (double)a->psi_n * 4;
     const double all_ck = ck_bytes * a->nck;
-    a->on_host = all_ck > 0.5 * budget;
-    double left = budget - (a->on_host ? ck_bytes : all_ck);  // host checkpoints need one device stage
     const double rec_step = 9.0 * a->fld_n * 4, state = (double)a->state_n * 4;
-    int K2 = 0;
-    for (int k = every; k >= 1; k--) {
-        const int ns = (every + k - 1) / k;
-        const double need = k * rec_step + (ns > 1 ? ns * state : 0.0);
-        if (need <= left) {
-            K2 = k;
-            break;
+    // the largest sub-segment whose record and sub-segment states fit next to the checkpoints
+    auto largest_k2 = [&](bool host) {
+        const double left = budget - (host ? ck_bytes : all_ck);  // host checkpoints need one device stage
+        for (int k = every; k >= 1; k--) {
+            const int ns = (every + k - 1) / k;
+            if (k * rec_step + (ns > 1 ? ns * state : 0.0) <= left) return k;
         }
+        return 0;
+    };
+    a->on_host = all_ck > 0.5 * budget;
+    int K2 = largest_k2(a->on_host);
+    if (K2 == 0 && !a->on_host) {  // device-resident checkpoints crowd out the record: move them out
+        a->on_host = true;
+        K2 = largest_k2(true);
     }
     if (K2 == 0) return fail(WV_ENOMEM, "the adjoint's recomputation buffers do not fit the device-memory budget");
+    if (a->on_host) {
+        // pinned host memory cannot be swapped: refuse a plan the host cannot hold
+        const double host_avail = host_bytes_available();
+        if (host_avail >= 0 && all_ck > 0.8 * host_avail)
+            return fail(WV_ENOMEM, "the checkpoints (" + std::to_string((long long)(all_ck / 1073741824.0)) +
+                                       " GiB) do not fit the host memory available (" +
+                                       std::to_string((long long)(host_avail / 1073741824.0)) +
+                                       " GiB): checkpoint less often or run fewer shots per handle");
+    }
     a->K2 = K2;
     a->nsub = (every + K2 - 1) / K2;
     if ((rc = dal((void **)&a->rec, (size_t)K2 * 9 * a->fld_n * 4))) return rc;
