@@ -79,7 +79,8 @@ typedef struct {
 } wv_config;
 
 /* dimension "2D"/"3D", type (case-insensitive): 2D "acoustic", "elastic", "viscoelastic", "sh",
- * "viscosh", "tmem"; 3D "acoustic", "elastic".  The factory's other types ("emem", "viscotmem",
+ * "viscosh", "tmem", "emem" (TE mode: fields hz, ex, ey; EX / EY / HZ sources and receivers,
+ * types 2 / 3 / 4); 3D "acoustic", "elastic".  The factory's other types ("viscotmem",
  * "viscoemem", 3D others) return WV_EINVAL ("not provided"), unknown ones WV_EINVAL. */
 int wv_create(const char *dimension, const char *type, wv_solver **out);
 int wv_destroy(wv_solver *s);

@@ -21,7 +21,8 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 
-PHYSICS = {"acoustic": 0, "elastic": 1, "tmem": 2, "sh": 3, "viscoelastic": 4, "visco": 4, "viscosh": 5}
+PHYSICS = {"acoustic": 0, "elastic": 1, "tmem": 2, "sh": 3, "viscoelastic": 4, "visco": 4, "viscosh": 5,
+           "emem": 6}
 
 
 class Params(C.Structure):
@@ -102,7 +103,7 @@ def run(params: Params, m0, m1, m2, src_idx, src_type, src_sig, rec_idx, rec_typ
     rec_idx = np.ascontiguousarray(rec_idx, dtype=np.int64)
     rec_type = np.ascontiguousarray(rec_type, dtype=np.int32)
     seis = np.zeros((len(rec_idx), params.NT), dtype=dt)
-    nf = {0: 3 if params.dim == 2 else 4, 1: 5 if params.dim == 2 else 9, 2: 3, 3: 3, 4: 5, 5: 3}[params.physics]
+    nf = {0: 3 if params.dim == 2 else 4, 1: 5 if params.dim == 2 else 9, 2: 3, 3: 3, 4: 5, 5: 3, 6: 3}[params.physics]
     fields = np.zeros((nf, n), dtype=dt) if want_fields else None
     if params.physics in (4, 5):
         freqs = np.ascontiguousarray(visco["freqs"], dtype=np.float64)

@@ -25,7 +25,8 @@
  *   averaging                   src/Modelparameter/Modelparameter.cpp:336-639
  *   shear-modulus averaging     src/Modelparameter/ModelparameterSeismic.cpp:421-433
  *   EM Ca/Cb                    src/ForwardSolverEM/ForwardSolverEM.cpp:14-68
- *   EM averaging, constants     src/ModelparameterEM/TMEM.cpp:366-373, Modelparameter.hpp:359-360
+ *   EM averaging, constants     src/ModelparameterEM/TMEM.cpp:366-373, EMEM.cpp:374-395,
+ *                               Modelparameter.hpp:359-360
  *   CPML coefficients           src/ForwardSolver/BoundaryCondition/CPML.cpp:39-95
  *   CPML placement              CPML2D.cpp:170-282, CPML2DAcoustic.cpp:106-210,
  *                               CPMLEM2D.cpp:138-243, CPML3D.cpp:222-366
@@ -33,7 +34,8 @@
  *   free surface                FreeSurfaceElastic.cpp:11-90, FreeSurface2Delastic.cpp:14-45,
  *                               FreeSurface3Delastic.cpp, FreeSurface.cpp:14-21
  *   time steps                  ForwardSolver2Dacoustic.cpp:91-194, ForwardSolver2Delastic.cpp:117-294,
- *                               ForwardSolver2Dtmem.cpp:108-171, ForwardSolver3Delastic.cpp:120-416
+ *                               ForwardSolver2Dtmem.cpp:108-171, ForwardSolver2Demem.cpp:80-171,
+ *                               ForwardSolver3Delastic.cpp:120-416
  *   source / receiver           SourceReceiverImpl.cpp:12-37, SourceReceiverImplSeismic.cpp:34-105,
  *                               FDTD2Delastic.cpp:12-49, FDTDacoustic.cpp:12-45
  *   Ricker wavelet              src/Acquisition/SourceSignal/Ricker.cpp:29-53
@@ -622,7 +624,7 @@ typedef struct {
     double damping_coeff, npower, fc_cpml, vmax_cpml;
 } oracle_params;
 
-enum { PH_ACOUSTIC = 0, PH_ELASTIC = 1, PH_TMEM = 2, PH_SH = 3, PH_VISCO = 4, PH_VISCOSH = 5 };
+enum { PH_ACOUSTIC = 0, PH_ELASTIC = 1, PH_TMEM = 2, PH_SH = 3, PH_VISCO = 4, PH_VISCOSH = 5, PH_TEMEM = 6 };
 
 /* Viscoelastic parameters of the next oracle_run (2D viscoelastic and visco-SH): L relaxation
  * mechanisms with their relaxation frequencies [Hz], tauP and tauS per cell (grid order).
@@ -773,12 +775,12 @@ int oracle_abs_layout(const oracle_params *p, real *out)
  *   acoustic: m0 = vp, m1 unused, m2 = rho
  *   sh      : m0 unused, m1 = vs, m2 = rho
  *   elastic : m0 = vp, m1 = vs,   m2 = rho
- *   tmem    : m0 = epsilon_r, m1 = sigma, m2 = mu_r
- * Source / receiver type: 1 = P (EZ for tmem), 2 = VX, 3 = VY, 4 = VZ.
+ *   tmem, emem : m0 = epsilon_r, m1 = sigma, m2 = mu_r
+ * Source / receiver type: 1 = P (EZ for tmem), 2 = VX (EX), 3 = VY (EY), 4 = VZ (HZ for emem).
  * src_sig is [nsrc][NT]; seis is [nrec][NT], in receiver input order.
  * fields_out (optional) receives the final wavefields:
  *   acoustic 2D: vx vy p ; acoustic 3D: vx vy vz p ; elastic 2D: vx vy sxx syy sxy ; tmem: hx hy ez ;
- *   sh: vz sxz syz ;
+ *   sh: vz sxz syz ; emem: hz ex ey ;
  *   elastic 3D: vx vy vz sxx syy szz syz sxz sxy
  * t_end (<= NT) allows truncated runs for timing; seis columns past it stay 0.
  */
@@ -790,7 +792,7 @@ int oracle_run(const oracle_params *p, const real *m0, const real *m1, const rea
     if (p->nthreads > 0) omp_set_num_threads(p->nthreads);
 #endif
     if (p->dim == 3 && (p->physics == PH_TMEM || p->physics == PH_SH || p->physics == PH_VISCO ||
-                        p->physics == PH_VISCOSH))
+                        p->physics == PH_VISCOSH || p->physics == PH_TEMEM))
         return -2;
     if ((p->physics == PH_VISCO || p->physics == PH_VISCOSH) && (g_visco_L < 1 || !g_tauP || !g_tauS)) return -3;
     grid_t g = {p->NX, p->NY, p->NZ, (long)p->NX * p->NY * p->NZ};
@@ -806,6 +808,7 @@ int oracle_run(const oracle_params *p, const real *m0, const real *m1, const rea
     /* model coefficients */
     real *M = NULL, *mu = NULL, *irx = NULL, *iry = NULL, *irz = NULL, *muxy = NULL, *muxz = NULL, *muyz = NULL;
     real *Ca = NULL, *Cb = NULL, *imxz = NULL, *imyz = NULL;
+    real *CaY = NULL, *CbY = NULL, *imxy = NULL; /* emem: Ca, Cb above are the x-averaged ones */
     /* viscoelastic: relaxed moduli, (1 + L tau), averaged tauS, per-mechanism constants */
     int VL = (p->physics == PH_VISCO || p->physics == PH_VISCOSH) ? g_visco_L : 0;
     real *tauSxz = NULL, *tauSyz = NULL;
@@ -932,6 +935,55 @@ int oracle_run(const oracle_params *p, const real *m0, const real *m1, const rea
         muyz = vnew(n);
         shear_average_2pt(&g, mu, 0, muxz);
         shear_average_2pt(&g, mu, 1, muyz);
+    } else if (p->physics == PH_TEMEM) {
+        /* EMEM.cpp:374-395: 1/mu with the 4-point xy average (calcInverseAveragedParameter,
+         * Modelparameter.cpp:633-639, invalid -> 0); epsilon and sigma with the 2-point averages
+         * along x and y (calcAveragedParameter, averageMatrixX/Y, Modelparameter.cpp:336-408);
+         * ForwardSolver2Demem.cpp:80-92: Ca, Cb of the averaged values (ForwardSolverEM.cpp:14-68) */
+        real eps0 = (real)8.8541878176e-12, mu0 = (real)1.2566370614e-6;
+        real *eps = vnew(n), *mup = vnew(n);
+        for (long i = 0; i < n; i++) {
+            eps[i] = m0[i] * eps0;
+            mup[i] = m2[i] * mu0;
+        }
+        imxy = vnew(n);
+        Ca = vnew(n);
+        Cb = vnew(n);
+        CaY = vnew(n);
+        CbY = vnew(n);
+        for (long i = 0; i < n; i++) {
+            long x, y, z;
+            gcoord(&g, i, &x, &y, &z);
+            real v = (real)1 / avg4_row(&g, mup, x, y, z, 0, 1);
+            if (isnan(v) || isinf(v)) v = 0;
+            imxy[i] = v;
+            for (int ax = 0; ax < 2; ax++) {
+                long L = ax == 0 ? g.NX : g.NY, q = ax == 0 ? x : y;
+                real e, sg, h = (real)0.5;
+                if (q + 1 < L) {
+                    long j = ax == 0 ? gidx(&g, x + 1, y, z) : gidx(&g, x, y + 1, z);
+                    e = h * eps[i];
+                    e += h * eps[j];
+                    sg = h * m1[i];
+                    sg += h * m1[j];
+                } else {
+                    e = (real)1.0 * eps[i];
+                    sg = (real)1.0 * m1[i];
+                }
+                real c = (real)0.5 / e;
+                c *= sg;
+                c *= DT;
+                real cinv = (real)1 / (c + (real)1);
+                real ca = (real)1 - c;
+                ca *= cinv;
+                real cb = (real)1 / e;
+                cb *= cinv;
+                (ax == 0 ? Ca : CaY)[i] = ca;
+                (ax == 0 ? Cb : CbY)[i] = cb;
+            }
+        }
+        free(eps);
+        free(mup);
     } else {
         real eps0 = (real)8.8541878176e-12, mu0 = (real)1.2566370614e-6;
         real *eps = vnew(n), *mup = vnew(n);
@@ -1442,6 +1494,32 @@ int oracle_run(const oracle_params *p, const real *m0, const real *m1, const rea
                 v_mul(n, syz, damp);
                 v_mul(n, vz, damp);
             }
+        } else if (p->physics == PH_TEMEM) {
+            /* ForwardSolver2Demem.cpp:108-171 ; F: hz ex ey ; CPMLEM2D.cpp: eyx, exy on the half
+             * profiles, hzx, hzy on the full ones */
+            real *hz = F[0], *ex = F[1], *ey = F[2];
+            spmv(&o.Dxf, ey, upd);
+            spmv(&o.Dyf, ex, tmp);
+            if (use_cpml) {
+                cpml_apply(n, upd, psi[0], axh, bxh); /* eyx */
+                cpml_apply(n, tmp, psi[1], ayh, byh); /* exy */
+            }
+            v_sub(n, upd, tmp);
+            v_mul(n, upd, imxy);
+            v_sub(n, hz, upd);
+            spmv(&o.Dyb, hz, upd);
+            if (use_cpml) cpml_apply(n, upd, psi[2], ay, by); /* hzy */
+            v_mul(n, upd, Cb);
+            VLOOP(n) ex[i] = Ca[i] * ex[i] + upd[i];
+            spmv(&o.Dxb, hz, upd);
+            if (use_cpml) cpml_apply(n, upd, psi[3], ax, bx); /* hzx */
+            v_mul(n, upd, CbY);
+            VLOOP(n) ey[i] = CaY[i] * ey[i] - upd[i];
+            if (damp) {
+                v_mul(n, ey, damp);
+                v_mul(n, ex, damp);
+                v_mul(n, hz, damp);
+            }
         } else {
             /* ForwardSolver2Dtmem.cpp:108-171 ; F: hx hy ez */
             real *hx = F[0], *hy = F[1], *ez = F[2];
@@ -1499,6 +1577,11 @@ int oracle_run(const oracle_params *p, const real *m0, const real *m1, const rea
                     if (ty == 4) F[2][i] += v;
                 } else if (p->physics == PH_SH || p->physics == PH_VISCOSH) {
                     if (ty == 4) F[0][i] += v; /* only VZ sources in SH (FDTD2Dsh.hpp:25-27) */
+                } else if (p->physics == PH_TEMEM) {
+                    /* EX, EY, HZ (SourceReceiverImplEM.cpp:100-165; FDTD2Demem.hpp: no EZ) */
+                    if (ty == 2) F[1][i] += v;
+                    if (ty == 3) F[2][i] += v;
+                    if (ty == 4) F[0][i] += v;
                 } else {
                     if (ty == 1) F[2][i] += v;
                 }
@@ -1527,6 +1610,8 @@ int oracle_run(const oracle_params *p, const real *m0, const real *m1, const rea
                     v = ty == 2 ? F[0][i] : (ty == 3 ? F[1][i] : F[2][i]);
             } else if (p->physics == PH_SH || p->physics == PH_VISCOSH) {
                 v = F[0][i];
+            } else if (p->physics == PH_TEMEM) {
+                v = ty == 2 ? F[1][i] : (ty == 3 ? F[2][i] : F[0][i]);
             } else {
                 v = F[2][i];
             }
@@ -1564,6 +1649,9 @@ int oracle_run(const oracle_params *p, const real *m0, const real *m1, const rea
     free(Cb);
     free(imxz);
     free(imyz);
+    free(CaY);
+    free(CbY);
+    free(imxy);
     free(fsH);
     free(fsV);
     free(fsZero);

@@ -5,7 +5,7 @@ import numpy as np
 
 from oracle import pyoracle as po
 
-PHYS = {"acoustic": 0, "elastic": 1, "tmem": 2, "sh": 3, "viscoelastic": 4, "viscosh": 5}
+PHYS = {"acoustic": 0, "elastic": 1, "tmem": 2, "sh": 3, "viscoelastic": 4, "viscosh": 5, "emem": 6}
 
 
 def oracle_threads():

@@ -224,3 +224,65 @@ def test_cli_rejects_unknown_wavelets(tmp_path):
                             "SeismogramFormat=1"],
                            capture_output=True, text=True, timeout=60)
         assert r.returncode == 1 and msg in r.stderr, (line, r.stderr)
+
+
+@pytest.mark.parametrize("eq", ["tmem", "emem", "viscosh"])
+def test_cli_em_and_viscosh_models_from_files(tmp_path, eq):
+    """equationType tmem / emem (EM model files <ModelFilename>.epsilonr/.sigma/.mur.mtx, seismograms
+    named ez / ex / ey / hz, Acquisition.hpp:40-55) and viscosh (vs, density, tauS files; relaxation
+    mechanisms from the configuration): the driver's seismograms of every component equal the oracle's
+    float build on the same model and Ricker sources (relative L2 <= 1e-5)."""
+    import wavesim
+    from _cases import PHYS, oracle_threads, rel_l2
+
+    NX, NY, W = 110, 84, 10
+    em = eq != "viscosh"
+    NT, DT, DH = (400, 3e-11, 0.02) if em else (500, 1e-4, 1.0)
+    fc = 2e8 if em else 60.0
+    rng = np.random.default_rng(5)
+    blk = lambda lo, hi: np.kron(rng.uniform(lo, hi, (-(-NY // 12), -(-NX // 12))), np.ones((12, 12)))[:NY, :NX].ravel()
+    if em:
+        m = {"epsilonr": blk(1.0, 12.0), "sigma": blk(0.0, 0.008), "mur": blk(1.0, 2.5)}
+        order = [m["epsilonr"], m["sigma"], m["mur"]]
+    else:
+        m = {"vp": np.full(NX * NY, 3000.0), "vs": blk(1200.0, 1700.0), "density": blk(1900.0, 2400.0),
+             "tauS": blk(0.05, 0.2)}
+        order = [m["vp"], m["vs"], m["density"]]
+    base = tmp_path / "model"
+    for k, v in m.items():
+        _write_vec(f"{base}.{k}.mtx", v.astype(np.float32))
+    srcs = {"tmem": [(30, 40, 1), (75, 52, 1)], "emem": [(30, 40, 2), (75, 52, 4)], "viscosh": [(30, 40, 4)]}[eq]
+    rty = {"tmem": [1], "emem": [2, 3, 4], "viscosh": [4]}[eq]
+    with open(tmp_path / "src.txt", "w") as f:
+        for k, (x, y, t) in enumerate(srcs):
+            f.write(f"0 {x} {y} 0 {t} 1 1 {fc:g} {1.0 - 0.3 * k:g} 0\n")
+    recs = [(x, 22 + 4 * (x % 3), rty[x % len(rty)]) for x in range(14, NX - 14, 5)]
+    with open(tmp_path / "rec.txt", "w") as f:
+        for x, y, t in recs:
+            f.write(f"{x} {y} 0 {t}\n")
+    out = tmp_path / "seis"
+    args = [EXE, os.path.join(CI, "configuration_ci.2D.acoustic.txt"), f"equationType={eq}", f"NX={NX}", f"NY={NY}",
+            f"DH={DH}", f"DT={DT}", f"T={NT * DT!r}", f"seismoDT={DT!r}", "spatialFDorder=8", "useStencilMatrix=0",
+            "DampingBoundary=2", f"BoundaryWidth={W}", f"CenterFrequencyCPML={fc:g}",
+            f"VMaxCPML={3e8 if em else 3000.0:g}", "ModelRead=1", f"ModelFilename={base}",
+            f"SourceFilename={tmp_path / 'src'}", f"ReceiverFilename={tmp_path / 'rec'}", f"SeismogramFilename={out}",
+            "SeismogramFormat=1", "normalizeTraces=0", "numRelaxationMechanisms=2", "relaxationFrequency=25",
+            "relaxationFrequency2=80"]
+    r = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    p = po.make_params(dim=2, physics=PHYS[eq], NX=NX, NY=NY, NT=NT, order=8, damping=2, width=W, DH=DH, DT=DT,
+                       npower=4.0, fc_cpml=fc, vmax_cpml=3e8 if em else 3000.0, nthreads=oracle_threads())
+    sig = np.stack([wavesim.source_signal(1, NT, DT, fc, 1.0 - 0.3 * k, 0.0) for k in range(len(srcs))])
+    visco = None if em else dict(freqs=[25.0, 80.0], tauP=np.zeros(NX * NY, np.float32),
+                                 tauS=m["tauS"].astype(np.float32))
+    o32, _ = po.run(p, *(v.astype(np.float32) for v in order), [x + y * NX for x, y, _ in srcs],
+                    [t for _, _, t in srcs], sig.astype(np.float32), [x + y * NX for x, y, _ in recs],
+                    [t for _, _, t in recs], "f32", visco=visco)
+    names = {1: "ez", 2: "ex", 3: "ey", 4: "hz"} if em else {4: "vz"}
+    for t in rty:
+        rows = [k for k, rr in enumerate(recs) if rr[2] == t]
+        seis = po.read_mtx_array(f"{out}.shot_0.{names[t]}.mtx")
+        e = rel_l2(seis, o32[rows])
+        print(f"{eq} {names[t]}: {len(rows)} traces, relL2 vs f32 oracle {e:.2e}")
+        assert np.abs(o32[rows]).max() > 0 and e <= 1e-5

@@ -1,4 +1,3 @@
-import dataclasses
 """Invariants that pin the oracle's CPML and TM-EM paths (no reference fixture covers them).
 
 * CPML absorbs: a run in a small domain with CPML matches, at the receivers, a
@@ -10,7 +9,15 @@ import dataclasses
   path is pinned by the reference's acoustic golden seismogram.
 * TM-EM with sigma > 0: the energy of a propagating wave decays as
   exp(-sigma t / epsilon) (Ca/Cb of ForwardSolverEM.cpp:14-68).
+* TE-EM (ForwardSolver2Demem.cpp:108-171) on a model that varies along one axis
+  only, driven by a line source across the other, is the TM-EM system of the
+  same model along that axis (ey <-> ez, hz <-> -hy; ex <-> ez, hz <-> -hx),
+  away from the edges the line source meets, including the CPML terms of that
+  axis and the loss: the averaging axis of each E component, its Ca / Cb, the
+  4-point 1/mu and the derivative directions all enter.
 """
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -244,3 +251,49 @@ def test_viscosh_mechanisms_split_at_one_frequency():
         sL, _ = oracle_shot(c, c.shots[0], "f64")
         assert np.linalg.norm(sL - s1) / np.linalg.norm(s1) < 1e-7, L
 
+
+
+def _te_tm_line(axis, NT=250, W=10, precision="f64"):
+    """TE and TM runs of the same model varying along `axis` only (blocky eps_r 4..9, sigma
+    0..4e-3 S/m, mu_r 1..2), a line source across the other axis at position 40 (EY for TE along
+    x, EX along y; EZ for TM), CPML all round; returns the middle rows' (or columns') fields."""
+    N_a, N_o = 100, 140  # cells along the model axis, across it
+    NX, NY = (N_a, N_o) if axis == 0 else (N_o, N_a)
+    rng = np.random.default_rng(11 + axis)
+    prof = [rng.uniform(lo, hi, N_a // 10).repeat(10) for lo, hi in ((4.0, 9.0), (0.0, 4e-3), (1.0, 2.0))]
+    xs, ys = np.meshgrid(np.arange(NX), np.arange(NY))
+    pos = xs if axis == 0 else ys
+    eps_r, sigma, mu_r = (pr[pos].ravel() for pr in prof)
+    DT = 3e-11
+    sig = po.ricker(NT, DT, 2e8, 1.0, 0.0, "f64")
+    line = [(40 + y * NX) if axis == 0 else (x + 40 * NX) for y in range(NY) for x in range(NX)
+            if (axis == 0 and x == 0) or (axis == 1 and y == 0)]
+    line = [(40 + k * NX) if axis == 0 else (k + 40 * NX) for k in range(N_o)]
+    mid = N_o // 2
+    rec = [(x + mid * NX) if axis == 0 else (mid + x * NX) for x in range(12, N_a - 12, 6)]
+    out = {}
+    for phys, ty in (("emem", 3 if axis == 0 else 2), ("tmem", 1)):
+        p = po.make_params(dim=2, physics=po.PHYSICS[phys], NX=NX, NY=NY, NT=NT, order=8, damping=2, width=W,
+                           DH=0.02, DT=DT, npower=4.0, fc_cpml=2e8, vmax_cpml=1.5e8, nthreads=8)
+        seis, F = po.run(p, eps_r, sigma, mu_r, line, [ty] * len(line), np.tile(sig, (len(line), 1)), rec,
+                         [ty] * len(rec), precision, want_fields=True)
+        F = F.reshape(3, NY, NX)
+        out[phys] = (seis, F[:, mid, :] if axis == 0 else F[:, :, mid])
+    return out
+
+
+@pytest.mark.parametrize("axis", [0, 1])
+def test_emem_line_source_is_the_tmem_system(axis):
+    out = _te_tm_line(axis)
+    te_s, te_f = out["emem"]
+    tm_s, tm_f = out["tmem"]
+    hz, ex, ey = te_f
+    hx, hy, ez = tm_f
+    e_comp, h_te, h_tm = (ey, hz, -hy) if axis == 0 else (ex, hz, -hx)
+    assert np.abs(tm_s).max() > 0 and np.abs(h_tm).max() > 0
+    errs = [np.linalg.norm(a - b) / np.linalg.norm(b) for a, b in ((te_s, tm_s), (e_comp, ez), (h_te, h_tm))]
+    print(f"TE line source along {'yx'[axis]} vs TM: seismograms {errs[0]:.1e}, E {errs[1]:.1e}, H {errs[2]:.1e}")
+    assert max(errs) < 1e-12
+    # the other E component stays negligible in the middle (a line source excites one; the
+    # edges' disturbance reaches the middle only through the stencils' far reach)
+    assert np.abs(ex if axis == 0 else ey).max() < 1e-12 * np.abs(e_comp).max()

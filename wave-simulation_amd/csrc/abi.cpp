@@ -236,11 +236,12 @@ double wv_solver::kernel_bytes(int kind) const
         return cells * (B * per_shot[kind - 3] + per_cell[kind - 3]);
     }
     // {field reads+writes per shot-cell, coefficient reads per cell, psi per x-frame cell, psi per y-frame cell}
-    static const int phased[4][2][4] = {
+    static const int phased[5][2][4] = {
         {{5, 2, 1, 1}, {4, 1, 1, 1}},  // acoustic: velocity, pressure
         {{7, 2, 2, 2}, {8, 3, 2, 2}},  // elastic: velocity, stress
         {{5, 2, 1, 1}, {4, 2, 1, 1}},  // tmem: H, E
         {{4, 1, 1, 1}, {5, 2, 1, 1}},  // sh: vz, stresses
+        {{4, 1, 1, 1}, {5, 4, 1, 1}},  // emem: hz, (ex, ey)
     };
     static const int single[4] = {10, 5, 4, 4};  // elastic single pass
     if (variant == V_SINGLE && kind == 1) return 0.0;
@@ -283,6 +284,8 @@ int wv_create(const char *dimension, const char *type, wv_solver **out)
         phys = wv::TMEM;
     else if (t == "sh")
         phys = wv::SH;
+    else if (t == "emem" && d == "2d")
+        phys = wv::TEMEM;
     else if (t == "viscoelastic" && d == "2d")
         phys = wv::ELASTIC;  // the elastic step with memory variables in the stress phase
     else if (t == "viscosh" && d == "2d")
@@ -317,7 +320,8 @@ int wv_init(wv_solver *s, const wv_config *c)
     if (N < 2 || N > 12 || (N & 1)) return fail(WV_EINVAL, "spatialFDorder must be one of 2,4,6,8,10,12");
     if (k.ops_mode != WV_OPS_SPARSE && k.ops_mode != WV_OPS_STENCIL) return fail(WV_EINVAL, "bad ops_mode");
     if (k.free_surface != 0 && k.free_surface != 1) return fail(WV_EINVAL, "FreeSurface must be 0 or 1");
-    if (k.free_surface && s->physics == wv::TMEM) return fail(WV_EINVAL, "FreeSurface is not available for tmem");
+    if (k.free_surface && (s->physics == wv::TMEM || s->physics == wv::TEMEM))
+        return fail(WV_EINVAL, "FreeSurface is not available for the EM solvers");
     if (k.damping_boundary < 0 || k.damping_boundary > 2) return fail(WV_EINVAL, "DampingBoundary must be 0, 1 or 2");
     if (k.damping_boundary && (k.boundary_width < 1 || 2 * k.boundary_width > std::min(k.NX, k.NY)))
         return fail(WV_EINVAL, "BoundaryWidth out of range");
@@ -334,7 +338,7 @@ int wv_init(wv_solver *s, const wv_config *c)
         return fail(WV_EINVAL, "tile_y of the row-strip kernels must be 0 (default), 8, 16 or 32");
     if (k.path == WV_PATH_PLAIN && k.nshots != 1) return fail(WV_EINVAL, "the plain path runs one shot per handle");
     if (s->dim == 3 && k.path == WV_PATH_PLAIN) return fail(WV_EINVAL, "the plain path is 2D only");
-    if (s->physics == wv::SH && k.path == WV_PATH_PLAIN)
+    if ((s->physics == wv::SH || s->physics == wv::TEMEM) && k.path == WV_PATH_PLAIN)
         return fail(WV_EINVAL, "the plain path is provided for acoustic, elastic and tmem");
     const bool single = s->dim == 2 && k.path == WV_PATH_FUSED && s->physics == wv::ELASTIC && !s->visco;
     if (s->dim == 2 && single && k.tile_y != 0 && std::abs(k.tile_y) != 8 && std::abs(k.tile_y) != 16 && std::abs(k.tile_y) != 24)
@@ -616,6 +620,43 @@ int wv_set_model(wv_solver *s, const float *m0, const float *m1, const float *m2
                     if (v < 4.0) v = 0.0;
                     c[1 + ax][li(x, y)] = (float)v;
                 }
+    } else if (s->physics == wv::TEMEM) {
+        // TE-EM (EMEM.cpp:374-395, ForwardSolver2Demem.cpp:80-92): 1/mu with the 4-point xy average
+        // (Modelparameter.cpp:449-526, 633-639; invalid -> 0); epsilon and sigma averaged along x for
+        // ex and along y for ey (2-point, the last point alone), then Ca, Cb in the reference's float
+        // steps; the averages are formed in float as the float build's sparse products form them
+        const double mu0 = 1.2566370614e-6;
+        const float eps0f = 8.8541878176e-12f, dtf = (float)s->cfg.DT;
+        std::vector<float> eps((size_t)NX * NY);
+        for (size_t i = 0; i < eps.size(); i++) eps[i] = m0[i] * eps0f;
+        for (int y = 0; y < NY; y++)
+            for (int x = 0; x < NX; x++) {
+                const bool inx = x + 1 < NX, iny = y + 1 < NY;
+                const double a = (double)m2[gi(x, y)] * mu0;
+                double avg;
+                if (inx && iny)
+                    avg = 0.25 * (a + (double)m2[gi(x + 1, y)] * mu0 + (double)m2[gi(x, y + 1)] * mu0 +
+                                  (double)m2[gi(x + 1, y + 1)] * mu0);
+                else if (inx)
+                    avg = 0.5 * (a + (double)m2[gi(x + 1, y)] * mu0);
+                else if (iny)
+                    avg = 0.5 * (a + (double)m2[gi(x, y + 1)] * mu0);
+                else
+                    avg = a;
+                double v = 1.0 / avg;
+                if (!std::isfinite(v)) v = 0.0;
+                c[0][li(x, y)] = (float)v;
+                for (int ax = 0; ax < 2; ax++) {
+                    const bool last = ax == 0 ? !inx : !iny;
+                    const size_t i = gi(x, y), j = last ? i : (ax == 0 ? gi(x + 1, y) : gi(x, y + 1));
+                    float e = eps[i], sg = m1[i];
+                    if (!last) {
+                        e = 0.5f * eps[i] + 0.5f * eps[j];
+                        sg = 0.5f * m1[i] + 0.5f * m1[j];
+                    }
+                    wv::em_ca_cb(e, sg, dtf, c[1 + 2 * ax][li(x, y)], c[2 + 2 * ax][li(x, y)]);
+                }
+            }
     } else {
         const double mu0 = 1.2566370614e-6;
         const float eps0f = 8.8541878176e-12f, dtf = (float)s->cfg.DT;
@@ -793,9 +834,11 @@ int wv_set_acquisition(wv_solver *s, int shot, int nsrc, const long long *src_in
     if ((nsrc && (!src_index || !src_type || !src_signal)) || (nrec && (!rec_index || !rec_type)))
         return fail(WV_EINVAL, "missing acquisition array");
     const long long n = (long long)s->cfg.NX * s->cfg.NY * s->cfg.NZ;
-    const int maxtype = s->physics == wv::TMEM ? 1 : (s->dim == 3 || s->physics == wv::SH ? 4 : 3);
-    // SH has VZ sources and receivers only (FDTD2Dsh.hpp:25-27 throws for P, VX, VY)
-    const int mintype = s->physics == wv::SH ? 4 : 1;
+    const int maxtype =
+        s->physics == wv::TMEM ? 1 : (s->dim == 3 || s->physics == wv::SH || s->physics == wv::TEMEM ? 4 : 3);
+    // SH has VZ sources and receivers only (FDTD2Dsh.hpp:25-27 throws for P, VX, VY); TE-EM
+    // EX, EY, HZ (FDTD2Demem.hpp:29-30 throws for EZ)
+    const int mintype = s->physics == wv::SH ? 4 : (s->physics == wv::TEMEM ? 2 : 1);
     for (int k = 0; k < nsrc; k++) {
         if (src_index[k] < 0 || src_index[k] >= n) return fail(WV_EINVAL, "source outside the grid");
         if (src_type[k] < mintype || src_type[k] > maxtype) return fail(WV_EINVAL, "unsupported source type");
@@ -1103,9 +1146,11 @@ int wv_kernel_stats(wv_solver *s, int kind, const char **name, double *avg_ms, l
     if (!s || !s->inited) return fail(WV_ESTATE, "not initialised");
     if (kind < 0 || kind > 6 || (s->e3 && kind > 2)) return fail(WV_EINVAL, "unknown kernel kind");
     if (s->plain) return fail(WV_EINVAL, "kernel timing is provided by the tiled kernels");
-    static const char *names[3][3] = {{"ac2d_velocity", "ac2d_pressure", "flush2d"},
+    static const char *names[5][3] = {{"ac2d_velocity", "ac2d_pressure", "flush2d"},
                                       {"el2d_velocity", "el2d_stress", "flush2d"},
-                                      {"tm2d_h", "tm2d_e", "flush2d"}};
+                                      {"tm2d_h", "tm2d_e", "flush2d"},
+                                      {"sh2d_v", "sh2d_s", "flush2d"},
+                                      {"te2d_h", "te2d_e", "flush2d"}};
     static const char *adj_names[4] = {"velocity phase (recompute)", "stress phase (recompute)", "adj_el2d_stress",
                                        "adj_el2d_velocity"};
     HIP_TRY(hipStreamSynchronize(s->st));

@@ -7,18 +7,21 @@
 namespace wv {
 namespace {
 
-// velocity-type components: VX (2) and VY (3), VZ (4) in SH; stress-type: P / EZ (1)
+// velocity-type components (the field the first phase of a step writes): VX (2) and
+// VY (3), VZ (4) in SH, HZ (4) in TE-EM; stress-type: P / EZ (1), EX (2) / EY (3) in TE-EM
 __device__ __forceinline__ bool velocity_type(int ty) { return ty == 2 || ty == 3; }
 template <int PHYS>
 __device__ __forceinline__ bool velocity_type_of(int ty)
 {
-    return PHYS == SH ? ty == 4 : velocity_type(ty);
+    return (PHYS == SH || PHYS == TEMEM) ? ty == 4 : velocity_type(ty);
 }
-// field of a velocity-type component: vx, vy (fields 0, 1); vz is field 0 of SH
+// field of a component: vx, vy (fields 0, 1), P / EZ field 2; vz of SH and hz of TE-EM
+// field 0, ex / ey of TE-EM fields 1 / 2
 template <int PHYS>
 __device__ __forceinline__ int velocity_field(int ty)
 {
-    return PHYS == SH ? 0 : ty - 2;
+    if (PHYS == TEMEM) return ty == 4 ? 0 : ty - 1;
+    return PHYS == SH ? 0 : (ty == 1 ? 2 : ty - 2);
 }
 
 // Sources and receivers owned by this block (shot-block sb) at step t, for
@@ -43,8 +46,6 @@ __device__ void block_acquisition(const Acq &A, const Fields2D &F, long long so,
                 if (PHYS == ELASTIC && ty == 1) {
                     F.f[2][c] += v;
                     F.f[3][c] += v;
-                } else if (ty == 1) {
-                    F.f[2][c] += v;
                 } else {
                     F.f[velocity_field<PHYS>(ty)][c] += v;
                 }

@@ -639,6 +639,134 @@ __global__ __launch_bounds__(NTHREADS) void sh2d_s(Layout2D L, Fields2D F, Coef2
     block_acquisition<SH>(A, F, so, shot_block(A, B), A.tbase[0] + tofs, false);
 }
 
+// ---------------------------------------------------------------------------
+// TE-EM (ForwardSolver2Demem.cpp:108-171): phase 0 hz from ex, ey (forward
+// derivatives; CPMLEM2D eyx / exy on the half profiles), phase 1 ex, ey from hz
+// (backward derivatives; hzy / hzx on the full profiles).  HZ sources and
+// receivers are velocity-type (the field phase 0 writes), EX / EY stress-type.
+// ---------------------------------------------------------------------------
+template <int H, int TY>
+__global__ __launch_bounds__(NTHREADS) void te2d_h(Layout2D L, Fields2D F, Coef2D C, Taps T, Cpml2D P, Acq A,
+                                                   int tofs, int pend)
+{
+    using TL = Tile<H, TY>;
+    constexpr int R = TY / BLOCK_Y;
+    __shared__ float s_ex[TL::RH][TL::RW];
+    __shared__ float s_ey[TL::RH][TL::RW];
+    const Blk B = xcd_block();
+    const int shot = B.shot;
+    const long long so = (long long)shot * L.plane;
+    const int x0 = B.bx * TILE_X, y0 = B.by * TY;
+    const int tid = threadIdx.y * TILE_X + threadIdx.x, tx = threadIdx.x;
+    if (pend) block_acquisition<TEMEM>(A, F, so, shot_block(A, B), A.tbase[0] + tofs - 1, true);
+    TileRegs<H, TY> rex, rey;
+    rex.load(F.f[1] + so, L, x0, y0, tid);
+    rey.load(F.f[2] + so, L, x0, y0, tid);
+    float *__restrict__ hz = F.f[0] + so;
+    const Col col = column(L, P, x0 + tx);
+    float cxf[2 * H + 1];
+    load_taps<H>(cxf, T.xf, col.x);
+    const int W2 = 2 * P.W;
+    const int wy = __builtin_amdgcn_readfirstlane(threadIdx.y);
+    float oh[R], cm[R], p0[R], q0[R];
+    int yi[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int y = y0 + wy + r * BLOCK_Y;
+        const long long o = L.at(col.x, y);
+        oh[r] = hz[o];
+        cm[r] = C.c[0][o];
+        p0[r] = q0[r] = 0.f;
+        yi[r] = P.on ? frame_index(y, L.NY, P.W, 0) : -1;
+        if (col.xi >= 0 && y < L.NY) p0[r] = P.psx[0][((long long)shot * L.NY + y) * W2 + col.xi];
+        if (yi[r] >= 0 && col.in) q0[r] = P.psy[0][((long long)shot * W2 + yi[r]) * L.NX + col.x];
+    }
+    rex.store(s_ex, tid);
+    rey.store(s_ey, tid);
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int ly = wy + r * BLOCK_Y;
+        const int y = y0 + ly;
+        if (y >= L.NY) break;
+        const float *cyf = T.yf + (long long)y * (2 * H + 1);
+        const int Rr = ly + H;
+        float u = dx<H>(s_ey, Rr, tx, cxf);
+        float t = dy<H>(s_ex, Rr, tx, cyf);
+        if (!col.in) continue;
+        if (col.xi >= 0) P.psx[0][((long long)shot * L.NY + y) * W2 + col.xi] = cpml(u, p0[r], col.axh, col.bxh);
+        if (yi[r] >= 0) P.psy[0][((long long)shot * W2 + yi[r]) * L.NX + col.x] = cpml(t, q0[r], P.ayh[y], P.byh[y]);
+        // hz -= 1/mu_xy * (CPML(Dxf ey) - CPML(Dyf ex))
+        u -= t;
+        u *= cm[r];
+        hz[L.at(col.x, y)] = oh[r] - u;
+    }
+}
+
+template <int H, int TY>
+__global__ __launch_bounds__(NTHREADS) void te2d_e(Layout2D L, Fields2D F, Coef2D C, Taps T, Cpml2D P, Acq A,
+                                                   int tofs, int pend)
+{
+    (void)pend;
+    using TL = Tile<H, TY>;
+    constexpr int R = TY / BLOCK_Y;
+    __shared__ float s_h[TL::RH][TL::RW];
+    const Blk B = xcd_block();
+    const int shot = B.shot;
+    const long long so = (long long)shot * L.plane;
+    const int x0 = B.bx * TILE_X, y0 = B.by * TY;
+    const int tid = threadIdx.y * TILE_X + threadIdx.x, tx = threadIdx.x;
+    TileRegs<H, TY> rh;
+    rh.load(F.f[0] + so, L, x0, y0, tid);
+    float *__restrict__ ex = F.f[1] + so;
+    float *__restrict__ ey = F.f[2] + so;
+    const Col col = column(L, P, x0 + tx);
+    float cxb[2 * H + 1];
+    load_taps<H>(cxb, T.xb, col.x);
+    const int W2 = 2 * P.W;
+    const int wy = __builtin_amdgcn_readfirstlane(threadIdx.y);
+    float oex[R], oey[R], p1[R], q1[R];
+    int yi[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int y = y0 + wy + r * BLOCK_Y;
+        const long long o = L.at(col.x, y);
+        oex[r] = ex[o];
+        oey[r] = ey[o];
+        p1[r] = q1[r] = 0.f;
+        yi[r] = P.on ? frame_index(y, L.NY, P.W, 0) : -1;
+        if (col.xi >= 0 && y < L.NY) p1[r] = P.psx[1][((long long)shot * L.NY + y) * W2 + col.xi];
+        if (yi[r] >= 0 && col.in) q1[r] = P.psy[1][((long long)shot * W2 + yi[r]) * L.NX + col.x];
+    }
+    rh.store(s_h, tid);
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int ly = wy + r * BLOCK_Y;
+        const int y = y0 + ly;
+        if (y >= L.NY) break;
+        const float *cyb = T.yb + (long long)y * (2 * H + 1);
+        const int Rr = ly + H;
+        float u = dy<H>(s_h, Rr, tx, cyb);
+        float w = dx<H>(s_h, Rr, tx, cxb);
+        if (!col.in) continue;
+        if (yi[r] >= 0) P.psy[1][((long long)shot * W2 + yi[r]) * L.NX + col.x] = cpml(u, q1[r], P.ay[y], P.by[y]);
+        if (col.xi >= 0) P.psx[1][((long long)shot * L.NY + y) * W2 + col.xi] = cpml(w, p1[r], col.ax, col.bx);
+        const long long o = L.at(col.x, y);
+        const float dmp = C.damp ? C.damp[o] : 1.f;
+        // ex = Ca_x ex + Cb_x CPML(Dyb hz) ; ey = Ca_y ey - Cb_y CPML(Dxb hz)
+        u *= C.c[2][o];
+        float e = C.c[1][o] * oex[r];
+        e += u;
+        ex[o] = e * dmp;
+        w *= C.c[4][o];
+        float f = C.c[3][o] * oey[r];
+        f -= w;
+        ey[o] = f * dmp;
+    }
+    block_acquisition<TEMEM>(A, F, so, shot_block(A, B), A.tbase[0] + tofs, false);
+}
+
 // velocity-type sources / receivers of step t for every shot (end of a run segment)
 template <int PHYS>
 __global__ __launch_bounds__(256) void flush2d(Layout2D L, Fields2D F, Acq A, int tofs)
@@ -664,7 +792,7 @@ __global__ __launch_bounds__(256) void flush2d(Layout2D L, Fields2D F, Acq A, in
     }
 }
 
-// ABS of the velocity-type fields (b may be null: SH has one, vz)
+// ABS of the velocity-type fields (b may be null: SH has one, vz; TE-EM one, hz)
 __global__ void abs_velocity(long long n, float *a, float *b, const float *__restrict__ damp, long long plane)
 {
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
@@ -715,6 +843,11 @@ void launch_h(int physics, int which, dim3 grid, const Layout2D &L, const Fields
             go(sh2d_v<H, TY>, grid, block, st, e0, e1, L, F, C, T, P, A, tofs, pend);
         else
             go(sh2d_s<H, TY>, grid, block, st, e0, e1, L, F, C, T, P, A, tofs, pend);
+    } else if (physics == TEMEM) {
+        if (which == 0)
+            go(te2d_h<H, TY>, grid, block, st, e0, e1, L, F, C, T, P, A, tofs, pend);
+        else
+            go(te2d_e<H, TY>, grid, block, st, e0, e1, L, F, C, T, P, A, tofs, pend);
     } else {
         if (which == 0)
             go(tm2d_h<H, TY>, grid, block, st, e0, e1, L, F, C, T, P, A, tofs, pend);
@@ -763,13 +896,17 @@ void launch_flush2d(int physics, const Layout2D &L, const Fields2D &F, const Acq
         flush2d<ACOUSTIC><<<A.nshots, 256, 0, stream>>>(L, F, A, tofs);
     else if (physics == SH)
         flush2d<SH><<<A.nshots, 256, 0, stream>>>(L, F, A, tofs);
+    else if (physics == TEMEM)
+        flush2d<TEMEM><<<A.nshots, 256, 0, stream>>>(L, F, A, tofs);
 }
 
 void launch_abs_velocity2d(int physics, const Layout2D &L, int nshots, const Fields2D &F, const Coef2D &C,
                            hipStream_t stream)
 {
     const long long n = (long long)nshots * L.plane;
-    abs_velocity<<<1024, 256, 0, stream>>>(n, F.f[0], physics == SH ? nullptr : F.f[1], C.damp, L.plane);
+    // SH: vz; TE-EM: hz (phase 0's field; ex, ey are damped by phase 1)
+    abs_velocity<<<1024, 256, 0, stream>>>(n, F.f[0], (physics == SH || physics == TEMEM) ? nullptr : F.f[1], C.damp,
+                                           L.plane);
 }
 
 void launch_advance(int *tbase, int k, hipStream_t stream) { advance_kernel<<<1, 64, 0, stream>>>(tbase, k); }

@@ -35,7 +35,7 @@ struct StateBuffers {
     }
 };
 
-enum Physics { ACOUSTIC = 0, ELASTIC = 1, TMEM = 2, SH = 3 };
+enum Physics { ACOUSTIC = 0, ELASTIC = 1, TMEM = 2, SH = 3, TEMEM = 4 };
 
 // Maximum half-width of a derivative stencil window (order 12 -> 6).
 constexpr int MAX_H = 6;
@@ -110,6 +110,7 @@ struct Coef2D {
     // elastic : c0 = M, c1 = mu, c2 = 1/rho_x, c3 = 1/rho_y, c4 = mu_xy
     // tmem    : c0 = Ca, c1 = Cb, c2 = 1/mu_xz, c3 = 1/mu_yz
     // sh      : c0 = 1/rho, c1 = mu_xz, c2 = mu_yz
+    // emem    : c0 = 1/mu_xy, c1 = Ca_x, c2 = Cb_x, c3 = Ca_y, c4 = Cb_y
     const float *c[5];
     const float *damp;  // ABS damping factors (padded layout) or nullptr
     const float *fsH;   // [NXpad] free-surface horizontal scale at y = 0 (elastic) or nullptr

@@ -137,7 +137,9 @@ int main(int argc, char **argv)
             config.add(k, kv.substr(eq + 1), true);
         }
         const std::string dim = config.getString("dimension"), type = config.getString("equationType");
-        const bool em = type == "tmem" || type == "TMEM";
+        std::string tl = type;
+        for (auto &c : tl) c = (char)std::tolower((unsigned char)c);
+        const bool em = tl == "tmem" || tl == "emem";
         const int NX = (int)config.getInt("NX"), NY = (int)config.getInt("NY");
         const int NZ = (dim == "2D" || dim == "2d") ? 1 : (int)config.getInt("NZ");
         config.add("nz", std::to_string(NZ), true);
@@ -164,11 +166,10 @@ int main(int argc, char **argv)
                 m[q]->assign(n, (float)config.getDouble(em ? keys_em[q] : keys[q]));
             }
         }
-        // viscoelastic: relaxation mechanisms from the configuration, tauP / tauS from the
-        // model files or the configuration (Viscoelastic.cpp:160-200, 260-290)
-        std::string tl = type;
-        for (auto &c : tl) c = (char)std::tolower((unsigned char)c);
-        const bool visco = tl == "viscoelastic";
+        // viscoelastic / visco-SH: relaxation mechanisms from the configuration, tauP / tauS from
+        // the model files or the configuration (Viscoelastic.cpp:160-200, 260-290; visco-SH reads
+        // tauS only, ViscoSH.cpp)
+        const bool visco = tl == "viscoelastic" || tl == "viscosh";
         std::vector<double> freqs;
         std::vector<float> tauP, tauS;
         if (visco) {
@@ -178,10 +179,10 @@ int main(int argc, char **argv)
             for (long l = 0; l < nm; l++) freqs.push_back(l < 4 ? config.getDouble(fk[l]) : 0.0);
             if (config.getInt("ModelRead", 0) == 1) {
                 const std::string base = config.getString("ModelFilename");
-                tauP = readVectorMtx(base + ".tauP.mtx", n);
+                if (tl != "viscosh") tauP = readVectorMtx(base + ".tauP.mtx", n);
                 tauS = readVectorMtx(base + ".tauS.mtx", n);
             } else {
-                tauP.assign(n, (float)config.getDouble("tauP"));
+                if (tl != "viscosh") tauP.assign(n, (float)config.getDouble("tauP"));
                 tauS.assign(n, (float)config.getDouble("tauS"));
             }
         }

@@ -163,6 +163,33 @@ def tmem_c3(NX=800, NY=400, NT=3000, DT=3e-11, nshots=64, seed=3, block=25, W=20
     return c
 
 
+def emem_2d(NX=170, NY=97, NT=500, DT=3e-11, nshots=3, order=8, W=12, damping=2, seed=13, block=10):
+    """2D TE-EM (ForwardSolver2Demem.cpp): blocky epsilon_r (1-16), sigma (0-0.01 S/m) and mu_r
+    (1-3) so every average differs from the node values; per shot EX, EY and HZ sources (one or
+    two) at seeded positions and a receiver row of all three components, uneven counts."""
+    s = dict(NX=NX, NY=NY, NT=NT, DH=0.02, DT=DT, spatial_fd_order=order, damping_boundary=damping,
+             boundary_width=W, npower=4.0, fc_cpml=2e8, vmax_cpml=3e8, damping_coeff=8.0)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    nby, nbx = -(-NY // block), -(-NX // block)
+    blocks = [rng.uniform(lo, hi, (nby, nbx)) for lo, hi in ((1.0, 16.0), (0.0, 0.01), (1.0, 3.0))]
+    eps, sig, mur = (np.kron(b, np.ones((block, block)))[:NY, :NX].astype(np.float32).ravel() for b in blocks)
+    c = Case("emem_2d", "2D", "emem", s, eps, sig, mur)
+    lo = W + 4 if damping else 4
+    w = ricker(NT, DT, 2e8, 1.0)
+    kinds = ([3], [2, 4], [4])
+    for k in range(nshots):
+        ty = kinds[k % 3]
+        pos = [int(rng.integers(lo, NX - lo)) + int(rng.integers(lo, NY - lo)) * NX for _ in ty]
+        sigs = np.stack([w * (1.0 if j == 0 else 0.6) for j in range(len(ty))])
+        nr = int(rng.integers(10, 40))
+        rx = np.sort(rng.choice(np.arange(lo, NX - lo), nr, replace=False))
+        ry = lo + 2 * k
+        rec = np.concatenate([rx + ry * NX, (NX // 2) + np.arange(lo, NY - lo, 9) * NX])
+        rty = (np.arange(len(rec)) % 3 + 2).astype(np.int32)
+        c.shots.append(Shot(np.array(pos), np.array(ty), sigs, rec, rty))
+    return c
+
+
 def elastic_c4(N=600, NT=3000, DT=1e-4, nshots=16, seed=4, W=30, anomaly=True):
     NX = NY = N
     s = dict(NX=NX, NY=NY, NT=NT, DH=1.0, DT=DT, spatial_fd_order=8, damping_boundary=2, boundary_width=W,

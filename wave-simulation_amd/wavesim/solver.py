@@ -28,6 +28,7 @@ class ForwardSolver:
         ("2D", "elastic"): ("vx", "vy", "sxx", "syy", "sxy"),
         ("2D", "viscoelastic"): ("vx", "vy", "sxx", "syy", "sxy"),
         ("2D", "tmem"): ("hx", "hy", "ez"),
+        ("2D", "emem"): ("hz", "ex", "ey"),
         ("2D", "sh"): ("vz", "sxz", "syz"),
         ("2D", "viscosh"): ("vz", "sxz", "syz"),
         ("3D", "elastic"): ("vx", "vy", "vz", "sxx", "syy", "szz", "syz", "sxz", "sxy"),
