@@ -79,9 +79,12 @@ typedef struct {
 } wv_config;
 
 /* dimension "2D"/"3D", type (case-insensitive): 2D "acoustic", "elastic", "viscoelastic", "sh",
- * "viscosh", "tmem", "emem" (TE mode: fields hz, ex, ey; EX / EY / HZ sources and receivers,
- * types 2 / 3 / 4); 3D "acoustic", "elastic".  The factory's other types ("viscotmem",
- * "viscoemem", 3D others) return WV_EINVAL ("not provided"), unknown ones WV_EINVAL. */
+ * "viscosh", "tmem", "viscotmem", "emem" / "viscoemem" (TE mode: fields hz, ex, ey; EX / EY / HZ
+ * sources and receivers, types 2 / 3 / 4); 3D "acoustic", "elastic", "viscoelastic", "emem" /
+ * "viscoemem" (fields hx, hy, hz, ex, ey, ez; EZ / EX / EY / HZ sources and receivers, types
+ * 1..4), as ForwardSolverFactory.cpp:4-66 creates them.  The combinations the reference factory
+ * answers with an empty pointer (3D "tmem", "sh", "viscosh", "viscotmem") return WV_EINVAL
+ * ("not provided"), unknown dimensions and types WV_EINVAL. */
 int wv_create(const char *dimension, const char *type, wv_solver **out);
 int wv_destroy(wv_solver *s);
 
@@ -92,16 +95,24 @@ int wv_init(wv_solver *s, const wv_config *cfg);
  *   elastic : m0 = velocityP, m1 = velocityS, m2 = density
  *   viscoelastic: as elastic, after wv_set_relaxation
  *   sh      : m0 ignored (may be NULL), m1 = velocityS, m2 = density
- *   tmem    : m0 = epsilon_r, m1 = sigma [S/m], m2 = mu_r                  */
+ *   tmem, emem: m0 = epsilon_r, m1 = sigma [S/m], m2 = mu_r
+ *   viscotmem, viscoemem: m0 = static epsilon_r, m1 = static sigma [S/m], m2 = mu_r (what
+ *             Modelparameter::getDielectricPermittivity / getElectricConductivity hold, over
+ *             eps0), after wv_set_relaxation                                           */
 int wv_set_model(wv_solver *s, const float *m0, const float *m1, const float *m2);
 
-/* Relaxation mechanisms of the 2D viscoelastic solvers ("viscoelastic", "viscosh"; tauP unused
- * and may be NULL for visco-SH, ViscoSH.cpp:450-575): nmech
+/* Relaxation mechanisms of the viscoelastic solvers ("viscoelastic" 2D / 3D, "viscosh"; tauP
+ * unused and may be NULL for visco-SH, ViscoSH.cpp:450-575) and of the visco-EM solvers
+ * ("viscotmem", "viscoemem": tauP = tau of the dielectric permittivity (relative), tauS = tau of
+ * the electric conductivity [s], as Modelparameter::getTauDielectricPermittivity /
+ * getTauElectricConductivity hold them; ForwardSolver2Dviscotmem.cpp:77-100,
+ * ForwardSolverEM.cpp:70-157 derive Ca, Cb, Cc, Cd from them): nmech
  * (1..4) relaxation frequencies [Hz] and tauP, tauS per cell [NX*NY] (grid order), as
  * Modelparameter::Viscoelastic holds them (numRelaxationMechanisms, relaxationFrequency..4,
  * the .tauP / .tauS model files; Viscoelastic.cpp:160-260, 585-596) and
- * FD2Dviscoelastic::initForwardSolver derives its constants from them
- * (ForwardSolver2Dviscoelastic.cpp:40-60).  Call after wv_init and before wv_set_model: the
+ * FD2Dviscoelastic / FD3Dviscoelastic::initForwardSolver derive their constants from them
+ * (ForwardSolver2Dviscoelastic.cpp:40-60, ForwardSolver3Dviscoelastic.cpp:50-66).  Call after
+ * wv_init and before wv_set_model: the
  * moduli velocityP / velocityS given to wv_set_model are taken at the reference frequency
  * CenterFrequencyCPML (wv_config.fc_cpml) and scaled to the relaxed ones
  * (Viscoelastic.cpp:650-700).  WV_EINVAL for other solvers or bad arguments. */

@@ -22,7 +22,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 PHYSICS = {"acoustic": 0, "elastic": 1, "tmem": 2, "sh": 3, "viscoelastic": 4, "visco": 4, "viscosh": 5,
-           "emem": 6}
+           "emem": 6, "viscotmem": 7, "viscoemem": 8}
 
 
 class Params(C.Structure):
@@ -88,8 +88,10 @@ def make_params(**kw) -> Params:
 
 def run(params: Params, m0, m1, m2, src_idx, src_type, src_sig, rec_idx, rec_type, precision="f32", want_fields=False,
         t_end=-1, visco=None):
-    """Run one shot; returns (seis [nrec, NT], fields or None).  Viscoelastic (physics 4) needs
-    visco = dict(freqs=[relaxation frequencies, one per mechanism], tauP=..., tauS=... per cell)."""
+    """Run one shot; returns (seis [nrec, NT], fields or None).  Viscoelastic, visco-SH and visco-EM
+    (physics 4, 5, 7, 8) need visco = dict(freqs=[relaxation frequencies, one per mechanism],
+    tauP=..., tauS=... per cell); visco-EM: tauP = tau of the dielectric permittivity, tauS = tau of
+    the electric conductivity [s]."""
     L = lib(precision)
     dt = dtype_of(precision)
     real = C.c_float if precision == "f32" else C.c_double
@@ -103,9 +105,11 @@ def run(params: Params, m0, m1, m2, src_idx, src_type, src_sig, rec_idx, rec_typ
     rec_idx = np.ascontiguousarray(rec_idx, dtype=np.int64)
     rec_type = np.ascontiguousarray(rec_type, dtype=np.int32)
     seis = np.zeros((len(rec_idx), params.NT), dtype=dt)
-    nf = {0: 3 if params.dim == 2 else 4, 1: 5 if params.dim == 2 else 9, 2: 3, 3: 3, 4: 5, 5: 3, 6: 3}[params.physics]
+    nf = {0: 3 if params.dim == 2 else 4, 1: 5 if params.dim == 2 else 9, 2: 3, 3: 3,
+          4: 5 if params.dim == 2 else 9, 5: 3, 6: 3 if params.dim == 2 else 6, 7: 3,
+          8: 3 if params.dim == 2 else 6}[params.physics]
     fields = np.zeros((nf, n), dtype=dt) if want_fields else None
-    if params.physics in (4, 5):
+    if params.physics in (4, 5, 7, 8):
         freqs = np.ascontiguousarray(visco["freqs"], dtype=np.float64)
         tp = np.ascontiguousarray(np.broadcast_to(visco["tauP"], (n,)), dtype=dt)
         ts = np.ascontiguousarray(np.broadcast_to(visco["tauS"], (n,)), dtype=dt)

@@ -624,10 +624,22 @@ typedef struct {
     double damping_coeff, npower, fc_cpml, vmax_cpml;
 } oracle_params;
 
-enum { PH_ACOUSTIC = 0, PH_ELASTIC = 1, PH_TMEM = 2, PH_SH = 3, PH_VISCO = 4, PH_VISCOSH = 5, PH_TEMEM = 6 };
+enum {
+    PH_ACOUSTIC = 0,
+    PH_ELASTIC = 1,
+    PH_TMEM = 2,
+    PH_SH = 3,
+    PH_VISCO = 4,
+    PH_VISCOSH = 5,
+    PH_TEMEM = 6,
+    PH_VISCOTMEM = 7,
+    PH_VISCOTEMEM = 8
+};
 
-/* Viscoelastic parameters of the next oracle_run (2D viscoelastic and visco-SH): L relaxation
- * mechanisms with their relaxation frequencies [Hz], tauP and tauS per cell (grid order).
+/* Viscoelastic parameters of the next oracle_run (viscoelastic, visco-SH, visco-EM): L relaxation
+ * mechanisms with their relaxation frequencies [Hz], tauP and tauS per cell (grid order; visco-EM:
+ * tauP = tau of the dielectric permittivity (relative), tauS = tau of the electric conductivity
+ * [s], as Modelparameter::getTauDielectricPermittivity / getTauElectricConductivity hold them).
  * Held by pointer: the caller keeps the arrays alive during the run. */
 static int g_visco_L = 0;
 static double g_visco_f[4];
@@ -640,6 +652,54 @@ int oracle_set_visco(int L, const double *freqs, const real *tauP, const real *t
     g_tauP = tauP;
     g_tauS = tauS;
     return 0;
+}
+
+/* Visco-EM coefficients (ForwardSolverEM.cpp:70-157 as ForwardSolver2Dviscotmem.cpp:77-100 and
+ * ForwardSolver2Dviscoemem.cpp:62-104 call them) from the static permittivity eps_s, conductivity
+ * sig_s and the two tau of each cell (node or averaged values):
+ *   eps_opt = (1 - tauE) eps_s + sig_s tauS, raised to eps0 where smaller (searchAndReplace type 1)
+ *   sig_opt = tauE (sum_l 1/tau_l) / L eps_s + sig_s
+ *   Ca, Cb of (eps_opt, sig_opt) as for TM-EM; Cd_l = -DT eps_s tauE / (L tau_l^2) / (1 + DT/(2 tau_l))
+ * in the value type, operation by operation. */
+static void em_visco_coefs(long n, const real *eps_s, const real *sig_s, const real *tE, const real *tS, real DT,
+                           int L, const real *tau, real *Ca, real *Cb, real **Cd)
+{
+    real eps0 = (real)8.8541878176e-12;
+    real sum = 0;
+    for (int l = 0; l < L; l++) sum = (real)((double)sum + 1.0 / (double)tau[l]);
+    sum /= (real)L;
+    for (long i = 0; i < n; i++) {
+        real e = (real)1 - tE[i];
+        e *= eps_s[i];
+        real t = sig_s[i] * tS[i];
+        e += t;
+        if (e < eps0) e = eps0;
+        real sg = tE[i] * sum;
+        sg *= eps_s[i];
+        sg += sig_s[i];
+        real c = (real)0.5 / e;
+        c *= sg;
+        c *= DT;
+        real cinv = (real)1 / (c + (real)1);
+        real ca = (real)1 - c;
+        ca *= cinv;
+        real cb = (real)1 / e;
+        cb *= cinv;
+        Ca[i] = ca;
+        Cb[i] = cb;
+    }
+    for (int l = 0; l < L; l++) {
+        real tv = (real)(1.0 / (1.0 + 0.5 * (double)DT / (double)tau[l]));
+        real den = (real)L * tau[l];
+        den *= tau[l];
+        tv /= den;
+        for (long i = 0; i < n; i++) {
+            real t = tE[i] * tv;
+            t *= eps_s[i];
+            t *= -DT;
+            Cd[l][i] = t;
+        }
+    }
 }
 
 typedef struct {
@@ -791,9 +851,17 @@ int oracle_run(const oracle_params *p, const real *m0, const real *m1, const rea
 #ifdef _OPENMP
     if (p->nthreads > 0) omp_set_num_threads(p->nthreads);
 #endif
-    if (p->dim == 3 && (p->physics == PH_TMEM || p->physics == PH_SH || p->physics == PH_VISCO ||
-                        p->physics == PH_VISCOSH || p->physics == PH_TEMEM))
-        return -2;
+    /* visco-EM runs the TM / TE loop with memory variables of the electric field */
+    oracle_params pv;
+    int emv = 0;
+    if (p->physics == PH_VISCOTMEM || p->physics == PH_VISCOTEMEM) {
+        if (g_visco_L < 1 || !g_tauP || !g_tauS) return -3;
+        pv = *p;
+        pv.physics = p->physics == PH_VISCOTMEM ? PH_TMEM : PH_TEMEM;
+        p = &pv;
+        emv = 1;
+    }
+    if (p->dim == 3 && (p->physics == PH_TMEM || p->physics == PH_SH || p->physics == PH_VISCOSH)) return -2;
     if ((p->physics == PH_VISCO || p->physics == PH_VISCOSH) && (g_visco_L < 1 || !g_tauP || !g_tauS)) return -3;
     grid_t g = {p->NX, p->NY, p->NZ, (long)p->NX * p->NY * p->NZ};
     long n = g.n;
@@ -809,11 +877,32 @@ int oracle_run(const oracle_params *p, const real *m0, const real *m1, const rea
     real *M = NULL, *mu = NULL, *irx = NULL, *iry = NULL, *irz = NULL, *muxy = NULL, *muxz = NULL, *muyz = NULL;
     real *Ca = NULL, *Cb = NULL, *imxz = NULL, *imyz = NULL;
     real *CaY = NULL, *CbY = NULL, *imxy = NULL; /* emem: Ca, Cb above are the x-averaged ones */
+    real *CaZ = NULL, *CbZ = NULL;               /* 3D emem: the z-averaged ones */
     /* viscoelastic: relaxed moduli, (1 + L tau), averaged tauS, per-mechanism constants */
     int VL = (p->physics == PH_VISCO || p->physics == PH_VISCOSH) ? g_visco_L : 0;
     real *tauSxz = NULL, *tauSyz = NULL;
-    real *opP = NULL, *opS = NULL, *tauSxy = NULL, *Rv[12] = {0};
+    real *opP = NULL, *opS = NULL, *tauSxy = NULL, *Rv[24] = {0};
     real vtau[4] = {0}, vitau[4] = {0}, vc1[4] = {0}, vc2[4] = {0}, dth = 0;
+    /* visco-EM: Cc_l, Cd_l (x-averaged or node), Cd_l (y-averaged), memory variables r (x / z) and
+     * r (y) of each mechanism */
+    int EL = emv ? g_visco_L : 0;
+    real ecc[4] = {0}, *eCd[4] = {0}, *eCdY[4] = {0}, *eR[4] = {0}, *eRY[4] = {0}, *eCdZ[4] = {0}, *eRZ[4] = {0};
+    if (emv)
+        for (int l = 0; l < EL; l++) {
+            real tau = (real)(1.0 / (2.0 * M_PI * (double)(real)g_visco_f[l]));
+            vtau[l] = tau;
+            ecc[l] = (real)((1.0 - 0.5 * (double)DT / (double)tau) / (1.0 + 0.5 * (double)DT / (double)tau));
+            eCd[l] = vnew(n);
+            eR[l] = vnew(n);
+            if (p->physics == PH_TEMEM) {
+                eCdY[l] = vnew(n);
+                eRY[l] = vnew(n);
+            }
+            if (p->physics == PH_TEMEM && p->dim == 3) {
+                eCdZ[l] = vnew(n);
+                eRZ[l] = vnew(n);
+            }
+        }
     if (p->physics == PH_ACOUSTIC || p->physics == PH_ELASTIC || p->physics == PH_VISCO) {
         M = vnew(n);
         for (long i = 0; i < n; i++) M[i] = m2[i] * m0[i] * m0[i];
@@ -847,12 +936,26 @@ int oracle_run(const oracle_params *p, const real *m0, const real *m1, const rea
             }
             muxy = vnew(n);
             shear_average(&g, mu, 0, 1, muxy); /* mu < 1 -> 1 in place, as the reference */
-            /* ModelparameterSeismic averaging: tauS with the 4-point matrix of mu_xy */
+            /* Viscoelastic.cpp:562-576: tauS with the 4-point matrix of each shear modulus */
             tauSxy = vnew(n);
             for (long i = 0; i < n; i++) {
                 long x, y, z;
                 gcoord(&g, i, &x, &y, &z);
                 tauSxy[i] = avg4_row(&g, g_tauS, x, y, z, 0, 1);
+            }
+            if (p->dim == 3) {
+                muxz = vnew(n);
+                muyz = vnew(n);
+                shear_average(&g, mu, 0, 2, muxz);
+                shear_average(&g, mu, 1, 2, muyz);
+                tauSxz = vnew(n);
+                tauSyz = vnew(n);
+                for (long i = 0; i < n; i++) {
+                    long x, y, z;
+                    gcoord(&g, i, &x, &y, &z);
+                    tauSxz[i] = avg4_row(&g, g_tauS, x, y, z, 0, 2);
+                    tauSyz[i] = avg4_row(&g, g_tauS, x, y, z, 1, 2);
+                }
             }
             /* ForwardSolver2Dviscoelastic.cpp:40-60, 85-100 */
             opP = vnew(n);
@@ -870,7 +973,8 @@ int oracle_run(const oracle_params *p, const real *m0, const real *m1, const rea
                 vc2[l] = (real)(1.0 / (1.0 + (double)DT / (2.0 * (double)vtau[l])));
             }
             dth = (real)((double)DT / 2.0);
-            for (int k = 0; k < 3 * VL; k++) Rv[k] = vnew(n); /* Rxx[l], Ryy[l], Rxy[l] */
+            /* 2D: Rxx[l], Ryy[l], Rxy[l]; 3D: Rxx, Ryy, Rzz, Ryz, Rxz, Rxy */
+            for (int k = 0; k < (p->dim == 3 ? 6 : 3) * VL; k++) Rv[k] = vnew(n);
         }
         if (p->physics == PH_ELASTIC) {
             mu = vnew(n);
@@ -935,6 +1039,79 @@ int oracle_run(const oracle_params *p, const real *m0, const real *m1, const rea
         muyz = vnew(n);
         shear_average_2pt(&g, mu, 0, muxz);
         shear_average_2pt(&g, mu, 1, muyz);
+    } else if (p->physics == PH_TEMEM && p->dim == 3) {
+        /* 3D EM (EMEM.cpp:378-395): 1/mu with the 4-point yz, xz, xy averages (invalid -> 0);
+         * epsilon, sigma (visco-EM also both tau: ViscoEMEM.cpp:409-435) with the 2-point averages
+         * along x, y, z; ForwardSolver3Demem.cpp:85-100 / ForwardSolver3Dviscoemem.cpp:78-135: Ca, Cb
+         * (and Cd) of each axis' averages */
+        real eps0 = (real)8.8541878176e-12, mu0 = (real)1.2566370614e-6;
+        real *eps = vnew(n), *mup = vnew(n);
+        for (long i = 0; i < n; i++) {
+            eps[i] = m0[i] * eps0;
+            mup[i] = m2[i] * mu0;
+        }
+        imyz = vnew(n);
+        imxz = vnew(n);
+        imxy = vnew(n);
+        real *im[3] = {imyz, imxz, imxy};
+        const int pl[3][2] = {{1, 2}, {0, 2}, {0, 1}};
+        for (long i = 0; i < n; i++) {
+            long x, y, z;
+            gcoord(&g, i, &x, &y, &z);
+            for (int k = 0; k < 3; k++) {
+                real v = (real)1 / avg4_row(&g, mup, x, y, z, pl[k][0], pl[k][1]);
+                if (isnan(v) || isinf(v)) v = 0;
+                im[k][i] = v;
+            }
+        }
+        Ca = vnew(n);
+        Cb = vnew(n);
+        CaY = vnew(n);
+        CbY = vnew(n);
+        CaZ = vnew(n);
+        CbZ = vnew(n);
+        real *CA[3] = {Ca, CaY, CaZ}, *CB[3] = {Cb, CbY, CbZ}, **CD[3] = {eCd, eCdY, eCdZ};
+        const real *src[4] = {eps, m1, g_tauP, g_tauS};
+        int nq = emv ? 4 : 2;
+        real *av[4];
+        for (int q = 0; q < nq; q++) av[q] = vnew(n);
+        for (int ax = 0; ax < 3; ax++) {
+            for (long i = 0; i < n; i++) {
+                long x, y, z;
+                gcoord(&g, i, &x, &y, &z);
+                long L = axis_len(&g, ax), q = axis_pos(x, y, z, ax);
+                real h = (real)0.5;
+                for (int k = 0; k < nq; k++) {
+                    if (q + 1 < L) {
+                        long j = axis_idx(&g, x, y, z, ax, q + 1);
+                        real v = h * src[k][i];
+                        v += h * src[k][j];
+                        av[k][i] = v;
+                    } else {
+                        av[k][i] = (real)1.0 * src[k][i];
+                    }
+                }
+            }
+            if (emv) {
+                em_visco_coefs(n, av[0], av[1], av[2], av[3], DT, EL, vtau, CA[ax], CB[ax], CD[ax]);
+            } else {
+                for (long i = 0; i < n; i++) {
+                    real c = (real)0.5 / av[0][i];
+                    c *= av[1][i];
+                    c *= DT;
+                    real cinv = (real)1 / (c + (real)1);
+                    real ca = (real)1 - c;
+                    ca *= cinv;
+                    real cb = (real)1 / av[0][i];
+                    cb *= cinv;
+                    CA[ax][i] = ca;
+                    CB[ax][i] = cb;
+                }
+            }
+        }
+        for (int q = 0; q < nq; q++) free(av[q]);
+        free(eps);
+        free(mup);
     } else if (p->physics == PH_TEMEM) {
         /* EMEM.cpp:374-395: 1/mu with the 4-point xy average (calcInverseAveragedParameter,
          * Modelparameter.cpp:633-639, invalid -> 0); epsilon and sigma with the 2-point averages
@@ -951,7 +1128,44 @@ int oracle_run(const oracle_params *p, const real *m0, const real *m1, const rea
         Cb = vnew(n);
         CaY = vnew(n);
         CbY = vnew(n);
-        for (long i = 0; i < n; i++) {
+        if (emv) {
+            /* ViscoEMEM.cpp:409-435: static epsilon, sigma and both tau with the 2-point averages
+             * along x and y; ForwardSolver2Dviscoemem.cpp:62-104: the coefficients of those */
+            const real *src[4] = {eps, m1, g_tauP, g_tauS};
+            real *av[2][4];
+            for (int ax = 0; ax < 2; ax++)
+                for (int q = 0; q < 4; q++) av[ax][q] = vnew(n);
+            for (long i = 0; i < n; i++) {
+                long x, y, z;
+                gcoord(&g, i, &x, &y, &z);
+                for (int ax = 0; ax < 2; ax++) {
+                    long L = ax == 0 ? g.NX : g.NY, q = ax == 0 ? x : y;
+                    real h = (real)0.5;
+                    for (int k = 0; k < 4; k++) {
+                        if (q + 1 < L) {
+                            long j = ax == 0 ? gidx(&g, x + 1, y, z) : gidx(&g, x, y + 1, z);
+                            real v = h * src[k][i];
+                            v += h * src[k][j];
+                            av[ax][k][i] = v;
+                        } else {
+                            av[ax][k][i] = (real)1.0 * src[k][i];
+                        }
+                    }
+                }
+            }
+            em_visco_coefs(n, av[0][0], av[0][1], av[0][2], av[0][3], DT, EL, vtau, Ca, Cb, eCd);
+            em_visco_coefs(n, av[1][0], av[1][1], av[1][2], av[1][3], DT, EL, vtau, CaY, CbY, eCdY);
+            for (long i = 0; i < n; i++) {
+                long x, y, z;
+                gcoord(&g, i, &x, &y, &z);
+                real v = (real)1 / avg4_row(&g, mup, x, y, z, 0, 1);
+                if (isnan(v) || isinf(v)) v = 0;
+                imxy[i] = v;
+            }
+            for (int ax = 0; ax < 2; ax++)
+                for (int q = 0; q < 4; q++) free(av[ax][q]);
+        }
+        for (long i = 0; i < n && !emv; i++) {
             long x, y, z;
             gcoord(&g, i, &x, &y, &z);
             real v = (real)1 / avg4_row(&g, mup, x, y, z, 0, 1);
@@ -997,7 +1211,9 @@ int oracle_run(const oracle_params *p, const real *m0, const real *m1, const rea
         inverse_average_2pt(&g, 1, mup, imyz);
         Ca = vnew(n);
         Cb = vnew(n);
-        for (long i = 0; i < n; i++) {
+        if (emv) /* ForwardSolver2Dviscotmem.cpp:77-100: node values (no averaging for ez) */
+            em_visco_coefs(n, eps, m1, g_tauP, g_tauS, DT, EL, vtau, Ca, Cb, eCd);
+        for (long i = 0; i < n && !emv; i++) {
             /* ForwardSolverEM.cpp:14-68 */
             real c = (real)0.5 / eps[i];
             c *= m1[i];
@@ -1089,8 +1305,9 @@ int oracle_run(const oracle_params *p, const real *m0, const real *m1, const rea
     if (p->damping == 1) damp = abs_layout(&g, p->dim, p->width, (real)p->damping_coeff, p->free_surface);
 
     /* wavefields and scratch */
-    int nf = p->physics == PH_ELASTIC ? (p->dim == 3 ? 9 : 5)
-                                      : (p->physics == PH_VISCO ? 5 : (p->dim == 3 && p->physics == PH_ACOUSTIC ? 4 : 3));
+    int nf = (p->physics == PH_ELASTIC || p->physics == PH_VISCO)
+                 ? (p->dim == 3 ? 9 : 5)
+                 : (p->dim == 3 && p->physics == PH_ACOUSTIC ? 4 : (p->dim == 3 && p->physics == PH_TEMEM ? 6 : 3));
     real *F[9];
     for (int k = 0; k < nf; k++) F[k] = vnew(n);
     real *upd = vnew(n), *tmp = vnew(n), *vxx = vnew(n), *vyy = vnew(n), *vzz = vnew(n);
@@ -1315,8 +1532,9 @@ int oracle_run(const oracle_params *p, const real *m0, const real *m1, const rea
                     v_mul(n, vy, damp);
                 }
             }
-        } else if (p->physics == PH_ELASTIC && p->dim == 3) {
-            /* ForwardSolver3Delastic.cpp:120-416 ; F: vx vy vz sxx syy szz syz sxz sxy */
+        } else if ((p->physics == PH_ELASTIC || p->physics == PH_VISCO) && p->dim == 3) {
+            /* ForwardSolver3Delastic.cpp:120-416 ; F: vx vy vz sxx syy szz syz sxz sxy
+             * (ForwardSolver3Dviscoelastic.cpp:185-268: the same velocity update) */
             real *vx = F[0], *vy = F[1], *vz = F[2], *sxx = F[3], *syy = F[4], *szz = F[5], *syz = F[6], *sxz = F[7],
                  *sxy = F[8];
             spmv(&o.Dxf, sxx, upd);
@@ -1359,6 +1577,121 @@ int oracle_run(const oracle_params *p, const real *m0, const real *m1, const rea
                 cpml_apply(n, vxx, psi[9], PLANT(9, ax, axh), PLANT(9, bx, bxh));
                 cpml_apply(n, vyy, psi[10], PLANT(10, ay, ayh), PLANT(10, by, byh));
                 cpml_apply(n, vzz, psi[11], PLANT(11, az, azh), PLANT(11, bz, bzh));
+            }
+            if (p->physics == PH_VISCO) {
+                /* ForwardSolver3Dviscoelastic.cpp:270-444: memory variables of the six stresses
+                 * for each mechanism, in the reference's order of operations */
+                real **Rxx = Rv, **Ryy = Rv + VL, **Rzz = Rv + 2 * VL, **Ryz = Rv + 3 * VL, **Rxz = Rv + 4 * VL,
+                     **Rxy = Rv + 5 * VL;
+                v_copy(n, upd, vxx);
+                v_add(n, upd, vyy);
+                v_add(n, upd, vzz);
+                v_mul(n, upd, M);
+                for (int l = 0; l < VL; l++)
+                    VLOOP(n)
+                    {
+                        real u2 = vitau[l] * upd[i];
+                        u2 *= g_tauP[i];
+                        sxx[i] += dth * Rxx[l][i];
+                        Rxx[l][i] *= vc1[l];
+                        Rxx[l][i] -= u2;
+                        syy[i] += dth * Ryy[l][i];
+                        Ryy[l][i] *= vc1[l];
+                        Ryy[l][i] -= u2;
+                        szz[i] += dth * Rzz[l][i];
+                        Rzz[l][i] *= vc1[l];
+                        Rzz[l][i] -= u2;
+                    }
+                v_mul(n, upd, opP);
+                v_add(n, sxx, upd);
+                v_add(n, syy, upd);
+                v_add(n, szz, upd);
+                /* the deviatoric parts: sxx from vyy + vzz, syy from vxx + vzz, szz from vxx + vyy */
+                real *nrm[3] = {sxx, syy, szz}, **Rn[3] = {Rxx, Ryy, Rzz};
+                real *da[3] = {vyy, vxx, vxx}, *db[3] = {vzz, vzz, vyy};
+                for (int c = 0; c < 3; c++) {
+                    real *sc = nrm[c], **Rc = Rn[c], *pa = da[c], *pb = db[c];
+                    VLOOP(n)
+                    {
+                        real u = pa[i] + pb[i];
+                        u *= mu[i];
+                        upd[i] = u * (real)2.0;
+                    }
+                    for (int l = 0; l < VL; l++)
+                        VLOOP(n)
+                        {
+                            real u2 = vitau[l] * upd[i];
+                            u2 *= g_tauS[i];
+                            Rc[l][i] += u2;
+                            Rc[l][i] *= vc2[l];
+                            sc[i] += dth * Rc[l][i];
+                        }
+                    v_mul(n, upd, opS);
+                    v_sub(n, sc, upd);
+                }
+                /* shear stresses: xy (psi 12, 13), xz (14, 15), yz (16, 17) */
+                const csr_t *D1[3] = {&o.Dyf, &o.Dzf, &o.Dzf}, *D2[3] = {&o.Dxf, &o.Dxf, &o.Dyf};
+                real *u1[3] = {vx, vx, vy}, *u2v[3] = {vy, vz, vz};
+                real *a1[3] = {ayh, azh, azh}, *b1[3] = {byh, bzh, bzh}, *a2[3] = {axh, axh, ayh},
+                     *b2[3] = {bxh, bxh, byh};
+                real *shr[3] = {sxy, sxz, syz}, *mus[3] = {muxy, muxz, muyz}, *tau3[3] = {tauSxy, tauSxz, tauSyz};
+                real **Rs[3] = {Rxy, Rxz, Ryz};
+                for (int c = 0; c < 3; c++) {
+                    spmv(D1[c], u1[c], upd);
+                    if (use_cpml) cpml_apply(n, upd, psi[12 + 2 * c], a1[c], b1[c]);
+                    spmv(D2[c], u2v[c], tmp);
+                    if (use_cpml) cpml_apply(n, tmp, psi[13 + 2 * c], a2[c], b2[c]);
+                    v_add(n, upd, tmp);
+                    v_mul(n, upd, mus[c]);
+                    real *sc = shr[c], **Rc = Rs[c], *tc = tau3[c];
+                    for (int l = 0; l < VL; l++)
+                        VLOOP(n)
+                        {
+                            sc[i] += dth * Rc[l][i];
+                            Rc[l][i] *= vc1[l];
+                            real g2 = vitau[l] * upd[i];
+                            g2 *= tc[i];
+                            Rc[l][i] -= g2;
+                            Rc[l][i] *= vc2[l];
+                            sc[i] += dth * Rc[l][i];
+                        }
+                    v_mul(n, upd, opS);
+                    v_add(n, sc, upd);
+                }
+                if (fs) {
+                    /* FreeSurface3Dviscoelastic.cpp:17-75 on the y = 0 plane, then syy and the Ryy
+                     * of every mechanism zero there */
+                    for (long i = 0; i < g.NX * g.NZ; i++) {
+                        real hsum = vxx[i] + vzz[i];
+                        for (int l = 0; l < VL; l++) {
+                            sxx[i] -= dth * Rxx[l][i];
+                            szz[i] -= dth * Rzz[l][i];
+                        }
+                        real a = fsH[i] * hsum;
+                        sxx[i] += a;
+                        szz[i] += a;
+                        real b = fsV[i] * vyy[i];
+                        sxx[i] -= b;
+                        szz[i] -= b;
+                        for (int l = 0; l < VL; l++) {
+                            real c = fsRH[l][i] * hsum;
+                            Rxx[l][i] += c;
+                            Rzz[l][i] += c;
+                            real d = fsRV[l][i] * vyy[i];
+                            Rxx[l][i] -= d;
+                            Rzz[l][i] -= d;
+                            sxx[i] += dth * Rxx[l][i];
+                            szz[i] += dth * Rzz[l][i];
+                        }
+                    }
+                    v_mul(n, syy, fsZero);
+                    for (int l = 0; l < VL; l++) v_mul(n, Ryy[l], fsZero);
+                }
+                if (damp) {
+                    real *all[9] = {sxx, syy, szz, sxy, sxz, syz, vx, vy, vz};
+                    for (int k = 0; k < 9; k++) v_mul(n, all[k], damp);
+                }
+                goto step_done_3d;
             }
             v_copy(n, upd, vxx);
             v_add(n, upd, vyy);
@@ -1418,6 +1751,7 @@ int oracle_run(const oracle_params *p, const real *m0, const real *m1, const rea
                 real *all[9] = {sxx, syy, szz, sxy, sxz, syz, vx, vy, vz};
                 for (int k = 0; k < 9; k++) v_mul(n, all[k], damp);
             }
+        step_done_3d:;
         } else if (p->physics == PH_VISCOSH) {
             /* ForwardSolver2Dviscosh.cpp:110-190 ; F: vz sxz syz; the SH velocity update, then
              * the stresses with the memory variables Rxz, Ryz of each mechanism */
@@ -1494,6 +1828,80 @@ int oracle_run(const oracle_params *p, const real *m0, const real *m1, const rea
                 v_mul(n, syz, damp);
                 v_mul(n, vz, damp);
             }
+        } else if (p->physics == PH_TEMEM && p->dim == 3) {
+            /* ForwardSolver3Demem.cpp:117-235 (ForwardSolver3Dviscoemem.cpp:170-305 with the memory
+             * variables); F: hx hy hz ex ey ez.  CPMLEM3D.cpp:25-104: the E derivatives on the half
+             * profiles, the H derivatives on the full ones except hyx, which takes the half x profile
+             * (CPMLEM3D.cpp:67-70).  psi: 0 ezy 1 eyz 2 exz 3 ezx 4 eyx 5 exy 6 hzy 7 hyz 8 hxz 9 hzx
+             * 10 hyx 11 hxy */
+            real *hx = F[0], *hy = F[1], *hz = F[2], *ex = F[3], *ey = F[4], *ez = F[5];
+            spmv(&o.Dyf, ez, upd);
+            spmv(&o.Dzf, ey, tmp);
+            if (use_cpml) {
+                cpml_apply(n, upd, psi[0], PLANT(0, ayh, ay), PLANT(0, byh, by));
+                cpml_apply(n, tmp, psi[1], PLANT(1, azh, az), PLANT(1, bzh, bz));
+            }
+            v_sub(n, upd, tmp);
+            v_mul(n, upd, imyz);
+            v_sub(n, hx, upd);
+            spmv(&o.Dzf, ex, upd);
+            spmv(&o.Dxf, ez, tmp);
+            if (use_cpml) {
+                cpml_apply(n, upd, psi[2], PLANT(2, azh, az), PLANT(2, bzh, bz));
+                cpml_apply(n, tmp, psi[3], PLANT(3, axh, ax), PLANT(3, bxh, bx));
+            }
+            v_sub(n, upd, tmp);
+            v_mul(n, upd, imxz);
+            v_sub(n, hy, upd);
+            spmv(&o.Dxf, ey, upd);
+            spmv(&o.Dyf, ex, tmp);
+            if (use_cpml) {
+                cpml_apply(n, upd, psi[4], PLANT(4, axh, ax), PLANT(4, bxh, bx));
+                cpml_apply(n, tmp, psi[5], PLANT(5, ayh, ay), PLANT(5, byh, by));
+            }
+            v_sub(n, upd, tmp);
+            v_mul(n, upd, imxy);
+            v_sub(n, hz, upd);
+            for (int l = 0; l < EL; l++)
+                VLOOP(n)
+                {
+                    real a = ecc[l] * eR[l][i];
+                    real b = eCd[l][i] * ex[i];
+                    eR[l][i] = b + a;
+                    a = ecc[l] * eRY[l][i];
+                    b = eCdY[l][i] * ey[i];
+                    eRY[l][i] = b + a;
+                    a = ecc[l] * eRZ[l][i];
+                    b = eCdZ[l][i] * ez[i];
+                    eRZ[l][i] = b + a;
+                }
+            real *eo[3] = {ex, ey, ez}, **er[3] = {eR, eRY, eRZ};
+            real *ca3[3] = {Ca, CaY, CaZ}, *cb3[3] = {Cb, CbY, CbZ};
+            /* ex: Dyb hz - Dzb hy ; ey: Dzb hx - Dxb hz ; ez: Dxb hy - Dyb hx */
+            const csr_t *D1[3] = {&o.Dyb, &o.Dzb, &o.Dxb}, *D2[3] = {&o.Dzb, &o.Dxb, &o.Dyb};
+            real *h1[3] = {hz, hx, hy}, *h2[3] = {hy, hz, hx};
+            /* coefficients of the two terms of each component, and the other profile of the same
+             * axis (what a planted half/full swap uses) */
+            real *a1[3] = {ay, az, axh}, *b1[3] = {by, bz, bxh}, *a2[3] = {az, ax, ay}, *b2[3] = {bz, bx, by};
+            real *a1s[3] = {ayh, azh, ax}, *b1s[3] = {byh, bzh, bx}, *a2s[3] = {azh, axh, ayh}, *b2s[3] = {bzh, bxh, byh};
+            for (int c = 0; c < 3; c++) {
+                spmv(D1[c], h1[c], upd);
+                spmv(D2[c], h2[c], tmp);
+                if (use_cpml) {
+                    /* psi: ex 6 hzy, 7 hyz ; ey 8 hxz, 9 hzx ; ez 10 hyx, 11 hxy */
+                    cpml_apply(n, upd, psi[6 + 2 * c], PLANT(6 + 2 * c, a1[c], a1s[c]), PLANT(6 + 2 * c, b1[c], b1s[c]));
+                    cpml_apply(n, tmp, psi[7 + 2 * c], PLANT(7 + 2 * c, a2[c], a2s[c]), PLANT(7 + 2 * c, b2[c], b2s[c]));
+                }
+                v_sub(n, upd, tmp);
+                for (int l = 0; l < EL; l++) VLOOP(n) upd[i] -= DT * er[c][l][i];
+                v_mul(n, upd, cb3[c]);
+                real *e = eo[c], *cac = ca3[c];
+                VLOOP(n) e[i] = cac[i] * e[i] + upd[i];
+            }
+            if (damp) {
+                real *all[6] = {ez, ey, ex, hx, hy, hz};
+                for (int k = 0; k < 6; k++) v_mul(n, all[k], damp);
+            }
         } else if (p->physics == PH_TEMEM) {
             /* ForwardSolver2Demem.cpp:108-171 ; F: hz ex ey ; CPMLEM2D.cpp: eyx, exy on the half
              * profiles, hzx, hzy on the full ones */
@@ -1507,14 +1915,33 @@ int oracle_run(const oracle_params *p, const real *m0, const real *m1, const rea
             v_sub(n, upd, tmp);
             v_mul(n, upd, imxy);
             v_sub(n, hz, upd);
+            /* ForwardSolver2Dviscoemem.cpp:175-218: r = Cd e + Cc r, then -DT r in the e updates */
+            for (int l = 0; l < EL; l++)
+                VLOOP(n)
+                {
+                    real a = ecc[l] * eR[l][i];
+                    real b = eCd[l][i] * ex[i];
+                    eR[l][i] = b + a;
+                    a = ecc[l] * eRY[l][i];
+                    b = eCdY[l][i] * ey[i];
+                    eRY[l][i] = b + a;
+                }
             spmv(&o.Dyb, hz, upd);
             if (use_cpml) cpml_apply(n, upd, psi[2], ay, by); /* hzy */
+            for (int l = 0; l < EL; l++) VLOOP(n) upd[i] -= DT * eR[l][i];
             v_mul(n, upd, Cb);
             VLOOP(n) ex[i] = Ca[i] * ex[i] + upd[i];
             spmv(&o.Dxb, hz, upd);
             if (use_cpml) cpml_apply(n, upd, psi[3], ax, bx); /* hzx */
-            v_mul(n, upd, CbY);
-            VLOOP(n) ey[i] = CaY[i] * ey[i] - upd[i];
+            if (emv) {
+                VLOOP(n) upd[i] = -upd[i];
+                for (int l = 0; l < EL; l++) VLOOP(n) upd[i] -= DT * eRY[l][i];
+                v_mul(n, upd, CbY);
+                VLOOP(n) ey[i] = CaY[i] * ey[i] + upd[i];
+            } else {
+                v_mul(n, upd, CbY);
+                VLOOP(n) ey[i] = CaY[i] * ey[i] - upd[i];
+            }
             if (damp) {
                 v_mul(n, ey, damp);
                 v_mul(n, ex, damp);
@@ -1532,13 +1959,22 @@ int oracle_run(const oracle_params *p, const real *m0, const real *m1, const rea
             VLOOP(n) upd[i] = -tmp[i];
             v_mul(n, upd, imxz);
             v_sub(n, hy, upd);
+            /* ForwardSolver2Dviscotmem.cpp:172-199: r = Cd ez + Cc r, then -DT r in the ez update */
+            for (int l = 0; l < EL; l++)
+                VLOOP(n)
+                {
+                    real a = ecc[l] * eR[l][i];
+                    real b = eCd[l][i] * ez[i];
+                    eR[l][i] = b + a;
+                }
             spmv(&o.Dxb, hy, upd);
             spmv(&o.Dyb, hx, tmp);
             if (use_cpml) {
-                cpml_apply(n, upd, psi[2], ax, bx); /* hyx */
-                cpml_apply(n, tmp, psi[3], ay, by); /* hxy */
+                cpml_apply(n, upd, psi[2], PLANT(2, ax, axh), PLANT(2, bx, bxh)); /* hyx */
+                cpml_apply(n, tmp, psi[3], ay, by);                                 /* hxy */
             }
             v_sub(n, upd, tmp);
+            for (int l = 0; l < EL; l++) VLOOP(n) upd[i] -= DT * eR[l][i];
             v_mul(n, upd, Cb);
             VLOOP(n) ez[i] = Ca[i] * ez[i] + upd[i];
             if (damp) {
@@ -1566,7 +2002,7 @@ int oracle_run(const oracle_params *p, const real *m0, const real *m1, const rea
                     }
                     if (ty == 2) F[0][i] += v;
                     if (ty == 3) F[1][i] += v;
-                } else if (p->physics == PH_ELASTIC) {
+                } else if (p->physics == PH_ELASTIC || p->physics == PH_VISCO) {
                     if (ty == 1) {
                         F[3][i] += v;
                         F[4][i] += v;
@@ -1577,6 +2013,12 @@ int oracle_run(const oracle_params *p, const real *m0, const real *m1, const rea
                     if (ty == 4) F[2][i] += v;
                 } else if (p->physics == PH_SH || p->physics == PH_VISCOSH) {
                     if (ty == 4) F[0][i] += v; /* only VZ sources in SH (FDTD2Dsh.hpp:25-27) */
+                } else if (p->physics == PH_TEMEM && p->dim == 3) {
+                    /* EZ, EX, EY, HZ (SourceReceiverImplEM.cpp:100-115) ; F: hx hy hz ex ey ez */
+                    if (ty == 1) F[5][i] += v;
+                    if (ty == 2) F[3][i] += v;
+                    if (ty == 3) F[4][i] += v;
+                    if (ty == 4) F[2][i] += v;
                 } else if (p->physics == PH_TEMEM) {
                     /* EX, EY, HZ (SourceReceiverImplEM.cpp:100-165; FDTD2Demem.hpp: no EZ) */
                     if (ty == 2) F[1][i] += v;
@@ -1600,7 +2042,7 @@ int oracle_run(const oracle_params *p, const real *m0, const real *m1, const rea
                     v *= (real)0.5;
                 } else
                     v = ty == 2 ? F[0][i] : F[1][i];
-            } else if (p->physics == PH_ELASTIC) {
+            } else if (p->physics == PH_ELASTIC || p->physics == PH_VISCO) {
                 if (ty == 1) {
                     v = F[3][i];
                     v += F[4][i];
@@ -1610,6 +2052,8 @@ int oracle_run(const oracle_params *p, const real *m0, const real *m1, const rea
                     v = ty == 2 ? F[0][i] : (ty == 3 ? F[1][i] : F[2][i]);
             } else if (p->physics == PH_SH || p->physics == PH_VISCOSH) {
                 v = F[0][i];
+            } else if (p->physics == PH_TEMEM && p->dim == 3) {
+                v = ty == 1 ? F[5][i] : (ty == 2 ? F[3][i] : (ty == 3 ? F[4][i] : F[2][i]));
             } else if (p->physics == PH_TEMEM) {
                 v = ty == 2 ? F[1][i] : (ty == 3 ? F[2][i] : F[0][i]);
             } else {
@@ -1651,6 +2095,8 @@ int oracle_run(const oracle_params *p, const real *m0, const real *m1, const rea
     free(imyz);
     free(CaY);
     free(CbY);
+    free(CaZ);
+    free(CbZ);
     free(imxy);
     free(fsH);
     free(fsV);
@@ -1659,7 +2105,15 @@ int oracle_run(const oracle_params *p, const real *m0, const real *m1, const rea
         free(fsRH[l]);
         free(fsRV[l]);
     }
-    for (int k = 0; k < 12; k++) free(Rv[k]);
+    for (int k = 0; k < 24; k++) free(Rv[k]);
+    for (int l = 0; l < 4; l++) {
+        free(eCd[l]);
+        free(eCdY[l]);
+        free(eR[l]);
+        free(eRY[l]);
+        free(eCdZ[l]);
+        free(eRZ[l]);
+    }
     free(opP);
     free(opS);
     free(tauSxy);

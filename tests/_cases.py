@@ -5,7 +5,8 @@ import numpy as np
 
 from oracle import pyoracle as po
 
-PHYS = {"acoustic": 0, "elastic": 1, "tmem": 2, "sh": 3, "viscoelastic": 4, "viscosh": 5, "emem": 6}
+PHYS = {"acoustic": 0, "elastic": 1, "tmem": 2, "sh": 3, "viscoelastic": 4, "viscosh": 5, "emem": 6, "viscotmem": 7,
+        "viscoemem": 8}
 
 
 def oracle_threads():

@@ -171,6 +171,9 @@ int wv_solver::enqueue_steps(int nsteps, bool timed)
                 const wv::Rec2D *rec = adjoint_record(this);
                 wv::launch_strip2d(order, tile_y, L, cfg.nshots, F, C, T, P, A, j, j > 0, st, 0, e0, e1, nullptr, rec);
                 wv::launch_strip2d(order, tile_y, L, cfg.nshots, F, C, T, P, A, j, 0, st, 1, e2, e3, nullptr, rec);
+            } else if (visco && (physics == wv::TMEM || physics == wv::TEMEM)) {
+                wv::launch_step2d(physics, order, tile_y, L, cfg.nshots, F, C, T, P, A, j, j > 0, st, 0, e0, e1);
+                wv::launch_viscoem2d_e(physics, order, tile_y, L, cfg.nshots, F, C, T, P, A, j, visco2d(), st, e2, e3);
             } else if (visco) {
                 wv::launch_step2d(physics, order, tile_y, L, cfg.nshots, F, C, T, P, A, j, j > 0, st, 0, e0, e1);
                 wv::launch_visco2d_stress(physics, order, tile_y, L, cfg.nshots, F, C, T, P, A, j, visco2d(), st, e2,
@@ -255,8 +258,17 @@ double wv_solver::kernel_bytes(int kind) const
     if (cfg.damping_boundary == 1 && (kind == 1 || variant == V_SINGLE)) bytes += 4.0 * cells;
     // viscoelastic stress phase: the memory variables (3 L read and written per shot-cell) and
     // tauP, tauS, tauS_xy, 1 + L tauP, 1 + L tauS per cell
-    if (visco && kind == 1)
-        bytes += physics == wv::SH ? 4.0 * (cells * B * 4.0 * vL + cells * 3.0) : 4.0 * (cells * B * 6.0 * vL + cells * 5.0);
+    // (visco-EM: the memory variables of ez, or of ex and ey, read and written per shot-cell and
+    // their Cd per cell)
+    if (visco && kind == 1) {
+        if (physics == wv::TMEM || physics == wv::TEMEM) {
+            const double g = physics == wv::TMEM ? 1.0 : 2.0;
+            bytes += 4.0 * (cells * B * 2.0 * g * vL + cells * g * vL);
+        } else {
+            bytes += physics == wv::SH ? 4.0 * (cells * B * 4.0 * vL + cells * 3.0)
+                                       : 4.0 * (cells * B * 6.0 * vL + cells * 5.0);
+        }
+    }
     return bytes;
 }
 
@@ -284,20 +296,24 @@ int wv_create(const char *dimension, const char *type, wv_solver **out)
         phys = wv::TMEM;
     else if (t == "sh")
         phys = wv::SH;
-    else if (t == "emem" && d == "2d")
-        phys = wv::TEMEM;
-    else if (t == "viscoelastic" && d == "2d")
+    else if (t == "emem")
+        phys = wv::TEMEM;  // 2D: TE mode (hz, ex, ey); 3D: all six components
+    else if (t == "viscoelastic")
         phys = wv::ELASTIC;  // the elastic step with memory variables in the stress phase
     else if (t == "viscosh" && d == "2d")
         phys = wv::SH;  // the SH step with memory variables in the stress phase
+    else if (t == "viscotmem" && d == "2d")
+        phys = wv::TMEM;  // the TM step with memory variables in the electric phase
+    else if (t == "viscoemem")
+        phys = wv::TEMEM;  // the TE (3D: EM) step with memory variables in the electric phase
     else
         return fail(WV_EINVAL, "equationType " + t + " is not provided by this solver");
-    if (d == "3d" && phys != wv::ELASTIC && phys != wv::ACOUSTIC)
+    if (d == "3d" && phys != wv::ELASTIC && phys != wv::ACOUSTIC && phys != wv::TEMEM)
         return fail(WV_EINVAL, "3D " + t + " is not provided by this build");
     wv_solver *s = new wv_solver();
     s->dim = d == "3d" ? 3 : 2;
     s->physics = phys;
-    s->visco = t == "viscoelastic" || t == "viscosh";
+    s->visco = t == "viscoelastic" || t == "viscosh" || t == "viscotmem" || t == "viscoemem";
     *out = s;
     return WV_OK;
 }
@@ -357,7 +373,7 @@ int wv_init(wv_solver *s, const wv_config *c)
 
     if (s->dim == 3) {
         s->e3 = new wv::Elastic3D();
-        int rc = s->e3->init(s->cfg, s->st, s->physics);
+        int rc = s->e3->init(s->cfg, s->st, s->physics, s->visco);
         if (rc) return fail(rc, s->e3->error);
         s->shots.assign(k.nshots, ShotAcq());
         s->inited = true;
@@ -541,6 +557,26 @@ int wv_set_model(wv_solver *s, const float *m0, const float *m1, const float *m2
     }
     auto li = [&](int x, int y) { return (size_t)L.at(x, y); };
     std::vector<std::vector<float>> c(5, std::vector<float>((size_t)L.plane, 0.f));
+    // visco-EM: relaxation times tau_l in the value type, the mean of 1 / tau_l and the
+    // per-mechanism factor of Cd (ForwardSolverEM.cpp:100-157), Cd planes (TM: l; TE: l, 4 + l)
+    const bool vem = s->visco && (s->physics == wv::TMEM || s->physics == wv::TEMEM);
+    float vinv = 0.f, vcdl[4] = {0.f, 0.f, 0.f, 0.f};
+    std::vector<std::vector<float>> vcd(8);
+    if (vem) {
+        const float dtf = (float)s->cfg.DT;
+        for (int l = 0; l < s->vL; l++) {
+            const float tau = (float)(1.0 / (2.0 * M_PI * (double)(float)s->vfreq[l]));
+            vinv = (float)((double)vinv + 1.0 / (double)tau);
+            float tv = (float)(1.0 / (1.0 + 0.5 * (double)dtf / (double)tau));
+            float den = (float)s->vL * tau;
+            den *= tau;
+            tv /= den;
+            vcdl[l] = tv;
+            vcd[l].assign((size_t)L.plane, 0.f);
+            if (s->physics == wv::TEMEM) vcd[4 + l].assign((size_t)L.plane, 0.f);
+        }
+        vinv /= (float)s->vL;
+    }
     auto inv_avg = [&](const std::vector<double> &m, std::vector<float> &out, int ax) {
         // 1/((m_i + m_next)/2), last point 1/m_i, inf/nan -> 0 (Modelparameter.cpp:336-446, 633-639)
         for (int y = 0; y < NY; y++)
@@ -654,6 +690,19 @@ int wv_set_model(wv_solver *s, const float *m0, const float *m1, const float *m2
                         e = 0.5f * eps[i] + 0.5f * eps[j];
                         sg = 0.5f * m1[i] + 0.5f * m1[j];
                     }
+                    if (s->visco) {
+                        // ViscoEMEM.cpp:409-435: both tau averaged as epsilon and sigma
+                        float te = s->vtauP[i], ts = s->vtauS[i];
+                        if (!last) {
+                            te = 0.5f * s->vtauP[i] + 0.5f * s->vtauP[j];
+                            ts = 0.5f * s->vtauS[i] + 0.5f * s->vtauS[j];
+                        }
+                        float cd[4];
+                        wv::em_visco_coef(e, sg, te, ts, dtf, vinv, s->vL, vcdl, c[1 + 2 * ax][li(x, y)],
+                                          c[2 + 2 * ax][li(x, y)], cd);
+                        for (int l = 0; l < s->vL; l++) vcd[4 * ax + l][li(x, y)] = cd[l];
+                        continue;
+                    }
                     wv::em_ca_cb(e, sg, dtf, c[1 + 2 * ax][li(x, y)], c[2 + 2 * ax][li(x, y)]);
                 }
             }
@@ -668,6 +717,14 @@ int wv_set_model(wv_solver *s, const float *m0, const float *m1, const float *m2
                 // Ca, Cb in the reference's float arithmetic, step by step as ForwardSolverEM.cpp:14-68
                 // (epsilon = epsilon_r * eps0, TMEM.cpp:230-241): the time loop multiplies ez by Ca
                 // every step, so a differently rounded Ca drifts from the float build's seismograms
+                if (s->visco) {
+                    // ForwardSolver2Dviscotmem.cpp:77-100: node values
+                    float cd[4];
+                    wv::em_visco_coef(m0[i] * eps0f, m1[i], s->vtauP[i], s->vtauS[i], dtf, vinv, s->vL, vcdl,
+                                      c[0][li(x, y)], c[1][li(x, y)], cd);
+                    for (int l = 0; l < s->vL; l++) vcd[l][li(x, y)] = cd[l];
+                    continue;
+                }
                 wv::em_ca_cb(m0[i] * eps0f, m1[i], dtf, c[0][li(x, y)], c[1][li(x, y)]);
             }
         inv_avg(perm, c[2], 0);  // 1/mu averaged along x (TMEM.cpp:366-373)
@@ -679,6 +736,15 @@ int wv_set_model(wv_solver *s, const float *m0, const float *m1, const float *m2
             if ((rc = s->dalloc(&s->coef[q], (size_t)L.plane, s->allocs))) return rc;
         }
         HIP_TRY(hipMemcpyAsync(s->coef[q], c[q].data(), c[q].size() * sizeof(float), hipMemcpyHostToDevice, s->st));
+    }
+    if (vem) {
+        for (int k = 0; k < 8; k++) {
+            if (vcd[k].empty()) continue;
+            if (!s->vcd[k] && (rc = s->dalloc(&s->vcd[k], (size_t)L.plane, s->allocs))) return rc;
+            HIP_TRY(hipMemcpyAsync(s->vcd[k], vcd[k].data(), vcd[k].size() * sizeof(float), hipMemcpyHostToDevice,
+                                   s->st));
+        }
+        HIP_TRY(hipStreamSynchronize(s->st));  // vcd is freed at the end of this call
     }
     if (s->visco && s->physics == wv::SH) {
         // visco-SH: tauS averaged along x and y with the 2-point matrices of mu_xz / mu_yz
@@ -788,8 +854,17 @@ int wv_set_relaxation(wv_solver *s, int nmech, const double *freqs, const float 
     if (nmech < 1 || nmech > 4)
         return fail(WV_EINVAL, "The number of relaxation mechanisms should be >0 in an viscoelastic simulation (1..4)");
     if (!freqs || !tauS || (!tauP && s->physics != wv::SH)) return fail(WV_EINVAL, "missing relaxation argument");
+    if (s->dim == 2 && (s->physics == wv::TMEM || s->physics == wv::TEMEM) && (!tauP || !tauS))
+        return fail(WV_EINVAL, "visco-EM needs tau of the permittivity and of the conductivity");
     for (int l = 0; l < nmech; l++)
         if (!(freqs[l] > 0)) return fail(WV_EINVAL, "The relaxation frequency should be >=0 in an viscoelastic simulation");
+    if (s->e3) {
+        int rc = s->e3->set_relaxation(nmech, freqs, tauP, tauS);
+        if (rc) return fail(rc, s->e3->error);
+        s->vL = nmech;
+        s->model_ok = false;
+        return WV_OK;
+    }
     const size_t n = (size_t)s->L.NX * s->L.NY;
     s->vfreq.assign(freqs, freqs + nmech);
     if (tauP)
@@ -809,7 +884,15 @@ int wv_set_relaxation(wv_solver *s, int nmech, const double *freqs, const float 
     }
     s->vdth = (float)((double)DTf / 2.0);
     const size_t fn = (size_t)s->L.plane * s->cfg.nshots;
-    const int ngroups = s->physics == wv::SH ? 2 : 3;  // Rxz, Ryz | Rxx, Ryy, Rxy
+    // Rxz, Ryz | Rxx, Ryy, Rxy | visco-EM: r of ez | r of ex, ey
+    const int ngroups = s->physics == wv::SH ? 2 : (s->physics == wv::TMEM ? 1 : (s->physics == wv::TEMEM ? 2 : 3));
+    // visco-EM (ForwardSolverEM.cpp:90-100): Cc_l = (1 - DT / (2 tau_l)) / (1 + DT / (2 tau_l)), DT
+    for (int l = 0; l < 4; l++) s->vcc[l] = 0.f;
+    for (int l = 0; l < nmech; l++) {
+        const double tau = (double)(float)(1.0 / (2.0 * M_PI * (double)(float)freqs[l]));
+        s->vcc[l] = (float)((1.0 - 0.5 * (double)DTf / tau) / (1.0 + 0.5 * (double)DTf / tau));
+    }
+    s->vdt = DTf;
     for (int l = 0; l < nmech; l++)
         for (int g = 0; g < ngroups; g++) {
             float *&p = s->vR[4 * g + l];
@@ -836,9 +919,9 @@ int wv_set_acquisition(wv_solver *s, int shot, int nsrc, const long long *src_in
     const long long n = (long long)s->cfg.NX * s->cfg.NY * s->cfg.NZ;
     const int maxtype =
         s->physics == wv::TMEM ? 1 : (s->dim == 3 || s->physics == wv::SH || s->physics == wv::TEMEM ? 4 : 3);
-    // SH has VZ sources and receivers only (FDTD2Dsh.hpp:25-27 throws for P, VX, VY); TE-EM
-    // EX, EY, HZ (FDTD2Demem.hpp:29-30 throws for EZ)
-    const int mintype = s->physics == wv::SH ? 4 : (s->physics == wv::TEMEM ? 2 : 1);
+    // SH has VZ sources and receivers only (FDTD2Dsh.hpp:25-27 throws for P, VX, VY); 2D TE-EM
+    // EX, EY, HZ (FDTD2Demem.hpp:29-30 throws for EZ); 3D EM EZ, EX, EY, HZ
+    const int mintype = s->physics == wv::SH ? 4 : (s->physics == wv::TEMEM && s->dim == 2 ? 2 : 1);
     for (int k = 0; k < nsrc; k++) {
         if (src_index[k] < 0 || src_index[k] >= n) return fail(WV_EINVAL, "source outside the grid");
         if (src_type[k] < mintype || src_type[k] > maxtype) return fail(WV_EINVAL, "unsupported source type");

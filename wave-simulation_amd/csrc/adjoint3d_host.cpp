@@ -253,7 +253,7 @@ int Elastic3D::enqueue_rec(int nsteps, float *rec)
 
 int Elastic3D::set_checkpointing(int every, int fp16, double budget_gib)
 {
-    if (physics != ELASTIC) return fail(WV_EINVAL, "the 3D adjoint is provided for elastic handles");
+    if (physics != ELASTIC || visco) return fail(WV_EINVAL, "the 3D adjoint is provided for elastic handles");
     if (free_surface || damping == 1)
         return fail(WV_EINVAL, "the adjoint supports CPML or no absorbing boundary, without free surface");
     if (every < 0 || every > NT) return fail(WV_EINVAL, "checkpoint interval out of range");

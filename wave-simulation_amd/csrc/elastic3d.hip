@@ -228,12 +228,30 @@ __device__ __forceinline__ float cpml(float &d, float psi, float a, float b)
     return p;
 }
 
-__device__ __forceinline__ bool velocity_type(int ty) { return ty >= 2; }
+// Acquisition modes of the 3D kernels: elastic, acoustic, EM (fields hx hy hz ex ey ez)
+enum { ACQ_ELASTIC = 0, ACQ_ACOUSTIC = 1, ACQ_EM = 2 };
+
+// Source / receiver types that belong to the first phase of a step (velocities, or H in EM:
+// HZ) and are applied after the second phase has read that phase's fields, i.e. at the start of
+// the next step's first launch or by the flush at the end of a run segment.
+template <int MODE>
+__device__ __forceinline__ bool velocity_type(int ty)
+{
+    return MODE == ACQ_EM ? ty == 4 : ty >= 2;
+}
+// field of a source / receiver type other than the elastic P (EM: EZ, EX, EY, HZ -> 5, 3, 4, 2;
+// SourceReceiverImplEM.cpp:100-115; seismic: VX, VY, VZ -> 0, 1, 2, acoustic P -> 3)
+template <int MODE>
+__device__ __forceinline__ int type_field(int ty)
+{
+    if (MODE == ACQ_EM) return ty == 1 ? 5 : (ty == 4 ? 2 : ty + 1);
+    return ty == 1 ? 3 : ty - 2;
+}
 
 // Sources / receivers owned by block sb of one shot at step t, velocity-type
-// (VX, VY, VZ) or stress-type (P: sxx, syy, szz in elastic, p = field 3 in acoustic).
-// Uniform per block.
-template <bool AC>
+// (VX, VY, VZ; EM: HZ) or stress-type (P: sxx, syy, szz in elastic, p = field 3 in acoustic;
+// EM: EZ, EX, EY).  Uniform per block.
+template <int MODE>
 __device__ void acquisition3d(const Acq &A, const Fields3D &F, long long so, int sb, int t, bool vel)
 {
     const int s0 = A.bsrc_ptr[sb], s1 = A.bsrc_ptr[sb + 1];
@@ -242,22 +260,20 @@ __device__ void acquisition3d(const Acq &A, const Fields3D &F, long long so, int
     const int tid = threadIdx.y * TILE_X + threadIdx.x;
     __syncthreads();
     if (tid == 0) {
-        // components in the reference's order P, VX, VY, VZ; each in list order
+        // components in the reference's order P (EZ), VX (EX), VY (EY), VZ (HZ); each in list order
         for (int ty = 1; ty <= 4; ty++) {
-            if (velocity_type(ty) != vel) continue;
+            if (velocity_type<MODE>(ty) != vel) continue;
             for (int e = s0; e < s1; e++) {
                 const int s = A.bsrc[e];
                 if (A.src_type[s] != ty) continue;
                 const float v = A.src_sig[(long long)s * A.NT + t];
                 const long long c = so + A.src_cell[s];
-                if (ty == 1 && AC) {
-                    F.f[3][c] += v;
-                } else if (ty == 1) {
+                if (ty == 1 && MODE == ACQ_ELASTIC) {
                     F.f[3][c] += v;
                     F.f[4][c] += v;
                     F.f[5][c] += v;
                 } else {
-                    F.f[ty - 2][c] += v;
+                    F.f[type_field<MODE>(ty)][c] += v;
                 }
             }
         }
@@ -266,18 +282,16 @@ __device__ void acquisition3d(const Acq &A, const Fields3D &F, long long so, int
     for (int e = r0 + tid; e < r1; e += NTH) {
         const int r = A.brec[e];
         const int ty = A.rec_type[r];
-        if (velocity_type(ty) != vel) continue;
+        if (velocity_type<MODE>(ty) != vel) continue;
         const long long c = so + A.rec_cell[r];
         float v;
-        if (ty == 1 && AC) {
-            v = F.f[3][c];
-        } else if (ty == 1) {
+        if (ty == 1 && MODE == ACQ_ELASTIC) {
             v = F.f[3][c];
             v += F.f[4][c];
             v += F.f[5][c];
             v /= 3.f;
         } else {
-            v = F.f[ty - 2][c];
+            v = F.f[type_field<MODE>(ty)][c];
         }
         A.seis[(long long)t * A.rec_total + r] = v;
     }
@@ -329,7 +343,7 @@ __global__ __launch_bounds__(NTH) void el3d_velocity(Layout3D L, Fields3D F, Coe
     const long long so = (long long)q.shot * L.vol;
     const int tid = threadIdx.y * TILE_X + threadIdx.x, tx = threadIdx.x;
     const int wz = __builtin_amdgcn_readfirstlane(threadIdx.y);
-    if (pend) acquisition3d<false>(A, F, so, q.sb, A.tbase[0] + tofs - 1, true);
+    if (pend) acquisition3d<ACQ_ELASTIC>(A, F, so, q.sb, A.tbase[0] + tofs - 1, true);
 
     const float *sxx = F.f[3] + so, *syy = F.f[4] + so, *szz = F.f[5] + so;
     const float *syz = F.f[6] + so, *sxz = F.f[7] + so, *sxy = F.f[8] + so;
@@ -534,9 +548,129 @@ __global__ __launch_bounds__(NTH) void el3d_velocity(Layout3D L, Fields3D F, Coe
     }
 }
 
-template <int H, int RZ, bool REC>
+// Viscoelastic stress update of one cell from its nine CPML-modified derivatives
+// (ForwardSolver3Dviscoelastic.cpp:270-444, generalised standard linear solid with the
+// relaxation term in Crank-Nicolson form, in the reference's order of operations):
+//   u = M (vxx + vyy + vzz); for l: g = u tauP / tau_l; s_ii += dt/2 R_ii,l; R_ii,l = c1 R_ii,l - g
+//   s_ii += (1 + L tauP) u                                        (ii = xx, yy, zz)
+//   u = 2 mu (vyy + vzz); for l: R_xx,l = c2 (R_xx,l + u tauS / tau_l); s_xx += dt/2 R_xx,l
+//   s_xx -= (1 + L tauS) u                      (yy from vxx + vzz, zz from vxx + vyy)
+//   u = mu_ij (d_i v_j + d_j v_i); for l: s_ij += dt/2 R_ij,l;
+//       R_ij,l = c2 (c1 R_ij,l - u tauS_ij / tau_l); s_ij += dt/2 R_ij,l;  s_ij += (1 + L tauS) u
+// then on the free surface the exchange of FreeSurface3Dviscoelastic.cpp:17-75 for sxx, szz and
+// their memory variables, syy and every Ryy zero; ABS damping of the stresses on the store.
+// s[] = sxx syy szz syz sxz sxy (updated in place); o = byte offset of the cell in its plane.
+__device__ __forceinline__ void ve3d_cell(const Visco3D &V, const Layout3D &L, long long so, int y, unsigned o,
+                                          float (&s)[6], float M, float mu, float mxy, float mxz, float myz,
+                                          float vxx, float vyy, float vzz, float dxy, float dxz, float dyz,
+                                          const float *fsH, const float *fsV, long long fo)
+{
+    const int NL = V.L;
+    const float tP = bload(plane_rsrc(V.tauP, L, y), o, 0), tS = bload(plane_rsrc(V.tauS, L, y), o, 0);
+    const float oP = bload(plane_rsrc(V.opP, L, y), o, 0), oS = bload(plane_rsrc(V.opS, L, y), o, 0);
+    const float tsh[3] = {bload(plane_rsrc(V.tSyz, L, y), o, 0), bload(plane_rsrc(V.tSxz, L, y), o, 0),
+                          bload(plane_rsrc(V.tSxy, L, y), o, 0)};
+    float R[6][4];
+#pragma unroll
+    for (int g = 0; g < 6; g++)
+#pragma unroll
+        for (int l = 0; l < 4; l++) R[g][l] = l < NL ? bload(plane_rsrc(V.R[4 * g + l] + so, L, y), o, 0) : 0.f;
+    const float dth = V.dth;
+    float u = vxx + vyy;
+    u += vzz;
+    u *= M;
+#pragma unroll
+    for (int l = 0; l < 4; l++)
+        if (l < NL) {
+            float g = V.itau[l] * u;
+            g *= tP;
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                s[c] += dth * R[c][l];
+                R[c][l] *= V.c1[l];
+                R[c][l] -= g;
+            }
+        }
+    u *= oP;
+    s[0] += u;
+    s[1] += u;
+    s[2] += u;
+    const float da[3] = {vyy, vxx, vxx}, db[3] = {vzz, vzz, vyy};
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        u = da[c] + db[c];
+        u *= mu;
+        u *= 2.f;
+#pragma unroll
+        for (int l = 0; l < 4; l++)
+            if (l < NL) {
+                float g = V.itau[l] * u;
+                g *= tS;
+                R[c][l] += g;
+                R[c][l] *= V.c2[l];
+                s[c] += dth * R[c][l];
+            }
+        u *= oS;
+        s[c] -= u;
+    }
+    // shear: groups 3 yz, 4 xz, 5 xy
+    const float dsh[3] = {dyz, dxz, dxy}, msh[3] = {myz, mxz, mxy};
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        u = dsh[c] * msh[c];
+#pragma unroll
+        for (int l = 0; l < 4; l++)
+            if (l < NL) {
+                s[3 + c] += dth * R[3 + c][l];
+                R[3 + c][l] *= V.c1[l];
+                float g = V.itau[l] * u;
+                g *= tsh[c];
+                R[3 + c][l] -= g;
+                R[3 + c][l] *= V.c2[l];
+                s[3 + c] += dth * R[3 + c][l];
+            }
+        u *= oS;
+        s[3 + c] += u;
+    }
+    if (fsH && y == 0) {
+        const float hs = vxx + vzz;
+#pragma unroll
+        for (int l = 0; l < 4; l++)
+            if (l < NL) {
+                s[0] -= dth * R[0][l];
+                s[2] -= dth * R[2][l];
+            }
+        const float a = fsH[fo] * hs;
+        s[0] += a;
+        s[2] += a;
+        const float b = fsV[fo] * vyy;
+        s[0] -= b;
+        s[2] -= b;
+#pragma unroll
+        for (int l = 0; l < 4; l++)
+            if (l < NL) {
+                const float c = V.fsRH[l][fo] * hs;
+                R[0][l] += c;
+                R[2][l] += c;
+                const float d = V.fsRV[l][fo] * vyy;
+                R[0][l] -= d;
+                R[2][l] -= d;
+                s[0] += dth * R[0][l];
+                s[2] += dth * R[2][l];
+                R[1][l] *= 0.f;
+            }
+        s[1] *= 0.f;
+    }
+#pragma unroll
+    for (int g = 0; g < 6; g++)
+#pragma unroll
+        for (int l = 0; l < 4; l++)
+            if (l < NL) bstore(R[g][l], plane_rsrc(V.R[4 * g + l] + so, L, y), o, 0);
+}
+
+template <int H, int RZ, bool REC, bool VE = false>
 __global__ __launch_bounds__(NTH) void el3d_stress(Layout3D L, Fields3D F, Coef3D C, Taps3D T, Cpml3D P, Acq A,
-                                                   int nyc, int LY, int tofs, Rec3D R)
+                                                   int nyc, int LY, int tofs, Rec3D R, Visco3D V)
 {
     constexpr int TZ = RZ * BLOCK_Y, K = 2 * H + 1;
     using RXZ = Region<H, H, TZ>;
@@ -740,6 +874,20 @@ __global__ __launch_bounds__(NTH) void el3d_stress(Layout3D L, Fields3D F, Coef3
                     bstore(vxz + vzx, plane_rsrc(R.r[7] + so, L, y), oc[r], 0);
                     bstore(vyz + vzy, plane_rsrc(R.r[8] + so, L, y), oc[r], 0);
                 }
+                if constexpr (VE) {
+                    float sv[6] = {cur.o[0][r], cur.o[1][r], cur.o[2][r], cur.o[3][r], cur.o[4][r], cur.o[5][r]};
+                    const long long fo = (long long)(z + L.H) * L.pitch + x + XPAD;
+                    ve3d_cell(V, L, so, y, oc[r], sv, cur.c[0][r], cur.c[1][r], cur.c[2][r], cur.c[3][r], cur.c[4][r],
+                              vxx, vyy, vzz, vxy + vyx, vxz + vzx, vyz + vzy, C.fsH, C.fsV, fo);
+                    const float d = cur.c[5][r];
+                    bstore(sv[0] * d, plane_rsrc(sxx, L, y), oc[r], 0);
+                    bstore(sv[1] * d, plane_rsrc(syy, L, y), oc[r], 0);
+                    bstore(sv[2] * d, plane_rsrc(szz, L, y), oc[r], 0);
+                    bstore(sv[3] * d, plane_rsrc(syz, L, y), oc[r], 0);
+                    bstore(sv[4] * d, plane_rsrc(sxz, L, y), oc[r], 0);
+                    bstore(sv[5] * d, plane_rsrc(sxy, L, y), oc[r], 0);
+                    continue;
+                }
                 // normal stresses (ForwardSolver3Delastic.cpp: M (vxx+vyy+vzz) - 2 mu (...))
                 float s = vxx + vyy;
                 s += vzz;
@@ -787,7 +935,7 @@ __global__ __launch_bounds__(NTH) void el3d_stress(Layout3D L, Fields3D F, Coef3
         }
         block_sync_lds();  // the next plane rewrites the LDS regions
     }
-    acquisition3d<false>(A, F, so, q.sb, A.tbase[0] + tofs, false);
+    acquisition3d<ACQ_ELASTIC>(A, F, so, q.sb, A.tbase[0] + tofs, false);
 }
 
 // ---------------------------------------------------------------------------
@@ -810,7 +958,7 @@ __global__ __launch_bounds__(NTH) void ac3d_velocity(Layout3D L, Fields3D F, Coe
     const long long so = (long long)q.shot * L.vol;
     const int tid = threadIdx.y * TILE_X + threadIdx.x, tx = threadIdx.x;
     const int wz = __builtin_amdgcn_readfirstlane(threadIdx.y);
-    if (pend) acquisition3d<true>(A, F, so, q.sb, A.tbase[0] + tofs - 1, true);
+    if (pend) acquisition3d<ACQ_ACOUSTIC>(A, F, so, q.sb, A.tbase[0] + tofs - 1, true);
     const float *pr = F.f[3] + so;
     float *const vv[3] = {F.f[0] + so, F.f[1] + so, F.f[2] + so};
     const float *const cr[3] = {C.irx, C.iry, C.irz};
@@ -1001,9 +1149,302 @@ __global__ __launch_bounds__(NTH) void ac3d_pressure(Layout3D L, Fields3D F, Coe
         }
         block_sync_lds();
     }
-    acquisition3d<true>(A, F, so, q.sb, A.tbase[0] + tofs, false);
+    acquisition3d<ACQ_ACOUSTIC>(A, F, so, q.sb, A.tbase[0] + tofs, false);
 }
 
+// ---------------------------------------------------------------------------
+// 3D EM (ForwardSolver3Demem.cpp:117-235).  Fields hx hy hz ex ey ez (0..5),
+// coefficients 1/mu_yz 1/mu_xz 1/mu_xy (Coef3D irx iry irz) and the Ca, Cb of the
+// x-, y-, z-averaged permittivity and conductivity (Coef3D em).  CPML memory variables
+// (CPMLEM3D.cpp:25-104): E derivatives on the half profiles 0 ezy 1 eyz 2 exz 3 ezx 4 eyx
+// 5 exy; H derivatives on the full ones 6 hzy 7 hyz 8 hxz 9 hzx 11 hxy, except 10 hyx,
+// which the reference gives the half x profile (CPMLEM3D.cpp:67-70).  Same tiles and y march
+// as the elastic kernels: x-z regions of the three fields read in LDS, y queues of the two
+// fields with a y derivative.
+// ---------------------------------------------------------------------------
+template <int H, int RZ>
+__global__ __launch_bounds__(NTH) void em3d_h(Layout3D L, Fields3D F, Coef3D C, Taps3D T, Cpml3D P, Acq A,
+                                              int nyc, int LY, int tofs, int pend)
+{
+    constexpr int TZ = RZ * BLOCK_Y, K = 2 * H + 1;
+    using RXZ = Region<H, H, TZ>;
+    __shared__ float s_ex[RXZ::RHP][RXZ::RW];
+    __shared__ float s_ey[RXZ::RHP][RXZ::RW];
+    __shared__ float s_ez[RXZ::RHP][RXZ::RW];
+    __shared__ ZTab<TZ, K> s_zt;
+    __shared__ float s_yt[NTH];
+    const Pos q = position(L, TZ, nyc, LY, A);
+    const long long so = (long long)q.shot * L.vol;
+    const int tid = threadIdx.y * TILE_X + threadIdx.x, tx = threadIdx.x;
+    const int wz = __builtin_amdgcn_readfirstlane(threadIdx.y);
+    // the HZ sources / receivers of the previous step (applied after its E update read hz)
+    if (pend) acquisition3d<ACQ_EM>(A, F, so, q.sb, A.tbase[0] + tofs - 1, true);
+    float *const hh[3] = {F.f[0] + so, F.f[1] + so, F.f[2] + so};
+    const float *ex = F.f[3] + so, *ey = F.f[4] + so, *ez = F.f[5] + so;
+    const float *const cm[3] = {C.irx, C.iry, C.irz};
+    const int x = q.x0 + tx;
+    const bool xin = x < L.NX;
+    float cxf[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) cxf[k] = T.xf[(long long)x * K + k];
+    const int xi = (P.on && xin) ? P.xi[x] : -1;
+    const float axh = xi >= 0 ? P.axh[x] : 0.f, bxh = xi >= 0 ? P.bxh[x] : 0.f;
+    unsigned oc[RZ];
+#pragma unroll
+    for (int r = 0; r < RZ; r++) oc[r] = (unsigned)(((q.z0 + wz + r * BLOCK_Y + L.H) * L.pitch + x + XPAD) * 4);
+    RegionOff<H, H, TZ> ro;
+    ro.init(L, q.x0, q.z0, tid);
+    // y queues: ex (exy), ez (ezy)
+    float qx[K][RZ], qz[K][RZ];
+#pragma unroll
+    for (int k = 0; k < 2 * H; k++) {
+        const auto b0 = plane_rsrc(ex, L, q.ys - H + k), b1 = plane_rsrc(ez, L, q.ys - H + k);
+#pragma unroll
+        for (int r = 0; r < RZ; r++) {
+            qx[k + 1][r] = bload(b0, oc[r], 0);
+            qz[k + 1][r] = bload(b1, oc[r], 0);
+        }
+    }
+    const bool tfr = tile_in_frame(L, P, q.x0, q.z0, TZ);
+    s_zt.fill(T, P, L, q.z0, tid);
+    PsiRes3D ps;  // x frame: 3 ezx, 4 eyx | y frame: 0 ezy, 5 exy | z frame: 1 eyz, 2 exz
+    ps.init(P, L, q.shot, 3, 4, 4, 0, 5, 5, 1, 2, 2);
+    const int W2 = 2 * P.W;
+    RXZ rx, ry, rzr;
+    for (int y = q.ys; y < q.ye; y++) {
+        const float yt = ytab_elem<K>(T.ys, y, tid);
+        load_region(rx, ro, ex, L, y);
+        load_region(ry, ro, ey, L, y);
+        load_region(rzr, ro, ez, L, y);
+        float nx[RZ], nz[RZ], o[3][RZ], c[3][RZ];
+        {
+            const auto b0 = plane_rsrc(ex, L, y + H), b1 = plane_rsrc(ez, L, y + H);
+#pragma unroll
+            for (int r = 0; r < RZ; r++) {
+                nx[r] = bload(b0, oc[r], 0);
+                nz[r] = bload(b1, oc[r], 0);
+            }
+        }
+#pragma unroll
+        for (int f = 0; f < 3; f++) {
+            const auto b = plane_rsrc(hh[f], L, y), bc = plane_rsrc(cm[f], L, y);
+#pragma unroll
+            for (int r = 0; r < RZ; r++) {
+                o[f][r] = bload(b, oc[r], 0);
+                c[f][r] = bload(bc, oc[r], 0);
+            }
+        }
+        shift<H, RZ>(qx);
+        shift<H, RZ>(qz);
+#pragma unroll
+        for (int r = 0; r < RZ; r++) {
+            qx[2 * H][r] = nx[r];
+            qz[2 * H][r] = nz[r];
+        }
+        rx.store_padded(s_ex, tid);
+        ry.store_padded(s_ey, tid);
+        rzr.store_padded(s_ez, tid);
+        s_yt[tid] = yt;
+        block_sync_lds();
+        const int yi = __builtin_amdgcn_readfirstlane(P.on ? __float_as_int(s_yt[2 * K]) : -1);
+        const bool fr = __builtin_amdgcn_readfirstlane((int)(tfr || yi >= 0)) != 0;
+        const float *cyf = s_yt + K;
+        const float ayh = yi >= 0 ? s_yt[2 * K + 3] : 0.f, byh = yi >= 0 ? s_yt[2 * K + 4] : 0.f;
+#pragma unroll
+        for (int r = 0; r < RZ; r++) {
+            const int lz = wz + r * BLOCK_Y;
+            const int z = q.z0 + lz;
+            if (z >= L.NZ || !xin) continue;
+            const float *czf = s_zt.t[1][lz];
+            float ezy = ddy<H, RZ>(qz, r, cyf);
+            float exy = ddy<H, RZ>(qx, r, cyf);
+            float ezx = ddx<H>(s_ez, lz + H, tx, cxf);
+            float eyx = ddx<H>(s_ey, lz + H, tx, cxf);
+            float eyz = ddz<H>(s_ey, lz, tx + H, czf);
+            float exz = ddz<H>(s_ex, lz, tx + H, czf);
+            if (fr) {
+                const unsigned ox = psi_ox(L, W2, xi, y, z, true), oy = psi_oy(L, yi, x, z, true),
+                               oz = psi_oz(L, W2, s_zt.zi[lz], x, y, true);
+                const float azh = s_zt.c[lz][2], bzh = s_zt.c[lz][3];
+                bstore(cpml(ezx, bload(ps.x[0], ox, 0), axh, bxh), ps.x[0], ox, 0);
+                bstore(cpml(eyx, bload(ps.x[1], ox, 0), axh, bxh), ps.x[1], ox, 0);
+                bstore(cpml(ezy, bload(ps.y[0], oy, 0), ayh, byh), ps.y[0], oy, 0);
+                bstore(cpml(exy, bload(ps.y[1], oy, 0), ayh, byh), ps.y[1], oy, 0);
+                bstore(cpml(eyz, bload(ps.z[0], oz, 0), azh, bzh), ps.z[0], oz, 0);
+                bstore(cpml(exz, bload(ps.z[1], oz, 0), azh, bzh), ps.z[1], oz, 0);
+            }
+            // hx -= (ezy - eyz) / mu_yz ; hy -= (exz - ezx) / mu_xz ; hz -= (eyx - exy) / mu_xy
+            float u = ezy - eyz;
+            u *= c[0][r];
+            bstore(o[0][r] - u, plane_rsrc(hh[0], L, y), oc[r], 0);
+            u = exz - ezx;
+            u *= c[1][r];
+            bstore(o[1][r] - u, plane_rsrc(hh[1], L, y), oc[r], 0);
+            u = eyx - exy;
+            u *= c[2][r];
+            bstore(o[2][r] - u, plane_rsrc(hh[2], L, y), oc[r], 0);
+        }
+        block_sync_lds();
+    }
+}
+
+// E update: ex = Ca_x ex + Cb_x ((hzy - hyz) [- sum_l DT rx_l]), ey from (hxz - hzx), ez from
+// (hyx - hxy); VE: first r_l = Cd_l e + Cc_l r_l with the e of the previous step
+// (ForwardSolver3Dviscoemem.cpp:238-300); then the ABS damping of E and the EZ, EX, EY
+// sources / receivers of the step.
+template <int H, int RZ, bool VE>
+__global__ __launch_bounds__(NTH) void em3d_e(Layout3D L, Fields3D F, Coef3D C, Taps3D T, Cpml3D P, Acq A,
+                                              int nyc, int LY, int tofs, Visco3D V)
+{
+    constexpr int TZ = RZ * BLOCK_Y, K = 2 * H + 1;
+    using RXZ = Region<H, H, TZ>;
+    __shared__ float s_hx[RXZ::RHP][RXZ::RW];
+    __shared__ float s_hy[RXZ::RHP][RXZ::RW];
+    __shared__ float s_hz[RXZ::RHP][RXZ::RW];
+    __shared__ ZTab<TZ, K> s_zt;
+    __shared__ float s_yt[NTH];
+    const Pos q = position(L, TZ, nyc, LY, A);
+    const long long so = (long long)q.shot * L.vol;
+    const int tid = threadIdx.y * TILE_X + threadIdx.x, tx = threadIdx.x;
+    const int wz = __builtin_amdgcn_readfirstlane(threadIdx.y);
+    const float *hx = F.f[0] + so, *hy = F.f[1] + so, *hz = F.f[2] + so;
+    float *const ee[3] = {F.f[3] + so, F.f[4] + so, F.f[5] + so};
+    const int x = q.x0 + tx;
+    const bool xin = x < L.NX;
+    float cxb[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) cxb[k] = T.xb[(long long)x * K + k];
+    const int xi = (P.on && xin) ? P.xi[x] : -1;
+    float ax = 0.f, bx = 0.f, axh = 0.f, bxh = 0.f;
+    if (xi >= 0) {
+        ax = P.ax[x];
+        bx = P.bx[x];
+        axh = P.axh[x];
+        bxh = P.bxh[x];
+    }
+    unsigned oc[RZ];
+#pragma unroll
+    for (int r = 0; r < RZ; r++) oc[r] = (unsigned)(((q.z0 + wz + r * BLOCK_Y + L.H) * L.pitch + x + XPAD) * 4);
+    RegionOff<H, H, TZ> ro;
+    ro.init(L, q.x0, q.z0, tid);
+    // y queues: hx (hxy), hz (hzy)
+    float qx[K][RZ], qz[K][RZ];
+#pragma unroll
+    for (int k = 0; k < 2 * H; k++) {
+        const auto b0 = plane_rsrc(hx, L, q.ys - H + k), b1 = plane_rsrc(hz, L, q.ys - H + k);
+#pragma unroll
+        for (int r = 0; r < RZ; r++) {
+            qx[k + 1][r] = bload(b0, oc[r], 0);
+            qz[k + 1][r] = bload(b1, oc[r], 0);
+        }
+    }
+    const bool tfr = tile_in_frame(L, P, q.x0, q.z0, TZ);
+    s_zt.fill(T, P, L, q.z0, tid);
+    PsiRes3D ps;  // x frame: 9 hzx, 10 hyx | y frame: 6 hzy, 11 hxy | z frame: 7 hyz, 8 hxz
+    ps.init(P, L, q.shot, 9, 10, 10, 6, 11, 11, 7, 8, 8);
+    const int W2 = 2 * P.W;
+    RXZ rx, ry, rzr;
+    for (int y = q.ys; y < q.ye; y++) {
+        // the y operators of the E update are the backward ones (Dyb in the first K entries)
+        const float yt = ytab_elem<K>(T.ys, y, tid);
+        load_region(rx, ro, hx, L, y);
+        load_region(ry, ro, hy, L, y);
+        load_region(rzr, ro, hz, L, y);
+        float nx[RZ], nz[RZ], o[3][RZ], ca[3][RZ], cb[3][RZ], d[RZ];
+        {
+            const auto b0 = plane_rsrc(hx, L, y + H), b1 = plane_rsrc(hz, L, y + H);
+#pragma unroll
+            for (int r = 0; r < RZ; r++) {
+                nx[r] = bload(b0, oc[r], 0);
+                nz[r] = bload(b1, oc[r], 0);
+            }
+        }
+#pragma unroll
+        for (int f = 0; f < 3; f++) {
+            const auto b = plane_rsrc(ee[f], L, y), ba = plane_rsrc(C.em[2 * f], L, y),
+                       bb = plane_rsrc(C.em[2 * f + 1], L, y);
+#pragma unroll
+            for (int r = 0; r < RZ; r++) {
+                o[f][r] = bload(b, oc[r], 0);
+                ca[f][r] = bload(ba, oc[r], 0);
+                cb[f][r] = bload(bb, oc[r], 0);
+            }
+        }
+        if (C.damp) {
+            const auto b = plane_rsrc(C.damp, L, y);
+#pragma unroll
+            for (int r = 0; r < RZ; r++) d[r] = bload(b, oc[r], 0);
+        } else {
+#pragma unroll
+            for (int r = 0; r < RZ; r++) d[r] = 1.f;
+        }
+        shift<H, RZ>(qx);
+        shift<H, RZ>(qz);
+#pragma unroll
+        for (int r = 0; r < RZ; r++) {
+            qx[2 * H][r] = nx[r];
+            qz[2 * H][r] = nz[r];
+        }
+        rx.store_padded(s_hx, tid);
+        ry.store_padded(s_hy, tid);
+        rzr.store_padded(s_hz, tid);
+        s_yt[tid] = yt;
+        block_sync_lds();
+        const int yi = __builtin_amdgcn_readfirstlane(P.on ? __float_as_int(s_yt[2 * K]) : -1);
+        const bool fr = __builtin_amdgcn_readfirstlane((int)(tfr || yi >= 0)) != 0;
+        const float *cyb = s_yt;
+        const float ay = yi >= 0 ? s_yt[2 * K + 1] : 0.f, by = yi >= 0 ? s_yt[2 * K + 2] : 0.f;
+#pragma unroll
+        for (int r = 0; r < RZ; r++) {
+            const int lz = wz + r * BLOCK_Y;
+            const int z = q.z0 + lz;
+            if (z >= L.NZ || !xin) continue;
+            const float *czb = s_zt.t[0][lz];
+            float hzy = ddy<H, RZ>(qz, r, cyb);
+            float hxy = ddy<H, RZ>(qx, r, cyb);
+            float hzx = ddx<H>(s_hz, lz + H, tx, cxb);
+            float hyx = ddx<H>(s_hy, lz + H, tx, cxb);
+            float hyz = ddz<H>(s_hy, lz, tx + H, czb);
+            float hxz = ddz<H>(s_hx, lz, tx + H, czb);
+            if (fr) {
+                const unsigned ox = psi_ox(L, W2, xi, y, z, true), oy = psi_oy(L, yi, x, z, true),
+                               oz = psi_oz(L, W2, s_zt.zi[lz], x, y, true);
+                const float az = s_zt.c[lz][0], bz = s_zt.c[lz][1];
+                bstore(cpml(hzx, bload(ps.x[0], ox, 0), ax, bx), ps.x[0], ox, 0);
+                bstore(cpml(hyx, bload(ps.x[1], ox, 0), axh, bxh), ps.x[1], ox, 0);
+                bstore(cpml(hzy, bload(ps.y[0], oy, 0), ay, by), ps.y[0], oy, 0);
+                bstore(cpml(hxy, bload(ps.y[1], oy, 0), ay, by), ps.y[1], oy, 0);
+                bstore(cpml(hyz, bload(ps.z[0], oz, 0), az, bz), ps.z[0], oz, 0);
+                bstore(cpml(hxz, bload(ps.z[1], oz, 0), az, bz), ps.z[1], oz, 0);
+            }
+            float u[3] = {hzy - hyz, hxz - hzx, hyx - hxy};
+            if constexpr (VE) {
+#pragma unroll
+                for (int f = 0; f < 3; f++)
+#pragma unroll
+                    for (int l = 0; l < 4; l++)
+                        if (l < V.L) {
+                            const auto br = plane_rsrc(V.R[4 * f + l] + so, L, y);
+                            const float a = V.cc[l] * bload(br, oc[r], 0);
+                            const float b = bload(plane_rsrc(V.cd[4 * f + l], L, y), oc[r], 0) * o[f][r];
+                            const float rn = b + a;
+                            bstore(rn, br, oc[r], 0);
+                            u[f] -= V.dt * rn;
+                        }
+            }
+#pragma unroll
+            for (int f = 0; f < 3; f++) {
+                u[f] *= cb[f][r];
+                float e = ca[f][r] * o[f][r] + u[f];
+                bstore(e * d[r], plane_rsrc(ee[f], L, y), oc[r], 0);
+            }
+        }
+        block_sync_lds();
+    }
+    acquisition3d<ACQ_EM>(A, F, so, q.sb, A.tbase[0] + tofs, false);
+}
+
+template <int MODE>
 __global__ __launch_bounds__(256) void flush3d(Layout3D L, Fields3D F, Acq A, int tofs)
 {
     const int shot = blockIdx.x;
@@ -1012,18 +1453,20 @@ __global__ __launch_bounds__(256) void flush3d(Layout3D L, Fields3D F, Acq A, in
     if (t < 0 || t >= A.NT) return;
     if (threadIdx.x == 0) {
         const int s0 = A.src_ptr[shot], s1 = A.src_ptr[shot + 1];
-        for (int ty = 2; ty <= 4; ty++)
+        for (int ty = 2; ty <= 4; ty++) {
+            if (!velocity_type<MODE>(ty)) continue;
             for (int s = s0; s < s1; s++) {
                 if (A.src_type[s] != ty) continue;
-                F.f[ty - 2][so + A.src_cell[s]] += A.src_sig[(long long)s * A.NT + t];
+                F.f[type_field<MODE>(ty)][so + A.src_cell[s]] += A.src_sig[(long long)s * A.NT + t];
             }
+        }
     }
     __syncthreads();
     const int r0 = A.rec_ptr[shot], r1 = A.rec_ptr[shot + 1];
     for (int r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
         const int ty = A.rec_type[r];
-        if (!velocity_type(ty)) continue;
-        A.seis[(long long)t * A.rec_total + r] = F.f[ty - 2][so + A.rec_cell[r]];
+        if (!velocity_type<MODE>(ty)) continue;
+        A.seis[(long long)t * A.rec_total + r] = F.f[type_field<MODE>(ty)][so + A.rec_cell[r]];
     }
 }
 
@@ -1049,51 +1492,54 @@ void go(Kern k, dim3 grid, dim3 block, hipStream_t st, hipEvent_t e0, hipEvent_t
 template <int H, int RZ>
 void launch_h3(int which, const Layout3D &L, int nshots, int nyc, const Fields3D &F, const Coef3D &C,
                const Taps3D &T, const Cpml3D &P, const Acq &A, int tofs, int pend, hipStream_t st, hipEvent_t e0,
-               hipEvent_t e1, const Rec3D *R)
+               hipEvent_t e1, const Rec3D *R, const Visco3D *V)
 {
     constexpr int TZ = RZ * BLOCK_Y;
     const int LY = (L.NY + nyc - 1) / nyc;
     dim3 grid((L.NX + TILE_X - 1) / TILE_X, (L.NZ + TZ - 1) / TZ, nyc * nshots), block(TILE_X, BLOCK_Y);
     const Rec3D r = R ? *R : Rec3D{};
+    const Visco3D v = V ? *V : Visco3D{};
     if (which == 0) {
         if (R)
             go(el3d_velocity<H, RZ, true>, grid, block, st, e0, e1, L, F, C, T, P, A, nyc, LY, tofs, pend, r);
         else
             go(el3d_velocity<H, RZ, false>, grid, block, st, e0, e1, L, F, C, T, P, A, nyc, LY, tofs, pend, r);
     } else {
-        if (R)
-            go(el3d_stress<H, RZ, true>, grid, block, st, e0, e1, L, F, C, T, P, A, nyc, LY, tofs, r);
+        if (V)
+            go(el3d_stress<H, RZ, false, true>, grid, block, st, e0, e1, L, F, C, T, P, A, nyc, LY, tofs, r, v);
+        else if (R)
+            go(el3d_stress<H, RZ, true>, grid, block, st, e0, e1, L, F, C, T, P, A, nyc, LY, tofs, r, v);
         else
-            go(el3d_stress<H, RZ, false>, grid, block, st, e0, e1, L, F, C, T, P, A, nyc, LY, tofs, r);
+            go(el3d_stress<H, RZ, false>, grid, block, st, e0, e1, L, F, C, T, P, A, nyc, LY, tofs, r, v);
     }
 }
 
 template <int H>
 void launch_rz(int rz, int which, const Layout3D &L, int nshots, int nyc, const Fields3D &F, const Coef3D &C,
                const Taps3D &T, const Cpml3D &P, const Acq &A, int tofs, int pend, hipStream_t st, hipEvent_t e0,
-               hipEvent_t e1, const Rec3D *R)
+               hipEvent_t e1, const Rec3D *R, const Visco3D *V)
 {
     if (rz == 1)
-        launch_h3<H, 1>(which, L, nshots, nyc, F, C, T, P, A, tofs, pend, st, e0, e1, R);
+        launch_h3<H, 1>(which, L, nshots, nyc, F, C, T, P, A, tofs, pend, st, e0, e1, R, V);
     else if (rz == 4)
-        launch_h3<H, 4>(which, L, nshots, nyc, F, C, T, P, A, tofs, pend, st, e0, e1, R);
+        launch_h3<H, 4>(which, L, nshots, nyc, F, C, T, P, A, tofs, pend, st, e0, e1, R, V);
     else
-        launch_h3<H, 2>(which, L, nshots, nyc, F, C, T, P, A, tofs, pend, st, e0, e1, R);
+        launch_h3<H, 2>(which, L, nshots, nyc, F, C, T, P, A, tofs, pend, st, e0, e1, R, V);
 }
 
 }  // namespace
 
 void launch_el3d(int which, int order, int rz, const Layout3D &L, int nshots, int nyc, const Fields3D &F,
                  const Coef3D &C, const Taps3D &T, const Cpml3D &P, const Acq &A, int tofs, int pend,
-                 hipStream_t stream, hipEvent_t e0, hipEvent_t e1, const Rec3D *R)
+                 hipStream_t stream, hipEvent_t e0, hipEvent_t e1, const Rec3D *R, const Visco3D *V)
 {
     switch (order / 2) {
-    case 1: launch_rz<1>(rz, which, L, nshots, nyc, F, C, T, P, A, tofs, pend, stream, e0, e1, R); break;
-    case 2: launch_rz<2>(rz, which, L, nshots, nyc, F, C, T, P, A, tofs, pend, stream, e0, e1, R); break;
-    case 3: launch_rz<3>(rz, which, L, nshots, nyc, F, C, T, P, A, tofs, pend, stream, e0, e1, R); break;
-    case 4: launch_rz<4>(rz, which, L, nshots, nyc, F, C, T, P, A, tofs, pend, stream, e0, e1, R); break;
-    case 5: launch_rz<5>(rz, which, L, nshots, nyc, F, C, T, P, A, tofs, pend, stream, e0, e1, R); break;
-    case 6: launch_rz<6>(rz, which, L, nshots, nyc, F, C, T, P, A, tofs, pend, stream, e0, e1, R); break;
+    case 1: launch_rz<1>(rz, which, L, nshots, nyc, F, C, T, P, A, tofs, pend, stream, e0, e1, R, V); break;
+    case 2: launch_rz<2>(rz, which, L, nshots, nyc, F, C, T, P, A, tofs, pend, stream, e0, e1, R, V); break;
+    case 3: launch_rz<3>(rz, which, L, nshots, nyc, F, C, T, P, A, tofs, pend, stream, e0, e1, R, V); break;
+    case 4: launch_rz<4>(rz, which, L, nshots, nyc, F, C, T, P, A, tofs, pend, stream, e0, e1, R, V); break;
+    case 5: launch_rz<5>(rz, which, L, nshots, nyc, F, C, T, P, A, tofs, pend, stream, e0, e1, R, V); break;
+    case 6: launch_rz<6>(rz, which, L, nshots, nyc, F, C, T, P, A, tofs, pend, stream, e0, e1, R, V); break;
     default: break;
     }
 }
@@ -1141,15 +1587,63 @@ void launch_ac3d(int which, int order, int rz, const Layout3D &L, int nshots, in
     }
 }
 
+namespace {
+template <int H, int RZ>
+void launch_em3d_h(int which, const Layout3D &L, int nshots, int nyc, const Fields3D &F, const Coef3D &C,
+                   const Taps3D &T, const Cpml3D &P, const Acq &A, int tofs, int pend, hipStream_t st, hipEvent_t e0,
+                   hipEvent_t e1, const Visco3D *V)
+{
+    constexpr int TZ = RZ * BLOCK_Y;
+    const int LY = (L.NY + nyc - 1) / nyc;
+    dim3 grid((L.NX + TILE_X - 1) / TILE_X, (L.NZ + TZ - 1) / TZ, nyc * nshots), block(TILE_X, BLOCK_Y);
+    if (which == 0)
+        go(em3d_h<H, RZ>, grid, block, st, e0, e1, L, F, C, T, P, A, nyc, LY, tofs, pend);
+    else if (V)
+        go(em3d_e<H, RZ, true>, grid, block, st, e0, e1, L, F, C, T, P, A, nyc, LY, tofs, *V);
+    else
+        go(em3d_e<H, RZ, false>, grid, block, st, e0, e1, L, F, C, T, P, A, nyc, LY, tofs, Visco3D{});
+}
+template <int H>
+void launch_em3d_rz(int rz, int which, const Layout3D &L, int nshots, int nyc, const Fields3D &F, const Coef3D &C,
+                    const Taps3D &T, const Cpml3D &P, const Acq &A, int tofs, int pend, hipStream_t st, hipEvent_t e0,
+                    hipEvent_t e1, const Visco3D *V)
+{
+    if (rz == 1)
+        launch_em3d_h<H, 1>(which, L, nshots, nyc, F, C, T, P, A, tofs, pend, st, e0, e1, V);
+    else if (rz == 4)
+        launch_em3d_h<H, 4>(which, L, nshots, nyc, F, C, T, P, A, tofs, pend, st, e0, e1, V);
+    else
+        launch_em3d_h<H, 2>(which, L, nshots, nyc, F, C, T, P, A, tofs, pend, st, e0, e1, V);
+}
+}  // namespace
+
+void launch_em3d(int which, int order, int rz, const Layout3D &L, int nshots, int nyc, const Fields3D &F,
+                 const Coef3D &C, const Taps3D &T, const Cpml3D &P, const Acq &A, int tofs, int pend,
+                 hipStream_t stream, hipEvent_t e0, hipEvent_t e1, const Visco3D *V)
+{
+    switch (order / 2) {
+    case 1: launch_em3d_rz<1>(rz, which, L, nshots, nyc, F, C, T, P, A, tofs, pend, stream, e0, e1, V); break;
+    case 2: launch_em3d_rz<2>(rz, which, L, nshots, nyc, F, C, T, P, A, tofs, pend, stream, e0, e1, V); break;
+    case 3: launch_em3d_rz<3>(rz, which, L, nshots, nyc, F, C, T, P, A, tofs, pend, stream, e0, e1, V); break;
+    case 4: launch_em3d_rz<4>(rz, which, L, nshots, nyc, F, C, T, P, A, tofs, pend, stream, e0, e1, V); break;
+    case 5: launch_em3d_rz<5>(rz, which, L, nshots, nyc, F, C, T, P, A, tofs, pend, stream, e0, e1, V); break;
+    case 6: launch_em3d_rz<6>(rz, which, L, nshots, nyc, F, C, T, P, A, tofs, pend, stream, e0, e1, V); break;
+    default: break;
+    }
+}
+
 void launch_abs_velocity3d(const Layout3D &L, int nshots, const Fields3D &F, const Coef3D &C, hipStream_t stream)
 {
     const long long n = (long long)nshots * L.vol;
     abs_velocity3d<<<2048, 256, 0, stream>>>(n, F.f[0], F.f[1], F.f[2], C.damp, L.vol);
 }
 
-void launch_flush3d(const Layout3D &L, const Fields3D &F, const Acq &A, int tofs, hipStream_t stream)
+void launch_flush3d(const Layout3D &L, const Fields3D &F, const Acq &A, int tofs, hipStream_t stream, bool em)
 {
-    flush3d<<<A.nshots, 256, 0, stream>>>(L, F, A, tofs);
+    if (em)
+        flush3d<ACQ_EM><<<A.nshots, 256, 0, stream>>>(L, F, A, tofs);
+    else
+        flush3d<ACQ_ELASTIC><<<A.nshots, 256, 0, stream>>>(L, F, A, tofs);
 }
 
 }  // namespace wv

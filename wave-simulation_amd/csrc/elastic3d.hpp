@@ -30,7 +30,7 @@ struct Layout3D {
 };
 
 struct Fields3D {
-    float *f[9];  // vx vy vz sxx syy szz syz sxz sxy
+    float *f[9];  // vx vy vz sxx syy szz syz sxz sxy (EM: hx hy hz ex ey ez)
 };
 
 struct Coef3D {
@@ -39,6 +39,9 @@ struct Coef3D {
     const float *damp;      // ABS factors (layout of one field) or nullptr
     const float *fsH, *fsV;  // free-surface scales on the y = 0 plane ((z + H) * pitch + x + XPAD) or nullptr
     int fs;                  // FreeSurface = 1 (acoustic: p = 0 on the y = 0 plane)
+    // EM: irx, iry, irz hold 1/mu averaged in the yz, xz, xy planes (for hx, hy, hz) and em[] the
+    // Ca, Cb of the x-, y- and z-averaged permittivity and conductivity (for ex, ey, ez)
+    const float *em[6];
 };
 
 struct Taps3D {
@@ -75,25 +78,53 @@ struct Rec3D {
     float *r[9];
 };
 
+// Viscoelastic stress update (ForwardSolver3Dviscoelastic.cpp:270-444): per cell tauP, tauS, the
+// 4-point averages of tauS with the matrices of mu_xy, mu_xz, mu_yz, 1 + L tauP, 1 + L tauS
+// (layout of one field, shared by the shots), the memory variables R[6 * 4] (group g of
+// xx yy zz yz xz xy, mechanism l at 4 g + l; layout of the fields), per mechanism the
+// constants 1 / tau_l, c1 = 1 - DT / (2 tau_l), c2 = 1 / (1 + DT / (2 tau_l)), and the
+// free-surface relaxation updates on the y = 0 plane (layout of Coef3D::fsH).
+struct Visco3D {
+    int L;
+    const float *tauP, *tauS, *tSxy, *tSxz, *tSyz, *opP, *opS;
+    float *R[24];
+    const float *fsRH[4], *fsRV[4];
+    float itau[4], c1[4], c2[4];
+    float dth;
+    // visco-EM: the memory variables of ex, ey, ez are R[4 g + l] (g = 0, 1, 2), their Cd_l per
+    // cell (x-, y-, z-averaged, DT included) cd[4 g + l], Cc_l and DT
+    const float *cd[12];
+    float cc[4], dt;
+};
+
 // One time step = velocity launch + stress launch (+ ABS velocity damping).
 // rz = rows in z per thread (tile height 4 * rz), nyc = chunks along y; R (optional)
-// receives this step's record.
+// receives this step's record; V (optional) makes the stress launch the viscoelastic one.
 void launch_el3d(int which, int order, int rz, const Layout3D &L, int nshots, int nyc, const Fields3D &F,
                  const Coef3D &C, const Taps3D &T, const Cpml3D &P, const Acq &A, int tofs, int pend,
-                 hipStream_t stream, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr, const Rec3D *R = nullptr);
+                 hipStream_t stream, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr, const Rec3D *R = nullptr,
+                 const Visco3D *V = nullptr);
 // 3D acoustic step (fields vx vy vz p, memory variables 0..5 of Cpml3D::ps): which = 0 velocity, 1 pressure
 void launch_ac3d(int which, int order, int rz, const Layout3D &L, int nshots, int nyc, const Fields3D &F,
                  const Coef3D &C, const Taps3D &T, const Cpml3D &P, const Acq &A, int tofs, int pend,
                  hipStream_t stream, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 void launch_abs_velocity3d(const Layout3D &L, int nshots, const Fields3D &F, const Coef3D &C, hipStream_t stream);
-void launch_flush3d(const Layout3D &L, const Fields3D &F, const Acq &A, int tofs, hipStream_t stream);
+void launch_flush3d(const Layout3D &L, const Fields3D &F, const Acq &A, int tofs, hipStream_t stream, bool em = false);
+// 3D EM step (ForwardSolver3Demem.cpp:117-235; fields hx hy hz ex ey ez, CPML memory variables
+// 0..11 of Cpml3D::ps as the EM terms): which = 0 H update, 1 E update (+ ABS of E); V (optional)
+// adds the memory variables of the visco-EM solver (ForwardSolver3Dviscoemem.cpp:170-305)
+void launch_em3d(int which, int order, int rz, const Layout3D &L, int nshots, int nyc, const Fields3D &F,
+                 const Coef3D &C, const Taps3D &T, const Cpml3D &P, const Acq &A, int tofs, int pend,
+                 hipStream_t stream, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr, const Visco3D *V = nullptr);
 
 // Host driver of the 3D elastic solver (owned by a wv_solver handle).
 class Elastic3D {
   public:
     ~Elastic3D();
-    int init(const wv_config &cfg, hipStream_t stream, int physics = ELASTIC);
-    int set_model(const float *vp, const float *vs, const float *rho);
+    int init(const wv_config &cfg, hipStream_t stream, int physics = ELASTIC, bool visco = false);
+    // viscoelastic (3D): relaxation mechanisms before set_model (tauP, tauS in grid order)
+    int set_relaxation(int nmech, const double *freqs, const float *tauP, const float *tauS);
+    int set_model(const float *vp, const float *vs, const float *rho);  // EM: eps_r, sigma, mu_r
     int set_acquisition(int shot, const std::vector<long long> &sidx, const std::vector<int> &stype,
                         const std::vector<float> &ssig, const std::vector<long long> &ridx,
                         const std::vector<int> &rtype);
@@ -129,6 +160,8 @@ class Elastic3D {
     Adj3State *adj3 = nullptr;
     std::vector<float> model_h[3];  // vp, vs, rho (chain rule of the gradient)
     int run_segment(int t_start, int t_end, bool timed);
+    int set_model_em(const float *m0, const float *m1, const float *m2);
+    int psi_axis(int q) const;
     int checkpoint_interval() const;
     int enqueue_rec(int nsteps, float *rec);
     int save_checkpoint(int k);
@@ -152,15 +185,29 @@ class Elastic3D {
     Taps3D taps() const;
     Cpml3D cpml() const;
     Acq acq() const;
+    Visco3D visco3d() const;
 
     hipStream_t st = nullptr;
     int NX = 0, NY = 0, NZ = 0, NT = 0, H = 0, order = 0, nshots = 1;
     int damping = 0, free_surface = 0, W = 0, steps_per_graph = 0;
     bool ops_stencil = false;
     int physics = ELASTIC;
-    int nf = 9;    // fields: elastic vx vy vz sxx syy szz syz sxz sxy, acoustic vx vy vz p
-    int nps = 18;  // CPML memory variables (acoustic: p_x p_y p_z vx_x vy_y vz_z)
-    double DT = 0, DH = 0;
+    // viscoelastic: relaxation mechanisms and their device data
+    bool visco = false;
+    int vL = 0;
+    std::vector<double> vfreq;
+    std::vector<float> vtauP, vtauS;  // grid order
+    float *vR[24] = {};               // memory variables (Visco3D::R)
+    float *vcoef[7] = {};             // tauP, tauS, tauS_xy, tauS_xz, tauS_yz, 1 + L tauP, 1 + L tauS
+    float *vfs[8] = {};               // free-surface relaxation updates: horizontal[4], vertical[4]
+    float vitau[4] = {}, vc1[4] = {}, vc2[4] = {}, vdth = 0.f;
+    // EM: Ca, Cb of the x-, y-, z-averaged values (Coef3D::em); visco-EM: Cd_l per axis, Cc_l, DT
+    float *emc[6] = {};
+    float *vcd[12] = {};
+    float vcc[4] = {}, vdt = 0.f;
+    int nf = 9;    // fields: elastic vx vy vz sxx syy szz syz sxz sxy, acoustic vx vy vz p, EM hx hy hz ex ey ez
+    int nps = 18;  // CPML memory variables (acoustic: p_x p_y p_z vx_x vy_y vz_z; EM: 12)
+    double DT = 0, DH = 0, fc_ref = 0;  // fc_ref: CenterFrequencyCPML, the viscoelastic reference frequency
     int rz = 2, nyc = 1, nbx = 0, nbz = 0;
     Layout3D L{};
     std::vector<void *> allocs, acq_allocs;

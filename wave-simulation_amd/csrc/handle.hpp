@@ -76,6 +76,9 @@ struct wv_solver {
     float *vcoef[7] = {};             // tauP, tauS, tauS_xy, 1 + L tauP, 1 + L tauS, tauS_xz, tauS_yz
     float *vfs[8] = {};               // free-surface relaxation updates: horizontal[4], vertical[4]
     float vitau[4] = {}, vc1[4] = {}, vc2[4] = {}, vdth = 0.f;
+    // visco-EM: Cd_l per cell (TM: node; TE: x-averaged at l, y-averaged at 4 + l), Cc_l, DT
+    float *vcd[8] = {};
+    float vcc[4] = {}, vdt = 0.f;
     float *cpx[4] = {}, *cpy[4] = {};
     float *psv[2][4] = {};  // ping-pong: velocity-derivative memory variables psx0 psx1 psy0 psy1
     float *pss[4] = {};     // stress-derivative memory variables psx2 psx3 psy2 psy3
@@ -210,6 +213,9 @@ struct wv_solver {
             V.c2[l] = vc2[l];
         }
         V.dth = vdth;
+        for (int k = 0; k < 8; k++) V.cd[k] = vcd[k];
+        for (int l = 0; l < 4; l++) V.cc[l] = vcc[l];
+        V.dt = vdt;
         return V;
     }
     wv::Cpml2D cpml2d(int b) const

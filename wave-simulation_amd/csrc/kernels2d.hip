@@ -443,9 +443,11 @@ __global__ __launch_bounds__(NTHREADS) void tm2d_h(Layout2D L, Fields2D F, Coef2
     }
 }
 
-template <int H, int TY>
+// VE: visco TM-EM (ForwardSolver2Dviscotmem.cpp:172-199): per mechanism r_l = Cd_l ez + Cc_l r_l
+// from the ez of the previous step, then ez = Ca ez + Cb ((Dxb hy - Dyb hx) - sum_l DT r_l)
+template <int H, int TY, bool VE = false>
 __global__ __launch_bounds__(NTHREADS) void tm2d_e(Layout2D L, Fields2D F, Coef2D C, Taps T, Cpml2D P, Acq A,
-                                                   int tofs, int pend)
+                                                   int tofs, int pend, Visco2D V)
 {
     (void)pend;
     using TL = Tile<H, TY>;
@@ -508,6 +510,18 @@ __global__ __launch_bounds__(NTHREADS) void tm2d_e(Layout2D L, Fields2D F, Coef2
         if (col.xi >= 0) P.psx[1][((long long)shot * L.NY + y) * W2 + col.xi] = cpml(u, p1[r], col.ax, col.bx);
         if (yi[r] >= 0) P.psy[1][((long long)shot * W2 + yi[r]) * L.NX + col.x] = cpml(t, q1[r], s_cy[ly][0], s_cy[ly][1]);
         u -= t;
+        if constexpr (VE) {
+            const long long c0 = L.at(col.x, y), o = so + c0;
+#pragma unroll
+            for (int l = 0; l < 4; l++)
+                if (l < V.L) {
+                    const float a = V.cc[l] * V.R[l][o];
+                    const float b = V.cd[l][c0] * oez[r];
+                    const float rn = b + a;
+                    V.R[l][o] = rn;
+                    u -= V.dt * rn;
+                }
+        }
         u *= cb[r];
         float e = ca[r] * oez[r] + u;
         e *= dmp[r];
@@ -703,9 +717,12 @@ __global__ __launch_bounds__(NTHREADS) void te2d_h(Layout2D L, Fields2D F, Coef2
     }
 }
 
-template <int H, int TY>
+// VE: visco TE-EM (ForwardSolver2Dviscoemem.cpp:175-218): r_l = Cd_l e + Cc_l r_l for ex and ey
+// (x- and y-averaged Cd), then ex = Ca_x ex + Cb_x (CPML(Dyb hz) - sum_l DT rx_l),
+// ey = Ca_y ey + Cb_y (-CPML(Dxb hz) - sum_l DT ry_l)
+template <int H, int TY, bool VE = false>
 __global__ __launch_bounds__(NTHREADS) void te2d_e(Layout2D L, Fields2D F, Coef2D C, Taps T, Cpml2D P, Acq A,
-                                                   int tofs, int pend)
+                                                   int tofs, int pend, Visco2D V)
 {
     (void)pend;
     using TL = Tile<H, TY>;
@@ -755,6 +772,33 @@ __global__ __launch_bounds__(NTHREADS) void te2d_e(Layout2D L, Fields2D F, Coef2
         const long long o = L.at(col.x, y);
         const float dmp = C.damp ? C.damp[o] : 1.f;
         // ex = Ca_x ex + Cb_x CPML(Dyb hz) ; ey = Ca_y ey - Cb_y CPML(Dxb hz)
+        if constexpr (VE) {
+            const long long os = so + o;
+            float v = -w;
+#pragma unroll
+            for (int l = 0; l < 4; l++)
+                if (l < V.L) {
+                    float a = V.cc[l] * V.R[l][os];
+                    float b = V.cd[l][o] * oex[r];
+                    const float rx = b + a;
+                    a = V.cc[l] * V.R[4 + l][os];
+                    b = V.cd[4 + l][o] * oey[r];
+                    const float ry = b + a;
+                    V.R[l][os] = rx;
+                    V.R[4 + l][os] = ry;
+                    u -= V.dt * rx;
+                    v -= V.dt * ry;
+                }
+            u *= C.c[2][o];
+            float e = C.c[1][o] * oex[r];
+            e += u;
+            ex[o] = e * dmp;
+            v *= C.c[4][o];
+            float f = C.c[3][o] * oey[r];
+            f += v;
+            ey[o] = f * dmp;
+            continue;
+        }
         u *= C.c[2][o];
         float e = C.c[1][o] * oex[r];
         e += u;
@@ -847,12 +891,37 @@ void launch_h(int physics, int which, dim3 grid, const Layout2D &L, const Fields
         if (which == 0)
             go(te2d_h<H, TY>, grid, block, st, e0, e1, L, F, C, T, P, A, tofs, pend);
         else
-            go(te2d_e<H, TY>, grid, block, st, e0, e1, L, F, C, T, P, A, tofs, pend);
+            go(te2d_e<H, TY>, grid, block, st, e0, e1, L, F, C, T, P, A, tofs, pend, Visco2D{});
     } else {
         if (which == 0)
             go(tm2d_h<H, TY>, grid, block, st, e0, e1, L, F, C, T, P, A, tofs, pend);
         else
-            go(tm2d_e<H, TY>, grid, block, st, e0, e1, L, F, C, T, P, A, tofs, pend);
+            go(tm2d_e<H, TY>, grid, block, st, e0, e1, L, F, C, T, P, A, tofs, pend, Visco2D{});
+    }
+}
+
+template <int H>
+void launch_vem_ty(int ty, int physics, const Layout2D &L, int nshots, const Fields2D &F, const Coef2D &C,
+                   const Taps &T, const Cpml2D &P, const Acq &A, int tofs, const Visco2D &V, hipStream_t st,
+                   hipEvent_t e0, hipEvent_t e1)
+{
+    const int TY = ty == 8 ? 8 : (ty == 32 ? 32 : 16);
+    dim3 grid((L.NX + TILE_X - 1) / TILE_X, (L.NY + TY - 1) / TY, nshots), block(TILE_X, BLOCK_Y);
+    auto pick = [&](auto k) { go(k, grid, block, st, e0, e1, L, F, C, T, P, A, tofs, 0, V); };
+    if (physics == TEMEM) {
+        if (TY == 8)
+            pick(te2d_e<H, 8, true>);
+        else if (TY == 32)
+            pick(te2d_e<H, 32, true>);
+        else
+            pick(te2d_e<H, 16, true>);
+    } else {
+        if (TY == 8)
+            pick(tm2d_e<H, 8, true>);
+        else if (TY == 32)
+            pick(tm2d_e<H, 32, true>);
+        else
+            pick(tm2d_e<H, 16, true>);
     }
 }
 
@@ -884,6 +953,21 @@ void launch_step2d(int physics, int order, int tile_y, const Layout2D &L, int ns
     case 4: launch_ty2<4>(tile_y, physics, which, L, nshots, F, C, T, P, A, tofs, pend, stream, e0, e1, S); break;
     case 5: launch_ty2<5>(tile_y, physics, which, L, nshots, F, C, T, P, A, tofs, pend, stream, e0, e1, S); break;
     case 6: launch_ty2<6>(tile_y, physics, which, L, nshots, F, C, T, P, A, tofs, pend, stream, e0, e1, S); break;
+    default: break;
+    }
+}
+
+void launch_viscoem2d_e(int physics, int order, int tile_y, const Layout2D &L, int nshots, const Fields2D &F,
+                        const Coef2D &C, const Taps &T, const Cpml2D &P, const Acq &A, int tofs, const Visco2D &V,
+                        hipStream_t stream, hipEvent_t e0, hipEvent_t e1)
+{
+    switch (order / 2) {
+    case 1: launch_vem_ty<1>(tile_y, physics, L, nshots, F, C, T, P, A, tofs, V, stream, e0, e1); break;
+    case 2: launch_vem_ty<2>(tile_y, physics, L, nshots, F, C, T, P, A, tofs, V, stream, e0, e1); break;
+    case 3: launch_vem_ty<3>(tile_y, physics, L, nshots, F, C, T, P, A, tofs, V, stream, e0, e1); break;
+    case 4: launch_vem_ty<4>(tile_y, physics, L, nshots, F, C, T, P, A, tofs, V, stream, e0, e1); break;
+    case 5: launch_vem_ty<5>(tile_y, physics, L, nshots, F, C, T, P, A, tofs, V, stream, e0, e1); break;
+    case 6: launch_vem_ty<6>(tile_y, physics, L, nshots, F, C, T, P, A, tofs, V, stream, e0, e1); break;
     default: break;
     }
 }

@@ -186,6 +186,27 @@ void em_ca_cb(float eps, float sigma, float DT, float &ca, float &cb)
     cb = b;
 }
 
+void em_visco_coef(float eps_s, float sig_s, float tE, float tS, float DT, float inv_tau_mean, int L,
+                   const float *cdl, float &ca, float &cb, float *cd)
+{
+    const float eps0 = 8.8541878176e-12f;
+    float e = 1.f - tE;
+    e *= eps_s;
+    const float t = sig_s * tS;
+    e += t;
+    if (e < eps0) e = eps0;  // Common::searchAndReplace(..., eps0, eps0, 1)
+    float sg = tE * inv_tau_mean;
+    sg *= eps_s;
+    sg += sig_s;
+    em_ca_cb(e, sg, DT, ca, cb);
+    for (int l = 0; l < L; l++) {
+        float v = tE * cdl[l];
+        v *= eps_s;
+        v *= -DT;
+        cd[l] = v;
+    }
+}
+
 std::vector<float> transpose_taps(const std::vector<float> &t, int H)
 {
     const int K = 2 * H + 1;

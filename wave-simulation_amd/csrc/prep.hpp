@@ -46,6 +46,13 @@ void cpml_axis(int n, int padded_n, int W, bool skip_near_side, double npower, d
 // arithmetic and operation order of ForwardSolverEM.cpp:14-68 (the time loop multiplies ez
 // by Ca every step, so a Ca rounded differently drifts from the reference float build).
 void em_ca_cb(float eps, float sigma, float DT, float &ca, float &cb);
+// Visco-EM coefficients of one cell (ForwardSolverEM.cpp:70-157) from the static permittivity
+// eps_s [F/m], conductivity sig_s, tau of the permittivity tE and of the conductivity tS [s]:
+// the effective optical values eps_opt = max((1 - tE) eps_s + sig_s tS, eps0) and
+// sig_opt = tE inv_tau_mean eps_s + sig_s give Ca, Cb; cd[l] = tE cdl[l] eps_s (-DT) with
+// cdl[l] = (1 / (1 + DT / (2 tau_l))) / (L tau_l^2).  Float steps of the reference's float build.
+void em_visco_coef(float eps_s, float sig_s, float tE, float tS, float DT, float inv_tau_mean, int L,
+                   const float *cdl, float &ca, float &cb, float *cd);
 
 // ABS damping factor of one grid cell (ABS2D.cpp:110-178, ABS3D.cpp:150-215).
 std::vector<double> abs_profile(int W, double damping_coeff);

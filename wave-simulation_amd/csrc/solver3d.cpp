@@ -30,6 +30,15 @@ int round_up(int v, int m) { return (v + m - 1) / m * m; }
         if (e_ != hipSuccess) return fail(WV_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
+// axis of CPML memory variable q (see Cpml3D; EM: 0 ezy 1 eyz 2 exz 3 ezx 4 eyx 5 exy 6 hzy 7 hyz
+// 8 hxz 9 hzx 10 hyx 11 hxy)
+int Elastic3D::psi_axis(int q) const
+{
+    static const int seismic[18] = {0, 1, 2, 0, 1, 2, 0, 1, 2, 0, 1, 2, 1, 0, 2, 0, 2, 1};
+    static const int em[12] = {1, 2, 2, 0, 0, 1, 1, 2, 2, 0, 0, 1};
+    return physics == TEMEM ? em[q] : seismic[q];
+}
+
 int Elastic3D::fail(int code, const std::string &m)
 {
     error = m;
@@ -101,12 +110,13 @@ int Elastic3D::upload_pos(T **p0, const std::vector<T> &h, int per_pos, T fill)
     return 0;
 }
 
-int Elastic3D::init(const wv_config &k, hipStream_t stream, int phys)
+int Elastic3D::init(const wv_config &k, hipStream_t stream, int phys, bool visco_)
 {
     st = stream;
     physics = phys;
-    nf = physics == ACOUSTIC ? 4 : 9;
-    nps = physics == ACOUSTIC ? 6 : 18;
+    visco = visco_ && (phys == ELASTIC || phys == TEMEM);
+    nf = physics == ACOUSTIC ? 4 : (physics == TEMEM ? 6 : 9);
+    nps = physics == ACOUSTIC ? 6 : (physics == TEMEM ? 12 : 18);
     if (k.NX < 2 || k.NY < 2 || k.NZ < 2) return fail(WV_EINVAL, "3D grid needs NX, NY, NZ >= 2");
     if (k.nshots < 1) return fail(WV_EINVAL, "nshots must be >= 1");
     NX = k.NX;
@@ -121,6 +131,7 @@ int Elastic3D::init(const wv_config &k, hipStream_t stream, int phys)
     W = k.boundary_width;
     DT = k.DT;
     DH = k.DH;
+    fc_ref = k.fc_cpml;
     steps_per_graph = std::max(0, k.steps_per_graph);
     if (damping && 2 * W > std::min(NX, std::min(NY, NZ))) return fail(WV_EINVAL, "BoundaryWidth out of range");
     if (k.tile_y != 0 && k.tile_y != 4 && k.tile_y != 8 && k.tile_y != 16)
@@ -188,9 +199,8 @@ int Elastic3D::init(const wv_config &k, hipStream_t stream, int phys)
         ps_n[1] = (size_t)nshots * 2 * W * NZ * NX;
         ps_n[2] = (size_t)nshots * NY * 2 * W * NX;
         // axis of each memory variable (see Cpml3D)
-        static const int axis_of[18] = {0, 1, 2, 0, 1, 2, 0, 1, 2, 0, 1, 2, 1, 0, 2, 0, 2, 1};
         for (int q = 0; q < nps; q++)
-            if ((rc = dalloc(&ps[q], ps_n[axis_of[q]]))) return rc;
+            if ((rc = dalloc(&ps[q], ps_n[psi_axis(q)]))) return rc;
     }
     {
         // per-plane y tables of the two kernels: taps, CPML y index and coefficients (see Taps3D)
@@ -274,20 +284,105 @@ int Elastic3D::set_rz(int r)
     return 0;
 }
 
+int Elastic3D::set_relaxation(int nmech, const double *freqs, const float *tauP, const float *tauS)
+{
+    const size_t n = (size_t)NX * NY * NZ;
+    vfreq.assign(freqs, freqs + nmech);
+    vtauP.assign(tauP, tauP + n);
+    vtauS.assign(tauS, tauS + n);
+    // per-mechanism constants in the reference's value type, step by step as
+    // ForwardSolver3Dviscoelastic.cpp:55-66 (they multiply the memory variables every step)
+    const float DTf = (float)DT;
+    for (int l = 0; l < 4; l++) vitau[l] = vc1[l] = vc2[l] = 0.f;
+    for (int l = 0; l < nmech; l++) {
+        const float tau = (float)(1.0 / (2.0 * M_PI * (double)(float)freqs[l]));
+        vitau[l] = (float)(1.0 / (double)tau);
+        vc1[l] = (float)(1.0 - (double)DTf / (2.0 * (double)tau));
+        vc2[l] = (float)(1.0 / (1.0 + (double)DTf / (2.0 * (double)tau)));
+    }
+    vdth = (float)((double)DTf / 2.0);
+    // visco-EM (ForwardSolverEM.cpp:90-100): Cc_l = (1 - DT / (2 tau_l)) / (1 + DT / (2 tau_l)), DT
+    for (int l = 0; l < 4; l++) vcc[l] = 0.f;
+    for (int l = 0; l < nmech; l++) {
+        const double tau = (double)(float)(1.0 / (2.0 * M_PI * (double)(float)freqs[l]));
+        vcc[l] = (float)((1.0 - 0.5 * (double)DTf / tau) / (1.0 + 0.5 * (double)DTf / tau));
+    }
+    vdt = DTf;
+    // memory-variable groups per mechanism (six stresses; EM: ex, ey, ez), laid out as the fields
+    // (one plane of slack, as they)
+    const int ngroups = physics == TEMEM ? 3 : 6;
+    for (int l = 0; l < nmech; l++)
+        for (int g = 0; g < ngroups; g++) {
+            float *&p = vR[4 * g + l];
+            if (!p) {
+                int rc = dalloc(&p, (size_t)L.vol * nshots + (size_t)L.slab);
+                if (rc) return rc;
+            }
+        }
+    vL = nmech;
+    model_ok = false;  // the moduli depend on tauP, tauS: set_model follows
+    drop_graph();
+    HIP3(hipStreamSynchronize(st));
+    return 0;
+}
+
+Visco3D Elastic3D::visco3d() const
+{
+    Visco3D V{};
+    V.L = vL;
+    for (int k = 0; k < 24; k++) V.R[k] = vR[k];
+    V.tauP = vcoef[0];
+    V.tauS = vcoef[1];
+    V.tSxy = vcoef[2];
+    V.tSxz = vcoef[3];
+    V.tSyz = vcoef[4];
+    V.opP = vcoef[5];
+    V.opS = vcoef[6];
+    for (int l = 0; l < 4; l++) {
+        V.fsRH[l] = vfs[l];
+        V.fsRV[l] = vfs[4 + l];
+        V.itau[l] = vitau[l];
+        V.c1[l] = vc1[l];
+        V.c2[l] = vc2[l];
+    }
+    V.dth = vdth;
+    for (int k = 0; k < 12; k++) V.cd[k] = vcd[k];
+    for (int l = 0; l < 4; l++) V.cc[l] = vcc[l];
+    V.dt = vdt;
+    return V;
+}
+
 int Elastic3D::set_model(const float *vp, const float *vs, const float *rho)
 {
     const size_t n = (size_t)NX * NY * NZ;
     auto gi = [&](int x, int y, int z) { return (size_t)x + (size_t)z * NX + (size_t)y * NX * NZ; };
     const bool ac = physics == ACOUSTIC;
+    if (visco && vL == 0) return fail(WV_ESTATE, "the viscoelastic model needs wv_set_relaxation first");
+    if (physics == TEMEM) return set_model_em(vp, vs, rho);
     model_h[0].assign(vp, vp + n);
     if (!ac) model_h[1].assign(vs, vs + n);
     model_h[2].assign(rho, rho + n);
+    // viscoelastic: the moduli given at the reference frequency CenterFrequencyCPML are scaled to
+    // the relaxed ones, M / (1 + sum tauP) and mu / (1 + sum tauS) with
+    // sum = sum_l w^2 ts_l^2 / (1 + w^2 ts_l^2), w = 2 pi fc, ts_l = 1 / (2 pi f_l)
+    // (Viscoelastic.cpp:650-700)
+    double vsum = 0.0;
+    if (visco) {
+        const double w = 2.0 * M_PI * fc_ref;
+        for (int l = 0; l < vL; l++) {
+            const double ts = 1.0 / (2.0 * M_PI * vfreq[l]);
+            vsum += w * w * ts * ts / (1.0 + w * w * ts * ts);
+        }
+    }
     std::vector<double> r(n), M(n), mu(n, 1.0), imu(n, 1.0);
     for (size_t i = 0; i < n; i++) {
         r[i] = rho[i];
         M[i] = (double)rho[i] * vp[i] * vp[i];
+        if (visco) M[i] /= 1.0 + vsum * vtauP[i];
         if (ac) continue;
-        mu[i] = std::max((double)rho[i] * vs[i] * vs[i], 1.0);  // ModelparameterSeismic.cpp:424
+        double m = (double)rho[i] * vs[i] * vs[i];
+        if (visco) m /= 1.0 + vsum * vtauS[i];
+        mu[i] = std::max(m, 1.0);  // ModelparameterSeismic.cpp:424
         imu[i] = 1.0 / mu[i];
     }
     std::vector<std::vector<float>> c(8, std::vector<float>((size_t)L.vol, 0.f));
@@ -341,21 +436,196 @@ int Elastic3D::set_model(const float *vp, const float *vs, const float *rho)
         if (!coef[q] && (rc = dalloc(&coef[q], (size_t)L.vol))) return rc;
         HIP3(hipMemcpyAsync(coef[q], c[q].data(), c[q].size() * sizeof(float), hipMemcpyHostToDevice, st));
     }
+    HIP3(hipStreamSynchronize(st));  // c is reused below
+    if (visco) {
+        // per cell tauP, tauS, tauS averaged with the 4-point matrices of mu_xy, mu_xz, mu_yz
+        // (Viscoelastic.cpp:562-576, Modelparameter.cpp:449-526), 1 + L tauP, 1 + L tauS
+        // (ForwardSolver3Dviscoelastic.cpp:105-118)
+        const int len[3] = {NX, NY, NZ};
+        static const int planes[3][2] = {{0, 1}, {0, 2}, {1, 2}};
+        for (int q = 0; q < 7; q++) std::fill(c[q].begin(), c[q].end(), 0.f);
+        for (int y = 0; y < NY; y++)
+            for (int z = 0; z < NZ; z++)
+                for (int x = 0; x < NX; x++) {
+                    const size_t i = gi(x, y, z), o = (size_t)L.at(x, y, z);
+                    const int pos[3] = {x, y, z};
+                    c[0][o] = vtauP[i];
+                    c[1][o] = vtauS[i];
+                    for (int pl = 0; pl < 3; pl++) {
+                        const int a0 = planes[pl][0], a1 = planes[pl][1];
+                        const bool in0 = pos[a0] + 1 < len[a0], in1 = pos[a1] + 1 < len[a1];
+                        int p0[3] = {x, y, z}, p1[3] = {x, y, z}, p2[3] = {x, y, z};
+                        p0[a0]++;
+                        p1[a1]++;
+                        p2[a0]++;
+                        p2[a1]++;
+                        double a;
+                        if (in0 && in1)
+                            a = 0.25 * ((double)vtauS[i] + vtauS[gi(p0[0], p0[1], p0[2])] +
+                                        vtauS[gi(p1[0], p1[1], p1[2])] + vtauS[gi(p2[0], p2[1], p2[2])]);
+                        else if (in0)
+                            a = 0.5 * ((double)vtauS[i] + vtauS[gi(p0[0], p0[1], p0[2])]);
+                        else if (in1)
+                            a = 0.5 * ((double)vtauS[i] + vtauS[gi(p1[0], p1[1], p1[2])]);
+                        else
+                            a = vtauS[i];
+                        c[2 + pl][o] = (float)a;
+                    }
+                    c[5][o] = (float)(1.0 + (double)vL * vtauP[i]);
+                    c[6][o] = (float)(1.0 + (double)vL * vtauS[i]);
+                }
+        for (int q = 0; q < 7; q++) {
+            if (!vcoef[q] && (rc = dalloc(&vcoef[q], (size_t)L.vol))) return rc;
+            HIP3(hipMemcpyAsync(vcoef[q], c[q].data(), c[q].size() * sizeof(float), hipMemcpyHostToDevice, st));
+        }
+        HIP3(hipStreamSynchronize(st));
+    }
     if (free_surface == 1 && !ac) {
-        // FreeSurfaceElastic.cpp:11-47: scales of the horizontal update at y = 0
-        std::vector<float> h((size_t)L.slab, 0.f), v((size_t)L.slab, 0.f);
+        // FreeSurfaceElastic.cpp:11-47: scales of the horizontal update at y = 0; viscoelastic
+        // (FreeSurfaceViscoelastic.cpp:10-60): with the unrelaxed moduli M (1 + L tauP),
+        // mu (1 + L tauS), and per mechanism the updates of Rxx, Rzz
+        std::vector<float> h((size_t)L.slab, 0.f), v((size_t)L.slab, 0.f), rh((size_t)4 * L.slab, 0.f),
+            rv((size_t)4 * L.slab, 0.f);
         for (int z = 0; z < NZ; z++)
             for (int x = 0; x < NX; x++) {
                 const size_t i = gi(x, 0, z), o = (size_t)(z + H) * L.pitch + x + XPAD;
-                const double t = M[i] - 2.0 * mu[i];
+                if (!visco) {
+                    const double t = M[i] - 2.0 * mu[i];
+                    v[o] = (float)t;
+                    h[o] = (float)(-(t * t) / M[i]);
+                    continue;
+                }
+                const double oP = 1.0 + (double)vL * vtauP[i], oS = 1.0 + (double)vL * vtauS[i];
+                const double Mu = M[i] * oP, t = Mu - 2.0 * mu[i] * oS;
                 v[o] = (float)t;
-                h[o] = (float)(-(t * t) / M[i]);
+                h[o] = (float)(-(t * t) / Mu);
+                const double t3 = 2.0 * mu[i] * vtauS[i] - M[i] * vtauP[i];
+                const double ratio = 2.0 * mu[i] * oS / Mu - 1.0;
+                for (int l = 0; l < vL; l++) {
+                    const double tau = 1.0 / (2.0 * M_PI * vfreq[l]);
+                    const double c2 = 1.0 / (1.0 + DT / (2.0 * tau));
+                    rv[(size_t)l * L.slab + o] = (float)(t3 * c2 / tau);
+                    rh[(size_t)l * L.slab + o] = (float)(t3 * ratio * c2 / tau);
+                }
             }
         if (!fsH && ((rc = dalloc(&fsH, (size_t)L.slab)) || (rc = dalloc(&fsV, (size_t)L.slab)))) return rc;
         HIP3(hipMemcpyAsync(fsH, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice, st));
         HIP3(hipMemcpyAsync(fsV, v.data(), v.size() * sizeof(float), hipMemcpyHostToDevice, st));
+        for (int l = 0; visco && l < vL; l++) {
+            if (!vfs[l] && ((rc = dalloc(&vfs[l], (size_t)L.slab)) || (rc = dalloc(&vfs[4 + l], (size_t)L.slab))))
+                return rc;
+            HIP3(hipMemcpyAsync(vfs[l], rh.data() + (size_t)l * L.slab, (size_t)L.slab * sizeof(float),
+                                hipMemcpyHostToDevice, st));
+            HIP3(hipMemcpyAsync(vfs[4 + l], rv.data() + (size_t)l * L.slab, (size_t)L.slab * sizeof(float),
+                                hipMemcpyHostToDevice, st));
+        }
+        HIP3(hipStreamSynchronize(st));  // h, v, rh, rv are freed at the end of this block
     }
     HIP3(hipStreamSynchronize(st));
+    model_ok = true;
+    return 0;
+}
+
+// 3D EM (EMEM.cpp:378-395, ForwardSolver3Demem.cpp:85-100): m0 = relative permittivity, m1 =
+// conductivity, m2 = relative permeability.  1/mu with the 4-point yz, xz, xy averages (invalid ->
+// 0, Modelparameter.cpp:449-526, 633-639); epsilon and sigma with the 2-point averages along x, y, z
+// (the last point alone), formed in float as the float build's sparse products form them, then Ca,
+// Cb in the reference's float steps.  Visco-EM (ViscoEMEM.cpp:409-435, ForwardSolver3Dviscoemem.cpp:
+// 78-135): both tau averaged the same way, Ca, Cb of the effective optical values and Cd_l.
+int Elastic3D::set_model_em(const float *m0, const float *m1, const float *m2)
+{
+    auto gi = [&](int x, int y, int z) { return (size_t)x + (size_t)z * NX + (size_t)y * NX * NZ; };
+    const size_t n = (size_t)NX * NY * NZ;
+    const double mu0 = 1.2566370614e-6;
+    const float eps0f = 8.8541878176e-12f, dtf = (float)DT;
+    std::vector<float> eps(n);
+    for (size_t i = 0; i < n; i++) eps[i] = m0[i] * eps0f;
+    float vinv = 0.f, vcdl[4] = {0.f, 0.f, 0.f, 0.f};
+    if (visco) {
+        for (int l = 0; l < vL; l++) {
+            const float tau = (float)(1.0 / (2.0 * M_PI * (double)(float)vfreq[l]));
+            vinv = (float)((double)vinv + 1.0 / (double)tau);
+            float tv = (float)(1.0 / (1.0 + 0.5 * (double)dtf / (double)tau));
+            float den = (float)vL * tau;
+            den *= tau;
+            tv /= den;
+            vcdl[l] = tv;
+        }
+        vinv /= (float)vL;
+    }
+    std::vector<std::vector<float>> c(9, std::vector<float>((size_t)L.vol, 0.f));
+    std::vector<std::vector<float>> cd(visco ? 3 * vL : 0, std::vector<float>((size_t)L.vol, 0.f));
+    const int len[3] = {NX, NY, NZ};
+    static const int planes[3][2] = {{1, 2}, {0, 2}, {0, 1}};  // hx: yz, hy: xz, hz: xy
+    for (int y = 0; y < NY; y++)
+        for (int z = 0; z < NZ; z++)
+            for (int x = 0; x < NX; x++) {
+                const size_t i = gi(x, y, z), o = (size_t)L.at(x, y, z);
+                const int pos[3] = {x, y, z};
+                for (int pl = 0; pl < 3; pl++) {
+                    const int a0 = planes[pl][0], a1 = planes[pl][1];
+                    const bool in0 = pos[a0] + 1 < len[a0], in1 = pos[a1] + 1 < len[a1];
+                    int p0[3] = {x, y, z}, p1[3] = {x, y, z}, p2[3] = {x, y, z};
+                    p0[a0]++;
+                    p1[a1]++;
+                    p2[a0]++;
+                    p2[a1]++;
+                    auto mu = [&](const int *p) { return (double)m2[gi(p[0], p[1], p[2])] * mu0; };
+                    const double m00 = (double)m2[i] * mu0;
+                    double avg;
+                    if (in0 && in1)
+                        avg = 0.25 * (m00 + mu(p0) + mu(p1) + mu(p2));
+                    else if (in0)
+                        avg = 0.5 * (m00 + mu(p0));
+                    else if (in1)
+                        avg = 0.5 * (m00 + mu(p1));
+                    else
+                        avg = m00;
+                    double v = 1.0 / avg;
+                    if (!std::isfinite(v)) v = 0.0;
+                    c[pl][o] = (float)v;
+                }
+                for (int ax = 0; ax < 3; ax++) {
+                    const bool last = pos[ax] + 1 >= len[ax];
+                    int pj[3] = {x, y, z};
+                    pj[ax]++;
+                    const size_t j = last ? i : gi(pj[0], pj[1], pj[2]);
+                    float e = eps[i], sg = m1[i];
+                    if (!last) {
+                        e = 0.5f * eps[i] + 0.5f * eps[j];
+                        sg = 0.5f * m1[i] + 0.5f * m1[j];
+                    }
+                    if (visco) {
+                        float te = vtauP[i], ts = vtauS[i];
+                        if (!last) {
+                            te = 0.5f * vtauP[i] + 0.5f * vtauP[j];
+                            ts = 0.5f * vtauS[i] + 0.5f * vtauS[j];
+                        }
+                        float cdv[4];
+                        em_visco_coef(e, sg, te, ts, dtf, vinv, vL, vcdl, c[3 + 2 * ax][o], c[4 + 2 * ax][o], cdv);
+                        for (int l = 0; l < vL; l++) cd[ax * vL + l][o] = cdv[l];
+                    } else {
+                        em_ca_cb(e, sg, dtf, c[3 + 2 * ax][o], c[4 + 2 * ax][o]);
+                    }
+                }
+            }
+    int rc;
+    for (int q = 0; q < 3; q++) {
+        if (!coef[q] && (rc = dalloc(&coef[q], (size_t)L.vol))) return rc;
+        HIP3(hipMemcpyAsync(coef[q], c[q].data(), c[q].size() * sizeof(float), hipMemcpyHostToDevice, st));
+    }
+    for (int q = 0; q < 6; q++) {
+        if (!emc[q] && (rc = dalloc(&emc[q], (size_t)L.vol))) return rc;
+        HIP3(hipMemcpyAsync(emc[q], c[3 + q].data(), c[3 + q].size() * sizeof(float), hipMemcpyHostToDevice, st));
+    }
+    for (int ax = 0; visco && ax < 3; ax++)
+        for (int l = 0; l < vL; l++) {
+            float *&d = vcd[4 * ax + l];
+            if (!d && (rc = dalloc(&d, (size_t)L.vol))) return rc;
+            const std::vector<float> &h = cd[ax * vL + l];
+            HIP3(hipMemcpyAsync(d, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice, st));
+        }
+    HIP3(hipStreamSynchronize(st));  // c and cd are freed at the end of this call
     model_ok = true;
     return 0;
 }
@@ -445,10 +715,11 @@ int Elastic3D::sync_acquisition()
 int Elastic3D::reset()
 {
     for (int f = 0; f < nf; f++) HIP3(hipMemsetAsync(F[f], 0, (size_t)L.vol * nshots * sizeof(float), st));
-    static const int axis_of[18] = {0, 1, 2, 0, 1, 2, 0, 1, 2, 0, 1, 2, 1, 0, 2, 0, 2, 1};
     for (int q = 0; q < nps; q++)
-        if (ps[q]) HIP3(hipMemsetAsync(ps[q], 0, ps_n[axis_of[q]] * sizeof(float), st));
+        if (ps[q]) HIP3(hipMemsetAsync(ps[q], 0, ps_n[psi_axis(q)] * sizeof(float), st));
     if (d_seis) HIP3(hipMemsetAsync(d_seis, 0, (size_t)NT * std::max(rec_total, 1) * sizeof(float), st));
+    for (int k = 0; k < 24; k++)
+        if (vR[k]) HIP3(hipMemsetAsync(vR[k], 0, (size_t)L.vol * nshots * sizeof(float), st));
     return 0;
 }
 
@@ -474,6 +745,7 @@ Coef3D Elastic3D::coefs() const
     c.fsH = free_surface == 1 ? fsH : nullptr;
     c.fsV = free_surface == 1 ? fsV : nullptr;
     c.fs = free_surface == 1;
+    for (int k = 0; k < 6; k++) c.em[k] = emc[k];
     return c;
 }
 
@@ -568,16 +840,20 @@ int Elastic3D::enqueue(int nsteps, bool timed)
         pending = 0;
         return 0;
     };
+    const Visco3D V = visco3d();
     for (int j = 0; j < nsteps; j++) {
         hipEvent_t *e = timed ? &kev[4 * pending] : nullptr;
-        auto launch = physics == ACOUSTIC ? launch_ac3d : [](int w, int o, int r, const Layout3D &L_, int ns, int ny,
-                                                              const Fields3D &F_, const Coef3D &C_, const Taps3D &T_,
-                                                              const Cpml3D &P_, const Acq &A_, int tofs, int pend,
-                                                              hipStream_t s_, hipEvent_t a, hipEvent_t b) {
-            launch_el3d(w, o, r, L_, ns, ny, F_, C_, T_, P_, A_, tofs, pend, s_, a, b);
-        };
-        launch(0, order, rz, L, nshots, nyc, Fd, C, T, P, A, j, j > 0, st, e ? e[0] : nullptr, e ? e[1] : nullptr);
-        launch(1, order, rz, L, nshots, nyc, Fd, C, T, P, A, j, 0, st, e ? e[2] : nullptr, e ? e[3] : nullptr);
+        hipEvent_t e0 = e ? e[0] : nullptr, e1 = e ? e[1] : nullptr, e2 = e ? e[2] : nullptr, e3 = e ? e[3] : nullptr;
+        if (physics == ACOUSTIC) {
+            launch_ac3d(0, order, rz, L, nshots, nyc, Fd, C, T, P, A, j, j > 0, st, e0, e1);
+            launch_ac3d(1, order, rz, L, nshots, nyc, Fd, C, T, P, A, j, 0, st, e2, e3);
+        } else if (physics == TEMEM) {
+            launch_em3d(0, order, rz, L, nshots, nyc, Fd, C, T, P, A, j, j > 0, st, e0, e1);
+            launch_em3d(1, order, rz, L, nshots, nyc, Fd, C, T, P, A, j, 0, st, e2, e3, visco ? &V : nullptr);
+        } else {
+            launch_el3d(0, order, rz, L, nshots, nyc, Fd, C, T, P, A, j, j > 0, st, e0, e1);
+            launch_el3d(1, order, rz, L, nshots, nyc, Fd, C, T, P, A, j, 0, st, e2, e3, nullptr, visco ? &V : nullptr);
+        }
         if (damping == 1) launch_abs_velocity3d(L, nshots, Fd, C, st);
         if (timed && ++pending == BATCH) {
             int rc = collect();
@@ -588,7 +864,7 @@ int Elastic3D::enqueue(int nsteps, bool timed)
         int rc = collect();
         if (rc) return rc;
     }
-    if (nsteps > 0) launch_flush3d(L, Fd, A, nsteps - 1, st);
+    if (nsteps > 0) launch_flush3d(L, Fd, A, nsteps - 1, st, physics == TEMEM);
     launch_advance(d_tbase, nsteps, st);
     HIP3(hipGetLastError());
     return 0;
@@ -734,13 +1010,23 @@ double Elastic3D::kernel_bytes(int kind) const
     // per shot-cell field reads + writes, per-cell coefficient reads, memory variables per frame cell
     // elastic: velocity 6 + 3 read, 3 written, 3 coefficients; stress 3 + 6 read, 6 written, 5 coefficients
     // acoustic: velocity 1 + 3 read, 3 written, 3 coefficients; pressure 3 + 1 read, 1 written, M (+ ABS)
-    const bool ac = physics == ACOUSTIC;
-    const double fr = ac ? (kind == 0 ? 7 : 5) : (kind == 0 ? 12 : 15), co = ac ? (kind == 0 ? 3 : 1) : (kind == 0 ? 3 : 5);
+    // EM: H 3 + 3 read, 3 written, 3 coefficients; E 3 + 3 read, 3 written, 6 coefficients (+ ABS);
+    // visco-EM: the memory variables (3 L read and written per shot-cell) and 3 L Cd per cell
+    const bool ac = physics == ACOUSTIC, em = physics == TEMEM;
+    double fr = ac ? (kind == 0 ? 7 : 5) : (kind == 0 ? 12 : 15), co = ac ? (kind == 0 ? 3 : 1) : (kind == 0 ? 3 : 5);
+    if (em) {
+        fr = 9;
+        co = kind == 0 ? 3 : 6 + (damping == 1 ? 1 : 0);
+        if (visco && kind == 1) {
+            fr += 6.0 * vL;
+            co += 3.0 * vL;
+        }
+    }
     double bytes = 4.0 * (cells * B * fr + cells * co);
     if (damping == 2) {
         const double fx = (double)NY * NZ * 2 * W, fy = (double)NX * NZ * (free_surface ? W : 2 * W),
                      fz = (double)NX * NY * 2 * W;
-        bytes += 8.0 * B * (ac ? 1 : 3) * (fx + fy + fz);
+        bytes += 8.0 * B * (ac ? 1 : (em ? 2 : 3)) * (fx + fy + fz);
     }
     return bytes;
 }
@@ -749,7 +1035,8 @@ const char *Elastic3D::kernel_name(int kind) const
 {
     static const char *n[3] = {"el3d_velocity", "el3d_stress", "flush3d"};
     static const char *a[3] = {"ac3d_velocity", "ac3d_pressure", "flush3d"};
-    return physics == ACOUSTIC ? a[kind] : n[kind];
+    static const char *e[3] = {"em3d_h", "em3d_e", "flush3d"};
+    return physics == ACOUSTIC ? a[kind] : (physics == TEMEM ? e[kind] : n[kind]);
 }
 
 }  // namespace wv

@@ -159,6 +159,11 @@ struct Visco2D {
     const float *tauP, *tauS, *tauSxy, *opP, *opS, *tauSxz, *tauSyz;
     const float *fsRH[4], *fsRV[4];
     float itau[4], c1[4], c2[4], dth;
+    // visco-EM (ForwardSolver2Dviscotmem.cpp:172-199, ForwardSolver2Dviscoemem.cpp:175-218): memory
+    // variables of the electric field in R[l] (TM: ez; TE: ex) and R[4 + l] (TE: ey), their per-cell
+    // coefficients Cd_l in cd[l] / cd[4 + l] (Cd includes -DT), Cc_l, and DT
+    const float *cd[8];
+    float cc[4], dt;
 };
 
 // Forward quantities a 2D elastic step hands to the adjoint's gradient (all
@@ -215,6 +220,11 @@ int fused2d_tile_x(int order);
 void launch_fused2d(int order, int tile_y, const Layout2D &L, int nshots, const Fields2D &F, const Fields2D &Fn,
                     const Coef2D &C, const Taps &T, const Cpml2D &P, const PsiNew2D &PN, const Acq &A, int tofs,
                     int pend, hipStream_t stream, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
+// Visco-EM electric phase (kernels2d.hip: tm2d_e / te2d_e with memory variables); the magnetic
+// phase is launch_step2d(TMEM or TEMEM, which = 0).
+void launch_viscoem2d_e(int physics, int order, int tile_y, const Layout2D &L, int nshots, const Fields2D &F,
+                        const Coef2D &C, const Taps &T, const Cpml2D &P, const Acq &A, int tofs, const Visco2D &V,
+                        hipStream_t stream, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 // Viscoelastic stress phase (visco2d.hip); the velocity phase is launch_step2d(ELASTIC or SH,
 // which = 0).  physics ELASTIC: P-SV, SH: visco-SH.  tile_y in {8, 16, 32}.
 void launch_visco2d_stress(int physics, int order, int tile_y, const Layout2D &L, int nshots, const Fields2D &F,

@@ -190,6 +190,91 @@ def emem_2d(NX=170, NY=97, NT=500, DT=3e-11, nshots=3, order=8, W=12, damping=2,
     return c
 
 
+def _em_visco(c, freqs, tau_eps, tau_sig, seed):
+    """Relaxation of a 2D EM case: tau of the permittivity (relative) and of the conductivity [s]
+    per 10-cell block around tau_eps, tau_sig; L mechanisms at `freqs` [Hz]."""
+    s = c.settings
+    rng = np.random.Generator(np.random.PCG64(seed + 500))
+    nby, nbx = -(-s["NY"] // 10), -(-s["NX"] // 10)
+    te = np.kron(rng.uniform(0.5, 1.5, (nby, nbx)) * tau_eps, np.ones((10, 10)))[:s["NY"], :s["NX"]]
+    ts = np.kron(rng.uniform(0.0, 2.0, (nby, nbx)) * tau_sig, np.ones((10, 10)))[:s["NY"], :s["NX"]]
+    c.visco = dict(freqs=list(freqs), tauP=te.astype(np.float32).ravel(), tauS=ts.astype(np.float32).ravel())
+    return c
+
+
+def viscotmem_2d(NX=170, NY=97, NT=500, DT=3e-11, nshots=3, order=8, W=12, damping=2, seed=17, freqs=(3e8,),
+                 tau_eps=0.3, tau_sig=2e-10):
+    """2D visco TM-EM (ForwardSolver2Dviscotmem.cpp): the emem_2d model (blocky static epsilon_r,
+    sigma, mu_r), tau of the permittivity and of the conductivity per block, L mechanisms at
+    `freqs`; per shot one EZ source and a row and a column of EZ receivers (uneven counts)."""
+    c = emem_2d(NX=NX, NY=NY, NT=NT, DT=DT, nshots=nshots, order=order, W=W, damping=damping, seed=seed)
+    c.name, c.type = "viscotmem_2d", "viscotmem"
+    rng = np.random.Generator(np.random.PCG64(seed + 1))
+    lo = W + 4 if damping else 4
+    w = ricker(NT, DT, 2e8, 1.0)
+    c.shots = []
+    for k in range(nshots):
+        src = int(rng.integers(lo, NX - lo)) + int(rng.integers(lo, NY - lo)) * NX
+        nr = int(rng.integers(10, 40))
+        rx = np.sort(rng.choice(np.arange(lo, NX - lo), nr, replace=False))
+        rec = np.concatenate([rx + (lo + 2 * k) * NX, (NX // 2) + np.arange(lo, NY - lo, 9) * NX])
+        c.shots.append(Shot(np.array([src]), np.array([1]), w[None, :], rec, np.ones(len(rec), np.int32)))
+    return _em_visco(c, freqs, tau_eps, tau_sig, seed)
+
+
+def viscoemem_2d(NX=170, NY=97, NT=500, DT=3e-11, nshots=3, order=8, W=12, damping=2, seed=19, freqs=(3e8,),
+                 tau_eps=0.3, tau_sig=2e-10):
+    """2D visco TE-EM (ForwardSolver2Dviscoemem.cpp): the emem_2d model and acquisition with tau of
+    the permittivity and of the conductivity per block, L mechanisms at `freqs`."""
+    c = emem_2d(NX=NX, NY=NY, NT=NT, DT=DT, nshots=nshots, order=order, W=W, damping=damping, seed=seed)
+    c.name, c.type = "viscoemem_2d", "viscoemem"
+    return _em_visco(c, freqs, tau_eps, tau_sig, seed)
+
+
+def emem_3d(NX=44, NY=36, NZ=30, NT=300, DT=3e-11, nshots=2, order=8, W=8, damping=2, seed=23, block=6):
+    """3D EM (ForwardSolver3Demem.cpp): blocky epsilon_r (1-16), sigma (0-0.01 S/m) and mu_r (1-3)
+    in 6-cell blocks so every average differs from the node values; per shot EZ, EX, EY and HZ
+    sources (one or two) at seeded positions and a receiver line of all four components."""
+    s = dict(NX=NX, NY=NY, NZ=NZ, NT=NT, DH=0.02, DT=DT, spatial_fd_order=order, damping_boundary=damping,
+             boundary_width=W, npower=4.0, fc_cpml=2e8, vmax_cpml=3e8, damping_coeff=8.0)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    nb = [-(-n // block) for n in (NY, NZ, NX)]
+    ones = np.ones((block, block, block))
+    blocks = [rng.uniform(lo, hi, nb) for lo, hi in ((1.0, 16.0), (0.0, 0.01), (1.0, 3.0))]
+    eps, sig, mur = (np.kron(b, ones)[:NY, :NZ, :NX].astype(np.float32).ravel() for b in blocks)
+    c = Case("emem_3d", "3D", "emem", s, eps, sig, mur)
+    lo = W + 3 if damping else 3
+    w = ricker(NT, DT, 2e8, 1.0)
+    kinds = ([1], [2, 4], [3], [4, 1])
+    for k in range(nshots):
+        ty = kinds[k % 4]
+        pos = [int(rng.integers(lo, NX - lo)) + int(rng.integers(lo, NZ - lo)) * NX +
+               int(rng.integers(lo, NY - lo)) * NX * NZ for _ in ty]
+        sigs = np.stack([w * (1.0 if j == 0 else 0.6) for j in range(len(ty))])
+        rx = np.arange(lo, NX - lo, 2)
+        cells = rx + (NZ // 2 + k) * NX + (lo + 2) * NX * NZ
+        rec = np.concatenate([cells, cells[::2], cells[1::2], cells[::3]])
+        rty = np.concatenate([np.full(len(cells), 1), np.full(len(cells[::2]), 2), np.full(len(cells[1::2]), 3),
+                              np.full(len(cells[::3]), 4)]).astype(np.int32)
+        c.shots.append(Shot(np.array(pos), np.array(ty), sigs, rec, rty))
+    return c
+
+
+def viscoemem_3d(freqs=(3e8,), tau_eps=0.3, tau_sig=2e-10, seed=29, **kw):
+    """3D visco-EM (ForwardSolver3Dviscoemem.cpp): the emem_3d model and acquisition with tau of the
+    permittivity and of the conductivity per 6-cell block, L mechanisms at `freqs`."""
+    c = emem_3d(seed=seed, **kw)
+    c.name, c.type = "viscoemem_3d", "viscoemem"
+    s = c.settings
+    rng = np.random.Generator(np.random.PCG64(seed + 500))
+    nb = [-(-n // 6) for n in (s["NY"], s["NZ"], s["NX"])]
+    ones = np.ones((6, 6, 6))
+    te = np.kron(rng.uniform(0.5, 1.5, nb) * tau_eps, ones)[:s["NY"], :s["NZ"], :s["NX"]]
+    ts = np.kron(rng.uniform(0.0, 2.0, nb) * tau_sig, ones)[:s["NY"], :s["NZ"], :s["NX"]]
+    c.visco = dict(freqs=list(freqs), tauP=te.astype(np.float32).ravel(), tauS=ts.astype(np.float32).ravel())
+    return c
+
+
 def elastic_c4(N=600, NT=3000, DT=1e-4, nshots=16, seed=4, W=30, anomaly=True):
     NX = NY = N
     s = dict(NX=NX, NY=NY, NT=NT, DH=1.0, DT=DT, spatial_fd_order=8, damping_boundary=2, boundary_width=W,
@@ -310,6 +395,42 @@ def acoustic_3d(NX=72, NY=60, NZ=52, NT=260, DT=2e-4, nshots=2, order=8, W=10, d
     for k in range(nshots):
         sx, sy, sz = (int(rng.integers(lo, n - lo)) for n in (NX, NY, NZ))
         ty = 1 if k % 2 == 0 else 2 + k % 3
+        rx = np.arange(lo, NX - lo, 3)
+        yr = 0 if free_surface else lo
+        cells = rx + (NZ // 2) * NX + yr * NX * NZ
+        rec = np.concatenate([cells, cells[::2], cells[1::2], cells[::3]])
+        rty = np.concatenate([np.full(len(cells), 1), np.full(len(cells[::2]), 2), np.full(len(cells[1::2]), 3),
+                              np.full(len(cells[::3]), 4)]).astype(np.int32)
+        c.shots.append(Shot(np.array([sx + sz * NX + sy * NX * NZ]), np.array([ty]), w[None, :], rec, rty))
+    return c
+
+
+def visco_3d(NX=44, NY=36, NZ=30, NT=260, DT=2e-4, nshots=2, order=8, W=8, damping=2, free_surface=0,
+             freqs=(30.0,), tau=0.15, seed=13):
+    """3D viscoelastic (ForwardSolver3Dviscoelastic.cpp): layered vp/vs/rho with Gaussian
+    perturbations, tauP / tauS varying by layer around `tau`, L relaxation mechanisms at `freqs`,
+    P / VX / VY / VZ Ricker sources at seeded positions, receivers of every component on a line."""
+    s = dict(NX=NX, NY=NY, NZ=NZ, NT=NT, DH=2.0, DT=DT, spatial_fd_order=order, damping_boundary=damping,
+             boundary_width=W, npower=4.0, fc_cpml=40.0, vmax_cpml=3600.0, free_surface=free_surface,
+             damping_coeff=8.0)
+    rows = np.arange(NY) * 4 // NY
+    g = gaussian_field((NY, NZ, NX), 4.0, seed)
+    vp = (1800.0 + 400.0 * rows)[:, None, None] * (1.0 + 0.03 * g)
+    rho = (1900.0 + 150.0 * rows)[:, None, None] * np.ones((1, NZ, NX))
+    vs = vp / 1.8
+    lay = np.repeat(rows, NZ * NX)
+    tp = (tau * (1.0 + 0.25 * lay)).astype(np.float32)
+    ts = (tau * (1.2 - 0.15 * lay)).astype(np.float32)
+    c = Case("visco_3d", "3D", "viscoelastic", s, vp.astype(np.float32).ravel(), vs.astype(np.float32).ravel(),
+             rho.astype(np.float32).ravel(), visco=dict(freqs=list(freqs), tauP=tp, tauS=ts))
+    rng = np.random.Generator(np.random.PCG64(seed))
+    w = ricker(NT, DT, 60.0, 1.0)
+    lo = W + 3 if damping else 3
+    for k in range(nshots):
+        sx, sy, sz = (int(rng.integers(lo, n - lo)) for n in (NX, NY, NZ))
+        if free_surface:
+            sy = int(rng.integers(1, NY - lo))
+        ty = 1 + k % 4
         rx = np.arange(lo, NX - lo, 3)
         yr = 0 if free_surface else lo
         cells = rx + (NZ // 2) * NX + yr * NX * NZ

@@ -29,10 +29,15 @@ class ForwardSolver:
         ("2D", "viscoelastic"): ("vx", "vy", "sxx", "syy", "sxy"),
         ("2D", "tmem"): ("hx", "hy", "ez"),
         ("2D", "emem"): ("hz", "ex", "ey"),
+        ("2D", "viscotmem"): ("hx", "hy", "ez"),
+        ("2D", "viscoemem"): ("hz", "ex", "ey"),
         ("2D", "sh"): ("vz", "sxz", "syz"),
         ("2D", "viscosh"): ("vz", "sxz", "syz"),
         ("3D", "elastic"): ("vx", "vy", "vz", "sxx", "syy", "szz", "syz", "sxz", "sxy"),
+        ("3D", "viscoelastic"): ("vx", "vy", "vz", "sxx", "syy", "szz", "syz", "sxz", "sxy"),
         ("3D", "acoustic"): ("vx", "vy", "vz", "p"),
+        ("3D", "emem"): ("hx", "hy", "hz", "ex", "ey", "ez"),
+        ("3D", "viscoemem"): ("hx", "hy", "hz", "ex", "ey", "ez"),
     }
 
     def __init__(self, dimension: str, type_: str):
@@ -86,7 +91,10 @@ class ForwardSolver:
 
     def set_relaxation(self, freqs, tauP, tauS):
         """Viscoelastic: relaxation frequencies [Hz] (1..4 mechanisms) and tauP, tauS per cell (or
-        scalars), before prepare_for_modelling (Modelparameter::Viscoelastic, Viscoelastic.cpp:160-260)."""
+        scalars), before prepare_for_modelling (Modelparameter::Viscoelastic, Viscoelastic.cpp:160-260).
+        Visco-EM: tauP = tau of the dielectric permittivity, tauS = tau of the electric conductivity
+        [s] (Modelparameter::getTauDielectricPermittivity / getTauElectricConductivity); the model
+        vectors are then the static relative permittivity, static conductivity and mu_r."""
         n = self.cfg.NX * self.cfg.NY * self.cfg.NZ
         f = np.ascontiguousarray(np.atleast_1d(freqs), dtype=np.float64)
         tp = np.ascontiguousarray(np.broadcast_to(np.asarray(tauP, np.float32), (n,)))
