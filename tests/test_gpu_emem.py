@@ -78,4 +78,4 @@ def test_emem_rejections():
         ok.set_acquisition(0, sh.src_index, np.ones(len(sh.src_index), np.int32), sh.src_signal, sh.rec_index,
                            sh.rec_type)
     with pytest.raises(wavesim.WaveError, match="not provided"):
-        wavesim.create("3D", "emem")
+        wavesim.create("3D", "tmem")  # the reference factory has no 3D TM solver

@@ -118,6 +118,61 @@ std::vector<float> signalFromFile(const Configuration &config, long row, int NT)
     return readMatrixRowMtx(config.getString("SourceSignalFilename") + ".mtx", row, (size_t)NT);
 }
 
+// Visco-EM model (ViscoTMEM.cpp / ViscoEMEM.cpp init): tau of the permittivity (".tauEpsilon",
+// relative) and of the conductivity (".tauSigmar", relative to tau_ref = 1 / (2 pi
+// CenterFrequencyCPML), stored times tau_ref).  Constant models (ModelRead = 0) give the static
+// epsilon_r and sigma directly; model files hold the "real effective" values at the reference
+// frequency, which become the static ones (calculateType 2, ModelparameterEM.cpp:473-554):
+//   a = mean_l w^2 t_l^2 / (1 + w^2 t_l^2), b = mean_l w^2 t_l / (1 + w^2 t_l^2)   (w = 2 pi fc)
+//   eps_s = (eps_re - sig_re tauS) / (1 - a tauE - b tauE tauS), floored at eps0
+//   sig_s = sig_re - b tauE eps_s, floored at 0
+// in the value type.  m0 returns eps_s / eps0, m1 sig_s; tauP / tauS the two tau as the solver
+// consumes them.
+void viscoEmModel(const Configuration &config, size_t n, const std::vector<double> &freqs, std::vector<float> &m0,
+                  std::vector<float> &m1, std::vector<float> &tauP, std::vector<float> &tauS)
+{
+    const float fc = (float)config.getDouble("CenterFrequencyCPML");
+    const float tref = (float)(1.0 / (2.0 * M_PI * (double)fc));
+    if (config.getInt("ModelRead", 0) == 1) {
+        const std::string base = config.getString("ModelFilename");
+        tauP = readVectorMtx(base + ".tauEpsilon.mtx", n);
+        tauS = readVectorMtx(base + ".tauSigmar.mtx", n);
+    } else {
+        tauP.assign(n, (float)config.getDouble("tauEpsilon"));
+        tauS.assign(n, (float)config.getDouble("tauSigmar"));
+    }
+    for (auto &v : tauS) v *= tref;
+    if (config.getInt("ModelRead", 0) != 1) return;
+    const float eps0 = 8.8541878176e-12f;
+    const float w = (float)(2.0 * M_PI * (double)fc);
+    float a = 0.f, b = 0.f;
+    for (double f : freqs) {
+        const float t = (float)(1.0 / (2.0 * M_PI * (double)(float)f));
+        a += w * w * t * t / (1 + w * w * t * t);
+        b += w * w * t / (1 + w * w * t * t);
+    }
+    a /= (float)freqs.size();
+    b /= (float)freqs.size();
+    for (size_t i = 0; i < n; i++) {
+        const float ere = m0[i] * eps0, sre = m1[i];
+        float at = a * tauP[i];
+        at = 1 - at;
+        float bt = b * tauP[i];
+        bt *= tauS[i];
+        at -= bt;
+        float es = sre * tauS[i];
+        es = ere - es;
+        es /= at;
+        if (es < eps0) es = eps0;
+        float sb = b * tauP[i];
+        sb *= es;
+        float ss = sre - sb;
+        if (ss < 0) ss = 0;
+        m0[i] = es / eps0;
+        m1[i] = ss;
+    }
+}
+
 }  // namespace
 
 int main(int argc, char **argv)
@@ -139,7 +194,8 @@ int main(int argc, char **argv)
         const std::string dim = config.getString("dimension"), type = config.getString("equationType");
         std::string tl = type;
         for (auto &c : tl) c = (char)std::tolower((unsigned char)c);
-        const bool em = tl == "tmem" || tl == "emem";
+        const bool em = tl == "tmem" || tl == "emem" || tl == "viscotmem" || tl == "viscoemem";
+        const bool vem = tl == "viscotmem" || tl == "viscoemem";
         const int NX = (int)config.getInt("NX"), NY = (int)config.getInt("NY");
         const int NZ = (dim == "2D" || dim == "2d") ? 1 : (int)config.getInt("NZ");
         config.add("nz", std::to_string(NZ), true);
@@ -169,7 +225,7 @@ int main(int argc, char **argv)
         // viscoelastic / visco-SH: relaxation mechanisms from the configuration, tauP / tauS from
         // the model files or the configuration (Viscoelastic.cpp:160-200, 260-290; visco-SH reads
         // tauS only, ViscoSH.cpp)
-        const bool visco = tl == "viscoelastic" || tl == "viscosh";
+        const bool visco = tl == "viscoelastic" || tl == "viscosh" || vem;
         std::vector<double> freqs;
         std::vector<float> tauP, tauS;
         if (visco) {
@@ -177,7 +233,9 @@ int main(int argc, char **argv)
             const char *fk[4] = {"relaxationFrequency", "relaxationFrequency2", "relaxationFrequency3",
                                  "relaxationFrequency4"};
             for (long l = 0; l < nm; l++) freqs.push_back(l < 4 ? config.getDouble(fk[l]) : 0.0);
-            if (config.getInt("ModelRead", 0) == 1) {
+            if (vem) {
+                viscoEmModel(config, n, freqs, m0, m1, tauP, tauS);
+            } else if (config.getInt("ModelRead", 0) == 1) {
                 const std::string base = config.getString("ModelFilename");
                 if (tl != "viscosh") tauP = readVectorMtx(base + ".tauP.mtx", n);
                 tauS = readVectorMtx(base + ".tauS.mtx", n);
