@@ -94,3 +94,47 @@ def test_tuner_with_strip_candidates_keeps_parity():
     assert rel_l2(sol.seismograms(0), o32) <= TOL
     strip = cases.make_solver(c, path="strip", tile_y=32)
     assert strip.tune(20) in ("strip-8", "strip-16", "strip-32")
+
+
+# ---- TM-EM row-strip kernels (tm2d_h_strip, tm2d_e_strip): the ForwardSolver2Dtmem.cpp:108-171
+# step with the elastic strips' split of the work
+
+
+@pytest.mark.parametrize("tile_y", [8, 16, 32])
+def test_tmem_strip_batched_uneven_receivers(tile_y):
+    """Five batched shots of uneven receiver counts on a blocky lossy model (200 x 120: ragged
+    against the 64-column blocks and every strip height), CPML, order 8."""
+    c = cases.tmem_c3(NX=200, NY=120, NT=900, nshots=5, block=10, W=12)
+    gpu = _check(c, tile_y=tile_y)
+    assert [g.shape[0] for g in gpu] == [len(s.rec_index) for s in c.shots]
+
+
+@pytest.mark.parametrize("order", [2, 4, 12])
+def test_tmem_strip_orders_and_abs(order):
+    c = cases.tmem_c3(NX=150, NY=90, NT=500, nshots=2, block=10, W=10)
+    c.settings.update(spatial_fd_order=order, damping_boundary=1 if order == 4 else 2, damping_coeff=8.0)
+    _check(c)
+
+
+def test_tmem_strip_batched_equals_single_graphs_and_fields():
+    c = cases.tmem_c3(NX=170, NY=97, NT=400, nshots=3, block=10, W=12)
+    sol, batched = _run(c)
+    graphed, gb = _run(c, steps_per_graph=50)
+    for k, sh in enumerate(c.shots):
+        _, one = _run(c, shots=[sh])
+        assert np.array_equal(one[0], batched[k]), k
+        assert np.array_equal(gb[k], batched[k]), k
+    _, f32 = oracle_shot(c, c.shots[2], "f32", want_fields=True)
+    for f, name in enumerate(("hx", "hy", "ez")):
+        assert rel_l2(sol.wavefield(name, shot=2), f32[f]) <= TOL, name
+
+
+def test_tuner_with_tmem_strip_candidates_keeps_parity():
+    c = cases.tmem_c3(NX=200, NY=120, NT=500, nshots=3, block=10, W=12)
+    sol = cases.make_solver(c)
+    choice = sol.tune(20)
+    assert choice in sol.TUNE_CHOICES
+    sol.run(0, 500)
+    for k, sh in enumerate(c.shots):
+        o32, _ = oracle_shot(c, sh, "f32")
+        assert rel_l2(sol.seismograms(k), o32) <= TOL, (choice, k)

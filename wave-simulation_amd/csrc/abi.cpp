@@ -167,7 +167,10 @@ int wv_solver::enqueue_steps(int nsteps, bool timed)
         } else {
             const wv::Fields2D F = fields2d(cur);
             const wv::Cpml2D P = cpml2d(cur);
-            if (variant == V_STRIP) {
+            if (variant == V_STRIP && physics == wv::TMEM) {
+                wv::launch_strip2d_tm(order, tile_y, L, cfg.nshots, F, C, T, P, A, j, j > 0, st, 0, e0, e1);
+                wv::launch_strip2d_tm(order, tile_y, L, cfg.nshots, F, C, T, P, A, j, 0, st, 1, e2, e3);
+            } else if (variant == V_STRIP) {
                 const wv::Rec2D *rec = adjoint_record(this);
                 wv::launch_strip2d(order, tile_y, L, cfg.nshots, F, C, T, P, A, j, j > 0, st, 0, e0, e1, nullptr, rec);
                 wv::launch_strip2d(order, tile_y, L, cfg.nshots, F, C, T, P, A, j, 0, st, 1, e2, e3, nullptr, rec);
@@ -346,8 +349,8 @@ int wv_init(wv_solver *s, const wv_config *c)
     if (k.nshots < 1) return fail(WV_EINVAL, "nshots must be >= 1");
     if (k.path != WV_PATH_FUSED && k.path != WV_PATH_PLAIN && k.path != WV_PATH_PHASED && k.path != WV_PATH_STRIP)
         return fail(WV_EINVAL, "bad path");
-    if (k.path == WV_PATH_STRIP && (s->dim != 2 || s->physics != wv::ELASTIC || s->visco))
-        return fail(WV_EINVAL, "the row-strip kernels are 2D elastic only");
+    if (k.path == WV_PATH_STRIP && (s->dim != 2 || (s->physics != wv::ELASTIC && s->physics != wv::TMEM) || s->visco))
+        return fail(WV_EINVAL, "the row-strip kernels are 2D elastic and TM-EM only");
     if (s->visco && k.path == WV_PATH_PLAIN)
         return fail(WV_EINVAL, "the plain path is provided for acoustic, elastic and tmem");
     if (k.path == WV_PATH_STRIP && k.tile_y != 0 && k.tile_y != 8 && k.tile_y != 16 && k.tile_y != 32)
@@ -1155,8 +1158,10 @@ int wv_tune(wv_solver *s, int nsteps, int *choice)
     const int cand_var[9] = {V_PHASED, V_PHASED, V_PHASED, V_SINGLE, V_SINGLE, V_SINGLE, V_STRIP, V_STRIP, V_STRIP};
     const int cand_ty[9] = {8, 16, 32, 8, 16, 24, 8, 16, 32};
     const int cand_id[9] = {0, 1, 2, 3, 4, 5, 9, 10, 11};
+    // (TM-EM: the two-launch and the row-strip kernels)
     const bool all2d = s->physics == wv::ELASTIC && s->cfg.path == WV_PATH_FUSED && !s->visco;
-    const int ncand = all2d ? (wv::strip2d_fits(s->L) ? 9 : 6) : 3;
+    const bool tm = s->physics == wv::TMEM && s->cfg.path == WV_PATH_FUSED && !s->visco;
+    const int ncand = (all2d || tm) ? (wv::strip2d_fits(s->L) ? 9 : 6) : 3;
     float best = 1e30f;
     const int k0 = s->cfg.path == WV_PATH_STRIP ? 6 : 0, k1 = s->cfg.path == WV_PATH_STRIP ? 9 : ncand;
     // with checkpointing on, the adjoint recomputes with a two-launch pair: the
@@ -1164,7 +1169,7 @@ int wv_tune(wv_solver *s, int nsteps, int *choice)
     const int ck_every = adjoint_every(s);
     int best_k = ck_every && k0 == 0 ? 0 : k0;
     for (int k = k0; k < k1; k++) {
-        if (ck_every && cand_var[k] == V_SINGLE) continue;
+        if ((ck_every || s->physics != wv::ELASTIC) && cand_var[k] == V_SINGLE) continue;
         s->variant = cand_var[k];
         s->tile_x = s->variant == V_SINGLE ? wv::fused2d_tile_x(s->cfg.spatial_fd_order) : wv::TILE_X;
         s->tile_y = cand_ty[k];
@@ -1247,6 +1252,8 @@ int wv_kernel_stats(wv_solver *s, int kind, const char **name, double *avg_ms, l
     if (name)
         *name = kind >= 3 ? adj_names[kind - 3]
                           : ((s->variant == V_SINGLE && kind < 2)   ? (kind == 0 ? "el2d_fused" : "-")
+                             : (s->variant == V_STRIP && kind < 2 && s->physics == wv::TMEM)
+                                 ? (kind == 0 ? "tm2d_h_strip" : "tm2d_e_strip")
                              : (s->variant == V_STRIP && kind < 2) ? (kind == 0 ? "el2d_velocity_strip" : "el2d_stress_strip")
                              : (s->visco && kind == 1)              ? (s->physics == wv::SH ? "vsh2d_s" : "ve2d_stress")
                                                                    : names[s->physics][kind]);

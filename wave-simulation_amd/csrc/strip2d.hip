@@ -1,4 +1,4 @@
-// Row-strip 2D elastic P-SV kernels for gfx950.
+// Row-strip 2D elastic P-SV and TM-EM kernels for gfx950.
 //
 // The same two-launch time step as the tiled kernels (velocity phase, then
 // stress phase: ForwardSolver2Delastic.cpp:117-294) with another split of the
@@ -463,6 +463,189 @@ __global__ __launch_bounds__(NTHREADS) void el2d_stress_strip(Layout2D L, Fields
     block_acquisition<ELASTIC>(A, F, c.so, shot_block(A, B), A.tbase[0] + tofs, false);
 }
 
+// TM-EM H update of one row (ForwardSolver2Dtmem.cpp:108-135):
+// hx -= 1/mu_yz * CPML(Dyf ez); hy -= 1/mu_xz * (-CPML(Dxf ez)).  The column window of ez serves the
+// y derivative, its own rows (and the halo columns) the LDS strip of the x derivative.
+template <int H, int S, int U>
+__global__ __launch_bounds__(NTHREADS) void tm2d_h_strip(Layout2D L, Fields2D F, Coef2D C, Taps T, Cpml2D P, Acq A,
+                                                         int tofs, int pend)
+{
+    (void)A;
+    (void)tofs;
+    (void)pend;
+    constexpr int G = S / U, K = 2 * H + 1, N = S + 2 * H, RW = TILE_X + 2 * H;
+    static_assert(S % U == 0, "rows per wavefront must be a multiple of the group");
+    __shared__ float s_ez[BLOCK_Y][U][RW];
+    __shared__ float s_ty[BLOCK_Y][2][S * K];
+    __shared__ float s_cy[BLOCK_Y][S][4];  // per row: a_y, b_y, a_y(half), b_y(half) (0 outside the y frame)
+    const Blk B = xcd_block();
+    const int wy = __builtin_amdgcn_readfirstlane(threadIdx.y);
+    const Col col = column(L, P, B.bx * TILE_X + (int)threadIdx.x);
+    const StripCtx c = strip_ctx<H>(L, P, B, S, wy, col);
+    const int lane = c.lane;
+
+    const unsigned pb = (unsigned)(L.plane * 4);
+    const auto bhx = brsrc(F.f[0] + c.so, pb), bhy = brsrc(F.f[1] + c.so, pb), bez = brsrc(F.f[2] + c.so, pb);
+    const auto bcx = brsrc(C.c[2], pb), bcy = brsrc(C.c[3], pb);  // 1/mu_xz (hy), 1/mu_yz (hx)
+    float *__restrict__ px0 = P.psx[0] + (long long)c.shot * L.NY * c.W2;
+    float *__restrict__ py0 = P.psy[0] + (long long)c.shot * c.W2 * L.NX;
+    const int xi = col.xi;
+    const int hc = lane < H ? lane : TILE_X + lane;
+
+    YStage<H, S> ys;
+    ys.load(T.yf, T.yb, P, c.ya, lane);
+    float wez[N];
+    float hez[S], ohx[S], ohy[S], cxz[S], cyz[S], p0[S], q0[S];
+    auto load_group = [&](int g) {
+#pragma unroll
+        for (int i = 0; i < U; i++) {
+            const int r = g * U + i, y = c.ya + r, ro = boff(L, c.xl, y), rw = boff(L, c.xl, y + H);
+            wez[r + 2 * H] = bload(bez, c.vc, rw);
+            hez[r] = c.hl ? bload(bez, c.vh, ro) : 0.f;
+            ohx[r] = bload(bhx, c.vc, ro);
+            ohy[r] = bload(bhy, c.vc, ro);
+            cxz[r] = bload(bcx, c.vc, ro);
+            cyz[r] = bload(bcy, c.vc, ro);
+            p0[r] = q0[r] = 0.f;
+            if (y < L.NY) {
+                if (c.xf) p0[r] = px0[y * c.W2 + xi];
+                const int yi = P.on ? frame_index(y, L.NY, P.W, C.free_surface) : -1;
+                if (yi >= 0 && col.in) q0[r] = py0[yi * L.NX + c.x0 + lane];
+            }
+        }
+    };
+#pragma unroll
+    for (int i = 0; i < 2 * H; i++) wez[i] = bload(bez, c.vc, boff(L, c.xl, c.ya - H + i));
+    load_group(0);
+    float cxf[K];
+    load_taps<H>(cxf, T.xf, col.x);
+    ys.store(s_ty[wy], s_cy[wy], 0, lane);
+
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+        if (g + 1 < G) load_group(g + 1);
+#pragma unroll
+        for (int i = 0; i < U; i++) {
+            const int r = g * U + i;
+            s_ez[wy][i][H + lane] = wez[r + H];
+            if (c.hl) s_ez[wy][i][hc] = hez[r];
+        }
+        wave_sync();
+#pragma unroll
+        for (int i = 0; i < U; i++) {
+            const int r = g * U + i, y = c.ya + r;
+            if (y >= L.NY) break;
+            float u = ydiff<H, N>(wez, r + H, &s_ty[wy][0][r * K]);
+            float u2 = dtab<K>(&s_ez[wy][i][lane], cxf);
+            if (!col.in) continue;
+            if (c.xf) px0[y * c.W2 + xi] = cpml(u2, p0[r], col.axh, col.bxh);
+            const int yi = P.on ? frame_index(y, L.NY, P.W, C.free_surface) : -1;
+            if (yi >= 0) {
+                const float *cy = s_cy[wy][r];
+                py0[yi * L.NX + c.x0 + lane] = cpml(u, q0[r], cy[2], cy[3]);
+            }
+            const int ro = boff(L, c.xl, y);
+            u *= cyz[r];
+            bstore(ohx[r] - u, bhx, c.vc, ro);
+            float w = -u2;
+            w *= cxz[r];
+            bstore(ohy[r] - w, bhy, c.vc, ro);
+        }
+        wave_sync();
+    }
+}
+
+// TM-EM E update of one row (ForwardSolver2Dtmem.cpp:137-171):
+// ez = Ca ez + Cb (CPML(Dxb hy) - CPML(Dyb hx)), ABS damping, then the EZ sources / receivers.
+template <int H, int S, int U>
+__global__ __launch_bounds__(NTHREADS) void tm2d_e_strip(Layout2D L, Fields2D F, Coef2D C, Taps T, Cpml2D P, Acq A,
+                                                         int tofs, int pend)
+{
+    (void)pend;
+    constexpr int G = S / U, K = 2 * H + 1, N = S + 2 * H, RW = TILE_X + 2 * H;
+    static_assert(S % U == 0, "rows per wavefront must be a multiple of the group");
+    __shared__ float s_hy[BLOCK_Y][U][RW];
+    __shared__ float s_ty[BLOCK_Y][2][S * K];
+    __shared__ float s_cy[BLOCK_Y][S][4];
+    const Blk B = xcd_block();
+    const int wy = __builtin_amdgcn_readfirstlane(threadIdx.y);
+    const Col col = column(L, P, B.bx * TILE_X + (int)threadIdx.x);
+    const StripCtx c = strip_ctx<H>(L, P, B, S, wy, col);
+    const int lane = c.lane;
+
+    const unsigned pb = (unsigned)(L.plane * 4);
+    const auto bhx = brsrc(F.f[0] + c.so, pb), bhy = brsrc(F.f[1] + c.so, pb), bez = brsrc(F.f[2] + c.so, pb);
+    const auto bca = brsrc(C.c[0], pb), bcb = brsrc(C.c[1], pb);
+    float *__restrict__ px1 = P.psx[1] + (long long)c.shot * L.NY * c.W2;
+    float *__restrict__ py1 = P.psy[1] + (long long)c.shot * c.W2 * L.NX;
+    const int xi = col.xi;
+    const int hc = lane < H ? lane : TILE_X + lane;
+
+    YStage<H, S> ys;
+    ys.load(T.yb, T.yf, P, c.ya, lane);
+    float whx[N];
+    float ahy[S], hhy[S], oez[S], ca[S], cb[S], dmp[S], p1[S], q1[S];
+    auto load_group = [&](int g) {
+#pragma unroll
+        for (int i = 0; i < U; i++) {
+            const int r = g * U + i, y = c.ya + r, ro = boff(L, c.xl, y), rw = boff(L, c.xl, y + H);
+            whx[r + 2 * H] = bload(bhx, c.vc, rw);
+            ahy[r] = bload(bhy, c.vc, ro);
+            hhy[r] = c.hl ? bload(bhy, c.vh, ro) : 0.f;
+            oez[r] = bload(bez, c.vc, ro);
+            ca[r] = bload(bca, c.vc, ro);
+            cb[r] = bload(bcb, c.vc, ro);
+            dmp[r] = 1.f;
+            p1[r] = q1[r] = 0.f;
+            if (y < L.NY) {
+                if (C.damp) dmp[r] = C.damp[L.at(c.x0, y) + lane];
+                if (c.xf) p1[r] = px1[y * c.W2 + xi];
+                const int yi = P.on ? frame_index(y, L.NY, P.W, C.free_surface) : -1;
+                if (yi >= 0 && col.in) q1[r] = py1[yi * L.NX + c.x0 + lane];
+            }
+        }
+    };
+#pragma unroll
+    for (int i = 0; i < 2 * H; i++) whx[i] = bload(bhx, c.vc, boff(L, c.xl, c.ya - H + i));
+    load_group(0);
+    float cxb[K];
+    load_taps<H>(cxb, T.xb, col.x);
+    ys.store(s_ty[wy], s_cy[wy], 0, lane);
+
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+        if (g + 1 < G) load_group(g + 1);
+#pragma unroll
+        for (int i = 0; i < U; i++) {
+            const int r = g * U + i;
+            s_hy[wy][i][H + lane] = ahy[r];
+            if (c.hl) s_hy[wy][i][hc] = hhy[r];
+        }
+        wave_sync();
+#pragma unroll
+        for (int i = 0; i < U; i++) {
+            const int r = g * U + i, y = c.ya + r;
+            if (y >= L.NY) break;
+            float u = dtab<K>(&s_hy[wy][i][lane], cxb);
+            float t = ydiff<H, N>(whx, r + H, &s_ty[wy][0][r * K]);
+            if (!col.in) continue;
+            if (c.xf) px1[y * c.W2 + xi] = cpml(u, p1[r], col.ax, col.bx);
+            const int yi = P.on ? frame_index(y, L.NY, P.W, C.free_surface) : -1;
+            if (yi >= 0) {
+                const float *cy = s_cy[wy][r];
+                py1[yi * L.NX + c.x0 + lane] = cpml(t, q1[r], cy[0], cy[1]);
+            }
+            u -= t;
+            u *= cb[r];
+            float e = ca[r] * oez[r] + u;
+            e *= dmp[r];
+            bstore(e, bez, c.vc, boff(L, c.xl, y));
+        }
+        wave_sync();
+    }
+    block_acquisition<TMEM>(A, F, c.so, shot_block(A, B), A.tbase[0] + tofs, false);
+}
+
 template <typename Kern, typename... Args>
 void go(Kern k, dim3 grid, dim3 block, hipStream_t st, hipEvent_t e0, hipEvent_t e1, Args... args)
 {
@@ -499,9 +682,45 @@ void launch_s(int tile_y, int which, const Layout2D &L, int nshots, const Fields
         launch_hs<H, 4, 4>(which, L, nshots, F, C, T, P, A, tofs, pend, st, e0, e1, Sn, Rc);
 }
 
+template <int H>
+void launch_tm_s(int tile_y, int which, const Layout2D &L, int nshots, const Fields2D &F, const Coef2D &C,
+                 const Taps &T, const Cpml2D &P, const Acq &A, int tofs, int pend, hipStream_t st, hipEvent_t e0,
+                 hipEvent_t e1)
+{
+    const int S = tile_y == 8 ? 2 : (tile_y == 32 ? 8 : 4), TY = S * BLOCK_Y;
+    const dim3 grid((L.NX + TILE_X - 1) / TILE_X, (L.NY + TY - 1) / TY, nshots), block(TILE_X, BLOCK_Y);
+    auto pick = [&](auto kh, auto ke) {
+        if (which == 0)
+            go(kh, grid, block, st, e0, e1, L, F, C, T, P, A, tofs, pend);
+        else
+            go(ke, grid, block, st, e0, e1, L, F, C, T, P, A, tofs, pend);
+    };
+    if (S == 2)
+        pick(tm2d_h_strip<H, 2, 2>, tm2d_e_strip<H, 2, 2>);
+    else if (S == 8)
+        pick(tm2d_h_strip<H, 8, 8>, tm2d_e_strip<H, 8, 8>);
+    else
+        pick(tm2d_h_strip<H, 4, 4>, tm2d_e_strip<H, 4, 4>);
+}
+
 }  // namespace
 
 bool strip2d_fits(const Layout2D &L) { return L.plane * 4 < (1LL << 31); }
+
+void launch_strip2d_tm(int order, int tile_y, const Layout2D &L, int nshots, const Fields2D &F, const Coef2D &C,
+                       const Taps &T, const Cpml2D &P, const Acq &A, int tofs, int pend, hipStream_t stream, int which,
+                       hipEvent_t e0, hipEvent_t e1)
+{
+    switch (order / 2) {
+    case 1: launch_tm_s<1>(tile_y, which, L, nshots, F, C, T, P, A, tofs, pend, stream, e0, e1); break;
+    case 2: launch_tm_s<2>(tile_y, which, L, nshots, F, C, T, P, A, tofs, pend, stream, e0, e1); break;
+    case 3: launch_tm_s<3>(tile_y, which, L, nshots, F, C, T, P, A, tofs, pend, stream, e0, e1); break;
+    case 4: launch_tm_s<4>(tile_y, which, L, nshots, F, C, T, P, A, tofs, pend, stream, e0, e1); break;
+    case 5: launch_tm_s<5>(tile_y, which, L, nshots, F, C, T, P, A, tofs, pend, stream, e0, e1); break;
+    case 6: launch_tm_s<6>(tile_y, which, L, nshots, F, C, T, P, A, tofs, pend, stream, e0, e1); break;
+    default: break;
+    }
+}
 
 void launch_strip2d(int order, int tile_y, const Layout2D &L, int nshots, const Fields2D &F, const Coef2D &C,
                     const Taps &T, const Cpml2D &P, const Acq &A, int tofs, int pend, hipStream_t stream, int which,

@@ -201,6 +201,11 @@ void launch_step2d(int physics, int order, int tile_y, const Layout2D &L, int ns
                    hipStream_t stream, int which, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr,
                    const Snap2D *snap = nullptr);
 // velocity-type sources / receivers of step tbase + tofs (end of a run segment)
+// Row-strip TM-EM step (strip2d.hip, blocks as the elastic row-strip kernels below): which = 0
+// H update, 1 E update (+ ABS, EZ sources / receivers)
+void launch_strip2d_tm(int order, int tile_y, const Layout2D &L, int nshots, const Fields2D &F, const Coef2D &C,
+                       const Taps &T, const Cpml2D &P, const Acq &A, int tofs, int pend, hipStream_t stream, int which,
+                       hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 // Row-strip 2D elastic kernels (strip2d.hip): blocks of 64 columns x tile_y rows,
 // tile_y in {8, 16, 32} (each of the 4 wavefronts owns tile_y/4 rows); same
 // phases, acquisition lists and recorded quantities (snap) as launch_step2d.
