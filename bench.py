@@ -138,6 +138,7 @@ def _step_roofline(kernels, steps, alg_bytes, traffic_file=None):
 VARIANTS = {"phased-8": ("phased", 8), "phased-16": ("phased", 16), "phased-32": ("phased", 32),
             "fused-8": ("fused", 8), "fused-16": ("fused", 16), "fused-24": ("fused", 24),
             "strip-8": ("strip", 8), "strip-16": ("strip", 16), "strip-32": ("strip", 32),
+            "persistent-8": ("persistent", 8), "persistent-16": ("persistent", 16), "persistent-32": ("persistent", 32),
             "3d-z4": ("fused", 4), "3d-z8": ("fused", 8), "3d-z16": ("fused", 16)}
 
 

@@ -51,8 +51,11 @@ extern "C" {
                             where a single-pass kernel exists (2D elastic), else WV_PATH_PHASED */
 #define WV_PATH_PLAIN 1  /* reference operator sequence: CSR SpMV + vector passes, shot by shot */
 #define WV_PATH_PHASED 2 /* hand-written kernels, velocity-type and stress-type update as two launches */
-#define WV_PATH_STRIP 3  /* 2D elastic only: two launches of row-strip kernels (each wavefront owns
-                            tile_y/4 whole rows of 64 columns; y derivatives from registers) */
+#define WV_PATH_STRIP 3  /* 2D elastic and TM-EM: two launches of row-strip kernels (each wavefront
+                            owns tile_y/4 whole rows of 64 columns; y derivatives from registers) */
+#define WV_PATH_PERSISTENT 4 /* 2D TM-EM without ABS: one cooperative launch per run call runs every
+                                time step of every shot, grid barriers between the H and E updates
+                                (no hipGraph; tile_y 8, 16, 32) */
 
 typedef struct wv_solver wv_solver;
 

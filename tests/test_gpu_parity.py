@@ -277,3 +277,34 @@ def test_autotuner_keeps_parity():
     sol.run(0, 500)
     o32, _ = oracle_shot(c, c.shots[0], "f32")
     assert rel_l2(sol.seismograms(0), o32) <= TOL
+
+
+# ---- persistent TM-EM kernel (one cooperative launch per run call, every step of every shot)
+
+
+@pytest.mark.parametrize("tile_y", [8, 16, 32])
+def test_tmem_persistent_equals_two_launch_and_oracle(tile_y):
+    """The persistent loop runs the same tile code as the two-launch kernels in another order, so
+    its seismograms equal theirs to the bit; and the oracle's at the usual bar."""
+    c = cases.tmem_c3(NX=200, NY=120, NT=600, nshots=5, block=10, W=12)
+    pers = cases.make_solver(c, path="persistent", tile_y=tile_y)
+    pers.run(0, 300)
+    pers.run(300, 600)  # two cooperative launches: the step counter carries over
+    two = cases.make_solver(c, path="phased", tile_y=tile_y, steps_per_graph=50)
+    two.run(0, 600)
+    for k, sh in enumerate(c.shots):
+        assert np.array_equal(pers.seismograms(k), two.seismograms(k)), k
+        o32, _ = oracle_shot(c, sh, "f32")
+        assert rel_l2(pers.seismograms(k), o32) <= TOL, k
+
+
+def test_tmem_persistent_rejections():
+    c = cases.tmem_c3(NX=120, NY=80, NT=50, nshots=1, block=10, W=10)
+    s = dict(c.settings, damping_boundary=1, damping_coeff=8.0)
+    sol = wavesim.create("2D", "tmem")
+    with pytest.raises(wavesim.WaveError, match="persistent"):
+        sol.init_forward_solver(nshots=1, path="persistent", **s)
+    el = wavesim.create("2D", "elastic")
+    e = cases.elastic_c2(N=96, NT=50, W=12)
+    with pytest.raises(wavesim.WaveError, match="persistent"):
+        el.init_forward_solver(nshots=1, path="persistent", **e.settings)

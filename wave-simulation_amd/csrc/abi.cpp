@@ -144,6 +144,23 @@ int wv_solver::enqueue_steps(int nsteps, bool timed)
         kev.resize(4 * BATCH + 2);
         for (auto &e : kev) HIP_TRY(hipEventCreate(&e));
     }
+    if (variant == V_PERSIST) {
+        // one cooperative launch for all nsteps (timed: its events, counted as nsteps H+E steps)
+        if (nsteps <= 0) return 0;
+        if (timed) HIP_TRY(hipEventRecord(kev[0], st));
+        HIP_TRY(wv::launch_tm2d_persistent(order, tile_y, L, cfg.nshots, fields2d(0), C, T, cpml2d(0), A, nsteps, st));
+        if (timed) {
+            HIP_TRY(hipEventRecord(kev[1], st));
+            HIP_TRY(hipEventSynchronize(kev[1]));
+            float ms = 0;
+            HIP_TRY(hipEventElapsedTime(&ms, kev[0], kev[1]));
+            kms[0] += ms;
+            klaunch[0] += nsteps;
+        }
+        wv::launch_advance(d_tbase, nsteps, st);
+        HIP_TRY(hipGetLastError());
+        return 0;
+    }
     const int nk = variant == V_SINGLE ? 1 : 2;
     int pending = 0;
     auto collect = [&](int n) -> int {
@@ -250,7 +267,15 @@ double wv_solver::kernel_bytes(int kind) const
         {{4, 1, 1, 1}, {5, 4, 1, 1}},  // emem: hz, (ex, ey)
     };
     static const int single[4] = {10, 5, 4, 4};  // elastic single pass
-    if (variant == V_SINGLE && kind == 1) return 0.0;
+    if ((variant == V_SINGLE || variant == V_PERSIST) && kind == 1) return 0.0;
+    if (variant == V_PERSIST) {
+        // one launch per step loop, counted per time step: the H and E updates together
+        wv_solver *self = const_cast<wv_solver *>(this);
+        self->variant = V_PHASED;
+        const double b = kernel_bytes(0) + kernel_bytes(1);
+        self->variant = V_PERSIST;
+        return b;
+    }
     const int *t = variant == V_SINGLE ? single : phased[physics][kind];
     double bytes = 4.0 * (cells * B * t[0] + cells * t[1]);
     if (cfg.damping_boundary == 2) {
@@ -347,8 +372,13 @@ int wv_init(wv_solver *s, const wv_config *c)
     if (k.damping_boundary == 1 && !(k.damping_coeff > 0 && k.damping_coeff < 100))
         return fail(WV_EINVAL, "DampingCoeff must be in (0,100)");
     if (k.nshots < 1) return fail(WV_EINVAL, "nshots must be >= 1");
-    if (k.path != WV_PATH_FUSED && k.path != WV_PATH_PLAIN && k.path != WV_PATH_PHASED && k.path != WV_PATH_STRIP)
+    if (k.path != WV_PATH_FUSED && k.path != WV_PATH_PLAIN && k.path != WV_PATH_PHASED && k.path != WV_PATH_STRIP &&
+        k.path != WV_PATH_PERSISTENT)
         return fail(WV_EINVAL, "bad path");
+    if (k.path == WV_PATH_PERSISTENT && (s->dim != 2 || s->physics != wv::TMEM || s->visco || k.damping_boundary == 1))
+        return fail(WV_EINVAL, "the persistent kernel is 2D TM-EM without ABS only");
+    if (k.path == WV_PATH_PERSISTENT && k.tile_y != 0 && k.tile_y != 8 && k.tile_y != 16 && k.tile_y != 32)
+        return fail(WV_EINVAL, "tile_y of the persistent kernel must be 0 (default), 8, 16 or 32");
     if (k.path == WV_PATH_STRIP && (s->dim != 2 || (s->physics != wv::ELASTIC && s->physics != wv::TMEM) || s->visco))
         return fail(WV_EINVAL, "the row-strip kernels are 2D elastic and TM-EM only");
     if (s->visco && k.path == WV_PATH_PLAIN)
@@ -392,13 +422,14 @@ int wv_init(wv_solver *s, const wv_config *c)
         return WV_OK;
     }
 
-    s->variant = single ? V_SINGLE : (k.path == WV_PATH_STRIP ? V_STRIP : V_PHASED);
+    s->variant = single ? V_SINGLE
+                        : (k.path == WV_PATH_STRIP ? V_STRIP : (k.path == WV_PATH_PERSISTENT ? V_PERSIST : V_PHASED));
     s->H = N / 2;
     if (s->variant == V_SINGLE) {
         s->tile_x = wv::fused2d_tile_x(N);
         s->tile_y = k.tile_y ? std::abs(k.tile_y) : 16;
         s->tile_load = k.tile_y < 0 ? -1 : 1;
-    } else if (s->variant == V_STRIP) {
+    } else if (s->variant == V_STRIP || s->variant == V_PERSIST) {
         s->tile_x = wv::TILE_X;
         s->tile_y = k.tile_y ? k.tile_y : 16;
     } else {
@@ -1016,7 +1047,8 @@ int wv_run(wv_solver *s, int t_start, int t_end)
         if (ck && t % ck == 0 && (rc = adjoint_checkpoint(s, t))) return rc;
         int n = t_end - t;
         if (ck) n = std::min(n, ck - t % ck);
-        if (K > 0 && n >= K + s->cur) {
+        // (the persistent kernel is one cooperative launch per segment already: no graph)
+        if (K > 0 && n >= K + s->cur && s->variant != V_PERSIST) {
             if (s->cur) {  // graphs start in buffer 0
                 if ((rc = eager(1))) return rc;
                 continue;
@@ -1155,13 +1187,15 @@ int wv_tune(wv_solver *s, int nsteps, int *choice)
     // candidates: two-launch kernels with 8/16/32-row tiles; for elastic P-SV
     // also the single-pass kernel with register-staged tiles of 8/16/24 rows
     // and the row-strip kernels with 8/16/32-row blocks
-    const int cand_var[9] = {V_PHASED, V_PHASED, V_PHASED, V_SINGLE, V_SINGLE, V_SINGLE, V_STRIP, V_STRIP, V_STRIP};
-    const int cand_ty[9] = {8, 16, 32, 8, 16, 24, 8, 16, 32};
-    const int cand_id[9] = {0, 1, 2, 3, 4, 5, 9, 10, 11};
+    // (and, for TM-EM without ABS, the persistent kernel with 16-row tiles)
+    const int cand_var[10] = {V_PHASED, V_PHASED, V_PHASED, V_SINGLE, V_SINGLE, V_SINGLE, V_STRIP, V_STRIP, V_STRIP,
+                              V_PERSIST};
+    const int cand_ty[10] = {8, 16, 32, 8, 16, 24, 8, 16, 32, 16};
+    const int cand_id[10] = {0, 1, 2, 3, 4, 5, 9, 10, 11, 12};
     // (TM-EM: the two-launch and the row-strip kernels)
     const bool all2d = s->physics == wv::ELASTIC && s->cfg.path == WV_PATH_FUSED && !s->visco;
     const bool tm = s->physics == wv::TMEM && s->cfg.path == WV_PATH_FUSED && !s->visco;
-    const int ncand = (all2d || tm) ? (wv::strip2d_fits(s->L) ? 9 : 6) : 3;
+    const int ncand = (all2d || tm) ? (wv::strip2d_fits(s->L) ? (tm && s->cfg.damping_boundary != 1 ? 10 : 9) : 6) : 3;
     float best = 1e30f;
     const int k0 = s->cfg.path == WV_PATH_STRIP ? 6 : 0, k1 = s->cfg.path == WV_PATH_STRIP ? 9 : ncand;
     // with checkpointing on, the adjoint recomputes with a two-launch pair: the
@@ -1252,6 +1286,7 @@ int wv_kernel_stats(wv_solver *s, int kind, const char **name, double *avg_ms, l
     if (name)
         *name = kind >= 3 ? adj_names[kind - 3]
                           : ((s->variant == V_SINGLE && kind < 2)   ? (kind == 0 ? "el2d_fused" : "-")
+                             : (s->variant == V_PERSIST && kind < 2) ? (kind == 0 ? "tm2d_persistent" : "")
                              : (s->variant == V_STRIP && kind < 2 && s->physics == wv::TMEM)
                                  ? (kind == 0 ? "tm2d_h_strip" : "tm2d_e_strip")
                              : (s->variant == V_STRIP && kind < 2) ? (kind == 0 ? "el2d_velocity_strip" : "el2d_stress_strip")

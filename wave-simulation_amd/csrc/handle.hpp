@@ -39,7 +39,7 @@ struct ShotAcq {
     std::vector<int> rtype;
 };
 
-enum Variant { V_SINGLE = 0, V_PHASED = 1, V_STRIP = 2 };
+enum Variant { V_SINGLE = 0, V_PHASED = 1, V_STRIP = 2, V_PERSIST = 3 };
 
 struct AdjointState;
 

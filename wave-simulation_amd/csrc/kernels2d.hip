@@ -16,6 +16,10 @@
 // kernel streams at memory-level parallelism instead of serialised round trips.
 // Derivative taps along x sit in registers (one column per lane), taps along y
 // in scalar registers (one row per wavefront).
+#include <algorithm>
+
+#include <hip/hip_cooperative_groups.h>
+
 #include "acq2d.hpp"
 
 namespace wv {
@@ -369,19 +373,25 @@ __global__ __launch_bounds__(NTHREADS) void ac2d_pressure(Layout2D L, Fields2D F
 // ---------------------------------------------------------------------------
 // TM-mode EM
 // ---------------------------------------------------------------------------
+// LDS of one TM-EM H tile
 template <int H, int TY>
-__global__ __launch_bounds__(NTHREADS) void tm2d_h(Layout2D L, Fields2D F, Coef2D C, Taps T, Cpml2D P, Acq A,
-                                                   int tofs, int pend)
+struct TmHLds {
+    float ez[Tile<H, TY>::RH][Tile<H, TY>::RW];
+    float ty[TY][2 * H + 1];  // Dyf taps of the tile's rows
+    float cy[TY][2];          // a_y(half), b_y(half) of the tile's rows (0 outside the frame)
+};
+
+// H update of tile B (called by tm2d_h with the block's own tile, and tile after tile by the
+// persistent kernel; the LDS is reused only after a barrier)
+template <int H, int TY>
+__device__ __forceinline__ void tm_h_tile(const Blk &B, const Layout2D &L, const Fields2D &F, const Coef2D &C,
+                                          const Taps &T, const Cpml2D &P, TmHLds<H, TY> &S)
 {
-    (void)A;
-    (void)tofs;
-    (void)pend;
     using TL = Tile<H, TY>;
     constexpr int R = TY / BLOCK_Y;
-    __shared__ float s_ez[TL::RH][TL::RW];
-    __shared__ float s_ty[TY][2 * H + 1];  // Dyf taps of the tile's rows
-    __shared__ float s_cy[TY][2];          // a_y(half), b_y(half) of the tile's rows (0 outside the frame)
-    const Blk B = xcd_block();
+    auto &s_ez = S.ez;
+    auto &s_ty = S.ty;
+    auto &s_cy = S.cy;
     const int shot = B.shot;
     const long long so = (long long)shot * L.plane;
     const int x0 = B.bx * TILE_X, y0 = B.by * TY;
@@ -443,20 +453,40 @@ __global__ __launch_bounds__(NTHREADS) void tm2d_h(Layout2D L, Fields2D F, Coef2
     }
 }
 
+template <int H, int TY>
+__global__ __launch_bounds__(NTHREADS) void tm2d_h(Layout2D L, Fields2D F, Coef2D C, Taps T, Cpml2D P, Acq A,
+                                                   int tofs, int pend)
+{
+    (void)A;
+    (void)tofs;
+    (void)pend;
+    __shared__ TmHLds<H, TY> S;
+    tm_h_tile<H, TY>(xcd_block(), L, F, C, T, P, S);
+}
+
 // VE: visco TM-EM (ForwardSolver2Dviscotmem.cpp:172-199): per mechanism r_l = Cd_l ez + Cc_l r_l
 // from the ez of the previous step, then ez = Ca ez + Cb ((Dxb hy - Dyb hx) - sum_l DT r_l)
-template <int H, int TY, bool VE = false>
-__global__ __launch_bounds__(NTHREADS) void tm2d_e(Layout2D L, Fields2D F, Coef2D C, Taps T, Cpml2D P, Acq A,
-                                                   int tofs, int pend, Visco2D V)
+// LDS of one TM-EM E tile
+template <int H, int TY>
+struct TmELds {
+    float hx[Tile<H, TY>::RH][Tile<H, TY>::RW];
+    float hy[Tile<H, TY>::RH][Tile<H, TY>::RW];
+    float ty[TY][2 * H + 1];  // Dyb taps of the tile's rows
+    float cy[TY][2];          // a_y, b_y of the tile's rows (0 outside the frame)
+};
+
+// E update of tile B at step tbase + tofs, then its EZ sources / receivers
+template <int H, int TY, bool VE>
+__device__ __forceinline__ void tm_e_tile(const Blk &B, const Layout2D &L, const Fields2D &F, const Coef2D &C,
+                                          const Taps &T, const Cpml2D &P, const Acq &A, int tofs, const Visco2D &V,
+                                          TmELds<H, TY> &S)
 {
-    (void)pend;
     using TL = Tile<H, TY>;
     constexpr int R = TY / BLOCK_Y;
-    __shared__ float s_hx[TL::RH][TL::RW];
-    __shared__ float s_hy[TL::RH][TL::RW];
-    __shared__ float s_ty[TY][2 * H + 1];  // Dyb taps of the tile's rows
-    __shared__ float s_cy[TY][2];          // a_y, b_y of the tile's rows (0 outside the frame)
-    const Blk B = xcd_block();
+    auto &s_hx = S.hx;
+    auto &s_hy = S.hy;
+    auto &s_ty = S.ty;
+    auto &s_cy = S.cy;
     const int shot = B.shot;
     const long long so = (long long)shot * L.plane;
     const int x0 = B.bx * TILE_X, y0 = B.by * TY;
@@ -528,6 +558,50 @@ __global__ __launch_bounds__(NTHREADS) void tm2d_e(Layout2D L, Fields2D F, Coef2
         ez[L.at(col.x, y)] = e;
     }
     block_acquisition<TMEM>(A, F, so, shot_block(A, B), A.tbase[0] + tofs, false);
+}
+
+template <int H, int TY, bool VE = false>
+__global__ __launch_bounds__(NTHREADS) void tm2d_e(Layout2D L, Fields2D F, Coef2D C, Taps T, Cpml2D P, Acq A,
+                                                   int tofs, int pend, Visco2D V)
+{
+    (void)pend;
+    __shared__ TmELds<H, TY> S;
+    tm_e_tile<H, TY, VE>(xcd_block(), L, F, C, T, P, A, tofs, V, S);
+}
+
+// Persistent TM-EM step loop: one cooperative launch runs nsteps time steps of every shot.  The
+// resident blocks take the tiles of the H update in turn, meet at a grid barrier, take the tiles of
+// the E update (with its sources / receivers), meet again; between two tiles of one block the LDS
+// is reused after a block barrier.  ABS is not supported here (the two-launch path damps H in a
+// third launch).  Tiles: as the two-launch kernels' grid, numbered shot-major.
+template <int H, int TY>
+__global__ __launch_bounds__(NTHREADS) void tm2d_persistent(Layout2D L, Fields2D F, Coef2D C, Taps T, Cpml2D P,
+                                                            Acq A, int nsteps, int nbx, int nby, int nshots)
+{
+    __shared__ TmHLds<H, TY> SH;
+    __shared__ TmELds<H, TY> SE;
+    cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+    const int ntiles = nbx * nby * nshots;
+    for (int j = 0; j < nsteps; j++) {
+        for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+            Blk B;
+            B.bx = t % nbx;
+            B.by = (t / nbx) % nby;
+            B.shot = t / (nbx * nby);
+            tm_h_tile<H, TY>(B, L, F, C, T, P, SH);
+            __syncthreads();
+        }
+        grid.sync();
+        for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+            Blk B;
+            B.bx = t % nbx;
+            B.by = (t / nbx) % nby;
+            B.shot = t / (nbx * nby);
+            tm_e_tile<H, TY, false>(B, L, F, C, T, P, A, j, Visco2D{}, SE);
+            __syncthreads();
+        }
+        grid.sync();
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -954,6 +1028,56 @@ void launch_step2d(int physics, int order, int tile_y, const Layout2D &L, int ns
     case 5: launch_ty2<5>(tile_y, physics, which, L, nshots, F, C, T, P, A, tofs, pend, stream, e0, e1, S); break;
     case 6: launch_ty2<6>(tile_y, physics, which, L, nshots, F, C, T, P, A, tofs, pend, stream, e0, e1, S); break;
     default: break;
+    }
+}
+
+namespace {
+template <int H, int TY>
+hipError_t tm_persist_ty(const Layout2D &L, int nshots, const Fields2D &F, const Coef2D &C, const Taps &T,
+                         const Cpml2D &P, const Acq &A, int nsteps, hipStream_t st, int *grid_out)
+{
+    auto k = tm2d_persistent<H, TY>;
+    int dev = 0, cus = 0, nb = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k, NTHREADS, 0);
+    if (e != hipSuccess) return e;
+    int nbx = (L.NX + TILE_X - 1) / TILE_X, nby = (L.NY + TY - 1) / TY, ns = nshots, steps = nsteps;
+    // every block of a cooperative launch must be resident at once
+    const int grid = std::min(nbx * nby * ns, nb * cus);
+    if (grid < 1) return hipErrorInvalidConfiguration;
+    if (grid_out) *grid_out = grid;
+    Layout2D L_ = L;
+    Fields2D F_ = F;
+    Coef2D C_ = C;
+    Taps T_ = T;
+    Cpml2D P_ = P;
+    Acq A_ = A;
+    void *args[] = {&L_, &F_, &C_, &T_, &P_, &A_, &steps, &nbx, &nby, &ns};
+    return hipLaunchCooperativeKernel((const void *)k, dim3(grid), dim3(TILE_X, BLOCK_Y), args, 0, st);
+}
+template <int H>
+hipError_t tm_persist_h(int tile_y, const Layout2D &L, int nshots, const Fields2D &F, const Coef2D &C, const Taps &T,
+                        const Cpml2D &P, const Acq &A, int nsteps, hipStream_t st, int *grid_out)
+{
+    if (tile_y == 8) return tm_persist_ty<H, 8>(L, nshots, F, C, T, P, A, nsteps, st, grid_out);
+    if (tile_y == 32) return tm_persist_ty<H, 32>(L, nshots, F, C, T, P, A, nsteps, st, grid_out);
+    return tm_persist_ty<H, 16>(L, nshots, F, C, T, P, A, nsteps, st, grid_out);
+}
+}  // namespace
+
+hipError_t launch_tm2d_persistent(int order, int tile_y, const Layout2D &L, int nshots, const Fields2D &F,
+                                  const Coef2D &C, const Taps &T, const Cpml2D &P, const Acq &A, int nsteps,
+                                  hipStream_t stream, int *grid_out)
+{
+    switch (order / 2) {
+    case 1: return tm_persist_h<1>(tile_y, L, nshots, F, C, T, P, A, nsteps, stream, grid_out);
+    case 2: return tm_persist_h<2>(tile_y, L, nshots, F, C, T, P, A, nsteps, stream, grid_out);
+    case 3: return tm_persist_h<3>(tile_y, L, nshots, F, C, T, P, A, nsteps, stream, grid_out);
+    case 4: return tm_persist_h<4>(tile_y, L, nshots, F, C, T, P, A, nsteps, stream, grid_out);
+    case 5: return tm_persist_h<5>(tile_y, L, nshots, F, C, T, P, A, nsteps, stream, grid_out);
+    case 6: return tm_persist_h<6>(tile_y, L, nshots, F, C, T, P, A, nsteps, stream, grid_out);
+    default: return hipErrorInvalidValue;
     }
 }
 

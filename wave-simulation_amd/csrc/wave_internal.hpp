@@ -201,6 +201,12 @@ void launch_step2d(int physics, int order, int tile_y, const Layout2D &L, int ns
                    hipStream_t stream, int which, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr,
                    const Snap2D *snap = nullptr);
 // velocity-type sources / receivers of step tbase + tofs (end of a run segment)
+// Persistent TM-EM loop (kernels2d.hip): nsteps time steps of every shot in one cooperative launch
+// of as many blocks as can be resident (grid barriers between the H and E updates); no ABS.
+// grid_out (optional) receives the block count.  Returns the launch status.
+hipError_t launch_tm2d_persistent(int order, int tile_y, const Layout2D &L, int nshots, const Fields2D &F,
+                                  const Coef2D &C, const Taps &T, const Cpml2D &P, const Acq &A, int nsteps,
+                                  hipStream_t stream, int *grid_out = nullptr);
 // Row-strip TM-EM step (strip2d.hip, blocks as the elastic row-strip kernels below): which = 0
 // H update, 1 E update (+ ABS, EZ sources / receivers)
 void launch_strip2d_tm(int order, int tile_y, const Layout2D &L, int nshots, const Fields2D &F, const Coef2D &C,

@@ -18,6 +18,7 @@ WV_PATH_FUSED = 0
 WV_PATH_PLAIN = 1
 WV_PATH_PHASED = 2
 WV_PATH_STRIP = 3
+WV_PATH_PERSISTENT = 4
 WV_ENONFINITE = -5
 
 # every entry point of include/wave_abi.h

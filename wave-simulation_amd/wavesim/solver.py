@@ -70,7 +70,7 @@ class ForwardSolver:
         cfg.damping_coeff, cfg.npower, cfg.fc_cpml, cfg.vmax_cpml = damping_coeff, npower, fc_cpml, vmax_cpml
         cfg.nshots = nshots
         cfg.path = {"fused": _abi.WV_PATH_FUSED, "plain": _abi.WV_PATH_PLAIN, "phased": _abi.WV_PATH_PHASED,
-                    "strip": _abi.WV_PATH_STRIP}[path]
+                    "strip": _abi.WV_PATH_STRIP, "persistent": _abi.WV_PATH_PERSISTENT}[path]
         cfg.steps_per_graph = steps_per_graph
         cfg.tile_y = tile_y
         _abi.check(_abi.load().wv_init(self._h, C.byref(cfg)))
@@ -129,7 +129,7 @@ class ForwardSolver:
         _abi.check(_abi.load().wv_run(self._h, int(t_start), int(t_end)))
 
     TUNE_CHOICES = ("phased-8", "phased-16", "phased-32", "fused-8", "fused-16", "fused-24", "3d-z4", "3d-z8",
-                    "3d-z16", "strip-8", "strip-16", "strip-32")
+                    "3d-z16", "strip-8", "strip-16", "strip-32", "persistent-16")
 
     def tune(self, nsteps=100):
         """Time every kernel variant on this problem, keep the fastest (resets the wavefields)."""
