@@ -355,3 +355,82 @@ def test_cli_visco_em_models_from_files(tmp_path, eq):
         e = rel_l2(seis, o32[rows])
         print(f"{eq} {names[t]}: {len(rows)} traces, relL2 vs f32 oracle {e:.2e}")
         assert np.abs(o32[rows]).max() > 0 and e <= 1e-5
+
+
+def _write_su(path, traces, coords, DH, dt_us=100):
+    """SU file: per trace a 240-byte header (coordinates in millimetres, scalel = scalco = -3) and
+    FP32 samples.  coords: per trace (kind, x, y, z) grid points, kind "s" (sx, sdepth, sy) or "g"
+    (gx, gelev, gy)."""
+    import struct
+
+    with open(path, "wb") as f:
+        for tr, (kind, x, y, z) in zip(traces, coords):
+            h = bytearray(240)
+            mm = [int(round(v * DH * 1000.0)) for v in (x, y, z)]
+            struct.pack_into("<hh", h, 68, -3, -3)
+            if kind == "s":
+                struct.pack_into("<i", h, 48, mm[1])
+                struct.pack_into("<ii", h, 72, mm[0], mm[2])
+            else:
+                struct.pack_into("<i", h, 40, mm[1])
+                struct.pack_into("<ii", h, 80, mm[0], mm[2])
+            struct.pack_into("<HH", h, 114, len(tr), dt_us)
+            f.write(bytes(h))
+            f.write(np.asarray(tr, np.float32).tobytes())
+
+
+def test_cli_acquisition_from_su_files(tmp_path):
+    """initSourcesFromSU / initReceiverFromSU (Sources.cpp:231-249, 500-515; suHandler.cpp:13-113):
+    sources from <SourceFilename>.{p,vx,vy,vz}.su headers (all four files, here two empty), each
+    of wavelet type 3 reading trace 0 of the SU signal file with the amplitude the SU header does not
+    set (0: the signal enters negated, Sources.cpp:354-375), all in shot 0; receivers from
+    <ReceiverFilename>.<comp>.su (missing components skipped).  The seismograms equal the oracle's
+    float build on the same acquisition.  (No reference fixture uses SU acquisition: the parity of
+    these reading rules is pinned only by this restatement of suHandler.cpp.)"""
+    import wavesim
+    from _cases import oracle_threads, rel_l2
+
+    NX, NY, NT, DT, DH, W = 120, 90, 500, 1e-4, 1.0, 12
+    rng = np.random.default_rng(11)
+    vp = np.full(NX * NY, 3000.0)
+    vs = np.kron(rng.uniform(1300.0, 1700.0, (8, 10)), np.ones((12, 12)))[:NY, :NX].ravel()
+    rho = np.full(NX * NY, 2000.0)
+    base = tmp_path / "model"
+    for k, v in (("vp", vp), ("vs", vs), ("density", rho)):
+        _write_vec(f"{base}.{k}.mtx", v.astype(np.float32))
+    sig = wavesim.source_signal(1, NT, DT, 40.0, 1.0, 0.0)
+    _write_su(tmp_path / "sig.su", [sig, 0.5 * sig], [("s", 0, 0, 0)] * 2, DH)
+    srcs = [(40, 45, 1), (75, 30, 3)]  # a P source and a VY source
+    _write_su(tmp_path / "src.p.su", [np.zeros(NT)], [("s", 40, 45, 0)], DH)
+    _write_su(tmp_path / "src.vy.su", [np.zeros(NT)], [("s", 75, 30, 0)], DH)
+    for comp in ("vx", "vz"):
+        open(tmp_path / f"src.{comp}.su", "wb").close()
+    recs = [(x, 20, 2) for x in range(15, NX - 15, 6)] + [(x, 20, 3) for x in range(18, NX - 15, 6)]
+    _write_su(tmp_path / "rec.vx.su", [np.zeros(NT) for r in recs if r[2] == 2],
+              [("g", x, y, 0) for x, y, t in recs if t == 2], DH)
+    _write_su(tmp_path / "rec.vy.su", [np.zeros(NT) for r in recs if r[2] == 3],
+              [("g", x, y, 0) for x, y, t in recs if t == 3], DH)
+    out = tmp_path / "seis"
+    args = [EXE, os.path.join(CI, "configuration_ci.2D.acoustic.txt"), "equationType=elastic", f"NX={NX}", f"NY={NY}",
+            f"DH={DH}", f"DT={DT}", f"T={NT * DT!r}", f"seismoDT={DT!r}", "spatialFDorder=8", "useStencilMatrix=0",
+            "DampingBoundary=2", f"BoundaryWidth={W}", "CenterFrequencyCPML=40", "VMaxCPML=3000", "ModelRead=1",
+            f"ModelFilename={base}", "initSourcesFromSU=1", "initReceiverFromSU=1",
+            f"SourceFilename={tmp_path / 'src'}", f"SourceSignalFilename={tmp_path / 'sig.su'}",
+            f"ReceiverFilename={tmp_path / 'rec'}", f"SeismogramFilename={out}", "SeismogramFormat=1",
+            "normalizeTraces=0"]
+    r = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    p = po.make_params(dim=2, physics=1, NX=NX, NY=NY, NT=NT, order=8, damping=2, width=W, DH=DH, DT=DT,
+                       npower=4.0, fc_cpml=40.0, vmax_cpml=3000.0, nthreads=oracle_threads())
+    # every SU source reads trace 0 and enters negated
+    ssig = np.stack([-sig, -sig]).astype(np.float32)
+    o32, _ = po.run(p, vp.astype(np.float32), vs.astype(np.float32), rho.astype(np.float32),
+                    [x + y * NX for x, y, _ in srcs], [t for _, _, t in srcs], ssig,
+                    [x + y * NX for x, y, _ in recs], [t for _, _, t in recs], "f32")
+    for t, name in ((2, "vx"), (3, "vy")):
+        rows = [k for k, rr in enumerate(recs) if rr[2] == t]
+        seis = po.read_mtx_array(f"{out}.shot_0.{name}.mtx")
+        e = rel_l2(seis, o32[rows])
+        print(f"SU acquisition {name}: {len(rows)} traces, relL2 vs f32 oracle {e:.2e}")
+        assert np.abs(o32[rows]).max() > 0 and e <= 1e-5

@@ -105,12 +105,120 @@ void writeSU(const std::string &fn, const std::vector<float> &d, int ntr, int ns
     if (!f) throw std::runtime_error("write failed: " + fn);
 }
 
-// Sources::readSignalFromFile (Sources.cpp:231-249) with initSourcesFromSU = 0: row `row`
-// of <SourceSignalFilename>.mtx (SeismogramFormat 1; IO.hpp:172-195), NT samples
+// One trace header of an SU file (the fields acquisition input reads; SEG-Y byte offsets:
+// sdepth 48, scalel 68, scalco 70, sx 72, sy 76, gx 80, gy 84, gelev 40, ns 114).
+struct SuHeader {
+    int32_t gelev, sdepth, sx, sy, gx, gy;
+    int16_t scalel, scalco;
+    uint16_t ns;
+};
+
+// SUIO::readHeaderSU (SUIO.hpp:380-405): every trace header of an SU file (native endian)
+std::vector<SuHeader> readSuHeaders(const std::string &fn)
+{
+    std::ifstream f(fn, std::ios::binary);
+    if (!f) throw std::runtime_error("Error opening file: " + fn);
+    std::vector<SuHeader> out;
+    unsigned char h[240];
+    while (f.read((char *)h, sizeof(h))) {
+        SuHeader t;
+        std::memcpy(&t.gelev, h + 40, 4);
+        std::memcpy(&t.sdepth, h + 48, 4);
+        std::memcpy(&t.scalel, h + 68, 2);
+        std::memcpy(&t.scalco, h + 70, 2);
+        std::memcpy(&t.sx, h + 72, 4);
+        std::memcpy(&t.sy, h + 76, 4);
+        std::memcpy(&t.gx, h + 80, 4);
+        std::memcpy(&t.gy, h + 84, 4);
+        std::memcpy(&t.ns, h + 114, 2);
+        out.push_back(t);
+        f.seekg((std::streamoff)t.ns * 4, std::ios::cur);
+    }
+    return out;
+}
+
+// SUIO::readSingleDataSU (SUIO.hpp:350-372): the samples of trace `trace` (every trace of the
+// file holds the first header's ns samples)
+std::vector<float> readSuTrace(const std::string &fn, long trace)
+{
+    std::ifstream f(fn, std::ios::binary);
+    if (!f) throw std::runtime_error("Error opening file: " + fn);
+    unsigned char h[240];
+    if (!f.read((char *)h, sizeof(h))) throw std::runtime_error("empty SU file: " + fn);
+    uint16_t ns;
+    std::memcpy(&ns, h + 114, 2);
+    f.seekg((std::streamoff)(240 + 4 * (std::streamoff)ns) * trace, std::ios::cur);
+    std::vector<float> d(ns);
+    if (!f.read((char *)d.data(), 4 * (std::streamoff)ns)) throw std::runtime_error("short SU file: " + fn);
+    return d;
+}
+
+// Acquisition components in the order of SeismogramTypeString (Acquisition.hpp:28-32): the SU
+// acquisition files are <name>.p.su, .vx.su, .vy.su, .vz.su (the seismic names, also for EM)
+const char *const kSuComp[4] = {"p", "vx", "vy", "vz"};
+
+// suHandler::readAllSettingsFromSU for sources (suHandler.cpp:13-27, 63-90): every trace of the
+// four component files is a source of that component at grid point
+// x = int(sx 10^scalco / DH + 0.5), y = int(sdepth 10^scalel / DH + 0.5), z = int(sy 10^scalco / DH
+// + 0.5) (value-type arithmetic); the settings the SU header has no field for stay zero: every
+// source belongs to shot 0, reads row 0 of the signal file (wavelet type 3) and has amp 0.  All four
+// files must exist (readHeaderSU throws otherwise).
+std::vector<SourceRow> readSourcesSU(const std::string &base, float DH)
+{
+    std::vector<SourceRow> out;
+    for (int c = 0; c < 4; c++) {
+        for (const SuHeader &h : readSuHeaders(base + "." + kSuComp[c] + ".su")) {
+            const float sco = std::pow(10.f, (float)h.scalco), sel = std::pow(10.f, (float)h.scalel);
+            SourceRow r{};
+            r.x = (int)((double)((float)h.sx * sco / DH) + 0.5);
+            r.y = (int)((double)((float)h.sdepth * sel / DH) + 0.5);
+            r.z = (int)((double)((float)h.sy * sco / DH) + 0.5);
+            r.type = c + 1;
+            r.wavelet = 3;
+            out.push_back(r);
+        }
+    }
+    return out;
+}
+
+// suHandler::readAllSettingsFromSU for receivers (suHandler.cpp:29-49, 92-113): as for sources
+// with gx, gelev, gy and no rounding (truncation); missing component files are skipped, all four
+// missing is an error.
+std::vector<ReceiverRow> readReceiversSU(const std::string &base, float DH)
+{
+    std::vector<ReceiverRow> out;
+    int missing = 0;
+    for (int c = 0; c < 4; c++) {
+        std::vector<SuHeader> hs;
+        try {
+            hs = readSuHeaders(base + "." + kSuComp[c] + ".su");
+        } catch (const std::runtime_error &) {
+            if (++missing == 4) throw std::runtime_error("No file with name: " + base + ".'comp'.su could be read");
+            continue;
+        }
+        for (const SuHeader &h : hs) {
+            const float sco = std::pow(10.f, (float)h.scalco), sel = std::pow(10.f, (float)h.scalel);
+            ReceiverRow r{};
+            r.x = (int)((float)h.gx * sco / DH);
+            r.y = (int)((float)h.gelev * sel / DH);
+            r.z = (int)((float)h.gy * sco / DH);
+            r.type = c + 1;
+            out.push_back(r);
+        }
+    }
+    return out;
+}
+
+// Sources::readSignalFromFile (Sources.cpp:231-249): with initSourcesFromSU = 1 trace `row` of the
+// SU file <SourceSignalFilename> as given, else row `row` of <SourceSignalFilename>.mtx
+// (SeismogramFormat 1; IO.hpp:172-195); NT samples
 std::vector<float> signalFromFile(const Configuration &config, long row, int NT)
 {
-    if (config.getInt("initSourcesFromSU", 0) != 0)
-        throw std::runtime_error("initSourcesFromSU=1 (acquisition and signals from SU files) is not read");
+    if (config.getInt("initSourcesFromSU", 0) != 0) {
+        std::vector<float> d = readSuTrace(config.getString("SourceSignalFilename"), row);
+        if ((long)d.size() != NT) throw std::runtime_error("Source signal has invalid length");
+        return d;
+    }
     const int format = (int)config.getInt("SeismogramFormat", 1);
     if (format == 2 || format == 3)
         throw std::runtime_error("source signal files in the lmf/frv formats are not read (SeismogramFormat 1 = mtx)");
@@ -246,7 +354,10 @@ int main(int argc, char **argv)
         }
 
         // acquisition: shots in order of first appearance in the source file
-        const std::vector<SourceRow> src = readSources(config.getString("SourceFilename") + ".txt");
+        const bool srcSU = config.getInt("initSourcesFromSU", 0) != 0, recSU = config.getInt("initReceiverFromSU", 0) != 0;
+        const float DHf = (float)config.getDouble("DH");
+        const std::vector<SourceRow> src = srcSU ? readSourcesSU(config.getString("SourceFilename"), DHf)
+                                                 : readSources(config.getString("SourceFilename") + ".txt");
         std::vector<int> shotNumbers;
         for (auto &s : src)
             if (std::find(shotNumbers.begin(), shotNumbers.end(), s.shot) == shotNumbers.end())
@@ -296,7 +407,7 @@ int main(int argc, char **argv)
             const std::string rf = config.getString("ReceiverFilename") +
                                    (config.getInt("useReceiversPerShot", 0) ? ".shot_" + std::to_string(shotNumbers[k]) : "") +
                                    ".txt";
-            recs[k] = readReceivers(rf);
+            recs[k] = recSU ? readReceiversSU(config.getString("ReceiverFilename"), DHf) : readReceivers(rf);
             for (auto &r : recs[k]) {
                 ri.push_back(lin(r.x, r.y, r.z));
                 rt.push_back(r.type);
@@ -324,7 +435,6 @@ int main(int argc, char **argv)
             throw std::runtime_error("SeismogramFormat " + std::to_string(format) + " is not written (1 = mtx, 4 = su)");
         if (norm < 0 || norm > 2)
             throw std::runtime_error("normalizeTraces " + std::to_string(norm) + " is not applied (0, 1, 2)");
-        const float DHf = (float)config.getDouble("DH");
         for (int k = 0; k < nshots; k++) {
             const int nr = (int)recs[k].size();
             std::vector<float> all = solver->seismograms(k, nr);
