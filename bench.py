@@ -428,8 +428,9 @@ def cpu_baseline_leg(case, nsteps, nsamples, gpu_seis, parity=True):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported in this build "
-              "(shots shard across GPUs with no in-loop communication; see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
+              "(shots would shard across GPUs, one handle per GPU, with no communication inside the "
+              "time loop)", file=sys.stderr)
         sys.exit(2)
     import numpy as np
 
